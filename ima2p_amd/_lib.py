@@ -1,0 +1,95 @@
+"""ctypes binding of libima_gpu.so (the C ABI in include/ima_gpu.h).
+
+Loading fails loudly when the built library is missing: the engine has no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libima_gpu.so")
+
+IMA_OK = 0
+MODEL_HKY, MODEL_IS, MODEL_SW = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE = 0, 1, 2, 3
+LMODE_REF_COMPAT, LMODE_EXACT = 0, 1
+REJECT_IS = -1000000000.0
+MAX_GENES = 256
+K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3 = range(7)
+
+
+class Item(C.Structure):
+    _fields_ = [("slot", C.c_int32), ("flags", C.c_int32), ("u", C.c_double),
+                ("kappa", C.c_double), ("tr", C.c_double)]
+
+
+i32p = C.POINTER(C.c_int32)
+i8p = C.POINTER(C.c_int8)
+dp = C.POINTER(C.c_double)
+fp = C.POINTER(C.c_float)
+vp = C.c_void_p
+
+# name -> (restype, argtypes)
+PROTOS = {
+    "ima_abi_version": (C.c_int, []),
+    "ima_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ima_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "ima_ctx_destroy": (None, [vp]),
+    "ima_ctx_last_error": (C.c_char_p, [vp]),
+    "ima_locus_add_hky": (C.c_int, [vp, C.c_int, C.c_int, i8p, i32p, C.c_int, dp,
+                                    C.POINTER(C.c_int)]),
+    "ima_locus_add_is": (C.c_int, [vp, C.c_int, C.c_int, i8p, C.POINTER(C.c_int)]),
+    "ima_locus_add_sw": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "ima_locus_set_sumlogk": (C.c_int, [vp, C.c_int, C.c_double]),
+    "ima_locus_sumlogk_from_slot": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+    "ima_slots_alloc": (C.c_int, [vp, C.c_int, i32p, C.POINTER(C.c_int)]),
+    "ima_slots_write": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p, i32p]),
+    "ima_slots_read": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p, i32p]),
+    "ima_score": (C.c_int, [vp, C.POINTER(Item), C.c_int, C.c_int, dp, i32p]),
+    "ima_is_site_nodes": (C.c_int, [vp, C.c_int, i32p]),
+    "ima_sw_dlike": (C.c_int, [vp, C.POINTER(Item), dp]),
+    "ima_likelihood_hky": (C.c_int, [vp, C.c_int, i32p, i32p, i32p, dp, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+    "ima_swap": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, C.c_uint64, C.c_uint64,
+                           C.POINTER(C.c_int), i32p]),
+    "ima_lmode_table": (C.c_int, [vp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_int)]),
+    "ima_lmode_table_free": (C.c_int, [vp, C.c_int]),
+    "ima_lmode_jointp": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, dp, dp]),
+    "ima_lmode_margincalc": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int,
+                                       C.c_double, C.c_int, C.c_double, dp]),
+    "ima_timing_enable": (C.c_int, [vp, C.c_int]),
+    "ima_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), dp]),
+    "ima_timing_reset": (C.c_int, [vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libima_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ima2p_amd.build` "
+                           "(the engine has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols(header=None):
+    """Entry points declared in include/ima_gpu.h."""
+    import re
+    header = header or os.path.join(os.path.dirname(HERE), "include", "ima_gpu.h")
+    txt = open(header).read()
+    return sorted(set(re.findall(r"\b(ima_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,888 @@
+// ctx.hip -- context, device-resident data and the extern "C" entry points.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ima_internal.h"
+
+using namespace ima;
+
+namespace {
+
+struct LTableHost {
+  bool live = false;
+  float *rows = nullptr;
+  int G = 0, K = 0, npsize = 0, nmig = 0, npops = 0;
+};
+
+}  // namespace
+
+struct ima_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // loci (host mirror + device copy)
+  std::vector<LocusMeta> loci;
+  std::vector<int8_t> seq;
+  std::vector<int32_t> mult;
+  std::vector<uint64_t> mask;
+  bool loci_dirty = true;
+  DevBuf<LocusMeta> d_loci;
+  DevBuf<int8_t> d_seq;
+  DevBuf<int32_t> d_mult;
+  DevBuf<uint64_t> d_mask;
+
+  // genealogy slots
+  std::vector<SlotMeta> slots;
+  size_t nodes = 0, progs = 0;
+  std::vector<int32_t> up0, up1, down, A;
+  std::vector<double> time;
+  std::vector<ProgOp> prog;
+  DevBuf<SlotMeta> d_slots;
+  DevBuf<int32_t> d_up0, d_up1, d_down, d_A;
+  DevBuf<double> d_time;
+  DevBuf<ProgOp> d_prog;
+  bool slots_meta_dirty = true;
+  std::vector<int> scratch_slot;  // per locus, for ima_likelihood_hky
+
+  // scoring scratch
+  DevBuf<ima_item> d_items;
+  DevBuf<int32_t> d_idx;
+  DevBuf<double> d_out;
+  DevBuf<int32_t> d_status;
+  DevBuf<int32_t> d_aux;
+  DevBuf<double> d_aux64;
+
+  // L mode
+  std::vector<LTableHost> tables;
+  DevBuf<double> d_lx, d_lwork, d_lout, d_less;
+
+  // swap
+  DevBuf<double> d_S, d_beta;
+  DevBuf<int32_t> d_pairs;
+
+  Timing timing;
+  int max_lds = 65536;
+};
+
+namespace {
+
+int fail(ima_ctx *c, int rc, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return rc;
+}
+
+#define HIPCHK(c, call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((c), IMA_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                  __FILE__, __LINE__);                                               \
+  } while (0)
+
+template <typename T>
+hipError_t reserve(DevBuf<T> &b, size_t n, bool keep = false) {
+  if (n <= b.n && b.p) return hipSuccess;
+  size_t cap = std::max<size_t>(n, b.n * 2);
+  if (cap == 0) cap = 1;
+  T *p = nullptr;
+  hipError_t e = hipMalloc(&p, cap * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (keep && b.p && b.n) {
+    e = hipMemcpy(p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) return e;
+  }
+  if (b.p) hipFree(b.p);
+  b.p = p;
+  b.n = cap;
+  return hipSuccess;
+}
+
+template <typename T>
+void release(DevBuf<T> &b) {
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.n = 0;
+}
+
+int upload_loci(ima_ctx *c) {
+  if (!c->loci_dirty) return IMA_OK;
+  HIPCHK(c, reserve(c->d_loci, c->loci.size() + 1));
+  HIPCHK(c, reserve(c->d_seq, c->seq.size() + 1));
+  HIPCHK(c, reserve(c->d_mult, c->mult.size() + 1));
+  HIPCHK(c, reserve(c->d_mask, c->mask.size() + 1));
+  if (!c->loci.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_loci.p, c->loci.data(), c->loci.size() * sizeof(LocusMeta),
+                             hipMemcpyHostToDevice, c->stream));
+  if (!c->seq.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_seq.p, c->seq.data(), c->seq.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  if (!c->mult.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_mult.p, c->mult.data(), c->mult.size() * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  if (!c->mask.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_mask.p, c->mask.data(), c->mask.size() * 8,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->loci_dirty = false;
+  return IMA_OK;
+}
+
+int upload_slot_meta(ima_ctx *c) {
+  if (!c->slots_meta_dirty) return IMA_OK;
+  HIPCHK(c, reserve(c->d_slots, c->slots.size() + 1));
+  if (!c->slots.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_slots.p, c->slots.data(), c->slots.size() * sizeof(SlotMeta),
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->slots_meta_dirty = false;
+  return IMA_OK;
+}
+
+DeviceView view(ima_ctx *c) {
+  DeviceView v;
+  v.loci = c->d_loci.p;
+  v.slots = c->d_slots.p;
+  v.up0 = c->d_up0.p;
+  v.up1 = c->d_up1.p;
+  v.down = c->d_down.p;
+  v.time = c->d_time.p;
+  v.A = c->d_A.p;
+  v.prog = c->d_prog.p;
+  v.seq = c->d_seq.p;
+  v.mult = c->d_mult.p;
+  v.mask = c->d_mask.p;
+  return v;
+}
+
+// Slot need of the subtree below edge e (0 for a tip): Strahler-style.
+int need_of(const int32_t *u0, const int32_t *u1, int e, std::vector<int> &need) {
+  if (u0[e] < 0) return need[e] = 0;
+  const int a = need_of(u0, u1, u0[e], need), b = need_of(u0, u1, u1[e], need);
+  return need[e] = (a == b) ? a + 1 : std::max(a, b);
+}
+
+void emit(const int32_t *u0, const int32_t *u1, int e, int base, const std::vector<int> &need,
+          std::vector<ProgOp> &out) {
+  int a = u0[e], b = u1[e];
+  if (need[a] < need[b]) std::swap(a, b);
+  int a_slot = kTipSlot, b_slot = kTipSlot, next = base;
+  if (u0[a] >= 0) {
+    emit(u0, u1, a, base, need, out);
+    a_slot = base;
+    next = base + 1;
+  }
+  if (u0[b] >= 0) {
+    emit(u0, u1, b, next, need, out);
+    b_slot = next;
+  }
+  ProgOp op;
+  op.a = a | (a_slot << 16);
+  op.b = b | (b_slot << 16);
+  op.out = base;
+  op.node = e;
+  out.push_back(op);
+}
+
+// Check that the arrays of one slot form a binary genealogy rooted at root.
+bool valid_tree(int n, const int32_t *u0, const int32_t *u1, const int32_t *dn,
+                const double *tm, int root, std::string &why) {
+  const int ne = 2 * n - 1;
+  if (root < n || root >= ne) return why = "root out of range", false;
+  if (dn[root] != -1) return why = "root edge has a parent", false;
+  for (int e = 0; e < ne; ++e) {
+    if (e < n) {
+      if (u0[e] != -1 || u1[e] != -1) return why = "tip with daughters", false;
+    } else {
+      if (u0[e] < 0 || u0[e] >= ne || u1[e] < 0 || u1[e] >= ne || u0[e] == u1[e])
+        return why = "internal edge with bad daughters", false;
+      if (dn[u0[e]] != e || dn[u1[e]] != e) return why = "up/down links disagree", false;
+    }
+    if (e != root && (dn[e] < n || dn[e] >= ne)) return why = "bad parent link", false;
+    if (!(tm[e] == tm[e]) || fabs(tm[e]) > 1e300) return why = "non-finite time", false;
+  }
+  // every edge reachable from the root exactly once
+  std::vector<int> stack{root};
+  std::vector<char> seen(ne, 0);
+  int cnt = 0;
+  while (!stack.empty()) {
+    const int e = stack.back();
+    stack.pop_back();
+    if (seen[e]) return why = "cycle", false;
+    seen[e] = 1;
+    ++cnt;
+    if (u0[e] >= 0) {
+      stack.push_back(u0[e]);
+      stack.push_back(u1[e]);
+    }
+  }
+  if (cnt != ne) return why = "disconnected genealogy", false;
+  return true;
+}
+
+int timing_begin(ima_ctx *c, int k) {
+  if (c->timing.on) HIPCHK(c, hipEventRecord(c->timing.ev0[k], c->stream));
+  return IMA_OK;
+}
+int timing_end(ima_ctx *c, int k) {
+  if (c->timing.on) {
+    HIPCHK(c, hipEventRecord(c->timing.ev1[k], c->stream));
+  }
+  return IMA_OK;
+}
+int timing_collect(ima_ctx *c, int k) {
+  if (c->timing.on) {
+    HIPCHK(c, hipEventSynchronize(c->timing.ev1[k]));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->timing.ev0[k], c->timing.ev1[k]));
+    c->timing.ms[k] += ms;
+    c->timing.launches[k] += 1;
+  }
+  return IMA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ima_abi_version(void) { return IMA_ABI_VERSION; }
+
+int ima_device_count(int *count) {
+  if (!count) return IMA_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return IMA_OK;
+}
+
+int ima_ctx_create(int device, ima_ctx **out) {
+  if (!out) return IMA_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IMA_ERR_NODEV;
+  if (device < 0 || device >= n) return IMA_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return IMA_ERR_HIP;
+  std::unique_ptr<ima_ctx> c(new ima_ctx);
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) return IMA_ERR_HIP;
+  for (int k = 0; k < 8; ++k) {
+    if (hipEventCreate(&c->timing.ev0[k]) != hipSuccess) return IMA_ERR_HIP;
+    if (hipEventCreate(&c->timing.ev1[k]) != hipSuccess) return IMA_ERR_HIP;
+  }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+          hipSuccess &&
+      lds > 0)
+    c->max_lds = lds;
+  *out = c.release();
+  return IMA_OK;
+}
+
+void ima_ctx_destroy(ima_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  release(c->d_loci);
+  release(c->d_seq);
+  release(c->d_mult);
+  release(c->d_mask);
+  release(c->d_slots);
+  release(c->d_up0);
+  release(c->d_up1);
+  release(c->d_down);
+  release(c->d_A);
+  release(c->d_time);
+  release(c->d_prog);
+  release(c->d_items);
+  release(c->d_idx);
+  release(c->d_out);
+  release(c->d_status);
+  release(c->d_aux);
+  release(c->d_aux64);
+  release(c->d_lx);
+  release(c->d_lwork);
+  release(c->d_lout);
+  release(c->d_less);
+  release(c->d_S);
+  release(c->d_beta);
+  release(c->d_pairs);
+  for (auto &t : c->tables)
+    if (t.rows) hipFree(t.rows);
+  for (int k = 0; k < 8; ++k) {
+    hipEventDestroy(c->timing.ev0[k]);
+    hipEventDestroy(c->timing.ev1[k]);
+  }
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *ima_ctx_last_error(const ima_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+// ------------------------------------------------------------------ loci --
+
+int ima_locus_add_hky(ima_ctx *c, int n, int S, const int8_t *seq, const int32_t *mult,
+                      int totsites, const double pi[4], int *locus) {
+  if (!c || !seq || !mult || !pi || !locus) return IMA_ERR_ARG;
+  if (n < 2 || n > kMaxGenes) return fail(c, IMA_ERR_SIZE, "HKY locus n=%d outside [2,%d]", n, kMaxGenes);
+  if (S < 1 || totsites < 1) return fail(c, IMA_ERR_ARG, "HKY locus with no sites");
+  for (size_t i = 0; i < (size_t)n * S; ++i)
+    if (seq[i] < 0 || seq[i] > 3) return fail(c, IMA_ERR_ARG, "HKY base outside 0..3");
+  LocusMeta m{};
+  m.model = IMA_MODEL_HKY;
+  m.n = n;
+  m.S = S;
+  m.totsites = totsites;
+  m.seq_off = (int64_t)c->seq.size();
+  m.mult_off = (int64_t)c->mult.size();
+  for (int i = 0; i < 4; ++i) m.pi[i] = pi[i];
+  c->seq.insert(c->seq.end(), seq, seq + (size_t)n * S);
+  c->mult.insert(c->mult.end(), mult, mult + S);
+  *locus = (int)c->loci.size();
+  c->loci.push_back(m);
+  c->scratch_slot.push_back(-1);
+  c->loci_dirty = true;
+  return IMA_OK;
+}
+
+int ima_locus_add_is(ima_ctx *c, int n, int S, const int8_t *seq, int *locus) {
+  if (!c || (!seq && S > 0) || !locus) return IMA_ERR_ARG;
+  if (n < 2 || n > kMaxGenes) return fail(c, IMA_ERR_SIZE, "IS locus n=%d outside [2,%d]", n, kMaxGenes);
+  if (S < 0) return IMA_ERR_ARG;
+  LocusMeta m{};
+  m.model = IMA_MODEL_IS;
+  m.n = n;
+  m.S = S;
+  m.W = (n + 63) / 64;
+  m.mask_off = (int64_t)c->mask.size();
+  for (int s = 0; s < S; ++s) {
+    uint64_t w[4] = {0, 0, 0, 0};
+    for (int t = 0; t < n; ++t) {
+      const int8_t v = seq[(size_t)t * S + s];
+      if (v != 0 && v != 1) return fail(c, IMA_ERR_ARG, "IS state outside 0/1");
+      if (v) w[t >> 6] |= 1ull << (t & 63);
+    }
+    for (int k = 0; k < m.W; ++k) c->mask.push_back(w[k]);
+  }
+  *locus = (int)c->loci.size();
+  c->loci.push_back(m);
+  c->scratch_slot.push_back(-1);
+  c->loci_dirty = true;
+  return IMA_OK;
+}
+
+int ima_locus_add_sw(ima_ctx *c, int n, int *locus) {
+  if (!c || !locus) return IMA_ERR_ARG;
+  if (n < 2 || n > kMaxGenes) return fail(c, IMA_ERR_SIZE, "SW locus n=%d outside [2,%d]", n, kMaxGenes);
+  LocusMeta m{};
+  m.model = IMA_MODEL_SW;
+  m.n = n;
+  *locus = (int)c->loci.size();
+  c->loci.push_back(m);
+  c->scratch_slot.push_back(-1);
+  c->loci_dirty = true;
+  return IMA_OK;
+}
+
+int ima_locus_set_sumlogk(ima_ctx *c, int locus, double sumlogk) {
+  if (!c || locus < 0 || locus >= (int)c->loci.size()) return IMA_ERR_ARG;
+  c->loci[locus].sumlogk = sumlogk;
+  c->loci_dirty = true;
+  return IMA_OK;
+}
+
+// ----------------------------------------------------------------- slots --
+
+int ima_slots_alloc(ima_ctx *c, int count, const int32_t *locus_of, int *first) {
+  if (!c || count < 0 || (!locus_of && count > 0) || !first) return IMA_ERR_ARG;
+  hipSetDevice(c->device);
+  *first = (int)c->slots.size();
+  for (int i = 0; i < count; ++i) {
+    const int l = locus_of[i];
+    if (l < 0 || l >= (int)c->loci.size()) return fail(c, IMA_ERR_ARG, "slot locus %d out of range", l);
+    SlotMeta s{};
+    s.locus = l;
+    s.n = c->loci[l].n;
+    s.root = -1;
+    s.nslots = 1;
+    s.node_off = (int64_t)c->nodes;
+    s.prog_off = (int64_t)c->progs;
+    c->nodes += 2 * (size_t)s.n - 1;
+    c->progs += (size_t)s.n - 1;
+    c->slots.push_back(s);
+  }
+  c->up0.resize(c->nodes, -1);
+  c->up1.resize(c->nodes, -1);
+  c->down.resize(c->nodes, -1);
+  c->A.resize(c->nodes, 0);
+  c->time.resize(c->nodes, 0.0);
+  c->prog.resize(c->progs);
+  HIPCHK(c, reserve(c->d_up0, c->nodes, true));
+  HIPCHK(c, reserve(c->d_up1, c->nodes, true));
+  HIPCHK(c, reserve(c->d_down, c->nodes, true));
+  HIPCHK(c, reserve(c->d_A, c->nodes, true));
+  HIPCHK(c, reserve(c->d_time, c->nodes, true));
+  HIPCHK(c, reserve(c->d_prog, c->progs + 1, true));
+  c->slots_meta_dirty = true;
+  return IMA_OK;
+}
+
+int ima_slots_write(ima_ctx *c, int first, int count, const int32_t *up0, const int32_t *up1,
+                    const int32_t *down, const double *time, const int32_t *root,
+                    const int32_t *A) {
+  if (!c || !up0 || !up1 || !down || !time || !root) return IMA_ERR_ARG;
+  if (first < 0 || count < 0 || first + count > (int)c->slots.size())
+    return fail(c, IMA_ERR_ARG, "slot range [%d,%d) out of range", first, first + count);
+  if (count == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  size_t src = 0;
+  std::vector<int> need;
+  std::vector<ProgOp> ops;
+  for (int i = 0; i < count; ++i) {
+    SlotMeta &s = c->slots[first + i];
+    const int n = s.n, ne = 2 * n - 1;
+    std::string why;
+    if (!valid_tree(n, up0 + src, up1 + src, down + src, time + src, root[i], why))
+      return fail(c, IMA_ERR_ARG, "slot %d: %s", first + i, why.c_str());
+    std::copy(up0 + src, up0 + src + ne, c->up0.begin() + s.node_off);
+    std::copy(up1 + src, up1 + src + ne, c->up1.begin() + s.node_off);
+    std::copy(down + src, down + src + ne, c->down.begin() + s.node_off);
+    std::copy(time + src, time + src + ne, c->time.begin() + s.node_off);
+    if (A) std::copy(A + src, A + src + ne, c->A.begin() + s.node_off);
+    s.root = root[i];
+    need.assign(ne, 0);
+    const int r = need_of(up0 + src, up1 + src, root[i], need);
+    ops.clear();
+    emit(up0 + src, up1 + src, root[i], 0, need, ops);
+    std::copy(ops.begin(), ops.end(), c->prog.begin() + s.prog_off);
+    s.nslots = std::max(1, r);
+    src += ne;
+  }
+  const SlotMeta &s0 = c->slots[first];
+  const SlotMeta &s1 = c->slots[first + count - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  const size_t pb = s1.prog_off + (size_t)s1.n - 1 - s0.prog_off;
+  HIPCHK(c, hipMemcpyAsync(c->d_up0.p + s0.node_off, c->up0.data() + s0.node_off, nb * 4,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_up1.p + s0.node_off, c->up1.data() + s0.node_off, nb * 4,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_down.p + s0.node_off, c->down.data() + s0.node_off, nb * 4,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_time.p + s0.node_off, c->time.data() + s0.node_off, nb * 8,
+                           hipMemcpyHostToDevice, c->stream));
+  if (A)
+    HIPCHK(c, hipMemcpyAsync(c->d_A.p + s0.node_off, c->A.data() + s0.node_off, nb * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  if (pb)
+    HIPCHK(c, hipMemcpyAsync(c->d_prog.p + s0.prog_off, c->prog.data() + s0.prog_off,
+                             pb * sizeof(ProgOp), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->slots_meta_dirty = true;
+  return IMA_OK;
+}
+
+int ima_slots_read(ima_ctx *c, int first, int count, int32_t *up0, int32_t *up1,
+                   int32_t *down, double *time, int32_t *root, int32_t *A) {
+  if (!c || first < 0 || count < 0 || first + count > (int)c->slots.size()) return IMA_ERR_ARG;
+  if (count == 0) return IMA_OK;
+  const SlotMeta &s0 = c->slots[first];
+  const SlotMeta &s1 = c->slots[first + count - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  if (up0) HIPCHK(c, hipMemcpy(up0, c->d_up0.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (up1) HIPCHK(c, hipMemcpy(up1, c->d_up1.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (down) HIPCHK(c, hipMemcpy(down, c->d_down.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (time) HIPCHK(c, hipMemcpy(time, c->d_time.p + s0.node_off, nb * 8, hipMemcpyDeviceToHost));
+  if (A) HIPCHK(c, hipMemcpy(A, c->d_A.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (root)
+    for (int i = 0; i < count; ++i) root[i] = c->slots[first + i].root;
+  return IMA_OK;
+}
+
+// --------------------------------------------------------------- scoring --
+
+static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precision,
+                      double *loglik, int32_t *status) {
+  if (!c || (!items && nitems > 0) || !loglik || nitems < 0) return IMA_ERR_ARG;
+  if (precision != IMA_PREC_FP64 && precision != IMA_PREC_FP32) return IMA_ERR_ARG;
+  if (nitems == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  std::vector<int32_t> idx_h(nitems), idx_i, idx_s;
+  std::vector<int32_t> hky;
+  int hmax_n = 2, hmax_slots = 1, imax_n = 2, imax_w = 1;
+  hky.reserve(nitems);
+  for (int i = 0; i < nitems; ++i) {
+    const int sl = items[i].slot;
+    if (sl < 0 || sl >= (int)c->slots.size()) return fail(c, IMA_ERR_ARG, "item %d: slot %d out of range", i, sl);
+    const SlotMeta &s = c->slots[sl];
+    if (s.root < 0) return fail(c, IMA_ERR_STATE, "item %d: slot %d holds no genealogy", i, sl);
+    const LocusMeta &m = c->loci[s.locus];
+    if (m.model == IMA_MODEL_HKY) {
+      if (!(items[i].u > 0) || !(items[i].kappa > 0))
+        return fail(c, IMA_ERR_ARG, "item %d: HKY needs u > 0 and kappa > 0", i);
+      hky.push_back(i);
+      hmax_n = std::max(hmax_n, s.n);
+      hmax_slots = std::max(hmax_slots, s.nslots);
+    } else if (m.model == IMA_MODEL_IS) {
+      if (!(items[i].u > 0)) return fail(c, IMA_ERR_ARG, "item %d: IS needs u > 0", i);
+      idx_i.push_back(i);
+      imax_n = std::max(imax_n, s.n);
+      imax_w = std::max(imax_w, m.W);
+    } else {
+      idx_s.push_back(i);
+    }
+  }
+  const size_t elt = precision == IMA_PREC_FP32 ? 4 : 8;
+  const size_t hlds = (2 * (size_t)hmax_n - 1) * 8 * elt + (size_t)hmax_slots * kWave * 4 * elt +
+                      (size_t)hmax_n * kWave;
+  if (!hky.empty() && hlds > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "HKY batch needs %zu B of LDS (limit %d)", hlds, c->max_lds);
+  HIPCHK(c, reserve(c->d_items, nitems));
+  HIPCHK(c, reserve(c->d_idx, nitems));
+  HIPCHK(c, reserve(c->d_out, nitems));
+  HIPCHK(c, reserve(c->d_status, nitems));
+  size_t k = 0;
+  for (int i : hky) idx_h[k++] = i;
+  for (int i : idx_i) idx_h[k++] = i;
+  for (int i : idx_s) idx_h[k++] = i;
+  HIPCHK(c, hipMemcpyAsync(c->d_items.p, items, nitems * sizeof(ima_item), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_idx.p, idx_h.data(), nitems * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  const DeviceView v = view(c);
+  const int kh = precision == IMA_PREC_FP32 ? K_HKY32 : K_HKY64;
+  if (!hky.empty()) {
+    if ((rc = timing_begin(c, kh)) != IMA_OK) return rc;
+    HIPCHK(c, launch_hky(v, c->d_items.p, c->d_idx.p, (int)hky.size(), precision, hmax_n,
+                         hmax_slots, c->d_out.p, c->d_status.p, c->stream));
+    if ((rc = timing_end(c, kh)) != IMA_OK) return rc;
+  }
+  if (!idx_i.empty()) {
+    if ((rc = timing_begin(c, K_IS)) != IMA_OK) return rc;
+    HIPCHK(c, launch_is(v, c->d_items.p, c->d_idx.p + hky.size(), (int)idx_i.size(), imax_n,
+                        imax_w, c->d_out.p, c->d_status.p, nullptr, nullptr, c->stream));
+    if ((rc = timing_end(c, K_IS)) != IMA_OK) return rc;
+  }
+  if (!idx_s.empty()) {
+    if ((rc = timing_begin(c, K_SW)) != IMA_OK) return rc;
+    HIPCHK(c, launch_sw(v, c->d_items.p, c->d_idx.p + hky.size() + idx_i.size(),
+                        (int)idx_s.size(), c->d_out.p, c->d_status.p, nullptr, c->stream));
+    if ((rc = timing_end(c, K_SW)) != IMA_OK) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(loglik, c->d_out.p, nitems * 8, hipMemcpyDeviceToHost, c->stream));
+  if (status)
+    HIPCHK(c, hipMemcpyAsync(status, c->d_status.p, nitems * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!hky.empty() && (rc = timing_collect(c, kh)) != IMA_OK) return rc;
+  if (!idx_i.empty() && (rc = timing_collect(c, K_IS)) != IMA_OK) return rc;
+  if (!idx_s.empty() && (rc = timing_collect(c, K_SW)) != IMA_OK) return rc;
+  return IMA_OK;
+}
+
+int ima_score(ima_ctx *c, const ima_item *items, int nitems, int precision, double *loglik,
+              int32_t *status) {
+  return score_impl(c, items, nitems, precision, loglik, status);
+}
+
+int ima_is_site_nodes(ima_ctx *c, int slot, int32_t *site_node) {
+  if (!c || !site_node || slot < 0 || slot >= (int)c->slots.size()) return IMA_ERR_ARG;
+  const SlotMeta &s = c->slots[slot];
+  const LocusMeta &m = c->loci[s.locus];
+  if (m.model != IMA_MODEL_IS) return fail(c, IMA_ERR_ARG, "slot %d is not an IS locus", slot);
+  if (s.root < 0) return fail(c, IMA_ERR_STATE, "slot %d holds no genealogy", slot);
+  hipSetDevice(c->device);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  ima_item it{slot, 0, 1.0, 1.0, 1.0};
+  const int32_t zero = 0;
+  HIPCHK(c, reserve(c->d_items, 1));
+  HIPCHK(c, reserve(c->d_idx, 1));
+  HIPCHK(c, reserve(c->d_out, 1));
+  HIPCHK(c, reserve(c->d_status, 1));
+  HIPCHK(c, reserve(c->d_aux, (size_t)std::max(m.S, 1)));
+  HIPCHK(c, hipMemcpyAsync(c->d_items.p, &it, sizeof it, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_idx.p, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_is(view(c), c->d_items.p, c->d_idx.p, 1, s.n, m.W, c->d_out.p,
+                      c->d_status.p, c->d_aux.p, nullptr, c->stream));
+  if (m.S > 0)
+    HIPCHK(c, hipMemcpyAsync(site_node, c->d_aux.p, (size_t)m.S * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+int ima_locus_sumlogk_from_slot(ima_ctx *c, int locus, int slot, double *sumlogk) {
+  if (!c || !sumlogk || locus < 0 || locus >= (int)c->loci.size() || slot < 0 ||
+      slot >= (int)c->slots.size())
+    return IMA_ERR_ARG;
+  const SlotMeta &s = c->slots[slot];
+  if (s.locus != locus) return fail(c, IMA_ERR_ARG, "slot %d belongs to locus %d", slot, s.locus);
+  const LocusMeta &m = c->loci[locus];
+  if (m.model != IMA_MODEL_IS) return fail(c, IMA_ERR_ARG, "locus %d is not IS", locus);
+  if (s.root < 0) return fail(c, IMA_ERR_STATE, "slot %d holds no genealogy", slot);
+  hipSetDevice(c->device);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  const int ne = 2 * s.n - 1;
+  ima_item it{slot, 0, 1.0, 1.0, 1.0};
+  const int32_t zero = 0;
+  HIPCHK(c, reserve(c->d_items, 1));
+  HIPCHK(c, reserve(c->d_idx, 1));
+  HIPCHK(c, reserve(c->d_out, 1));
+  HIPCHK(c, reserve(c->d_status, 1));
+  HIPCHK(c, reserve(c->d_aux, (size_t)ne));
+  HIPCHK(c, hipMemcpyAsync(c->d_items.p, &it, sizeof it, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_idx.p, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_is(view(c), c->d_items.p, c->d_idx.p, 1, s.n, m.W, c->d_out.p,
+                      c->d_status.p, nullptr, c->d_aux.p, c->stream));
+  std::vector<int32_t> cnt(ne);
+  int32_t st = 0;
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d_aux.p, (size_t)ne * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(&st, c->d_status.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // calc_prob_data.cpp:654-658, 705-717: an incompatible genealogy leaves
+  // the constant at zero; otherwise sum over internal nodes of log(k!).
+  double sum = 0.0;
+  if (st == IMA_STATUS_OK) {
+    for (int i = s.n; i < ne; ++i) {
+      double fact = 1.0;
+      for (int j = 1; j <= cnt[i]; ++j) fact *= (double)j;
+      sum += log(fact);
+    }
+  }
+  c->loci[locus].sumlogk = sum;
+  c->loci_dirty = true;
+  *sumlogk = sum;
+  if (st == IMA_STATUS_IS_NOSITE)
+    return fail(c, IMA_ERR_ARG, "a site has no 0/1 node on this genealogy");
+  return IMA_OK;
+}
+
+int ima_sw_dlike(ima_ctx *c, const ima_item *item, double *dlikeA) {
+  if (!c || !item || !dlikeA || item->slot < 0 || item->slot >= (int)c->slots.size())
+    return IMA_ERR_ARG;
+  const SlotMeta &s = c->slots[item->slot];
+  if (c->loci[s.locus].model != IMA_MODEL_SW) return fail(c, IMA_ERR_ARG, "not an SW locus");
+  if (s.root < 0) return fail(c, IMA_ERR_STATE, "slot holds no genealogy");
+  hipSetDevice(c->device);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  const int ne = 2 * s.n - 1;
+  const int32_t zero = 0;
+  HIPCHK(c, reserve(c->d_items, 1));
+  HIPCHK(c, reserve(c->d_idx, 1));
+  HIPCHK(c, reserve(c->d_out, 1));
+  HIPCHK(c, reserve(c->d_status, 1));
+  HIPCHK(c, reserve(c->d_aux64, (size_t)ne));
+  HIPCHK(c, hipMemcpyAsync(c->d_items.p, item, sizeof *item, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_idx.p, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_sw(view(c), c->d_items.p, c->d_idx.p, 1, c->d_out.p, c->d_status.p,
+                      c->d_aux64.p, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dlikeA, c->d_aux64.p, (size_t)ne * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+int ima_likelihood_hky(ima_ctx *c, int locus, const int32_t *up0, const int32_t *up1,
+                       const int32_t *down, const double *time, int root, double u,
+                       double kappa, int e1, int e2, int e3, int e4, double *loglik) {
+  (void)e1;
+  (void)e2;
+  (void)e3;
+  (void)e4;
+  if (!c || !loglik || locus < 0 || locus >= (int)c->loci.size()) return IMA_ERR_ARG;
+  if (c->loci[locus].model != IMA_MODEL_HKY) return fail(c, IMA_ERR_ARG, "locus %d is not HKY", locus);
+  int rc;
+  if (c->scratch_slot[locus] < 0) {
+    int first = 0;
+    const int32_t l = locus;
+    if ((rc = ima_slots_alloc(c, 1, &l, &first)) != IMA_OK) return rc;
+    c->scratch_slot[locus] = first;
+  }
+  const int slot = c->scratch_slot[locus];
+  const int32_t r = root;
+  if ((rc = ima_slots_write(c, slot, 1, up0, up1, down, time, &r, nullptr)) != IMA_OK) return rc;
+  ima_item it{slot, 0, u, kappa, 1.0};
+  return score_impl(c, &it, 1, IMA_PREC_FP64, loglik, nullptr);
+}
+
+// ------------------------------------------------------------------ swap --
+
+int ima_swap(ima_ctx *c, int nchains, const double *S, double *beta, int swaptries,
+             uint64_t seed, uint64_t counter, int *swaps_out, int32_t *pairs_out) {
+  if (!c || !S || !beta || nchains < 2 || swaptries < 0) return IMA_ERR_ARG;
+  hipSetDevice(c->device);
+  HIPCHK(c, reserve(c->d_S, nchains));
+  HIPCHK(c, reserve(c->d_beta, nchains));
+  HIPCHK(c, reserve(c->d_pairs, 3 * (size_t)swaptries + 1));
+  HIPCHK(c, reserve(c->d_aux, 1));
+  HIPCHK(c, hipMemcpyAsync(c->d_S.p, S, nchains * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
+  int rc;
+  if ((rc = timing_begin(c, K_SWAP)) != IMA_OK) return rc;
+  HIPCHK(c, launch_swap(nchains, c->d_S.p, c->d_beta.p, swaptries, seed, counter, c->d_aux.p,
+                        c->d_pairs.p, c->stream));
+  if ((rc = timing_end(c, K_SWAP)) != IMA_OK) return rc;
+  int32_t nacc = 0;
+  HIPCHK(c, hipMemcpyAsync(beta, c->d_beta.p, nchains * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&nacc, c->d_aux.p, 4, hipMemcpyDeviceToHost, c->stream));
+  if (pairs_out && swaptries)
+    HIPCHK(c, hipMemcpyAsync(pairs_out, c->d_pairs.p, 3 * (size_t)swaptries * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = timing_collect(c, K_SWAP)) != IMA_OK) return rc;
+  if (swaps_out) *swaps_out = nacc;
+  return IMA_OK;
+}
+
+// ---------------------------------------------------------------- L mode --
+
+int ima_lmode_table(ima_ctx *c, const float *table, int G, int K, int npsize, int nmig,
+                    int npops, int *table_id) {
+  if (!c || !table || !table_id || G < 1 || npsize < 0 || nmig < 0) return IMA_ERR_ARG;
+  if (npsize + nmig > 16 || npsize + nmig < 1) return fail(c, IMA_ERR_SIZE, "1..16 parameters supported");
+  if (K < 4 * npsize + 3 * nmig + 2) return fail(c, IMA_ERR_ARG, "K=%d too small for the layout", K);
+  if ((size_t)128 * K * 4 > (size_t)c->max_lds) return fail(c, IMA_ERR_SIZE, "row too long");
+  hipSetDevice(c->device);
+  LTableHost t;
+  HIPCHK(c, hipMalloc(&t.rows, (size_t)G * K * sizeof(float)));
+  HIPCHK(c, hipMemcpyAsync(t.rows, table, (size_t)G * K * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  t.live = true;
+  t.G = G;
+  t.K = K;
+  t.npsize = npsize;
+  t.nmig = nmig;
+  t.npops = npops;
+  *table_id = (int)c->tables.size();
+  c->tables.push_back(t);
+  return IMA_OK;
+}
+
+int ima_lmode_table_free(ima_ctx *c, int id) {
+  if (!c || id < 0 || id >= (int)c->tables.size() || !c->tables[id].live) return IMA_ERR_ARG;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  hipFree(c->tables[id].rows);
+  c->tables[id].rows = nullptr;
+  c->tables[id].live = false;
+  return IMA_OK;
+}
+
+static LTable ltable(const LTableHost &h) {
+  LTable t;
+  t.rows = h.rows;
+  t.G = h.G;
+  t.K = h.K;
+  t.npsize = h.npsize;
+  t.nmig = h.nmig;
+  t.npops = h.npops;
+  return t;
+}
+
+int ima_lmode_jointp(ima_ctx *c, int id, const double *x, int npoints, int modeltype,
+                     int r0, int r1, int mode, double *out, double *ess) {
+  if (!c || !x || !out || npoints < 0 || id < 0 || id >= (int)c->tables.size() ||
+      !c->tables[id].live)
+    return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  const int np = h.npsize + h.nmig;
+  if (r0 < 0 || r1 > np || r0 >= r1) return fail(c, IMA_ERR_ARG, "bad parameter range");
+  if (modeltype < 0 || modeltype > 2 || (mode != IMA_LMODE_EXACT && mode != IMA_LMODE_REF_COMPAT))
+    return IMA_ERR_ARG;
+  if (npoints == 0) return IMA_OK;
+  for (size_t i = 0; i < (size_t)npoints * np; ++i)
+    if (!(x[i] > 0)) return fail(c, IMA_ERR_ARG, "parameter values must be > 0");
+  hipSetDevice(c->device);
+  const LTable t = ltable(h);
+  const size_t wb = jointp_work_bytes(t, npoints);
+  HIPCHK(c, reserve(c->d_lx, (size_t)npoints * np));
+  HIPCHK(c, reserve(c->d_lwork, wb / 8 + 1));
+  HIPCHK(c, reserve(c->d_lout, (size_t)npoints));
+  HIPCHK(c, reserve(c->d_less, (size_t)npoints));
+  HIPCHK(c, hipMemcpyAsync(c->d_lx.p, x, (size_t)npoints * np * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  int rc;
+  if ((rc = timing_begin(c, K_LMODE)) != IMA_OK) return rc;
+  HIPCHK(c, launch_jointp(t, c->d_lx.p, npoints, modeltype, r0, r1, mode, c->d_lwork.p,
+                          c->d_lwork.n * 8, c->d_lout.p, c->d_less.p, c->stream));
+  if ((rc = timing_end(c, K_LMODE)) != IMA_OK) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->d_lout.p, (size_t)npoints * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  if (ess)
+    HIPCHK(c, hipMemcpyAsync(ess, c->d_less.p, (size_t)npoints * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return timing_collect(c, K_LMODE);
+}
+
+int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx, int logi,
+                         double yadjust, int expo, double prior_mean, double *out) {
+  if (!c || !x || !out || nx < 0 || id < 0 || id >= (int)c->tables.size() ||
+      !c->tables[id].live)
+    return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  if (param < 0 || param >= h.npsize + h.nmig) return fail(c, IMA_ERR_ARG, "bad parameter index");
+  if (expo && !(prior_mean > 0)) return fail(c, IMA_ERR_ARG, "prior mean must be > 0");
+  if (nx == 0) return IMA_OK;
+  for (int i = 0; i < nx; ++i)
+    if (!(x[i] > 0)) return fail(c, IMA_ERR_ARG, "x must be > 0");
+  hipSetDevice(c->device);
+  const LTable t = ltable(h);
+  const size_t wb = margincalc_work_bytes(t, nx);
+  HIPCHK(c, reserve(c->d_lx, (size_t)nx));
+  HIPCHK(c, reserve(c->d_lwork, wb / 8 + 1));
+  HIPCHK(c, reserve(c->d_lout, (size_t)nx));
+  HIPCHK(c, hipMemcpyAsync(c->d_lx.p, x, (size_t)nx * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_margincalc(t, param, c->d_lx.p, nx, logi, yadjust, expo, prior_mean,
+                              c->d_lwork.p, c->d_lout.p, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_lout.p, (size_t)nx * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+// ---------------------------------------------------------------- timing --
+
+int ima_timing_enable(ima_ctx *c, int on) {
+  if (!c) return IMA_ERR_ARG;
+  c->timing.on = on != 0;
+  return IMA_OK;
+}
+
+int ima_timing_get(ima_ctx *c, int k, int64_t *launches, double *total_ms) {
+  if (!c || k < 0 || k >= 8) return IMA_ERR_ARG;
+  if (launches) *launches = c->timing.launches[k];
+  if (total_ms) *total_ms = c->timing.ms[k];
+  return IMA_OK;
+}
+
+int ima_timing_reset(ima_ctx *c) {
+  if (!c) return IMA_ERR_ARG;
+  for (int k = 0; k < 8; ++k) {
+    c->timing.launches[k] = 0;
+    c->timing.ms[k] = 0;
+  }
+  return IMA_OK;
+}
+
+}  // extern "C"

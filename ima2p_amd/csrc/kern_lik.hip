@@ -1,0 +1,461 @@
+// kern_lik.hip -- batched per-locus data likelihood kernels for gfx950.
+//
+//   HKY  Felsenstein pruning (calc_prob_data.cpp:118-607)
+//   IS   infinite-sites mutation placement (calc_prob_data.cpp:537-836)
+//   SW   stepwise-mutation Bessel terms (calc_prob_data.cpp:842-909,
+//        utilities.cpp:55-124, 1453-1491)
+//
+// Every kernel runs one 64-lane wavefront per scoring item (one genealogy
+// of one chain at one locus).  Items of all chains and loci, with uneven
+// sample sizes and pattern counts, go into a single launch.
+#include <float.h>
+
+#include "ima_internal.h"
+
+namespace ima {
+
+// ------------------------------------------------------------ helpers --
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// ================================================================ HKY ==
+//
+// Pruning runs on chip.  The host turns each genealogy into a postorder
+// program whose operands are tips or LDS "partial slots"; daughters with the
+// larger slot need are evaluated first, so a genealogy of n tips needs at
+// most floor(log2 n)+1 slots (Strahler number) of 64 lanes x 4 states.
+// Nothing per node goes to HBM: a proposal is always re-scored over the
+// whole genealogy, which costs ~(36n-58)S flops on chip instead of the
+// ~36 B per node-pattern of HBM traffic that stored partials would need.
+//
+// Branch transition (HKY85 closed form, calc_prob_data.cpp:197-208):
+//   P_jk(t) = pi_k (1-e) + [cls(j)=cls(k)] pi_k (e - eA_c)/PI_c + [j=k] eA_c
+// with e = exp(-mu t), eA_c = exp(-mu t A_c), A_c = 1 + PI_c (kappa-1),
+// c the purine {A,G} or pyrimidine {C,T} class.  Hence for a daughter with
+// partial L:  (P L)_j = a W + b_c W_c + eA_c L_j,  a = 1-e,
+// b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.
+// Partials are rescaled at every node by an exact power of two; one
+// integer exponent per lane carries the sum of the node scale factors.
+
+template <typename real>
+struct __align__(16) HkyCoef {
+  real a, bR, bY, eAR, eAY, pad0, pad1, pad2;
+};
+
+template <typename real>
+__device__ __forceinline__ void hky_child(const HkyCoef<real> &c, int slot,
+                                          int edge, const real *part,
+                                          const int8_t *tips, int lane,
+                                          const real pi[4], real v[4]) {
+  if (slot == kTipSlot) {
+    const int s = tips[edge * kWave + lane];
+    const bool R = (s & 1) == 0;  // A=0, G=2 purines; C=1, T=3 pyrimidines
+    const real ps = s == 0 ? pi[0] : (s == 1 ? pi[1] : (s == 2 ? pi[2] : pi[3]));
+    const real bs = R ? c.bR : c.bY;
+    const real es = R ? c.eAR : c.eAY;
+    const real base = ps * c.a;
+    const real same = base + ps * bs;
+    v[0] = (R ? same : base) + (s == 0 ? es : real(0));
+    v[2] = (R ? same : base) + (s == 2 ? es : real(0));
+    v[1] = (R ? base : same) + (s == 1 ? es : real(0));
+    v[3] = (R ? base : same) + (s == 3 ? es : real(0));
+  } else {
+    const real *L = part + ((size_t)slot * kWave + lane) * 4;
+    const real L0 = L[0], L1 = L[1], L2 = L[2], L3 = L[3];
+    const real WR = pi[0] * L0 + pi[2] * L2;
+    const real WY = pi[1] * L1 + pi[3] * L3;
+    const real aW = c.a * (WR + WY);
+    const real tR = aW + c.bR * WR;
+    const real tY = aW + c.bY * WY;
+    v[0] = tR + c.eAR * L0;
+    v[2] = tR + c.eAR * L2;
+    v[1] = tY + c.eAY * L1;
+    v[3] = tY + c.eAY * L3;
+  }
+}
+
+__device__ __forceinline__ float ldexp_r(float x, int e) { return ldexpf(x, e); }
+__device__ __forceinline__ double ldexp_r(double x, int e) { return ldexp(x, e); }
+__device__ __forceinline__ int frexp_e(float x) { int e; frexpf(x, &e); return e; }
+__device__ __forceinline__ int frexp_e(double x) { int e; frexp(x, &e); return e; }
+
+template <typename real>
+__global__ __launch_bounds__(kWave) void hky_kernel(
+    DeviceView v, const ima_item *__restrict__ items,
+    const int32_t *__restrict__ idx, double *__restrict__ out,
+    int32_t *__restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  const int ii = idx[blockIdx.x];
+  const ima_item item = items[ii];
+  const SlotMeta sm = v.slots[item.slot];
+  const LocusMeta lm = v.loci[sm.locus];
+  const int n = sm.n, ne = 2 * n - 1, S = lm.S;
+
+  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);
+  real *part = reinterpret_cast<real *>(smem + (size_t)ne * sizeof(HkyCoef<real>));
+  int8_t *tips = reinterpret_cast<int8_t *>(part + (size_t)sm.nslots * kWave * 4);
+
+  const double *pid = lm.pi;
+  const double kappa = item.kappa;
+  // getstandfactor (calc_prob_data.cpp:474-493) and the per-site rate
+  // u / (totsites * standfactor) (:593).
+  double sf = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      if (i != j) sf += (i + j == 2 || i + j == 4) ? pid[i] * pid[j] * kappa : pid[i] * pid[j];
+  const double mutrate = item.u / (lm.totsites * sf);
+  const double PR = pid[0] + pid[2], PY = pid[1] + pid[3];
+  const double AR = 1.0 + PR * (kappa - 1.0), AY = 1.0 + PY * (kappa - 1.0);
+
+  const int32_t *up0 = v.up0 + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  for (int e = lane; e < ne; e += kWave) {
+    if (e == sm.root) continue;
+    const int u0 = up0[e];
+    const double bl = tm[e] - (u0 >= 0 ? tm[u0] : 0.0);
+    const double mt = mutrate * bl;
+    const double e1 = exp(-mt), eR = exp(-mt * AR), eY = exp(-mt * AY);
+    HkyCoef<real> c;
+    c.a = real(1.0 - e1);
+    c.bR = real((e1 - eR) / PR);
+    c.bY = real((e1 - eY) / PY);
+    c.eAR = real(eR);
+    c.eAY = real(eY);
+    c.pad0 = c.pad1 = c.pad2 = real(0);
+    coef[e] = c;
+  }
+  real pi[4] = {real(pid[0]), real(pid[1]), real(pid[2]), real(pid[3])};
+  const ProgOp *prog = v.prog + sm.prog_off;
+  const int8_t *seq = v.seq + lm.seq_off;
+  const int32_t *mult = v.mult + lm.mult_off;
+
+  double acc = 0.0;
+  const int nblocks = (S + kWave - 1) / kWave;
+  for (int blk = 0; blk < nblocks; ++blk) {
+    const int p = blk * kWave + lane;
+    const bool valid = p < S;
+    __syncthreads();
+    for (int t = 0; t < n; ++t) tips[t * kWave + lane] = valid ? seq[(size_t)t * S + p] : 0;
+    __syncthreads();
+    int E = 0;
+    int last = 0;
+    for (int k = 0; k < n - 1; ++k) {
+      const ProgOp op = prog[k];
+      const int a = op.a & 0xFFFF, as = (op.a >> 16) & 0xFFFF;
+      const int b = op.b & 0xFFFF, bs = (op.b >> 16) & 0xFFFF;
+      real va[4], vb[4];
+      hky_child<real>(coef[a], as, a, part, tips, lane, pi, va);
+      hky_child<real>(coef[b], bs, b, part, tips, lane, pi, vb);
+      real L0 = va[0] * vb[0], L1 = va[1] * vb[1], L2 = va[2] * vb[2], L3 = va[3] * vb[3];
+      const real mx = fmax(fmax(L0, L1), fmax(L2, L3));
+      const int ex = frexp_e(mx);
+      E += ex;
+      real *o = part + ((size_t)op.out * kWave + lane) * 4;
+      o[0] = ldexp_r(L0, -ex);
+      o[1] = ldexp_r(L1, -ex);
+      o[2] = ldexp_r(L2, -ex);
+      o[3] = ldexp_r(L3, -ex);
+      last = op.out;
+    }
+    const real *R = part + ((size_t)last * kWave + lane) * 4;
+    const real fr = pi[0] * R[0] + pi[1] * R[1] + pi[2] * R[2] + pi[3] * R[3];
+    if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    out[ii] = acc;
+    if (status) status[ii] = IMA_STATUS_OK;
+  }
+}
+
+hipError_t launch_hky(const DeviceView &v, const ima_item *items,
+                      const int32_t *item_idx, int nitems, int precision,
+                      int max_n, int max_slots, double *out, int32_t *status,
+                      hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  const int ne = 2 * max_n - 1;
+  if (precision == IMA_PREC_FP32) {
+    size_t lds = (size_t)ne * sizeof(HkyCoef<float>) +
+                 (size_t)max_slots * kWave * 4 * sizeof(float) + (size_t)max_n * kWave;
+    hipLaunchKernelGGL(hky_kernel<float>, dim3(nitems), dim3(kWave), lds, st, v,
+                       items, item_idx, out, status);
+  } else {
+    size_t lds = (size_t)ne * sizeof(HkyCoef<double>) +
+                 (size_t)max_slots * kWave * 4 * sizeof(double) + (size_t)max_n * kWave;
+    hipLaunchKernelGGL(hky_kernel<double>, dim3(nitems), dim3(kWave), lds, st, v,
+                       items, item_idx, out, status);
+  }
+  return hipGetLastError();
+}
+
+// ================================================================= IS ==
+//
+// The reference labels every site bottom-up (labelgtree) and looks for the
+// nodes whose daughters are labelled 0 and 1; one such node means the site
+// fits the genealogy, two mean rejection.  Those labels are Fitch parsimony
+// sets, so exactly one such node exists iff the tips carrying 1 (or those
+// carrying 0) are exactly the clade below one edge x.  The 0/1 node is then
+// down(x) and the mutation sits on x, or on both root daughters when
+// down(x) is the root (calc_prob_data.cpp:790-818).  Each lane takes one
+// site and compares its tip masks against every edge's clade mask.
+
+template <int W>
+__global__ __launch_bounds__(kWave) void is_kernel(
+    DeviceView v, const ima_item *__restrict__ items,
+    const int32_t *__restrict__ idx, double *__restrict__ out,
+    int32_t *__restrict__ status, int32_t *__restrict__ site_node,
+    int32_t *__restrict__ counts) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  const int ii = idx[blockIdx.x];
+  const ima_item item = items[ii];
+  const SlotMeta sm = v.slots[item.slot];
+  const LocusMeta lm = v.loci[sm.locus];
+  const int n = sm.n, ne = 2 * n - 1, S = lm.S;
+  unsigned long long *clade = reinterpret_cast<unsigned long long *>(smem);  // [ne][W]
+  double *bl = reinterpret_cast<double *>(clade + (size_t)ne * W);           // [ne]
+  int32_t *dn = reinterpret_cast<int32_t *>(bl + ne);                        // [ne]
+  int32_t *u0 = dn + ne;
+  int32_t *u1 = u0 + ne;
+  int32_t *cnt = u1 + ne;                                                    // [ne]
+
+  const int32_t *gup0 = v.up0 + sm.node_off, *gup1 = v.up1 + sm.node_off;
+  const int32_t *gdn = v.down + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  double len = 0.0;
+  for (int e = lane; e < ne; e += kWave) {
+    const int a = gup0[e];
+    dn[e] = gdn[e];
+    u0[e] = a;
+    u1[e] = gup1[e];
+    cnt[e] = 0;
+    const double b = (gdn[e] == -1) ? 0.0 : tm[e] - (a >= 0 ? tm[a] : 0.0);
+    bl[e] = b;
+    len += b;
+#pragma unroll
+    for (int w = 0; w < W; ++w) clade[(size_t)e * W + w] = 0ull;
+  }
+  len = wave_sum(len);
+  __syncthreads();
+  for (int t = lane; t < n; t += kWave) {
+    const unsigned long long bit = 1ull << (t & 63);
+    for (int e = t; e != -1; e = dn[e]) atomicOr(&clade[(size_t)e * W + (t >> 6)], bit);
+  }
+  __syncthreads();
+
+  unsigned long long full[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int lo = w * 64, hi = min(n, lo + 64);
+    full[w] = (hi <= lo) ? 0ull : (hi - lo == 64 ? ~0ull : ((1ull << (hi - lo)) - 1ull));
+  }
+  const double u = item.u;
+  double acc = 0.0;
+  int bad = 0;  // bit0: reject, bit1: no 0/1 node
+  const uint64_t *mask = v.mask + lm.mask_off;
+  const int Wl = lm.W;  // this locus' mask stride (<= W)
+  for (int s0 = 0; s0 < S; s0 += kWave) {
+    const int s = s0 + lane;
+    if (s < S) {
+      unsigned long long m1[W], m0[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        m1[w] = (w < Wl) ? mask[(size_t)s * Wl + w] : 0ull;
+        m0[w] = full[w] & ~m1[w];
+      }
+      int x = -1, y = -1;
+      for (int e = 0; e < ne; ++e) {
+        if (e == sm.root) continue;
+        bool eq1 = true, eq0 = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const unsigned long long c = clade[(size_t)e * W + w];
+          eq1 = eq1 && (c == m1[w]);
+          eq0 = eq0 && (c == m0[w]);
+        }
+        x = eq1 ? e : x;
+        y = eq0 ? e : y;
+      }
+      const int c = (x >= 0) ? x : y;
+      if (c < 0) {
+        bool empty = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) empty = empty && (m1[w] == 0ull);
+        bad |= empty ? 2 : 1;
+        if (site_node) site_node[s] = -1;
+      } else {
+        const int N = dn[c];
+        const int sis = (u0[N] == c) ? u1[N] : u0[N];
+        const double ptime = (dn[N] == -1) ? bl[c] + bl[sis] : bl[c];
+        acc += log(ptime * u);
+        if (site_node) site_node[s] = N;
+        if (counts) atomicAdd(&cnt[N], 1);
+      }
+    }
+  }
+  acc = wave_sum(acc);
+  bad = wave_or(bad);
+  __syncthreads();
+  if (counts)
+    for (int e = lane; e < ne; e += kWave) counts[e] = cnt[e];
+  if (lane == 0) {
+    if (bad & 2) {
+      out[ii] = IMA_REJECT_IS;
+      if (status) status[ii] = IMA_STATUS_IS_NOSITE;
+    } else if (bad & 1) {
+      out[ii] = IMA_REJECT_IS;
+      if (status) status[ii] = IMA_STATUS_IS_REJECT;
+    } else {
+      out[ii] = -len * u + acc - lm.sumlogk;
+      if (status) status[ii] = IMA_STATUS_OK;
+    }
+  }
+}
+
+hipError_t launch_is(const DeviceView &v, const ima_item *items,
+                     const int32_t *item_idx, int nitems, int max_n, int max_w,
+                     double *out, int32_t *status, int32_t *site_node,
+                     int32_t *counts, hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  const int ne = 2 * max_n - 1;
+  const size_t lds = (size_t)ne * max_w * 8 + (size_t)ne * 8 + (size_t)ne * 4 * 4;
+  switch (max_w) {
+    case 1:
+      hipLaunchKernelGGL(is_kernel<1>, dim3(nitems), dim3(kWave), lds, st, v, items,
+                         item_idx, out, status, site_node, counts);
+      break;
+    case 2:
+      hipLaunchKernelGGL(is_kernel<2>, dim3(nitems), dim3(kWave), lds, st, v, items,
+                         item_idx, out, status, site_node, counts);
+      break;
+    default:
+      hipLaunchKernelGGL(is_kernel<4>, dim3(nitems), dim3(kWave), lds, st, v, items,
+                         item_idx, out, status, site_node, counts);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// ================================================================= SW ==
+//
+// Modified Bessel functions I_n in the reference's own method:
+// Abramowitz & Stegun 9.8.1-9.8.4 polynomial approximations for I_0 and
+// I_1 and Miller's downward recurrence (start 2(n + floor(sqrt(40 n))),
+// renormalised at 1e10) for n >= 2, as in utilities.cpp:55-124, 1453-1491.
+
+__device__ double d_bessi0(double x) {
+  const double ax = fabs(x);
+  if (ax < 3.75) {
+    double y = x / 3.75;
+    y *= y;
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 +
+           y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  const double y = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) *
+         (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 +
+          y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 +
+          y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+__device__ double d_bessi1(double x) {
+  const double ax = fabs(x);
+  double ans;
+  if (ax < 3.75) {
+    double y = x / 3.75;
+    y *= y;
+    ans = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 +
+          y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    const double y = 3.75 / ax;
+    ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 +
+          y * (0.163801e-2 + y * (-0.1031555e-1 + y * ans))));
+    ans *= (exp(ax) / sqrt(ax));
+  }
+  return x < 0.0 ? -ans : ans;
+}
+
+__device__ double d_bessi(int n, double x) {
+  n = abs(n);
+  if (x > 700) return DBL_MAX / 1e10;
+  if (n == 0) return d_bessi0(x);
+  if (n == 1) return d_bessi1(x);
+  if (x == 0.0) return 0.0;
+  const double tox = 2.0 / fabs(x);
+  double bip = 0.0, ans = 0.0, bi = 1.0;
+  for (int j = 2 * (n + (int)sqrt(40.0 * n)); j > 0; j--) {
+    const double bim = bip + j * tox * bi;
+    bip = bi;
+    bi = bim;
+    if (fabs(bi) > 1.0e10) {
+      ans *= 1.0e-10;
+      bi *= 1.0e-10;
+      bip *= 1.0e-10;
+    }
+    if (j == n) ans = bip;
+  }
+  ans *= d_bessi0(x) / bi;
+  return (x < 0.0 && (n & 1)) ? -ans : ans;
+}
+
+__global__ __launch_bounds__(kWave) void sw_kernel(
+    DeviceView v, const ima_item *__restrict__ items,
+    const int32_t *__restrict__ idx, double *__restrict__ out,
+    int32_t *__restrict__ status, double *__restrict__ dlike) {
+  const int lane = threadIdx.x;
+  const int ii = idx[blockIdx.x];
+  const ima_item item = items[ii];
+  const SlotMeta sm = v.slots[item.slot];
+  const int n = sm.n, ne = 2 * n - 1;
+  const int32_t *up0 = v.up0 + sm.node_off, *dn = v.down + sm.node_off;
+  const int32_t *A = v.A + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  const double u = item.u, tr = item.tr;
+  double acc = 0.0;
+  int zero = 0;
+  for (int e = lane; e < ne; e += kWave) {
+    double d_l = 0.0;
+    if (dn[e] != -1) {
+      const int d = A[e] - A[dn[e]];
+      const double t = (up0[e] == -1) ? tm[e] * tr : tr * (tm[e] - tm[up0[e]]);
+      const double b = d_bessi(d, t * u);
+      if (!(b > 0.0)) {
+        zero = 1;
+        d_l = -1e+100;  // IM_BESSI_MIN imamp.hpp:173
+      } else {
+        d_l = -(t * u) + log(b);
+      }
+    }
+    if (dlike) dlike[e] = d_l;
+    acc += d_l;
+  }
+  acc = wave_sum(acc);
+  zero = wave_or(zero);
+  if (lane == 0) {
+    out[ii] = zero ? -DBL_MAX : acc;
+    if (status) status[ii] = zero ? IMA_STATUS_SW_ZERO : IMA_STATUS_OK;
+  }
+}
+
+hipError_t launch_sw(const DeviceView &v, const ima_item *items,
+                     const int32_t *item_idx, int nitems, double *out,
+                     int32_t *status, double *dlike, hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  hipLaunchKernelGGL(sw_kernel, dim3(nitems), dim3(kWave), 0, st, v, items, item_idx,
+                     out, status, dlike);
+  return hipGetLastError();
+}
+
+}  // namespace ima

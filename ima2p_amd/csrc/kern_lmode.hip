@@ -1,0 +1,461 @@
+// kern_lmode.hip -- L-mode surfaces over stored genealogy tables.
+//
+//   jointp      jointfind.cpp:885-1047   joint -log posterior at a point
+//   margincalc  surface_call_functions.cpp:120-173   marginal for one param
+//
+// Tables hold one row of K floats per saved genealogy in the savegsampinf
+// layout (ginfo.cpp:288-316): cc | fc | hcc (npsize each), mc | fm (nmig
+// each), qintegrate (npsize), mintegrate (nmig), pdg, probg, split times.
+//
+// Mapping: one thread per query point, one workgroup per (64-point tile,
+// chunk of rows).  The workgroup stages its rows in LDS and every lane
+// walks them in order, so each row is read from HBM once per point tile
+// and broadcast from LDS.  Chunk partials are merged by a second kernel in
+// chunk order, which keeps the result independent of scheduling.
+#include <float.h>
+
+#include "ima_internal.h"
+
+namespace ima {
+
+constexpr int kLTile = 64;      // points per workgroup
+constexpr int kLStage = 128;    // rows staged in LDS at a time
+constexpr int kMaxParams = 16;  // npsize + nmig
+
+struct JointLayout {
+  int ccp, fcp, hccp, mcp, fmp, qip, mip, pdgp, probgp;
+};
+
+__host__ __device__ inline JointLayout joint_layout(int npsize, int nmig) {
+  JointLayout L;
+  L.ccp = 0;
+  L.fcp = L.ccp + npsize;
+  L.hccp = L.fcp + npsize;
+  L.mcp = L.hccp + npsize;
+  L.fmp = L.mcp + nmig;
+  L.qip = L.fmp + nmig;
+  L.mip = L.qip + npsize;
+  L.pdgp = L.mip + nmig;
+  L.probgp = L.pdgp + 1;
+  return L;
+}
+
+struct PointTerms {
+  double log2diffx[kMaxParams], logx[kMaxParams], divx[kMaxParams], x[kMaxParams];
+};
+
+__device__ inline void point_terms(const double *xp, int npsize, int np, int modeltype,
+                                   int r0, int r1, PointTerms &T) {
+  for (int i = 0; i < np; ++i) {
+    T.x[i] = xp[i];
+    T.log2diffx[i] = 0.0;
+    T.logx[i] = 0.0;
+    T.divx[i] = 0.0;
+  }
+  if (modeltype == 0 || modeltype == 1)
+    for (int i = 0; i < npsize; ++i) T.log2diffx[i] = kLn2 - log(xp[i]);
+  for (int i = r0; i < r1; ++i) {
+    T.logx[i] = log(xp[i]);
+    T.divx[i] = 1.0 / xp[i];
+  }
+}
+
+// Per-genealogy term of jointp (jointfind.cpp:966-995), same operation order.
+__device__ inline double joint_term(const float *g, const JointLayout &L, int npsize,
+                                    int nmig, int npops, int modeltype, int r0, int r1,
+                                    const PointTerms &T) {
+  double p;
+  if (npops == 2) {
+    p = -g[L.probgp];
+  } else {
+    double s = 0.0;
+    if (modeltype == 1)
+      for (int i = 0; i < npsize; ++i) s += g[L.qip + i];
+    else
+      for (int i = 0; i < nmig; ++i) s += g[L.mip + i];
+    p = -s;
+  }
+  for (int i = r0; i < r1; ++i) {
+    if (i < npsize) {
+      const double dg = 2.0 * g[L.fcp + i];
+      p += g[L.ccp + i] * T.log2diffx[i] - g[L.hccp + i] - dg * T.divx[i];
+    } else {
+      const int i1 = i - npsize;
+      p += g[L.mcp + i1] * T.logx[i] - g[L.fmp + i1] * T.x[i];
+    }
+  }
+  return p;
+}
+
+// eexp (utilities.cpp:1507-1539): e^x as m * 10^z, with a degree-10 Taylor
+// series of e^u on u = x - n ln2 in [0, ln2).  Coefficients are 1/k!.
+__device__ inline void d_eexp(double x, double *m, int *z) {
+  const double LOG_10_2 = 0.30102999566398119521;
+  const int nn = (int)floor(x / kLn2);
+  double zr = LOG_10_2 * (double)nn;
+  *z = (int)zr;
+  zr -= (double)*z;
+  const double u = x - (((double)nn) * kLn2);
+  const double temp = 1 + u * (1.0 + u * (0.5 + u * (0.16666666666666666666666666667 +
+      u * (0.04166666666666666666666666667 + u * (0.00833333333333333333333333333 +
+      u * (0.001388888888888888888888888889 + u * (0.000198412698412698412698412698 +
+      u * (0.000024801587301587301587301587301 + u * (2.75573192239858906525573192239859e-6 +
+      u * (2.75573192239858906525573192239e-7))))))))));
+  *m = temp * pow(10.0, zr);
+  if (fabs(*m) > 10) {
+    *m = *m / 10.0;
+    *z = *z + 1;
+  }
+  if (fabs(*m) < 1) {
+    *m = *m * 10.0;
+    *z = *z - 1;
+  }
+}
+
+__device__ inline double pow10i(int k) {
+  // exact powers of ten (10^k is exactly representable for 0 <= k <= 22)
+  double r = 1.0;
+  if (k >= 0) {
+    for (int i = 0; i < k; ++i) r *= 10.0;
+  } else {
+    r = pow(10.0, (double)k);
+  }
+  return r;
+}
+
+struct LArgs {
+  const float *rows;
+  int G, K, npsize, nmig, npops, modeltype, r0, r1;
+  const double *x;  // [npoints][np]
+  int npoints, nchunks, chunk_rows;
+};
+
+// Stage rows [g0, g0+cnt) into LDS.
+__device__ inline void stage_rows(const LArgs &a, float *sh, int g0, int cnt) {
+  const int tot = cnt * a.K;
+  const float *src = a.rows + (size_t)g0 * a.K;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) sh[i] = src[i];
+}
+
+// --------------------------------------------------------------- EXACT --
+// Per (point, chunk): running max M and s = sum exp(p - M), s2 = sum
+// exp(2(p - M)) for the effective sample number.
+__global__ __launch_bounds__(kLTile) void jointp_exact_partial(LArgs a, double *work) {
+  extern __shared__ float shrows[];
+  const int pt = blockIdx.x * kLTile + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int np = a.npsize + a.nmig;
+  const JointLayout L = joint_layout(a.npsize, a.nmig);
+  PointTerms T;
+  const bool active = pt < a.npoints;
+  point_terms(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  double M = -DBL_MAX, s = 0.0, s2 = 0.0;
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.G, gbeg + a.chunk_rows);
+  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
+    const int cnt = min(kLStage, gend - g0);
+    __syncthreads();
+    stage_rows(a, shrows, g0, cnt);
+    __syncthreads();
+    for (int r = 0; r < cnt; ++r) {
+      const double p = joint_term(shrows + (size_t)r * a.K, L, a.npsize, a.nmig, a.npops,
+                                  a.modeltype, a.r0, a.r1, T);
+      if (p > M) {
+        const double sc = exp(M - p);
+        s = s * sc + 1.0;
+        s2 = s2 * sc * sc + 1.0;
+        M = p;
+      } else {
+        const double w = exp(p - M);
+        s += w;
+        s2 += w * w;
+      }
+    }
+  }
+  if (active) {
+    double *o = work + ((size_t)pt * a.nchunks + chunk) * 3;
+    o[0] = M;
+    o[1] = s;
+    o[2] = s2;
+  }
+}
+
+__global__ void jointp_exact_final(LArgs a, const double *work, double *out, double *ess) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= a.npoints) return;
+  const double *w = work + (size_t)pt * a.nchunks * 3;
+  double M = -DBL_MAX;
+  for (int c = 0; c < a.nchunks; ++c) M = fmax(M, w[c * 3]);
+  double s = 0.0, s2 = 0.0;
+  for (int c = 0; c < a.nchunks; ++c) {
+    if (w[c * 3 + 1] == 0.0) continue;
+    const double sc = exp(w[c * 3] - M);
+    s += w[c * 3 + 1] * sc;
+    s2 += w[c * 3 + 2] * sc * sc;
+  }
+  out[pt] = log((double)a.G) - (M + log(s));
+  if (ess) ess[pt] = s * s / s2;
+}
+
+// ---------------------------------------------------------- REF_COMPAT --
+// The reference keeps only values within PRANGELOG = 10 nats of the running
+// maximum (the first value always), then sums at most (kept - 1) of them
+// from the top while they stay within 10 nats of the final maximum, as
+// eexp mantissa/exponent pairs scaled to 10^(z - (z_max - 10)).  Every
+// value within the final window is always kept, so the sum covers the
+// whole final window, minus its smallest member when nothing outside the
+// window was kept and the window holds more than one value.
+//
+// pass 1: chunk maxima; pass 2: per chunk, the running maximum starts at
+// the maximum of the earlier chunks and each lane walks its rows in order;
+// pass 3: merge chunks in order and apply the list cap.
+
+__global__ __launch_bounds__(kLTile) void jointp_ref_pass1(LArgs a, double *cmax) {
+  extern __shared__ float shrows[];
+  const int pt = blockIdx.x * kLTile + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int np = a.npsize + a.nmig;
+  const JointLayout L = joint_layout(a.npsize, a.nmig);
+  PointTerms T;
+  const bool active = pt < a.npoints;
+  point_terms(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  double M = -DBL_MAX;
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.G, gbeg + a.chunk_rows);
+  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
+    const int cnt = min(kLStage, gend - g0);
+    __syncthreads();
+    stage_rows(a, shrows, g0, cnt);
+    __syncthreads();
+    for (int r = 0; r < cnt; ++r)
+      M = fmax(M, joint_term(shrows + (size_t)r * a.K, L, a.npsize, a.nmig, a.npops,
+                             a.modeltype, a.r0, a.r1, T));
+  }
+  if (active) cmax[(size_t)pt * a.nchunks + chunk] = M;
+}
+
+// Per (point, chunk) record of pass 2.
+struct RefPartial {
+  double sum, sum2;   // sum of scaled eexp terms (and squares) in the window
+  double minp, minterm;
+  int32_t cnt;        // values in the final window
+  int32_t outside;    // a kept value lies outside the final window
+};
+
+__global__ __launch_bounds__(kLTile) void jointp_ref_pass2(LArgs a, const double *cmax,
+                                                           RefPartial *part) {
+  extern __shared__ float shrows[];
+  const int pt = blockIdx.x * kLTile + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int np = a.npsize + a.nmig;
+  const JointLayout L = joint_layout(a.npsize, a.nmig);
+  PointTerms T;
+  const bool active = pt < a.npoints;
+  const int ptc = active ? pt : 0;
+  point_terms(a.x + (size_t)ptc * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  const double *cm = cmax + (size_t)ptc * a.nchunks;
+  double Mfin = -DBL_MAX, run = -DBL_MAX;
+  for (int c = 0; c < a.nchunks; ++c) {
+    Mfin = fmax(Mfin, cm[c]);
+    if (c < chunk) run = fmax(run, cm[c]);
+  }
+  double mM;
+  int zM;
+  d_eexp(Mfin, &mM, &zM);
+  const int maxz = zM - 10;
+  RefPartial r{0.0, 0.0, DBL_MAX, 0.0, 0, 0};
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.G, gbeg + a.chunk_rows);
+  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
+    const int cnt = min(kLStage, gend - g0);
+    __syncthreads();
+    stage_rows(a, shrows, g0, cnt);
+    __syncthreads();
+    for (int rr = 0; rr < cnt; ++rr) {
+      const double p = joint_term(shrows + (size_t)rr * a.K, L, a.npsize, a.nmig, a.npops,
+                                  a.modeltype, a.r0, a.r1, T);
+      const int g = g0 + rr;
+      const bool kept = (g == 0) || (run - p < 10);
+      if (kept && p > run) run = p;
+      if (Mfin - p < 10) {
+        double m;
+        int z;
+        d_eexp(p, &m, &z);
+        const int zadj = z - maxz;
+        const double term = (zadj > -308 && zadj < 308)
+                                ? m * pow10i(zadj)
+                                : (zadj <= -308 ? 0.0 : DBL_MAX);
+        r.sum += term;
+        r.sum2 += term * term;
+        r.cnt += 1;
+        if (p < r.minp) {
+          r.minp = p;
+          r.minterm = term;
+        }
+      } else if (kept) {
+        r.outside = 1;
+      }
+    }
+  }
+  if (active) part[(size_t)pt * a.nchunks + chunk] = r;
+}
+
+__global__ void jointp_ref_final(LArgs a, const double *cmax, const RefPartial *part,
+                                 double *out, double *ess) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= a.npoints) return;
+  const RefPartial *P = part + (size_t)pt * a.nchunks;
+  const double *cm = cmax + (size_t)pt * a.nchunks;
+  double Mfin = -DBL_MAX;
+  for (int c = 0; c < a.nchunks; ++c) Mfin = fmax(Mfin, cm[c]);
+  double mM;
+  int zM;
+  d_eexp(Mfin, &mM, &zM);
+  const int maxz = zM - 10;
+  double sum = 0.0, sum2 = 0.0, minp = DBL_MAX, minterm = 0.0;
+  int cnt = 0, outside = 0;
+  for (int c = 0; c < a.nchunks; ++c) {
+    sum += P[c].sum;
+    sum2 += P[c].sum2;
+    cnt += P[c].cnt;
+    outside |= P[c].outside;
+    if (P[c].cnt > 0 && P[c].minp < minp) {
+      minp = P[c].minp;
+      minterm = P[c].minterm;
+    }
+  }
+  if (!outside && cnt > 1) {  // the list cap drops the smallest kept value
+    sum -= minterm;
+    sum2 -= minterm * minterm;
+  }
+  out[pt] = log((double)a.G) - (log(sum) + maxz * 2.3025850929940456840);
+  if (ess) ess[pt] = sum * sum / sum2;
+}
+
+static void chunking(const LTable &t, int npoints, int *nchunks, int *chunk_rows) {
+  const int tiles = (npoints + kLTile - 1) / kLTile;
+  int want = (2048 + tiles - 1) / tiles;  // aim at >= 2048 workgroups
+  int rows = (t.G + want - 1) / want;
+  rows = ((rows + kLStage - 1) / kLStage) * kLStage;
+  if (rows < kLStage) rows = kLStage;
+  *chunk_rows = rows;
+  *nchunks = (t.G + rows - 1) / rows;
+}
+
+size_t jointp_work_bytes(const LTable &t, int npoints) {
+  int nc, cr;
+  chunking(t, npoints, &nc, &cr);
+  return (size_t)npoints * nc * (sizeof(double) + sizeof(RefPartial)) + 64;
+}
+
+hipError_t launch_jointp(const LTable &t, const double *x, int npoints, int modeltype,
+                         int r0, int r1, int mode, double *work, size_t work_bytes,
+                         double *out, double *ess, hipStream_t st) {
+  if (npoints == 0) return hipSuccess;
+  LArgs a{t.rows, t.G, t.K, t.npsize, t.nmig, t.npops, modeltype, r0, r1, x, npoints, 0, 0};
+  chunking(t, npoints, &a.nchunks, &a.chunk_rows);
+  if (jointp_work_bytes(t, npoints) > work_bytes) return hipErrorInvalidValue;
+  const dim3 grid((npoints + kLTile - 1) / kLTile, a.nchunks);
+  const size_t lds = (size_t)kLStage * t.K * sizeof(float);
+  const int fin_blocks = (npoints + 255) / 256;
+  if (mode == IMA_LMODE_EXACT) {
+    hipLaunchKernelGGL(jointp_exact_partial, grid, dim3(kLTile), lds, st, a, work);
+    hipLaunchKernelGGL(jointp_exact_final, dim3(fin_blocks), dim3(256), 0, st, a, work, out,
+                       ess);
+  } else {
+    double *cmax = work;
+    RefPartial *part = reinterpret_cast<RefPartial *>(cmax + (size_t)npoints * a.nchunks);
+    hipLaunchKernelGGL(jointp_ref_pass1, grid, dim3(kLTile), lds, st, a, cmax);
+    hipLaunchKernelGGL(jointp_ref_pass2, grid, dim3(kLTile), lds, st, a, cmax, part);
+    hipLaunchKernelGGL(jointp_ref_final, dim3(fin_blocks), dim3(256), 0, st, a, cmax, part,
+                       out, ess);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------- margincalc --
+
+struct MArgs {
+  const float *rows;
+  int G, K, npsize, nmig, param, expo;
+  double meani;
+  const double *x;
+  int nx, nchunks, chunk_rows;
+};
+
+__global__ __launch_bounds__(kLTile) void margin_partial(MArgs a, double *work) {
+  extern __shared__ float shrows[];
+  const int pt = blockIdx.x * kLTile + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const bool active = pt < a.nx;
+  const double x = a.x[active ? pt : 0];
+  const JointLayout L = joint_layout(a.npsize, a.nmig);
+  const double lx = log(x), l2 = kLn2 - log(x), lmean = log(a.meani);
+  double s = 0.0;
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.G, gbeg + a.chunk_rows);
+  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
+    const int cnt = min(kLStage, gend - g0);
+    __syncthreads();
+    const int tot = cnt * a.K;
+    const float *src = a.rows + (size_t)g0 * a.K;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) shrows[i] = src[i];
+    __syncthreads();
+    for (int r = 0; r < cnt; ++r) {
+      const float *g = shrows + (size_t)r * a.K;
+      if (a.param < a.npsize) {
+        const int p = a.param;
+        const float cc = g[L.ccp + p];
+        const long ccr = cc >= 0 ? (long)(cc + 0.5) : (long)(cc - 0.5);  // INTEGERROUND
+        const double temp = -g[L.qip + p] + ccr * l2 - (double)g[L.hccp + p] -
+                            (double)(2 * g[L.fcp + p]) / x;
+        s += exp(temp);
+      } else {
+        const int p = a.param - a.npsize;
+        const float mc = g[L.mcp + p];
+        const long mcr = mc >= 0 ? (long)(mc + 0.5) : (long)(mc - 0.5);
+        if (a.expo)
+          s += exp(-g[L.mip + p] + lmean - x * a.meani + mcr * lx - g[L.fmp + p] * x);
+        else
+          s += exp(-g[L.mip + p] + mcr * lx - g[L.fmp + p] * x);
+      }
+    }
+  }
+  if (active) work[(size_t)pt * a.nchunks + chunk] = s;
+}
+
+__global__ void margin_final(MArgs a, int logi, double yadjust, const double *work,
+                             double *out) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= a.nx) return;
+  double s = 0.0;
+  for (int c = 0; c < a.nchunks; ++c) s += work[(size_t)pt * a.nchunks + c];
+  s /= a.G;
+  if (logi) s = (s <= 0) ? -1e200 : log(s);
+  out[pt] = s - yadjust;
+}
+
+size_t margincalc_work_bytes(const LTable &t, int nx) {
+  int nc, cr;
+  chunking(t, nx, &nc, &cr);
+  return (size_t)nx * nc * sizeof(double) + 64;
+}
+
+hipError_t launch_margincalc(const LTable &t, int param, const double *x, int nx, int logi,
+                             double yadjust, int expo, double prior_mean, double *work,
+                             double *out, hipStream_t st) {
+  if (nx == 0) return hipSuccess;
+  MArgs a{t.rows, t.G, t.K, t.npsize, t.nmig, param, expo,
+          expo ? 1.0 / prior_mean : 1.0, x, nx, 0, 0};
+  chunking(t, nx, &a.nchunks, &a.chunk_rows);
+  const dim3 grid((nx + kLTile - 1) / kLTile, a.nchunks);
+  const size_t lds = (size_t)kLStage * t.K * sizeof(float);
+  hipLaunchKernelGGL(margin_partial, grid, dim3(kLTile), lds, st, a, work);
+  hipLaunchKernelGGL(margin_final, dim3((nx + 255) / 256), dim3(256), 0, st, a, logi,
+                     yadjust, work, out);
+  return hipGetLastError();
+}
+
+hipError_t upload_pow10_table(hipStream_t) { return hipSuccess; }
+
+}  // namespace ima

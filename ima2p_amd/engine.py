@@ -1,0 +1,275 @@
+"""Host-side front end of the HIP engine (thin layer over the C ABI).
+
+``Engine`` owns one context (one HIP stream) on one device.  ``RefShim``
+mirrors the reference's per-(chain, locus) entry points
+(``likelihoodHKY``/``likelihoodIS``/``likelihoodSW``, imamp.hpp:1167-1174,
+and ``swapchains``, swapchains.cpp:527) over genealogy slots, so callers
+written against the reference read the same here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, np.int32)
+    return a, a.ctypes.data_as(L.i32p)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return a, a.ctypes.data_as(L.dp)
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.ima_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise EngineError(f"ima_ctx_create failed with code {rc}")
+        self.h = h
+        self.loci = []          # (model, n, S)
+        self.slot_locus = []
+
+    def close(self):
+        if self.h:
+            self.lib.ima_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.ima_ctx_last_error(self.h).decode(errors="replace")
+            raise EngineError(f"{what}: code {rc}: {msg}")
+
+    # -------------------------------------------------------------- loci --
+    def add_locus_hky(self, loc) -> int:
+        seq = np.ascontiguousarray(loc.seq, np.int8)
+        mult, mp = _i32(loc.mult)
+        pi, pp = _f64(loc.pi)
+        out = C.c_int()
+        self._chk(self.lib.ima_locus_add_hky(self.h, seq.shape[0], seq.shape[1],
+                                             seq.ctypes.data_as(L.i8p), mp, int(loc.totsites),
+                                             pp, C.byref(out)), "add_locus_hky")
+        self.loci.append((L.MODEL_HKY, seq.shape[0], seq.shape[1]))
+        return out.value
+
+    def add_locus_is(self, loc) -> int:
+        seq = np.ascontiguousarray(loc.seq, np.int8)
+        out = C.c_int()
+        self._chk(self.lib.ima_locus_add_is(self.h, seq.shape[0], seq.shape[1],
+                                            seq.ctypes.data_as(L.i8p), C.byref(out)),
+                  "add_locus_is")
+        self.loci.append((L.MODEL_IS, seq.shape[0], seq.shape[1]))
+        return out.value
+
+    def add_locus_sw(self, n: int) -> int:
+        out = C.c_int()
+        self._chk(self.lib.ima_locus_add_sw(self.h, int(n), C.byref(out)), "add_locus_sw")
+        self.loci.append((L.MODEL_SW, int(n), 0))
+        return out.value
+
+    def set_sumlogk(self, locus, v):
+        self._chk(self.lib.ima_locus_set_sumlogk(self.h, locus, float(v)), "set_sumlogk")
+
+    def sumlogk_from_slot(self, locus, slot) -> float:
+        out = C.c_double()
+        self._chk(self.lib.ima_locus_sumlogk_from_slot(self.h, locus, slot, C.byref(out)),
+                  "sumlogk_from_slot")
+        return out.value
+
+    # ------------------------------------------------------------- slots --
+    def alloc_slots(self, loci) -> int:
+        la, lp = _i32(loci)
+        first = C.c_int()
+        self._chk(self.lib.ima_slots_alloc(self.h, len(la), lp, C.byref(first)), "alloc_slots")
+        self.slot_locus += [int(x) for x in la]
+        return first.value
+
+    def write_slots(self, first, gens, A=None):
+        up0, p0 = _i32(np.concatenate([g.up0 for g in gens]))
+        up1, p1 = _i32(np.concatenate([g.up1 for g in gens]))
+        dn, pd = _i32(np.concatenate([g.down for g in gens]))
+        tm, pt = _f64(np.concatenate([g.time for g in gens]))
+        rt, pr = _i32([g.root for g in gens])
+        if A is not None:
+            aa, pa = _i32(np.concatenate([np.asarray(a) for a in A]))
+        else:
+            aa, pa = None, None
+        self._chk(self.lib.ima_slots_write(self.h, first, len(gens), p0, p1, pd, pt, pr, pa),
+                  "write_slots")
+
+    def read_slots(self, first, count):
+        n = [self.loci[self.slot_locus[first + i]][1] for i in range(count)]
+        tot = sum(2 * k - 1 for k in n)
+        up0 = np.zeros(tot, np.int32)
+        up1 = np.zeros(tot, np.int32)
+        dn = np.zeros(tot, np.int32)
+        tm = np.zeros(tot)
+        A = np.zeros(tot, np.int32)
+        rt = np.zeros(count, np.int32)
+        self._chk(self.lib.ima_slots_read(self.h, first, count, up0.ctypes.data_as(L.i32p),
+                                          up1.ctypes.data_as(L.i32p), dn.ctypes.data_as(L.i32p),
+                                          tm.ctypes.data_as(L.dp), rt.ctypes.data_as(L.i32p),
+                                          A.ctypes.data_as(L.i32p)), "read_slots")
+        return up0, up1, dn, tm, rt, A
+
+    # ----------------------------------------------------------- scoring --
+    @staticmethod
+    def items(slots, u, kappa=None, tr=None):
+        slots = np.asarray(slots)
+        arr = (L.Item * len(slots))()
+        u = np.broadcast_to(np.asarray(u, float), slots.shape)
+        kappa = np.broadcast_to(np.asarray(1.0 if kappa is None else kappa, float), slots.shape)
+        tr = np.broadcast_to(np.asarray(1.0 if tr is None else tr, float), slots.shape)
+        for i in range(len(slots)):
+            arr[i].slot = int(slots[i])
+            arr[i].flags = 0
+            arr[i].u = float(u[i])
+            arr[i].kappa = float(kappa[i])
+            arr[i].tr = float(tr[i])
+        return arr
+
+    def score(self, items, precision=L.PREC_FP64):
+        n = len(items)
+        out = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        self._chk(self.lib.ima_score(self.h, items, n, precision, out.ctypes.data_as(L.dp),
+                                     st.ctypes.data_as(L.i32p)), "score")
+        return out, st
+
+    def is_site_nodes(self, slot, S):
+        out = np.zeros(max(S, 1), np.int32)
+        self._chk(self.lib.ima_is_site_nodes(self.h, slot, out.ctypes.data_as(L.i32p)),
+                  "is_site_nodes")
+        return out[:S]
+
+    def sw_dlike(self, slot, u, tr=1.0):
+        it = self.items([slot], u, tr=tr)
+        n = self.loci[self.slot_locus[slot]][1]
+        out = np.zeros(2 * n - 1)
+        self._chk(self.lib.ima_sw_dlike(self.h, it, out.ctypes.data_as(L.dp)), "sw_dlike")
+        return out
+
+    def likelihood_hky(self, locus, g, u, kappa, e=(-1, -1, -1, -1)):
+        up0, p0 = _i32(g.up0)
+        up1, p1 = _i32(g.up1)
+        dn, pd = _i32(g.down)
+        tm, pt = _f64(g.time)
+        out = C.c_double()
+        self._chk(self.lib.ima_likelihood_hky(self.h, locus, p0, p1, pd, pt, int(g.root),
+                                              float(u), float(kappa), *[int(x) for x in e],
+                                              C.byref(out)), "likelihood_hky")
+        return out.value
+
+    # -------------------------------------------------------------- swap --
+    def swap(self, S, beta, swaptries, seed, counter):
+        S, ps = _f64(S)
+        b = np.array(beta, np.float64)
+        pairs = np.zeros(3 * max(swaptries, 1), np.int32)
+        nacc = C.c_int()
+        self._chk(self.lib.ima_swap(self.h, len(S), ps, b.ctypes.data_as(L.dp), swaptries,
+                                    seed, counter, C.byref(nacc), pairs.ctypes.data_as(L.i32p)),
+                  "swap")
+        return b, nacc.value, pairs[: 3 * swaptries].reshape(-1, 3)
+
+    # ------------------------------------------------------------ L mode --
+    def lmode_table(self, table, npsize, nmig, npops) -> int:
+        t = np.ascontiguousarray(table, np.float32)
+        out = C.c_int()
+        self._chk(self.lib.ima_lmode_table(self.h, t.ctypes.data_as(L.fp), t.shape[0],
+                                           t.shape[1], npsize, nmig, npops, C.byref(out)),
+                  "lmode_table")
+        return out.value
+
+    def lmode_free(self, tid):
+        self._chk(self.lib.ima_lmode_table_free(self.h, tid), "lmode_free")
+
+    def jointp(self, tid, x, mode=L.LMODE_EXACT, modeltype=0, prange=None):
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+        npnt, np_ = x.shape
+        r0, r1 = prange if prange is not None else (0, np_)
+        out = np.zeros(npnt)
+        ess = np.zeros(npnt)
+        self._chk(self.lib.ima_lmode_jointp(self.h, tid, x.ctypes.data_as(L.dp), npnt,
+                                            modeltype, r0, r1, mode,
+                                            out.ctypes.data_as(L.dp), ess.ctypes.data_as(L.dp)),
+                  "jointp")
+        return out, ess
+
+    def margincalc(self, tid, param, x, logi=1, yadjust=0.0, expo=False, mean=1.0):
+        x = np.ascontiguousarray(np.atleast_1d(x), np.float64)
+        out = np.zeros(len(x))
+        self._chk(self.lib.ima_lmode_margincalc(self.h, tid, param, x.ctypes.data_as(L.dp),
+                                                len(x), int(logi), float(yadjust), int(expo),
+                                                float(mean), out.ctypes.data_as(L.dp)),
+                  "margincalc")
+        return out
+
+    # ------------------------------------------------------------ timing --
+    def timing(self, on=True):
+        self._chk(self.lib.ima_timing_enable(self.h, int(on)), "timing")
+
+    def timing_get(self, k):
+        n = C.c_int64()
+        ms = C.c_double()
+        self._chk(self.lib.ima_timing_get(self.h, k, C.byref(n), C.byref(ms)), "timing_get")
+        return n.value, ms.value
+
+    def timing_reset(self):
+        self._chk(self.lib.ima_timing_reset(self.h), "timing_reset")
+
+
+class RefShim:
+    """The reference's per-(chain, locus) calls over engine slots.
+
+    ``trees[ci][li]`` holds the Genealogy of chain ci at locus li.  Each
+    call uploads that genealogy to its slot and scores it, like the
+    reference's in-place evaluation of C[ci]->G[li].
+    """
+
+    def __init__(self, engine: Engine, loci_ids, nchains):
+        self.e = engine
+        self.loci_ids = list(loci_ids)
+        self.nchains = nchains
+        slots = [l for _ in range(nchains) for l in self.loci_ids]
+        self.first = engine.alloc_slots(slots)
+        self.sumlogk_done = set()
+
+    def slot(self, ci, li):
+        return self.first + ci * len(self.loci_ids) + li
+
+    def likelihoodHKY(self, ci, li, g, mutrate, kappa, e1=-1, e2=-1, e3=-1, e4=-1):
+        s = self.slot(ci, li)
+        self.e.write_slots(s, [g])
+        out, _ = self.e.score(self.e.items([s], mutrate, kappa))
+        return float(out[0])
+
+    def likelihoodIS(self, ci, li, g, mutrate):
+        s = self.slot(ci, li)
+        self.e.write_slots(s, [g])
+        if li not in self.sumlogk_done:  # first genealogy seen (calc_prob_data.cpp:760-765)
+            self.e.sumlogk_from_slot(self.loci_ids[li], s)
+            self.sumlogk_done.add(li)
+        out, _ = self.e.score(self.e.items([s], mutrate))
+        return float(out[0])
+
+    def likelihoodSW(self, ci, li, g, A, u, tr=1.0):
+        s = self.slot(ci, li)
+        self.e.write_slots(s, [g], A=[A])
+        out, _ = self.e.score(self.e.items([s], u, tr=tr))
+        return float(out[0])

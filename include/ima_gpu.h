@@ -1,0 +1,185 @@
+/*
+ * ima_gpu.h -- C ABI of the MI355X engine for the IMa2p hot path.
+ *
+ * libima_gpu.so (built from ima2p_amd/csrc/) exports exactly the functions
+ * declared here.  Every signature uses plain C types: pointers, sizes and
+ * enums.  All host buffers are owned by the caller and only read or written
+ * during the call; device buffers are owned by the context.
+ *
+ * Conventions (reference boundary: imamp.hpp:1166-1174, 1269-1280,
+ * 1295-1316 declare the functions these replace):
+ *   - every entry point returns IMA_OK (0) or an IMA_ERR_* code; there is
+ *     no exit().  ima_ctx_last_error() returns a text for the last error;
+ *   - per-genealogy outcomes the reference signals with sentinel values
+ *     (REJECTINFINITESITESCONSTANT for IS, -DBL_MAX for SW) are reported
+ *     both as those sentinel values in loglik[] and as status codes;
+ *   - one context = one HIP stream = one host thread at a time.  Contexts
+ *     are independent (no globals), so several may share one device.
+ *
+ * Genealogies use the reference's edge layout (imamp.hpp:651-679): for a
+ * locus with n gene copies, edges 0..n-1 are tips and n..2n-2 internal;
+ * up0/up1 are the daughter edges (-1 at tips), down the parent edge (-1 at
+ * the root edge), time the time at the BOTTOM of the edge.
+ */
+#ifndef IMA_GPU_H
+#define IMA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMA_ABI_VERSION 1
+
+enum ima_rc {
+  IMA_OK = 0,
+  IMA_ERR_ARG = 1,    /* bad argument (index out of range, null pointer) */
+  IMA_ERR_HIP = 2,    /* HIP runtime error */
+  IMA_ERR_STATE = 3,  /* call made in the wrong state */
+  IMA_ERR_SIZE = 4,   /* a size limit of the engine was exceeded */
+  IMA_ERR_NODEV = 5   /* no usable device */
+};
+
+enum ima_model { IMA_MODEL_HKY = 0, IMA_MODEL_IS = 1, IMA_MODEL_SW = 2 };
+enum ima_precision { IMA_PREC_FP64 = 0, IMA_PREC_FP32 = 1 };
+enum ima_status {
+  IMA_STATUS_OK = 0,
+  IMA_STATUS_IS_REJECT = 1, /* two 0/1 nodes at a site: calc_prob_data.cpp:784-787 */
+  IMA_STATUS_SW_ZERO = 2,   /* a Bessel term <= 0: calc_prob_data.cpp:876-897 */
+  IMA_STATUS_IS_NOSITE = 3  /* a site with no 0/1 node (reference: IM_err exit) */
+};
+enum ima_lmode { IMA_LMODE_REF_COMPAT = 0, IMA_LMODE_EXACT = 1 };
+
+#define IMA_REJECT_IS (-1000000000.0) /* REJECTINFINITESITESCONSTANT imamp.hpp:157 */
+#define IMA_MAX_GENES 256             /* gene copies per locus in this engine */
+
+typedef struct ima_ctx ima_ctx;
+
+/* One scoring request: the genealogy in `slot` under its locus' model.
+ * HKY uses u (G->uvals[0]) and kappa; IS uses u; SW uses u and tr (the
+ * time scalar applied to branch lengths). */
+typedef struct {
+  int32_t slot;
+  int32_t flags; /* reserved, 0 */
+  double u;
+  double kappa;
+  double tr;
+} ima_item;
+
+/* ---- context ---------------------------------------------------------- */
+int ima_device_count(int *count);
+int ima_ctx_create(int device, ima_ctx **out);
+void ima_ctx_destroy(ima_ctx *ctx);
+const char *ima_ctx_last_error(const ima_ctx *ctx);
+int ima_abi_version(void);
+
+/* ---- locus data (readata.cpp:246-588 produce these encodings) ---------- */
+/* HKY: seq[n][S] bases 0..3 after gap removal and pattern compression,
+ * mult[S] pattern counts, totsites = sites after gap removal, pi[4]
+ * empirical base frequencies (L[li].seq/mult/totsites, pi[li]). */
+int ima_locus_add_hky(ima_ctx *ctx, int n, int S, const int8_t *seq,
+                      const int32_t *mult, int totsites, const double pi[4],
+                      int *locus);
+/* IS: seq[n][S] 0/1 relative to gene copy 0 (readseqIS). */
+int ima_locus_add_is(ima_ctx *ctx, int n, int S, const int8_t *seq,
+                     int *locus);
+/* SW: allele states travel with each genealogy (ima_slots_write A[]). */
+int ima_locus_add_sw(ima_ctx *ctx, int n, int *locus);
+/* IS constant sum_v log(k_v!) (calc_prob_data.cpp:8,752-765).  The
+ * reference computes it once per locus from the first genealogy scored;
+ * ima_locus_sumlogk_from_slot does the same from a stored genealogy. */
+int ima_locus_set_sumlogk(ima_ctx *ctx, int locus, double sumlogk);
+int ima_locus_sumlogk_from_slot(ima_ctx *ctx, int locus, int slot,
+                                double *sumlogk);
+
+/* ---- genealogy slots --------------------------------------------------- */
+/* Reserve `count` slots, slot i holding a genealogy of locus locus_of[i].
+ * Slot ids are returned consecutively from *first. */
+int ima_slots_alloc(ima_ctx *ctx, int count, const int32_t *locus_of,
+                    int *first);
+/* Write genealogies into slots first..first+count-1.  Arrays are packed
+ * slot after slot, 2n-1 entries per slot (n of that slot's locus).
+ * A may be NULL; for SW loci it holds the allele state of every edge
+ * (gtree[i].A[0]). */
+int ima_slots_write(ima_ctx *ctx, int first, int count, const int32_t *up0,
+                    const int32_t *up1, const int32_t *down,
+                    const double *time, const int32_t *root,
+                    const int32_t *A);
+int ima_slots_read(ima_ctx *ctx, int first, int count, int32_t *up0,
+                   int32_t *up1, int32_t *down, double *time, int32_t *root,
+                   int32_t *A);
+
+/* ---- batched data likelihood ------------------------------------------ */
+/* Scores every item in one launch per mutation model.  Replaces the
+ * per-(chain, locus) calls likelihoodHKY (calc_prob_data.cpp:584),
+ * likelihoodIS (:732) and likelihoodSW (:842).  loglik[i] and status[i]
+ * (may be NULL) receive each item's result. */
+int ima_score(ima_ctx *ctx, const ima_item *items, int nitems, int precision,
+              double *loglik, int32_t *status);
+/* IS diagnostics: the 0/1 node of every site (calc_prob_data.cpp:777-821)
+ * for the genealogy in `slot`; site_node has S entries. */
+int ima_is_site_nodes(ima_ctx *ctx, int slot, int32_t *site_node);
+/* SW per-edge terms dlikeA (calc_prob_data.cpp:879-889) for one item. */
+int ima_sw_dlike(ima_ctx *ctx, const ima_item *item, double *dlikeA);
+
+/* ---- reference-shaped single call (plain per-locus FP64 path) --------- */
+/* likelihoodHKY(ci, li, mutrate, kappa, e1, e2, e3, e4) for one genealogy
+ * given as host arrays: uploads it, launches one FP64 kernel for this
+ * locus and returns the log-likelihood.  e1..e4 name the edges the
+ * reference would recompute; the engine recomputes the whole genealogy on
+ * chip, which gives the same value (the reference's committed partials are
+ * those of the same genealogy). */
+int ima_likelihood_hky(ima_ctx *ctx, int locus, const int32_t *up0,
+                       const int32_t *up1, const int32_t *down,
+                       const double *time, int root, double u, double kappa,
+                       int e1, int e2, int e3, int e4, double *loglik);
+
+/* ---- MC3 swap (swapchains.cpp:527-662, swapweight :13-34) -------------- */
+/* Performs `swaptries` swap attempts among nchains chains whose total
+ * log posterior sums are S[c] = sum_li pdg + probg.  beta[] is updated in
+ * place (betas are swapped, never chain state).  Randomness comes from a
+ * counter-based generator keyed by (seed, counter).  swaps_out (may be
+ * NULL) receives the number of accepted swaps; pairs_out (may be NULL,
+ * 3*swaptries ints) receives (ci, cj, accepted) per try. */
+int ima_swap(ima_ctx *ctx, int nchains, const double *S, double *beta,
+             int swaptries, uint64_t seed, uint64_t counter, int *swaps_out,
+             int32_t *pairs_out);
+
+/* ---- L mode (jointfind.cpp:885-1047, surface_call_functions.cpp:120-173) */
+/* Upload a genealogy table in the savegsampinf layout (ginfo.cpp:288-316):
+ * G rows of K floats; npsize population-size and nmig migration
+ * parameters, npops sampled populations. */
+int ima_lmode_table(ima_ctx *ctx, const float *table, int G, int K,
+                    int npsize, int nmig, int npops, int *table_id);
+int ima_lmode_table_free(ima_ctx *ctx, int table_id);
+/* jointp at npoints points x[npoints][npsize+nmig]; params
+ * [prange0, prange1) are used; modeltype 0 all, 1 popsize-only, 2
+ * migration-only (npops > 2).  mode REF_COMPAT reproduces the reference's
+ * 10-nat window, list cap and eexp summation; EXACT is a plain FP64
+ * log-sum-exp over every genealogy.  out[i] = -log posterior (reference
+ * sign convention); ess (may be NULL) receives the effective sample
+ * number (calc_ess). */
+int ima_lmode_jointp(ima_ctx *ctx, int table_id, const double *x, int npoints,
+                     int modeltype, int prange0, int prange1, int mode,
+                     double *out, double *ess);
+/* margincalc for parameter `param` at nx values x[] (logi, yadjust,
+ * exponential migration prior with mean prior_mean when expo != 0). */
+int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
+                         const double *x, int nx, int logi, double yadjust,
+                         int expo, double prior_mean, double *out);
+
+/* ---- timing ------------------------------------------------------------ */
+/* Per-kernel HIP-event timing on the context's stream (off by default).
+ * ima_timing_get returns, for kernel class k (0 HKY FP64, 1 HKY FP32,
+ * 2 IS, 3 SW, 4 L-mode, 5 swap, 6 MC3 sweep), the number of timed
+ * launches and their total milliseconds. */
+int ima_timing_enable(ima_ctx *ctx, int on);
+int ima_timing_get(ima_ctx *ctx, int kclass, int64_t *launches,
+                   double *total_ms);
+int ima_timing_reset(ima_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

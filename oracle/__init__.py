@@ -1,0 +1,280 @@
+"""ctypes front end of the oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is the CPU restatement of the reference hot path in
+``oracle/ima_oracle.c``.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py may import this module: it is the checker,
+never the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libima_oracle.so")
+REJECT_IS = -1000000000.0
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _SO
+
+
+def _load():
+    if not os.path.exists(_SO):
+        build()
+    lib = C.CDLL(_SO)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    fp = C.POINTER(C.c_float)
+    lib.ora_getstandfactor.restype = C.c_double
+    lib.ora_getstandfactor.argtypes = [dp, C.c_double]
+    lib.ora_likelihoodHKY.restype = C.c_double
+    lib.ora_likelihoodHKY.argtypes = [C.c_void_p, C.c_int, ip, ip, C.c_int, dp,
+                                      C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_void_p]
+    for f in ("ora_copyfraclike", "ora_storescalefactors", "ora_restorescalefactors"):
+        getattr(lib, f).restype = None
+        getattr(lib, f).argtypes = [C.c_void_p, C.c_void_p]
+    lib.ora_labelgtree.restype = None
+    lib.ora_labelgtree.argtypes = [C.c_void_p, ip, C.c_int]
+    lib.ora_calc_sumlogk.restype = C.c_int
+    lib.ora_calc_sumlogk.argtypes = [C.c_void_p, C.c_int, ip, dp]
+    lib.ora_likelihoodIS.restype = C.c_double
+    lib.ora_likelihoodIS.argtypes = [C.c_void_p, C.c_int, ip, C.c_double,
+                                     C.c_double, C.c_double, ip, ip]
+    lib.ora_tree_length.restype = C.c_double
+    lib.ora_tree_length.argtypes = [C.c_void_p]
+    for f in ("ora_bessi0", "ora_bessi1"):
+        getattr(lib, f).restype = C.c_double
+        getattr(lib, f).argtypes = [C.c_double]
+    lib.ora_bessi.restype = C.c_double
+    lib.ora_bessi.argtypes = [C.c_int, C.c_double]
+    lib.ora_likelihoodSW.restype = C.c_double
+    lib.ora_likelihoodSW.argtypes = [C.c_void_p, ip, C.c_double, C.c_double, dp]
+    lib.ora_swapweight.restype = C.c_double
+    lib.ora_swapweight.argtypes = [C.c_double] * 4
+    lib.ora_setheat_geometric.restype = C.c_double
+    lib.ora_setheat_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+    lib.ora_setheat_linear.restype = C.c_double
+    lib.ora_setheat_linear.argtypes = [C.c_int, C.c_double]
+    lib.ora_eexp.restype = None
+    lib.ora_eexp.argtypes = [C.c_double, dp, ip]
+    lib.ora_jointp.restype = C.c_double
+    lib.ora_jointp.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_int, dp, C.c_int, dp]
+    lib.ora_jointp_exact.restype = C.c_double
+    lib.ora_jointp_exact.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, dp]
+    lib.ora_margincalc.restype = C.c_double
+    lib.ora_margincalc.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_double, C.c_int, C.c_int,
+                                   C.c_int, C.c_double]
+    return lib
+
+
+lib = _load()
+
+
+class _Tree(C.Structure):
+    _fields_ = [("n", C.c_int), ("up0", C.POINTER(C.c_int)),
+                ("up1", C.POINTER(C.c_int)), ("down", C.POINTER(C.c_int)),
+                ("time", C.POINTER(C.c_double)), ("root", C.c_int)]
+
+
+class _HKYState(C.Structure):
+    _fields_ = [("n", C.c_int), ("S", C.c_int),
+                ("frac", C.POINTER(C.c_double)), ("newfrac", C.POINTER(C.c_double)),
+                ("scale", C.POINTER(C.c_double)), ("oldscale", C.POINTER(C.c_double))]
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Tree:
+    """Keeps numpy buffers alive for the C struct."""
+
+    def __init__(self, g):
+        self.n = int(g.n)
+        self.up0 = np.ascontiguousarray(g.up0, np.int32)
+        self.up1 = np.ascontiguousarray(g.up1, np.int32)
+        self.down = np.ascontiguousarray(g.down, np.int32)
+        self.time = np.ascontiguousarray(g.time, np.float64)
+        self.s = _Tree(self.n, _ip(self.up0), _ip(self.up1), _ip(self.down),
+                       _dp(self.time), int(g.root))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+class HKYState:
+    def __init__(self, n, S):
+        self.frac = np.zeros((n - 1) * S * 4)
+        self.newfrac = np.zeros((n - 1) * S * 4)
+        self.scale = np.zeros((n - 1) * S)
+        self.oldscale = np.zeros((n - 1) * S)
+        self.s = _HKYState(n, S, _dp(self.frac), _dp(self.newfrac),
+                           _dp(self.scale), _dp(self.oldscale))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def getstandfactor(pi, kappa):
+    pi = np.ascontiguousarray(pi, np.float64)
+    return lib.ora_getstandfactor(_dp(pi), kappa)
+
+
+def likelihood_hky(g, locus, u, kappa, e=(-1, -1, -1, -1), state=None,
+                   commit=True):
+    """likelihoodHKY on genealogy g; a full evaluation commits its partials
+    (copyfraclike + storescalefactors) when commit is set."""
+    t = Tree(g)
+    S = locus.seq.shape[1]
+    if state is None:
+        state = HKYState(g.n, S)
+    seq = np.ascontiguousarray(locus.seq, np.int32)
+    mult = np.ascontiguousarray(locus.mult, np.int32)
+    pi = np.ascontiguousarray(locus.pi, np.float64)
+    v = lib.ora_likelihoodHKY(t.ref, S, _ip(seq), _ip(mult), int(locus.totsites),
+                              _dp(pi), float(u), float(kappa), *[int(x) for x in e],
+                              state.ref)
+    if commit:
+        lib.ora_copyfraclike(t.ref, state.ref)
+        lib.ora_storescalefactors(t.ref, state.ref)
+    return v, state
+
+
+def hky_commit(g, state, accept):
+    t = Tree(g)
+    if accept:
+        lib.ora_copyfraclike(t.ref, state.ref)
+        lib.ora_storescalefactors(t.ref, state.ref)
+    else:
+        lib.ora_restorescalefactors(t.ref, state.ref)
+
+
+def labelgtree_site(g, col):
+    """Run the reference's site labelling; returns labels of all edges."""
+    t = Tree(g)
+    mut = np.full(2 * g.n - 1, -1, np.int32)
+    mut[: g.n] = col
+    for j in range(g.n):
+        lib.ora_labelgtree(t.ref, _ip(mut), j)
+    return mut
+
+
+def calc_sumlogk(g, locus):
+    t = Tree(g)
+    seq = np.ascontiguousarray(locus.seq, np.int32)
+    out = np.zeros(1)
+    rc = lib.ora_calc_sumlogk(t.ref, seq.shape[1], _ip(seq), _dp(out))
+    return rc, float(out[0])
+
+
+def tree_length(g):
+    return lib.ora_tree_length(Tree(g).ref)
+
+
+def likelihood_is(g, locus, u, sumlogk, length=None):
+    t = Tree(g)
+    seq = np.ascontiguousarray(locus.seq, np.int32)
+    S = seq.shape[1]
+    if length is None:
+        length = lib.ora_tree_length(t.ref)
+    st = np.zeros(1, np.int32)
+    nodes = np.full(max(S, 1), -1, np.int32)
+    v = lib.ora_likelihoodIS(t.ref, S, _ip(seq), float(length), float(u),
+                             float(sumlogk), _ip(st), _ip(nodes))
+    return v, int(st[0]), nodes[:S]
+
+
+def bessi(n, x):
+    return lib.ora_bessi(int(n), float(x))
+
+
+def bessi0(x):
+    return lib.ora_bessi0(float(x))
+
+
+def bessi1(x):
+    return lib.ora_bessi1(float(x))
+
+
+def likelihood_sw(g, A, u, tr=1.0):
+    t = Tree(g)
+    A = np.ascontiguousarray(A, np.int32)
+    d = np.zeros(2 * g.n - 1)
+    v = lib.ora_likelihoodSW(t.ref, _ip(A), float(u), float(tr), _dp(d))
+    return v, d
+
+
+def swapweight(bi, bj, si, sj):
+    return lib.ora_swapweight(bi, bj, si, sj)
+
+
+def setheat_geometric(i, nchains, h1, h2):
+    return lib.ora_setheat_geometric(i, nchains, h1, h2)
+
+
+def eexp(x):
+    m = np.zeros(1)
+    z = np.zeros(1, np.int32)
+    lib.ora_eexp(float(x), _dp(m), _ip(z))
+    return float(m[0]), int(z[0])
+
+
+def jointp(table, npsize, nmig, npops, x, modeltype=0, prange=None, ess=False):
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    x = np.ascontiguousarray(x, np.float64)
+    r0, r1 = prange if prange is not None else (0, npsize + nmig)
+    e = np.zeros(1)
+    v = lib.ora_jointp(table.ctypes.data_as(C.POINTER(C.c_float)), G, K, npsize,
+                       nmig, npops, modeltype, r0, r1, _dp(x), int(ess), _dp(e))
+    return (v, float(e[0])) if ess else v
+
+
+def jointp_exact(table, npsize, nmig, npops, x, modeltype=0, prange=None):
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    x = np.ascontiguousarray(x, np.float64)
+    r0, r1 = prange if prange is not None else (0, npsize + nmig)
+    return lib.ora_jointp_exact(table.ctypes.data_as(C.POINTER(C.c_float)), G, K,
+                                npsize, nmig, npops, modeltype, r0, r1, _dp(x))
+
+
+def margincalc(table, npsize, nmig, x, pi, logi=1, yadjust=0.0, expo=False,
+               mean=1.0):
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    return lib.ora_margincalc(table.ctypes.data_as(C.POINTER(C.c_float)), G, K,
+                              npsize, nmig, float(x), float(yadjust), int(pi),
+                              int(logi), int(expo), float(mean))
+
+
+lib.ora_swapchains.restype = C.c_int
+lib.ora_swapchains.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                               C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]
+
+
+def swapchains(S, beta, swaptries, uniforms):
+    S = np.ascontiguousarray(S, np.float64)
+    b = np.array(beta, np.float64)
+    uni = np.ascontiguousarray(uniforms, np.float64)
+    used = np.zeros(1, np.int32)
+    pairs = np.zeros(3 * max(swaptries, 1), np.int32)
+    acc = lib.ora_swapchains(len(S), _dp(S), _dp(b), swaptries, _dp(uni), len(uni),
+                             _ip(used), _ip(pairs))
+    return b, acc, pairs[: 3 * swaptries].reshape(-1, 3), int(used[0])

@@ -1,0 +1,678 @@
+/*
+ * ima_oracle.c -- CPU restatement of the IMa2p hot path (TEST INFRASTRUCTURE).
+ *
+ * This file is the oracle: the HIP engine is checked against it.  It is
+ * built by oracle/Makefile into oracle/_build/libima_oracle.so and loaded
+ * only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Nothing in the product path calls it.
+ *
+ * Each function follows the cited reference function step by step, in code
+ * of our own, on plain arrays instead of the reference's global structs.
+ */
+#include "ima_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORA_LOG2 0.69314718055994530941723212146   /* imamp.hpp:188 */
+#define ORA_LOG10 2.3025850929940456840            /* imamp.hpp:187 */
+#define ORA_MYDBL_MAX (DBL_MAX / 1e10)             /* imamp.hpp:171 */
+#define ORA_BESSI_MIN (-1e+100)                    /* imamp.hpp:173 */
+
+static double branch_len(const ora_tree *t, int e) {
+  int u = t->up0[e];
+  return (u == -1) ? t->time[e] : t->time[e] - t->time[u];
+}
+
+/* =====================================================================
+ * HKY -- calc_prob_data.cpp:118-471 (makefrac), :474-493 (getstandfactor),
+ * :584-607 (likelihoodHKY); commit/rollback update_gtree_common.cpp:2140-2181
+ * ===================================================================== */
+
+double ora_getstandfactor(const double pi[4], double kappa) {
+  /* calc_prob_data.cpp:474-493: sum over i != j of pi_i pi_j, times kappa
+   * for transitions (purine<->purine A/G = 0/2, pyrimidine C/T = 1/3). */
+  double p = 0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      if (i == j) continue;
+      if (i + j == 2 || i + j == 4)
+        p += pi[i] * pi[j] * kappa;
+      else
+        p += pi[i] * pi[j];
+    }
+  return p;
+}
+
+/* HKY transition matrix P[from][to] for mutation-scaled branch length
+ * (-mutrate * t) = mt, in the closed form used by makefrac
+ * (calc_prob_data.cpp:197-208, 250-293): the class sum PIj and the
+ * eigenvalue A = 1 + PIj (kappa - 1) are those of the destination base. */
+static void hky_pmatrix(const double pi[4], double kappa, double mt,
+                        double P[4][4]) {
+  double kpa = kappa - 1.0;
+  double expp = exp(mt);
+  for (int to = 0; to < 4; to++) {
+    double PIj = (to == 0 || to == 2) ? pi[0] + pi[2] : pi[1] + pi[3];
+    double A = 1.0 + PIj * kpa;
+    double divm = 1.0 / PIj - 1.0;
+    double eA = exp(mt * A);
+    for (int from = 0; from < 4; from++) {
+      double v;
+      if (from == to)
+        v = pi[to] + pi[to] * expp * divm + eA * ((PIj - pi[to]) / PIj);
+      else if (from + to == 2 || from + to == 4)
+        v = pi[to] + pi[to] * expp * divm - (pi[to] / PIj) * eA;
+      else
+        v = pi[to] * (1.0 - expp);
+      P[from][to] = v;
+    }
+  }
+}
+
+typedef struct {
+  const ora_tree *t;
+  int S;
+  const int *seq;
+  const double *pi;
+  double mutrate, kappa;
+  int e1, e2, e3, e4;
+  ora_hky_state *st;
+} hky_ctx;
+
+static double *nf_row(ora_hky_state *st, int n, int edge) {
+  return st->newfrac + (size_t)(edge - n) * st->S * 4;
+}
+static double *f_row(ora_hky_state *st, int n, int edge) {
+  return st->frac + (size_t)(edge - n) * st->S * 4;
+}
+static double *sc_row(ora_hky_state *st, int n, int edge) {
+  return st->scale + (size_t)(edge - n) * st->S;
+}
+
+/* makefrac (calc_prob_data.cpp:118-471): returns 1 when the partial at the
+ * top of `node` (or below a tip) must be recomputed, i.e. node is one of
+ * e1..e4 (or e1 == -1) or any descendant was recomputed. */
+static int makefrac(hky_ctx *c, int node) {
+  const ora_tree *t = c->t;
+  int n = t->n;
+  int ret = (node == c->e1 || node == c->e2 || node == c->e3 ||
+             node == c->e4 || c->e1 == -1);
+  int up1 = t->up0[node];
+  if (up1 == -1) return ret;
+  int up2 = t->up1[node];
+  const double *fp1 = NULL, *fp2 = NULL;
+  int r1 = makefrac(c, up1);
+  if (up1 >= n) fp1 = r1 ? nf_row(c->st, n, up1) : f_row(c->st, n, up1);
+  if (r1) ret = 1;
+  int r2 = makefrac(c, up2);
+  if (up2 >= n) fp2 = r2 ? nf_row(c->st, n, up2) : f_row(c->st, n, up2);
+  if (r2) ret = 1;
+  if (!ret) return 0;
+
+  double P1[4][4], P2[4][4];
+  hky_pmatrix(c->pi, c->kappa, -c->mutrate * branch_len(t, up1), P1);
+  hky_pmatrix(c->pi, c->kappa, -c->mutrate * branch_len(t, up2), P2);
+  double *out = nf_row(c->st, n, node);
+  double *osc = sc_row(c->st, n, node);
+  const double *s1 = (up1 >= n) ? sc_row(c->st, n, up1) : NULL;
+  const double *s2 = (up2 >= n) ? sc_row(c->st, n, up2) : NULL;
+  int S = c->S;
+  for (int i = 0; i < S; i++) {
+    double v1[4], v2[4], mx = 0.0;
+    for (int j = 0; j < 4; j++) {
+      if (fp1 == NULL) {
+        v1[j] = P1[j][c->seq[(size_t)up1 * S + i]];
+      } else {
+        const double *f = fp1 + (size_t)i * 4;
+        v1[j] = P1[j][0] * f[0] + P1[j][1] * f[1] + P1[j][2] * f[2] + P1[j][3] * f[3];
+      }
+      if (fp2 == NULL) {
+        v2[j] = P2[j][c->seq[(size_t)up2 * S + i]];
+      } else {
+        const double *f = fp2 + (size_t)i * 4;
+        v2[j] = P2[j][0] * f[0] + P2[j][1] * f[1] + P2[j][2] * f[2] + P2[j][3] * f[3];
+      }
+    }
+    for (int j = 0; j < 4; j++) {
+      out[i * 4 + j] = v1[j] * v2[j];
+      if (out[i * 4 + j] > mx) mx = out[i * 4 + j];
+    }
+    for (int j = 0; j < 4; j++) out[i * 4 + j] = out[i * 4 + j] / mx;
+    osc[i] = (s1 ? s1[i] : 0.0) + (s2 ? s2[i] : 0.0) + log(mx);
+  }
+  return 1;
+}
+
+double ora_likelihoodHKY(const ora_tree *t, int S, const int *seq,
+                         const int *mult, int totsites, const double pi[4],
+                         double u, double kappa, int e1, int e2, int e3, int e4,
+                         ora_hky_state *st) {
+  int n = t->n;
+  /* calc_prob_data.cpp:593: per-site rate normalised by totsites and the
+   * expected substitution rate of the unnormalised HKY matrix. */
+  double mutrate = u / (totsites * ora_getstandfactor(pi, kappa));
+  for (int i = n; i < 2 * n - 1; i++) nf_row(st, n, i)[0] = -1; /* :594-595 */
+  hky_ctx c = {t, S, seq, pi, mutrate, kappa, e1, e2, e3, e4, st};
+  makefrac(&c, t->root);
+  const double *r = nf_row(st, n, t->root);
+  const double *rs = sc_row(st, n, t->root);
+  double p = 0;
+  for (int i = 0; i < S; i++) { /* :597-605 */
+    double fracp = 0;
+    for (int j = 0; j < 4; j++) fracp += pi[j] * r[i * 4 + j];
+    p += mult[i] * (log(fracp) + rs[i]);
+  }
+  return p;
+}
+
+void ora_copyfraclike(const ora_tree *t, ora_hky_state *st) {
+  int n = t->n;
+  for (int i = n; i < 2 * n - 1; i++) {
+    double *nf = nf_row(st, n, i);
+    if (nf[0] != -1) memcpy(f_row(st, n, i), nf, sizeof(double) * st->S * 4);
+  }
+}
+void ora_storescalefactors(const ora_tree *t, ora_hky_state *st) {
+  memcpy(st->oldscale, st->scale, sizeof(double) * (size_t)(t->n - 1) * st->S);
+}
+void ora_restorescalefactors(const ora_tree *t, ora_hky_state *st) {
+  memcpy(st->scale, st->oldscale, sizeof(double) * (size_t)(t->n - 1) * st->S);
+}
+
+/* =====================================================================
+ * IS -- calc_prob_data.cpp:537-581 (labelgtree), :610-719 (calc_sumlogk),
+ * :732-836 (likelihoodIS)
+ * ===================================================================== */
+
+void ora_labelgtree(const ora_tree *t, int *mut, int edge) {
+  /* Push the label of `edge` down toward the root, combining with the
+   * sister's label (2 = conflict, -1 = not yet labelled). */
+  for (;;) {
+    int dow1 = t->down[edge];
+    if (dow1 == -1) break;
+    int sis = t->up0[dow1];
+    if (sis == edge) sis = t->up1[dow1];
+    int changed = 0;
+    if (mut[dow1] != mut[edge]) {
+      if (mut[edge] == 2) {
+        mut[dow1] = (mut[sis] != -1) ? mut[sis] : 2;
+      } else if (mut[sis] != -1 && mut[sis] != 2 && mut[sis] != mut[edge]) {
+        mut[dow1] = 2;
+      } else {
+        mut[dow1] = mut[edge];
+      }
+      changed = 1;
+    } else if (mut[dow1] == 2) {
+      if (mut[sis] != -1 && mut[sis] != 2) {
+        mut[dow1] = mut[sis];
+        changed = 1;
+      }
+    }
+    if (!changed) break;
+    edge = dow1;
+  }
+}
+
+static void label_site(const ora_tree *t, const int *seq, int S, int site,
+                       int *mut) {
+  int n = t->n;
+  for (int j = 0; j < n; j++) mut[j] = seq[(size_t)j * S + site];
+  for (int j = n; j < 2 * n - 1; j++) mut[j] = -1;
+  for (int j = 0; j < n; j++) ora_labelgtree(t, mut, j);
+}
+
+static int is01(const int *mut, int a, int b) {
+  return (mut[a] == 0 && mut[b] == 1) || (mut[a] == 1 && mut[b] == 0);
+}
+
+int ora_calc_sumlogk(const ora_tree *t, int S, const int *seq,
+                     double *sumlogk) {
+  int n = t->n;
+  int *mut = (int *)malloc(sizeof(int) * (2 * n - 1));
+  int *mutcount = (int *)calloc((size_t)n, sizeof(int));
+  for (int site = 0; site < S; site++) {
+    label_site(t, seq, S, site, mut);
+    int ret = -1;
+    for (int node = n; node < 2 * n - 1; node++) {
+      if (is01(mut, t->up0[node], t->up1[node])) {
+        if (node != ret && ret != -1) { /* :654-658 leaves the sum untouched */
+          free(mut);
+          free(mutcount);
+          return -2;
+        }
+        ret = node;
+        mutcount[node - n]++;
+      }
+    }
+    if (ret == -1) {
+      free(mut);
+      free(mutcount);
+      return -1;
+    }
+  }
+  double sum = 0.0;
+  for (int i = 0; i < n - 1; i++) { /* :710-716 log(k!) as a product */
+    double fact = 1.0;
+    for (int j = 1; j <= mutcount[i]; j++) fact *= (double)j;
+    sum += log(fact);
+  }
+  *sumlogk = sum;
+  free(mut);
+  free(mutcount);
+  return 0;
+}
+
+double ora_tree_length(const ora_tree *t) {
+  double L = 0;
+  for (int i = 0; i < 2 * t->n - 1; i++)
+    if (t->down[i] != -1) L += branch_len(t, i);
+  return L;
+}
+
+double ora_likelihoodIS(const ora_tree *t, int S, const int *seq,
+                        double length, double u, double sumlogk, int *status,
+                        int *site_node) {
+  int n = t->n;
+  int *mut = (int *)malloc(sizeof(int) * (2 * n - 1));
+  double p = -length * u; /* :766 */
+  *status = 0;
+  for (int site = 0; site < S; site++) {
+    label_site(t, seq, S, site, mut);
+    int ret = -1;
+    double ptime = 0;
+    for (int node = n; node < 2 * n - 1; node++) {
+      int up1 = t->up0[node], up2 = t->up1[node];
+      if (!is01(mut, up1, up2)) continue;
+      if (node != ret && ret != -1) {
+        free(mut);
+        return ORA_REJECT_IS; /* :784-787 */
+      }
+      if (t->down[node] == -1) { /* root: the two daughter branches */
+        ptime = branch_len(t, up1) + branch_len(t, up2);
+      } else if (mut[t->down[node]] == mut[up1]) {
+        ptime = branch_len(t, up2);
+      } else {
+        ptime = branch_len(t, up1);
+      }
+      ret = node;
+    }
+    if (site_node) site_node[site] = ret;
+    if (ret == -1) { /* reference: IM_err(IMERR_INFINITESITESFAIL) */
+      *status = -1;
+      free(mut);
+      return ORA_REJECT_IS;
+    }
+    p = p + log(ptime * u);
+  }
+  p -= sumlogk;
+  free(mut);
+  return p;
+}
+
+/* =====================================================================
+ * SW -- utilities.cpp:55-124 (bessi0/bessi1, Abramowitz & Stegun 9.8.1-4
+ * polynomial approximations), :1453-1491 (bessi, Miller downward
+ * recurrence), calc_prob_data.cpp:842-909 (likelihoodSW)
+ * ===================================================================== */
+
+double ora_bessi0(double x) {
+  double ax = fabs(x), y;
+  if (ax < 3.75) {
+    y = x / 3.75;
+    y *= y;
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 +
+           y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  y = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) *
+         (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 +
+          y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 +
+          y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+double ora_bessi1(double x) {
+  double ax = fabs(x), y, ans;
+  if (ax < 3.75) {
+    y = x / 3.75;
+    y *= y;
+    ans = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 +
+          y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    y = 3.75 / ax;
+    ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 +
+          y * (0.163801e-2 + y * (-0.1031555e-1 + y * ans))));
+    ans *= (exp(ax) / sqrt(ax));
+  }
+  return x < 0.0 ? -ans : ans;
+}
+
+double ora_bessi(int n, double x) {
+  const double ACC = 40.0, BIGNO = 1.0e10, BIGNI = 1.0e-10;
+  n = abs(n);
+  if (x > 700) return ORA_MYDBL_MAX;
+  if (n == 0) return ora_bessi0(x);
+  if (n == 1) return ora_bessi1(x);
+  if (x == 0.0) return 0.0;
+  double tox = 2.0 / fabs(x), bip = 0.0, ans = 0.0, bi = 1.0, bim;
+  for (int j = 2 * (n + (int)sqrt(ACC * n)); j > 0; j--) {
+    bim = bip + j * tox * bi;
+    bip = bi;
+    bi = bim;
+    if (fabs(bi) > BIGNO) {
+      ans *= BIGNI;
+      bi *= BIGNI;
+      bip *= BIGNI;
+    }
+    if (j == n) ans = bip;
+  }
+  ans *= ora_bessi0(x) / bi;
+  return (x < 0.0 && (n & 1)) ? -ans : ans;
+}
+
+double ora_likelihoodSW(const ora_tree *t, const int *A, double u, double tr,
+                        double *dlikeA) {
+  double like = 0.0;
+  int iszero = 0;
+  for (int i = 0; i < 2 * t->n - 1; i++) {
+    if (t->down[i] == -1) {
+      if (dlikeA) dlikeA[i] = 0.0;
+      continue;
+    }
+    int d = A[i] - A[t->down[i]];
+    double tt = (t->up0[i] == -1) ? t->time[i] * tr
+                                  : tr * (t->time[i] - t->time[t->up0[i]]);
+    double b = ora_bessi(d, tt * u);
+    double v;
+    if (!(b > 0.0)) {
+      iszero = 1;
+      v = ORA_BESSI_MIN;
+    } else {
+      v = -(tt * u) + log(b);
+    }
+    if (dlikeA) dlikeA[i] = v;
+    like += v;
+  }
+  if (iszero) like = -DBL_MAX;
+  return like;
+}
+
+/* =====================================================================
+ * MC3 swap -- swapchains.cpp:13-34 (swapweight), :72-178 (setheat)
+ * ===================================================================== */
+
+double ora_swapweight(double beta_i, double beta_j, double sum_i,
+                      double sum_j) {
+  return exp((beta_i - beta_j) * (sum_j - sum_i));
+}
+
+double ora_setheat_geometric(int i, int nchains, double hval1, double hval2) {
+  return 1 - (1 - hval2) * (i) * pow(hval1, (double)(nchains - 1 - i)) /
+                 (double)(nchains - 1);
+}
+
+double ora_setheat_linear(int i, double hval1) {
+  if (hval1 < 0.05) hval1 = 0.05;
+  return 1.0 / (1.0 + hval1 * i);
+}
+
+/* =====================================================================
+ * L mode -- utilities.cpp:1507-1539 (eexp), jointfind.cpp:885-1047 (jointp),
+ * surface_call_functions.cpp:120-173 (margincalc)
+ * ===================================================================== */
+
+static const double eexp_us[10] = {
+    1.0, 0.5, 0.16666666666666666666666666667, 0.04166666666666666666666666667,
+    0.00833333333333333333333333333, 0.001388888888888888888888888889,
+    0.000198412698412698412698412698, 0.000024801587301587301587301587301,
+    2.75573192239858906525573192239859e-6, 2.75573192239858906525573192239e-7};
+
+void ora_eexp(double x, double *m, int *z) {
+  /* e^x = m * 10^z with a Taylor series for e^u, u = x - n ln2 in [0, ln2) */
+  const double LOG_10_2 = 0.30102999566398119521;
+  int n = (int)floor(x / ORA_LOG2);
+  double zr = LOG_10_2 * (double)n;
+  *z = (int)zr;
+  zr -= (double)*z;
+  double u = x - (((double)n) * ORA_LOG2);
+  double temp = 1 + u * (eexp_us[0] + u * (eexp_us[1] + u * (eexp_us[2] +
+                u * (eexp_us[3] + u * (eexp_us[4] + u * (eexp_us[5] +
+                u * (eexp_us[6] + u * (eexp_us[7] + u * (eexp_us[8] +
+                u * (eexp_us[9]))))))))));
+  *m = temp * pow(10.0, zr);
+  if (fabs(*m) > 10) {
+    *m = *m / 10.0;
+    *z = *z + 1;
+  }
+  if (fabs(*m) < 1) {
+    *m = *m * 10.0;
+    *z = *z - 1;
+  }
+}
+
+/* Per-genealogy log term of jointp (jointfind.cpp:966-995). */
+static double jointp_term(const float *g, int npsize, int nmig, int npops,
+                          int modeltype, int r0, int r1, const double *x,
+                          const double *log2diffx, const double *logx,
+                          const double *divx) {
+  int ccp = 0, fcp = ccp + npsize, hccp = fcp + npsize, mcp = hccp + npsize;
+  int fmp = mcp + nmig, qip = fmp + nmig, mip = qip + npsize;
+  int pdgp = mip + nmig, probgp = pdgp + 1;
+  double p;
+  if (npops == 2) {
+    p = -g[probgp];
+  } else {
+    double s = 0;
+    if (modeltype == 1)
+      for (int i = 0; i < npsize; i++) s += g[qip + i];
+    else
+      for (int i = 0; i < nmig; i++) s += g[mip + i];
+    p = -s;
+  }
+  for (int i = r0; i < r1; i++) {
+    if (i < npsize) {
+      double dg = 2.0 * g[fcp + i];
+      p += g[ccp + i] * log2diffx[i] - g[hccp + i] - dg * divx[i];
+    } else {
+      int i1 = i - npsize;
+      p += g[mcp + i1] * logx[i] - g[fmp + i1] * x[i];
+    }
+  }
+  return p;
+}
+
+static int cmp_desc(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x < y) - (x > y);
+}
+
+double ora_jointp(const float *table, int G, int K, int npsize, int nmig,
+                  int npops, int modeltype, int r0, int r1, const double *x,
+                  int calc_ess, double *effective_n) {
+  int np = npsize + nmig;
+  double *log2diffx = (double *)calloc((size_t)np + 1, sizeof(double));
+  double *logx = (double *)calloc((size_t)np + 1, sizeof(double));
+  double *divx = (double *)calloc((size_t)np + 1, sizeof(double));
+  if (modeltype == 0 || modeltype == 1)
+    for (int i = 0; i < npsize; i++) log2diffx[i] = ORA_LOG2 - log(x[i]);
+  for (int i = r0; i < r1; i++) {
+    logx[i] = log(x[i]);
+    divx[i] = 1.0 / x[i];
+  }
+  /* The reference keeps a sorted list of the values that lie within
+   * PRANGELOG = 10 of the running maximum at the time they are seen
+   * (jointfind.cpp:997-1009; the first value is always kept).  It then walks
+   * down from the maximum, taking at most (list size - 1) values while they
+   * stay within 10 of the maximum (:1010-1022), and sums them as eexp
+   * mantissa/exponent pairs (:1016-1044). */
+  double *kept = (double *)malloc(sizeof(double) * (size_t)G);
+  int nkept = 0;
+  double runmax = -INFINITY;
+  for (int gi = 0; gi < G; gi++) {
+    double p = jointp_term(table + (size_t)gi * K, npsize, nmig, npops,
+                           modeltype, r0, r1, x, log2diffx, logx, divx);
+    if (gi == 0 || runmax - p < 10) {
+      kept[nkept++] = p;
+      if (p > runmax) runmax = p;
+    }
+  }
+  qsort(kept, (size_t)nkept, sizeof(double), cmp_desc);
+  int iin = nkept - 1;
+  int gin = 0;
+  double top = kept[0];
+  double *mm = (double *)malloc(sizeof(double) * (size_t)nkept);
+  int *zz = (int *)malloc(sizeof(int) * (size_t)nkept);
+  int maxz = -10000000;
+  do {
+    ora_eexp(kept[gin], &mm[gin], &zz[gin]);
+    if (zz[gin] > maxz) maxz = zz[gin];
+    gin++;
+  } while (gin < iin && top - kept[gin] < 10);
+  maxz -= 10;
+  double acumm = 0, acumm_sqr = 0;
+  for (int gi = 0; gi < gin; gi++) {
+    int zadj = zz[gi] - maxz;
+    double m = mm[gi];
+    if (zadj > -308 && zadj < 308)
+      m *= pow(10.0, (double)zadj);
+    else
+      m = (zadj <= -308) ? 0.0 : DBL_MAX;
+    acumm += m;
+    if (calc_ess) acumm_sqr += m * m;
+  }
+  if (calc_ess && effective_n) *effective_n = acumm * acumm / acumm_sqr;
+  double sum = log(acumm) + maxz * ORA_LOG10;
+  double q = log((double)G) - sum;
+  free(kept);
+  free(mm);
+  free(zz);
+  free(log2diffx);
+  free(logx);
+  free(divx);
+  return q;
+}
+
+double ora_jointp_exact(const float *table, int G, int K, int npsize, int nmig,
+                        int npops, int modeltype, int r0, int r1,
+                        const double *x) {
+  int np = npsize + nmig;
+  double *log2diffx = (double *)calloc((size_t)np + 1, sizeof(double));
+  double *logx = (double *)calloc((size_t)np + 1, sizeof(double));
+  double *divx = (double *)calloc((size_t)np + 1, sizeof(double));
+  for (int i = 0; i < npsize; i++) log2diffx[i] = ORA_LOG2 - log(x[i]);
+  for (int i = r0; i < r1; i++) {
+    logx[i] = log(x[i]);
+    divx[i] = 1.0 / x[i];
+  }
+  double mx = -INFINITY, s = 0;
+  for (int gi = 0; gi < G; gi++) {
+    double p = jointp_term(table + (size_t)gi * K, npsize, nmig, npops,
+                           modeltype, r0, r1, x, log2diffx, logx, divx);
+    if (p > mx) {
+      s = s * exp(mx - p) + 1.0;
+      mx = p;
+    } else {
+      s += exp(p - mx);
+    }
+  }
+  free(log2diffx);
+  free(logx);
+  free(divx);
+  return log((double)G) - (mx + log(s));
+}
+
+static long int_round(double x) {
+  return (x >= 0) ? (long)(x + 0.5) : (long)(x - 0.5); /* imamp.hpp:180 */
+}
+
+double ora_margincalc(const float *table, int G, int K, int npsize, int nmig,
+                      double x, double yadjust, int pi, int logi,
+                      int expo_prior, double prior_mean) {
+  int ccp = 0, fcp = ccp + npsize, hccp = fcp + npsize, mcp = hccp + npsize;
+  int fmp = mcp + nmig, qip = fmp + nmig, mip = qip + npsize;
+  double sum = 0, meani = 0;
+  (void)K;
+  if (expo_prior && pi >= npsize && pi < npsize + nmig) meani = 1.0 / prior_mean;
+  for (int ei = 0; ei < G; ei++) {
+    const float *g = table + (size_t)ei * K;
+    if (pi < npsize) {
+      int p = pi;
+      double hval = g[hccp + p];
+      double temp = -g[qip + p] +
+                    int_round(g[ccp + p]) * (ORA_LOG2 - log(x)) - hval -
+                    2 * g[fcp + p] / x;
+      sum += exp(temp);
+    } else {
+      int p = pi - npsize;
+      if (expo_prior)
+        sum += exp(-g[mip + p] + log(meani) - x * meani +
+                   int_round(g[mcp + p]) * log(x) - g[fmp + p] * x);
+      else
+        sum += exp(-g[mip + p] + int_round(g[mcp + p]) * log(x) -
+                   g[fmp + p] * x);
+    }
+  }
+  sum /= G;
+  if (logi) sum = (sum <= 0) ? -1e200 : log(sum);
+  sum -= yadjust;
+  return sum;
+}
+
+/* =====================================================================
+ * swapchains (swapchains.cpp:527-653), serial within-process form, drawing
+ * its uniforms from uni[] in order (the reference calls uniform()).
+ * Returns the number of accepted swaps; *used = uniforms consumed;
+ * pairs (optional, 3 per try) = (ci, cj, accepted).  -1 when uni runs out.
+ * ===================================================================== */
+int ora_swapchains(int nchains, const double *S, double *beta, int swaptries,
+                   const double *uni, int nuni, int *used, int *pairs) {
+  const int SWAPDIST = 7;
+  int k = 0, acc = 0;
+#define NEXTU() (k < nuni ? uni[k++] : -1.0)
+  for (int i = 0; i < swaptries; i++) {
+    int ci, cj, cjmin, cjrange;
+    do {
+      double u = NEXTU();
+      if (u < 0) return -1;
+      ci = (int)(u * nchains);
+    } while (ci < 0 || ci >= nchains);
+    if (nchains < 2 * SWAPDIST + 3) {
+      cjmin = 0;
+      cjrange = nchains;
+    } else {
+      cjmin = (ci - SWAPDIST > 0) ? ci - SWAPDIST : 0;
+      cjrange = ((nchains < ci + SWAPDIST) ? nchains : ci + SWAPDIST) - cjmin;
+    }
+    do {
+      double u = NEXTU();
+      if (u < 0) return -1;
+      cj = cjmin + (int)(u * cjrange);
+    } while (cj == ci || cj < 0 || cj >= nchains);
+    double w = ora_swapweight(beta[ci], beta[cj], S[ci], S[cj]);
+    int ok = 0;
+    if (w >= 1.0) {
+      ok = 1;
+    } else {
+      double u = NEXTU();
+      if (u < 0) return -1;
+      ok = w > u;
+    }
+    if (ok) {
+      double b = beta[ci];
+      beta[ci] = beta[cj];
+      beta[cj] = b;
+      acc++;
+    }
+    if (pairs) {
+      pairs[3 * i] = ci;
+      pairs[3 * i + 1] = cj;
+      pairs[3 * i + 2] = ok;
+    }
+  }
+#undef NEXTU
+  if (used) *used = k;
+  return acc;
+}

@@ -1,0 +1,114 @@
+/*
+ * ima_oracle.h -- CPU restatement of the IMa2p hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * engine (ima2p_amd/csrc) is compared against.  Only tests/, the smoke()
+ * entry point and the cpu_baseline leg of bench.py may load it; the
+ * product path never links or calls it.
+ *
+ * Every function restates one reference function, cited file:line under
+ * the reference tree (src/...).  Trees use the reference's edge layout
+ * (imamp.hpp:651-679): edges 0..n-1 are tips, n..2n-2 are internal;
+ * up0/up1 = daughter edges (-1 for tips), down = parent edge (-1 for the
+ * root edge), time = time at the BOTTOM of the edge (the coalescence
+ * where it joins its sister).  Branch length of edge i is
+ * time[i] - time[up0[i]] (internal) or time[i] (tip).
+ *
+ * Parity status of each function (how it is pinned) is documented in the
+ * test files that exercise it; functions with no independent check say
+ * "parity unpinned" below.
+ */
+#ifndef IMA_ORACLE_H
+#define IMA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_REJECT_IS (-1000000000.0) /* REJECTINFINITESITESCONSTANT imamp.hpp:157 */
+#define ORA_TIMEMAX 1000000.0         /* TIMEMAX imamp.hpp:129 */
+
+typedef struct {
+  int n;             /* tips */
+  const int *up0;    /* [2n-1] */
+  const int *up1;    /* [2n-1] */
+  const int *down;   /* [2n-1] */
+  const double *time;/* [2n-1] */
+  int root;          /* root edge (down == -1) */
+} ora_tree;
+
+/* ---------------- HKY (calc_prob_data.cpp:118-607) ---------------- */
+typedef struct {
+  int n, S;
+  double *frac;      /* committed partials  [(n-1)][S][4] (internal edge i -> row i-n) */
+  double *newfrac;   /* proposal partials   [(n-1)][S][4] */
+  double *scale;     /* current scale       [(n-1)][S]    */
+  double *oldscale;  /* committed scale     [(n-1)][S]    */
+} ora_hky_state;
+
+double ora_getstandfactor(const double pi[4], double kappa);
+/* likelihoodHKY: u is the locus mutation scalar (G->uvals[0]); e1 == -1
+ * recomputes every node, otherwise only e1..e4 and their ancestors. */
+double ora_likelihoodHKY(const ora_tree *t, int S, const int *seq /*[n][S]*/,
+                         const int *mult, int totsites, const double pi[4],
+                         double u, double kappa, int e1, int e2, int e3, int e4,
+                         ora_hky_state *st);
+void ora_copyfraclike(const ora_tree *t, ora_hky_state *st);
+void ora_storescalefactors(const ora_tree *t, ora_hky_state *st);
+void ora_restorescalefactors(const ora_tree *t, ora_hky_state *st);
+
+/* ---------------- IS (calc_prob_data.cpp:537-836) ---------------- */
+/* labelgtree over a caller-provided label array mut[2n-1] */
+void ora_labelgtree(const ora_tree *t, int *mut, int edge);
+/* calc_sumlogk: returns 0 on success, -1 if a site has no 0/1 node,
+ * -2 if the genealogy is incompatible (early return, sum untouched). */
+int ora_calc_sumlogk(const ora_tree *t, int S, const int *seq /*[n][S]*/,
+                     double *sumlogk);
+/* likelihoodIS; length = total branch length of the genealogy
+ * (G->length from treeweight).  Returns ORA_REJECT_IS when two 0/1 nodes
+ * exist.  *status = -1 when a site has no 0/1 node (reference: IM_err).
+ * site_node (optional, [S]) receives the 0/1 node of each site. */
+double ora_likelihoodIS(const ora_tree *t, int S, const int *seq,
+                        double length, double u, double sumlogk,
+                        int *status, int *site_node);
+double ora_tree_length(const ora_tree *t);
+
+/* ---------------- SW (calc_prob_data.cpp:842-909, utilities.cpp) ---- */
+double ora_bessi0(double x);
+double ora_bessi1(double x);
+double ora_bessi(int n, double x);
+/* A[2n-1] allele states (tips = data, internal = current states). dlikeA
+ * (optional, [2n-1]) receives the per-edge terms. */
+double ora_likelihoodSW(const ora_tree *t, const int *A, double u, double tr,
+                        double *dlikeA);
+
+/* ---------------- MC3 swap (swapchains.cpp) ---------------- */
+double ora_swapweight(double beta_i, double beta_j, double sum_i, double sum_j);
+double ora_setheat_geometric(int i, int nchains, double hval1, double hval2);
+double ora_setheat_linear(int i, double hval1);
+int ora_swapchains(int nchains, const double *S, double *beta, int swaptries,
+                   const double *uni, int nuni, int *used, int *pairs);
+
+/* ---------------- L mode (jointfind.cpp, surface_call_functions.cpp) ---- */
+void ora_eexp(double x, double *m, int *z);
+/* jointp for a 2-population table (or popsize/migration-only models):
+ * table is [G][K] float in the savegsampinf layout (ginfo.cpp:288-316);
+ * x has npsize + nmig entries; params [prange0, prange1) are used.
+ * modeltype: 0 = all, 1 = popsize-only (npops>2), 2 = migration-only. */
+double ora_jointp(const float *table, int G, int K, int npsize, int nmig,
+                  int npops, int modeltype, int prange0, int prange1,
+                  const double *x, int calc_ess, double *effective_n);
+/* Exact log-sum-exp variant of the same quantity (no truncation). */
+double ora_jointp_exact(const float *table, int G, int K, int npsize, int nmig,
+                        int npops, int modeltype, int prange0, int prange1,
+                        const double *x);
+double ora_margincalc(const float *table, int G, int K, int npsize, int nmig,
+                      double x, double yadjust, int pi, int logi,
+                      int expo_prior, double prior_mean);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
