@@ -48,6 +48,8 @@ PROTOS = {
     "ima_slots_alloc": (C.c_int, [vp, C.c_int, i32p, C.POINTER(C.c_int)]),
     "ima_slots_write": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p, i32p]),
     "ima_slots_read": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p, i32p]),
+    "ima_slots_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ima_slot_info": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ima_score": (C.c_int, [vp, C.POINTER(Item), C.c_int, C.c_int, dp, i32p]),
     "ima_is_site_nodes": (C.c_int, [vp, C.c_int, i32p]),
     "ima_sw_dlike": (C.c_int, [vp, C.POINTER(Item), dp]),

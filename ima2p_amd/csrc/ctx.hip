@@ -509,6 +509,19 @@ int ima_slots_read(ima_ctx *c, int first, int count, int32_t *up0, int32_t *up1,
   return IMA_OK;
 }
 
+int ima_slots_count(ima_ctx *c, int *count) {
+  if (!c || !count) return IMA_ERR_ARG;
+  *count = (int)c->slots.size();
+  return IMA_OK;
+}
+
+int ima_slot_info(ima_ctx *c, int slot, int *locus, int *n) {
+  if (!c || slot < 0 || slot >= (int)c->slots.size()) return IMA_ERR_ARG;
+  if (locus) *locus = c->slots[slot].locus;
+  if (n) *n = c->slots[slot].n;
+  return IMA_OK;
+}
+
 // --------------------------------------------------------------- scoring --
 
 static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precision,

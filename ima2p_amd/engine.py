@@ -38,7 +38,6 @@ class Engine:
             raise EngineError(f"ima_ctx_create failed with code {rc}")
         self.h = h
         self.loci = []          # (model, n, S)
-        self.slot_locus = []
 
     def close(self):
         if self.h:
@@ -97,8 +96,12 @@ class Engine:
         la, lp = _i32(loci)
         first = C.c_int()
         self._chk(self.lib.ima_slots_alloc(self.h, len(la), lp, C.byref(first)), "alloc_slots")
-        self.slot_locus += [int(x) for x in la]
         return first.value
+
+    def slot_info(self, slot):
+        locus, n = C.c_int(), C.c_int()
+        self._chk(self.lib.ima_slot_info(self.h, slot, C.byref(locus), C.byref(n)), "slot_info")
+        return locus.value, n.value
 
     def write_slots(self, first, gens, A=None):
         up0, p0 = _i32(np.concatenate([g.up0 for g in gens]))
@@ -114,7 +117,7 @@ class Engine:
                   "write_slots")
 
     def read_slots(self, first, count):
-        n = [self.loci[self.slot_locus[first + i]][1] for i in range(count)]
+        n = [self.slot_info(first + i)[1] for i in range(count)]
         tot = sum(2 * k - 1 for k in n)
         up0 = np.zeros(tot, np.int32)
         up1 = np.zeros(tot, np.int32)
@@ -160,7 +163,7 @@ class Engine:
 
     def sw_dlike(self, slot, u, tr=1.0):
         it = self.items([slot], u, tr=tr)
-        n = self.loci[self.slot_locus[slot]][1]
+        n = self.slot_info(slot)[1]
         out = np.zeros(2 * n - 1)
         self._chk(self.lib.ima_sw_dlike(self.h, it, out.ctypes.data_as(L.dp)), "sw_dlike")
         return out

@@ -109,6 +109,10 @@ int ima_slots_write(ima_ctx *ctx, int first, int count, const int32_t *up0,
 int ima_slots_read(ima_ctx *ctx, int first, int count, int32_t *up0,
                    int32_t *up1, int32_t *down, double *time, int32_t *root,
                    int32_t *A);
+/* Number of slots in the context, and the locus / gene-copy count of one
+ * slot (slots may also be created by ima_likelihood_hky). */
+int ima_slots_count(ima_ctx *ctx, int *count);
+int ima_slot_info(ima_ctx *ctx, int slot, int *locus, int *n);
 
 /* ---- batched data likelihood ------------------------------------------ */
 /* Scores every item in one launch per mutation model.  Replaces the

@@ -154,12 +154,12 @@ def test_sw_matches_oracle(engine, seed, u):
 def test_sw_zero_bessel(engine):
     g, A = D.sw_case(4, (5, 5), u=1.0)
     A = A.copy()
-    A[0] += 50  # a large jump on a short branch underflows I_d
+    A[0] += 120  # I_120(x) underflows to 0 for tiny x (calc_prob_data.cpp:870-880)
     li = engine.add_locus_sw(g.n)
     s = engine.alloc_slots([li])
     engine.write_slots(s, [g], A=[A])
-    out, st = engine.score(engine.items([s], 1e-3))
-    ref, _ = oracle.likelihood_sw(g, A, 1e-3, 1.0)
+    out, st = engine.score(engine.items([s], 1e-6))
+    ref, _ = oracle.likelihood_sw(g, A, 1e-6, 1.0)
     assert ref == -np.finfo(float).max
     assert st[0] == L.STATUS_SW_ZERO and out[0] == ref
 
