@@ -19,6 +19,7 @@ LMODE_REF_COMPAT, LMODE_EXACT = 0, 1
 REJECT_IS = -1000000000.0
 MAX_GENES = 256
 K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3 = range(7)
+MC3_GRAPH, MC3_SCALE, MC3_TIMED = 1, 2, 4
 
 
 class Item(C.Structure):
@@ -64,6 +65,10 @@ PROTOS = {
                                    C.c_int, dp, dp]),
     "ima_lmode_margincalc": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int,
                                        C.c_double, C.c_int, C.c_double, dp]),
+    "ima_mc3_setup": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_uint64, C.c_int,
+                                C.c_int, C.c_double]),
+    "ima_mc3_run": (C.c_int, [vp, C.c_int, C.c_int]),
+    "ima_mc3_read": (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ima_timing_enable": (C.c_int, [vp, C.c_int]),
     "ima_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), dp]),
     "ima_timing_reset": (C.c_int, [vp]),

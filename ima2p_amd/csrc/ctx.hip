@@ -70,7 +70,25 @@ struct ima_ctx {
 
   Timing timing;
   int max_lds = 65536;
+
+  // device-resident MC3 sweep
+  struct {
+    bool live = false;
+    int C = 0, L = 0, first = 0, precision = IMA_PREC_FP32, swaptries = 1;
+    int nh = 0, ni = 0, ns = 0, hmax_n = 2, hmax_slots = 1, imax_n = 2, imax_w = 1;
+    MC3Dev dev{};
+    DevBuf<double> pdg, newpdg, beta, S, rec_t, tbackup, logs;
+    DevBuf<int32_t> status, rec_v, decision, idx;
+    DevBuf<uint64_t> counter;
+    DevBuf<unsigned long long> stats;
+    DevBuf<ima_item> items;
+    hipGraphExec_t exec[2]{nullptr, nullptr};
+    hipGraph_t graph[2]{nullptr, nullptr};
+    hipEvent_t ev[4]{};
+  } mc3;
 };
+
+static void mc3_release(ima_ctx *c);
 
 namespace {
 
@@ -294,6 +312,7 @@ void ima_ctx_destroy(ima_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  mc3_release(c);
   release(c->d_loci);
   release(c->d_seq);
   release(c->d_mult);
@@ -871,6 +890,229 @@ int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx,
                               c->d_lwork.p, c->d_lout.p, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, c->d_lout.p, (size_t)nx * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+// ------------------------------------------------------------------- MC3 --
+
+static void mc3_release(ima_ctx *c) {
+  auto &m = c->mc3;
+  for (int k = 0; k < 2; ++k) {
+    if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
+    if (m.graph[k]) hipGraphDestroy(m.graph[k]);
+    m.exec[k] = nullptr;
+    m.graph[k] = nullptr;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (m.ev[k]) {
+      hipEventDestroy(m.ev[k]);
+      m.ev[k] = nullptr;
+    }
+  release(m.pdg);
+  release(m.newpdg);
+  release(m.beta);
+  release(m.S);
+  release(m.rec_t);
+  release(m.tbackup);
+  release(m.logs);
+  release(m.status);
+  release(m.rec_v);
+  release(m.decision);
+  release(m.idx);
+  release(m.counter);
+  release(m.stats);
+  release(m.items);
+  m.live = false;
+}
+
+int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const double *u,
+                  const double *kappa, const double *beta, uint64_t seed, int precision,
+                  int swaptries, double scale_width) {
+  if (!c || !u || !beta || nchains < 1 || nloci < 1 || swaptries < 0) return IMA_ERR_ARG;
+  if (precision != IMA_PREC_FP64 && precision != IMA_PREC_FP32) return IMA_ERR_ARG;
+  const int N = nchains * nloci;
+  if (first_slot < 0 || first_slot + N > (int)c->slots.size())
+    return fail(c, IMA_ERR_ARG, "MC3 slots [%d,%d) out of range", first_slot, first_slot + N);
+  hipSetDevice(c->device);
+  mc3_release(c);
+  auto &m = c->mc3;
+  m.C = nchains;
+  m.L = nloci;
+  m.first = first_slot;
+  m.precision = precision;
+  m.swaptries = swaptries;
+  std::vector<ima_item> items(N);
+  std::vector<int32_t> hky, is, sw;
+  m.hmax_n = 2;
+  m.hmax_slots = 1;
+  m.imax_n = 2;
+  m.imax_w = 1;
+  for (int i = 0; i < N; ++i) {
+    const SlotMeta &s = c->slots[first_slot + i];
+    if (s.root < 0) return fail(c, IMA_ERR_STATE, "MC3 slot %d holds no genealogy", first_slot + i);
+    items[i].slot = first_slot + i;
+    items[i].flags = 0;
+    items[i].u = u[i];
+    items[i].kappa = kappa ? kappa[i] : 1.0;
+    items[i].tr = 1.0;
+    const LocusMeta &lm = c->loci[s.locus];
+    if (lm.model == IMA_MODEL_HKY) {
+      hky.push_back(i);
+      m.hmax_n = std::max(m.hmax_n, s.n);
+      m.hmax_slots = std::max(m.hmax_slots, s.nslots);
+    } else if (lm.model == IMA_MODEL_IS) {
+      is.push_back(i);
+      m.imax_n = std::max(m.imax_n, s.n);
+      m.imax_w = std::max(m.imax_w, lm.W);
+    } else {
+      sw.push_back(i);
+    }
+  }
+  // topology never changes inside the sweep, so the slot need is fixed; leave
+  // room for a re-uploaded genealogy with a deeper program.
+  m.hmax_slots = std::max(m.hmax_slots, 1 + (int)floor(log2((double)m.hmax_n)));
+  m.nh = (int)hky.size();
+  m.ni = (int)is.size();
+  m.ns = (int)sw.size();
+  std::vector<int32_t> idx;
+  idx.insert(idx.end(), hky.begin(), hky.end());
+  idx.insert(idx.end(), is.begin(), is.end());
+  idx.insert(idx.end(), sw.begin(), sw.end());
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  HIPCHK(c, reserve(m.pdg, N));
+  HIPCHK(c, reserve(m.newpdg, N));
+  HIPCHK(c, reserve(m.status, N));
+  HIPCHK(c, reserve(m.beta, nchains));
+  HIPCHK(c, reserve(m.S, nchains));
+  HIPCHK(c, reserve(m.rec_v, N));
+  HIPCHK(c, reserve(m.rec_t, N));
+  HIPCHK(c, reserve(m.tbackup, c->nodes + 1));
+  HIPCHK(c, reserve(m.decision, nchains));
+  HIPCHK(c, reserve(m.logs, nchains));
+  HIPCHK(c, reserve(m.counter, 1));
+  HIPCHK(c, reserve(m.stats, 8));
+  HIPCHK(c, reserve(m.items, N));
+  HIPCHK(c, reserve(m.idx, N));
+  HIPCHK(c, hipMemcpyAsync(m.items.p, items.data(), N * sizeof(ima_item), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(m.idx.p, idx.data(), N * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(m.beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
+  const uint64_t zero = 0;
+  const unsigned long long zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(m.counter.p, &zero, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(m.stats.p, zeros, sizeof zeros, hipMemcpyHostToDevice, c->stream));
+  MC3Dev &d = m.dev;
+  d.C = nchains;
+  d.L = nloci;
+  d.first_slot = first_slot;
+  d.pdg = m.pdg.p;
+  d.newpdg = m.newpdg.p;
+  d.status = m.status.p;
+  d.beta = m.beta.p;
+  d.S = m.S.p;
+  d.rec_v = m.rec_v.p;
+  d.rec_t = m.rec_t.p;
+  d.tbackup = m.tbackup.p;
+  d.decision = m.decision.p;
+  d.logs = m.logs.p;
+  d.counter = m.counter.p;
+  d.stats = m.stats.p;
+  d.seed = seed;
+  d.scale_width = scale_width;
+  for (int k = 0; k < 4; ++k) HIPCHK(c, hipEventCreate(&m.ev[k]));
+  // initial likelihood of every genealogy
+  const DeviceView v = view(c);
+  if (m.nh)
+    HIPCHK(c, launch_hky(v, m.items.p, m.idx.p, m.nh, precision, m.hmax_n, m.hmax_slots,
+                         m.pdg.p, m.status.p, c->stream));
+  if (m.ni)
+    HIPCHK(c, launch_is(v, m.items.p, m.idx.p + m.nh, m.ni, m.imax_n, m.imax_w, m.pdg.p,
+                        m.status.p, nullptr, nullptr, c->stream));
+  if (m.ns)
+    HIPCHK(c, launch_sw(v, m.items.p, m.idx.p + m.nh + m.ni, m.ns, m.pdg.p, m.status.p,
+                        nullptr, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  m.live = true;
+  return IMA_OK;
+}
+
+static int mc3_enqueue(ima_ctx *c, bool with_scale, hipEvent_t *score_ev) {
+  auto &m = c->mc3;
+  HIPCHK(c, enqueue_mc3_sweep(view(c), m.dev, with_scale, m.items.p, m.idx.p, m.nh, m.ni, m.ns,
+                              m.precision, m.hmax_n, m.hmax_slots, m.imax_n, m.imax_w,
+                              m.swaptries, score_ev, c->stream));
+  return IMA_OK;
+}
+
+int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
+  if (!c || nsweeps < 0) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  hipSetDevice(c->device);
+  int rc;
+  const bool use_graph = (flags & IMA_MC3_GRAPH) != 0;
+  const bool scale = (flags & IMA_MC3_SCALE) != 0;
+  const bool timed = (flags & IMA_MC3_TIMED) != 0;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if (use_graph && !timed) {
+    for (int k = 0; k < 2; ++k) {
+      if (m.exec[k]) continue;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = mc3_enqueue(c, k == 1, nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc != IMA_OK) return rc;
+      HIPCHK(c, e);
+      m.graph[k] = g;
+      HIPCHK(c, hipGraphInstantiate(&m.exec[k], g, nullptr, nullptr, 0));
+    }
+  }
+  uint64_t done = 0;
+  HIPCHK(c, hipMemcpy(&done, m.counter.p, 8, hipMemcpyDeviceToHost));
+  for (int s = 0; s < nsweeps; ++s) {
+    const bool with_scale = scale && ((done + s) & 1);
+    if (timed) {
+      // events bracket the whole sweep and each batched scoring launch
+      Timing &T = c->timing;
+      HIPCHK(c, hipEventRecord(T.ev0[K_MC3], c->stream));
+      if ((rc = mc3_enqueue(c, with_scale, m.ev)) != IMA_OK) return rc;
+      HIPCHK(c, hipEventRecord(T.ev1[K_MC3], c->stream));
+      HIPCHK(c, hipEventSynchronize(T.ev1[K_MC3]));
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, T.ev0[K_MC3], T.ev1[K_MC3]));
+      T.ms[K_MC3] += ms;
+      T.launches[K_MC3] += 1;
+      const int ks = m.precision == IMA_PREC_FP32 ? K_HKY32 : K_HKY64;
+      HIPCHK(c, hipEventElapsedTime(&ms, m.ev[0], m.ev[1]));
+      T.ms[ks] += ms;
+      T.launches[ks] += 1;
+      if (with_scale) {
+        HIPCHK(c, hipEventElapsedTime(&ms, m.ev[2], m.ev[3]));
+        T.ms[ks] += ms;
+        T.launches[ks] += 1;
+      }
+    } else if (use_graph) {
+      HIPCHK(c, hipGraphLaunch(m.exec[with_scale ? 1 : 0], c->stream));
+    } else {
+      if ((rc = mc3_enqueue(c, with_scale, nullptr)) != IMA_OK) return rc;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_t *sweeps) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  hipSetDevice(c->device);
+  if (pdg)
+    HIPCHK(c, hipMemcpy(pdg, m.pdg.p, (size_t)m.C * m.L * 8, hipMemcpyDeviceToHost));
+  if (beta) HIPCHK(c, hipMemcpy(beta, m.beta.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
+  if (stats) HIPCHK(c, hipMemcpy(stats, m.stats.p, 8 * 8, hipMemcpyDeviceToHost));
+  if (sweeps) HIPCHK(c, hipMemcpy(sweeps, m.counter.p, 8, hipMemcpyDeviceToHost));
   return IMA_OK;
 }
 

@@ -111,6 +111,30 @@ hipError_t launch_margincalc(const LTable &t, int param, const double *x,
 size_t margincalc_work_bytes(const LTable &t, int nx);
 hipError_t upload_pow10_table(hipStream_t st);
 
+// Device-resident MC3 state (kern_mc3.hip).
+struct MC3Dev {
+  int C, L;
+  int first_slot;
+  double *pdg, *newpdg;  // [C*L]
+  int32_t *status;       // [C*L]
+  double *beta, *S;      // [C]
+  int32_t *rec_v;        // [C*L]
+  double *rec_t;         // [C*L]
+  double *tbackup;       // node times backup (indexed like time[])
+  int32_t *decision;     // [C]
+  double *logs;          // [C] log s of the rescaling
+  uint64_t *counter;     // sweep counter
+  unsigned long long *stats;  // [8]
+  uint64_t seed;
+  double scale_width;
+};
+
+hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_scale,
+                             const ima_item *items, const int32_t *idx, int nh, int ni,
+                             int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
+                             int imax_w, int swaptries, hipEvent_t *score_ev,
+                             hipStream_t st);
+
 // Swap (kern_swap.hip).
 hipError_t launch_swap(int nchains, const double *S, double *beta,
                        int swaptries, uint64_t seed, uint64_t counter,

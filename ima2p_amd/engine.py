@@ -223,6 +223,34 @@ class Engine:
                   "margincalc")
         return out
 
+    # --------------------------------------------------------------- MC3 --
+    def mc3_setup(self, nchains, nloci, first_slot, u, kappa, beta, seed=1,
+                  precision=L.PREC_FP32, swaptries=None, scale_width=0.2):
+        u, pu = _f64(np.broadcast_to(np.asarray(u, float), (nchains * nloci,)))
+        k, pk = _f64(np.broadcast_to(np.asarray(kappa, float), (nchains * nloci,)))
+        b, pb = _f64(beta)
+        if swaptries is None:
+            swaptries = max(1, nchains // 10)  # ima_main_mpi.cpp:1376-1378
+        self._chk(self.lib.ima_mc3_setup(self.h, nchains, nloci, first_slot, pu, pk, pb,
+                                         int(seed), precision, int(swaptries),
+                                         float(scale_width)), "mc3_setup")
+        self.mc3_shape = (nchains, nloci)
+
+    def mc3_run(self, nsweeps, flags=L.MC3_GRAPH | L.MC3_SCALE):
+        self._chk(self.lib.ima_mc3_run(self.h, int(nsweeps), int(flags)), "mc3_run")
+
+    def mc3_read(self):
+        C_, L_ = self.mc3_shape
+        pdg = np.zeros(C_ * L_)
+        beta = np.zeros(C_)
+        stats = np.zeros(8, np.uint64)
+        sweeps = C.c_uint64()
+        self._chk(self.lib.ima_mc3_read(self.h, pdg.ctypes.data_as(L.dp),
+                                        beta.ctypes.data_as(L.dp),
+                                        stats.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                        C.byref(sweeps)), "mc3_read")
+        return pdg.reshape(C_, L_), beta, stats, sweeps.value
+
     # ------------------------------------------------------------ timing --
     def timing(self, on=True):
         self._chk(self.lib.ima_timing_enable(self.h, int(on)), "timing")

@@ -1,0 +1,97 @@
+"""Synthetic workloads at the benchmark configurations.
+
+``three_pop_hky`` is the headline configuration: three-population IM model
+with tree ((0,1),2), theta ~ U(1,10) for the five populations, split times
+U(0.5,3), migration U(0,1) between coexisting populations (seed 2); fifty
+HKY loci of 1000 bp with kappa = 4, base frequencies ~ Dirichlet(10,10,10,10),
+15 gene copies per population, per-locus mutation scalars ~ LogNormal(0,0.5);
+64 heated chains.
+
+``two_pop_is`` is the reference's CPU-runnable case: two populations,
+theta 5/5/5, t = 1, m = 0.5 both ways, five IS loci with 10+10 gene copies
+(seed 1), 4 chains with geometric heating (-ha 0.96 -hb 0.9).
+
+Chains start from genealogies simulated under the true model (a pool per
+locus, relabelled per chain), since the reference's starting-tree builder
+is not part of this engine.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import sim
+
+
+@dataclass
+class Workload:
+    name: str
+    nchains: int
+    model: str                     # "hky" or "is"
+    loci: list                     # HKYLocus / ISLocus
+    u: np.ndarray                  # per-locus mutation scalar
+    kappa: float
+    chains: list = field(default_factory=list)  # chains[c][l] -> Genealogy
+    beta: np.ndarray = None
+    seed: int = 0
+
+    @property
+    def nloci(self):
+        return len(self.loci)
+
+    def patterns(self):
+        return np.array([loc.seq.shape[1] for loc in self.loci])
+
+
+def geometric_betas(nchains, ha=0.96, hb=0.9):
+    """setheat HGEOMETRIC (swapchains.cpp:152-153)."""
+    if nchains == 1:
+        return np.ones(1)
+    i = np.arange(nchains, dtype=float)
+    return 1 - (1 - hb) * i * np.power(ha, nchains - 1 - i) / (nchains - 1)
+
+
+def _chain_pool(rng, simulate, nloci, nchains, pool):
+    trees = []
+    for _ in range(nloci):
+        base = [simulate() for _ in range(min(pool, nchains))]
+        trees.append([sim.relabel_internal(base[c % len(base)], rng) for c in range(nchains)])
+    return [[trees[l][c] for l in range(nloci)] for c in range(nchains)]
+
+
+def three_pop_hky(nchains=64, nloci=50, nsites=1000, per_pop=15, seed=2, pool=16):
+    rng = np.random.default_rng(seed)
+    pt, theta, ts, mig = sim.im_model_three_pop(rng)
+    u = rng.lognormal(0.0, 0.5, nloci)
+    loci = []
+    for l in range(nloci):
+        g = sim.simulate_genealogy(rng, [per_pop] * 3, theta, ts, mig, pt)
+        pi = rng.dirichlet([10, 10, 10, 10])
+        raw = sim.simulate_hky_sequences(rng, g, nsites, pi, 4.0, u[l])
+        loci.append(sim.encode_hky(raw))
+    chains = _chain_pool(rng, lambda: sim.simulate_genealogy(rng, [per_pop] * 3, theta, ts,
+                                                             mig, pt), nloci, nchains, pool)
+    return Workload("3pop-50xHKY1000bp-n45-64chains", nchains, "hky", loci, u, 4.0, chains,
+                    geometric_betas(nchains), seed)
+
+
+def two_pop_is(nchains=4, nloci=5, per_pop=10, seed=1, pool=4):
+    rng = np.random.default_rng(seed)
+    pt, theta, ts, mig = sim.im_model_two_pop()
+    loci, truth = [], []
+    for _ in range(nloci):
+        g = sim.simulate_genealogy(rng, [per_pop] * 2, theta, ts, mig, pt)
+        loci.append(sim.simulate_is(rng, g, 1.0))
+        truth.append(g)
+    # IS chains must start on genealogies compatible with the data: use the
+    # generating genealogy (times perturbed per chain keep the topology)
+    chains = []
+    for c in range(nchains):
+        row = []
+        for l in range(nloci):
+            g = sim.relabel_internal(truth[l], rng)
+            row.append(g)
+        chains.append(row)
+    return Workload("2pop-5xIS-n20-4chains", nchains, "is", loci, np.ones(nloci), 1.0, chains,
+                    geometric_betas(nchains), seed)

@@ -173,6 +173,30 @@ int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
                          const double *x, int nx, int logi, double yadjust,
                          int expo, double prior_mean, double *out);
 
+/* ---- device-resident MC3 sweep ------------------------------------------ */
+/* All heated chains on this device: slots first_slot + c*nloci + l hold the
+ * genealogy of chain c at locus l.  A sweep proposes one genealogy change
+ * per (chain, locus), scores all of them in one batched launch per model
+ * and accepts each with the reference's Metropolis rule
+ * (update_gtree.cpp:919-927); with IMA_MC3_SCALE, alternate sweeps also
+ * rescale each chain's genealogies and re-score every locus before one
+ * accept decision per chain (the full re-scoring of changet_RY1,
+ * update_t_RY.cpp:343-388); every sweep ends with `swaptries` swap
+ * attempts (swapchains.cpp:527-662).  u/kappa are per (chain, locus);
+ * scale_width is the width of the log rescaling step. */
+#define IMA_MC3_GRAPH 1 /* replay each sweep as a captured hipGraph */
+#define IMA_MC3_SCALE 2 /* whole-chain rescaling on alternate sweeps */
+#define IMA_MC3_TIMED 4 /* eager launches with HIP events per scoring pass */
+int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
+                  const double *u, const double *kappa, const double *beta,
+                  uint64_t seed, int precision, int swaptries,
+                  double scale_width);
+int ima_mc3_run(ima_ctx *ctx, int nsweeps, int flags);
+/* pdg[nchains*nloci], beta[nchains]; stats[8] = genealogy proposals,
+ * accepts, rescale proposals, accepts, swap tries, swaps; sweeps done. */
+int ima_mc3_read(ima_ctx *ctx, double *pdg, double *beta, uint64_t *stats,
+                 uint64_t *sweeps);
+
 /* ---- timing ------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the context's stream (off by default).
  * ima_timing_get returns, for kernel class k (0 HKY FP64, 1 HKY FP32,
