@@ -78,7 +78,9 @@ struct ima_ctx {
     int nh = 0, ni = 0, ns = 0, hmax_n = 2, hmax_slots = 1, imax_n = 2, imax_w = 1;
     MC3Dev dev{};
     DevBuf<double> pdg, newpdg, beta, S, rec_t, tbackup, logs;
-    DevBuf<int32_t> status, rec_v, decision, idx;
+    DevBuf<int32_t> status, rec_v, decision, idx, node_item;
+    DevBuf<uint32_t> acc, cacc;
+    int64_t scale_sweeps = 0;
     DevBuf<uint64_t> counter;
     DevBuf<unsigned long long> stats;
     DevBuf<ima_item> items;
@@ -192,24 +194,19 @@ int need_of(const int32_t *u0, const int32_t *u1, int e, std::vector<int> &need)
   return need[e] = (a == b) ? a + 1 : std::max(a, b);
 }
 
-void emit(const int32_t *u0, const int32_t *u1, int e, int base, const std::vector<int> &need,
+// Postorder stack program: the daughter with the larger need goes first, so
+// its result sits below the other's (or is the only one on the stack).
+void emit(const int32_t *u0, const int32_t *u1, int e, const std::vector<int> &need,
           std::vector<ProgOp> &out) {
   int a = u0[e], b = u1[e];
   if (need[a] < need[b]) std::swap(a, b);
-  int a_slot = kTipSlot, b_slot = kTipSlot, next = base;
-  if (u0[a] >= 0) {
-    emit(u0, u1, a, base, need, out);
-    a_slot = base;
-    next = base + 1;
-  }
-  if (u0[b] >= 0) {
-    emit(u0, u1, b, next, need, out);
-    b_slot = next;
-  }
+  const bool ai = u0[a] >= 0, bi = u0[b] >= 0;
+  if (ai) emit(u0, u1, a, need, out);
+  if (bi) emit(u0, u1, b, need, out);
   ProgOp op;
-  op.a = a | (a_slot << 16);
-  op.b = b | (b_slot << 16);
-  op.out = base;
+  op.a = a;
+  op.b = b;
+  op.type = (ai && bi) ? OP_SS : (ai ? OP_ST : OP_TT);
   op.node = e;
   out.push_back(op);
 }
@@ -483,7 +480,7 @@ int ima_slots_write(ima_ctx *c, int first, int count, const int32_t *up0, const 
     need.assign(ne, 0);
     const int r = need_of(up0 + src, up1 + src, root[i], need);
     ops.clear();
-    emit(up0 + src, up1 + src, root[i], 0, need, ops);
+    emit(up0 + src, up1 + src, root[i], need, ops);
     std::copy(ops.begin(), ops.end(), c->prog.begin() + s.prog_off);
     s.nslots = std::max(1, r);
     src += ne;
@@ -577,9 +574,9 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
       idx_s.push_back(i);
     }
   }
-  const size_t elt = precision == IMA_PREC_FP32 ? 4 : 8;
-  const size_t hlds = (2 * (size_t)hmax_n - 1) * 8 * elt + (size_t)hmax_slots * kWave * 4 * elt +
-                      (size_t)hmax_n * kWave;
+  const size_t hlds = hky_lds_bytes(precision, hmax_n);
+  if (!hky.empty() && hmax_slots > 8)
+    return fail(c, IMA_ERR_SIZE, "a genealogy needs %d partial slots (limit 8)", hmax_slots);
   if (!hky.empty() && hlds > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "HKY batch needs %zu B of LDS (limit %d)", hlds, c->max_lds);
   HIPCHK(c, reserve(c->d_items, nitems));
@@ -919,6 +916,9 @@ static void mc3_release(ima_ctx *c) {
   release(m.rec_v);
   release(m.decision);
   release(m.idx);
+  release(m.node_item);
+  release(m.acc);
+  release(m.cacc);
   release(m.counter);
   release(m.stats);
   release(m.items);
@@ -995,6 +995,23 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   HIPCHK(c, reserve(m.stats, 8));
   HIPCHK(c, reserve(m.items, N));
   HIPCHK(c, reserve(m.idx, N));
+  // node -> item map over the contiguous node range of the MC3 slots
+  const SlotMeta &s0 = c->slots[first_slot], &s1 = c->slots[first_slot + N - 1];
+  const int64_t node_first = s0.node_off;
+  const int64_t nnodes = s1.node_off + 2 * (int64_t)s1.n - 1 - node_first;
+  std::vector<int32_t> node_item((size_t)nnodes);
+  for (int i = 0; i < N; ++i) {
+    const SlotMeta &s = c->slots[first_slot + i];
+    for (int k = 0; k < 2 * s.n - 1; ++k) node_item[(size_t)(s.node_off - node_first + k)] = i;
+  }
+  HIPCHK(c, reserve(m.node_item, (size_t)nnodes));
+  HIPCHK(c, reserve(m.acc, N));
+  HIPCHK(c, reserve(m.cacc, nchains));
+  HIPCHK(c, hipMemcpyAsync(m.node_item.p, node_item.data(), (size_t)nnodes * 4,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(m.acc.p, 0, N * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(m.cacc.p, 0, nchains * 4, c->stream));
+  m.scale_sweeps = 0;
   HIPCHK(c, hipMemcpyAsync(m.items.p, items.data(), N * sizeof(ima_item), hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(m.idx.p, idx.data(), N * 4, hipMemcpyHostToDevice, c->stream));
@@ -1019,6 +1036,13 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.logs = m.logs.p;
   d.counter = m.counter.p;
   d.stats = m.stats.p;
+  d.acc = m.acc.p;
+  d.cacc = m.cacc.p;
+  d.node_first = node_first;
+  d.nnodes = nnodes;
+  d.node_item = m.node_item.p;
+  HIPCHK(c, reserve(m.tbackup, (size_t)nnodes + 1));
+  d.tbackup = m.tbackup.p;
   d.seed = seed;
   d.scale_width = scale_width;
   for (int k = 0; k < 4; ++k) HIPCHK(c, hipEventCreate(&m.ev[k]));
@@ -1073,6 +1097,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
   HIPCHK(c, hipMemcpy(&done, m.counter.p, 8, hipMemcpyDeviceToHost));
   for (int s = 0; s < nsweeps; ++s) {
     const bool with_scale = scale && ((done + s) & 1);
+    m.scale_sweeps += with_scale ? 1 : 0;
     if (timed) {
       // events bracket the whole sweep and each batched scoring launch
       Timing &T = c->timing;
@@ -1111,7 +1136,18 @@ int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_
   if (pdg)
     HIPCHK(c, hipMemcpy(pdg, m.pdg.p, (size_t)m.C * m.L * 8, hipMemcpyDeviceToHost));
   if (beta) HIPCHK(c, hipMemcpy(beta, m.beta.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
-  if (stats) HIPCHK(c, hipMemcpy(stats, m.stats.p, 8 * 8, hipMemcpyDeviceToHost));
+  if (stats) {
+    HIPCHK(c, hipMemcpy(stats, m.stats.p, 8 * 8, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> acc((size_t)m.C * m.L), cacc(m.C);
+    HIPCHK(c, hipMemcpy(acc.data(), m.acc.p, acc.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cacc.data(), m.cacc.p, cacc.size() * 4, hipMemcpyDeviceToHost));
+    uint64_t a = 0, ca = 0;
+    for (uint32_t x : acc) a += x;
+    for (uint32_t x : cacc) ca += x;
+    stats[1] = a;
+    stats[2] = (uint64_t)m.scale_sweeps * m.C;
+    stats[3] = ca;
+  }
   if (sweeps) HIPCHK(c, hipMemcpy(sweeps, m.counter.p, 8, hipMemcpyDeviceToHost));
   return IMA_OK;
 }

@@ -43,13 +43,16 @@ struct SlotMeta {
 };
 
 // One postorder operation of the HKY pruning program: compute the partial
-// at the top of edge `node` from daughters a and b.  Operands that are
-// internal come from LDS slot a_slot/b_slot; tips from the site data.
+// at the top of edge `node` from daughters a and b.  The program is a stack
+// machine: OP_TT combines two tips and pushes, OP_ST combines the stack top
+// (daughter a) with tip b in place, OP_SS pops a (second) and b (top) and
+// pushes their combination.  Depth never exceeds the Strahler number.
+enum { OP_TT = 0, OP_ST = 1, OP_SS = 2 };
 struct ProgOp {
-  int32_t a;       // low 16: edge, high 16: slot (kTipSlot for a tip)
-  int32_t b;
-  int32_t out;     // LDS slot receiving the result
-  int32_t node;
+  int32_t a;       // edge of daughter a
+  int32_t b;       // edge of daughter b
+  int32_t type;    // OP_TT / OP_ST / OP_SS
+  int32_t node;    // edge whose top the result belongs to
 };
 
 template <typename T>
@@ -81,7 +84,8 @@ struct DeviceView {
   const uint64_t *mask;
 };
 
-// Kernel launchers (kern_lik.hip).
+// Kernel launchers (kern_hky.hip, kern_lik.hip).
+size_t hky_lds_bytes(int precision, int max_n);
 hipError_t launch_hky(const DeviceView &v, const ima_item *items,
                       const int32_t *item_idx, int nitems, int precision,
                       int max_n, int max_slots, double *out, int32_t *status,
@@ -124,7 +128,12 @@ struct MC3Dev {
   int32_t *decision;     // [C]
   double *logs;          // [C] log s of the rescaling
   uint64_t *counter;     // sweep counter
-  unsigned long long *stats;  // [8]
+  unsigned long long *stats;  // [8]: [0] proposals, [4] swap tries, [5] swaps
+  uint32_t *acc;         // [C*L] accepted genealogy proposals per item
+  uint32_t *cacc;        // [C] accepted rescalings per chain
+  int64_t node_first;    // first node of the MC3 slots in the node arrays
+  int64_t nnodes;        // nodes covered by the MC3 slots
+  int32_t *node_item;    // [nnodes] item owning each node
   uint64_t seed;
   double scale_width;
 };

@@ -1,0 +1,267 @@
+// kern_hky.hip -- batched HKY Felsenstein pruning (calc_prob_data.cpp:118-607).
+//
+// One 64-lane wavefront per scoring item (a genealogy of one chain at one
+// locus), one lane per site pattern, the site patterns of a locus walked in
+// blocks of 64.  Items of all chains and loci, with uneven gene counts and
+// pattern counts, share one launch.
+//
+// The host turns each genealogy into a postorder stack program (ProgOp):
+// the daughter with the larger need is evaluated first, so the stack never
+// holds more than floor(log2 n) partials (Strahler number), at most 8 for
+// n <= 256.  The stack is a register array of D quads addressed only at
+// compile-time positions (push shifts down, pop shifts up), selected per
+// operation by a wave-uniform op type, so a node's partial never leaves the
+// register file and no private (scratch) memory is used.  Nothing per node goes to
+// HBM: every proposal is re-scored over the whole genealogy, ~(36n-58)S
+// flops on chip, instead of reading and writing ~36 B per node-pattern of
+// stored partials.
+//
+// Branch transition (HKY85 closed form, calc_prob_data.cpp:197-208):
+//   P_jk(t) = pi_k (1-e) + [cls(j)=cls(k)] pi_k (e - eA_c)/PI_c + [j=k] eA_c
+// with e = exp(-mu t), eA_c = exp(-mu t A_c), A_c = 1 + PI_c (kappa-1), c the
+// purine {A,G} or pyrimidine {C,T} class of the destination base.  For a
+// daughter with partial L:  (P L)_j = a W + b_c W_c + eA_c L_j with a = 1-e,
+// b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.  The
+// five per-branch coefficients are computed once per item in FP64 and kept
+// in LDS.  Partials are rescaled at every node by an exact power of two; one
+// integer exponent per lane carries the sum of the node scale factors
+// (the reference keeps log(max) per node, :211-217, 465).
+#include "ima_internal.h"
+
+namespace ima {
+namespace {
+
+template <typename real>
+struct __align__(16) HkyCoef {
+  real a, bR, bY, eAR, eAY, pad0, pad1, pad2;
+};
+
+template <typename real>
+struct Q4 {
+  real x, y, z, w;
+};
+
+// Column s of P for a tip in state s.
+template <typename real>
+__device__ __forceinline__ Q4<real> tip_vec(const HkyCoef<real> &c, int s, const real pi[4]) {
+  const bool R = (s & 1) == 0;  // A=0, G=2 purines; C=1, T=3 pyrimidines
+  const real ps = s == 0 ? pi[0] : (s == 1 ? pi[1] : (s == 2 ? pi[2] : pi[3]));
+  const real bs = R ? c.bR : c.bY;
+  const real es = R ? c.eAR : c.eAY;
+  const real base = ps * c.a;
+  const real same = base + ps * bs;
+  Q4<real> v;
+  v.x = (R ? same : base) + (s == 0 ? es : real(0));
+  v.z = (R ? same : base) + (s == 2 ? es : real(0));
+  v.y = (R ? base : same) + (s == 1 ? es : real(0));
+  v.w = (R ? base : same) + (s == 3 ? es : real(0));
+  return v;
+}
+
+template <typename real>
+__device__ __forceinline__ Q4<real> int_vec(const HkyCoef<real> &c, const Q4<real> &L,
+                                            const real pi[4]) {
+  const real WR = pi[0] * L.x + pi[2] * L.z;
+  const real WY = pi[1] * L.y + pi[3] * L.w;
+  const real aW = c.a * (WR + WY);
+  const real tR = aW + c.bR * WR;
+  const real tY = aW + c.bY * WY;
+  Q4<real> v;
+  v.x = tR + c.eAR * L.x;
+  v.z = tR + c.eAR * L.z;
+  v.y = tY + c.eAY * L.y;
+  v.w = tY + c.eAY * L.w;
+  return v;
+}
+
+__device__ __forceinline__ float ldexp_r(float x, int e) { return ldexpf(x, e); }
+__device__ __forceinline__ double ldexp_r(double x, int e) { return ldexp(x, e); }
+__device__ __forceinline__ int frexp_e(float x) {
+  int e;
+  frexpf(x, &e);
+  return e;
+}
+__device__ __forceinline__ int frexp_e(double x) {
+  int e;
+  frexp(x, &e);
+  return e;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Decoded operation with its operands staged for the next compute step.
+template <typename real>
+struct Staged {
+  int type;
+  HkyCoef<real> ca, cb;
+  int ta, tb;  // tip states (when the operand is a tip)
+};
+
+template <typename real>
+__device__ __forceinline__ void stage(const ProgOp &raw, const HkyCoef<real> *coef,
+                                      const int8_t *tips, int lane, Staged<real> &s) {
+  const int a = uni(raw.a), b = uni(raw.b);
+  s.type = uni(raw.type);
+  s.ca = coef[a];
+  s.cb = coef[b];
+  s.ta = (s.type == OP_TT) ? tips[a * kWave + lane] : 0;
+  s.tb = (s.type != OP_SS) ? tips[b * kWave + lane] : 0;
+}
+
+}  // namespace
+
+template <typename real, int D>
+__global__ __launch_bounds__(kWave) void hky_kernel(
+    DeviceView v, const ima_item *__restrict__ items,
+    const int32_t *__restrict__ idx, double *__restrict__ out,
+    int32_t *__restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  const int ii = idx[blockIdx.x];
+  const ima_item item = items[ii];
+  const SlotMeta sm = v.slots[item.slot];
+  const LocusMeta lm = v.loci[sm.locus];
+  const int n = sm.n, ne = 2 * n - 1, S = lm.S, nops = n - 1;
+
+  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);
+  ProgOp *ops = reinterpret_cast<ProgOp *>(smem + (size_t)ne * sizeof(HkyCoef<real>));
+  int8_t *tips = reinterpret_cast<int8_t *>(ops + nops);
+
+  const double *pid = lm.pi;
+  const double kappa = item.kappa;
+  // getstandfactor (calc_prob_data.cpp:474-493) and the per-site rate
+  // u / (totsites * standfactor) (:593).
+  double sf = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      if (i != j) sf += (i + j == 2 || i + j == 4) ? pid[i] * pid[j] * kappa : pid[i] * pid[j];
+  const double mutrate = item.u / (lm.totsites * sf);
+  const double PR = pid[0] + pid[2], PY = pid[1] + pid[3];
+  const double AR = 1.0 + PR * (kappa - 1.0), AY = 1.0 + PY * (kappa - 1.0);
+
+  const int32_t *up0 = v.up0 + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  for (int e = lane; e < ne; e += kWave) {
+    if (e == sm.root) continue;
+    const int u0 = up0[e];
+    const double bl = tm[e] - (u0 >= 0 ? tm[u0] : 0.0);
+    const double mt = mutrate * bl;
+    const double e1 = exp(-mt), eR = exp(-mt * AR), eY = exp(-mt * AY);
+    HkyCoef<real> c;
+    c.a = real(1.0 - e1);
+    c.bR = real((e1 - eR) / PR);
+    c.bY = real((e1 - eY) / PY);
+    c.eAR = real(eR);
+    c.eAY = real(eY);
+    c.pad0 = c.pad1 = c.pad2 = real(0);
+    coef[e] = c;
+  }
+  const ProgOp *prog = v.prog + sm.prog_off;
+  for (int k = lane; k < nops; k += kWave) ops[k] = prog[k];
+  const real pi[4] = {real(pid[0]), real(pid[1]), real(pid[2]), real(pid[3])};
+  const int8_t *seq = v.seq + lm.seq_off;
+  const int32_t *mult = v.mult + lm.mult_off;
+
+  double acc = 0.0;
+  const int nblocks = (S + kWave - 1) / kWave;
+  for (int blk = 0; blk < nblocks; ++blk) {
+    const int p = blk * kWave + lane;
+    const bool valid = p < S;
+    __syncthreads();
+    for (int t = 0; t < n; ++t) tips[t * kWave + lane] = valid ? seq[(size_t)t * S + p] : 0;
+    __syncthreads();
+
+    Q4<real> stk[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) stk[i].x = stk[i].y = stk[i].z = stk[i].w = real(0);
+    int E = 0;
+    Staged<real> cur, nxt;
+    ProgOp raw1 = ops[nops > 1 ? 1 : 0];
+    stage<real>(ops[0], coef, tips, lane, cur);
+    for (int k = 0; k < nops; ++k) {
+      // two-stage pipeline: raw op k+2 and the operands of op k+1 are read
+      // from LDS while op k computes
+      const ProgOp raw2 = ops[k + 2 < nops ? k + 2 : nops - 1];
+      if (k + 1 < nops) stage<real>(raw1, coef, tips, lane, nxt);
+      Q4<real> va, vb;
+      if (cur.type == OP_TT) {
+        va = tip_vec<real>(cur.ca, cur.ta, pi);
+        vb = tip_vec<real>(cur.cb, cur.tb, pi);
+      } else if (cur.type == OP_ST) {
+        va = int_vec<real>(cur.ca, stk[0], pi);
+        vb = tip_vec<real>(cur.cb, cur.tb, pi);
+      } else {
+        va = int_vec<real>(cur.ca, stk[1], pi);
+        vb = int_vec<real>(cur.cb, stk[0], pi);
+      }
+      Q4<real> L;
+      L.x = va.x * vb.x;
+      L.y = va.y * vb.y;
+      L.z = va.z * vb.z;
+      L.w = va.w * vb.w;
+      const real mx = fmax(fmax(L.x, L.y), fmax(L.z, L.w));
+      const int ex = frexp_e(mx);
+      E += ex;
+      L.x = ldexp_r(L.x, -ex);
+      L.y = ldexp_r(L.y, -ex);
+      L.z = ldexp_r(L.z, -ex);
+      L.w = ldexp_r(L.w, -ex);
+      if (cur.type == OP_TT) {  // push
+#pragma unroll
+        for (int i = D - 1; i > 0; --i) stk[i] = stk[i - 1];
+      } else if (cur.type == OP_SS) {  // pop one
+#pragma unroll
+        for (int i = 1; i < D - 1; ++i) stk[i] = stk[i + 1];
+      }
+      stk[0] = L;
+      cur = nxt;
+      raw1 = raw2;
+    }
+    const real fr = pi[0] * stk[0].x + pi[1] * stk[0].y + pi[2] * stk[0].z + pi[3] * stk[0].w;
+    if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
+  }
+  acc = wave_sum_d(acc);
+  if (lane == 0) {
+    out[ii] = acc;
+    if (status) status[ii] = IMA_STATUS_OK;
+  }
+}
+
+size_t hky_lds_bytes(int precision, int max_n) {
+  const size_t ne = 2 * (size_t)max_n - 1;
+  const size_t cs = precision == IMA_PREC_FP32 ? sizeof(HkyCoef<float>) : sizeof(HkyCoef<double>);
+  return ne * cs + ((size_t)max_n - 1) * sizeof(ProgOp) + (size_t)max_n * kWave;
+}
+
+template <typename real>
+static void launch_depth(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
+                         const ima_item *items, const int32_t *idx, double *out,
+                         int32_t *status) {
+  if (D <= 4)
+    hipLaunchKernelGGL((hky_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+  else if (D <= 6)
+    hipLaunchKernelGGL((hky_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+  else
+    hipLaunchKernelGGL((hky_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+}
+
+hipError_t launch_hky(const DeviceView &v, const ima_item *items, const int32_t *item_idx,
+                      int nitems, int precision, int max_n, int max_slots, double *out,
+                      int32_t *status, hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  if (max_slots > 8) return hipErrorInvalidValue;
+  const size_t lds = hky_lds_bytes(precision, max_n);
+  if (precision == IMA_PREC_FP32)
+    launch_depth<float>(max_slots, dim3(nitems), lds, st, v, items, item_idx, out, status);
+  else
+    launch_depth<double>(max_slots, dim3(nitems), lds, st, v, items, item_idx, out, status);
+  return hipGetLastError();
+}
+
+}  // namespace ima

@@ -1,6 +1,5 @@
 // kern_lik.hip -- batched per-locus data likelihood kernels for gfx950.
 //
-//   HKY  Felsenstein pruning (calc_prob_data.cpp:118-607)
 //   IS   infinite-sites mutation placement (calc_prob_data.cpp:537-836)
 //   SW   stepwise-mutation Bessel terms (calc_prob_data.cpp:842-909,
 //        utilities.cpp:55-124, 1453-1491)
@@ -26,177 +25,6 @@ __device__ __forceinline__ int wave_or(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
   return v;
-}
-
-// ================================================================ HKY ==
-//
-// Pruning runs on chip.  The host turns each genealogy into a postorder
-// program whose operands are tips or LDS "partial slots"; daughters with the
-// larger slot need are evaluated first, so a genealogy of n tips needs at
-// most floor(log2 n)+1 slots (Strahler number) of 64 lanes x 4 states.
-// Nothing per node goes to HBM: a proposal is always re-scored over the
-// whole genealogy, which costs ~(36n-58)S flops on chip instead of the
-// ~36 B per node-pattern of HBM traffic that stored partials would need.
-//
-// Branch transition (HKY85 closed form, calc_prob_data.cpp:197-208):
-//   P_jk(t) = pi_k (1-e) + [cls(j)=cls(k)] pi_k (e - eA_c)/PI_c + [j=k] eA_c
-// with e = exp(-mu t), eA_c = exp(-mu t A_c), A_c = 1 + PI_c (kappa-1),
-// c the purine {A,G} or pyrimidine {C,T} class.  Hence for a daughter with
-// partial L:  (P L)_j = a W + b_c W_c + eA_c L_j,  a = 1-e,
-// b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.
-// Partials are rescaled at every node by an exact power of two; one
-// integer exponent per lane carries the sum of the node scale factors.
-
-template <typename real>
-struct __align__(16) HkyCoef {
-  real a, bR, bY, eAR, eAY, pad0, pad1, pad2;
-};
-
-template <typename real>
-__device__ __forceinline__ void hky_child(const HkyCoef<real> &c, int slot,
-                                          int edge, const real *part,
-                                          const int8_t *tips, int lane,
-                                          const real pi[4], real v[4]) {
-  if (slot == kTipSlot) {
-    const int s = tips[edge * kWave + lane];
-    const bool R = (s & 1) == 0;  // A=0, G=2 purines; C=1, T=3 pyrimidines
-    const real ps = s == 0 ? pi[0] : (s == 1 ? pi[1] : (s == 2 ? pi[2] : pi[3]));
-    const real bs = R ? c.bR : c.bY;
-    const real es = R ? c.eAR : c.eAY;
-    const real base = ps * c.a;
-    const real same = base + ps * bs;
-    v[0] = (R ? same : base) + (s == 0 ? es : real(0));
-    v[2] = (R ? same : base) + (s == 2 ? es : real(0));
-    v[1] = (R ? base : same) + (s == 1 ? es : real(0));
-    v[3] = (R ? base : same) + (s == 3 ? es : real(0));
-  } else {
-    const real *L = part + ((size_t)slot * kWave + lane) * 4;
-    const real L0 = L[0], L1 = L[1], L2 = L[2], L3 = L[3];
-    const real WR = pi[0] * L0 + pi[2] * L2;
-    const real WY = pi[1] * L1 + pi[3] * L3;
-    const real aW = c.a * (WR + WY);
-    const real tR = aW + c.bR * WR;
-    const real tY = aW + c.bY * WY;
-    v[0] = tR + c.eAR * L0;
-    v[2] = tR + c.eAR * L2;
-    v[1] = tY + c.eAY * L1;
-    v[3] = tY + c.eAY * L3;
-  }
-}
-
-__device__ __forceinline__ float ldexp_r(float x, int e) { return ldexpf(x, e); }
-__device__ __forceinline__ double ldexp_r(double x, int e) { return ldexp(x, e); }
-__device__ __forceinline__ int frexp_e(float x) { int e; frexpf(x, &e); return e; }
-__device__ __forceinline__ int frexp_e(double x) { int e; frexp(x, &e); return e; }
-
-template <typename real>
-__global__ __launch_bounds__(kWave) void hky_kernel(
-    DeviceView v, const ima_item *__restrict__ items,
-    const int32_t *__restrict__ idx, double *__restrict__ out,
-    int32_t *__restrict__ status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
-  const int ii = idx[blockIdx.x];
-  const ima_item item = items[ii];
-  const SlotMeta sm = v.slots[item.slot];
-  const LocusMeta lm = v.loci[sm.locus];
-  const int n = sm.n, ne = 2 * n - 1, S = lm.S;
-
-  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);
-  real *part = reinterpret_cast<real *>(smem + (size_t)ne * sizeof(HkyCoef<real>));
-  int8_t *tips = reinterpret_cast<int8_t *>(part + (size_t)sm.nslots * kWave * 4);
-
-  const double *pid = lm.pi;
-  const double kappa = item.kappa;
-  // getstandfactor (calc_prob_data.cpp:474-493) and the per-site rate
-  // u / (totsites * standfactor) (:593).
-  double sf = 0.0;
-  for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++)
-      if (i != j) sf += (i + j == 2 || i + j == 4) ? pid[i] * pid[j] * kappa : pid[i] * pid[j];
-  const double mutrate = item.u / (lm.totsites * sf);
-  const double PR = pid[0] + pid[2], PY = pid[1] + pid[3];
-  const double AR = 1.0 + PR * (kappa - 1.0), AY = 1.0 + PY * (kappa - 1.0);
-
-  const int32_t *up0 = v.up0 + sm.node_off;
-  const double *tm = v.time + sm.node_off;
-  for (int e = lane; e < ne; e += kWave) {
-    if (e == sm.root) continue;
-    const int u0 = up0[e];
-    const double bl = tm[e] - (u0 >= 0 ? tm[u0] : 0.0);
-    const double mt = mutrate * bl;
-    const double e1 = exp(-mt), eR = exp(-mt * AR), eY = exp(-mt * AY);
-    HkyCoef<real> c;
-    c.a = real(1.0 - e1);
-    c.bR = real((e1 - eR) / PR);
-    c.bY = real((e1 - eY) / PY);
-    c.eAR = real(eR);
-    c.eAY = real(eY);
-    c.pad0 = c.pad1 = c.pad2 = real(0);
-    coef[e] = c;
-  }
-  real pi[4] = {real(pid[0]), real(pid[1]), real(pid[2]), real(pid[3])};
-  const ProgOp *prog = v.prog + sm.prog_off;
-  const int8_t *seq = v.seq + lm.seq_off;
-  const int32_t *mult = v.mult + lm.mult_off;
-
-  double acc = 0.0;
-  const int nblocks = (S + kWave - 1) / kWave;
-  for (int blk = 0; blk < nblocks; ++blk) {
-    const int p = blk * kWave + lane;
-    const bool valid = p < S;
-    __syncthreads();
-    for (int t = 0; t < n; ++t) tips[t * kWave + lane] = valid ? seq[(size_t)t * S + p] : 0;
-    __syncthreads();
-    int E = 0;
-    int last = 0;
-    for (int k = 0; k < n - 1; ++k) {
-      const ProgOp op = prog[k];
-      const int a = op.a & 0xFFFF, as = (op.a >> 16) & 0xFFFF;
-      const int b = op.b & 0xFFFF, bs = (op.b >> 16) & 0xFFFF;
-      real va[4], vb[4];
-      hky_child<real>(coef[a], as, a, part, tips, lane, pi, va);
-      hky_child<real>(coef[b], bs, b, part, tips, lane, pi, vb);
-      real L0 = va[0] * vb[0], L1 = va[1] * vb[1], L2 = va[2] * vb[2], L3 = va[3] * vb[3];
-      const real mx = fmax(fmax(L0, L1), fmax(L2, L3));
-      const int ex = frexp_e(mx);
-      E += ex;
-      real *o = part + ((size_t)op.out * kWave + lane) * 4;
-      o[0] = ldexp_r(L0, -ex);
-      o[1] = ldexp_r(L1, -ex);
-      o[2] = ldexp_r(L2, -ex);
-      o[3] = ldexp_r(L3, -ex);
-      last = op.out;
-    }
-    const real *R = part + ((size_t)last * kWave + lane) * 4;
-    const real fr = pi[0] * R[0] + pi[1] * R[1] + pi[2] * R[2] + pi[3] * R[3];
-    if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
-  }
-  acc = wave_sum(acc);
-  if (lane == 0) {
-    out[ii] = acc;
-    if (status) status[ii] = IMA_STATUS_OK;
-  }
-}
-
-hipError_t launch_hky(const DeviceView &v, const ima_item *items,
-                      const int32_t *item_idx, int nitems, int precision,
-                      int max_n, int max_slots, double *out, int32_t *status,
-                      hipStream_t st) {
-  if (nitems == 0) return hipSuccess;
-  const int ne = 2 * max_n - 1;
-  if (precision == IMA_PREC_FP32) {
-    size_t lds = (size_t)ne * sizeof(HkyCoef<float>) +
-                 (size_t)max_slots * kWave * 4 * sizeof(float) + (size_t)max_n * kWave;
-    hipLaunchKernelGGL(hky_kernel<float>, dim3(nitems), dim3(kWave), lds, st, v,
-                       items, item_idx, out, status);
-  } else {
-    size_t lds = (size_t)ne * sizeof(HkyCoef<double>) +
-                 (size_t)max_slots * kWave * 4 * sizeof(double) + (size_t)max_n * kWave;
-    hipLaunchKernelGGL(hky_kernel<double>, dim3(nitems), dim3(kWave), lds, st, v,
-                       items, item_idx, out, status);
-  }
-  return hipGetLastError();
 }
 
 // ================================================================= IS ==

@@ -17,13 +17,16 @@
 // by s = exp(d (U - 1/2)), with Hastings term (n-1) log s per genealogy.
 // Neither is the reference's proposal (slider/addmigration); they exercise
 // the same scoring, acceptance, commit/rollback and swap path.
+//
+// Work mapping: per-item kernels use one thread per (chain, locus);
+// per-node kernels one thread per genealogy node (node_item maps a node to
+// its item); per-chain reductions one wavefront per chain.
 #include <float.h>
 
 #include "ima_internal.h"
 #include "philox.h"
 
 namespace ima {
-
 
 __device__ inline PhiloxStream item_rng(const MC3Dev &m, int item, uint64_t sweep, int phase) {
   return PhiloxStream(m.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(item + 1)),
@@ -33,6 +36,17 @@ __device__ inline PhiloxStream item_rng(const MC3Dev &m, int item, uint64_t swee
 __device__ inline double normal01(PhiloxStream &r) {
   const double u1 = 1.0 - r.uniform(), u2 = r.uniform();  // u1 in (0, 1]
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ int wand(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v &= __shfl_xor(v, off, kWave);
+  return v;
 }
 
 __global__ void mc3_slide_propose(DeviceView v, MC3Dev m) {
@@ -86,7 +100,6 @@ __global__ void mc3_slide_accept(DeviceView v, MC3Dev m) {
   }
   if (acc) {
     m.pdg[it] = m.newpdg[it];
-    atomicAdd(&m.stats[1], 1ull);
   } else {
     const SlotMeta sm = v.slots[m.first_slot + it];
     const int e = m.rec_v[it];
@@ -95,80 +108,84 @@ __global__ void mc3_slide_accept(DeviceView v, MC3Dev m) {
     tm[up0[e]] = m.rec_t[it];
     tm[up1[e]] = m.rec_t[it];
   }
-  atomicAdd(&m.stats[0], 1ull);
+  m.acc[it] += acc ? 1u : 0u;
 }
 
-__global__ void mc3_scale_propose(DeviceView v, MC3Dev m) {
-  const int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= m.C * m.L) return;
-  const uint64_t sweep = *m.counter;
-  const int c = it / m.L;
-  PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 2);  // one draw per chain
-  const double ls = m.scale_width * (r.uniform() - 0.5);
-  const double s = exp(ls);
-  if (it % m.L == 0) m.logs[c] = ls;
-  const SlotMeta sm = v.slots[m.first_slot + it];
-  const int ne = 2 * sm.n - 1;
-  double *tm = const_cast<double *>(v.time) + sm.node_off;
-  double *bk = m.tbackup + sm.node_off;
-  const int32_t *dn = v.down + sm.node_off;
-  for (int e = 0; e < ne; ++e) {
-    bk[e] = tm[e];
-    if (dn[e] != -1) tm[e] = tm[e] * s;
+// One log scale factor per chain.
+__global__ void mc3_scale_draw(MC3Dev m) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  PhiloxStream r = item_rng(m, m.C * m.L + c, *m.counter, 2);
+  m.logs[c] = m.scale_width * (r.uniform() - 0.5);
+}
+
+// Node-parallel rescaling of every coalescence time (root edge excluded).
+__global__ void mc3_scale_apply(DeviceView v, MC3Dev m) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m.nnodes) return;
+  const int64_t g = m.node_first + j;
+  const double t = v.time[g];
+  m.tbackup[j] = t;
+  if (v.down[g] != -1) {
+    const int c = m.node_item[j] / m.L;
+    const_cast<double *>(v.time)[g] = t * exp(m.logs[c]);
   }
 }
 
-__global__ void mc3_scale_decide(DeviceView v, MC3Dev m) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.C) return;
-  const uint64_t sweep = *m.counter;
-  PhiloxStream r = item_rng(m, m.C * m.L + m.C + c, sweep, 3);
-  const double U = r.uniform();
+// One wavefront per chain: log acceptance = sum_l beta (new - old) +
+// (n_l - 1) log s.
+__global__ __launch_bounds__(kWave) void mc3_scale_decide(DeviceView v, MC3Dev m) {
+  const int c = blockIdx.x, lane = threadIdx.x;
   double a = 0.0;
-  bool ok = true;
-  for (int l = 0; l < m.L; ++l) {
+  int ok = 1;
+  for (int l = lane; l < m.L; l += kWave) {
     const int it = c * m.L + l;
-    ok = ok && (m.status[it] == IMA_STATUS_OK);
+    ok &= (m.status[it] == IMA_STATUS_OK);
     const int n = v.slots[m.first_slot + it].n;
     a += m.beta[c] * (m.newpdg[it] - m.pdg[it]) + (n - 1) * m.logs[c];
   }
-  const int acc = ok && (U < fmin(1.0, exp(a)));
-  m.decision[c] = acc;
-  atomicAdd(&m.stats[2], 1ull);
-  if (acc) atomicAdd(&m.stats[3], 1ull);
+  a = wsum(a);
+  ok = wand(ok);
+  if (lane == 0) {
+    PhiloxStream r = item_rng(m, m.C * m.L + m.C + c, *m.counter, 3);
+    const double U = r.uniform();
+    const int acc = ok && (U < fmin(1.0, exp(a)));
+    m.decision[c] = acc;
+    m.cacc[c] += (uint32_t)acc;
+  }
 }
 
-__global__ void mc3_scale_commit(DeviceView v, MC3Dev m) {
+__global__ void mc3_scale_commit_items(MC3Dev m) {
   const int it = blockIdx.x * blockDim.x + threadIdx.x;
   if (it >= m.C * m.L) return;
-  const int c = it / m.L;
-  if (m.decision[c]) {
-    m.pdg[it] = m.newpdg[it];
-    return;
-  }
-  const SlotMeta sm = v.slots[m.first_slot + it];
-  const int ne = 2 * sm.n - 1;
-  double *tm = const_cast<double *>(v.time) + sm.node_off;
-  const double *bk = m.tbackup + sm.node_off;
-  for (int e = 0; e < ne; ++e) tm[e] = bk[e];
+  if (m.decision[it / m.L]) m.pdg[it] = m.newpdg[it];
 }
 
-// Chain sums S_c = sum_l pdg (swapweight, swapchains.cpp:13-34), then the
-// swap attempts of swapchains (single lane), then advance the sweep counter.
-__global__ void mc3_swap(MC3Dev m, int swaptries) {
-  __shared__ double Ssh[1024];
-  for (int c = threadIdx.x; c < m.C; c += blockDim.x) {
+__global__ void mc3_scale_commit_nodes(DeviceView v, MC3Dev m) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m.nnodes) return;
+  if (!m.decision[m.node_item[j] / m.L])
+    const_cast<double *>(v.time)[m.node_first + j] = m.tbackup[j];
+}
+
+// Chain sums S_c = sum_l pdg (swapweight, swapchains.cpp:13-34) with one
+// wavefront per chain, then the swap attempts of swapchains on one lane,
+// then advance the sweep counter.
+__global__ __launch_bounds__(256) void mc3_swap(MC3Dev m, int swaptries) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int c = w; c < m.C; c += blockDim.x / kWave) {
     double s = 0.0;
-    for (int l = 0; l < m.L; ++l) s += m.pdg[c * m.L + l];
-    m.S[c] = s;
-    if (c < 1024) Ssh[c] = s;
+    for (int l = lane; l < m.L; l += kWave) s += m.pdg[c * m.L + l];
+    s = wsum(s);
+    if (lane == 0) m.S[c] = s;
   }
   __syncthreads();
+  __threadfence_block();
   if (threadIdx.x != 0) return;
   const uint64_t sweep = *m.counter;
   PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, sweep);
   const int SWAPDIST = 7, nchains = m.C;
-  const double *S = (m.C <= 1024) ? Ssh : m.S;
+  const double *S = m.S;
   unsigned long long acc = 0;
   for (int i = 0; i < swaptries && nchains > 1; ++i) {
     int ci, cj, cjmin, cjrange;
@@ -185,8 +202,8 @@ __global__ void mc3_swap(MC3Dev m, int swaptries) {
     do {
       cj = cjmin + (int)(rng.uniform() * cjrange);
     } while (cj == ci || cj < 0 || cj >= nchains);
-    const double w = exp((m.beta[ci] - m.beta[cj]) * (S[cj] - S[ci]));
-    if (w >= 1.0 || w > rng.uniform()) {
+    const double wgt = exp((m.beta[ci] - m.beta[cj]) * (S[cj] - S[ci]));
+    if (wgt >= 1.0 || wgt > rng.uniform()) {
       const double b = m.beta[ci];
       m.beta[ci] = m.beta[cj];
       m.beta[cj] = b;
@@ -195,6 +212,7 @@ __global__ void mc3_swap(MC3Dev m, int swaptries) {
   }
   m.stats[4] += (unsigned long long)(nchains > 1 ? swaptries : 0);
   m.stats[5] += acc;
+  m.stats[0] += (unsigned long long)(m.C * m.L);
   *m.counter = sweep + 1;
 }
 
@@ -206,6 +224,7 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              hipStream_t st) {
   const int N = m.C * m.L;
   const int tb = 256, gb = (N + tb - 1) / tb;
+  const int gn = (int)((m.nnodes + tb - 1) / tb);
   // score_ev (optional): events recorded around the two scoring passes
   auto score = [&](int pass) -> hipError_t {
     hipError_t e = hipSuccess;
@@ -224,11 +243,13 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(mc3_slide_accept, dim3(gb), dim3(tb), 0, st, v, m);
   if (with_scale) {
-    hipLaunchKernelGGL(mc3_scale_propose, dim3(gb), dim3(tb), 0, st, v, m);
+    hipLaunchKernelGGL(mc3_scale_draw, dim3((m.C + 63) / 64), dim3(64), 0, st, m);
+    hipLaunchKernelGGL(mc3_scale_apply, dim3(gn), dim3(tb), 0, st, v, m);
     e = score(1);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mc3_scale_decide, dim3((m.C + 63) / 64), dim3(64), 0, st, v, m);
-    hipLaunchKernelGGL(mc3_scale_commit, dim3(gb), dim3(tb), 0, st, v, m);
+    hipLaunchKernelGGL(mc3_scale_decide, dim3(m.C), dim3(kWave), 0, st, v, m);
+    hipLaunchKernelGGL(mc3_scale_commit_items, dim3(gb), dim3(tb), 0, st, m);
+    hipLaunchKernelGGL(mc3_scale_commit_nodes, dim3(gn), dim3(tb), 0, st, v, m);
   }
   hipLaunchKernelGGL(mc3_swap, dim3(1), dim3(256), 0, st, m, swaptries);
   return hipGetLastError();
