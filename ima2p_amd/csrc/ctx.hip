@@ -360,10 +360,17 @@ int ima_locus_add_hky(ima_ctx *c, int n, int S, const int8_t *seq, const int32_t
   m.n = n;
   m.S = S;
   m.totsites = totsites;
+  // rows of 64-pattern blocks, 64-byte aligned, zero padded: one block of
+  // one tip is 16 aligned dwords
+  m.Spad = (S + kWave - 1) / kWave * kWave;
+  c->seq.resize((c->seq.size() + 63) / 64 * 64, 0);
   m.seq_off = (int64_t)c->seq.size();
   m.mult_off = (int64_t)c->mult.size();
   for (int i = 0; i < 4; ++i) m.pi[i] = pi[i];
-  c->seq.insert(c->seq.end(), seq, seq + (size_t)n * S);
+  c->seq.resize(c->seq.size() + (size_t)n * m.Spad, 0);
+  for (int t = 0; t < n; ++t)
+    std::copy(seq + (size_t)t * S, seq + (size_t)(t + 1) * S,
+              c->seq.begin() + m.seq_off + (size_t)t * m.Spad);
   c->mult.insert(c->mult.end(), mult, mult + S);
   *locus = (int)c->loci.size();
   c->loci.push_back(m);

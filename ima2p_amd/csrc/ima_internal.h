@@ -27,7 +27,7 @@ struct LocusMeta {
   int64_t mult_off;  // HKY: int32 [S]
   int64_t mask_off;  // IS: uint64 [S][W] tips carrying state 1
   int32_t W;         // IS: 64-bit words per tip mask
-  int32_t pad;
+  int32_t Spad;      // HKY: row stride of seq (S rounded up to 64, zero padded)
   double pi[4];
   double sumlogk;
 };

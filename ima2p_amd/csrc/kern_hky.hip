@@ -11,52 +11,37 @@
 // n <= 256.  The stack is a register array of D quads addressed only at
 // compile-time positions (push shifts down, pop shifts up), selected per
 // operation by a wave-uniform op type, so a node's partial never leaves the
-// register file and no private (scratch) memory is used.  Nothing per node goes to
-// HBM: every proposal is re-scored over the whole genealogy, ~(36n-58)S
-// flops on chip, instead of reading and writing ~36 B per node-pattern of
-// stored partials.
+// register file and no private (scratch) memory is used.  Nothing per node
+// goes to HBM: every proposal is re-scored over the whole genealogy,
+// ~(36n-58)S flops on chip, instead of reading and writing ~36 B per
+// node-pattern of stored partials.
 //
 // Branch transition (HKY85 closed form, calc_prob_data.cpp:197-208):
 //   P_jk(t) = pi_k (1-e) + [cls(j)=cls(k)] pi_k (e - eA_c)/PI_c + [j=k] eA_c
 // with e = exp(-mu t), eA_c = exp(-mu t A_c), A_c = 1 + PI_c (kappa-1), c the
 // purine {A,G} or pyrimidine {C,T} class of the destination base.  For a
 // daughter with partial L:  (P L)_j = a W + b_c W_c + eA_c L_j with a = 1-e,
-// b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.  The
-// five per-branch coefficients are computed once per item in FP64 and kept
-// in LDS.  Partials are rescaled at every node by an exact power of two; one
-// integer exponent per lane carries the sum of the node scale factors
-// (the reference keeps log(max) per node, :211-217, 465).
+// b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.  For a
+// tip in state s the vector is column s of P.  Per branch the five
+// coefficients and the four columns of P are computed once per item in FP64
+// and kept in LDS; a tip operand is one 16-byte LDS read.  Partials are
+// rescaled at every node by an exact power of two; one integer exponent per
+// lane carries the sum of the node scale factors (the reference keeps
+// log(max) per node, :211-217, 465).
 #include "ima_internal.h"
 
 namespace ima {
 namespace {
 
 template <typename real>
-struct __align__(16) HkyCoef {
-  real a, bR, bY, eAR, eAY, pad0, pad1, pad2;
-};
-
-template <typename real>
 struct Q4 {
   real x, y, z, w;
 };
 
-// Column s of P for a tip in state s.
 template <typename real>
-__device__ __forceinline__ Q4<real> tip_vec(const HkyCoef<real> &c, int s, const real pi[4]) {
-  const bool R = (s & 1) == 0;  // A=0, G=2 purines; C=1, T=3 pyrimidines
-  const real ps = s == 0 ? pi[0] : (s == 1 ? pi[1] : (s == 2 ? pi[2] : pi[3]));
-  const real bs = R ? c.bR : c.bY;
-  const real es = R ? c.eAR : c.eAY;
-  const real base = ps * c.a;
-  const real same = base + ps * bs;
-  Q4<real> v;
-  v.x = (R ? same : base) + (s == 0 ? es : real(0));
-  v.z = (R ? same : base) + (s == 2 ? es : real(0));
-  v.y = (R ? base : same) + (s == 1 ? es : real(0));
-  v.w = (R ? base : same) + (s == 3 ? es : real(0));
-  return v;
-}
+struct __align__(16) HkyCoef {
+  real a, bR, bY, eAR, eAY, pad0, pad1, pad2;
+};
 
 template <typename real>
 __device__ __forceinline__ Q4<real> int_vec(const HkyCoef<real> &c, const Q4<real> &L,
@@ -98,7 +83,7 @@ __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlan
 // Decoded operation with its operands staged for the next compute step.
 template <typename real>
 struct Staged {
-  int type;
+  int type, a, b;
   HkyCoef<real> ca, cb;
   int ta, tb;  // tip states (when the operand is a tip)
 };
@@ -106,15 +91,23 @@ struct Staged {
 template <typename real>
 __device__ __forceinline__ void stage(const ProgOp &raw, const HkyCoef<real> *coef,
                                       const int8_t *tips, int lane, Staged<real> &s) {
-  const int a = uni(raw.a), b = uni(raw.b);
+  s.a = uni(raw.a);
+  s.b = uni(raw.b);
   s.type = uni(raw.type);
-  s.ca = coef[a];
-  s.cb = coef[b];
-  s.ta = (s.type == OP_TT) ? tips[a * kWave + lane] : 0;
-  s.tb = (s.type != OP_SS) ? tips[b * kWave + lane] : 0;
+  s.ca = coef[s.a];
+  s.cb = coef[s.b];
+  s.ta = (s.type == OP_TT) ? tips[s.a * kWave + lane] : 0;
+  s.tb = (s.type != OP_SS) ? tips[s.b * kWave + lane] : 0;
 }
 
 }  // namespace
+
+size_t hky_lds_bytes(int precision, int max_n) {
+  const size_t ne = 2 * (size_t)max_n - 1;
+  const size_t r = precision == IMA_PREC_FP32 ? 4 : 8;
+  return ne * (8 * r) + ne * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) +
+         (size_t)max_n * kWave;
+}
 
 template <typename real, int D>
 __global__ __launch_bounds__(kWave) void hky_kernel(
@@ -129,9 +122,10 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   const LocusMeta lm = v.loci[sm.locus];
   const int n = sm.n, ne = 2 * n - 1, S = lm.S, nops = n - 1;
 
-  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);
-  ProgOp *ops = reinterpret_cast<ProgOp *>(smem + (size_t)ne * sizeof(HkyCoef<real>));
-  int8_t *tips = reinterpret_cast<int8_t *>(ops + nops);
+  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);           // [ne]
+  Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);                 // [ne][4]
+  ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)ne * 4);          // [nops]
+  int8_t *tips = reinterpret_cast<int8_t *>(ops + nops);                    // [n][64]
 
   const double *pid = lm.pi;
   const double kappa = item.kappa;
@@ -153,20 +147,36 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     const double bl = tm[e] - (u0 >= 0 ? tm[u0] : 0.0);
     const double mt = mutrate * bl;
     const double e1 = exp(-mt), eR = exp(-mt * AR), eY = exp(-mt * AY);
+    const double bR = (e1 - eR) / PR, bY = (e1 - eY) / PY, a = 1.0 - e1;
     HkyCoef<real> c;
-    c.a = real(1.0 - e1);
-    c.bR = real((e1 - eR) / PR);
-    c.bY = real((e1 - eY) / PY);
+    c.a = real(a);
+    c.bR = real(bR);
+    c.bY = real(bY);
     c.eAR = real(eR);
     c.eAY = real(eY);
     c.pad0 = c.pad1 = c.pad2 = real(0);
     coef[e] = c;
+    // column s of P: P_js = pi_s a + [cls j = cls s] pi_s b_c + [j = s] eA_c
+    for (int s = 0; s < 4; ++s) {
+      const bool Rs = (s & 1) == 0;
+      const double ps = pid[s], base = ps * a, same = base + ps * (Rs ? bR : bY);
+      const double es = Rs ? eR : eY;
+      Q4<real> col;
+      col.x = real((Rs ? same : base) + (s == 0 ? es : 0.0));
+      col.y = real((Rs ? base : same) + (s == 1 ? es : 0.0));
+      col.z = real((Rs ? same : base) + (s == 2 ? es : 0.0));
+      col.w = real((Rs ? base : same) + (s == 3 ? es : 0.0));
+      colP[e * 4 + s] = col;
+    }
   }
   const ProgOp *prog = v.prog + sm.prog_off;
   for (int k = lane; k < nops; k += kWave) ops[k] = prog[k];
   const real pi[4] = {real(pid[0]), real(pid[1]), real(pid[2]), real(pid[3])};
-  const int8_t *seq = v.seq + lm.seq_off;
+  const uint32_t *seq32 = reinterpret_cast<const uint32_t *>(v.seq + lm.seq_off);
+  const int row32 = lm.Spad / 4;
   const int32_t *mult = v.mult + lm.mult_off;
+  uint32_t *tips32 = reinterpret_cast<uint32_t *>(tips);
+  const int words = n * (kWave / 4);  // dwords of one block: n rows x 16
 
   double acc = 0.0;
   const int nblocks = (S + kWave - 1) / kWave;
@@ -174,7 +184,21 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     const int p = blk * kWave + lane;
     const bool valid = p < S;
     __syncthreads();
-    for (int t = 0; t < n; ++t) tips[t * kWave + lane] = valid ? seq[(size_t)t * S + p] : 0;
+    // tip states of this block: n rows of 16 aligned dwords, all loads issued
+    // before the LDS writes
+    for (int w0 = 0; w0 < words; w0 += 16 * kWave) {
+      uint32_t r[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int w = min(w0 + j * kWave + lane, words - 1);
+        r[j] = seq32[(size_t)(w >> 4) * row32 + blk * 16 + (w & 15)];
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int w = w0 + j * kWave + lane;
+        if (w < words) tips32[w] = r[j];
+      }
+    }
     __syncthreads();
 
     Q4<real> stk[D];
@@ -185,17 +209,17 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     ProgOp raw1 = ops[nops > 1 ? 1 : 0];
     stage<real>(ops[0], coef, tips, lane, cur);
     for (int k = 0; k < nops; ++k) {
-      // two-stage pipeline: raw op k+2 and the operands of op k+1 are read
-      // from LDS while op k computes
+      // pipeline: raw op k+2 and the operands of op k+1 are read from LDS
+      // while op k computes
       const ProgOp raw2 = ops[k + 2 < nops ? k + 2 : nops - 1];
       if (k + 1 < nops) stage<real>(raw1, coef, tips, lane, nxt);
       Q4<real> va, vb;
       if (cur.type == OP_TT) {
-        va = tip_vec<real>(cur.ca, cur.ta, pi);
-        vb = tip_vec<real>(cur.cb, cur.tb, pi);
+        va = colP[cur.a * 4 + cur.ta];
+        vb = colP[cur.b * 4 + cur.tb];
       } else if (cur.type == OP_ST) {
+        vb = colP[cur.b * 4 + cur.tb];
         va = int_vec<real>(cur.ca, stk[0], pi);
-        vb = tip_vec<real>(cur.cb, cur.tb, pi);
       } else {
         va = int_vec<real>(cur.ca, stk[1], pi);
         vb = int_vec<real>(cur.cb, stk[0], pi);
@@ -231,12 +255,6 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     out[ii] = acc;
     if (status) status[ii] = IMA_STATUS_OK;
   }
-}
-
-size_t hky_lds_bytes(int precision, int max_n) {
-  const size_t ne = 2 * (size_t)max_n - 1;
-  const size_t cs = precision == IMA_PREC_FP32 ? sizeof(HkyCoef<float>) : sizeof(HkyCoef<double>);
-  return ne * cs + ((size_t)max_n - 1) * sizeof(ProgOp) + (size_t)max_n * kWave;
 }
 
 template <typename real>
