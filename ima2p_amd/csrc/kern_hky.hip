@@ -8,10 +8,11 @@
 // The host turns each genealogy into a postorder stack program (ProgOp):
 // the daughter with the larger need is evaluated first, so the stack never
 // holds more than floor(log2 n) partials (Strahler number), at most 8 for
-// n <= 256.  The stack is a register array of D quads addressed only at
-// compile-time positions (push shifts down, pop shifts up), selected per
-// operation by a wave-uniform op type, so a node's partial never leaves the
-// register file and no private (scratch) memory is used.  Nothing per node
+// n <= 256.  The stack is a register array of D quads.  Each operation is
+// dispatched on the current stack depth (wave-uniform) into code specialised
+// for that depth, so every stack access is a fixed register: a node's
+// partial never leaves the register file, nothing is shifted and no private
+// (scratch) memory is used.  Nothing per node
 // goes to HBM: every proposal is re-scored over the whole genealogy,
 // ~(36n-58)S flops on chip, instead of reading and writing ~36 B per
 // node-pattern of stored partials.
@@ -98,6 +99,66 @@ __device__ __forceinline__ void stage(const ProgOp &raw, const HkyCoef<real> *co
   s.cb = coef[s.b];
   s.ta = (s.type == OP_TT) ? tips[s.a * kWave + lane] : 0;
   s.tb = (s.type != OP_SS) ? tips[s.b * kWave + lane] : 0;
+}
+
+
+// Combine, rescale by a power of two and store one node at stack depth d.
+template <typename real>
+__device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &vb, int &E) {
+  Q4<real> L;
+  L.x = va.x * vb.x;
+  L.y = va.y * vb.y;
+  L.z = va.z * vb.z;
+  L.w = va.w * vb.w;
+  const real mx = fmax(fmax(L.x, L.y), fmax(L.z, L.w));
+  const int ex = frexp_e(mx);
+  E += ex;
+  L.x = ldexp_r(L.x, -ex);
+  L.y = ldexp_r(L.y, -ex);
+  L.z = ldexp_r(L.z, -ex);
+  L.w = ldexp_r(L.w, -ex);
+  return L;
+}
+
+// One program operation with the stack holding `d` partials (compile-time),
+// returns the new depth.  TT pushes at d, ST rewrites d-1, SS pops into d-2.
+template <typename real, int D, int d>
+__device__ __forceinline__ int do_op(const Staged<real> &c, Q4<real> (&stk)[D],
+                                     const Q4<real> *colP, const real pi[4], int &E) {
+  if (c.type == OP_TT) {
+    if constexpr (d < D) {
+      stk[d] = combine<real>(colP[c.a * 4 + c.ta], colP[c.b * 4 + c.tb], E);
+    }
+    return d + 1;
+  } else if (c.type == OP_ST) {
+    if constexpr (d >= 1) {
+      const Q4<real> vb = colP[c.b * 4 + c.tb];
+      stk[d - 1] = combine<real>(int_vec<real>(c.ca, stk[d - 1], pi), vb, E);
+    }
+    return d;
+  } else {
+    if constexpr (d >= 2) {
+      stk[d - 2] = combine<real>(int_vec<real>(c.ca, stk[d - 2], pi),
+                                 int_vec<real>(c.cb, stk[d - 1], pi), E);
+    }
+    return d - 1;
+  }
+}
+
+template <typename real, int D>
+__device__ __forceinline__ int apply(const Staged<real> &c, Q4<real> (&stk)[D], int d,
+                                     const Q4<real> *colP, const real pi[4], int &E) {
+  switch (d) {
+    case 0: return do_op<real, D, 0>(c, stk, colP, pi, E);
+    case 1: return do_op<real, D, 1>(c, stk, colP, pi, E);
+    case 2: return do_op<real, D, 2>(c, stk, colP, pi, E);
+    case 3: return do_op<real, D, 3>(c, stk, colP, pi, E);
+    case 4: return do_op<real, D, 4>(c, stk, colP, pi, E);
+    case 5: if constexpr (D >= 5) return do_op<real, D, 5>(c, stk, colP, pi, E); else return d;
+    case 6: if constexpr (D >= 6) return do_op<real, D, 6>(c, stk, colP, pi, E); else return d;
+    case 7: if constexpr (D >= 7) return do_op<real, D, 7>(c, stk, colP, pi, E); else return d;
+    default: if constexpr (D >= 8) return do_op<real, D, 8>(c, stk, colP, pi, E); else return d;
+  }
 }
 
 }  // namespace
@@ -204,48 +265,22 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     Q4<real> stk[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) stk[i].x = stk[i].y = stk[i].z = stk[i].w = real(0);
-    int E = 0;
-    Staged<real> cur, nxt;
+    int E = 0, d = 0;
+    // pipeline, unrolled by two with ping-pong staging: while one operation
+    // computes, the operands of the next are read from LDS, and the raw
+    // program entry after that is already in registers
+    Staged<real> A, B;
     ProgOp raw1 = ops[nops > 1 ? 1 : 0];
-    stage<real>(ops[0], coef, tips, lane, cur);
-    for (int k = 0; k < nops; ++k) {
-      // pipeline: raw op k+2 and the operands of op k+1 are read from LDS
-      // while op k computes
+    stage<real>(ops[0], coef, tips, lane, A);
+    for (int k = 0; k < nops; k += 2) {
       const ProgOp raw2 = ops[k + 2 < nops ? k + 2 : nops - 1];
-      if (k + 1 < nops) stage<real>(raw1, coef, tips, lane, nxt);
-      Q4<real> va, vb;
-      if (cur.type == OP_TT) {
-        va = colP[cur.a * 4 + cur.ta];
-        vb = colP[cur.b * 4 + cur.tb];
-      } else if (cur.type == OP_ST) {
-        vb = colP[cur.b * 4 + cur.tb];
-        va = int_vec<real>(cur.ca, stk[0], pi);
-      } else {
-        va = int_vec<real>(cur.ca, stk[1], pi);
-        vb = int_vec<real>(cur.cb, stk[0], pi);
-      }
-      Q4<real> L;
-      L.x = va.x * vb.x;
-      L.y = va.y * vb.y;
-      L.z = va.z * vb.z;
-      L.w = va.w * vb.w;
-      const real mx = fmax(fmax(L.x, L.y), fmax(L.z, L.w));
-      const int ex = frexp_e(mx);
-      E += ex;
-      L.x = ldexp_r(L.x, -ex);
-      L.y = ldexp_r(L.y, -ex);
-      L.z = ldexp_r(L.z, -ex);
-      L.w = ldexp_r(L.w, -ex);
-      if (cur.type == OP_TT) {  // push
-#pragma unroll
-        for (int i = D - 1; i > 0; --i) stk[i] = stk[i - 1];
-      } else if (cur.type == OP_SS) {  // pop one
-#pragma unroll
-        for (int i = 1; i < D - 1; ++i) stk[i] = stk[i + 1];
-      }
-      stk[0] = L;
-      cur = nxt;
-      raw1 = raw2;
+      if (k + 1 < nops) stage<real>(raw1, coef, tips, lane, B);
+      d = apply<real, D>(A, stk, d, colP, pi, E);
+      if (k + 1 >= nops) break;
+      const ProgOp raw3 = ops[k + 3 < nops ? k + 3 : nops - 1];
+      if (k + 2 < nops) stage<real>(raw2, coef, tips, lane, A);
+      d = apply<real, D>(B, stk, d, colP, pi, E);
+      raw1 = raw3;
     }
     const real fr = pi[0] * stk[0].x + pi[1] * stk[0].y + pi[2] * stk[0].z + pi[3] * stk[0].w;
     if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
