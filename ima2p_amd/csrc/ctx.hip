@@ -59,6 +59,8 @@ struct ima_ctx {
   DevBuf<int32_t> d_status;
   DevBuf<int32_t> d_aux;
   DevBuf<double> d_aux64;
+  DevBuf<int4> d_units, d_red;
+  DevBuf<double> d_partial;
 
   // L mode
   std::vector<LTableHost> tables;
@@ -84,6 +86,9 @@ struct ima_ctx {
     DevBuf<uint64_t> counter;
     DevBuf<unsigned long long> stats;
     DevBuf<ima_item> items;
+    DevBuf<int4> units, red;
+    DevBuf<double> partial;
+    HkyPlan plan;
     hipGraphExec_t exec[2]{nullptr, nullptr};
     hipGraph_t graph[2]{nullptr, nullptr};
     hipEvent_t ev[4]{};
@@ -327,6 +332,9 @@ void ima_ctx_destroy(ima_ctx *c) {
   release(c->d_status);
   release(c->d_aux);
   release(c->d_aux64);
+  release(c->d_units);
+  release(c->d_red);
+  release(c->d_partial);
   release(c->d_lx);
   release(c->d_lwork);
   release(c->d_lout);
@@ -600,10 +608,29 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
                            c->stream));
   const DeviceView v = view(c);
   const int kh = precision == IMA_PREC_FP32 ? K_HKY32 : K_HKY64;
+  HkyPlan plan;
+  if (!hky.empty()) {
+    std::vector<int4> units, red;
+    int nblk = 0;
+    for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[items[i].slot].locus].S, &nblk);
+    HIPCHK(c, reserve(c->d_units, units.size()));
+    HIPCHK(c, reserve(c->d_red, red.size() + 1));
+    HIPCHK(c, reserve(c->d_partial, (size_t)nblk + 1));
+    HIPCHK(c, hipMemcpyAsync(c->d_units.p, units.data(), units.size() * sizeof(int4),
+                             hipMemcpyHostToDevice, c->stream));
+    if (!red.empty())
+      HIPCHK(c, hipMemcpyAsync(c->d_red.p, red.data(), red.size() * sizeof(int4),
+                               hipMemcpyHostToDevice, c->stream));
+    plan.units = c->d_units.p;
+    plan.nunits = (int)units.size();
+    plan.red = c->d_red.p;
+    plan.nred = (int)red.size();
+    plan.partial = c->d_partial.p;
+  }
   if (!hky.empty()) {
     if ((rc = timing_begin(c, kh)) != IMA_OK) return rc;
-    HIPCHK(c, launch_hky(v, c->d_items.p, c->d_idx.p, (int)hky.size(), precision, hmax_n,
-                         hmax_slots, c->d_out.p, c->d_status.p, c->stream));
+    HIPCHK(c, launch_hky(v, c->d_items.p, plan, precision, hmax_n, hmax_slots, c->d_out.p,
+                         c->d_status.p, c->stream));
     if ((rc = timing_end(c, kh)) != IMA_OK) return rc;
   }
   if (!idx_i.empty()) {
@@ -929,6 +956,9 @@ static void mc3_release(ima_ctx *c) {
   release(m.counter);
   release(m.stats);
   release(m.items);
+  release(m.units);
+  release(m.red);
+  release(m.partial);
   m.live = false;
 }
 
@@ -1022,6 +1052,25 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   HIPCHK(c, hipMemcpyAsync(m.items.p, items.data(), N * sizeof(ima_item), hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(m.idx.p, idx.data(), N * 4, hipMemcpyHostToDevice, c->stream));
+  {
+    std::vector<int4> units, red;
+    int nblk = 0;
+    for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[first_slot + i].locus].S, &nblk);
+    HIPCHK(c, reserve(m.units, units.size() + 1));
+    HIPCHK(c, reserve(m.red, red.size() + 1));
+    HIPCHK(c, reserve(m.partial, (size_t)nblk + 1));
+    if (!units.empty())
+      HIPCHK(c, hipMemcpyAsync(m.units.p, units.data(), units.size() * sizeof(int4),
+                               hipMemcpyHostToDevice, c->stream));
+    if (!red.empty())
+      HIPCHK(c, hipMemcpyAsync(m.red.p, red.data(), red.size() * sizeof(int4),
+                               hipMemcpyHostToDevice, c->stream));
+    m.plan.units = m.units.p;
+    m.plan.nunits = (int)units.size();
+    m.plan.red = m.red.p;
+    m.plan.nred = (int)red.size();
+    m.plan.partial = m.partial.p;
+  }
   HIPCHK(c, hipMemcpyAsync(m.beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
   const uint64_t zero = 0;
   const unsigned long long zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1056,8 +1105,8 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   // initial likelihood of every genealogy
   const DeviceView v = view(c);
   if (m.nh)
-    HIPCHK(c, launch_hky(v, m.items.p, m.idx.p, m.nh, precision, m.hmax_n, m.hmax_slots,
-                         m.pdg.p, m.status.p, c->stream));
+    HIPCHK(c, launch_hky(v, m.items.p, m.plan, precision, m.hmax_n, m.hmax_slots, m.pdg.p,
+                         m.status.p, c->stream));
   if (m.ni)
     HIPCHK(c, launch_is(v, m.items.p, m.idx.p + m.nh, m.ni, m.imax_n, m.imax_w, m.pdg.p,
                         m.status.p, nullptr, nullptr, c->stream));
@@ -1071,7 +1120,8 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
 
 static int mc3_enqueue(ima_ctx *c, bool with_scale, hipEvent_t *score_ev) {
   auto &m = c->mc3;
-  HIPCHK(c, enqueue_mc3_sweep(view(c), m.dev, with_scale, m.items.p, m.idx.p, m.nh, m.ni, m.ns,
+  HIPCHK(c, enqueue_mc3_sweep(view(c), m.dev, with_scale, m.items.p, m.idx.p, m.plan, m.nh,
+                              m.ni, m.ns,
                               m.precision, m.hmax_n, m.hmax_slots, m.imax_n, m.imax_w,
                               m.swaptries, score_ev, c->stream));
   return IMA_OK;

@@ -84,12 +84,26 @@ struct DeviceView {
   const uint64_t *mask;
 };
 
+// HKY work plan: one wavefront per (item, 64-pattern block).  units[k] =
+// {item, block, first partial, blocks of the item}; items with more than
+// one block are merged from `partial` in block order by a second kernel
+// (red[k] = {item, first partial, blocks, 0}).
+struct HkyPlan {
+  const int4 *units = nullptr;
+  int nunits = 0;
+  const int4 *red = nullptr;
+  int nred = 0;
+  double *partial = nullptr;
+};
+// Host side: append the units of HKY item ii with S patterns.
+void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int S,
+                  int *nblocks_total);
+
 // Kernel launchers (kern_hky.hip, kern_lik.hip).
 size_t hky_lds_bytes(int precision, int max_n);
-hipError_t launch_hky(const DeviceView &v, const ima_item *items,
-                      const int32_t *item_idx, int nitems, int precision,
-                      int max_n, int max_slots, double *out, int32_t *status,
-                      hipStream_t st);
+hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
+                      int precision, int max_n, int max_slots, double *out,
+                      int32_t *status, hipStream_t st);
 hipError_t launch_is(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, int max_n, int max_w,
                      double *out, int32_t *status, int32_t *site_node,
@@ -139,8 +153,8 @@ struct MC3Dev {
 };
 
 hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_scale,
-                             const ima_item *items, const int32_t *idx, int nh, int ni,
-                             int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
+                             const ima_item *items, const int32_t *idx, const HkyPlan &plan,
+                             int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
                              int imax_w, int swaptries, hipEvent_t *score_ev,
                              hipStream_t st);
 

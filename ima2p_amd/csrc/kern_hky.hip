@@ -1,9 +1,11 @@
 // kern_hky.hip -- batched HKY Felsenstein pruning (calc_prob_data.cpp:118-607).
 //
-// One 64-lane wavefront per scoring item (a genealogy of one chain at one
-// locus), one lane per site pattern, the site patterns of a locus walked in
-// blocks of 64.  Items of all chains and loci, with uneven gene counts and
-// pattern counts, share one launch.
+// One 64-lane wavefront per work unit = (scoring item, block of 64 site
+// patterns), one lane per pattern.  A scoring item is a genealogy of one
+// chain at one locus; items of all chains and loci, with uneven gene counts
+// and pattern counts, share one launch, and every wavefront does the same
+// amount of work.  Items spanning several blocks are summed in block order
+// by a second kernel, so results do not depend on scheduling.
 //
 // The host turns each genealogy into a postorder stack program (ProgOp):
 // the daughter with the larger need is evaluated first, so the stack never
@@ -170,14 +172,23 @@ size_t hky_lds_bytes(int precision, int max_n) {
          (size_t)max_n * kWave;
 }
 
+void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int S,
+                  int *nblocks_total) {
+  const int nb = (S + kWave - 1) / kWave;
+  const int pbase = *nblocks_total;
+  for (int b = 0; b < nb; ++b) units.push_back(make_int4(ii, b, pbase, nb));
+  if (nb > 1) red.push_back(make_int4(ii, pbase, nb, 0));
+  *nblocks_total += nb;
+}
+
 template <typename real, int D>
 __global__ __launch_bounds__(kWave) void hky_kernel(
-    DeviceView v, const ima_item *__restrict__ items,
-    const int32_t *__restrict__ idx, double *__restrict__ out,
-    int32_t *__restrict__ status) {
+    DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
+    double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
-  const int ii = idx[blockIdx.x];
+  const int4 unit = units[blockIdx.x];
+  const int ii = unit.x, blk = unit.y;
   const ima_item item = items[ii];
   const SlotMeta sm = v.slots[item.slot];
   const LocusMeta lm = v.loci[sm.locus];
@@ -240,8 +251,7 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   const int words = n * (kWave / 4);  // dwords of one block: n rows x 16
 
   double acc = 0.0;
-  const int nblocks = (S + kWave - 1) / kWave;
-  for (int blk = 0; blk < nblocks; ++blk) {
+  {
     const int p = blk * kWave + lane;
     const bool valid = p < S;
     __syncthreads();
@@ -287,33 +297,55 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   }
   acc = wave_sum_d(acc);
   if (lane == 0) {
-    out[ii] = acc;
-    if (status) status[ii] = IMA_STATUS_OK;
+    if (unit.w == 1) {
+      out[ii] = acc;
+      if (status) status[ii] = IMA_STATUS_OK;
+    } else {
+      partial[unit.z + blk] = acc;
+    }
   }
+}
+
+__global__ void hky_reduce(const int4 *__restrict__ red, int nred,
+                           const double *__restrict__ partial, double *__restrict__ out,
+                           int32_t *__restrict__ status) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nred) return;
+  const int4 r = red[k];
+  double s = 0.0;
+  for (int b = 0; b < r.z; ++b) s += partial[r.y + b];
+  out[r.x] = s;
+  if (status) status[r.x] = IMA_STATUS_OK;
 }
 
 template <typename real>
 static void launch_depth(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
-                         const ima_item *items, const int32_t *idx, double *out,
+                         const ima_item *items, const HkyPlan &p, double *out,
                          int32_t *status) {
   if (D <= 4)
-    hipLaunchKernelGGL((hky_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+    hipLaunchKernelGGL((hky_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status);
   else if (D <= 6)
-    hipLaunchKernelGGL((hky_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+    hipLaunchKernelGGL((hky_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status);
   else
-    hipLaunchKernelGGL((hky_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, idx, out, status);
+    hipLaunchKernelGGL((hky_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status);
 }
 
-hipError_t launch_hky(const DeviceView &v, const ima_item *items, const int32_t *item_idx,
-                      int nitems, int precision, int max_n, int max_slots, double *out,
-                      int32_t *status, hipStream_t st) {
-  if (nitems == 0) return hipSuccess;
+hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
+                      int precision, int max_n, int max_slots, double *out, int32_t *status,
+                      hipStream_t st) {
+  if (plan.nunits == 0) return hipSuccess;
   if (max_slots > 8) return hipErrorInvalidValue;
   const size_t lds = hky_lds_bytes(precision, max_n);
   if (precision == IMA_PREC_FP32)
-    launch_depth<float>(max_slots, dim3(nitems), lds, st, v, items, item_idx, out, status);
+    launch_depth<float>(max_slots, dim3(plan.nunits), lds, st, v, items, plan, out, status);
   else
-    launch_depth<double>(max_slots, dim3(nitems), lds, st, v, items, item_idx, out, status);
+    launch_depth<double>(max_slots, dim3(plan.nunits), lds, st, v, items, plan, out, status);
+  if (plan.nred > 0)
+    hipLaunchKernelGGL(hky_reduce, dim3((plan.nred + 255) / 256), dim3(256), 0, st, plan.red,
+                       plan.nred, plan.partial, out, status);
   return hipGetLastError();
 }
 

@@ -218,8 +218,8 @@ __global__ __launch_bounds__(256) void mc3_swap(MC3Dev m, int swaptries) {
 
 // Enqueue the kernels of one sweep (no host synchronisation).
 hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_scale,
-                             const ima_item *items, const int32_t *idx, int nh, int ni,
-                             int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
+                             const ima_item *items, const int32_t *idx, const HkyPlan &plan,
+                             int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
                              int imax_w, int swaptries, hipEvent_t *score_ev,
                              hipStream_t st) {
   const int N = m.C * m.L;
@@ -230,7 +230,7 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
     hipError_t e = hipSuccess;
     if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);
     if (e == hipSuccess && nh)
-      e = launch_hky(v, items, idx, nh, precision, hmax_n, hmax_slots, m.newpdg, m.status, st);
+      e = launch_hky(v, items, plan, precision, hmax_n, hmax_slots, m.newpdg, m.status, st);
     if (e == hipSuccess && ni)
       e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr, nullptr, st);
     if (e == hipSuccess && ns)
