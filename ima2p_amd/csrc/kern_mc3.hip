@@ -18,9 +18,10 @@
 // Neither is the reference's proposal (slider/addmigration); they exercise
 // the same scoring, acceptance, commit/rollback and swap path.
 //
-// Work mapping: per-item kernels use one thread per (chain, locus);
-// per-node kernels one thread per genealogy node (node_item maps a node to
-// its item); per-chain reductions one wavefront per chain.
+// Work mapping: the proposal uses one thread per (chain, locus); accept,
+// rescale, commit and the chain sums use one 1024-thread workgroup per
+// chain (a chain's genealogies are contiguous in the node arrays) with
+// fixed-order reductions, so every result is independent of scheduling.
 #include <float.h>
 
 #include "ima_internal.h"
@@ -86,102 +87,132 @@ __global__ void mc3_slide_propose(DeviceView v, MC3Dev m) {
   tm[b] = Tn;
 }
 
-__global__ void mc3_slide_accept(DeviceView v, MC3Dev m) {
-  const int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= m.C * m.L) return;
+// Deterministic block reduction (fixed order) of one double per thread.
+__device__ inline double block_sum(double x, double *sh) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  x = wsum(x);
+  __syncthreads();
+  if (lane == 0) sh[w] = x;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
+__device__ inline void chain_nodes(const DeviceView &v, const MC3Dev &m, int c, int64_t *j0,
+                                   int64_t *j1) {
+  const SlotMeta s0 = v.slots[m.first_slot + c * m.L];
+  const SlotMeta s1 = v.slots[m.first_slot + c * m.L + m.L - 1];
+  *j0 = s0.node_off - m.node_first;
+  *j1 = s1.node_off + 2 * (int64_t)s1.n - 1 - m.node_first;
+}
+
+__device__ inline double chain_log_scale(const MC3Dev &m, int c, uint64_t sweep) {
+  PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 2);
+  return m.scale_width * (r.uniform() - 0.5);
+}
+
+// One workgroup per chain.  Accept or reject the genealogy proposal of every
+// locus (update_gtree.cpp:919-927; rejection restores the moved time).  On a
+// rescaling sweep then back up and rescale every coalescence time of the
+// chain; otherwise sum the chain's log-likelihoods for the swap.
+__global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int with_scale) {
+  __shared__ double sh[1024 / kWave];
+  const int c = blockIdx.x;
   const uint64_t sweep = *m.counter;
-  const int c = it / m.L;
-  PhiloxStream r = item_rng(m, it, sweep, 1);
-  const double U = r.uniform();
-  bool acc = false;
-  if (m.status[it] == IMA_STATUS_OK) {
-    const double mh = exp(m.beta[c] * (m.newpdg[it] - m.pdg[it]));
-    acc = U < fmin(1.0, mh);  // update_gtree.cpp:919-927
-  }
-  if (acc) {
-    m.pdg[it] = m.newpdg[it];
-  } else {
-    const SlotMeta sm = v.slots[m.first_slot + it];
-    const int e = m.rec_v[it];
-    double *tm = const_cast<double *>(v.time) + sm.node_off;
-    const int32_t *up0 = v.up0 + sm.node_off, *up1 = v.up1 + sm.node_off;
-    tm[up0[e]] = m.rec_t[it];
-    tm[up1[e]] = m.rec_t[it];
-  }
-  m.acc[it] += acc ? 1u : 0u;
-}
-
-// One log scale factor per chain.
-__global__ void mc3_scale_draw(MC3Dev m) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.C) return;
-  PhiloxStream r = item_rng(m, m.C * m.L + c, *m.counter, 2);
-  m.logs[c] = m.scale_width * (r.uniform() - 0.5);
-}
-
-// Node-parallel rescaling of every coalescence time (root edge excluded).
-__global__ void mc3_scale_apply(DeviceView v, MC3Dev m) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m.nnodes) return;
-  const int64_t g = m.node_first + j;
-  const double t = v.time[g];
-  m.tbackup[j] = t;
-  if (v.down[g] != -1) {
-    const int c = m.node_item[j] / m.L;
-    const_cast<double *>(v.time)[g] = t * exp(m.logs[c]);
-  }
-}
-
-// One wavefront per chain: log acceptance = sum_l beta (new - old) +
-// (n_l - 1) log s.
-__global__ __launch_bounds__(kWave) void mc3_scale_decide(DeviceView v, MC3Dev m) {
-  const int c = blockIdx.x, lane = threadIdx.x;
-  double a = 0.0;
-  int ok = 1;
-  for (int l = lane; l < m.L; l += kWave) {
+  double part = 0.0;
+  for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
     const int it = c * m.L + l;
-    ok &= (m.status[it] == IMA_STATUS_OK);
+    PhiloxStream r = item_rng(m, it, sweep, 1);
+    const double U = r.uniform();
+    bool acc = false;
+    if (m.status[it] == IMA_STATUS_OK) {
+      const double mh = exp(m.beta[c] * (m.newpdg[it] - m.pdg[it]));
+      acc = U < fmin(1.0, mh);
+    }
+    double p = m.pdg[it];
+    if (acc) {
+      p = m.newpdg[it];
+      m.pdg[it] = p;
+    } else {
+      const SlotMeta sm = v.slots[m.first_slot + it];
+      const int e = m.rec_v[it];
+      double *tm = const_cast<double *>(v.time) + sm.node_off;
+      const int32_t *up0 = v.up0 + sm.node_off, *up1 = v.up1 + sm.node_off;
+      tm[up0[e]] = m.rec_t[it];
+      tm[up1[e]] = m.rec_t[it];
+    }
+    m.acc[it] += acc ? 1u : 0u;
+    part += p;
+  }
+  if (!with_scale) {
+    const double S = block_sum(part, sh);
+    if (threadIdx.x == 0) m.S[c] = S;
+    return;
+  }
+  __syncthreads();  // restored times visible to the whole workgroup
+  const double ls = chain_log_scale(m, c, sweep);
+  const double s = exp(ls);
+  if (threadIdx.x == 0) m.logs[c] = ls;
+  int64_t j0, j1;
+  chain_nodes(v, m, c, &j0, &j1);
+  double *tm = const_cast<double *>(v.time) + m.node_first;
+  const int32_t *dn = v.down + m.node_first;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    const double t = tm[j];
+    m.tbackup[j] = t;
+    if (dn[j] != -1) tm[j] = t * s;
+  }
+}
+
+// One workgroup per chain: log acceptance of the rescaling
+// = sum_l beta (new - old) + (n_l - 1) log s; commit or restore; chain sum.
+__global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m) {
+  __shared__ double sh[1024 / kWave];
+  __shared__ int okall, decision;
+  const int c = blockIdx.x;
+  if (threadIdx.x == 0) okall = 1;
+  __syncthreads();
+  double a = 0.0;
+  for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+    const int it = c * m.L + l;
+    if (m.status[it] != IMA_STATUS_OK) okall = 0;
     const int n = v.slots[m.first_slot + it].n;
     a += m.beta[c] * (m.newpdg[it] - m.pdg[it]) + (n - 1) * m.logs[c];
   }
-  a = wsum(a);
-  ok = wand(ok);
-  if (lane == 0) {
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) {
     PhiloxStream r = item_rng(m, m.C * m.L + m.C + c, *m.counter, 3);
     const double U = r.uniform();
-    const int acc = ok && (U < fmin(1.0, exp(a)));
-    m.decision[c] = acc;
-    m.cacc[c] += (uint32_t)acc;
-  }
-}
-
-__global__ void mc3_scale_commit_items(MC3Dev m) {
-  const int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= m.C * m.L) return;
-  if (m.decision[it / m.L]) m.pdg[it] = m.newpdg[it];
-}
-
-__global__ void mc3_scale_commit_nodes(DeviceView v, MC3Dev m) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m.nnodes) return;
-  if (!m.decision[m.node_item[j] / m.L])
-    const_cast<double *>(v.time)[m.node_first + j] = m.tbackup[j];
-}
-
-// Chain sums S_c = sum_l pdg (swapweight, swapchains.cpp:13-34) with one
-// wavefront per chain, then the swap attempts of swapchains on one lane,
-// then advance the sweep counter.
-__global__ __launch_bounds__(256) void mc3_swap(MC3Dev m, int swaptries) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int c = w; c < m.C; c += blockDim.x / kWave) {
-    double s = 0.0;
-    for (int l = lane; l < m.L; l += kWave) s += m.pdg[c * m.L + l];
-    s = wsum(s);
-    if (lane == 0) m.S[c] = s;
+    decision = okall && (U < fmin(1.0, exp(a)));
+    m.decision[c] = decision;
+    m.cacc[c] += (uint32_t)decision;
   }
   __syncthreads();
-  __threadfence_block();
-  if (threadIdx.x != 0) return;
+  double part = 0.0;
+  if (decision) {
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      const int it = c * m.L + l;
+      m.pdg[it] = m.newpdg[it];
+      part += m.newpdg[it];
+    }
+  } else {
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) part += m.pdg[c * m.L + l];
+    int64_t j0, j1;
+    chain_nodes(v, m, c, &j0, &j1);
+    double *tm = const_cast<double *>(v.time) + m.node_first;
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) tm[j] = m.tbackup[j];
+  }
+  const double S = block_sum(part, sh);
+  if (threadIdx.x == 0) m.S[c] = S;
+}
+
+// Swap attempts of swapchains on one lane over the chain sums
+// S_c = sum_l pdg (swapweight, swapchains.cpp:13-34), then advance the
+// sweep counter.
+__global__ void mc3_swap(MC3Dev m, int swaptries) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t sweep = *m.counter;
   PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, sweep);
   const int SWAPDIST = 7, nchains = m.C;
@@ -224,7 +255,6 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              hipStream_t st) {
   const int N = m.C * m.L;
   const int tb = 256, gb = (N + tb - 1) / tb;
-  const int gn = (int)((m.nnodes + tb - 1) / tb);
   // score_ev (optional): events recorded around the two scoring passes
   auto score = [&](int pass) -> hipError_t {
     hipError_t e = hipSuccess;
@@ -241,17 +271,13 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   hipLaunchKernelGGL(mc3_slide_propose, dim3(gb), dim3(tb), 0, st, v, m);
   hipError_t e = score(0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(mc3_slide_accept, dim3(gb), dim3(tb), 0, st, v, m);
+  hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), 0, st, v, m, with_scale ? 1 : 0);
   if (with_scale) {
-    hipLaunchKernelGGL(mc3_scale_draw, dim3((m.C + 63) / 64), dim3(64), 0, st, m);
-    hipLaunchKernelGGL(mc3_scale_apply, dim3(gn), dim3(tb), 0, st, v, m);
     e = score(1);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mc3_scale_decide, dim3(m.C), dim3(kWave), 0, st, v, m);
-    hipLaunchKernelGGL(mc3_scale_commit_items, dim3(gb), dim3(tb), 0, st, m);
-    hipLaunchKernelGGL(mc3_scale_commit_nodes, dim3(gn), dim3(tb), 0, st, v, m);
+    hipLaunchKernelGGL(mc3_scale_finish, dim3(m.C), dim3(1024), 0, st, v, m);
   }
-  hipLaunchKernelGGL(mc3_swap, dim3(1), dim3(256), 0, st, m, swaptries);
+  hipLaunchKernelGGL(mc3_swap, dim3(1), dim3(64), 0, st, m, swaptries);
   return hipGetLastError();
 }
 
