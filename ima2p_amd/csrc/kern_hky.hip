@@ -83,26 +83,39 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Decoded operation with its operands staged for the next compute step.
-template <typename real>
-struct Staged {
+// Three-stage operand pipeline.  Mid: a decoded program entry whose tip
+// states are being read from LDS.  Nxt: an entry whose branch coefficients
+// and tip columns of P are being read.  The compute step of an entry reads
+// nothing from LDS.
+struct Mid {
   int type, a, b;
-  HkyCoef<real> ca, cb;
   int ta, tb;  // tip states (when the operand is a tip)
 };
 
 template <typename real>
-__device__ __forceinline__ void stage(const ProgOp &raw, const HkyCoef<real> *coef,
-                                      const int8_t *tips, int lane, Staged<real> &s) {
-  s.a = uni(raw.a);
-  s.b = uni(raw.b);
-  s.type = uni(raw.type);
-  s.ca = coef[s.a];
-  s.cb = coef[s.b];
-  s.ta = (s.type == OP_TT) ? tips[s.a * kWave + lane] : 0;
-  s.tb = (s.type != OP_SS) ? tips[s.b * kWave + lane] : 0;
+struct Nxt {
+  int type;
+  HkyCoef<real> ca, cb;
+  Q4<real> pa, pb;  // columns of P for tip operands
+};
+
+__device__ __forceinline__ void to_mid(const ProgOp &raw, const int8_t *tips, int lane, Mid &m) {
+  m.a = uni(raw.a);
+  m.b = uni(raw.b);
+  m.type = uni(raw.type);
+  m.ta = (m.type == OP_TT) ? tips[m.a * kWave + lane] : 0;
+  m.tb = (m.type != OP_SS) ? tips[m.b * kWave + lane] : 0;
 }
 
+template <typename real>
+__device__ __forceinline__ void to_nxt(const Mid &m, const HkyCoef<real> *coef,
+                                       const Q4<real> *colP, Nxt<real> &x) {
+  x.type = m.type;
+  if (m.type != OP_TT) x.ca = coef[m.a];
+  if (m.type == OP_SS) x.cb = coef[m.b];
+  if (m.type == OP_TT) x.pa = colP[m.a * 4 + m.ta];
+  if (m.type != OP_SS) x.pb = colP[m.b * 4 + m.tb];
+}
 
 // Combine, rescale by a power of two and store one node at stack depth d.
 template <typename real>
@@ -125,41 +138,35 @@ __device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &
 // One program operation with the stack holding `d` partials (compile-time),
 // returns the new depth.  TT pushes at d, ST rewrites d-1, SS pops into d-2.
 template <typename real, int D, int d>
-__device__ __forceinline__ int do_op(const Staged<real> &c, Q4<real> (&stk)[D],
-                                     const Q4<real> *colP, const real pi[4], int &E) {
+__device__ __forceinline__ int do_op(const Nxt<real> &c, Q4<real> (&stk)[D], const real pi[4],
+                                     int &E) {
   if (c.type == OP_TT) {
-    if constexpr (d < D) {
-      stk[d] = combine<real>(colP[c.a * 4 + c.ta], colP[c.b * 4 + c.tb], E);
-    }
+    if constexpr (d < D) stk[d] = combine<real>(c.pa, c.pb, E);
     return d + 1;
   } else if (c.type == OP_ST) {
-    if constexpr (d >= 1) {
-      const Q4<real> vb = colP[c.b * 4 + c.tb];
-      stk[d - 1] = combine<real>(int_vec<real>(c.ca, stk[d - 1], pi), vb, E);
-    }
+    if constexpr (d >= 1) stk[d - 1] = combine<real>(int_vec<real>(c.ca, stk[d - 1], pi), c.pb, E);
     return d;
   } else {
-    if constexpr (d >= 2) {
+    if constexpr (d >= 2)
       stk[d - 2] = combine<real>(int_vec<real>(c.ca, stk[d - 2], pi),
                                  int_vec<real>(c.cb, stk[d - 1], pi), E);
-    }
     return d - 1;
   }
 }
 
 template <typename real, int D>
-__device__ __forceinline__ int apply(const Staged<real> &c, Q4<real> (&stk)[D], int d,
-                                     const Q4<real> *colP, const real pi[4], int &E) {
+__device__ __forceinline__ int apply(const Nxt<real> &c, Q4<real> (&stk)[D], int d,
+                                     const real pi[4], int &E) {
   switch (d) {
-    case 0: return do_op<real, D, 0>(c, stk, colP, pi, E);
-    case 1: return do_op<real, D, 1>(c, stk, colP, pi, E);
-    case 2: return do_op<real, D, 2>(c, stk, colP, pi, E);
-    case 3: return do_op<real, D, 3>(c, stk, colP, pi, E);
-    case 4: return do_op<real, D, 4>(c, stk, colP, pi, E);
-    case 5: if constexpr (D >= 5) return do_op<real, D, 5>(c, stk, colP, pi, E); else return d;
-    case 6: if constexpr (D >= 6) return do_op<real, D, 6>(c, stk, colP, pi, E); else return d;
-    case 7: if constexpr (D >= 7) return do_op<real, D, 7>(c, stk, colP, pi, E); else return d;
-    default: if constexpr (D >= 8) return do_op<real, D, 8>(c, stk, colP, pi, E); else return d;
+    case 0: return do_op<real, D, 0>(c, stk, pi, E);
+    case 1: return do_op<real, D, 1>(c, stk, pi, E);
+    case 2: return do_op<real, D, 2>(c, stk, pi, E);
+    case 3: return do_op<real, D, 3>(c, stk, pi, E);
+    case 4: return do_op<real, D, 4>(c, stk, pi, E);
+    case 5: if constexpr (D >= 5) return do_op<real, D, 5>(c, stk, pi, E); else return d;
+    case 6: if constexpr (D >= 6) return do_op<real, D, 6>(c, stk, pi, E); else return d;
+    case 7: if constexpr (D >= 7) return do_op<real, D, 7>(c, stk, pi, E); else return d;
+    default: if constexpr (D >= 8) return do_op<real, D, 8>(c, stk, pi, E); else return d;
   }
 }
 
@@ -276,21 +283,28 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
 #pragma unroll
     for (int i = 0; i < D; ++i) stk[i].x = stk[i].y = stk[i].z = stk[i].w = real(0);
     int E = 0, d = 0;
-    // pipeline, unrolled by two with ping-pong staging: while one operation
-    // computes, the operands of the next are read from LDS, and the raw
-    // program entry after that is already in registers
-    Staged<real> A, B;
-    ProgOp raw1 = ops[nops > 1 ? 1 : 0];
-    stage<real>(ops[0], coef, tips, lane, A);
+    // op k computes while the coefficients and P columns of op k+1, the tip
+    // states of op k+2 and the program entry of op k+3 are read from LDS;
+    // unrolled by two so no staged operand is ever copied
+    const int last = nops - 1;
+    Mid M0, M1;
+    Nxt<real> N0, N1;
+    to_mid(ops[0], tips, lane, M0);
+    to_mid(ops[min(1, last)], tips, lane, M1);
+    to_nxt<real>(M0, coef, colP, N0);
+    ProgOp raw = ops[min(2, last)];
     for (int k = 0; k < nops; k += 2) {
-      const ProgOp raw2 = ops[k + 2 < nops ? k + 2 : nops - 1];
-      if (k + 1 < nops) stage<real>(raw1, coef, tips, lane, B);
-      d = apply<real, D>(A, stk, d, colP, pi, E);
+      // even: compute N0 (op k), N1 <- M1 (op k+1), M0 <- raw (op k+2)
+      to_nxt<real>(M1, coef, colP, N1);
+      to_mid(raw, tips, lane, M0);
+      raw = ops[min(k + 3, last)];
+      d = apply<real, D>(N0, stk, d, pi, E);
       if (k + 1 >= nops) break;
-      const ProgOp raw3 = ops[k + 3 < nops ? k + 3 : nops - 1];
-      if (k + 2 < nops) stage<real>(raw2, coef, tips, lane, A);
-      d = apply<real, D>(B, stk, d, colP, pi, E);
-      raw1 = raw3;
+      // odd: compute N1 (op k+1), N0 <- M0 (op k+2), M1 <- raw (op k+3)
+      to_nxt<real>(M0, coef, colP, N0);
+      to_mid(raw, tips, lane, M1);
+      raw = ops[min(k + 4, last)];
+      d = apply<real, D>(N1, stk, d, pi, E);
     }
     const real fr = pi[0] * stk[0].x + pi[1] * stk[0].y + pi[2] * stk[0].z + pi[3] * stk[0].w;
     if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);

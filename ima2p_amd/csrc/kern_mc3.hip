@@ -210,41 +210,57 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m)
 
 // Swap attempts of swapchains on one lane over the chain sums
 // S_c = sum_l pdg (swapweight, swapchains.cpp:13-34), then advance the
-// sweep counter.
-__global__ void mc3_swap(MC3Dev m, int swaptries) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const uint64_t sweep = *m.counter;
-  PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, sweep);
-  const int SWAPDIST = 7, nchains = m.C;
-  const double *S = m.S;
-  unsigned long long acc = 0;
-  for (int i = 0; i < swaptries && nchains > 1; ++i) {
-    int ci, cj, cjmin, cjrange;
-    do {
-      ci = (int)(rng.uniform() * nchains);
-    } while (ci < 0 || ci >= nchains);
-    if (nchains < 2 * SWAPDIST + 3) {
-      cjmin = 0;
-      cjrange = nchains;
-    } else {
-      cjmin = max(0, ci - SWAPDIST);
-      cjrange = min(nchains, ci + SWAPDIST) - cjmin;
-    }
-    do {
-      cj = cjmin + (int)(rng.uniform() * cjrange);
-    } while (cj == ci || cj < 0 || cj >= nchains);
-    const double wgt = exp((m.beta[ci] - m.beta[cj]) * (S[cj] - S[ci]));
-    if (wgt >= 1.0 || wgt > rng.uniform()) {
-      const double b = m.beta[ci];
-      m.beta[ci] = m.beta[cj];
-      m.beta[cj] = b;
-      ++acc;
+// sweep counter.  S and beta are staged in LDS first (up to kSwapLds
+// chains) so the serial attempts never wait on global memory.
+constexpr int kSwapLds = 2048;
+__global__ __launch_bounds__(256) void mc3_swap(MC3Dev m, int swaptries) {
+  __shared__ double Ssh[kSwapLds], Bsh[kSwapLds];
+  const bool staged = m.C <= kSwapLds;
+  if (staged) {
+    for (int c = threadIdx.x; c < m.C; c += blockDim.x) {
+      Ssh[c] = m.S[c];
+      Bsh[c] = m.beta[c];
     }
   }
-  m.stats[4] += (unsigned long long)(nchains > 1 ? swaptries : 0);
-  m.stats[5] += acc;
-  m.stats[0] += (unsigned long long)(m.C * m.L);
-  *m.counter = sweep + 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t sweep = *m.counter;
+    PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, sweep);
+    const int SWAPDIST = 7, nchains = m.C;
+    const double *S = staged ? Ssh : m.S;
+    double *beta = staged ? Bsh : m.beta;
+    unsigned long long acc = 0;
+    for (int i = 0; i < swaptries && nchains > 1; ++i) {
+      int ci, cj, cjmin, cjrange;
+      do {
+        ci = (int)(rng.uniform() * nchains);
+      } while (ci < 0 || ci >= nchains);
+      if (nchains < 2 * SWAPDIST + 3) {
+        cjmin = 0;
+        cjrange = nchains;
+      } else {
+        cjmin = max(0, ci - SWAPDIST);
+        cjrange = min(nchains, ci + SWAPDIST) - cjmin;
+      }
+      do {
+        cj = cjmin + (int)(rng.uniform() * cjrange);
+      } while (cj == ci || cj < 0 || cj >= nchains);
+      const double wgt = exp((beta[ci] - beta[cj]) * (S[cj] - S[ci]));
+      if (wgt >= 1.0 || wgt > rng.uniform()) {
+        const double b = beta[ci];
+        beta[ci] = beta[cj];
+        beta[cj] = b;
+        ++acc;
+      }
+    }
+    m.stats[4] += (unsigned long long)(nchains > 1 ? swaptries : 0);
+    m.stats[5] += acc;
+    m.stats[0] += (unsigned long long)(m.C * m.L);
+    *m.counter = sweep + 1;
+  }
+  __syncthreads();
+  if (staged)
+    for (int c = threadIdx.x; c < m.C; c += blockDim.x) m.beta[c] = Bsh[c];
 }
 
 // Enqueue the kernels of one sweep (no host synchronisation).
@@ -277,7 +293,7 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mc3_scale_finish, dim3(m.C), dim3(1024), 0, st, v, m);
   }
-  hipLaunchKernelGGL(mc3_swap, dim3(1), dim3(64), 0, st, m, swaptries);
+  hipLaunchKernelGGL(mc3_swap, dim3(1), dim3(256), 0, st, m, swaptries);
   return hipGetLastError();
 }
 
