@@ -81,6 +81,8 @@ struct ima_ctx {
     MC3Dev dev{};
     DevBuf<double> pdg, newpdg, beta, S, rec_t, tbackup, logs;
     DevBuf<int32_t> status, rec_v, decision, idx, node_item;
+    DevBuf<int2> iblk;
+    DevBuf<unsigned> done;
     DevBuf<uint32_t> acc, cacc;
     int64_t scale_sweeps = 0;
     DevBuf<uint64_t> counter;
@@ -951,6 +953,8 @@ static void mc3_release(ima_ctx *c) {
   release(m.decision);
   release(m.idx);
   release(m.node_item);
+  release(m.iblk);
+  release(m.done);
   release(m.acc);
   release(m.cacc);
   release(m.counter);
@@ -1070,6 +1074,14 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     m.plan.red = m.red.p;
     m.plan.nred = (int)red.size();
     m.plan.partial = m.partial.p;
+    std::vector<int2> iblk(N, make_int2(0, 0));
+    for (const int4 &u : units)
+      if (u.y == 0) iblk[u.x] = make_int2(u.z, u.w);
+    HIPCHK(c, reserve(m.iblk, N));
+    HIPCHK(c, hipMemcpyAsync(m.iblk.p, iblk.data(), N * sizeof(int2), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, reserve(m.done, 1));
+    HIPCHK(c, hipMemsetAsync(m.done.p, 0, sizeof(unsigned), c->stream));
   }
   HIPCHK(c, hipMemcpyAsync(m.beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
   const uint64_t zero = 0;
@@ -1097,6 +1109,9 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.node_first = node_first;
   d.nnodes = nnodes;
   d.node_item = m.node_item.p;
+  d.iblk = m.iblk.p;
+  d.partial = m.partial.p;
+  d.done = m.done.p;
   HIPCHK(c, reserve(m.tbackup, (size_t)nnodes + 1));
   d.tbackup = m.tbackup.p;
   d.seed = seed;
@@ -1152,6 +1167,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
   }
   uint64_t done = 0;
   HIPCHK(c, hipMemcpy(&done, m.counter.p, 8, hipMemcpyDeviceToHost));
+  if (nsweeps > 0) HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
   for (int s = 0; s < nsweeps; ++s) {
     const bool with_scale = scale && ((done + s) & 1);
     m.scale_sweeps += with_scale ? 1 : 0;
@@ -1181,6 +1197,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
       if ((rc = mc3_enqueue(c, with_scale, nullptr)) != IMA_OK) return rc;
     }
   }
+  if (nsweeps > 0) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return IMA_OK;
 }

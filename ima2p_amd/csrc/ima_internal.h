@@ -148,6 +148,9 @@ struct MC3Dev {
   int64_t node_first;    // first node of the MC3 slots in the node arrays
   int64_t nnodes;        // nodes covered by the MC3 slots
   int32_t *node_item;    // [nnodes] item owning each node
+  int2 *iblk;            // [C*L] HKY pattern blocks: {first partial, blocks} (blocks <= 1: direct)
+  double *partial;       // HKY per-block partial log-likelihoods
+  unsigned *done;        // chains finished in the current sweep (last one swaps)
   uint64_t seed;
   double scale_width;
 };
@@ -157,6 +160,9 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
                              int imax_w, int swaptries, hipEvent_t *score_ev,
                              hipStream_t st);
+
+hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 
 // Swap (kern_swap.hip).
 hipError_t launch_swap(int nchains, const double *S, double *beta,
