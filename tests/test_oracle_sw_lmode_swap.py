@@ -1,0 +1,209 @@
+"""Pins the oracle's stepwise-mutation, L-mode and swap functions.
+
+SW (utilities.cpp:55-124, 1453-1491; calc_prob_data.cpp:842-909):
+* bessi0/bessi1 are the Abramowitz & Stegun 9.8.1-9.8.4 polynomials: they
+  agree with scipy's I_0/I_1 to the published error bounds (1.9e-7 and
+  2.2e-7 on the scaled functions, about 5e-7 relative);
+* the Miller recurrence fixes the ratio I_n/I_0 independently of the
+  polynomial: bessi(n,x)/bessi0(x) equals scipy's iv(n,x)/iv(0,x) to 1e-9
+  while the start index dominates x, and drifts beyond (the reference's
+  method, reproduced as is);
+* special cases x > 700, x == 0, negative order;
+* likelihoodSW = sum over non-root edges of -tu + log I_|d|(tu).
+L mode (utilities.cpp:1507-1539, jointfind.cpp:885-1047,
+surface_call_functions.cpp:120-173):
+* eexp(x) = m 10^z reproduces exp(x) to the Taylor truncation (< 1e-9);
+* the exact variant equals a numpy log-sum-exp;
+* the reference's window/list-cap semantics on constructed tables;
+* margincalc equals a direct numpy mean.
+Swap (swapchains.cpp:13-178, 527-653): closed forms and a hand-checked
+sequence.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import iv, ive
+
+import oracle
+from ima2p_amd import sim
+from tests import datasets as D
+
+
+# ------------------------------------------------------------------- SW --
+
+def test_bessi0_bessi1_within_published_bounds():
+    xs = np.concatenate([np.linspace(0, 3.7499, 200), np.linspace(3.75, 600, 400)])
+    for x in xs:
+        r0 = ive(0, x) * math.exp(x)
+        r1 = ive(1, x) * math.exp(x)
+        # A&S bound the error of the scaled function (~0.4) by 1.9e-7 and
+        # 2.2e-7, i.e. about 5e-7 relative
+        assert abs(oracle.bessi0(x) - r0) <= 5e-7 * r0
+        if x > 0:
+            assert abs(oracle.bessi1(x) - r1) <= 6e-7 * r1
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 13, 21, 40, 60])
+def test_miller_recurrence_ratio(n):
+    # The downward recurrence starts at m = 2(n + floor(sqrt(40 n))) whatever
+    # x is, so it is accurate only while m dominates x; there it must match
+    # I_n/I_0 closely.  (Beyond, the reference's value drifts from I_n --
+    # a property of its method that the engine reproduces, see below.)
+    m = 2 * (n + int(math.sqrt(40 * n)))
+    for x in (0.01, 0.3, 1.0, 2.5, 7.0, 20.0, 55.0):
+        if x > m / 4:
+            continue
+        want = ive(n, x) / ive(0, x)
+        if want < 1e-290:
+            continue
+        got = oracle.bessi(n, x) / oracle.bessi0(x)
+        assert abs(got - want) <= 1e-9 * want
+
+
+def test_bessi_large_argument_drift_is_the_references():
+    # n = 2, x = 55: start index 20 is far below x; the recurrence result is
+    # ~3e-5 off the true ratio, and stays finite and positive
+    got = oracle.bessi(2, 55.0) / oracle.bessi0(55.0)
+    want = ive(2, 55.0) / ive(0, 55.0)
+    assert 1e-6 < abs(got - want) / want < 1e-3
+
+
+def test_bessi_special_cases():
+    assert oracle.bessi(3, 701.0) == np.finfo(float).max / 1e10
+    assert oracle.bessi(0, 701.0) == np.finfo(float).max / 1e10
+    assert oracle.bessi(4, 0.0) == 0.0
+    assert oracle.bessi(0, 0.0) == 1.0
+    assert oracle.bessi(-5, 2.0) == oracle.bessi(5, 2.0)
+    assert oracle.bessi(150, 1e-6) == 0.0  # underflow -> the SW zero case
+
+
+def test_likelihood_sw_formula():
+    g, A = D.sw_case(8, (12, 14), u=2.0)
+    for u, tr in ((0.7, 1.0), (3.0, 1.4)):
+        got, dl = oracle.likelihood_sw(g, A, u, tr)
+        bl = g.branch_lengths()
+        ref = 0.0
+        for e in range(2 * g.n - 1):
+            if g.down[e] == -1:
+                assert dl[e] == 0.0
+                continue
+            t = bl[e] * tr
+            d = abs(int(A[e]) - int(A[g.down[e]]))
+            term = -t * u + math.log(iv(d, t * u))
+            assert abs(dl[e] - term) <= 5e-7 * max(1.0, abs(term))
+            ref += term
+        assert abs(got - ref) <= 5e-7 * abs(ref)
+
+
+# --------------------------------------------------------------- L mode --
+
+def test_eexp_matches_exp():
+    for x in np.concatenate([np.linspace(-700, 700, 2001), [0.0, 1e-9, -1e-9]]):
+        m, z = oracle.eexp(x)
+        assert 1.0 <= abs(m) < 10.0 or x == 0.0
+        val = math.log(m) + z * math.log(10.0)
+        assert abs(val - x) <= 1e-9 * max(1.0, abs(x))
+
+
+def lmode_terms(tab, x, npsize=3, nmig=2):
+    L = tab.astype(np.float64)
+    ccp, fcp, hccp = 0, npsize, 2 * npsize
+    mcp, fmp = 3 * npsize, 3 * npsize + nmig
+    probgp = 4 * npsize + 3 * nmig + 1
+    p = -L[:, probgp].copy()
+    for i in range(npsize):
+        p += L[:, ccp + i] * (math.log(2) - math.log(x[i])) - L[:, hccp + i] - 2 * L[:, fcp + i] / x[i]
+    for i in range(nmig):
+        p += L[:, mcp + i] * math.log(x[npsize + i]) - L[:, fmp + i] * x[npsize + i]
+    return p
+
+
+def test_jointp_exact_is_log_sum_exp():
+    rng = np.random.default_rng(1)
+    tab = sim.lmode_table(rng, 3000)
+    for _ in range(10):
+        x = np.concatenate([rng.uniform(1, 12, 3), rng.uniform(0.05, 2, 2)])
+        p = lmode_terms(tab, x)
+        mx = p.max()
+        ref = math.log(len(p)) - (mx + math.log(np.exp(p - mx).sum()))
+        got = oracle.jointp_exact(tab, 3, 2, 2, x)
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+def table_with_terms(values):
+    """A 2-pop table whose per-genealogy term at x = (2,2,2,1,1) is exactly
+    `values` (everything zero except -probg)."""
+    tab = np.zeros((len(values), 21), np.float32)
+    tab[:, 4 * 3 + 3 * 2 + 1] = -np.asarray(values, np.float32)
+    return tab
+
+
+def test_jointp_ref_compat_semantics():
+    x = np.array([2.0, 2.0, 2.0, 1.0, 1.0])
+    # (a) every value inside the 10-nat window and every value kept: the
+    # list cap drops the smallest one
+    vals = np.array([-3.0, -1.0, -2.5, -0.5, -4.0], np.float32)
+    got = oracle.jointp(table_with_terms(vals), 3, 2, 2, x)
+    keep = np.sort(vals.astype(np.float64))[1:]
+    ref = math.log(5) - math.log(np.exp(keep).sum())
+    assert abs(got - ref) <= 1e-9 * abs(ref)
+    # (b) the first value is always kept even when far below the final
+    # maximum; then nothing inside the window is dropped
+    vals = np.array([-50.0, -1.0, -2.0, -3.0], np.float32)
+    got = oracle.jointp(table_with_terms(vals), 3, 2, 2, x)
+    ref = math.log(4) - math.log(np.exp(np.array([-1.0, -2.0, -3.0])).sum())
+    assert abs(got - ref) <= 1e-9 * abs(ref)
+    # (c) values more than 10 nats below the running maximum are not kept;
+    # the two kept ones are both inside the window, so the cap keeps one
+    vals = np.array([-1.0, -30.0, -2.0, -40.0], np.float32)
+    got = oracle.jointp(table_with_terms(vals), 3, 2, 2, x)
+    ref = math.log(4) + 1.0
+    assert abs(got - ref) <= 1e-9 * abs(ref)
+    # (d) one genealogy
+    got = oracle.jointp(table_with_terms([-7.0]), 3, 2, 2, x)
+    assert abs(got - 7.0) <= 1e-9 * 7.0
+
+
+def test_margincalc_is_plain_mean():
+    rng = np.random.default_rng(2)
+    tab = sim.lmode_table(rng, 500).astype(np.float32)
+    L = tab.astype(np.float64)
+    for x in (0.5, 3.0, 9.0):
+        for p in range(3):
+            cc = np.floor(L[:, p] + 0.5)
+            # layout cc 0-2 | fc 3-5 | hcc 6-8 | mc 9-10 | fm 11-12 | qi 13-15 | mi 16-17
+            t = -L[:, 13 + p] + cc * (math.log(2) - math.log(x)) - L[:, 6 + p] - 2 * L[:, 3 + p] / x
+            ref = math.log(np.exp(t).mean())
+            got = oracle.margincalc(tab, 3, 2, x, p)
+            assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+        for p in range(2):
+            mc = np.floor(L[:, 9 + p] + 0.5)
+            t = -L[:, 16 + p] + mc * math.log(x) - L[:, 11 + p] * x
+            ref = math.log(np.exp(t).mean())
+            got = oracle.margincalc(tab, 3, 2, x, 3 + p)
+            assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+# ----------------------------------------------------------------- swap --
+
+def test_swapweight_and_heating():
+    assert oracle.swapweight(1.0, 0.5, -10.0, -12.0) == pytest.approx(math.exp(0.5 * -2.0))
+    b = [oracle.setheat_geometric(i, 8, 0.96, 0.9) for i in range(8)]
+    assert b[0] == 1.0
+    assert all(b[i] > b[i + 1] for i in range(7))
+    assert b[7] == pytest.approx(1 - 0.1 * 7 / 7)
+
+
+def test_swapchains_sequence_by_hand():
+    S = np.array([-10.0, -12.0, -11.0, -9.0])
+    beta = np.array([1.0, 0.9, 0.8, 0.7])
+    # try 1: ci = int(0.1*4) = 0, cj = int(0.3*4) = 1, w = exp(0.1*(-2)) < 1,
+    #        U = 0.5 < w -> swap betas of 0 and 1
+    # try 2: ci = int(0.9*4) = 3, cj: 0.8 -> 3 == ci, retry 0.55 -> 2,
+    #        w = exp((0.7-0.8)(-11+9)) = exp(0.2) >= 1 -> swap 3 and 2
+    uni = [0.1, 0.3, 0.5, 0.9, 0.8, 0.55]
+    b, acc, pairs, used = oracle.swapchains(S, beta, 2, uni)
+    assert acc == 2 and used == 6
+    assert pairs.tolist() == [[0, 1, 1], [3, 2, 1]]
+    np.testing.assert_array_equal(b, [0.9, 1.0, 0.7, 0.8])
