@@ -278,3 +278,97 @@ def swapchains(S, beta, swaptries, uniforms):
     acc = lib.ora_swapchains(len(S), _dp(S), _dp(b), swaptries, _dp(uni), len(uni),
                              _ip(used), _ip(pairs))
     return b, acc, pairs[: 3 * swaptries].reshape(-1, 3), int(used[0])
+
+
+for _f in ("ora_uppergamma", "ora_lowergamma"):
+    getattr(lib, _f).restype = C.c_double
+    getattr(lib, _f).argtypes = [C.c_int, C.c_double]
+lib.ora_logfact.restype = C.c_double
+lib.ora_logfact.argtypes = [C.c_int]
+lib.ora_integrate_coalescent_term.restype = C.c_double
+lib.ora_integrate_coalescent_term.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                              C.c_double]
+lib.ora_integrate_migration_term.restype = C.c_double
+lib.ora_integrate_migration_term.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+lib.ora_treeweight.restype = C.c_int
+lib.ora_treeweight.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                               C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_int),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+
+
+def uppergamma(a, x):
+    return lib.ora_uppergamma(int(a), float(x))
+
+
+def lowergamma(a, x):
+    return lib.ora_lowergamma(int(a), float(x))
+
+
+def integrate_coalescent_term(cc, fc, hcc, qmax, qmin=0.0):
+    return lib.ora_integrate_coalescent_term(int(cc), float(fc), float(hcc), float(qmax),
+                                             float(qmin))
+
+
+def integrate_migration_term(cm, fm, mmax, mmin=0.0):
+    return lib.ora_integrate_migration_term(int(cm), float(fm), float(mmax), float(mmin))
+
+
+def treeweight(g, model, tvals, samppop, hval=1.0):
+    """Genealogy weights of one genealogy: dict of cc, fc, hcc [nper, P] and
+    mc, fm [nper, P, P], plus the total length."""
+    from ima2p_amd.immodel import mig_arrays
+    t = Tree(g)
+    P, nper = model.npops, model.nper
+    me, mt, mto = mig_arrays(g)
+    pop = np.ascontiguousarray(g.pop, np.int32)
+    plist = np.ascontiguousarray(model.plist, np.int32)
+    npres = np.ascontiguousarray(model.npresent, np.int32)
+    pdown = np.ascontiguousarray(model.popdown, np.int32)
+    pope = np.ascontiguousarray(model.pope, np.int32)
+    join = np.ascontiguousarray(model.join, np.int32)
+    tv = np.ascontiguousarray(np.append(tvals, 0.0), np.float64)
+    sp = np.ascontiguousarray(samppop, np.int32)
+    cc = np.zeros(nper * P, np.int32)
+    fc = np.zeros(nper * P)
+    hcc = np.zeros(nper * P)
+    mc = np.zeros(nper * P * P, np.int32)
+    fm = np.zeros(nper * P * P)
+    length = np.zeros(1)
+    rc = lib.ora_treeweight(t.ref, _ip(pop), len(me), _ip(me), _dp(mt), _ip(mto), P, nper,
+                            _ip(plist), _ip(npres), _ip(pdown), _ip(pope), _ip(join), _dp(tv),
+                            _ip(sp), float(hval), _ip(cc), _dp(fc), _dp(hcc), _ip(mc), _dp(fm),
+                            _dp(length))
+    if rc != 0:
+        raise ValueError("treeweight: event count mismatch")
+    return {"cc": cc.reshape(nper, P), "fc": fc.reshape(nper, P), "hcc": hcc.reshape(nper, P),
+            "mc": mc.reshape(nper, P, P), "fm": fm.reshape(nper, P, P), "length": float(length[0])}
+
+
+def integrate_tree_prob(weights_list, model):
+    """integrate_tree_prob over the sum of several genealogies' weights
+    (allgweight): returns (probg, qintegrate, mintegrate)."""
+    cc = sum(w["cc"] for w in weights_list)
+    fc = sum(w["fc"] for w in weights_list)
+    hcc = sum(w["hcc"] for w in weights_list)
+    mc = sum(w["mc"] for w in weights_list)
+    fm = sum(w["fm"] for w in weights_list)
+    q, m = [], []
+    for i, wp in enumerate(model.theta_wp):
+        c = sum(int(cc[k, r]) for k, r in wp)
+        f = 0.0
+        h = 0.0
+        for k, r in wp:
+            f += fc[k, r]
+            h += hcc[k, r]
+        q.append(integrate_coalescent_term(c, f, h, model.qmax[i]))
+    for i, wp in enumerate(model.mig_wp):
+        c = sum(int(mc[k, a, b]) for k, a, b in wp)
+        f = 0.0
+        for k, a, b in wp:
+            f += fm[k, a, b]
+        m.append(integrate_migration_term(c, f, model.mmax[i]))
+    return float(sum(q) + sum(m)), q, m

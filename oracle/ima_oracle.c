@@ -676,3 +676,360 @@ int ora_swapchains(int nchains, const double *S, double *beta, int swaptries,
   if (used) *used = k;
   return acc;
 }
+
+/* =====================================================================
+ * Genealogy prior -- treeweight (update_gtree_common.cpp:1680-1931),
+ * integrate_tree_prob (:1945-2053), integrate_coalescent_term (:109-203),
+ * integrate_migration_term (:206-296), incomplete gamma functions
+ * (utilities.cpp:812-1122: series and continued fraction with EPS = 3e-7,
+ * ITMAX = 1000, FPMIN = 1e-30; exponential integral MAXIT = 100).
+ * ===================================================================== */
+
+#define PRI_EPS 3.0e-7
+#define PRI_ITMAX 1000
+#define PRI_FPMIN 1.0e-30
+
+static double logfact_of(int k) { /* setlogfact: running sum of log(i) */
+  double s = 0.0;
+  for (int i = 1; i <= k; i++) s = s + log((double)i);
+  return s;
+}
+
+double ora_logfact(int k) { return logfact_of(k); }
+
+/* continued fraction for the upper incomplete gamma (log form) */
+static double gcf_log(double a, double x, double *gln) {
+  *gln = logfact_of((int)a - 1);
+  double b = x + 1.0 - a, c = 1.0 / PRI_FPMIN, d = 1.0 / b, h = d;
+  int i;
+  for (i = 1; i <= PRI_ITMAX; i++) {
+    double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < PRI_FPMIN) d = PRI_FPMIN;
+    c = b + an / c;
+    if (fabs(c) < PRI_FPMIN) c = PRI_FPMIN;
+    d = 1.0 / d;
+    double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < PRI_EPS) break;
+  }
+  return (-x + a * log(x) - (*gln)) + log(h);
+}
+
+static double gcf_lin(double a, double x, double *gln) {
+  *gln = logfact_of((int)a - 1);
+  double b = x + 1.0 - a, c = 1.0 / PRI_FPMIN, d = 1.0 / b, h = d;
+  int i;
+  for (i = 1; i <= PRI_ITMAX; i++) {
+    double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < PRI_FPMIN) d = PRI_FPMIN;
+    c = b + an / c;
+    if (fabs(c) < PRI_FPMIN) c = PRI_FPMIN;
+    d = 1.0 / d;
+    double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < PRI_EPS) break;
+  }
+  return exp(-x + a * log(x) - (*gln)) * h;
+}
+
+/* series for the lower incomplete gamma (linear and log forms) */
+static double gser_lin(int a, double x, double *gln) {
+  *gln = logfact_of(a - 1);
+  if (x <= 0.0) return 0.0;
+  double ap = a, del = 1.0 / a, sum = del;
+  for (int n = 1; n <= PRI_ITMAX; n++) {
+    ++ap;
+    del *= x / ap;
+    sum += del;
+    if (fabs(del) < fabs(sum) * PRI_EPS) return sum * exp(-x + a * log(x) - (*gln));
+  }
+  return sum * exp(-x + a * log(x) - (*gln));
+}
+
+static double gser_log(int a, double x, double *gln) {
+  *gln = logfact_of(a - 1);
+  if (x <= 0.0) return 0.0;
+  double ap = a, del = 1.0 / a, sum = del;
+  for (int n = 1; n <= PRI_ITMAX; n++) {
+    ++ap;
+    del *= x / ap;
+    sum += del;
+    if (fabs(del) < fabs(sum) * PRI_EPS) return log(sum) + (-x + a * log(x) - (*gln));
+  }
+  return log(sum) + (-x + a * log(x) - (*gln));
+}
+
+/* exponential integral E_n(x) (value, or its log when *islog = 1) */
+static double expint_n(int n, double x, int *islog) {
+  const double EULER = 0.5772156649;
+  *islog = 0;
+  int nm1 = n - 1;
+  if (n == 0) return exp(-x) / x;
+  if (x == 0.0) return 1.0 / nm1;
+  if (x > 1.0) {
+    double b = x + n, c = 1.0 / PRI_FPMIN, d = 1.0 / b, h = d;
+    for (int i = 1; i <= 100; i++) {
+      double a = -i * (nm1 + i);
+      b += 2.0;
+      d = 1.0 / (a * d + b);
+      c = b + a / c;
+      double del = c * d;
+      h *= del;
+      if (fabs(del - 1.0) < PRI_EPS) {
+        *islog = 1;
+        return log(h) - x;
+      }
+    }
+    *islog = 1;
+    return log(h) - x;
+  }
+  double ans = (nm1 != 0 ? 1.0 / nm1 : -log(x) - EULER), fact = 1.0;
+  for (int i = 1; i <= 100; i++) {
+    double del;
+    fact *= -x / i;
+    if (i != nm1) {
+      del = -fact / (i - nm1);
+    } else {
+      double psi = -EULER;
+      for (int ii = 1; ii <= nm1; ii++) psi += 1.0 / ii;
+      del = fact * (-log(x) + psi);
+    }
+    ans += del;
+    if (fabs(del) < fabs(ans) * PRI_EPS) return ans;
+  }
+  return ans;
+}
+
+double ora_uppergamma(int a, double x) {
+  double gln, p;
+  if (a == 0) {
+    int islog;
+    double t = expint_n(1, x, &islog);
+    p = islog ? t : log(t);
+  } else if (x < (a + 1.0)) {
+    double g = gser_lin(a, x, &gln);
+    p = gln + log(1.0 - g);
+  } else {
+    double g = gcf_log((double)a, x, &gln);
+    p = gln + g;
+  }
+  if (p < -1e200) p = -1e200;
+  return p;
+}
+
+double ora_lowergamma(int a, double x) {
+  double gln, p;
+  if (x < (a + 1.0)) {
+    double g = gser_log(a, x, &gln);
+    p = gln + g;
+  } else {
+    double g = gcf_lin((double)a, x, &gln);
+    p = gln + log(1 - g);
+  }
+  if (p < -1e200) p = -1e200;
+  return p;
+}
+
+static double log_diff(double a, double b) { /* LogDiff imamp.hpp:257-263 */
+  if (a - b < 7.0978271289338397e+02) return b + log(exp(a - b) - 1.0);
+  return a;
+}
+
+double ora_integrate_coalescent_term(int cc, double fc, double hcc, double max, double min) {
+  double p;
+  if (cc > 0) {
+    if (min == 0) {
+      double ug = ora_uppergamma(cc - 1, 2 * fc / max);
+      if (cc > 1) {
+        double fullg = logfact_of(cc - 2);
+        if (fullg - ug < 1e-15 || fullg - ug > 7.0978271289338397e+02) {
+          double lg = ora_lowergamma(cc - 1, 2 * fc / max);
+          if (fullg > lg) {
+            double ugalt = log_diff(fullg, lg);
+            if (fabs(ugalt - ug) > 1e-10) ug = ugalt;
+          }
+        }
+      }
+      p = ug + ORA_LOG2 - hcc + (1 - cc) * log(fc);
+    } else {
+      double a = ora_uppergamma(cc - 1, 2 * fc / max);
+      double b = ora_uppergamma(cc - 1, 2 * fc / min);
+      p = log_diff(a, b) + (ORA_LOG2 - hcc + (1 - cc) * log(fc));
+    }
+  } else {
+    if (2 * fc / max > 0) {
+      if (min == 0) {
+        double a = log(max) - 2.0 * fc / max;
+        double b = ORA_LOG2 + log(fc) + ora_uppergamma(0, 2.0 * fc / max);
+        p = log_diff(a, b);
+      } else {
+        double a = ora_uppergamma(0, 2 * fc / max), b = ora_uppergamma(0, 2 * fc / min);
+        double c = log_diff(a, b) + ORA_LOG2 + log(fc);
+        a = log(max) - 2.0 * fc / max;
+        b = log(min) - 2.0 * fc / min;
+        double d = log_diff(a, b);
+        p = log_diff(d, c);
+      }
+    } else {
+      p = log(max - min);
+    }
+  }
+  return p;
+}
+
+double ora_integrate_migration_term(int cm, double fm, double max, double min) {
+  double p;
+  const double MPRIORMIN = 0.000001; /* imamp.hpp:141 */
+  if (cm > 0) {
+    if (min == 0) {
+      double lg = ora_lowergamma(cm + 1, fm * max);
+      double fullg = logfact_of(cm);
+      if (fullg - lg < 1e-15 || fullg - lg > 7.0978271289338397e+02) {
+        double ug = ora_uppergamma(cm + 1, fm * max);
+        if (fullg > ug) {
+          double lgalt = log_diff(fullg, ug);
+          if (fabs(lgalt - lg) > 1e-12) lg = lgalt;
+        }
+      }
+      p = (-1 - cm) * log(fm) + lg;
+    } else {
+      double a = ora_uppergamma(cm + 1, fm * min), b = ora_uppergamma(cm + 1, fm * max);
+      p = (-1 - cm) * log(fm) + log_diff(a, b);
+    }
+  } else {
+    if (fm > MPRIORMIN) {
+      if (min == 0) {
+        if (max == MPRIORMIN)
+          p = 0;
+        else
+          p = log_diff(0.0, -fm * max) - log(fm);
+      } else {
+        p = log_diff(-fm * min, -fm * max) - log(fm);
+      }
+    } else {
+      p = log(max - min);
+    }
+  }
+  return p;
+}
+
+/* treeweight over one genealogy.  The model is flattened: periods
+ * k = 0..nper-1, plist[k*P + i] = i-th population present in period k
+ * (npresent[k] of them), popdown[p] / pope[p] = parent population and the
+ * period where p ends (-1 for the root population), join[k] = {a, b, new}
+ * for the split closing period k.  mig_* hold the migration events of the
+ * genealogy sorted by edge then time: edge, time, destination.  Outputs
+ * cc[k*P+i] (int), fc[k*P+i], hcc, mc[(k*P+i)*P+j] (int), fm (same). */
+typedef struct {
+  double t;
+  int type; /* 0 coalescence, 1 migration, -1 split */
+  int pop, topop;
+} pri_event;
+
+static int cmp_event(const void *a, const void *b) {
+  double x = ((const pri_event *)a)->t, y = ((const pri_event *)b)->t;
+  return (x > y) - (x < y);
+}
+
+int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_edge,
+                   const double *mig_time, const int *mig_to, int npops, int nper,
+                   const int *plist, const int *npresent, const int *popdown,
+                   const int *pope, const int *join, const double *tvals,
+                   const int *samppop, double hval, int *cc, double *fc, double *hcc,
+                   int *mc, double *fm, double *length) {
+  const int P = npops;
+  int ng = t->n, ne = 2 * ng - 1;
+  double roottime = t->time[t->up0[t->root]];
+  int kroot = 0;
+  while (kroot < nper - 1 && tvals[kroot] <= roottime) kroot++;
+  int nev = (ng - 1) + nmig + kroot;
+  pri_event *ev = (pri_event *)malloc(sizeof(pri_event) * (size_t)(nev + 1));
+  int ec = 0, mi = 0;
+  for (int i = 0; i < ne; i++) {
+    int nowpop = pop[i];
+    if (i >= ng) {
+      ev[ec].t = t->time[t->up0[i]];
+      ev[ec].type = 0;
+      ev[ec].pop = nowpop;
+      ev[ec].topop = -1;
+      ec++;
+    }
+    while (mi < nmig && mig_edge[mi] == i) {
+      double mt = mig_time[mi];
+      int period = 0;
+      while (period < nper - 1 && tvals[period] <= mt) period++;
+      while (pope[nowpop] != -1 && pope[nowpop] <= period) nowpop = popdown[nowpop];
+      ev[ec].t = mt;
+      ev[ec].type = 1;
+      ev[ec].pop = nowpop;
+      ev[ec].topop = mig_to[mi];
+      nowpop = mig_to[mi];
+      ec++;
+      mi++;
+    }
+  }
+  for (int k = 0; k < kroot; k++) {
+    ev[ec].t = tvals[k];
+    ev[ec].type = -1;
+    ev[ec].pop = ev[ec].topop = -1;
+    ec++;
+  }
+  if (ec != nev || mi != nmig) {
+    free(ev);
+    return -1;
+  }
+  qsort(ev, (size_t)nev, sizeof(pri_event), cmp_event);
+  int n[2 * 10 - 1];
+  for (int p = 0; p < 2 * P - 1; p++) n[p] = (p < P) ? samppop[p] : 0;
+  memset(cc, 0, sizeof(int) * (size_t)nper * P);
+  memset(fc, 0, sizeof(double) * (size_t)nper * P);
+  memset(hcc, 0, sizeof(double) * (size_t)nper * P);
+  memset(mc, 0, sizeof(int) * (size_t)nper * P * P);
+  memset(fm, 0, sizeof(double) * (size_t)nper * P * P);
+  int nsum = ng, k = 0;
+  double last = 0.0, len = 0.0, h2term = 1 / (2 * hval);
+  for (int j = 0; j < nev; j++) {
+    double dt = ev[j].t - last;
+    len += nsum * dt;
+    last = ev[j].t;
+    for (int ii = 0; ii < npresent[k]; ii++) {
+      int ip = plist[k * P + ii];
+      fc[k * P + ii] += ((double)n[ip] * ((double)n[ip] - 1)) * dt * h2term;
+      if (k < nper - 1) {
+        double fmtemp = n[ip] * dt;
+        for (int jj = 0; jj < npresent[k]; jj++)
+          if (jj != ii) fm[(k * P + ii) * P + jj] += fmtemp;
+      }
+    }
+    if (ev[j].type == 0) {
+      int ii = 0;
+      while (plist[k * P + ii] != ev[j].pop) ii++;
+      cc[k * P + ii]++;
+      n[ev[j].pop]--;
+      nsum--;
+    } else if (ev[j].type == 1) {
+      int ii = 0, jj = 0;
+      while (plist[k * P + ii] != ev[j].pop) ii++;
+      while (plist[k * P + jj] != ev[j].topop) jj++;
+      mc[(k * P + ii) * P + jj]++;
+      n[ev[j].pop]--;
+      n[ev[j].topop]++;
+    } else {
+      const int *jn = join + 3 * k;
+      k++;
+      n[jn[2]] = n[jn[0]] + n[jn[1]];
+      n[jn[0]] = n[jn[1]] = 0;
+    }
+  }
+  double hlog = log(hval);
+  if (hlog != 0.0)
+    for (int kk = 0; kk < nper; kk++)
+      for (int i = 0; i < npresent[kk]; i++) hcc[kk * P + i] += hlog * cc[kk * P + i];
+  *length = len;
+  free(ev);
+  return 0;
+}

@@ -108,6 +108,19 @@ double ora_margincalc(const float *table, int G, int K, int npsize, int nmig,
                       double x, double yadjust, int pi, int logi,
                       int expo_prior, double prior_mean);
 
+/* ---------------- genealogy prior (update_gtree_common.cpp, utilities.cpp) */
+double ora_logfact(int k);
+double ora_uppergamma(int a, double x);
+double ora_lowergamma(int a, double x);
+double ora_integrate_coalescent_term(int cc, double fc, double hcc, double max, double min);
+double ora_integrate_migration_term(int cm, double fm, double max, double min);
+int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_edge,
+                   const double *mig_time, const int *mig_to, int npops, int nper,
+                   const int *plist, const int *npresent, const int *popdown,
+                   const int *pope, const int *join, const double *tvals,
+                   const int *samppop, double hval, int *cc, double *fc, double *hcc,
+                   int *mc, double *fm, double *length);
+
 #ifdef __cplusplus
 }
 #endif
