@@ -4,18 +4,23 @@
 Workload (default): three-population IM model, 50 HKY loci x 1000 bp,
 45 gene copies per locus, 64 heated chains (geometric heating), synthetic
 data from the package's simulator.  One step = one MC3 sweep over all
-chains: per (chain, locus) one genealogy proposal scored in a single
-batched FP32-scaled launch and accepted or rejected, on alternate sweeps a
-whole-chain rescaling that re-scores all 50 loci of every chain, then the
-chain swap.  value = chain-steps/s = chains x sweeps / wall time, with all
-state resident in HBM.
+chains: per (chain, locus) one genealogy proposal, scored in a single
+batched FP32-scaled likelihood launch plus one launch of genealogy weights,
+then accepted or rejected locus by locus with beta (dprobg + dpdg), probg
+the chain's genealogy prior integrated over the population-size and
+migration priors (update_gtree.cpp:827-961); on alternate sweeps a
+whole-chain rescaling of node, migration and split times that re-scores
+all 50 loci and the prior; then the chain swap.  value = chain-steps/s =
+chains x sweeps / wall time, with all state resident in HBM.
 
-Also reported: the HKY kernel's roofline (HIP events on its stream during
-an eager pass of the same sweeps), the same sweep in FP64, the plain
+Also reported: the HKY kernel's roofline (HIP events on the context's
+stream around the sweep's scoring launch repeated back to back, over the
+chains' current genealogies), the same sweep in FP64, the plain
 per-locus FP64 GPU path (one launch per likelihood call, host control
-loop), L-mode jointp points/s, and the CPU baseline: the oracle (a C
-restatement of the reference's likelihoodHKY, incremental as the reference
-evaluates it) driving the same sweeps on all host cores.
+loop, one prior call per proposal), L-mode jointp points/s, and the CPU
+baseline: the oracle (a C restatement of the reference's likelihoodHKY,
+incremental as the reference evaluates it, and of treeweight and the
+prior integrals) driving the same sweeps on all host cores.
 """
 from __future__ import annotations
 
@@ -43,16 +48,111 @@ def hky_flops(n, S):
     return (36 * n - 58) * S
 
 
+def hky_traffic():
+    """HBM bytes per HKY launch of this workload, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by
+    the same counter's reading for a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "hky_kernel_3pop_50x1000bp_64chains_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 # ------------------------------------------------------------- CPU baseline --
 
 _WL = None
 
 
+class _OraclePrior:
+    """The genealogy prior of the CPU baseline, through the oracle: weights
+    per genealogy (treeweight), all-loci sums updated as
+    sum_subtract_treeinfo does, integrals recomputed only for parameters
+    whose (count, weight) changed."""
+
+    def __init__(self, model, samp):
+        import oracle
+        self.o = oracle
+        self.model, self.samp, P = model, samp, model.npops
+        self.P2, self.P3 = P * P, P * P * P
+        p2, p3 = self.P2, self.P3
+        self.params = []
+        for wp in model.theta_wp:
+            ix = np.array([k * P + i for k, i in wp], int)
+            self.params.append(("q", ix, p2 + ix, 2 * p2 + ix))
+        for wp in model.mig_wp:
+            ix = np.array([(k * P + i) * P + j for k, i, j in wp], int)
+            self.params.append(("m", 3 * p2 + ix, 3 * p2 + p3 + ix, None))
+        self.recs, self.all, self.cache, self.tv, self.pend = {}, {}, {}, {}, None
+
+    def _flat(self, g, tv):
+        w = self.o.treeweight(g, self.model, tv, self.samp)
+        return np.concatenate([w["cc"].ravel(), w["fc"].ravel(), w["hcc"].ravel(),
+                               w["mc"].ravel(), w["fm"].ravel(), [w["length"]]]).astype(float)
+
+    def _update(self, a, old, new):
+        p2, p3 = self.P2, self.P3
+        out = a + (new - old)
+        for lo, hi in ((p2, 2 * p2), (3 * p2 + p3, 3 * p2 + 2 * p3)):
+            out[lo:hi] = np.maximum(0.0, (a[lo:hi] - old[lo:hi]) + new[lo:hi])
+        return out
+
+    def _terms(self, a, cache):
+        out = []
+        for t, (kind, ic, jf, jh) in enumerate(self.params):
+            c, f = int(round(a[ic].sum())), float(a[jf].sum())
+            if cache is not None and cache[t][0] == c and cache[t][1] == f:
+                out.append(cache[t])
+                continue
+            if kind == "q":
+                v = self.o.integrate_coalescent_term(c, f, float(a[jh].sum()),
+                                                     self.model.qmax[t])
+            else:
+                v = self.o.integrate_migration_term(c, f, self.model.mmax[t - self.model.npsize])
+            out.append((c, f, v))
+        return out
+
+    def init(self, c, gens, tv):
+        self.tv[c] = np.array(tv, float)
+        self.recs[c] = [self._flat(g, tv) for g in gens]
+        self.all[c] = np.sum(self.recs[c], axis=0)
+        self.cache[c] = self._terms(self.all[c], None)
+        return sum(x[2] for x in self.cache[c])
+
+    def try_locus(self, c, l, g):
+        new = self._flat(g, self.tv[c])
+        a = self._update(self.all[c], self.recs[c][l], new)
+        cache = self._terms(a, self.cache[c])
+        self.pend = (new, a, cache)
+        return sum(x[2] for x in cache)
+
+    def settle_locus(self, c, l, accept, old=None):
+        if accept:
+            self.recs[c][l], self.all[c], self.cache[c] = self.pend
+
+    def try_chain(self, c, gens, tv):
+        recs = [self._flat(g, tv) for g in gens]
+        a = np.sum(recs, axis=0)
+        cache = self._terms(a, None)
+        self.pend = (recs, a, cache)
+        return sum(x[2] for x in cache)
+
+    def settle_chain(self, c, accept, old=None, tv=None):
+        if accept:
+            self.recs[c], self.all[c], self.cache[c] = self.pend
+            self.tv[c] = np.array(tv, float)
+
+
 def _cpu_worker(args):
-    """Run the host sweep with the oracle's likelihoodHKY on a chain subset
-    for `seconds`; returns (chain_steps, elapsed)."""
+    """Run the host sweep with the oracle's likelihoodHKY and genealogy prior
+    on a chain subset for `seconds`; returns (chain_steps, elapsed)."""
     wid, chain_ids, seconds = args
     import oracle
+    from ima2p_amd import immodel
     from ima2p_amd.hostmc3 import HostMC3
     wl = _WL
     chains = [[g.copy() for g in wl.chains[c]] for c in chain_ids]
@@ -72,7 +172,9 @@ def _cpu_worker(args):
         oracle.hky_commit(pend[(c, l)], states[(c, l)], acc)
 
     beta = workloads.geometric_betas(len(chain_ids))
-    h = HostMC3(chains, beta, lik, commit, seed=100 + wid)
+    prior = _OraclePrior(immodel.build(wl.poptree), wl.samppop)
+    h = HostMC3(chains, beta, lik, commit, seed=100 + wid, prior=prior, tv=wl.tsplit,
+                tmax=wl.tmax)
     t0 = time.perf_counter()
     k = 0
     while time.perf_counter() - t0 < seconds:
@@ -96,28 +198,43 @@ def cpu_baseline(wl, seconds, cores):
 
 # ----------------------------------------------------------------- GPU legs --
 
-def load_engine(wl, prec):
+def load_engine(wl, prec, prior=True):
     from ima2p_amd import _lib as L
+    from ima2p_amd import immodel
     from ima2p_amd.engine import Engine
     e = Engine(0)
+    model = immodel.build(wl.poptree)
+    mid = e.model_add(model)
     loci = [e.add_locus_hky(loc) for loc in wl.loci]
+    for lid in loci:
+        e.locus_set_model(lid, mid, wl.samppop)
     C, Lc = wl.nchains, wl.nloci
     first = e.alloc_slots([loci[l] for c in range(C) for l in range(Lc)])
-    e.write_slots(first, [wl.chains[c][l] for c in range(C) for l in range(Lc)])
+    gens = [wl.chains[c][l] for c in range(C) for l in range(Lc)]
+    e.write_slots(first, gens)
+    e.write_pops(first, gens)
     u = np.array([wl.u[l] for c in range(C) for l in range(Lc)])
     e.mc3_setup(C, Lc, first, u, wl.kappa, wl.beta, seed=12345, precision=prec)
-    return e, loci
+    if prior:
+        e.mc3_set_prior(np.tile(wl.tsplit, (C, 1)), wl.tmax, model.npops)
+    return e, loci, first, model
 
 
-def plain_fp64_path(e, loci, wl, sweeps, nchains):
-    """One likelihood call = one upload + one FP64 launch + one readback."""
-    from ima2p_amd.hostmc3 import HostMC3
+def plain_fp64_path(e, loci, model, wl, sweeps, nchains):
+    """One likelihood call = one upload + one FP64 launch + one readback;
+    one prior call per proposal likewise (the chain's genealogies in
+    slots of their own)."""
+    from ima2p_amd.hostmc3 import EnginePrior, HostMC3
     chains = [[g.copy() for g in wl.chains[c]] for c in range(nchains)]
+    Lc = wl.nloci
+    first = e.alloc_slots([loci[l] for c in range(nchains) for l in range(Lc)])
+    slots = [[first + c * Lc + l for l in range(Lc)] for c in range(nchains)]
 
     def lik(c, l, g, dirty, commit):
         return e.likelihood_hky(loci[l], g, wl.u[l], wl.kappa, e=(dirty[0],) * 4)
 
-    h = HostMC3(chains, workloads.geometric_betas(nchains), lik, lambda c, l, a: None, seed=7)
+    h = HostMC3(chains, workloads.geometric_betas(nchains), lik, lambda c, l, a: None, seed=7,
+                prior=EnginePrior(e, model, slots), tv=wl.tsplit, tmax=wl.tmax)
     t0 = time.perf_counter()
     for k in range(sweeps):
         h.sweep(with_scale=bool(k & 1))
@@ -162,11 +279,24 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64/plain/L-mode legs")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run 5 plain device copies of MIB MiB (fp32 copy_), the "
+                         "known-size kernel that scales PMC FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs all heated chains on one GPU; "
                                    "multi-GPU runs are not part of this build (use --gpus 1)"}))
         sys.exit(2)
+
+    if args.calibrate_copy > 0:
+        import torch
+        nel = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t_setup = time.perf_counter()
     wl = workloads.three_pop_hky()
@@ -179,12 +309,13 @@ def main():
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         rate, steps, wall = cpu_baseline(wl, args.cpu_seconds, cores)
         cpu = {"value": rate, "unit": "chain-steps/s", "cores": cores, "kind": "port",
-               "sample": f"oracle likelihoodHKY (C, incremental as the reference) driving the "
+               "sample": f"oracle likelihoodHKY (C, incremental as the reference) and genealogy "
+                         f"prior (C treeweight + integrals) driving the "
                          f"same sweeps: {cores} processes x 4 chains x 50 loci, "
                          f"{steps} chain-steps in {wall:.1f} s"}
 
     from ima2p_amd import _lib as L
-    e, loci = load_engine(wl, L.PREC_FP32)
+    e, loci, first, model = load_engine(wl, L.PREC_FP32)
     setup_s = time.perf_counter() - t_setup
     e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE)
     t0 = time.perf_counter()
@@ -193,13 +324,20 @@ def main():
     value = C * args.steps / dt
     pdg, beta, stats, sweeps = e.mc3_read()
 
-    # HKY kernel timing: eager pass of the same sweeps with HIP events
+    # eager sweeps with HIP events around each sweep
     e.timing(True)
     e.timing_reset()
     ksweeps = max(10, min(args.steps, 100))
     e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE)
-    launches, ms = e.timing_get(L.K_HKY32)
     sweep_n, sweep_ms = e.timing_get(L.K_MC3)
+    # HKY kernel: the sweep's scoring launch (every chain x locus of the
+    # chains' current genealogies) repeated back to back between one pair
+    # of HIP events on the context's stream
+    e.timing_reset()
+    items = e.items(first + np.arange(C * Lc), np.tile(wl.u, C), wl.kappa)
+    reps = 200
+    e.score_repeat(items, reps, L.PREC_FP32)
+    launches, ms = e.timing_get(L.K_HKY32)
     e.timing(False)
     avg_s = ms / launches / 1e3
     flops = float(sum(hky_flops(n, S[l]) for l in range(Lc))) * C
@@ -207,13 +345,13 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        e64, loci64 = load_engine(wl, L.PREC_FP64)
+        e64, loci64, _, model64 = load_engine(wl, L.PREC_FP64)
         e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE)
         t1 = time.perf_counter()
         k64 = max(10, args.steps // 4)
         e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE)
         extra["batched_fp64_chain_steps_per_s"] = C * k64 / (time.perf_counter() - t1)
-        plain = plain_fp64_path(e64, loci64, wl, sweeps=4, nchains=4)
+        plain = plain_fp64_path(e64, loci64, model64, wl, sweeps=4, nchains=4)
         extra["plain_per_locus_fp64_chain_steps_per_s"] = plain
         extra["speedup_vs_plain_fp64"] = value / plain
         e64.close()
@@ -235,13 +373,15 @@ def main():
         "config": {"workload": wl.name, "chains": C, "loci": Lc, "sites_per_locus": 1000,
                    "genes_per_locus": n, "patterns_mean": float(S.mean()),
                    "patterns_min": int(S.min()), "patterns_max": int(S.max()),
-                   "sweep": "1 genealogy proposal per (chain, locus) + whole-chain rescale "
+                   "sweep": "1 genealogy proposal per (chain, locus) scored with data likelihood "
+                            "and integrated genealogy prior, loci in order + whole-chain rescale "
+                            "of node, migration and split times "
                             "with full re-score on alternate sweeps + swap",
                    "parallelism": "all chains on one GPU"},
         "roofline": {"bound": "mfma", "pipe": "VALU FP32 (pruning is not a dense contraction)",
                      "kernel": "hky_kernel<float>", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": hky_traffic(),
                      "flops_per_launch": flops, "avg_launch_us": avg_s * 1e6,
                      "launches_timed": launches},
         "cpu_baseline": cpu,

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libima_gpu.so")
 IMA_OK = 0
 MODEL_HKY, MODEL_IS, MODEL_SW = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
-STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE = 0, 1, 2, 3
+STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE, STATUS_PRIOR_BAD = 0, 1, 2, 3, 4
 LMODE_REF_COMPAT, LMODE_EXACT = 0, 1
 REJECT_IS = -1000000000.0
 MAX_GENES = 256
@@ -52,6 +52,16 @@ PROTOS = {
     "ima_slots_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ima_slot_info": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ima_score": (C.c_int, [vp, C.POINTER(Item), C.c_int, C.c_int, dp, i32p]),
+    "ima_model_add": (C.c_int, [vp, C.c_int, i32p, i32p, i32p, i32p, i32p, C.c_int, i32p, i32p,
+                                C.c_int, i32p, i32p, dp, dp, C.POINTER(C.c_int)]),
+    "ima_locus_set_model": (C.c_int, [vp, C.c_int, C.c_int, i32p, C.c_double]),
+    "ima_slots_write_pops": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p]),
+    "ima_treeweight": (C.c_int, [vp, i32p, C.c_int, dp, i32p, dp, dp, i32p, dp, dp]),
+    "ima_prior": (C.c_int, [vp, i32p, i32p, C.c_int, dp, dp, dp, dp]),
+    "ima_mc3_set_prior": (C.c_int, [vp, dp, C.c_double]),
+    "ima_mc3_read_prior": (C.c_int, [vp, dp, dp]),
+    "ima_slots_read_mig": (C.c_int, [vp, C.c_int, dp]),
+    "ima_score_repeat": (C.c_int, [vp, C.POINTER(Item), C.c_int, C.c_int, C.c_int, dp, i32p]),
     "ima_is_site_nodes": (C.c_int, [vp, C.c_int, i32p]),
     "ima_sw_dlike": (C.c_int, [vp, C.POINTER(Item), dp]),
     "ima_likelihood_hky": (C.c_int, [vp, C.c_int, i32p, i32p, i32p, dp, C.c_int, C.c_double,

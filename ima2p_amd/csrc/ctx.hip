@@ -1,5 +1,6 @@
 // ctx.hip -- context, device-resident data and the extern "C" entry points.
 #include <math.h>
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -51,6 +52,24 @@ struct ima_ctx {
   DevBuf<ProgOp> d_prog;
   bool slots_meta_dirty = true;
   std::vector<int> scratch_slot;  // per locus, for ima_likelihood_hky
+  // population labels and migration events of the genealogies (prior)
+  std::vector<int32_t> pop, mig_edge, mig_to;
+  std::vector<double> mig_time;
+  size_t migs = 0;
+  DevBuf<int32_t> d_pop, d_mig_edge, d_mig_to;
+  DevBuf<double> d_mig_time;
+
+  // population models of the genealogy prior
+  std::vector<ModelMeta> models;
+  std::vector<int32_t> mint;
+  std::vector<double> mdbl, logfact;
+  bool models_dirty = true;
+  size_t logfact_dev = 0;
+  DevBuf<ModelMeta> d_models;
+  DevBuf<int32_t> d_mint;
+  DevBuf<double> d_mdbl, d_logfact;
+  DevBuf<double> d_rec, d_tv, d_probg, d_qint, d_mintg;
+  DevBuf<int32_t> d_pslots, d_prow, d_goff, d_pstatus;
 
   // scoring scratch
   DevBuf<ima_item> d_items;
@@ -90,6 +109,9 @@ struct ima_ctx {
     DevBuf<ima_item> items;
     DevBuf<int4> units, red;
     DevBuf<double> partial;
+    // genealogy prior
+    DevBuf<double> tv, tvbackup, wrec, wnew, wall, qterm, qf, probg, mtbackup;
+    DevBuf<int32_t> qc, tv_row, tw_slots, wstatus;
     HkyPlan plan;
     hipGraphExec_t exec[2]{nullptr, nullptr};
     hipGraph_t graph[2]{nullptr, nullptr};
@@ -191,7 +213,50 @@ DeviceView view(ima_ctx *c) {
   v.seq = c->d_seq.p;
   v.mult = c->d_mult.p;
   v.mask = c->d_mask.p;
+  v.pop = c->d_pop.p;
+  v.mig_edge = c->d_mig_edge.p;
+  v.mig_to = c->d_mig_to.p;
+  v.mig_time = c->d_mig_time.p;
+  v.models = c->d_models.p;
+  v.mint = c->d_mint.p;
+  v.mdbl = c->d_mdbl.p;
+  v.logfact = c->d_logfact.p;
   return v;
+}
+
+// log(k!) for k < need, by the running sum of log(i) (setlogfact).
+void grow_logfact(ima_ctx *c, size_t need) {
+  if (c->logfact.empty()) c->logfact.push_back(0.0);
+  while (c->logfact.size() < need) {
+    const double k = (double)c->logfact.size();
+    c->logfact.push_back(c->logfact.back() + log(k));
+  }
+}
+
+int upload_models(ima_ctx *c) {
+  if (c->models_dirty) {
+    HIPCHK(c, reserve(c->d_models, c->models.size() + 1));
+    HIPCHK(c, reserve(c->d_mint, c->mint.size() + 1));
+    HIPCHK(c, reserve(c->d_mdbl, c->mdbl.size() + 1));
+    if (!c->models.empty()) {
+      HIPCHK(c, hipMemcpyAsync(c->d_models.p, c->models.data(),
+                               c->models.size() * sizeof(ModelMeta), hipMemcpyHostToDevice,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mint.p, c->mint.data(), c->mint.size() * 4,
+                               hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mdbl.p, c->mdbl.data(), c->mdbl.size() * 8,
+                               hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->models_dirty = false;
+  }
+  if (c->logfact.size() > c->logfact_dev) {
+    HIPCHK(c, reserve(c->d_logfact, c->logfact.size()));
+    HIPCHK(c, hipMemcpy(c->d_logfact.p, c->logfact.data(), c->logfact.size() * 8,
+                        hipMemcpyHostToDevice));
+    c->logfact_dev = c->logfact.size();
+  }
+  return IMA_OK;
 }
 
 // Slot need of the subtree below edge e (0 for a tip): Strahler-style.
@@ -264,13 +329,13 @@ int timing_end(ima_ctx *c, int k) {
   }
   return IMA_OK;
 }
-int timing_collect(ima_ctx *c, int k) {
+int timing_collect(ima_ctx *c, int k, int count = 1) {
   if (c->timing.on) {
     HIPCHK(c, hipEventSynchronize(c->timing.ev1[k]));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->timing.ev0[k], c->timing.ev1[k]));
     c->timing.ms[k] += ms;
-    c->timing.launches[k] += 1;
+    c->timing.launches[k] += count;
   }
   return IMA_OK;
 }
@@ -366,6 +431,8 @@ int ima_locus_add_hky(ima_ctx *c, int n, int S, const int8_t *seq, const int32_t
   for (size_t i = 0; i < (size_t)n * S; ++i)
     if (seq[i] < 0 || seq[i] > 3) return fail(c, IMA_ERR_ARG, "HKY base outside 0..3");
   LocusMeta m{};
+  m.pmodel = -1;
+  m.hval = 1.0;
   m.model = IMA_MODEL_HKY;
   m.n = n;
   m.S = S;
@@ -394,6 +461,8 @@ int ima_locus_add_is(ima_ctx *c, int n, int S, const int8_t *seq, int *locus) {
   if (n < 2 || n > kMaxGenes) return fail(c, IMA_ERR_SIZE, "IS locus n=%d outside [2,%d]", n, kMaxGenes);
   if (S < 0) return IMA_ERR_ARG;
   LocusMeta m{};
+  m.pmodel = -1;
+  m.hval = 1.0;
   m.model = IMA_MODEL_IS;
   m.n = n;
   m.S = S;
@@ -419,6 +488,8 @@ int ima_locus_add_sw(ima_ctx *c, int n, int *locus) {
   if (!c || !locus) return IMA_ERR_ARG;
   if (n < 2 || n > kMaxGenes) return fail(c, IMA_ERR_SIZE, "SW locus n=%d outside [2,%d]", n, kMaxGenes);
   LocusMeta m{};
+  m.pmodel = -1;
+  m.hval = 1.0;
   m.model = IMA_MODEL_SW;
   m.n = n;
   *locus = (int)c->loci.size();
@@ -456,12 +527,14 @@ int ima_slots_alloc(ima_ctx *c, int count, const int32_t *locus_of, int *first) 
     c->slots.push_back(s);
   }
   c->up0.resize(c->nodes, -1);
+  c->pop.resize(c->nodes, 0);
   c->up1.resize(c->nodes, -1);
   c->down.resize(c->nodes, -1);
   c->A.resize(c->nodes, 0);
   c->time.resize(c->nodes, 0.0);
   c->prog.resize(c->progs);
   HIPCHK(c, reserve(c->d_up0, c->nodes, true));
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true));
   HIPCHK(c, reserve(c->d_up1, c->nodes, true));
   HIPCHK(c, reserve(c->d_down, c->nodes, true));
   HIPCHK(c, reserve(c->d_A, c->nodes, true));
@@ -488,14 +561,18 @@ int ima_slots_write(ima_ctx *c, int first, int count, const int32_t *up0, const 
     std::string why;
     if (!valid_tree(n, up0 + src, up1 + src, down + src, time + src, root[i], why))
       return fail(c, IMA_ERR_ARG, "slot %d: %s", first + i, why.c_str());
+    need.assign(ne, 0);
+    const int r = need_of(up0 + src, up1 + src, root[i], need);
+    if (c->mc3.live && first + i >= c->mc3.first &&
+        first + i < c->mc3.first + c->mc3.C * c->mc3.L && std::max(1, r) > c->mc3.hmax_slots)
+      return fail(c, IMA_ERR_STATE, "slot %d: program depth %d exceeds the running sweep's %d",
+                  first + i, r, c->mc3.hmax_slots);
     std::copy(up0 + src, up0 + src + ne, c->up0.begin() + s.node_off);
     std::copy(up1 + src, up1 + src + ne, c->up1.begin() + s.node_off);
     std::copy(down + src, down + src + ne, c->down.begin() + s.node_off);
     std::copy(time + src, time + src + ne, c->time.begin() + s.node_off);
     if (A) std::copy(A + src, A + src + ne, c->A.begin() + s.node_off);
     s.root = root[i];
-    need.assign(ne, 0);
-    const int r = need_of(up0 + src, up1 + src, root[i], need);
     ops.clear();
     emit(up0 + src, up1 + src, root[i], need, ops);
     std::copy(ops.begin(), ops.end(), c->prog.begin() + s.prog_off);
@@ -558,8 +635,8 @@ int ima_slot_info(ima_ctx *c, int slot, int *locus, int *n) {
 // --------------------------------------------------------------- scoring --
 
 static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precision,
-                      double *loglik, int32_t *status) {
-  if (!c || (!items && nitems > 0) || !loglik || nitems < 0) return IMA_ERR_ARG;
+                      double *loglik, int32_t *status, int reps = 1) {
+  if (!c || (!items && nitems > 0) || !loglik || nitems < 0 || reps < 1) return IMA_ERR_ARG;
   if (precision != IMA_PREC_FP64 && precision != IMA_PREC_FP32) return IMA_ERR_ARG;
   if (nitems == 0) return IMA_OK;
   hipSetDevice(c->device);
@@ -631,20 +708,26 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
   }
   if (!hky.empty()) {
     if ((rc = timing_begin(c, kh)) != IMA_OK) return rc;
-    HIPCHK(c, launch_hky(v, c->d_items.p, plan, precision, hmax_n, hmax_slots, c->d_out.p,
-                         c->d_status.p, c->stream));
+    // repeated passes write the same block partials; only the last merges them
+    HkyPlan pass = plan;
+    pass.nred = 0;
+    for (int r = 0; r < reps; ++r)
+      HIPCHK(c, launch_hky(v, c->d_items.p, r == reps - 1 ? plan : pass, precision, hmax_n,
+                           hmax_slots, c->d_out.p, c->d_status.p, c->stream));
     if ((rc = timing_end(c, kh)) != IMA_OK) return rc;
   }
   if (!idx_i.empty()) {
     if ((rc = timing_begin(c, K_IS)) != IMA_OK) return rc;
-    HIPCHK(c, launch_is(v, c->d_items.p, c->d_idx.p + hky.size(), (int)idx_i.size(), imax_n,
-                        imax_w, c->d_out.p, c->d_status.p, nullptr, nullptr, c->stream));
+    for (int r = 0; r < reps; ++r)
+      HIPCHK(c, launch_is(v, c->d_items.p, c->d_idx.p + hky.size(), (int)idx_i.size(), imax_n,
+                          imax_w, c->d_out.p, c->d_status.p, nullptr, nullptr, c->stream));
     if ((rc = timing_end(c, K_IS)) != IMA_OK) return rc;
   }
   if (!idx_s.empty()) {
     if ((rc = timing_begin(c, K_SW)) != IMA_OK) return rc;
-    HIPCHK(c, launch_sw(v, c->d_items.p, c->d_idx.p + hky.size() + idx_i.size(),
-                        (int)idx_s.size(), c->d_out.p, c->d_status.p, nullptr, c->stream));
+    for (int r = 0; r < reps; ++r)
+      HIPCHK(c, launch_sw(v, c->d_items.p, c->d_idx.p + hky.size() + idx_i.size(),
+                          (int)idx_s.size(), c->d_out.p, c->d_status.p, nullptr, c->stream));
     if ((rc = timing_end(c, K_SW)) != IMA_OK) return rc;
   }
   HIPCHK(c, hipMemcpyAsync(loglik, c->d_out.p, nitems * 8, hipMemcpyDeviceToHost, c->stream));
@@ -652,15 +735,20 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
     HIPCHK(c, hipMemcpyAsync(status, c->d_status.p, nitems * 4, hipMemcpyDeviceToHost,
                              c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (!hky.empty() && (rc = timing_collect(c, kh)) != IMA_OK) return rc;
-  if (!idx_i.empty() && (rc = timing_collect(c, K_IS)) != IMA_OK) return rc;
-  if (!idx_s.empty() && (rc = timing_collect(c, K_SW)) != IMA_OK) return rc;
+  if (!hky.empty() && (rc = timing_collect(c, kh, reps)) != IMA_OK) return rc;
+  if (!idx_i.empty() && (rc = timing_collect(c, K_IS, reps)) != IMA_OK) return rc;
+  if (!idx_s.empty() && (rc = timing_collect(c, K_SW, reps)) != IMA_OK) return rc;
   return IMA_OK;
 }
 
 int ima_score(ima_ctx *c, const ima_item *items, int nitems, int precision, double *loglik,
               int32_t *status) {
   return score_impl(c, items, nitems, precision, loglik, status);
+}
+
+int ima_score_repeat(ima_ctx *c, const ima_item *items, int nitems, int precision, int reps,
+                     double *loglik, int32_t *status) {
+  return score_impl(c, items, nitems, precision, loglik, status, reps);
 }
 
 int ima_is_site_nodes(ima_ctx *c, int slot, int32_t *site_node) {
@@ -963,6 +1051,20 @@ static void mc3_release(ima_ctx *c) {
   release(m.units);
   release(m.red);
   release(m.partial);
+  release(m.tv);
+  release(m.tvbackup);
+  release(m.wrec);
+  release(m.wnew);
+  release(m.wall);
+  release(m.qterm);
+  release(m.qf);
+  release(m.probg);
+  release(m.mtbackup);
+  release(m.qc);
+  release(m.tv_row);
+  release(m.tw_slots);
+  release(m.wstatus);
+  m.dev.prior = -1;
   m.live = false;
 }
 
@@ -1009,9 +1111,8 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
       sw.push_back(i);
     }
   }
-  // topology never changes inside the sweep, so the slot need is fixed; leave
-  // room for a re-uploaded genealogy with a deeper program.
-  m.hmax_slots = std::max(m.hmax_slots, 1 + (int)floor(log2((double)m.hmax_n)));
+  // the sweep's proposals never change a topology, so the stack depth the
+  // programs need is fixed for the run
   m.nh = (int)hky.size();
   m.ni = (int)is.size();
   m.ns = (int)sw.size();
@@ -1116,6 +1217,10 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.tbackup = m.tbackup.p;
   d.seed = seed;
   d.scale_width = scale_width;
+  d.prior = -1;
+  d.P = 1;
+  d.R = 0;
+  d.nparam = 0;
   for (int k = 0; k < 4; ++k) HIPCHK(c, hipEventCreate(&m.ev[k]));
   // initial likelihood of every genealogy
   const DeviceView v = view(c);
@@ -1223,6 +1328,447 @@ int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_
     stats[3] = ca;
   }
   if (sweeps) HIPCHK(c, hipMemcpy(sweeps, m.counter.p, 8, hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
+int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  if (!(tmax > 0)) return fail(c, IMA_ERR_ARG, "tmax must be > 0");
+  hipSetDevice(c->device);
+  const int N = m.C * m.L;
+  int model = -1, emax = 1;
+  for (int i = 0; i < N; ++i) {
+    const SlotMeta &s = c->slots[m.first + i];
+    const LocusMeta &lm = c->loci[s.locus];
+    if (lm.pmodel < 0)
+      return fail(c, IMA_ERR_STATE, "MC3 slot %d: locus has no population model", m.first + i);
+    if (model < 0) model = lm.pmodel;
+    if (lm.pmodel != model) return fail(c, IMA_ERR_ARG, "MC3 loci must share one population model");
+  }
+  const ModelMeta &mm = c->models[model];
+  const int P = mm.npops, tw = P > 1 ? P - 1 : 1, R = prior_rec_len(P);
+  const int np = mm.npsize + mm.nmig;
+  if (P > 1 && !tvals) return IMA_ERR_ARG;
+  for (int ch = 0; ch < m.C && P > 1; ++ch)
+    for (int k = 0; k < P - 1; ++k) {
+      const double t = tvals[ch * (P - 1) + k];
+      if (!(t > 0) || t > tmax || (k > 0 && t < tvals[ch * (P - 1) + k - 1]))
+        return fail(c, IMA_ERR_ARG, "chain %d: split times must be increasing in (0, tmax]", ch);
+    }
+  size_t lfn = 4;
+  for (int ch = 0; ch < m.C; ++ch) {
+    size_t tot = 4;
+    for (int l = 0; l < m.L; ++l) {
+      const SlotMeta &s = c->slots[m.first + ch * m.L + l];
+      tot += (size_t)s.n - 1 + (size_t)s.nmig;
+      emax = std::max(emax, (s.n - 1) + s.nmig + P - 1);
+    }
+    lfn = std::max(lfn, tot);
+  }
+  if (treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "a genealogy has %d prior events", emax);
+  // per-chain kernels: static swap staging (2 x 2048 doubles) + prior state
+  if (mc3_prior_lds_bytes(R, np) + 2 * 2048 * 8 + 1024 > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "prior state of %d populations exceeds LDS", P);
+  grow_logfact(c, lfn);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  std::vector<int32_t> row(N), sl(N);
+  for (int i = 0; i < N; ++i) {
+    row[i] = i / m.L;
+    sl[i] = m.first + i;
+  }
+  std::vector<double> tv((size_t)m.C * tw, 0.0);
+  if (P > 1) std::copy(tvals, tvals + (size_t)m.C * tw, tv.begin());
+  HIPCHK(c, reserve(m.tv, tv.size()));
+  HIPCHK(c, reserve(m.tvbackup, tv.size()));
+  HIPCHK(c, reserve(m.wrec, (size_t)N * R));
+  HIPCHK(c, reserve(m.wnew, (size_t)N * R));
+  HIPCHK(c, reserve(m.wall, (size_t)m.C * R));
+  HIPCHK(c, reserve(m.qterm, (size_t)m.C * std::max(np, 1)));
+  HIPCHK(c, reserve(m.qf, (size_t)m.C * std::max(np, 1)));
+  HIPCHK(c, reserve(m.qc, (size_t)m.C * std::max(np, 1)));
+  HIPCHK(c, reserve(m.probg, m.C));
+  HIPCHK(c, reserve(m.mtbackup, c->migs + 1));
+  HIPCHK(c, reserve(m.tv_row, N));
+  HIPCHK(c, reserve(m.tw_slots, N));
+  HIPCHK(c, reserve(m.wstatus, N));
+  HIPCHK(c, hipMemcpyAsync(m.tv.p, tv.data(), tv.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(m.tv_row.p, row.data(), N * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(m.tw_slots.p, sl.data(), N * 4, hipMemcpyHostToDevice, c->stream));
+  MC3Dev &d = m.dev;
+  d.prior = model;
+  d.P = P;
+  d.R = R;
+  d.nparam = np;
+  d.emax = emax;
+  d.tmax = tmax;
+  d.tv = m.tv.p;
+  d.tvbackup = m.tvbackup.p;
+  d.wrec = m.wrec.p;
+  d.wnew = m.wnew.p;
+  d.wall = m.wall.p;
+  d.qterm = m.qterm.p;
+  d.qf = m.qf.p;
+  d.qc = m.qc.p;
+  d.probg = m.probg.p;
+  d.mtbackup = m.mtbackup.p;
+  d.tv_row = m.tv_row.p;
+  d.tw_slots = m.tw_slots.p;
+  d.wstatus = m.wstatus.p;
+  HIPCHK(c, enqueue_mc3_prior_init(view(c), d, c->stream));
+  std::vector<int32_t> st(N);
+  HIPCHK(c, hipMemcpyAsync(st.data(), m.wstatus.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < N; ++i)
+    if (st[i] != IMA_STATUS_OK) {
+      d.prior = -1;
+      return fail(c, IMA_ERR_ARG, "MC3 slot %d: population labels or migrations do not fit the model",
+                  m.first + i);
+    }
+  // graphs captured before hold the sweep without the prior
+  for (int k = 0; k < 2; ++k) {
+    if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
+    if (m.graph[k]) hipGraphDestroy(m.graph[k]);
+    m.exec[k] = nullptr;
+    m.graph[k] = nullptr;
+  }
+  return IMA_OK;
+}
+
+int ima_mc3_read_prior(ima_ctx *c, double *probg, double *tvals) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
+  hipSetDevice(c->device);
+  if (probg) HIPCHK(c, hipMemcpy(probg, m.probg.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
+  if (tvals && m.dev.P > 1)
+    HIPCHK(c, hipMemcpy(tvals, m.tv.p, (size_t)m.C * (m.dev.P - 1) * 8, hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
+int ima_slots_read_mig(ima_ctx *c, int slot, double *mig_time) {
+  if (!c || !mig_time || slot < 0 || slot >= (int)c->slots.size()) return IMA_ERR_ARG;
+  const SlotMeta &s = c->slots[slot];
+  if (s.nmig)
+    HIPCHK(c, hipMemcpy(mig_time, c->d_mig_time.p + s.mig_off, (size_t)s.nmig * 8,
+                        hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
+// -------------------------------------------------------- genealogy prior --
+
+int ima_model_add(ima_ctx *c, int npops, const int32_t *plist, const int32_t *npresent,
+                  const int32_t *popdown, const int32_t *pope, const int32_t *join,
+                  int npsize, const int32_t *theta_off, const int32_t *theta_wp, int nmig,
+                  const int32_t *mig_off, const int32_t *mig_wp, const double *qmax,
+                  const double *mmax, int *model) {
+  if (!c || !plist || !npresent || !popdown || !pope || !join || !theta_off || !mig_off ||
+      !model || npsize < 0 || nmig < 0 || (npsize && (!theta_wp || !qmax)) ||
+      (nmig && (!mig_wp || !mmax)))
+    return IMA_ERR_ARG;
+  const int P = npops, T = 2 * npops - 1;
+  if (P < 1 || P > kMaxPops) return fail(c, IMA_ERR_SIZE, "npops %d outside 1..%d", P, kMaxPops);
+  for (int k = 0; k < P; ++k) {
+    if (npresent[k] < 1 || npresent[k] > P)
+      return fail(c, IMA_ERR_ARG, "npresent[%d] = %d", k, npresent[k]);
+    for (int i = 0; i < npresent[k]; ++i)
+      if (plist[k * P + i] < 0 || plist[k * P + i] >= T)
+        return fail(c, IMA_ERR_ARG, "plist[%d][%d] = %d", k, i, plist[k * P + i]);
+  }
+  for (int q = 0; q < T; ++q)
+    if (popdown[q] < -1 || popdown[q] >= T || pope[q] < -1 || pope[q] > P)
+      return fail(c, IMA_ERR_ARG, "population %d: popdown/pope out of range", q);
+  for (int k = 0; k + 1 < P; ++k)
+    for (int x = 0; x < 3; ++x)
+      if (join[3 * k + x] < 0 || join[3 * k + x] >= T)
+        return fail(c, IMA_ERR_ARG, "join[%d] out of range", k);
+  if (theta_off[0] != 0 || mig_off[0] != 0) return fail(c, IMA_ERR_ARG, "parameter maps must start at 0");
+  for (int t = 0; t < npsize; ++t) {
+    if (theta_off[t + 1] < theta_off[t]) return fail(c, IMA_ERR_ARG, "theta_off not monotone");
+    if (!(qmax[t] > 0)) return fail(c, IMA_ERR_ARG, "qmax[%d] must be > 0", t);
+    for (int w = theta_off[t]; w < theta_off[t + 1]; ++w) {
+      const int k = theta_wp[2 * w], i = theta_wp[2 * w + 1];
+      if (k < 0 || k >= P || i < 0 || i >= npresent[k])
+        return fail(c, IMA_ERR_ARG, "theta parameter %d: bad entry (%d, %d)", t, k, i);
+    }
+  }
+  for (int m = 0; m < nmig; ++m) {
+    if (mig_off[m + 1] < mig_off[m]) return fail(c, IMA_ERR_ARG, "mig_off not monotone");
+    if (!(mmax[m] > 0)) return fail(c, IMA_ERR_ARG, "mmax[%d] must be > 0", m);
+    for (int w = mig_off[m]; w < mig_off[m + 1]; ++w) {
+      const int k = mig_wp[3 * w], i = mig_wp[3 * w + 1], j = mig_wp[3 * w + 2];
+      if (k < 0 || k >= P || i < 0 || i >= npresent[k] || j < 0 || j >= npresent[k] || i == j)
+        return fail(c, IMA_ERR_ARG, "migration parameter %d: bad entry (%d, %d, %d)", m, k, i, j);
+    }
+  }
+  ModelMeta mm{};
+  mm.npops = P;
+  mm.npsize = npsize;
+  mm.nmig = nmig;
+  mm.ioff = (int64_t)c->mint.size();
+  mm.doff = (int64_t)c->mdbl.size();
+  std::vector<int32_t> &I = c->mint;
+  I.insert(I.end(), plist, plist + P * P);
+  I.insert(I.end(), npresent, npresent + P);
+  I.insert(I.end(), popdown, popdown + T);
+  I.insert(I.end(), pope, pope + T);
+  for (int k = 0; k < P; ++k)
+    for (int x = 0; x < 3; ++x) I.push_back(k + 1 < P ? join[3 * k + x] : -1);
+  for (int k = 0; k < P; ++k)
+    for (int q = 0; q < T; ++q) {
+      int pos = -1;
+      for (int i = 0; i < npresent[k]; ++i)
+        if (plist[k * P + i] == q) pos = i;
+      I.push_back(pos);
+    }
+  I.insert(I.end(), theta_off, theta_off + npsize + 1);
+  I.insert(I.end(), theta_wp, theta_wp + 2 * theta_off[npsize]);
+  I.insert(I.end(), mig_off, mig_off + nmig + 1);
+  I.insert(I.end(), mig_wp, mig_wp + 3 * mig_off[nmig]);
+  c->mdbl.insert(c->mdbl.end(), qmax, qmax + npsize);
+  c->mdbl.insert(c->mdbl.end(), mmax, mmax + nmig);
+  *model = (int)c->models.size();
+  c->models.push_back(mm);
+  c->models_dirty = true;
+  return IMA_OK;
+}
+
+int ima_locus_set_model(ima_ctx *c, int locus, int model, const int32_t *samppop, double hval) {
+  if (!c || !samppop || locus < 0 || locus >= (int)c->loci.size() || model < 0 ||
+      model >= (int)c->models.size() || !(hval > 0))
+    return IMA_ERR_ARG;
+  LocusMeta &lm = c->loci[locus];
+  const int P = c->models[model].npops;
+  int tot = 0;
+  for (int p = 0; p < P; ++p) {
+    if (samppop[p] < 0) return fail(c, IMA_ERR_ARG, "samppop[%d] < 0", p);
+    tot += samppop[p];
+  }
+  if (tot != lm.n) return fail(c, IMA_ERR_ARG, "samppop sums to %d, locus has %d gene copies", tot, lm.n);
+  lm.pmodel = model;
+  for (int p = 0; p < kMaxPops; ++p) lm.samppop[p] = p < P ? samppop[p] : 0;
+  lm.hval = hval;
+  c->loci_dirty = true;
+  return IMA_OK;
+}
+
+int ima_slots_write_pops(ima_ctx *c, int first, int count, const int32_t *pop,
+                         const int32_t *nmig, const int32_t *mig_edge, const double *mig_time,
+                         const int32_t *mig_to) {
+  if (!c || !pop || !nmig) return IMA_ERR_ARG;
+  if (first < 0 || count < 0 || first + count > (int)c->slots.size())
+    return fail(c, IMA_ERR_ARG, "slot range [%d,%d) out of range", first, first + count);
+  if (count == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  size_t src = 0, msrc = 0;
+  for (int i = 0; i < count; ++i) {
+    SlotMeta &s = c->slots[first + i];
+    const LocusMeta &lm = c->loci[s.locus];
+    if (lm.pmodel < 0) return fail(c, IMA_ERR_STATE, "slot %d: its locus has no population model", first + i);
+    const int T = 2 * c->models[lm.pmodel].npops - 1, ne = 2 * s.n - 1;
+    if (nmig[i] < 0) return fail(c, IMA_ERR_ARG, "slot %d: nmig < 0", first + i);
+    if (nmig[i] && (!mig_edge || !mig_time || !mig_to)) return IMA_ERR_ARG;
+    for (int e = 0; e < ne; ++e)
+      if (pop[src + e] < 0 || pop[src + e] >= T)
+        return fail(c, IMA_ERR_ARG, "slot %d: edge %d in population %d", first + i, e, pop[src + e]);
+    for (int j = 0; j < nmig[i]; ++j) {
+      const size_t q = msrc + j;
+      if (mig_edge[q] < 0 || mig_edge[q] >= ne || mig_to[q] < 0 || mig_to[q] >= T ||
+          !(mig_time[q] >= 0) || !std::isfinite(mig_time[q]))
+        return fail(c, IMA_ERR_ARG, "slot %d: migration %d out of range", first + i, j);
+      if (j > 0 && (mig_edge[q] < mig_edge[q - 1] ||
+                    (mig_edge[q] == mig_edge[q - 1] && mig_time[q] < mig_time[q - 1])))
+        return fail(c, IMA_ERR_ARG, "slot %d: migrations not sorted by edge then time", first + i);
+    }
+    std::copy(pop + src, pop + src + ne, c->pop.begin() + s.node_off);
+    if (nmig[i] > s.mig_cap) {
+      s.mig_cap = nmig[i] + nmig[i] / 2 + 4;
+      s.mig_off = (int64_t)c->migs;
+      c->migs += (size_t)s.mig_cap;
+      c->mig_edge.resize(c->migs, 0);
+      c->mig_to.resize(c->migs, 0);
+      c->mig_time.resize(c->migs, 0.0);
+    }
+    s.nmig = nmig[i];
+    if (nmig[i]) {
+      std::copy(mig_edge + msrc, mig_edge + msrc + nmig[i], c->mig_edge.begin() + s.mig_off);
+      std::copy(mig_to + msrc, mig_to + msrc + nmig[i], c->mig_to.begin() + s.mig_off);
+      std::copy(mig_time + msrc, mig_time + msrc + nmig[i], c->mig_time.begin() + s.mig_off);
+    }
+    src += ne;
+    msrc += nmig[i];
+  }
+  const SlotMeta &s0 = c->slots[first];
+  const SlotMeta &s1 = c->slots[first + count - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true));
+  HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s0.node_off, c->pop.data() + s0.node_off, nb * 4,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true));
+  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true));
+  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true));
+  if (c->migs) {
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p, c->mig_edge.data(), c->migs * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p, c->mig_to.data(), c->migs * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p, c->mig_time.data(), c->migs * 8,
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->slots_meta_dirty = true;
+  return IMA_OK;
+}
+
+// Checks the slots of one prior call: returns their shared model, its P,
+// the largest event count of a genealogy (*emax) and the log-factorial
+// table length the groups' summed counts need (*lf_need).
+static int prior_prepare(ima_ctx *c, const int32_t *slots, int nslots, int *model, int *P, int *emax,
+                  size_t *lf_need, const int32_t *group_off, int ngroups) {
+  *model = -1;
+  *emax = 1;
+  for (int i = 0; i < nslots; ++i) {
+    const int sl = slots[i];
+    if (sl < 0 || sl >= (int)c->slots.size()) return fail(c, IMA_ERR_ARG, "slot %d out of range", sl);
+    const SlotMeta &s = c->slots[sl];
+    if (s.root < 0) return fail(c, IMA_ERR_STATE, "slot %d holds no genealogy", sl);
+    const LocusMeta &lm = c->loci[s.locus];
+    if (lm.pmodel < 0) return fail(c, IMA_ERR_STATE, "slot %d: locus has no population model", sl);
+    if (*model < 0) *model = lm.pmodel;
+    if (lm.pmodel != *model) return fail(c, IMA_ERR_ARG, "slots of one call must share a model");
+    *emax = std::max(*emax, (s.n - 1) + s.nmig + c->models[*model].npops - 1);
+  }
+  *P = *model >= 0 ? c->models[*model].npops : 1;
+  const size_t lds = treeweight_lds_bytes(*P, *emax);
+  if (lds > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "a genealogy has %d prior events (LDS %zu B)", *emax, lds);
+  *lf_need = 4;
+  for (int g = 0; g < ngroups; ++g) {
+    size_t tot = 4;
+    for (int i = group_off[g]; i < group_off[g + 1]; ++i) {
+      const SlotMeta &s = c->slots[slots[i]];
+      tot += (size_t)s.n - 1 + (size_t)s.nmig;
+    }
+    *lf_need = std::max(*lf_need, tot);
+  }
+  return IMA_OK;
+}
+
+int ima_treeweight(ima_ctx *c, const int32_t *slots, int nslots, const double *tvals,
+                   int32_t *cc, double *fc, double *hcc, int32_t *mc, double *fm,
+                   double *length) {
+  if (!c || (!slots && nslots > 0) || nslots < 0) return IMA_ERR_ARG;
+  if (nslots == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  int model, P, emax, rc;
+  size_t lfn;
+  const int32_t goff[2] = {0, nslots};
+  if ((rc = prior_prepare(c, slots, nslots, &model, &P, &emax, &lfn, goff, 1)) != IMA_OK) return rc;
+  if (P > 1 && !tvals) return IMA_ERR_ARG;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  const int R = prior_rec_len(P), tw = P > 1 ? P - 1 : 1;
+  HIPCHK(c, reserve(c->d_pslots, nslots));
+  HIPCHK(c, reserve(c->d_tv, (size_t)nslots * tw));
+  HIPCHK(c, reserve(c->d_rec, (size_t)nslots * R));
+  HIPCHK(c, reserve(c->d_pstatus, nslots));
+  HIPCHK(c, hipMemcpyAsync(c->d_pslots.p, slots, nslots * 4, hipMemcpyHostToDevice, c->stream));
+  if (P > 1)
+    HIPCHK(c, hipMemcpyAsync(c->d_tv.p, tvals, (size_t)nslots * tw * 8, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, launch_treeweight(view(c), c->d_pslots.p, nslots, c->d_tv.p, nullptr, P, emax,
+                              c->d_rec.p, c->d_pstatus.p, c->stream));
+  std::vector<double> rec((size_t)nslots * R);
+  std::vector<int32_t> st(nslots);
+  HIPCHK(c, hipMemcpyAsync(rec.data(), c->d_rec.p, rec.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_pstatus.p, nslots * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nslots; ++i)
+    if (st[i] != IMA_STATUS_OK)
+      return fail(c, IMA_ERR_ARG, "slot %d: population labels or migrations do not fit the model",
+                  slots[i]);
+  const int P2 = P * P, P3 = P * P * P;
+  for (int i = 0; i < nslots; ++i) {
+    const double *r = rec.data() + (size_t)i * R;
+    for (int x = 0; x < P2; ++x) {
+      if (cc) cc[(size_t)i * P2 + x] = (int32_t)r[x];
+      if (fc) fc[(size_t)i * P2 + x] = r[P2 + x];
+      if (hcc) hcc[(size_t)i * P2 + x] = r[2 * P2 + x];
+    }
+    for (int x = 0; x < P3; ++x) {
+      if (mc) mc[(size_t)i * P3 + x] = (int32_t)r[3 * P2 + x];
+      if (fm) fm[(size_t)i * P3 + x] = r[3 * P2 + P3 + x];
+    }
+    if (length) length[i] = r[R - 1];
+  }
+  return IMA_OK;
+}
+
+int ima_prior(ima_ctx *c, const int32_t *slots, const int32_t *group_off, int ngroups,
+              const double *tvals, double *probg, double *qint, double *mint) {
+  if (!c || !group_off || ngroups < 0 || !probg) return IMA_ERR_ARG;
+  if (ngroups == 0) return IMA_OK;
+  if (group_off[0] != 0) return fail(c, IMA_ERR_ARG, "group_off[0] must be 0");
+  for (int g = 0; g < ngroups; ++g)
+    if (group_off[g + 1] <= group_off[g]) return fail(c, IMA_ERR_ARG, "group %d is empty", g);
+  const int nslots = group_off[ngroups];
+  if (!slots) return IMA_ERR_ARG;
+  hipSetDevice(c->device);
+  int model, P, emax, rc;
+  size_t lfn;
+  if ((rc = prior_prepare(c, slots, nslots, &model, &P, &emax, &lfn, group_off, ngroups)) != IMA_OK)
+    return rc;
+  if (P > 1 && !tvals) return IMA_ERR_ARG;
+  grow_logfact(c, lfn);
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  const ModelMeta &mm = c->models[model];
+  const int R = prior_rec_len(P), tw = P > 1 ? P - 1 : 1;
+  std::vector<int32_t> row(nslots);
+  for (int g = 0; g < ngroups; ++g)
+    for (int i = group_off[g]; i < group_off[g + 1]; ++i) row[i] = g;
+  HIPCHK(c, reserve(c->d_pslots, nslots));
+  HIPCHK(c, reserve(c->d_prow, nslots));
+  HIPCHK(c, reserve(c->d_goff, ngroups + 1));
+  HIPCHK(c, reserve(c->d_tv, (size_t)ngroups * tw));
+  HIPCHK(c, reserve(c->d_rec, (size_t)nslots * R));
+  HIPCHK(c, reserve(c->d_pstatus, nslots));
+  HIPCHK(c, reserve(c->d_probg, ngroups));
+  HIPCHK(c, reserve(c->d_qint, (size_t)ngroups * std::max(1, mm.npsize)));
+  HIPCHK(c, reserve(c->d_mintg, (size_t)ngroups * std::max(1, mm.nmig)));
+  HIPCHK(c, hipMemcpyAsync(c->d_pslots.p, slots, nslots * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_prow.p, row.data(), nslots * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_goff.p, group_off, (ngroups + 1) * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  if (P > 1)
+    HIPCHK(c, hipMemcpyAsync(c->d_tv.p, tvals, (size_t)ngroups * tw * 8, hipMemcpyHostToDevice,
+                             c->stream));
+  const DeviceView v = view(c);
+  HIPCHK(c, launch_treeweight(v, c->d_pslots.p, nslots, c->d_tv.p, c->d_prow.p, P, emax,
+                              c->d_rec.p, c->d_pstatus.p, c->stream));
+  HIPCHK(c, launch_prior(v, model, P, mm.npsize + mm.nmig, c->d_rec.p, c->d_goff.p, ngroups,
+                         c->d_probg.p, c->d_qint.p, c->d_mintg.p, c->stream));
+  std::vector<int32_t> st(nslots);
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_pstatus.p, nslots * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(probg, c->d_probg.p, ngroups * 8, hipMemcpyDeviceToHost, c->stream));
+  if (qint && mm.npsize)
+    HIPCHK(c, hipMemcpyAsync(qint, c->d_qint.p, (size_t)ngroups * mm.npsize * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (mint && mm.nmig)
+    HIPCHK(c, hipMemcpyAsync(mint, c->d_mintg.p, (size_t)ngroups * mm.nmig * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nslots; ++i)
+    if (st[i] != IMA_STATUS_OK)
+      return fail(c, IMA_ERR_ARG, "slot %d: population labels or migrations do not fit the model",
+                  slots[i]);
   return IMA_OK;
 }
 

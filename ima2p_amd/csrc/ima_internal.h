@@ -16,6 +16,7 @@ constexpr int kMaxGenes = IMA_MAX_GENES;
 constexpr int kMaxEdges = 2 * kMaxGenes - 1;
 constexpr int kTipSlot = 0xFFFF;   // program operand that is a tip
 constexpr double kLn2 = 0.69314718055994530941723212146;
+constexpr int kMaxPops = 10;       // sampled populations (the reference's MAXPOPS)
 
 // Per-locus metadata resident on the device.
 struct LocusMeta {
@@ -30,6 +31,9 @@ struct LocusMeta {
   int32_t Spad;      // HKY: row stride of seq (S rounded up to 64, zero padded)
   double pi[4];
   double sumlogk;
+  int32_t pmodel;               // population model of the prior, -1 if none
+  int32_t samppop[kMaxPops];    // gene copies sampled per population
+  double hval;                  // inheritance scalar h
 };
 
 // Per-slot metadata: where the genealogy and its postorder program live.
@@ -40,7 +44,32 @@ struct SlotMeta {
   int32_t nslots;    // LDS partial slots the program needs
   int64_t node_off;  // into up0/up1/down/time/A arrays (2n-1 entries)
   int64_t prog_off;  // into prog (n-1 ops)
+  int64_t mig_off;   // into mig_edge/mig_time/mig_to (events sorted by edge, time)
+  int32_t nmig;      // migration events of the genealogy
+  int32_t mig_cap;   // events the slot's region holds
 };
+
+// Population model of the genealogy prior.  Periods k = 0..P-1 lie between
+// split times; populations 0..2P-2 (sampled ones first, then ancestral
+// P + k created by the split closing period k).  The model's ints live in
+// one block at ioff:
+//   plist[P][P] (-1 padded) | npresent[P] | popdown[2P-1] | pope[2P-1] |
+//   join[P][3] = (a, b, new) | pos[P][2P-1] (position in plist or -1) |
+//   theta_off[npsize+1] | theta_wp[][2] = (k, i) |
+//   mig_off[nmig+1] | mig_wp[][3] = (k, i, j)
+// and qmax[npsize] | mmax[nmig] in the double block at doff.
+struct ModelMeta {
+  int32_t npops, npsize, nmig, pad;
+  int64_t ioff, doff;
+};
+struct ModelView {
+  const int32_t *plist, *npresent, *popdown, *pope, *join, *pos;
+  const int32_t *theta_off, *theta_wp, *mig_off, *mig_wp;
+  const double *qmax, *mmax;
+};
+// Doubles per genealogy weight record:
+//   cc[P][P] | fc[P][P] | hcc[P][P] | mc[P][P][P] | fm[P][P][P] | length
+__host__ __device__ inline int prior_rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
 
 // One postorder operation of the HKY pruning program: compute the partial
 // at the top of edge `node` from daughters a and b.  The program is a stack
@@ -82,7 +111,37 @@ struct DeviceView {
   const int8_t *seq;
   const int32_t *mult;
   const uint64_t *mask;
+  const int32_t *pop;        // population of each edge at its lower end
+  const int32_t *mig_edge, *mig_to;
+  const double *mig_time;
+  const ModelMeta *models;
+  const int32_t *mint;
+  const double *mdbl;
+  const double *logfact;     // log(k!) by running sums of log(i)
 };
+
+__host__ __device__ inline ModelView model_view(const int32_t *mi, const double *mdb,
+                                                const ModelMeta &m) {
+  const int P = m.npops, T = 2 * P - 1;
+  ModelView r;
+  const int32_t *b = mi + m.ioff;
+  r.plist = b;
+  r.npresent = r.plist + P * P;
+  r.popdown = r.npresent + P;
+  r.pope = r.popdown + T;
+  r.join = r.pope + T;
+  r.pos = r.join + 3 * P;
+  r.theta_off = r.pos + P * T;
+  r.theta_wp = r.theta_off + m.npsize + 1;
+  r.mig_off = r.theta_wp + 2 * r.theta_off[m.npsize];
+  r.mig_wp = r.mig_off + m.nmig + 1;
+  r.qmax = mdb + m.doff;
+  r.mmax = r.qmax + m.npsize;
+  return r;
+}
+__device__ inline ModelView model_view(const DeviceView &v, const ModelMeta &m) {
+  return model_view(v.mint, v.mdbl, m);
+}
 
 // HKY work plan: one wavefront per (item, 64-pattern block).  units[k] =
 // {item, block, first partial, blocks of the item}; items with more than
@@ -129,6 +188,15 @@ hipError_t launch_margincalc(const LTable &t, int param, const double *x,
 size_t margincalc_work_bytes(const LTable &t, int nx);
 hipError_t upload_pow10_table(hipStream_t st);
 
+// Genealogy prior (kern_prior.hip).
+size_t treeweight_lds_bytes(int P, int emax);
+hipError_t launch_treeweight(const DeviceView &v, const int32_t *slots, int nslots,
+                             const double *tvals, const int32_t *tv_row, int P, int emax,
+                             double *rec_out, int32_t *status, hipStream_t st);
+hipError_t launch_prior(const DeviceView &v, int model, int P, int nparams, const double *recs,
+                        const int32_t *group_off, int ngroups, double *probg, double *qint,
+                        double *mint, hipStream_t st);
+
 // Device-resident MC3 state (kern_mc3.hip).
 struct MC3Dev {
   int C, L;
@@ -153,7 +221,26 @@ struct MC3Dev {
   unsigned *done;        // chains finished in the current sweep (last one swaps)
   uint64_t seed;
   double scale_width;
+  // genealogy prior (prior < 0: the sweep scores the data likelihood only)
+  int prior;             // population model id
+  int P, R, nparam;      // populations, weight record length, parameters
+  int emax;              // largest event count of a genealogy
+  double tmax;           // split times: uniform prior on [0, tmax]
+  double *tv;            // [C][TW] split times of each chain (TW = max(P-1, 1))
+  double *tvbackup;      // [C][TW]
+  double *wrec;          // [C*L][R] committed genealogy weights
+  double *wnew;          // [C*L][R] weights of the proposals
+  double *wall;          // [C][R] all-loci sums (allgweight)
+  double *qterm;         // [C][nparam] integrals of the committed sums
+  int32_t *qc;           // [C][nparam] counts those integrals were taken at
+  double *qf;            // [C][nparam] weight sums those integrals were taken at
+  double *probg;         // [C]
+  double *mtbackup;      // migration times backup (indexed like mig_time)
+  int32_t *tv_row;       // [C*L] chain of each item
+  int32_t *tw_slots;     // [C*L] slot of each item
+  int32_t *wstatus;      // [C*L] weights status of the proposals
 };
+__host__ __device__ inline int mc3_tw(int P) { return P > 1 ? P - 1 : 1; }
 
 hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_scale,
                              const ima_item *items, const int32_t *idx, const HkyPlan &plan,
@@ -162,6 +249,8 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              hipStream_t st);
 
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+size_t mc3_prior_lds_bytes(int R, int nparam);
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 
 // Swap (kern_swap.hip).

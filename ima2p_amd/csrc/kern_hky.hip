@@ -26,8 +26,8 @@
 // daughter with partial L:  (P L)_j = a W + b_c W_c + eA_c L_j with a = 1-e,
 // b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.  For a
 // tip in state s the vector is column s of P.  Per branch the five
-// coefficients and the four columns of P are computed once per item in FP64
-// and kept in LDS; a tip operand is one 16-byte LDS read.  Partials are
+// coefficients, and per tip branch the four columns of P, are computed once
+// per item in FP64 and kept in LDS; a tip operand is one 16-byte LDS read.  Partials are
 // rescaled at every node by an exact power of two; one integer exponent per
 // lane carries the sum of the node scale factors (the reference keeps
 // log(max) per node, :211-217, 465).
@@ -175,7 +175,8 @@ __device__ __forceinline__ int apply(const Nxt<real> &c, Q4<real> (&stk)[D], int
 size_t hky_lds_bytes(int precision, int max_n) {
   const size_t ne = 2 * (size_t)max_n - 1;
   const size_t r = precision == IMA_PREC_FP32 ? 4 : 8;
-  return ne * (8 * r) + ne * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) +
+  // coefficients per edge, P columns per tip edge, program, tip states
+  return ne * (8 * r) + (size_t)max_n * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) +
          (size_t)max_n * kWave;
 }
 
@@ -202,8 +203,8 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   const int n = sm.n, ne = 2 * n - 1, S = lm.S, nops = n - 1;
 
   HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);           // [ne]
-  Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);                 // [ne][4]
-  ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)ne * 4);          // [nops]
+  Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);                 // [n][4] tips
+  ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)n * 4);           // [nops]
   int8_t *tips = reinterpret_cast<int8_t *>(ops + nops);                    // [n][64]
 
   const double *pid = lm.pi;
@@ -235,6 +236,7 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
     c.eAY = real(eY);
     c.pad0 = c.pad1 = c.pad2 = real(0);
     coef[e] = c;
+    if (e >= n) continue;  // P columns are read for tip operands only
     // column s of P: P_js = pi_s a + [cls j = cls s] pi_s b_c + [j = s] eA_c
     for (int s = 0; s < 4; ++s) {
       const bool Rs = (s & 1) == 0;

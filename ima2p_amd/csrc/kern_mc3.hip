@@ -11,31 +11,56 @@
 //
 // Proposals.  The genealogy proposal moves the time of one coalescence
 // (chosen uniformly) by a normal step reflected into the interval bounded
-// by its daughters' coalescences and its parent's; the step scale depends
-// only on those bounds, so the move is symmetric.  Topology is unchanged.
-// The rescaling multiplies every coalescence time of a chain's genealogies
-// by s = exp(d (U - 1/2)), with Hastings term (n-1) log s per genealogy.
+// by its daughters' coalescences and its parent's; with the genealogy prior
+// on, also by the period it lies in (split times) and by the migrations on
+// its daughter edges (below) and its parent edge (above), so every
+// population label and migration stays valid.  The step scale depends only
+// on the bounds, so the move is symmetric.  Topology and migrations are
+// unchanged.  The rescaling multiplies every coalescence time of a chain's
+// genealogies -- and with the prior every migration time and split time --
+// by s = exp(d (U - 1/2)), with Hastings term (number of scaled times) log s.
 // Neither is the reference's proposal (slider/addmigration); they exercise
 // the same scoring, acceptance, commit/rollback and swap path.
 //
-// Kernels per sweep: the batched likelihood, then one 1024-thread workgroup
-// per chain that accepts or rejects every locus of its chain (merging the
+// Acceptance (update_gtree.cpp:919-927): U < min(1, exp(beta (dprobg +
+// dpdg))).  Without the prior dprobg = 0 and the loci of a chain are
+// independent given its parameters, so they are decided in parallel.  With
+// the prior, loci are coupled through the chain's all-loci weight sums
+// (allgweight), so one wavefront per chain takes the loci in order: the sums
+// become max(0, (all - old) + new) as sum_subtract_treeinfo forms them
+// (ginfo.cpp:249-285), a parameter's integral is recomputed only when its
+// (count, weight) changed (integrate_tree_prob's reuse, update_gtree_common
+// .cpp:1997-2006), and a rejection restores the held sums.
+//
+// Kernels per sweep: the batched likelihood (plus the genealogy weights of
+// the proposals with the prior on), then one 1024-thread workgroup per
+// chain that accepts or rejects every locus of its chain (merging the
 // likelihood's pattern-block partials in block order), applies the
 // rescaling or computes the chain sum, and draws the next sweep's
 // genealogy proposals.  The workgroup that finishes last (device-scope
 // fence + counter) performs the swap.  A rescaling sweep adds a second
-// likelihood launch and a per-chain commit kernel.  Every reduction has a
-// fixed order, so results do not depend on scheduling.
+// scoring pass and a per-chain commit kernel.  Every reduction has a fixed
+// order, so results do not depend on scheduling.
 #include <float.h>
 
 #include "ima_internal.h"
 #include "philox.h"
+#include "prior_math.h"
 
 namespace ima {
 
+// Every (key, sweep, phase) stream gets its own range of 2^16 counter
+// blocks: a stream advances its counter by one per four 32-bit words, so
+// streams whose starting counters were adjacent would share blocks (and
+// hand the same uniform to, say, a proposal and its acceptance test).
+constexpr int kStreamBits = 16;
+__device__ inline uint64_t stream_counter(uint64_t sweep, int phase) {
+  return ((sweep << 3) | (uint64_t)phase) << kStreamBits;
+}
+
 __device__ inline PhiloxStream item_rng(const MC3Dev &m, int item, uint64_t sweep, int phase) {
   return PhiloxStream(m.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(item + 1)),
-                      sweep * 8 + (uint64_t)phase);
+                      stream_counter(sweep, phase));
 }
 
 __device__ inline double normal01(PhiloxStream &r) {
@@ -64,12 +89,27 @@ __device__ void propose_item(const DeviceView &v, const MC3Dev &m, int it, uint6
   const double T = tm[a];
   const double la = (a >= n) ? tm[up0[a]] : 0.0;
   const double lb = (b >= n) ? tm[up0[b]] : 0.0;
-  const double lo = fmax(la, lb);
-  const bool root = dn[e] == -1;
-  const double hi = root ? DBL_MAX : tm[e];
-  const double sigma = root ? 0.5 * fmax(lo, 0.05) : 0.5 * (hi - lo);
+  double lo = fmax(la, lb);
+  double hi = (dn[e] == -1) ? DBL_MAX : tm[e];
+  if (m.prior >= 0) {
+    // stay inside the period and between the migrations around the node
+    const double *tv = m.tv + (int64_t)(it / m.L) * mc3_tw(m.P);
+    int k = 0;
+    while (k < m.P - 1 && tv[k] <= T) k++;
+    if (k > 0) lo = fmax(lo, tv[k - 1]);
+    if (k < m.P - 1) hi = fmin(hi, tv[k]);
+    const int32_t *me = v.mig_edge + sm.mig_off;
+    const double *mt = v.mig_time + sm.mig_off;
+    for (int j = 0; j < sm.nmig; ++j) {
+      const int x = me[j];
+      if (x == a || x == b) lo = fmax(lo, mt[j]);
+      else if (x == e) hi = fmin(hi, mt[j]);
+    }
+  }
+  const bool open = hi == DBL_MAX;
+  const double sigma = open ? 0.5 * fmax(lo, 0.05) : 0.5 * (hi - lo);
   double Tn = T + sigma * normal01(r);
-  if (root) {
+  if (open) {
     Tn = lo + fabs(Tn - lo);
   } else {
     const double w = hi - lo;
@@ -158,7 +198,7 @@ __device__ void swap_chains(const MC3Dev &m, int swaptries, uint64_t sweep, doub
     }
   __syncthreads();
   if (threadIdx.x == 0) {
-    PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, sweep);
+    PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, stream_counter(sweep, 4));
     const int SWAPDIST = 7, nchains = m.C;
     const double *S = staged ? Ssh : m.S;
     double *beta = staged ? Bsh : m.beta;
@@ -214,34 +254,204 @@ __device__ void finish_chain(const MC3Dev &m, int c, double S, int swaptries, ui
   }
 }
 
+// Intra-wavefront ordering of LDS traffic (the accept loop runs in one wave).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Entry r of the all-loci sums after replacing one genealogy's weights
+// (sum_subtract_treeinfo, ginfo.cpp:249-285): counts add the difference,
+// fc / fm subtract then add and stay >= 0, hcc subtracts then adds.
+__device__ inline double wsum_update(int r, int P, double all, double old, double nw) {
+  const int P2 = P * P, P3 = P2 * P;
+  if (r < P2) return all + (nw - old);
+  if (r < 2 * P2) return fmax(0.0, (all - old) + nw);
+  if (r < 3 * P2) return (all - old) + nw;
+  if (r < 3 * P2 + P3) return all + (nw - old);
+  if (r < 3 * P2 + 2 * P3) return fmax(0.0, (all - old) + nw);
+  return (all - old) + nw;
+}
+
+// Dynamic LDS of the per-chain prior kernels: all[R] hold[R] | qt qf nt nf
+// [nparam] doubles | qc nc [nparam] ints.
+__host__ __device__ inline size_t mc3_prior_lds(int R, int np) {
+  return (size_t)(2 * R + 4 * np) * 8 + (size_t)2 * np * 4;
+}
+
+// All-loci sums of chain c from the records w (loci in order), every
+// parameter's (count, weight, integral) and probg = sum of the integrals;
+// called by the whole workgroup.
+__device__ double chain_prior_fresh(const DeviceView &v, const MC3Dev &m, int c,
+                                    const double *w, double *all, double *nt, double *nf,
+                                    int *nc) {
+  __shared__ double pg;
+  const int R = m.R, np = m.nparam;
+  const ModelMeta mm = v.models[m.prior];
+  const ModelView md = model_view(v, mm);
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    double s = 0.0;
+    for (int l = 0; l < m.L; ++l) s += w[(int64_t)(c * m.L + l) * R + r];
+    all[r] = s;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    int cc;
+    double f, h;
+    prior_param_sums(md, m.P, mm.npsize, t, all, &cc, &f, &h);
+    nc[t] = cc;
+    nf[t] = f;
+    nt[t] = prior_param_term(md, mm.npsize, t, cc, f, h, v.logfact);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double q = 0.0, mg = 0.0;
+    for (int t = 0; t < mm.npsize; ++t) q += nt[t];
+    for (int t = mm.npsize; t < np; ++t) mg += nt[t];
+    pg = q + mg;
+  }
+  __syncthreads();
+  return pg;
+}
+
+// Prior state of every chain from the committed weight records.
+__global__ __launch_bounds__(256) void mc3_prior_init(DeviceView v, MC3Dev m) {
+  extern __shared__ double dyn[];
+  const int c = blockIdx.x, R = m.R, np = m.nparam;
+  double *all = dyn, *nt = all + 2 * R + 2 * np, *nf = nt + np;
+  int *nc = reinterpret_cast<int *>(nf + np) + np;
+  const double pg = chain_prior_fresh(v, m, c, m.wrec, all, nt, nf, nc);
+  for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = all[r];
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    m.qterm[c * np + t] = nt[t];
+    m.qf[c * np + t] = nf[t];
+    m.qc[c * np + t] = nc[t];
+  }
+  if (threadIdx.x == 0) m.probg[c] = pg;
+}
+
+// Loci of chain c in order, each accepted with beta (dprobg + dpdg); the
+// first wavefront does the work, the workgroup waits at the end.
+__device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
+                             double *dyn) {
+  const int R = m.R, np = m.nparam, P = m.P;
+  double *all = dyn, *hold = all + R, *qt = hold + R, *qf = qt + np, *nt = qf + np,
+         *nf = nt + np;
+  int *qc = reinterpret_cast<int *>(nf + np), *nc = qc + np;
+  const ModelMeta mm = v.models[m.prior];
+  const ModelView md = model_view(v, mm);
+  for (int r = threadIdx.x; r < R; r += blockDim.x) all[r] = m.wall[(int64_t)c * R + r];
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    qt[t] = m.qterm[c * np + t];
+    qf[t] = m.qf[c * np + t];
+    qc[t] = m.qc[c * np + t];
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const double beta = m.beta[c];
+    double probg = m.probg[c];
+    for (int l = 0; l < m.L; ++l) {
+      const int it = c * m.L + l;
+      bool acc = false;
+      if (m.status[it] == IMA_STATUS_OK && m.wstatus[it] == IMA_STATUS_OK) {
+        const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
+        for (int r = lane; r < R; r += kWave) {
+          const double a = all[r];
+          hold[r] = a;
+          all[r] = wsum_update(r, P, a, wo[r], wn[r]);
+        }
+        wave_sync();
+        for (int t = lane; t < np; t += kWave) {
+          int cc;
+          double f, h;
+          prior_param_sums(md, P, mm.npsize, t, all, &cc, &f, &h);
+          nc[t] = cc;
+          nf[t] = f;
+          nt[t] = (cc == qc[t] && f == qf[t]) ? qt[t]
+                                               : prior_param_term(md, mm.npsize, t, cc, f, h,
+                                                                  v.logfact);
+        }
+        wave_sync();
+        double q = 0.0, mg = 0.0;
+        for (int t = 0; t < mm.npsize; ++t) q += nt[t];
+        for (int t = mm.npsize; t < np; ++t) mg += nt[t];
+        const double pn = q + mg;
+        const double npd = new_loglik(m, it);
+        PhiloxStream r = item_rng(m, it, sweep, 1);
+        const double U = r.uniform();
+        acc = U < fmin(1.0, exp(beta * ((pn - probg) + (npd - m.pdg[it]))));
+        if (acc) {
+          probg = pn;
+          for (int t = lane; t < np; t += kWave) {
+            qt[t] = nt[t];
+            qf[t] = nf[t];
+            qc[t] = nc[t];
+          }
+          double *wc = m.wrec + (int64_t)it * R;
+          for (int rr = lane; rr < R; rr += kWave) wc[rr] = wn[rr];
+          if (lane == 0) m.pdg[it] = npd;
+        } else {
+          for (int rr = lane; rr < R; rr += kWave) all[rr] = hold[rr];
+        }
+        wave_sync();
+      }
+      if (lane == 0) {
+        if (!acc) restore_item(v, m, it);
+        m.acc[it] += acc ? 1u : 0u;
+      }
+    }
+    if (lane == 0) m.probg[c] = probg;
+    for (int r = lane; r < R; r += kWave) m.wall[(int64_t)c * R + r] = all[r];
+    for (int t = lane; t < np; t += kWave) {
+      m.qterm[c * np + t] = qt[t];
+      m.qf[c * np + t] = qf[t];
+      m.qc[c * np + t] = qc[t];
+    }
+  }
+  __syncthreads();
+}
+
 // One workgroup per chain.  Accept or reject every locus' genealogy
 // proposal (update_gtree.cpp:919-927; rejection restores the moved time).
-// Rescaling sweep: back up and rescale every coalescence time of the chain.
-// Otherwise: chain sum, swap by the last chain, next sweep's proposals.
+// Rescaling sweep: back up and rescale every coalescence time of the chain
+// (and with the prior its migration and split times).  Otherwise: chain
+// sum, swap by the last chain, next sweep's proposals.
 __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int with_scale,
                                                    int swaptries) {
   __shared__ double sh[1024 / kWave];
   __shared__ double Ssh[kSwapLds], Bsh[kSwapLds];
+  extern __shared__ double dyn[];
   const int c = blockIdx.x;
   const uint64_t sweep = *m.counter;
   double part = 0.0;
-  for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
-    const int it = c * m.L + l;
-    PhiloxStream r = item_rng(m, it, sweep, 1);
-    const double U = r.uniform();
-    const double np = new_loglik(m, it);
-    bool acc = false;
-    if (m.status[it] == IMA_STATUS_OK) acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it])));
-    double p = m.pdg[it];
-    if (acc) {
-      p = np;
-      m.pdg[it] = p;
-    } else {
-      restore_item(v, m, it);
+  if (m.prior < 0) {
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      const int it = c * m.L + l;
+      PhiloxStream r = item_rng(m, it, sweep, 1);
+      const double U = r.uniform();
+      const double np = new_loglik(m, it);
+      bool acc = false;
+      if (m.status[it] == IMA_STATUS_OK) acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it])));
+      double p = m.pdg[it];
+      if (acc) {
+        p = np;
+        m.pdg[it] = p;
+      } else {
+        restore_item(v, m, it);
+      }
+      m.acc[it] += acc ? 1u : 0u;
+      part += p;
+      if (!with_scale) propose_item(v, m, it, sweep + 1);
     }
-    m.acc[it] += acc ? 1u : 0u;
-    part += p;
-    if (!with_scale) propose_item(v, m, it, sweep + 1);
+  } else {
+    prior_accept(v, m, c, sweep, dyn);
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      part += m.pdg[c * m.L + l];
+      if (!with_scale) propose_item(v, m, c * m.L + l, sweep + 1);
+    }
+    if (threadIdx.x == 0) part += m.probg[c];
   }
   if (!with_scale) {
     finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
@@ -260,27 +470,60 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
     m.tbackup[j] = t;
     if (dn[j] != -1) tm[j] = t * s;
   }
+  if (m.prior >= 0) {
+    double *mt = const_cast<double *>(v.mig_time);
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      const SlotMeta sm = v.slots[m.first_slot + c * m.L + l];
+      for (int j = 0; j < sm.nmig; ++j) {
+        const double t = mt[sm.mig_off + j];
+        m.mtbackup[sm.mig_off + j] = t;
+        mt[sm.mig_off + j] = t * s;
+      }
+    }
+    const int tw = mc3_tw(m.P);
+    for (int k = threadIdx.x; k < tw; k += blockDim.x) {
+      const double t = m.tv[c * tw + k];
+      m.tvbackup[c * tw + k] = t;
+      if (k < m.P - 1) m.tv[c * tw + k] = t * s;
+    }
+  }
 }
 
 // One workgroup per chain: log acceptance of the rescaling
-// = sum_l beta (new - old) + (n_l - 1) log s; commit or restore; chain sum;
-// swap by the last chain; next sweep's proposals.
+// = sum_l beta (new - old) + (scaled times) log s [+ beta dprobg]; commit
+// or restore; chain sum; swap by the last chain; next sweep's proposals.
 __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m, int swaptries) {
   __shared__ double sh[1024 / kWave];
   __shared__ double Ssh[kSwapLds], Bsh[kSwapLds];
   __shared__ int okall, decision;
+  extern __shared__ double dyn[];
   const int c = blockIdx.x;
   const uint64_t sweep = *m.counter;
+  const bool prior = m.prior >= 0;
   if (threadIdx.x == 0) okall = 1;
   __syncthreads();
   double a = 0.0;
   for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
     const int it = c * m.L + l;
     if (m.status[it] != IMA_STATUS_OK) okall = 0;
-    const int n = v.slots[m.first_slot + it].n;
-    a += m.beta[c] * (new_loglik(m, it) - m.pdg[it]) + (n - 1) * m.logs[c];
+    if (prior && m.wstatus[it] != IMA_STATUS_OK) okall = 0;
+    const SlotMeta sm = v.slots[m.first_slot + it];
+    const int scaled = (sm.n - 1) + (prior ? sm.nmig : 0);
+    a += m.beta[c] * (new_loglik(m, it) - m.pdg[it]) + scaled * m.logs[c];
   }
   a = block_sum(a, sh);
+  const int R = m.R, np = m.nparam, tw = mc3_tw(m.P);
+  double *all = dyn, *nt = all + 2 * R + 2 * np, *nf = nt + np;
+  int *nc = reinterpret_cast<int *>(nf + np) + np;
+  double pn = 0.0;
+  if (prior) {
+    pn = chain_prior_fresh(v, m, c, m.wnew, all, nt, nf, nc);
+    a += m.beta[c] * (pn - m.probg[c]) + (m.P - 1) * m.logs[c];
+    if (threadIdx.x == 0)
+      for (int k = 0; k < m.P - 1; ++k)
+        if (m.tv[c * tw + k] > m.tmax) okall = 0;
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     PhiloxStream r = item_rng(m, m.C * m.L + m.C + c, sweep, 3);
     const double U = r.uniform();
@@ -293,9 +536,23 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   if (decision) {
     for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
       const int it = c * m.L + l;
-      const double np = new_loglik(m, it);
-      m.pdg[it] = np;
-      part += np;
+      const double np_ = new_loglik(m, it);
+      m.pdg[it] = np_;
+      part += np_;
+    }
+    if (prior) {
+      const int64_t base = (int64_t)c * m.L * R, cnt = (int64_t)m.L * R;
+      for (int64_t x = threadIdx.x; x < cnt; x += blockDim.x) m.wrec[base + x] = m.wnew[base + x];
+      for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = all[r];
+      for (int t = threadIdx.x; t < np; t += blockDim.x) {
+        m.qterm[c * np + t] = nt[t];
+        m.qf[c * np + t] = nf[t];
+        m.qc[c * np + t] = nc[t];
+      }
+      if (threadIdx.x == 0) {
+        m.probg[c] = pn;
+        part += pn;
+      }
     }
   } else {
     for (int l = threadIdx.x; l < m.L; l += blockDim.x) part += m.pdg[c * m.L + l];
@@ -303,6 +560,15 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
     chain_nodes(v, m, c, &j0, &j1);
     double *tm = const_cast<double *>(v.time) + m.node_first;
     for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) tm[j] = m.tbackup[j];
+    if (prior) {
+      double *mt = const_cast<double *>(v.mig_time);
+      for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+        const SlotMeta sm = v.slots[m.first_slot + c * m.L + l];
+        for (int j = 0; j < sm.nmig; ++j) mt[sm.mig_off + j] = m.mtbackup[sm.mig_off + j];
+      }
+      for (int k = threadIdx.x; k < tw; k += blockDim.x) m.tv[c * tw + k] = m.tvbackup[c * tw + k];
+      if (threadIdx.x == 0) part += m.probg[c];
+    }
   }
   __syncthreads();  // committed times visible before the next proposals
   for (int l = threadIdx.x; l < m.L; l += blockDim.x) propose_item(v, m, c * m.L + l, sweep + 1);
@@ -319,6 +585,8 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   // the per-chain kernels merge pattern blocks themselves
   HkyPlan p = plan;
   p.nred = 0;
+  const bool prior = m.prior >= 0;
+  const size_t dl = prior ? mc3_prior_lds(m.R, m.nparam) : 0;
   // score_ev (optional): events recorded around the two scoring passes
   auto score = [&](int pass) -> hipError_t {
     hipError_t e = hipSuccess;
@@ -330,19 +598,33 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
     if (e == hipSuccess && ns)
       e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
     if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
+    if (e == hipSuccess && prior)
+      e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax, m.wnew,
+                            m.wstatus, st);
     return e;
   };
   hipError_t e = score(0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), 0, st, v, m, with_scale ? 1 : 0,
+  hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), dl, st, v, m, with_scale ? 1 : 0,
                      swaptries);
   if (with_scale) {
     e = score(1);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mc3_scale_finish, dim3(m.C), dim3(1024), 0, st, v, m, swaptries);
+    hipLaunchKernelGGL(mc3_scale_finish, dim3(m.C), dim3(1024), dl, st, v, m, swaptries);
   }
   return hipGetLastError();
 }
+
+hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st) {
+  hipError_t e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax,
+                                   m.wrec, m.wstatus, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mc3_prior_init, dim3(m.C), dim3(256), mc3_prior_lds(m.R, m.nparam), st, v,
+                     m);
+  return hipGetLastError();
+}
+
+size_t mc3_prior_lds_bytes(int R, int np) { return mc3_prior_lds(R, np); }
 
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st) {
   const int N = m.C * m.L;

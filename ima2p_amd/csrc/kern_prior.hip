@@ -1,0 +1,257 @@
+// kern_prior.hip -- genealogy prior: per-genealogy weights (treeweight,
+// update_gtree_common.cpp:1680-1931) and their integral over the uniform
+// priors on population sizes and migration rates for a set of genealogies
+// (integrate_tree_prob, :1945-2053).
+//
+// Weights.  One wavefront per genealogy.  Every coalescence (at the time
+// of its node, in the population of the node's edge), migration (in the
+// population the lineage is in just before it, after any splits it has
+// passed) and split time below the root is an event; the events are put
+// in time order by a rank sort in LDS (rank = events earlier, ties broken
+// by index).  The sweep over the sorted events is uniform across the wave:
+// lane p follows population p -- its lineage count, and for the current
+// period its coalescence and migration counts, sum n(n-1) dt / (2h) and
+// sum n dt -- accumulating in event order exactly as the sequential sweep
+// does, so the weights match it bit for bit (no floating-point
+// contraction in this file).  Split k merges the counts of join[k] through
+// lane shuffles.
+//
+// Weight record per genealogy (doubles, P = sampled populations = periods,
+// positions i, j index plist[k]):
+//   cc[k][i] | fc[k][i] | hcc[k][i] | mc[k][i][j] | fm[k][i][j] | length
+// at offsets 0, P^2, 2P^2, 3P^2, 3P^2 + P^3, 3P^2 + 2P^3.
+//
+// Integral.  One wavefront per group of genealogies (a chain's loci):
+// lanes sum the records over the group in group order, then one lane per
+// parameter adds its (period, position) entries in parameter-map order
+// and evaluates the analytic integral; lane 0 adds the terms.
+#pragma clang fp contract(off)
+
+#include "ima_internal.h"
+#include "prior_math.h"
+
+namespace ima {
+
+namespace {
+
+enum : int { EV_COAL = 0, EV_MIG = 1, EV_SPLIT = 2 };
+
+__device__ inline int ev_pack(int type, int pop, int topop) {
+  return type | (pop << 4) | ((topop + 1) << 12);
+}
+
+__device__ inline int period_of(const double *tv, int nper, double t) {
+  int k = 0;
+  while (k < nper - 1 && tv[k] <= t) k++;
+  return k;
+}
+
+__global__ void __launch_bounds__(kWave)
+treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *tvals,
+                  const int32_t *tv_row, double *rec_out, int32_t *status) {
+  extern __shared__ double lds[];
+  const int i = blockIdx.x;
+  if (i >= nslots) return;
+  const int lane = threadIdx.x;
+  const SlotMeta sm = v.slots[slots[i]];
+  const LocusMeta &lm = v.loci[sm.locus];
+  const ModelMeta mm = v.models[lm.pmodel];
+  const ModelView md = model_view(v, mm);
+  const int P = mm.npops, nper = P, T = 2 * P - 1;
+  const int R = prior_rec_len(P);
+  const double *tv = tvals + (int64_t)(tv_row ? tv_row[i] : i) * (P > 1 ? P - 1 : 1);
+  const int n = sm.n;
+  const int32_t *up0 = v.up0 + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  const int32_t *pop = v.pop + sm.node_off;
+  const int32_t *me = v.mig_edge + sm.mig_off;
+  const int32_t *mt_to = v.mig_to + sm.mig_off;
+  const double *mtime = v.mig_time + sm.mig_off;
+  const int nmig = sm.nmig;
+
+  const double roottime = tm[up0[sm.root]];
+  const int kroot = period_of(tv, nper, roottime);
+  const int E = (n - 1) + nmig + kroot;
+
+  double *rec = lds;                       // [R]
+  double *et = rec + R;                    // [E] unsorted times
+  double *st = et + E;                     // [E] sorted times
+  int *ek = (int *)(st + E);               // [E] unsorted codes
+  int *sk = ek + E;                        // [E] sorted codes
+  __shared__ int bad;
+  if (lane == 0) bad = 0;
+  for (int r = lane; r < R; r += kWave) rec[r] = 0.0;
+
+  // events
+  for (int e = lane; e < n - 1; e += kWave) {
+    const int edge = n + e;
+    et[e] = tm[up0[edge]];
+    ek[e] = ev_pack(EV_COAL, pop[edge], -1);
+  }
+  for (int j = lane; j < nmig; j += kWave) {
+    const int edge = me[j];
+    const double t = mtime[j];
+    int now = (j > 0 && me[j - 1] == edge) ? mt_to[j - 1] : pop[edge];
+    const int k = period_of(tv, nper, t);
+    while (now >= 0 && now < T && md.pope[now] != -1 && md.pope[now] <= k) now = md.popdown[now];
+    et[n - 1 + j] = t;
+    ek[n - 1 + j] = ev_pack(EV_MIG, now, mt_to[j]);
+  }
+  for (int k = lane; k < kroot; k += kWave) {
+    et[n - 1 + nmig + k] = tv[k];
+    ek[n - 1 + nmig + k] = ev_pack(EV_SPLIT, k, -1);
+  }
+  __syncthreads();
+  for (int j = lane; j < E; j += kWave) {
+    const double t = et[j];
+    int r = 0;
+    for (int q = 0; q < E; ++q) {
+      const double u = et[q];
+      r += (u < t) || (u == t && q < j);
+    }
+    st[r] = t;
+    sk[r] = ek[j];
+  }
+  __syncthreads();
+
+  // sweep: lane p follows population p
+  const int p = lane;
+  const bool mine = p < T;
+  int cnt = (p < P) ? lm.samppop[p] : 0;
+  int nsum = n, k = 0, cc = 0;
+  double last = 0.0, len = 0.0, fcacc = 0.0, fmacc = 0.0;
+  const double h2term = 1 / (2 * lm.hval);
+  for (int j = 0; j < E; ++j) {
+    const double t = st[j];
+    const int code = sk[j];
+    const double dt = t - last;
+    len += nsum * dt;
+    last = t;
+    const int ip = mine ? md.pos[k * T + p] : -1;
+    if (ip >= 0) {
+      fcacc += ((double)cnt * ((double)cnt - 1)) * dt * h2term;
+      if (k < nper - 1) fmacc += cnt * dt;
+    }
+    const int type = code & 15, a = (code >> 4) & 255, b = (code >> 12) - 1;
+    if (type == EV_COAL) {
+      if (p == a) {
+        if (ip < 0) bad = 1;
+        cc++;
+        cnt--;
+      }
+      nsum--;
+    } else if (type == EV_MIG) {
+      if (p == a) {
+        const int ia = md.pos[k * T + a];
+        const int ib = (b >= 0 && b < T) ? md.pos[k * T + b] : -1;
+        if (ia < 0 || ib < 0)
+          bad = 1;
+        else
+          rec[3 * P * P + (k * P + ia) * P + ib] += 1.0;
+        cnt--;
+      }
+      if (p == b) cnt++;
+    }
+    // population counts before a split are shared through the wave
+    const int ja = md.join[3 * k], jb = md.join[3 * k + 1], jn = md.join[3 * k + 2];
+    const int na = __shfl(cnt, ja < 0 ? 0 : ja, kWave);
+    const int nb = __shfl(cnt, jb < 0 ? 0 : jb, kWave);
+    if (type == EV_SPLIT) {
+      if (ip >= 0) {
+        rec[k * P + ip] = (double)cc;
+        rec[P * P + k * P + ip] = fcacc;
+        if (k < nper - 1)
+          for (int jj = 0; jj < md.npresent[k]; ++jj)
+            if (jj != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + jj] = fmacc;
+      }
+      if (p == jn) cnt = na + nb;
+      if (p == ja || p == jb) cnt = 0;
+      cc = 0;
+      fcacc = fmacc = 0.0;
+      k++;
+    }
+  }
+  {
+    const int ip = mine ? md.pos[k * T + p] : -1;
+    if (ip >= 0) {
+      rec[k * P + ip] = (double)cc;
+      rec[P * P + k * P + ip] = fcacc;
+      if (k < nper - 1)
+        for (int jj = 0; jj < md.npresent[k]; ++jj)
+          if (jj != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + jj] = fmacc;
+    }
+  }
+  __syncthreads();
+  const double hlog = log(lm.hval);
+  if (hlog != 0.0)
+    for (int r = lane; r < P * P; r += kWave) rec[2 * P * P + r] = hlog * rec[r];
+  if (lane == 0) rec[R - 1] = len;
+  __syncthreads();
+  double *out = rec_out + (int64_t)i * R;
+  for (int r = lane; r < R; r += kWave) out[r] = rec[r];
+  if (lane == 0 && status) status[i] = bad ? IMA_STATUS_PRIOR_BAD : IMA_STATUS_OK;
+}
+
+__global__ void __launch_bounds__(kWave)
+prior_kernel(DeviceView v, int model, const double *recs, const int32_t *group_off,
+             int ngroups, double *probg, double *qint, double *mint) {
+  extern __shared__ double sums[];
+  const int g = blockIdx.x;
+  if (g >= ngroups) return;
+  const int lane = threadIdx.x;
+  const ModelMeta mm = v.models[model];
+  const ModelView md = model_view(v, mm);
+  const int P = mm.npops, R = prior_rec_len(P);
+  const int s0 = group_off[g], s1 = group_off[g + 1];
+  for (int r = lane; r < R; r += kWave) {
+    double s = 0.0;
+    for (int i = s0; i < s1; ++i) s += recs[(int64_t)i * R + r];
+    sums[r] = s;
+  }
+  __syncthreads();
+  double *terms = sums + R;  // [npsize + nmig]
+  const int np = mm.npsize, nm = mm.nmig;
+  for (int t = lane; t < np + nm; t += kWave) {
+    int c;
+    double f, h;
+    prior_param_sums(md, P, np, t, sums, &c, &f, &h);
+    terms[t] = prior_param_term(md, np, t, c, f, h, v.logfact);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double q = 0.0, m = 0.0;
+    for (int t = 0; t < np; ++t) q += terms[t];
+    for (int t = 0; t < nm; ++t) m += terms[np + t];
+    probg[g] = q + m;
+  }
+  for (int t = lane; t < np; t += kWave) qint[(int64_t)g * np + t] = terms[t];
+  for (int t = lane; t < nm; t += kWave) mint[(int64_t)g * nm + t] = terms[np + t];
+}
+
+}  // namespace
+
+size_t treeweight_lds_bytes(int P, int emax) {
+  return (size_t)prior_rec_len(P) * 8 + (size_t)emax * 24;
+}
+
+hipError_t launch_treeweight(const DeviceView &v, const int32_t *slots, int nslots,
+                             const double *tvals, const int32_t *tv_row, int P, int emax,
+                             double *rec_out, int32_t *status, hipStream_t st) {
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(treeweight_kernel, dim3(nslots), dim3(kWave),
+                     treeweight_lds_bytes(P, emax), st, v, slots, nslots, tvals, tv_row,
+                     rec_out, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_prior(const DeviceView &v, int model, int P, int nparams, const double *recs,
+                        const int32_t *group_off, int ngroups, double *probg, double *qint,
+                        double *mint, hipStream_t st) {
+  if (ngroups <= 0) return hipSuccess;
+  const size_t lds = ((size_t)prior_rec_len(P) + nparams) * 8;
+  hipLaunchKernelGGL(prior_kernel, dim3(ngroups), dim3(kWave), lds, st, v, model, recs,
+                     group_off, ngroups, probg, qint, mint);
+  return hipGetLastError();
+}
+
+}  // namespace ima

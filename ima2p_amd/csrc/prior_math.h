@@ -1,0 +1,244 @@
+// prior_math.h -- log incomplete gamma functions and the analytic
+// integrals of the genealogy prior over uniform priors on population size
+// and migration rate (integrate_coalescent_term / integrate_migration_term,
+// update_gtree_common.cpp:109-296).
+//
+// The incomplete gammas follow the published Numerical Recipes methods the
+// reference uses (utilities.cpp:812-1122): series for x < a + 1, Lentz
+// continued fraction otherwise, relative tolerance 3e-7, at most 1000
+// terms, FPMIN 1e-30; E_1 by its series (x <= 1) or continued fraction
+// (at most 100 terms).  The termination rules decide the value to ~3e-7,
+// so they are kept as they are.  log((k)!) comes from `lf`, a table built
+// on the host as the running sum of log(i), as the reference's setlogfact
+// builds it.  Everything is FP64.  The batch prior kernels compile these
+// without floating-point contraction, so sums round as written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ima_internal.h"
+
+namespace ima {
+
+constexpr double kPriEps = 3.0e-7;
+constexpr int kPriItmax = 1000;
+constexpr double kPriFpmin = 1.0e-30;
+constexpr double kLog2 = 0.69314718055994530941723212146;
+
+__device__ inline double pri_gcf(double a, double x, const double *lf, double *gln,
+                                 double *h_out) {
+  *gln = lf[(int)a - 1];
+  double b = x + 1.0 - a, c = 1.0 / kPriFpmin, d = 1.0 / b, h = d;
+  for (int i = 1; i <= kPriItmax; i++) {
+    const double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < kPriFpmin) d = kPriFpmin;
+    c = b + an / c;
+    if (fabs(c) < kPriFpmin) c = kPriFpmin;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < kPriEps) break;
+  }
+  *h_out = h;
+  return -x + a * log(x) - (*gln);
+}
+
+// series: returns the sum and the log prefactor -x + a log x - log Gamma(a)
+__device__ inline double pri_gser(int a, double x, const double *lf, double *gln,
+                                  double *pre) {
+  *gln = lf[a - 1];
+  *pre = 0.0;
+  if (x <= 0.0) return 0.0;
+  double ap = a, del = 1.0 / a, sum = del;
+  for (int n = 1; n <= kPriItmax; n++) {
+    ++ap;
+    del *= x / ap;
+    sum += del;
+    if (fabs(del) < fabs(sum) * kPriEps) break;
+  }
+  *pre = -x + a * log(x) - (*gln);
+  return sum;
+}
+
+// E_1(x) or its log (*islog)
+__device__ inline double pri_expint1(double x, int *islog) {
+  const double euler = 0.5772156649;
+  *islog = 0;
+  if (x > 1.0) {
+    double b = x + 1.0, c = 1.0 / kPriFpmin, d = 1.0 / b, h = d;
+    for (int i = 1; i <= 100; i++) {
+      const double a = -i * (0 + i);
+      b += 2.0;
+      d = 1.0 / (a * d + b);
+      c = b + a / c;
+      const double del = c * d;
+      h *= del;
+      if (fabs(del - 1.0) < kPriEps) break;
+    }
+    *islog = 1;
+    return log(h) - x;
+  }
+  double ans = -log(x) - euler, fact = 1.0;
+  for (int i = 1; i <= 100; i++) {
+    fact *= -x / i;
+    const double del = -fact / (i - 0);
+    ans += del;
+    if (fabs(del) < fabs(ans) * kPriEps) break;
+  }
+  return ans;
+}
+
+__device__ inline double pri_uppergamma(int a, double x, const double *lf) {
+  double gln, p;
+  if (a == 0) {
+    int islog;
+    const double t = pri_expint1(x, &islog);
+    p = islog ? t : log(t);
+  } else if (x < (a + 1.0)) {
+    double pre;
+    const double s = pri_gser(a, x, lf, &gln, &pre);
+    const double g = (x <= 0.0) ? 0.0 : s * exp(pre);
+    p = gln + log(1.0 - g);
+  } else {
+    double h;
+    const double pre = pri_gcf((double)a, x, lf, &gln, &h);
+    p = gln + (pre + log(h));
+  }
+  if (p < -1e200) p = -1e200;
+  return p;
+}
+
+__device__ inline double pri_lowergamma(int a, double x, const double *lf) {
+  double gln, p;
+  if (x < (a + 1.0)) {
+    double pre;
+    const double s = pri_gser(a, x, lf, &gln, &pre);
+    const double g = (x <= 0.0) ? 0.0 : log(s) + pre;
+    p = gln + g;
+  } else {
+    double h;
+    const double pre = pri_gcf((double)a, x, lf, &gln, &h);
+    p = gln + log(1 - exp(pre) * h);
+  }
+  if (p < -1e200) p = -1e200;
+  return p;
+}
+
+// log(e^a - e^b) for a > b (LogDiff, imamp.hpp:257-263)
+__device__ inline double pri_logdiff(double a, double b) {
+  if (a - b < 7.0978271289338397e+02) return b + log(exp(a - b) - 1.0);
+  return a;
+}
+
+// log int_min^max (2/t)^cc e^(-2 fc/t - hcc) dt
+__device__ inline double pri_coalescent_term(int cc, double fc, double hcc, double max,
+                                             double min, const double *lf) {
+  double p;
+  if (cc > 0) {
+    if (min == 0) {
+      double ug = pri_uppergamma(cc - 1, 2 * fc / max, lf);
+      if (cc > 1) {
+        const double fullg = lf[cc - 2];
+        if (fullg - ug < 1e-15 || fullg - ug > 7.0978271289338397e+02) {
+          const double lg = pri_lowergamma(cc - 1, 2 * fc / max, lf);
+          if (fullg > lg) {
+            const double ugalt = pri_logdiff(fullg, lg);
+            if (fabs(ugalt - ug) > 1e-10) ug = ugalt;
+          }
+        }
+      }
+      p = ug + kLog2 - hcc + (1 - cc) * log(fc);
+    } else {
+      const double a = pri_uppergamma(cc - 1, 2 * fc / max, lf);
+      const double b = pri_uppergamma(cc - 1, 2 * fc / min, lf);
+      p = pri_logdiff(a, b) + (kLog2 - hcc + (1 - cc) * log(fc));
+    }
+  } else if (2 * fc / max > 0) {
+    if (min == 0) {
+      const double a = log(max) - 2.0 * fc / max;
+      const double b = kLog2 + log(fc) + pri_uppergamma(0, 2.0 * fc / max, lf);
+      p = pri_logdiff(a, b);
+    } else {
+      double a = pri_uppergamma(0, 2 * fc / max, lf), b = pri_uppergamma(0, 2 * fc / min, lf);
+      const double c = pri_logdiff(a, b) + kLog2 + log(fc);
+      a = log(max) - 2.0 * fc / max;
+      b = log(min) - 2.0 * fc / min;
+      p = pri_logdiff(pri_logdiff(a, b), c);
+    }
+  } else {
+    p = log(max - min);
+  }
+  return p;
+}
+
+// log int_min^max m^cm e^(-fm m) dm
+__device__ inline double pri_migration_term(int cm, double fm, double max, double min,
+                                            const double *lf) {
+  const double mpriormin = 0.000001;  // MPRIORMIN, imamp.hpp:141
+  double p;
+  if (cm > 0) {
+    if (min == 0) {
+      double lg = pri_lowergamma(cm + 1, fm * max, lf);
+      const double fullg = lf[cm];
+      if (fullg - lg < 1e-15 || fullg - lg > 7.0978271289338397e+02) {
+        const double ug = pri_uppergamma(cm + 1, fm * max, lf);
+        if (fullg > ug) {
+          const double lgalt = pri_logdiff(fullg, ug);
+          if (fabs(lgalt - lg) > 1e-12) lg = lgalt;
+        }
+      }
+      p = (-1 - cm) * log(fm) + lg;
+    } else {
+      const double a = pri_uppergamma(cm + 1, fm * min, lf);
+      const double b = pri_uppergamma(cm + 1, fm * max, lf);
+      p = (-1 - cm) * log(fm) + pri_logdiff(a, b);
+    }
+  } else if (fm > mpriormin) {
+    if (min == 0)
+      p = (max == mpriormin) ? 0.0 : pri_logdiff(0.0, -fm * max) - log(fm);
+    else
+      p = pri_logdiff(-fm * min, -fm * max) - log(fm);
+  } else {
+    p = log(max - min);
+  }
+  return p;
+}
+
+// Count, weight and hcc sums of parameter t (population sizes first, then
+// migration rates) over its (period, position) entries of a weight record,
+// in parameter-map order (integrate_tree_prob's loops).
+__device__ inline void prior_param_sums(const ModelView &md, int P, int npsize, int t,
+                                        const double *rec, int *c, double *f, double *h) {
+  int cc = 0;
+  double ff = 0.0, hh = 0.0;
+  if (t < npsize) {
+    for (int w = md.theta_off[t]; w < md.theta_off[t + 1]; ++w) {
+      const int kk = md.theta_wp[2 * w], ii = md.theta_wp[2 * w + 1];
+      cc += (int)rec[kk * P + ii];
+      ff += rec[P * P + kk * P + ii];
+      hh += rec[2 * P * P + kk * P + ii];
+    }
+  } else {
+    const int m = t - npsize;
+    for (int w = md.mig_off[m]; w < md.mig_off[m + 1]; ++w) {
+      const int kk = md.mig_wp[3 * w], ii = md.mig_wp[3 * w + 1], jj = md.mig_wp[3 * w + 2];
+      cc += (int)rec[3 * P * P + (kk * P + ii) * P + jj];
+      ff += rec[3 * P * P + P * P * P + (kk * P + ii) * P + jj];
+    }
+  }
+  *c = cc;
+  *f = ff;
+  *h = hh;
+}
+
+// The analytic integral of parameter t at those sums (uniform prior on
+// [0, qmax] / [0, mmax]).
+__device__ inline double prior_param_term(const ModelView &md, int npsize, int t, int c,
+                                          double f, double h, const double *lf) {
+  return t < npsize ? pri_coalescent_term(c, f, h, md.qmax[t], 0.0, lf)
+                    : pri_migration_term(c, f, md.mmax[t - npsize], 0.0, lf);
+}
+
+}  // namespace ima

@@ -131,6 +131,89 @@ class Engine:
                                           A.ctypes.data_as(L.i32p)), "read_slots")
         return up0, up1, dn, tm, rt, A
 
+    # ---------------------------------------------------- genealogy prior --
+    def model_add(self, model):
+        """Register an immodel.IMModel; returns its id."""
+        P = model.npops
+        toff = np.cumsum([0] + [len(w) for w in model.theta_wp]).astype(np.int32)
+        twp = np.array([x for w in model.theta_wp for kv in w for x in kv] or [0], np.int32)
+        moff = np.cumsum([0] + [len(w) for w in model.mig_wp]).astype(np.int32)
+        mwp = np.array([x for w in model.mig_wp for kv in w for x in kv] or [0], np.int32)
+        arrs = [np.ascontiguousarray(a, np.int32) for a in
+                (model.plist, model.npresent, model.popdown, model.pope, model.join)]
+        qm, pq = _f64(model.qmax if model.npsize else [1.0])
+        mm, pm = _f64(model.mmax if model.nmig else [1.0])
+        out = C.c_int(-1)
+        self._chk(self.lib.ima_model_add(
+            self.h, P, *[a.ctypes.data_as(L.i32p) for a in arrs], model.npsize,
+            toff.ctypes.data_as(L.i32p), twp.ctypes.data_as(L.i32p), model.nmig,
+            moff.ctypes.data_as(L.i32p), mwp.ctypes.data_as(L.i32p), pq, pm, C.byref(out)),
+            "model_add")
+        return out.value
+
+    def locus_set_model(self, locus, model_id, samppop, hval=1.0):
+        sp, psp = _i32(samppop)
+        self._chk(self.lib.ima_locus_set_model(self.h, locus, model_id, psp, float(hval)),
+                  "locus_set_model")
+
+    def write_pops(self, first, gens):
+        """Population labels and migration events (Genealogy.pop / .mig)."""
+        from .immodel import mig_arrays
+        pop, pp = _i32(np.concatenate([g.pop for g in gens]))
+        es, ts, tos, cnt = [], [], [], []
+        for g in gens:
+            e, t, to = mig_arrays(g)
+            es.append(e)
+            ts.append(t)
+            tos.append(to)
+            cnt.append(len(e))
+        nm, pn = _i32(cnt)
+        me, pme = _i32(np.concatenate(es) if sum(cnt) else [0])
+        mt, pmt = _f64(np.concatenate(ts) if sum(cnt) else [0.0])
+        mo, pmo = _i32(np.concatenate(tos) if sum(cnt) else [0])
+        self._chk(self.lib.ima_slots_write_pops(self.h, first, len(gens), pp, pn, pme, pmt, pmo),
+                  "write_pops")
+
+    def treeweight(self, slots, tvals, P):
+        """Per-slot weights: dict of cc/fc/hcc [n, P, P], mc/fm [n, P, P, P], length."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        k = len(slots)
+        if P > 1:
+            tv = np.asarray(tvals, float).reshape(-1, P - 1)
+            tv = np.ascontiguousarray(np.broadcast_to(tv, (k, P - 1)) if len(tv) == 1 else tv)
+        else:
+            tv = np.zeros((k, 1))
+        cc = np.zeros((k, P, P), np.int32)
+        fc = np.zeros((k, P, P))
+        hcc = np.zeros((k, P, P))
+        mc = np.zeros((k, P, P, P), np.int32)
+        fm = np.zeros((k, P, P, P))
+        ln = np.zeros(k)
+        self._chk(self.lib.ima_treeweight(
+            self.h, slots.ctypes.data_as(L.i32p), k, tv.ctypes.data_as(L.dp),
+            cc.ctypes.data_as(L.i32p), fc.ctypes.data_as(L.dp), hcc.ctypes.data_as(L.dp),
+            mc.ctypes.data_as(L.i32p), fm.ctypes.data_as(L.dp), ln.ctypes.data_as(L.dp)),
+            "treeweight")
+        return {"cc": cc, "fc": fc, "hcc": hcc, "mc": mc, "fm": fm, "length": ln}
+
+    def prior(self, groups, tvals, model):
+        """integrate_tree_prob over each group of slots; tvals[g] its split
+        times.  Returns (probg[G], qint[G, npsize], mint[G, nmig])."""
+        slots = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32) for g in groups]),
+                                     np.int32)
+        off = np.cumsum([0] + [len(g) for g in groups]).astype(np.int32)
+        G, P = len(groups), model.npops
+        tv = np.ascontiguousarray(np.asarray(tvals, float).reshape(G, max(P - 1, 1))
+                                  if P > 1 else np.zeros((G, 1)))
+        probg = np.zeros(G)
+        q = np.zeros((G, max(model.npsize, 1)))
+        m = np.zeros((G, max(model.nmig, 1)))
+        self._chk(self.lib.ima_prior(self.h, slots.ctypes.data_as(L.i32p),
+                                     off.ctypes.data_as(L.i32p), G, tv.ctypes.data_as(L.dp),
+                                     probg.ctypes.data_as(L.dp), q.ctypes.data_as(L.dp),
+                                     m.ctypes.data_as(L.dp)), "prior")
+        return probg, q[:, :model.npsize], m[:, :model.nmig]
+
     # ----------------------------------------------------------- scoring --
     @staticmethod
     def items(slots, u, kappa=None, tr=None):
@@ -153,6 +236,16 @@ class Engine:
         st = np.zeros(n, np.int32)
         self._chk(self.lib.ima_score(self.h, items, n, precision, out.ctypes.data_as(L.dp),
                                      st.ctypes.data_as(L.i32p)), "score")
+        return out, st
+
+    def score_repeat(self, items, reps, precision=L.PREC_FP64):
+        """The same scoring `reps` times back to back (kernel timing)."""
+        n = len(items)
+        out = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        self._chk(self.lib.ima_score_repeat(self.h, items, n, precision, int(reps),
+                                            out.ctypes.data_as(L.dp),
+                                            st.ctypes.data_as(L.i32p)), "score_repeat")
         return out, st
 
     def is_site_nodes(self, slot, S):
@@ -250,6 +343,30 @@ class Engine:
                                         stats.ctypes.data_as(C.POINTER(C.c_uint64)),
                                         C.byref(sweeps)), "mc3_read")
         return pdg.reshape(C_, L_), beta, stats, sweeps.value
+
+    def mc3_set_prior(self, tvals, tmax, P):
+        """Genealogy prior in the sweep; tvals [nchains, P-1] split times."""
+        C_, _ = self.mc3_shape
+        tv = np.ascontiguousarray(np.asarray(tvals, float).reshape(C_, P - 1) if P > 1
+                                  else np.zeros((C_, 1)))
+        self.mc3_P = P
+        self._chk(self.lib.ima_mc3_set_prior(self.h, tv.ctypes.data_as(L.dp), float(tmax)),
+                  "mc3_set_prior")
+
+    def mc3_read_prior(self):
+        C_, _ = self.mc3_shape
+        P = self.mc3_P
+        probg = np.zeros(C_)
+        tv = np.zeros((C_, max(P - 1, 1)))
+        self._chk(self.lib.ima_mc3_read_prior(self.h, probg.ctypes.data_as(L.dp),
+                                              tv.ctypes.data_as(L.dp)), "mc3_read_prior")
+        return probg, tv[:, :P - 1]
+
+    def read_mig_times(self, slot, nmig):
+        out = np.zeros(max(nmig, 1))
+        self._chk(self.lib.ima_slots_read_mig(self.h, slot, out.ctypes.data_as(L.dp)),
+                  "read_mig_times")
+        return out[:nmig]
 
     # ------------------------------------------------------------ timing --
     def timing(self, on=True):

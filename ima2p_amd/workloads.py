@@ -35,6 +35,10 @@ class Workload:
     chains: list = field(default_factory=list)  # chains[c][l] -> Genealogy
     beta: np.ndarray = None
     seed: int = 0
+    poptree: sim.PopTree = None    # population tree of the IM model
+    tsplit: np.ndarray = None      # split times (chains start at the truth)
+    samppop: list = None           # gene copies per sampled population
+    tmax: float = 5.0              # split-time prior upper bound (-t)
 
     @property
     def nloci(self):
@@ -73,7 +77,7 @@ def three_pop_hky(nchains=64, nloci=50, nsites=1000, per_pop=15, seed=2, pool=16
     chains = _chain_pool(rng, lambda: sim.simulate_genealogy(rng, [per_pop] * 3, theta, ts,
                                                              mig, pt), nloci, nchains, pool)
     return Workload("3pop-50xHKY1000bp-n45-64chains", nchains, "hky", loci, u, 4.0, chains,
-                    geometric_betas(nchains), seed)
+                    geometric_betas(nchains), seed, pt, np.asarray(ts, float), [per_pop] * 3)
 
 
 def two_pop_is(nchains=4, nloci=5, per_pop=10, seed=1, pool=4):
@@ -94,4 +98,4 @@ def two_pop_is(nchains=4, nloci=5, per_pop=10, seed=1, pool=4):
             row.append(g)
         chains.append(row)
     return Workload("2pop-5xIS-n20-4chains", nchains, "is", loci, np.ones(nloci), 1.0, chains,
-                    geometric_betas(nchains), seed)
+                    geometric_betas(nchains), seed, pt, np.asarray(ts, float), [per_pop] * 2)

@@ -47,7 +47,8 @@ enum ima_status {
   IMA_STATUS_OK = 0,
   IMA_STATUS_IS_REJECT = 1, /* two 0/1 nodes at a site: calc_prob_data.cpp:784-787 */
   IMA_STATUS_SW_ZERO = 2,   /* a Bessel term <= 0: calc_prob_data.cpp:876-897 */
-  IMA_STATUS_IS_NOSITE = 3  /* a site with no 0/1 node (reference: IM_err exit) */
+  IMA_STATUS_IS_NOSITE = 3, /* a site with no 0/1 node (reference: IM_err exit) */
+  IMA_STATUS_PRIOR_BAD = 4  /* population labels / migrations do not fit the model */
 };
 enum ima_lmode { IMA_LMODE_REF_COMPAT = 0, IMA_LMODE_EXACT = 1 };
 
@@ -121,6 +122,13 @@ int ima_slot_info(ima_ctx *ctx, int slot, int *locus, int *n);
  * (may be NULL) receive each item's result. */
 int ima_score(ima_ctx *ctx, const ima_item *items, int nitems, int precision,
               double *loglik, int32_t *status);
+/* ima_score run `reps` times back to back on the context's stream, every
+ * pass writing the same results.  With timing enabled one event pair
+ * brackets all passes of a kernel class, so ima_timing_get's average is
+ * the kernel's own duration rather than one launch's round trip.  A
+ * measurement aid with no counterpart in the reference. */
+int ima_score_repeat(ima_ctx *ctx, const ima_item *items, int nitems,
+                     int precision, int reps, double *loglik, int32_t *status);
 /* IS diagnostics: the 0/1 node of every site (calc_prob_data.cpp:777-821)
  * for the genealogy in `slot`; site_node has S entries. */
 int ima_is_site_nodes(ima_ctx *ctx, int slot, int32_t *site_node);
@@ -143,7 +151,10 @@ int ima_likelihood_hky(ima_ctx *ctx, int locus, const int32_t *up0,
 /* Performs `swaptries` swap attempts among nchains chains whose total
  * log posterior sums are S[c] = sum_li pdg + probg.  beta[] is updated in
  * place (betas are swapped, never chain state).  Randomness comes from a
- * counter-based generator keyed by (seed, counter).  swaps_out (may be
+ * counter-based generator keyed by seed, starting at block `counter`; a
+ * call reads one block of four 32-bit words per two uniforms (three or
+ * more uniforms per try), so independent calls need counters that many
+ * blocks apart, e.g. call index << 16.  swaps_out (may be
  * NULL) receives the number of accepted swaps; pairs_out (may be NULL,
  * 3*swaptries ints) receives (ci, cj, accepted) per try. */
 int ima_swap(ima_ctx *ctx, int nchains, const double *S, double *beta,
@@ -173,6 +184,53 @@ int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
                          const double *x, int nx, int logi, double yadjust,
                          int expo, double prior_mean, double *out);
 
+/* ---- genealogy prior (update_gtree_common.cpp:1680-2053) ----------------- */
+/* Population model, flattened (the reference's poptree and parameter maps,
+ * build_poptree.cpp, setup_iparams initialize.cpp:202-760).  P = npops
+ * sampled populations = periods k = 0..P-1 between split times; tree
+ * populations 0..2P-2, ancestral population P+k formed by the split that
+ * closes period k.  plist[k][i] (P x P, -1 padded) lists the npresent[k]
+ * populations of period k; popdown[q] / pope[q] give the population q
+ * merges into and the period where it ends (-1 for the root population);
+ * join[k] = (a, b, P+k) for k < P-1.  Population-size parameter t sums the
+ * (period, position) pairs theta_wp[theta_off[t] .. theta_off[t+1]) and
+ * migration parameter m the (period, from, to) triples
+ * mig_wp[mig_off[m] .. mig_off[m+1]) (the itheta/imig wp lists); uniform
+ * priors on [0, qmax[t]] and [0, mmax[m]]. */
+int ima_model_add(ima_ctx *ctx, int npops, const int32_t *plist,
+                  const int32_t *npresent, const int32_t *popdown,
+                  const int32_t *pope, const int32_t *join, int npsize,
+                  const int32_t *theta_off, const int32_t *theta_wp, int nmig,
+                  const int32_t *mig_off, const int32_t *mig_wp,
+                  const double *qmax, const double *mmax, int *model);
+/* The locus' model, gene copies per sampled population (L[li].samppop) and
+ * inheritance scalar h (L[li].hval). */
+int ima_locus_set_model(ima_ctx *ctx, int locus, int model,
+                        const int32_t *samppop, double hval);
+/* Population of every edge at its lower end (gtree[i].pop) and the
+ * migration events of each genealogy (gtree[i].mig, sorted by edge then
+ * time; mig_to is the population the lineage enters going back in time),
+ * for slots first..first+count-1, packed slot after slot. */
+int ima_slots_write_pops(ima_ctx *ctx, int first, int count,
+                         const int32_t *pop, const int32_t *nmig,
+                         const int32_t *mig_edge, const double *mig_time,
+                         const int32_t *mig_to);
+/* treeweight (update_gtree_common.cpp:1680-1931) for each slot, split
+ * times tvals[i][0..P-2]; outputs per slot cc[P][P], fc, hcc (P x P),
+ * mc[P][P][P], fm (P x P x P), indexed [period][position][position], and
+ * the total branch length.  Any output may be NULL. */
+int ima_treeweight(ima_ctx *ctx, const int32_t *slots, int nslots,
+                   const double *tvals, int32_t *cc, double *fc, double *hcc,
+                   int32_t *mc, double *fm, double *length);
+/* integrate_tree_prob (update_gtree_common.cpp:1945-2053) for groups of
+ * slots (a chain's loci: group g is slots[group_off[g] .. group_off[g+1]),
+ * split times tvals[g][0..P-2]): the groups' summed weights integrated over
+ * the uniform priors.  probg[g], qint[g][npsize] and mint[g][nmig] (may be
+ * NULL) are the reference's probg, qintegrate and mintegrate. */
+int ima_prior(ima_ctx *ctx, const int32_t *slots, const int32_t *group_off,
+              int ngroups, const double *tvals, double *probg, double *qint,
+              double *mint);
+
 /* ---- device-resident MC3 sweep ------------------------------------------ */
 /* All heated chains on this device: slots first_slot + c*nloci + l hold the
  * genealogy of chain c at locus l.  A sweep proposes one genealogy change
@@ -196,6 +254,22 @@ int ima_mc3_run(ima_ctx *ctx, int nsweeps, int flags);
  * accepts, rescale proposals, accepts, swap tries, swaps; sweeps done. */
 int ima_mc3_read(ima_ctx *ctx, double *pdg, double *beta, uint64_t *stats,
                  uint64_t *sweeps);
+
+/* Adds the genealogy prior to the sweep: every MC3 slot's locus has a
+ * population model (ima_locus_set_model) and its population labels and
+ * migrations (ima_slots_write_pops); tvals[c][0..P-2] are chain c's split
+ * times, uniform prior on [0, tmax].  The genealogy proposal then keeps
+ * each node inside its period and between its edges' migrations, the
+ * rescaling also scales migration and split times, and acceptance is
+ * beta (dprobg + dpdg) with probg from integrate_tree_prob over the
+ * chain's all-loci weights (update_gtree.cpp:827-961).  Call after
+ * ima_mc3_setup. */
+int ima_mc3_set_prior(ima_ctx *ctx, const double *tvals, double tmax);
+/* probg[nchains] and split times tvals[nchains][P-1] of the sweep. */
+int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
+/* Migration times of one slot as the device holds them (the rescaling
+ * moves them); mig_time has that slot's migration count entries. */
+int ima_slots_read_mig(ima_ctx *ctx, int slot, double *mig_time);
 
 /* ---- timing ------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the context's stream (off by default).

@@ -342,6 +342,8 @@ def treeweight(g, model, tvals, samppop, hval=1.0):
                             _ip(plist), _ip(npres), _ip(pdown), _ip(pope), _ip(join), _dp(tv),
                             _ip(sp), float(hval), _ip(cc), _dp(fc), _dp(hcc), _ip(mc), _dp(fm),
                             _dp(length))
+    if rc == -2:
+        raise ValueError("treeweight: a population label does not fit the model's periods")
     if rc != 0:
         raise ValueError("treeweight: event count mismatch")
     return {"cc": cc.reshape(nper, P), "fc": fc.reshape(nper, P), "hcc": hcc.reshape(nper, P),

@@ -1007,14 +1007,22 @@ int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_e
     }
     if (ev[j].type == 0) {
       int ii = 0;
-      while (plist[k * P + ii] != ev[j].pop) ii++;
+      while (ii < npresent[k] && plist[k * P + ii] != ev[j].pop) ii++;
+      if (ii == npresent[k]) { /* population absent from the period */
+        free(ev);
+        return -2;
+      }
       cc[k * P + ii]++;
       n[ev[j].pop]--;
       nsum--;
     } else if (ev[j].type == 1) {
       int ii = 0, jj = 0;
-      while (plist[k * P + ii] != ev[j].pop) ii++;
-      while (plist[k * P + jj] != ev[j].topop) jj++;
+      while (ii < npresent[k] && plist[k * P + ii] != ev[j].pop) ii++;
+      while (jj < npresent[k] && plist[k * P + jj] != ev[j].topop) jj++;
+      if (ii == npresent[k] || jj == npresent[k]) {
+        free(ev);
+        return -2;
+      }
       mc[(k * P + ii) * P + jj]++;
       n[ev[j].pop]--;
       n[ev[j].topop]++;

@@ -182,3 +182,7 @@ def test_mixed_models_one_call(engine):
     assert rel(out[0], oracle.likelihood_hky(gh, lh, 1.5, 4.0)[0]) < FP64_RTOL
     assert rel(out[1], oracle.likelihood_is(gi, li_, 2.5, sl)[0]) < FP64_RTOL
     assert rel(out[2], oracle.likelihood_sw(gs, A, 3.0, 1.2)[0]) < FP64_RTOL
+    # repeated passes (the kernel-timing entry point) give the same values
+    rep, st2 = engine.score_repeat(items, 3)
+    np.testing.assert_array_equal(rep, out)
+    np.testing.assert_array_equal(st2, st)

@@ -239,3 +239,17 @@ def test_integrate_tree_prob_is_the_integral_of_the_density(seed):
             return log_density(traced, pt, model.nper, th0, mm)
         want += log_integral_1d(f, mmax) - base
     assert abs(probg - want) <= 1e-5 * max(1.0, abs(want)), (probg, want)
+
+
+def test_treeweight_rejects_label_outside_period():
+    rng = np.random.default_rng(9)
+    pt, theta, ts, mig = sim.im_model_two_pop(t=0.9, m12=0.7, m21=0.4)
+    model = immodel.build(pt)
+    samp = [6, 6]
+    for _ in range(20):
+        g = sim.simulate_genealogy(rng, samp, theta, ts, mig, pt)
+        if g.time[g.up0[g.root]] > ts[0]:
+            break
+    g.pop[g.root] = 0  # population 0 has merged before the root's coalescence
+    with pytest.raises(ValueError):
+        oracle.treeweight(g, model, ts, samp)
