@@ -18,9 +18,9 @@ stream around the sweep's scoring launch repeated back to back, over the
 chains' current genealogies), the same sweep in FP64, the plain
 per-locus FP64 GPU path (one launch per likelihood call, host control
 loop, one prior call per proposal), L-mode jointp points/s, and the CPU
-baseline: the oracle (a C restatement of the reference's likelihoodHKY,
-incremental as the reference evaluates it, and of treeweight and the
-prior integrals) driving the same sweeps on all host cores.
+baseline: the oracle's sweep in C (its restatement of the reference's
+likelihoodHKY, incremental as the reference evaluates it, treeweight and
+the prior integrals, with the same proposals and swap) on the host cores.
 """
 from __future__ import annotations
 
@@ -68,119 +68,21 @@ def hky_traffic():
 _WL = None
 
 
-class _OraclePrior:
-    """The genealogy prior of the CPU baseline, through the oracle: weights
-    per genealogy (treeweight), all-loci sums updated as
-    sum_subtract_treeinfo does, integrals recomputed only for parameters
-    whose (count, weight) changed."""
-
-    def __init__(self, model, samp):
-        import oracle
-        self.o = oracle
-        self.model, self.samp, P = model, samp, model.npops
-        self.P2, self.P3 = P * P, P * P * P
-        p2, p3 = self.P2, self.P3
-        self.params = []
-        for wp in model.theta_wp:
-            ix = np.array([k * P + i for k, i in wp], int)
-            self.params.append(("q", ix, p2 + ix, 2 * p2 + ix))
-        for wp in model.mig_wp:
-            ix = np.array([(k * P + i) * P + j for k, i, j in wp], int)
-            self.params.append(("m", 3 * p2 + ix, 3 * p2 + p3 + ix, None))
-        self.recs, self.all, self.cache, self.tv, self.pend = {}, {}, {}, {}, None
-
-    def _flat(self, g, tv):
-        w = self.o.treeweight(g, self.model, tv, self.samp)
-        return np.concatenate([w["cc"].ravel(), w["fc"].ravel(), w["hcc"].ravel(),
-                               w["mc"].ravel(), w["fm"].ravel(), [w["length"]]]).astype(float)
-
-    def _update(self, a, old, new):
-        p2, p3 = self.P2, self.P3
-        out = a + (new - old)
-        for lo, hi in ((p2, 2 * p2), (3 * p2 + p3, 3 * p2 + 2 * p3)):
-            out[lo:hi] = np.maximum(0.0, (a[lo:hi] - old[lo:hi]) + new[lo:hi])
-        return out
-
-    def _terms(self, a, cache):
-        out = []
-        for t, (kind, ic, jf, jh) in enumerate(self.params):
-            c, f = int(round(a[ic].sum())), float(a[jf].sum())
-            if cache is not None and cache[t][0] == c and cache[t][1] == f:
-                out.append(cache[t])
-                continue
-            if kind == "q":
-                v = self.o.integrate_coalescent_term(c, f, float(a[jh].sum()),
-                                                     self.model.qmax[t])
-            else:
-                v = self.o.integrate_migration_term(c, f, self.model.mmax[t - self.model.npsize])
-            out.append((c, f, v))
-        return out
-
-    def init(self, c, gens, tv):
-        self.tv[c] = np.array(tv, float)
-        self.recs[c] = [self._flat(g, tv) for g in gens]
-        self.all[c] = np.sum(self.recs[c], axis=0)
-        self.cache[c] = self._terms(self.all[c], None)
-        return sum(x[2] for x in self.cache[c])
-
-    def try_locus(self, c, l, g):
-        new = self._flat(g, self.tv[c])
-        a = self._update(self.all[c], self.recs[c][l], new)
-        cache = self._terms(a, self.cache[c])
-        self.pend = (new, a, cache)
-        return sum(x[2] for x in cache)
-
-    def settle_locus(self, c, l, accept, old=None):
-        if accept:
-            self.recs[c][l], self.all[c], self.cache[c] = self.pend
-
-    def try_chain(self, c, gens, tv):
-        recs = [self._flat(g, tv) for g in gens]
-        a = np.sum(recs, axis=0)
-        cache = self._terms(a, None)
-        self.pend = (recs, a, cache)
-        return sum(x[2] for x in cache)
-
-    def settle_chain(self, c, accept, old=None, tv=None):
-        if accept:
-            self.recs[c], self.all[c], self.cache[c] = self.pend
-            self.tv[c] = np.array(tv, float)
-
-
 def _cpu_worker(args):
-    """Run the host sweep with the oracle's likelihoodHKY and genealogy prior
-    on a chain subset for `seconds`; returns (chain_steps, elapsed)."""
+    """The oracle's C sweep (likelihoodHKY incremental as the reference
+    evaluates it, treeweight, the prior integrals, the same proposals and
+    swap) on a chain subset for `seconds`; returns (chain_steps, elapsed)."""
     wid, chain_ids, seconds = args
     import oracle
     from ima2p_amd import immodel
-    from ima2p_amd.hostmc3 import HostMC3
     wl = _WL
     chains = [[g.copy() for g in wl.chains[c]] for c in chain_ids]
-    states = {}
-    pend = {}
-
-    def lik(c, l, g, dirty, commit):
-        st = states.get((c, l))
-        e = (-1, -1, -1, -1) if dirty[0] == -1 or st is None else (dirty[0],) * 4
-        v, st2 = oracle.likelihood_hky(g, wl.loci[l], wl.u[l], wl.kappa, e=e, state=st,
-                                       commit=False)
-        states[(c, l)] = st2
-        pend[(c, l)] = g
-        return v
-
-    def commit(c, l, acc):
-        oracle.hky_commit(pend[(c, l)], states[(c, l)], acc)
-
-    beta = workloads.geometric_betas(len(chain_ids))
-    prior = _OraclePrior(immodel.build(wl.poptree), wl.samppop)
-    h = HostMC3(chains, beta, lik, commit, seed=100 + wid, prior=prior, tv=wl.tsplit,
-                tmax=wl.tmax)
+    tvs = np.tile(wl.tsplit, (len(chain_ids), 1))
     t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < seconds:
-        h.sweep(with_scale=bool(k & 1))
-        k += 1
-    return len(chain_ids) * k, time.perf_counter() - t0
+    sweeps, _ = oracle.mc3_bench(wl.loci, wl.u, wl.kappa, wl.samppop, immodel.build(wl.poptree),
+                                 chains, tvs, workloads.geometric_betas(len(chain_ids)),
+                                 wl.tmax, seed=100 + wid, seconds=seconds)
+    return len(chain_ids) * sweeps, time.perf_counter() - t0
 
 
 def cpu_baseline(wl, seconds, cores):
@@ -309,10 +211,9 @@ def main():
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         rate, steps, wall = cpu_baseline(wl, args.cpu_seconds, cores)
         cpu = {"value": rate, "unit": "chain-steps/s", "cores": cores, "kind": "port",
-               "sample": f"oracle likelihoodHKY (C, incremental as the reference) and genealogy "
-                         f"prior (C treeweight + integrals) driving the "
-                         f"same sweeps: {cores} processes x 4 chains x 50 loci, "
-                         f"{steps} chain-steps in {wall:.1f} s"}
+               "sample": f"the oracle's C sweep (likelihoodHKY incremental as the reference, "
+                         f"treeweight, prior integrals, same proposals and swap): {cores} "
+                         f"processes x 4 chains x 50 loci, {steps} chain-steps in {wall:.1f} s"}
 
     from ima2p_amd import _lib as L
     e, loci, first, model = load_engine(wl, L.PREC_FP32)

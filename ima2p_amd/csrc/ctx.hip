@@ -27,6 +27,7 @@ struct LTableHost {
 struct ima_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // genealogy weights beside the likelihood launch
   std::string err;
 
   // loci (host mirror + device copy)
@@ -116,6 +117,7 @@ struct ima_ctx {
     hipGraphExec_t exec[2]{nullptr, nullptr};
     hipGraph_t graph[2]{nullptr, nullptr};
     hipEvent_t ev[4]{};
+    hipEvent_t fj[4]{};  // fork / join of the side stream, per scoring pass
   } mc3;
 };
 
@@ -364,6 +366,7 @@ int ima_ctx_create(int device, ima_ctx **out) {
   std::unique_ptr<ima_ctx> c(new ima_ctx);
   c->device = device;
   if (hipStreamCreate(&c->stream) != hipSuccess) return IMA_ERR_HIP;
+  if (hipStreamCreate(&c->side) != hipSuccess) return IMA_ERR_HIP;
   for (int k = 0; k < 8; ++k) {
     if (hipEventCreate(&c->timing.ev0[k]) != hipSuccess) return IMA_ERR_HIP;
     if (hipEventCreate(&c->timing.ev1[k]) != hipSuccess) return IMA_ERR_HIP;
@@ -416,6 +419,7 @@ void ima_ctx_destroy(ima_ctx *c) {
     hipEventDestroy(c->timing.ev1[k]);
   }
   hipStreamDestroy(c->stream);
+  if (c->side) hipStreamDestroy(c->side);
   delete c;
 }
 
@@ -914,7 +918,7 @@ int ima_lmode_table(ima_ctx *c, const float *table, int G, int K, int npsize, in
   if (!c || !table || !table_id || G < 1 || npsize < 0 || nmig < 0) return IMA_ERR_ARG;
   if (npsize + nmig > 16 || npsize + nmig < 1) return fail(c, IMA_ERR_SIZE, "1..16 parameters supported");
   if (K < 4 * npsize + 3 * nmig + 2) return fail(c, IMA_ERR_ARG, "K=%d too small for the layout", K);
-  if ((size_t)128 * K * 4 > (size_t)c->max_lds) return fail(c, IMA_ERR_SIZE, "row too long");
+  if ((size_t)8 * K * 8 > (size_t)c->max_lds) return fail(c, IMA_ERR_SIZE, "row too long");
   hipSetDevice(c->device);
   LTableHost t;
   HIPCHK(c, hipMalloc(&t.rows, (size_t)G * K * sizeof(float)));
@@ -1024,11 +1028,16 @@ static void mc3_release(ima_ctx *c) {
     m.exec[k] = nullptr;
     m.graph[k] = nullptr;
   }
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 4; ++k) {
     if (m.ev[k]) {
       hipEventDestroy(m.ev[k]);
       m.ev[k] = nullptr;
     }
+    if (m.fj[k]) {
+      hipEventDestroy(m.fj[k]);
+      m.fj[k] = nullptr;
+    }
+  }
   release(m.pdg);
   release(m.newpdg);
   release(m.beta);
@@ -1221,7 +1230,12 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.P = 1;
   d.R = 0;
   d.nparam = 0;
-  for (int k = 0; k < 4; ++k) HIPCHK(c, hipEventCreate(&m.ev[k]));
+  d.pchunk = 1;
+  d.mints = 0;
+  for (int k = 0; k < 4; ++k) {
+    HIPCHK(c, hipEventCreate(&m.ev[k]));
+    HIPCHK(c, hipEventCreateWithFlags(&m.fj[k], hipEventDisableTiming));
+  }
   // initial likelihood of every genealogy
   const DeviceView v = view(c);
   if (m.nh)
@@ -1243,7 +1257,7 @@ static int mc3_enqueue(ima_ctx *c, bool with_scale, hipEvent_t *score_ev) {
   HIPCHK(c, enqueue_mc3_sweep(view(c), m.dev, with_scale, m.items.p, m.idx.p, m.plan, m.nh,
                               m.ni, m.ns,
                               m.precision, m.hmax_n, m.hmax_slots, m.imax_n, m.imax_w,
-                              m.swaptries, score_ev, c->stream));
+                              m.swaptries, score_ev, c->stream, c->side, m.fj));
   return IMA_OK;
 }
 
@@ -1369,8 +1383,12 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   }
   if (treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "a genealogy has %d prior events", emax);
-  // per-chain kernels: static swap staging (2 x 2048 doubles) + prior state
-  if (mc3_prior_lds_bytes(R, np) + 2 * 2048 * 8 + 1024 > (size_t)c->max_lds)
+  // per-chain kernels: static swap staging + prior state + a chunk of loci
+  const size_t fixed = 2 * (size_t)kSwapLds * 8 + 2048;
+  int pchunk = 64;
+  while (pchunk > 1 && mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
+    pchunk /= 2;
+  if (mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "prior state of %d populations exceeds LDS", P);
   grow_logfact(c, lfn);
   int rc;
@@ -1420,6 +1438,8 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   d.tv_row = m.tv_row.p;
   d.tw_slots = m.tw_slots.p;
   d.wstatus = m.wstatus.p;
+  d.pchunk = pchunk;
+  d.mints = mm.nints;
   HIPCHK(c, enqueue_mc3_prior_init(view(c), d, c->stream));
   std::vector<int32_t> st(N);
   HIPCHK(c, hipMemcpyAsync(st.data(), m.wstatus.p, N * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1532,6 +1552,7 @@ int ima_model_add(ima_ctx *c, int npops, const int32_t *plist, const int32_t *np
   I.insert(I.end(), mig_wp, mig_wp + 3 * mig_off[nmig]);
   c->mdbl.insert(c->mdbl.end(), qmax, qmax + npsize);
   c->mdbl.insert(c->mdbl.end(), mmax, mmax + nmig);
+  mm.nints = (int32_t)((int64_t)I.size() - mm.ioff);
   *model = (int)c->models.size();
   c->models.push_back(mm);
   c->models_dirty = true;

@@ -59,7 +59,7 @@ struct SlotMeta {
 //   mig_off[nmig+1] | mig_wp[][3] = (k, i, j)
 // and qmax[npsize] | mmax[nmig] in the double block at doff.
 struct ModelMeta {
-  int32_t npops, npsize, nmig, pad;
+  int32_t npops, npsize, nmig, nints;  // nints: length of the int block
   int64_t ioff, doff;
 };
 struct ModelView {
@@ -239,6 +239,27 @@ struct MC3Dev {
   int32_t *tv_row;       // [C*L] chain of each item
   int32_t *tw_slots;     // [C*L] slot of each item
   int32_t *wstatus;      // [C*L] weights status of the proposals
+  int pchunk;            // loci prefetched per chunk by the accept kernel
+  int mints;             // length of the model's int block (staged in LDS)
+};
+
+// Dynamic LDS of the per-chain prior kernels (kern_mc3.hip), the same on
+// host and device: doubles all[R] hold[R] | qt qf nt nf [np] | npd pdg U
+// [LC] | dold dnew [LC][cap] | qmax/mmax, log Gamma cache [np], then ints
+// qc nc [np] | ok nch dec [LC] | didx [LC][cap] | aff [LC][aw] | the
+// model's int block [mints] | counts of the log Gamma cache [np].
+constexpr int kPriorCap = 24;  // changed weight entries kept per locus
+struct PriorLds {
+  int R, np, LC, cap, aw, mints;
+  __host__ __device__ PriorLds(int R_, int np_, int LC_, int mints_)
+      : R(R_), np(np_), LC(LC_), cap(kPriorCap), aw((np_ + 31) / 32), mints(mints_) {}
+  __host__ __device__ size_t doubles() const {
+    return (size_t)2 * R + 6 * np + 3 * LC + (size_t)2 * LC * cap;
+  }
+  __host__ __device__ size_t ints() const {
+    return (size_t)3 * np + (size_t)LC * (3 + cap + aw) + mints;
+  }
+  __host__ __device__ size_t bytes() const { return doubles() * 8 + ints() * 4; }
 };
 __host__ __device__ inline int mc3_tw(int P) { return P > 1 ? P - 1 : 1; }
 
@@ -246,11 +267,12 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              const ima_item *items, const int32_t *idx, const HkyPlan &plan,
                              int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots, int imax_n,
                              int imax_w, int swaptries, hipEvent_t *score_ev,
-                             hipStream_t st);
+                             hipStream_t st, hipStream_t side, hipEvent_t *fj);
 
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
-size_t mc3_prior_lds_bytes(int R, int nparam);
+size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
+constexpr int kSwapLds = 1024;  // chains whose S and beta the swap stages in LDS
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 
 // Swap (kern_swap.hip).

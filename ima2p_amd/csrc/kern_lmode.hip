@@ -8,10 +8,13 @@
 // each), qintegrate (npsize), mintegrate (nmig), pdg, probg, split times.
 //
 // Mapping: one thread per query point, one workgroup per (64-point tile,
-// chunk of rows).  The workgroup stages its rows in LDS and every lane
-// walks them in order, so each row is read from HBM once per point tile
-// and broadcast from LDS.  Chunk partials are merged by a second kernel in
-// chunk order, which keeps the result independent of scheduling.
+// chunk of rows).  The workgroup stages its rows in LDS, converted to FP64
+// once, and every lane walks them in order, so each row is read from HBM
+// once per point tile and broadcast from LDS.  A point's per-parameter
+// coefficients sit in registers (fully unrolled parameter loop).  Chunk
+// partials are merged by a second kernel in chunk order.  The chunk size
+// depends on the table only, never on how many points share a launch, so
+// a point's result is the same alone and batched with others.
 #include <float.h>
 
 #include "ima_internal.h"
@@ -40,30 +43,37 @@ __host__ __device__ inline JointLayout joint_layout(int npsize, int nmig) {
   return L;
 }
 
-struct PointTerms {
-  double log2diffx[kMaxParams], logx[kMaxParams], divx[kMaxParams], x[kMaxParams];
+// Per-point coefficients of the parameters in [r0, r1): a1 multiplies the
+// count (cc: log 2 - log x when population sizes are in the model, else 0;
+// mc: log x), a2 the weight (fc: 1/x; fm: x).
+struct PointCoef {
+  double a1[kMaxParams], a2[kMaxParams];
 };
 
-__device__ inline void point_terms(const double *xp, int npsize, int np, int modeltype,
-                                   int r0, int r1, PointTerms &T) {
-  for (int i = 0; i < np; ++i) {
-    T.x[i] = xp[i];
-    T.log2diffx[i] = 0.0;
-    T.logx[i] = 0.0;
-    T.divx[i] = 0.0;
-  }
-  if (modeltype == 0 || modeltype == 1)
-    for (int i = 0; i < npsize; ++i) T.log2diffx[i] = kLn2 - log(xp[i]);
-  for (int i = r0; i < r1; ++i) {
-    T.logx[i] = log(xp[i]);
-    T.divx[i] = 1.0 / xp[i];
+__device__ inline void point_coef(const double *xp, int npsize, int np, int modeltype, int r0,
+                                  int r1, PointCoef &T) {
+#pragma unroll
+  for (int i = 0; i < kMaxParams; ++i) {
+    double a1 = 0.0, a2 = 0.0;
+    if (i < np) {
+      const double x = xp[i];
+      if (i < npsize) {
+        if (modeltype == 0 || modeltype == 1) a1 = kLn2 - log(x);
+        if (i >= r0 && i < r1) a2 = 1.0 / x;
+      } else if (i >= r0 && i < r1) {
+        a1 = log(x);
+        a2 = x;
+      }
+    }
+    T.a1[i] = a1;
+    T.a2[i] = a2;
   }
 }
 
 // Per-genealogy term of jointp (jointfind.cpp:966-995), same operation order.
-__device__ inline double joint_term(const float *g, const JointLayout &L, int npsize,
+__device__ inline double joint_term(const double *g, const JointLayout &L, int npsize,
                                     int nmig, int npops, int modeltype, int r0, int r1,
-                                    const PointTerms &T) {
+                                    const PointCoef &T) {
   double p;
   if (npops == 2) {
     p = -g[L.probgp];
@@ -75,13 +85,16 @@ __device__ inline double joint_term(const float *g, const JointLayout &L, int np
       for (int i = 0; i < nmig; ++i) s += g[L.mip + i];
     p = -s;
   }
-  for (int i = r0; i < r1; ++i) {
-    if (i < npsize) {
-      const double dg = 2.0 * g[L.fcp + i];
-      p += g[L.ccp + i] * T.log2diffx[i] - g[L.hccp + i] - dg * T.divx[i];
-    } else {
-      const int i1 = i - npsize;
-      p += g[L.mcp + i1] * T.logx[i] - g[L.fmp + i1] * T.x[i];
+#pragma unroll
+  for (int i = 0; i < kMaxParams; ++i) {
+    if (i >= r0 && i < r1) {
+      if (i < npsize) {
+        const double dg = 2.0 * g[L.fcp + i];
+        p += g[L.ccp + i] * T.a1[i] - g[L.hccp + i] - dg * T.a2[i];
+      } else {
+        const int i1 = i - npsize;
+        p += g[L.mcp + i1] * T.a1[i] - g[L.fmp + i1] * T.a2[i];
+      }
     }
   }
   return p;
@@ -127,33 +140,34 @@ struct LArgs {
   const float *rows;
   int G, K, npsize, nmig, npops, modeltype, r0, r1;
   const double *x;  // [npoints][np]
-  int npoints, nchunks, chunk_rows;
+  int npoints, nchunks, chunk_rows, stage;
 };
 
-// Stage rows [g0, g0+cnt) into LDS.
-__device__ inline void stage_rows(const LArgs &a, float *sh, int g0, int cnt) {
+// Stage rows [g0, g0+cnt) into LDS as FP64 (the float to double conversion
+// is exact, so every lane sees the table's values).
+__device__ inline void stage_rows(const LArgs &a, double *sh, int g0, int cnt) {
   const int tot = cnt * a.K;
   const float *src = a.rows + (size_t)g0 * a.K;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) sh[i] = src[i];
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) sh[i] = (double)src[i];
 }
 
 // --------------------------------------------------------------- EXACT --
 // Per (point, chunk): running max M and s = sum exp(p - M), s2 = sum
 // exp(2(p - M)) for the effective sample number.
 __global__ __launch_bounds__(kLTile) void jointp_exact_partial(LArgs a, double *work) {
-  extern __shared__ float shrows[];
+  extern __shared__ double shrows[];
   const int pt = blockIdx.x * kLTile + threadIdx.x;
   const int chunk = blockIdx.y;
   const int np = a.npsize + a.nmig;
   const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointTerms T;
+  PointCoef T;
   const bool active = pt < a.npoints;
-  point_terms(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  point_coef(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
   double M = -DBL_MAX, s = 0.0, s2 = 0.0;
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
-  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
-    const int cnt = min(kLStage, gend - g0);
+  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
+    const int cnt = min(a.stage, gend - g0);
     __syncthreads();
     stage_rows(a, shrows, g0, cnt);
     __syncthreads();
@@ -211,19 +225,19 @@ __global__ void jointp_exact_final(LArgs a, const double *work, double *out, dou
 // pass 3: merge chunks in order and apply the list cap.
 
 __global__ __launch_bounds__(kLTile) void jointp_ref_pass1(LArgs a, double *cmax) {
-  extern __shared__ float shrows[];
+  extern __shared__ double shrows[];
   const int pt = blockIdx.x * kLTile + threadIdx.x;
   const int chunk = blockIdx.y;
   const int np = a.npsize + a.nmig;
   const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointTerms T;
+  PointCoef T;
   const bool active = pt < a.npoints;
-  point_terms(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  point_coef(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
   double M = -DBL_MAX;
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
-  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
-    const int cnt = min(kLStage, gend - g0);
+  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
+    const int cnt = min(a.stage, gend - g0);
     __syncthreads();
     stage_rows(a, shrows, g0, cnt);
     __syncthreads();
@@ -244,15 +258,15 @@ struct RefPartial {
 
 __global__ __launch_bounds__(kLTile) void jointp_ref_pass2(LArgs a, const double *cmax,
                                                            RefPartial *part) {
-  extern __shared__ float shrows[];
+  extern __shared__ double shrows[];
   const int pt = blockIdx.x * kLTile + threadIdx.x;
   const int chunk = blockIdx.y;
   const int np = a.npsize + a.nmig;
   const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointTerms T;
+  PointCoef T;
   const bool active = pt < a.npoints;
   const int ptc = active ? pt : 0;
-  point_terms(a.x + (size_t)ptc * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
+  point_coef(a.x + (size_t)ptc * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
   const double *cm = cmax + (size_t)ptc * a.nchunks;
   double Mfin = -DBL_MAX, run = -DBL_MAX;
   for (int c = 0; c < a.nchunks; ++c) {
@@ -266,8 +280,8 @@ __global__ __launch_bounds__(kLTile) void jointp_ref_pass2(LArgs a, const double
   RefPartial r{0.0, 0.0, DBL_MAX, 0.0, 0, 0};
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
-  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
-    const int cnt = min(kLStage, gend - g0);
+  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
+    const int cnt = min(a.stage, gend - g0);
     __syncthreads();
     stage_rows(a, shrows, g0, cnt);
     __syncthreads();
@@ -332,15 +346,24 @@ __global__ void jointp_ref_final(LArgs a, const double *cmax, const RefPartial *
   if (ess) ess[pt] = sum * sum / sum2;
 }
 
+// Rows staged in LDS at a time (FP64 rows, at most 32 KB) and rows per
+// chunk: a function of the table alone, aiming at about 256 chunks.
+static int stage_of(const LTable &t) {
+  int st = 32768 / (8 * t.K);
+  st = st >= kLStage ? kLStage : (st / 8) * 8;
+  return st < 8 ? 8 : st;
+}
 static void chunking(const LTable &t, int npoints, int *nchunks, int *chunk_rows) {
-  const int tiles = (npoints + kLTile - 1) / kLTile;
-  int want = (2048 + tiles - 1) / tiles;  // aim at >= 2048 workgroups
-  int rows = (t.G + want - 1) / want;
-  rows = ((rows + kLStage - 1) / kLStage) * kLStage;
-  if (rows < kLStage) rows = kLStage;
+  (void)npoints;
+  const int st = stage_of(t);
+  int rows = (t.G + 255) / 256;
+  rows = ((rows + st - 1) / st) * st;
+  if (rows < st) rows = st;
   *chunk_rows = rows;
   *nchunks = (t.G + rows - 1) / rows;
 }
+
+size_t jointp_lds_bytes(const LTable &t) { return (size_t)stage_of(t) * t.K * 8; }
 
 size_t jointp_work_bytes(const LTable &t, int npoints) {
   int nc, cr;
@@ -352,11 +375,12 @@ hipError_t launch_jointp(const LTable &t, const double *x, int npoints, int mode
                          int r0, int r1, int mode, double *work, size_t work_bytes,
                          double *out, double *ess, hipStream_t st) {
   if (npoints == 0) return hipSuccess;
-  LArgs a{t.rows, t.G, t.K, t.npsize, t.nmig, t.npops, modeltype, r0, r1, x, npoints, 0, 0};
+  LArgs a{t.rows, t.G, t.K, t.npsize, t.nmig, t.npops, modeltype, r0, r1, x, npoints, 0, 0,
+          stage_of(t)};
   chunking(t, npoints, &a.nchunks, &a.chunk_rows);
   if (jointp_work_bytes(t, npoints) > work_bytes) return hipErrorInvalidValue;
   const dim3 grid((npoints + kLTile - 1) / kLTile, a.nchunks);
-  const size_t lds = (size_t)kLStage * t.K * sizeof(float);
+  const size_t lds = jointp_lds_bytes(t);
   const int fin_blocks = (npoints + 255) / 256;
   if (mode == IMA_LMODE_EXACT) {
     hipLaunchKernelGGL(jointp_exact_partial, grid, dim3(kLTile), lds, st, a, work);
@@ -380,11 +404,12 @@ struct MArgs {
   int G, K, npsize, nmig, param, expo;
   double meani;
   const double *x;
-  int nx, nchunks, chunk_rows;
+  int nx, nchunks, chunk_rows, stage;
 };
 
 __global__ __launch_bounds__(kLTile) void margin_partial(MArgs a, double *work) {
-  extern __shared__ float shrows[];
+  extern __shared__ double shrows_raw[];
+  float *shrows = reinterpret_cast<float *>(shrows_raw);
   const int pt = blockIdx.x * kLTile + threadIdx.x;
   const int chunk = blockIdx.y;
   const bool active = pt < a.nx;
@@ -394,8 +419,8 @@ __global__ __launch_bounds__(kLTile) void margin_partial(MArgs a, double *work) 
   double s = 0.0;
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
-  for (int g0 = gbeg; g0 < gend; g0 += kLStage) {
-    const int cnt = min(kLStage, gend - g0);
+  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
+    const int cnt = min(a.stage, gend - g0);
     __syncthreads();
     const int tot = cnt * a.K;
     const float *src = a.rows + (size_t)g0 * a.K;
@@ -446,10 +471,10 @@ hipError_t launch_margincalc(const LTable &t, int param, const double *x, int nx
                              double *out, hipStream_t st) {
   if (nx == 0) return hipSuccess;
   MArgs a{t.rows, t.G, t.K, t.npsize, t.nmig, param, expo,
-          expo ? 1.0 / prior_mean : 1.0, x, nx, 0, 0};
+          expo ? 1.0 / prior_mean : 1.0, x, nx, 0, 0, stage_of(t)};
   chunking(t, nx, &a.nchunks, &a.chunk_rows);
   const dim3 grid((nx + kLTile - 1) / kLTile, a.nchunks);
-  const size_t lds = (size_t)kLStage * t.K * sizeof(float);
+  const size_t lds = (size_t)a.stage * t.K * sizeof(float);
   hipLaunchKernelGGL(margin_partial, grid, dim3(kLTile), lds, st, a, work);
   hipLaunchKernelGGL(margin_final, dim3((nx + 255) / 256), dim3(256), 0, st, a, logi,
                      yadjust, work, out);

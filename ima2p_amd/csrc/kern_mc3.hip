@@ -187,7 +187,6 @@ __device__ inline double chain_log_scale(const MC3Dev &m, int c, uint64_t sweep)
 // = 7 unless fewer than 2*SWAPDIST+3 chains, w = exp((b_i - b_j)(S_j - S_i))
 // (swapweight :13-34), accept when w >= 1 or w > U, swap the betas.
 // S and beta are staged in LDS (up to kSwapLds chains) first.
-constexpr int kSwapLds = 2048;
 __device__ void swap_chains(const MC3Dev &m, int swaptries, uint64_t sweep, double *Ssh,
                             double *Bsh) {
   const bool staged = m.C <= kSwapLds;
@@ -274,41 +273,64 @@ __device__ inline double wsum_update(int r, int P, double all, double old, doubl
   return (all - old) + nw;
 }
 
-// Dynamic LDS of the per-chain prior kernels: all[R] hold[R] | qt qf nt nf
-// [nparam] doubles | qc nc [nparam] ints.
-__host__ __device__ inline size_t mc3_prior_lds(int R, int np) {
-  return (size_t)(2 * R + 4 * np) * 8 + (size_t)2 * np * 4;
+// Pointers into the dynamic LDS of the per-chain prior kernels (PriorLds).
+struct PriorSh {
+  double *all, *hold, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *lga;
+  int *qc, *nc, *ok, *nch, *dec, *didx, *mi, *lgc;
+  uint32_t *aff;
+};
+__device__ inline PriorSh prior_carve(double *dyn, const PriorLds &Lz) {
+  PriorSh p;
+  p.all = dyn;
+  p.hold = p.all + Lz.R;
+  p.qt = p.hold + Lz.R;
+  p.qf = p.qt + Lz.np;
+  p.nt = p.qf + Lz.np;
+  p.nf = p.nt + Lz.np;
+  p.npd = p.nf + Lz.np;
+  p.pdg = p.npd + Lz.LC;
+  p.U = p.pdg + Lz.LC;
+  p.dold = p.U + Lz.LC;
+  p.dnew = p.dold + (size_t)Lz.LC * Lz.cap;
+  p.mdl = p.dnew + (size_t)Lz.LC * Lz.cap;
+  p.lga = p.mdl + Lz.np;
+  p.qc = reinterpret_cast<int *>(p.lga + Lz.np);
+  p.nc = p.qc + Lz.np;
+  p.ok = p.nc + Lz.np;
+  p.nch = p.ok + Lz.LC;
+  p.dec = p.nch + Lz.LC;
+  p.didx = p.dec + Lz.LC;
+  p.aff = reinterpret_cast<uint32_t *>(p.didx + (size_t)Lz.LC * Lz.cap);
+  p.mi = reinterpret_cast<int *>(p.aff + (size_t)Lz.LC * Lz.aw);
+  p.lgc = p.mi + Lz.mints;
+  return p;
 }
 
 // All-loci sums of chain c from the records w (loci in order), every
 // parameter's (count, weight, integral) and probg = sum of the integrals;
 // called by the whole workgroup.
 __device__ double chain_prior_fresh(const DeviceView &v, const MC3Dev &m, int c,
-                                    const double *w, double *all, double *nt, double *nf,
-                                    int *nc) {
+                                    const double *w, const PriorSh &sh) {
   __shared__ double pg;
   const int R = m.R, np = m.nparam;
   const ModelMeta mm = v.models[m.prior];
   const ModelView md = model_view(v, mm);
-  for (int r = threadIdx.x; r < R; r += blockDim.x) {
-    double s = 0.0;
-    for (int l = 0; l < m.L; ++l) s += w[(int64_t)(c * m.L + l) * R + r];
-    all[r] = s;
-  }
+  for (int r = threadIdx.x; r < R; r += blockDim.x)
+    sh.all[r] = ordered_rec_sum(w + (int64_t)c * m.L * R, R, m.L, r);
   __syncthreads();
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int cc;
     double f, h;
-    prior_param_sums(md, m.P, mm.npsize, t, all, &cc, &f, &h);
-    nc[t] = cc;
-    nf[t] = f;
-    nt[t] = prior_param_term(md, mm.npsize, t, cc, f, h, v.logfact);
+    prior_param_sums(md, m.P, mm.npsize, t, sh.all, &cc, &f, &h);
+    sh.nc[t] = cc;
+    sh.nf[t] = f;
+    sh.nt[t] = prior_param_term(md, mm.npsize, t, cc, f, h, v.logfact);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double q = 0.0, mg = 0.0;
-    for (int t = 0; t < mm.npsize; ++t) q += nt[t];
-    for (int t = mm.npsize; t < np; ++t) mg += nt[t];
+    for (int t = 0; t < mm.npsize; ++t) q += sh.nt[t];
+    for (int t = mm.npsize; t < np; ++t) mg += sh.nt[t];
     pg = q + mg;
   }
   __syncthreads();
@@ -319,96 +341,233 @@ __device__ double chain_prior_fresh(const DeviceView &v, const MC3Dev &m, int c,
 __global__ __launch_bounds__(256) void mc3_prior_init(DeviceView v, MC3Dev m) {
   extern __shared__ double dyn[];
   const int c = blockIdx.x, R = m.R, np = m.nparam;
-  double *all = dyn, *nt = all + 2 * R + 2 * np, *nf = nt + np;
-  int *nc = reinterpret_cast<int *>(nf + np) + np;
-  const double pg = chain_prior_fresh(v, m, c, m.wrec, all, nt, nf, nc);
-  for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = all[r];
+  const PriorSh sh = prior_carve(dyn, PriorLds(R, np, m.pchunk, m.mints));
+  const double pg = chain_prior_fresh(v, m, c, m.wrec, sh);
+  for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = sh.all[r];
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    m.qterm[c * np + t] = nt[t];
-    m.qf[c * np + t] = nf[t];
-    m.qc[c * np + t] = nc[t];
+    m.qterm[c * np + t] = sh.nt[t];
+    m.qf[c * np + t] = sh.nf[t];
+    m.qc[c * np + t] = sh.nc[t];
   }
   if (threadIdx.x == 0) m.probg[c] = pg;
 }
 
-// Loci of chain c in order, each accepted with beta (dprobg + dpdg); the
-// first wavefront does the work, the workgroup waits at the end.
+// True when parameter t adds up one of the record entries in idx[0..n).
+__device__ inline bool param_touches(const ModelView &md, int P, int npsize, int t,
+                                     const int *idx, int n) {
+  const int P2 = P * P, P3 = P2 * P;
+  if (t < npsize) {
+    for (int w = md.theta_off[t]; w < md.theta_off[t + 1]; ++w) {
+      const int e = md.theta_wp[2 * w] * P + md.theta_wp[2 * w + 1];
+      for (int i = 0; i < n; ++i) {
+        const int r = idx[i];
+        if (r == e || r == P2 + e || r == 2 * P2 + e) return true;
+      }
+    }
+  } else {
+    const int mm = t - npsize;
+    for (int w = md.mig_off[mm]; w < md.mig_off[mm + 1]; ++w) {
+      const int e = (md.mig_wp[3 * w] * P + md.mig_wp[3 * w + 1]) * P + md.mig_wp[3 * w + 2];
+      for (int i = 0; i < n; ++i) {
+        const int r = idx[i];
+        if (r == 3 * P2 + e || r == 3 * P2 + P3 + e) return true;
+      }
+    }
+  }
+  return false;
+}
+
+// Loci of chain c in order, each accepted with beta (dprobg + dpdg).  The
+// work that does not depend on earlier decisions is done in parallel, a
+// chunk of loci at a time: each wavefront reads one locus' likelihood,
+// acceptance uniform and the weight entries its proposal changed (up to
+// kPriorCap, else the locus takes the full record from memory) and marks
+// the parameters those entries feed.  The first wavefront then decides the
+// chunk's loci in order from LDS only, recomputing just the marked
+// parameters' integrals; commits and restores are applied in parallel.
 __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
                              double *dyn) {
   const int R = m.R, np = m.nparam, P = m.P;
-  double *all = dyn, *hold = all + R, *qt = hold + R, *qf = qt + np, *nt = qf + np,
-         *nf = nt + np;
-  int *qc = reinterpret_cast<int *>(nf + np), *nc = qc + np;
-  const ModelMeta mm = v.models[m.prior];
-  const ModelView md = model_view(v, mm);
-  for (int r = threadIdx.x; r < R; r += blockDim.x) all[r] = m.wall[(int64_t)c * R + r];
+  const PriorLds Lz(R, np, m.pchunk, m.mints);
+  const PriorSh sh = prior_carve(dyn, Lz);
+  // the model's maps and each parameter's log Gamma live in LDS, so the
+  // in-order loop below reads no global memory
+  ModelMeta mm = v.models[m.prior];
+  for (int x = threadIdx.x; x < m.mints; x += blockDim.x) sh.mi[x] = v.mint[mm.ioff + x];
+  for (int t = threadIdx.x; t < np; t += blockDim.x) sh.mdl[t] = v.mdbl[mm.doff + t];
+  mm.ioff = 0;
+  mm.doff = 0;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  for (int r = threadIdx.x; r < R; r += blockDim.x) sh.all[r] = m.wall[(int64_t)c * R + r];
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    qt[t] = m.qterm[c * np + t];
-    qf[t] = m.qf[c * np + t];
-    qc[t] = m.qc[c * np + t];
+    sh.qt[t] = m.qterm[c * np + t];
+    sh.qf[t] = m.qf[c * np + t];
+    const int cq = m.qc[c * np + t];
+    sh.qc[t] = cq;
+    sh.lgc[t] = cq;
+    sh.lga[t] = prior_param_lga(mm.npsize, t, cq, v.logfact);
   }
   __syncthreads();
-  if (threadIdx.x < kWave) {
-    const int lane = threadIdx.x;
-    const double beta = m.beta[c];
-    double probg = m.probg[c];
-    for (int l = 0; l < m.L; ++l) {
-      const int it = c * m.L + l;
-      bool acc = false;
-      if (m.status[it] == IMA_STATUS_OK && m.wstatus[it] == IMA_STATUS_OK) {
-        const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
-        for (int r = lane; r < R; r += kWave) {
-          const double a = all[r];
-          hold[r] = a;
-          all[r] = wsum_update(r, P, a, wo[r], wn[r]);
+  const ModelView md = model_view(sh.mi, sh.mdl, mm);
+  double probg = m.probg[c];
+  const double beta = m.beta[c];
+  for (int l0 = 0; l0 < m.L; l0 += Lz.LC) {
+    const int nl = min(Lz.LC, m.L - l0);
+    // ---- parallel: one wavefront per locus of the chunk
+    for (int x = wave; x < nl; x += nwaves) {
+      const int it = c * m.L + l0 + x;
+      const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
+      int count = 0;
+      for (int r0 = 0; r0 < R; r0 += kWave) {
+        const int r = r0 + lane;
+        double o = 0.0, w = 0.0;
+        if (r < R) {
+          o = wo[r];
+          w = wn[r];
         }
-        wave_sync();
-        for (int t = lane; t < np; t += kWave) {
-          int cc;
-          double f, h;
-          prior_param_sums(md, P, mm.npsize, t, all, &cc, &f, &h);
-          nc[t] = cc;
-          nf[t] = f;
-          nt[t] = (cc == qc[t] && f == qf[t]) ? qt[t]
-                                               : prior_param_term(md, mm.npsize, t, cc, f, h,
-                                                                  v.logfact);
+        const bool diff = r < R && o != w;
+        const unsigned long long bal = __ballot(diff);
+        const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+        if (diff && pos < Lz.cap) {
+          sh.didx[x * Lz.cap + pos] = r;
+          sh.dold[x * Lz.cap + pos] = o;
+          sh.dnew[x * Lz.cap + pos] = w;
         }
-        wave_sync();
-        double q = 0.0, mg = 0.0;
-        for (int t = 0; t < mm.npsize; ++t) q += nt[t];
-        for (int t = mm.npsize; t < np; ++t) mg += nt[t];
-        const double pn = q + mg;
-        const double npd = new_loglik(m, it);
-        PhiloxStream r = item_rng(m, it, sweep, 1);
-        const double U = r.uniform();
-        acc = U < fmin(1.0, exp(beta * ((pn - probg) + (npd - m.pdg[it]))));
-        if (acc) {
-          probg = pn;
-          for (int t = lane; t < np; t += kWave) {
-            qt[t] = nt[t];
-            qf[t] = nf[t];
-            qc[t] = nc[t];
-          }
-          double *wc = m.wrec + (int64_t)it * R;
-          for (int rr = lane; rr < R; rr += kWave) wc[rr] = wn[rr];
-          if (lane == 0) m.pdg[it] = npd;
-        } else {
-          for (int rr = lane; rr < R; rr += kWave) all[rr] = hold[rr];
-        }
-        wave_sync();
+        count += __popcll(bal);
       }
       if (lane == 0) {
-        if (!acc) restore_item(v, m, it);
+        sh.ok[x] = m.status[it] == IMA_STATUS_OK && m.wstatus[it] == IMA_STATUS_OK;
+        sh.npd[x] = new_loglik(m, it);
+        sh.pdg[x] = m.pdg[it];
+        PhiloxStream rr = item_rng(m, it, sweep, 1);
+        sh.U[x] = rr.uniform();
+        sh.nch[x] = count;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const bool dense = count > Lz.cap;
+      for (int t0 = 0; t0 < np; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool hit = t < np && (dense || param_touches(md, P, mm.npsize, t,
+                                                          sh.didx + x * Lz.cap, count));
+        const unsigned long long bal = __ballot(hit);
+        if (lane == 0) {
+          sh.aff[x * Lz.aw + t0 / 32] = (uint32_t)bal;
+          if (t0 / 32 + 1 < Lz.aw) sh.aff[x * Lz.aw + t0 / 32 + 1] = (uint32_t)(bal >> 32);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- in order: wave 0 updates the sums and decides; the marked
+    // parameters' integrals are spread one per wavefront, so their
+    // different series run side by side rather than diverging in one wave
+    double hv = 0.0;
+    for (int x = 0; x < nl; ++x) {
+      const int it = c * m.L + l0 + x;
+      const bool ok = sh.ok[x] != 0;
+      const int cnt = sh.nch[x];
+      const bool dense = cnt > Lz.cap;
+      if (wave == 0 && ok) {
+        if (!dense) {
+          if (lane < cnt) {
+            const int r = sh.didx[x * Lz.cap + lane];
+            hv = sh.all[r];
+            sh.all[r] = wsum_update(r, P, hv, sh.dold[x * Lz.cap + lane],
+                                    sh.dnew[x * Lz.cap + lane]);
+          }
+        } else {
+          const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
+          for (int r = lane; r < R; r += kWave) {
+            const double a = sh.all[r];
+            sh.hold[r] = a;
+            sh.all[r] = wsum_update(r, P, a, wo[r], wn[r]);
+          }
+        }
+      }
+      __syncthreads();
+      if (ok) {
+        for (int t = wave; t < np; t += nwaves) {
+          if ((sh.aff[x * Lz.aw + t / 32] >> (t % 32)) & 1u) {
+            int cc;
+            double f, h;
+            prior_param_sums(md, P, mm.npsize, t, sh.all, &cc, &f, &h);
+            double term = sh.qt[t];
+            if (!(cc == sh.qc[t] && f == sh.qf[t])) {
+              const double lga = cc == sh.lgc[t] ? sh.lga[t]
+                                                  : prior_param_lga(mm.npsize, t, cc, v.logfact);
+              term = prior_param_term_lga(md, mm.npsize, t, cc, f, h, lga);
+            }
+            if (lane == 0) {
+              sh.nc[t] = cc;
+              sh.nf[t] = f;
+              sh.nt[t] = term;
+            }
+          } else if (lane == 0) {
+            sh.nc[t] = sh.qc[t];
+            sh.nf[t] = sh.qf[t];
+            sh.nt[t] = sh.qt[t];
+          }
+        }
+      }
+      __syncthreads();
+      if (wave == 0) {
+        if (!ok) {
+          if (lane == 0) sh.dec[x] = 0;
+        } else {
+          double q = 0.0, mg = 0.0;
+          for (int t = 0; t < mm.npsize; ++t) q += sh.nt[t];
+          for (int t = mm.npsize; t < np; ++t) mg += sh.nt[t];
+          const double pn = q + mg;
+          const bool acc =
+              sh.U[x] < fmin(1.0, exp(beta * ((pn - probg) + (sh.npd[x] - sh.pdg[x]))));
+          if (acc) {
+            probg = pn;
+            for (int t = lane; t < np; t += kWave) {
+              sh.qt[t] = sh.nt[t];
+              sh.qf[t] = sh.nf[t];
+              sh.qc[t] = sh.nc[t];
+            }
+          } else if (!dense) {
+            if (lane < cnt) sh.all[sh.didx[x * Lz.cap + lane]] = hv;
+          } else {
+            for (int r = lane; r < R; r += kWave) sh.all[r] = sh.hold[r];
+          }
+          if (lane == 0) sh.dec[x] = acc ? 1 : 0;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- parallel: commit or restore the chunk's loci
+    for (int x = wave; x < nl; x += nwaves) {
+      const int it = c * m.L + l0 + x;
+      const bool acc = sh.dec[x] != 0;
+      if (acc) {
+        double *wc = m.wrec + (int64_t)it * R;
+        const double *wn = m.wnew + (int64_t)it * R;
+        if (sh.nch[x] > Lz.cap) {
+          for (int r = lane; r < R; r += kWave) wc[r] = wn[r];
+        } else if (lane < sh.nch[x]) {
+          wc[sh.didx[x * Lz.cap + lane]] = sh.dnew[x * Lz.cap + lane];
+        }
+      }
+      if (lane == 0) {
+        if (acc)
+          m.pdg[it] = sh.npd[x];
+        else
+          restore_item(v, m, it);
         m.acc[it] += acc ? 1u : 0u;
       }
     }
-    if (lane == 0) m.probg[c] = probg;
-    for (int r = lane; r < R; r += kWave) m.wall[(int64_t)c * R + r] = all[r];
-    for (int t = lane; t < np; t += kWave) {
-      m.qterm[c * np + t] = qt[t];
-      m.qf[c * np + t] = qf[t];
-      m.qc[c * np + t] = qc[t];
-    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) m.probg[c] = probg;
+  for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = sh.all[r];
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    m.qterm[c * np + t] = sh.qt[t];
+    m.qf[c * np + t] = sh.qf[t];
+    m.qc[c * np + t] = sh.qc[t];
   }
   __syncthreads();
 }
@@ -513,11 +672,12 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   }
   a = block_sum(a, sh);
   const int R = m.R, np = m.nparam, tw = mc3_tw(m.P);
-  double *all = dyn, *nt = all + 2 * R + 2 * np, *nf = nt + np;
-  int *nc = reinterpret_cast<int *>(nf + np) + np;
+  const PriorSh psh = prior_carve(dyn, PriorLds(R, np, m.pchunk, m.mints));
+  double *all = psh.all, *nt = psh.nt, *nf = psh.nf;
+  int *nc = psh.nc;
   double pn = 0.0;
   if (prior) {
-    pn = chain_prior_fresh(v, m, c, m.wnew, all, nt, nf, nc);
+    pn = chain_prior_fresh(v, m, c, m.wnew, psh);
     a += m.beta[c] * (pn - m.probg[c]) + (m.P - 1) * m.logs[c];
     if (threadIdx.x == 0)
       for (int k = 0; k < m.P - 1; ++k)
@@ -581,16 +741,26 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              const ima_item *items, const int32_t *idx, const HkyPlan &plan,
                              int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots,
                              int imax_n, int imax_w, int swaptries, hipEvent_t *score_ev,
-                             hipStream_t st) {
+                             hipStream_t st, hipStream_t side, hipEvent_t *fj) {
   // the per-chain kernels merge pattern blocks themselves
   HkyPlan p = plan;
   p.nred = 0;
   const bool prior = m.prior >= 0;
-  const size_t dl = prior ? mc3_prior_lds(m.R, m.nparam) : 0;
+  const size_t dl = prior ? PriorLds(m.R, m.nparam, m.pchunk, m.mints).bytes() : 0;
   // score_ev (optional): events recorded around the two scoring passes
+  // the proposals' genealogy weights run on the side stream beside the
+  // likelihood launch; the accept kernel waits for both
   auto score = [&](int pass) -> hipError_t {
     hipError_t e = hipSuccess;
-    if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);
+    if (prior) {
+      e = hipEventRecord(fj[2 * pass], st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
+      if (e == hipSuccess)
+        e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax, m.wnew,
+                              m.wstatus, side);
+      if (e == hipSuccess) e = hipEventRecord(fj[2 * pass + 1], side);
+    }
+    if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass], st);
     if (e == hipSuccess && nh)
       e = launch_hky(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, st);
     if (e == hipSuccess && ni)
@@ -598,9 +768,7 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
     if (e == hipSuccess && ns)
       e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
     if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
-    if (e == hipSuccess && prior)
-      e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax, m.wnew,
-                            m.wstatus, st);
+    if (e == hipSuccess && prior) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
     return e;
   };
   hipError_t e = score(0);
@@ -619,12 +787,14 @@ hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStrea
   hipError_t e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax,
                                    m.wrec, m.wstatus, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(mc3_prior_init, dim3(m.C), dim3(256), mc3_prior_lds(m.R, m.nparam), st, v,
-                     m);
+  hipLaunchKernelGGL(mc3_prior_init, dim3(m.C), dim3(256),
+                     PriorLds(m.R, m.nparam, m.pchunk, m.mints).bytes(), st, v, m);
   return hipGetLastError();
 }
 
-size_t mc3_prior_lds_bytes(int R, int np) { return mc3_prior_lds(R, np); }
+size_t mc3_prior_lds_bytes(int R, int np, int pchunk, int mints) {
+  return PriorLds(R, np, pchunk, mints).bytes();
+}
 
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st) {
   const int N = m.C * m.L;

@@ -40,6 +40,13 @@ __device__ inline int ev_pack(int type, int pop, int topop) {
   return type | (pop << 4) | ((topop + 1) << 12);
 }
 
+__device__ inline double bcast_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ inline int period_of(const double *tv, int nper, double t) {
   int k = 0;
   while (k < nper - 1 && tv[k] <= t) k++;
@@ -59,7 +66,7 @@ treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *
   const ModelView md = model_view(v, mm);
   const int P = mm.npops, nper = P, T = 2 * P - 1;
   const int R = prior_rec_len(P);
-  const double *tv = tvals + (int64_t)(tv_row ? tv_row[i] : i) * (P > 1 ? P - 1 : 1);
+  const double *tvg = tvals + (int64_t)(tv_row ? tv_row[i] : i) * (P > 1 ? P - 1 : 1);
   const int n = sm.n;
   const int32_t *up0 = v.up0 + sm.node_off;
   const double *tm = v.time + sm.node_off;
@@ -68,6 +75,24 @@ treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *
   const int32_t *mt_to = v.mig_to + sm.mig_off;
   const double *mtime = v.mig_time + sm.mig_off;
   const int nmig = sm.nmig;
+
+  // the model's small tables, read inside the event loops, live in LDS
+  __shared__ int s_pos[kMaxPops * (2 * kMaxPops - 1)], s_join[3 * kMaxPops];
+  __shared__ int s_pope[2 * kMaxPops - 1], s_popdown[2 * kMaxPops - 1];
+  __shared__ int s_npresent[kMaxPops];
+  __shared__ double s_tv[kMaxPops];
+  __shared__ int bad;
+  for (int x = lane; x < P * T; x += kWave) s_pos[x] = md.pos[x];
+  for (int x = lane; x < 3 * P; x += kWave) s_join[x] = md.join[x];
+  for (int x = lane; x < T; x += kWave) {
+    s_pope[x] = md.pope[x];
+    s_popdown[x] = md.popdown[x];
+  }
+  for (int x = lane; x < P; x += kWave) s_npresent[x] = md.npresent[x];
+  for (int x = lane; x < P - 1; x += kWave) s_tv[x] = tvg[x];
+  if (lane == 0) bad = 0;
+  __syncthreads();
+  const double *tv = s_tv;
 
   const double roottime = tm[up0[sm.root]];
   const int kroot = period_of(tv, nper, roottime);
@@ -78,8 +103,6 @@ treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *
   double *st = et + E;                     // [E] sorted times
   int *ek = (int *)(st + E);               // [E] unsorted codes
   int *sk = ek + E;                        // [E] sorted codes
-  __shared__ int bad;
-  if (lane == 0) bad = 0;
   for (int r = lane; r < R; r += kWave) rec[r] = 0.0;
 
   // events
@@ -93,7 +116,7 @@ treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *
     const double t = mtime[j];
     int now = (j > 0 && me[j - 1] == edge) ? mt_to[j - 1] : pop[edge];
     const int k = period_of(tv, nper, t);
-    while (now >= 0 && now < T && md.pope[now] != -1 && md.pope[now] <= k) now = md.popdown[now];
+    while (now >= 0 && now < T && s_pope[now] != -1 && s_pope[now] <= k) now = s_popdown[now];
     et[n - 1 + j] = t;
     ek[n - 1 + j] = ev_pack(EV_MIG, now, mt_to[j]);
   }
@@ -114,72 +137,83 @@ treeweight_kernel(DeviceView v, const int32_t *slots, int nslots, const double *
   }
   __syncthreads();
 
-  // sweep: lane p follows population p
+  // sweep: lane p follows population p; the period's values stay in
+  // registers until a split changes the period
   const int p = lane;
   const bool mine = p < T;
   int cnt = (p < P) ? lm.samppop[p] : 0;
   int nsum = n, k = 0, cc = 0;
   double last = 0.0, len = 0.0, fcacc = 0.0, fmacc = 0.0;
   const double h2term = 1 / (2 * lm.hval);
-  for (int j = 0; j < E; ++j) {
-    const double t = st[j];
-    const int code = sk[j];
-    const double dt = t - last;
-    len += nsum * dt;
-    last = t;
-    const int ip = mine ? md.pos[k * T + p] : -1;
-    if (ip >= 0) {
-      fcacc += ((double)cnt * ((double)cnt - 1)) * dt * h2term;
-      if (k < nper - 1) fmacc += cnt * dt;
+  int ip = mine ? s_pos[p] : -1;
+  int ja = s_join[0], jb_ = s_join[1], jn = s_join[2];
+  // the sorted events are read from registers: lane j holds event jb + j
+  // of the current block and v_readlane broadcasts it
+  for (int jb = 0; jb < E; jb += kWave) {
+    const int nb = min(kWave, E - jb);
+    double t_reg = 0.0;
+    int c_reg = 0;
+    if (lane < nb) {
+      t_reg = st[jb + lane];
+      c_reg = sk[jb + lane];
     }
-    const int type = code & 15, a = (code >> 4) & 255, b = (code >> 12) - 1;
-    if (type == EV_COAL) {
-      if (p == a) {
-        if (ip < 0) bad = 1;
-        cc++;
-        cnt--;
-      }
-      nsum--;
-    } else if (type == EV_MIG) {
-      if (p == a) {
-        const int ia = md.pos[k * T + a];
-        const int ib = (b >= 0 && b < T) ? md.pos[k * T + b] : -1;
-        if (ia < 0 || ib < 0)
-          bad = 1;
-        else
-          rec[3 * P * P + (k * P + ia) * P + ib] += 1.0;
-        cnt--;
-      }
-      if (p == b) cnt++;
-    }
-    // population counts before a split are shared through the wave
-    const int ja = md.join[3 * k], jb = md.join[3 * k + 1], jn = md.join[3 * k + 2];
-    const int na = __shfl(cnt, ja < 0 ? 0 : ja, kWave);
-    const int nb = __shfl(cnt, jb < 0 ? 0 : jb, kWave);
-    if (type == EV_SPLIT) {
+    for (int jj = 0; jj < nb; ++jj) {
+      const double t = bcast_d(t_reg, jj);
+      const int code = __builtin_amdgcn_readlane(c_reg, jj);
+      const double dt = t - last;
+      len += nsum * dt;
+      last = t;
       if (ip >= 0) {
-        rec[k * P + ip] = (double)cc;
-        rec[P * P + k * P + ip] = fcacc;
-        if (k < nper - 1)
-          for (int jj = 0; jj < md.npresent[k]; ++jj)
-            if (jj != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + jj] = fmacc;
+        fcacc += ((double)cnt * ((double)cnt - 1)) * dt * h2term;
+        if (k < nper - 1) fmacc += cnt * dt;
       }
-      if (p == jn) cnt = na + nb;
-      if (p == ja || p == jb) cnt = 0;
-      cc = 0;
-      fcacc = fmacc = 0.0;
-      k++;
+      const int type = code & 15, a = (code >> 4) & 255, b = (code >> 12) - 1;
+      if (type == EV_COAL) {
+        if (p == a) {
+          if (ip < 0) bad = 1;
+          cc++;
+          cnt--;
+        }
+        nsum--;
+      } else if (type == EV_MIG) {
+        if (p == a) {
+          const int ib = (b >= 0 && b < T) ? s_pos[k * T + b] : -1;
+          if (ip < 0 || ib < 0)
+            bad = 1;
+          else
+            rec[3 * P * P + (k * P + ip) * P + ib] += 1.0;
+          cnt--;
+        }
+        if (p == b) cnt++;
+      } else {
+        // split k: the new population takes the two merged counts
+        const int na = __shfl(cnt, ja < 0 ? 0 : ja, kWave);
+        const int nbb = __shfl(cnt, jb_ < 0 ? 0 : jb_, kWave);
+        if (ip >= 0) {
+          rec[k * P + ip] = (double)cc;
+          rec[P * P + k * P + ip] = fcacc;
+          if (k < nper - 1)
+            for (int q = 0; q < s_npresent[k]; ++q)
+              if (q != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + q] = fmacc;
+        }
+        if (p == jn) cnt = na + nbb;
+        if (p == ja || p == jb_) cnt = 0;
+        cc = 0;
+        fcacc = fmacc = 0.0;
+        k++;
+        ip = mine ? s_pos[k * T + p] : -1;
+        ja = s_join[3 * k];
+        jb_ = s_join[3 * k + 1];
+        jn = s_join[3 * k + 2];
+      }
     }
   }
-  {
-    const int ip = mine ? md.pos[k * T + p] : -1;
-    if (ip >= 0) {
-      rec[k * P + ip] = (double)cc;
-      rec[P * P + k * P + ip] = fcacc;
-      if (k < nper - 1)
-        for (int jj = 0; jj < md.npresent[k]; ++jj)
-          if (jj != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + jj] = fmacc;
-    }
+  if (ip >= 0) {
+    rec[k * P + ip] = (double)cc;
+    rec[P * P + k * P + ip] = fcacc;
+    if (k < nper - 1)
+      for (int jj = 0; jj < s_npresent[k]; ++jj)
+        if (jj != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + jj] = fmacc;
   }
   __syncthreads();
   const double hlog = log(lm.hval);
@@ -203,11 +237,7 @@ prior_kernel(DeviceView v, int model, const double *recs, const int32_t *group_o
   const ModelView md = model_view(v, mm);
   const int P = mm.npops, R = prior_rec_len(P);
   const int s0 = group_off[g], s1 = group_off[g + 1];
-  for (int r = lane; r < R; r += kWave) {
-    double s = 0.0;
-    for (int i = s0; i < s1; ++i) s += recs[(int64_t)i * R + r];
-    sums[r] = s;
-  }
+  for (int r = lane; r < R; r += kWave) sums[r] = ordered_rec_sum(recs + (int64_t)s0 * R, R, s1 - s0, r);
   __syncthreads();
   double *terms = sums + R;  // [npsize + nmig]
   const int np = mm.npsize, nm = mm.nmig;

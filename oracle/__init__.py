@@ -73,6 +73,12 @@ def _load():
     lib.ora_margincalc.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_int, C.c_double]
+    lib.ora_mc3_bench.restype = C.c_long
+    lib.ora_mc3_bench.argtypes = ([C.c_int, C.c_int] + [ip] * 7 + [dp, dp, C.c_double, ip, C.c_int]
+                                  + [ip] * 5 + [C.c_int, ip, ip, C.c_int, ip, ip, dp, dp,
+                                                C.c_double] + [ip] * 4
+                                  + [dp, ip, ip, ip, dp, ip, dp, dp, C.c_double, C.c_uint64,
+                                     C.c_double, C.c_long, C.POINTER(C.c_long)])
     return lib
 
 
@@ -374,3 +380,70 @@ def integrate_tree_prob(weights_list, model):
             f += fm[k, a, b]
         m.append(integrate_migration_term(c, f, model.mmax[i]))
     return float(sum(q) + sum(m)), q, m
+
+
+def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
+              seconds=1.0, max_sweeps=0, scale_width=0.2):
+    """The CPU baseline's C sweep over chains[c][l] (Genealogy objects with
+    population labels and migrations; updated in place) with split times
+    tvs[c] (a float array [nchains, P-1], updated in place).  Returns
+    (sweeps, stats[4] = genealogy proposals, accepts, rescale proposals,
+    accepts)."""
+    from ima2p_amd.immodel import mig_arrays
+    L = len(loci)
+    P = model.npops
+    n_l = np.array([chains[0][l].n for l in range(L)], np.int32)
+    S = np.array([lc.seq.shape[1] for lc in loci], np.int32)
+    seq = np.concatenate([np.ascontiguousarray(lc.seq, np.int32).ravel() for lc in loci])
+    seq_off = np.cumsum([0] + [lc.seq.size for lc in loci])[:-1].astype(np.int32)
+    mult = np.concatenate([np.ascontiguousarray(lc.mult, np.int32) for lc in loci])
+    mult_off = np.cumsum([0] + [len(lc.mult) for lc in loci])[:-1].astype(np.int32)
+    tot = np.array([lc.totsites for lc in loci], np.int32)
+    pi = np.concatenate([np.asarray(lc.pi, float) for lc in loci])
+    uu = np.ascontiguousarray(u, float)
+    sp = np.ascontiguousarray(np.broadcast_to(np.asarray(samppop, np.int32), (L, P)), np.int32)
+    toff = np.cumsum([0] + [len(w) for w in model.theta_wp]).astype(np.int32)
+    twp = np.array([x for w in model.theta_wp for kv in w for x in kv] or [0], np.int32)
+    moff = np.cumsum([0] + [len(w) for w in model.mig_wp]).astype(np.int32)
+    mwp = np.array([x for w in model.mig_wp for kv in w for x in kv] or [0], np.int32)
+    gens = [g for row in chains for g in row]
+    up0 = np.concatenate([g.up0 for g in gens]).astype(np.int32)
+    up1 = np.concatenate([g.up1 for g in gens]).astype(np.int32)
+    dn = np.concatenate([g.down for g in gens]).astype(np.int32)
+    pop = np.concatenate([g.pop for g in gens]).astype(np.int32)
+    tm = np.concatenate([g.time for g in gens]).astype(np.float64)
+    roots = np.array([g.root for g in gens], np.int32)
+    migs = [mig_arrays(g) for g in gens]
+    nm = np.array([len(m[0]) for m in migs], np.int32)
+    me = np.concatenate([m[0] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
+    mt = np.concatenate([m[1] for m in migs] + [np.zeros(1)]).astype(np.float64)
+    mo = np.concatenate([m[2] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
+    tv = tvs if P > 1 else np.zeros(1)
+    assert P == 1 or (tv.dtype == np.float64 and tv.flags.c_contiguous and tv.size == len(chains) * (P - 1))
+    b = np.ascontiguousarray(beta, float)
+    qm = np.ascontiguousarray(model.qmax if model.npsize else [1.0], float)
+    mm = np.ascontiguousarray(model.mmax if model.nmig else [1.0], float)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in
+            (model.plist, model.npresent, model.popdown, model.pope, model.join)]
+    stats = (C.c_long * 4)()
+    sw = lib.ora_mc3_bench(len(chains), L, _ip(n_l), _ip(S), _ip(seq), _ip(seq_off), _ip(mult),
+                           _ip(mult_off), _ip(tot), _dp(pi), _dp(uu), float(kappa), _ip(sp), P,
+                           *[_ip(a) for a in arrs], model.npsize, _ip(toff), _ip(twp),
+                           model.nmig, _ip(moff), _ip(mwp), _dp(qm), _dp(mm), float(tmax),
+                           _ip(up0), _ip(up1), _ip(dn), _ip(pop), _dp(tm), _ip(roots), _ip(nm),
+                           _ip(me), _dp(mt), _ip(mo), _dp(tv), _dp(b), float(scale_width),
+                           int(seed), float(seconds), int(max_sweeps), stats)
+    off = moff_ = 0
+    for g, m in zip(gens, migs):
+        ne = 2 * g.n - 1
+        g.time[:] = tm[off:off + ne]
+        off += ne
+        k = len(m[0])
+        times = mt[moff_:moff_ + k]
+        j = 0
+        for edge in range(ne):
+            lst = sorted(g.mig[edge])
+            g.mig[edge] = [(float(times[j + i]), q) for i, (_, q) in enumerate(lst)]
+            j += len(lst)
+        moff_ += k
+    return int(sw), list(stats)

@@ -1,0 +1,458 @@
+/*
+ * ora_mc3.c -- the CPU baseline's Metropolis-coupled sweep, in C.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ima_oracle.h): the bench's cpu_baseline
+ * leg times this loop on the host cores; nothing in the product calls it.
+ *
+ * The control structure is the reference's (qupdate, ima_main_mpi.cpp:
+ * 1788-2047; updategenealogy's scoring, update_gtree.cpp:827-961): chains
+ * in turn, each locus proposed, scored -- incremental likelihoodHKY on the
+ * edges that moved, treeweight, the all-loci weight sums updated as
+ * sum_subtract_treeinfo does (ginfo.cpp:249-285), integrals recomputed for
+ * parameters whose (count, weight) changed (update_gtree_common.cpp:
+ * 1997-2006) -- and accepted with beta (dprobg + dpdg) before the next; on
+ * alternate sweeps every genealogy of a chain, its migrations and split
+ * times are rescaled and fully re-scored; a swap closes the sweep
+ * (swapchains.cpp:527-662).  The proposals are the device sweep's
+ * (kern_mc3.hip): a reflected normal slide of one coalescence inside its
+ * period and between the migrations around it, and a global rescale by
+ * exp(d (U - 1/2)).  Randomness: xoshiro256** seeded by splitmix64.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "ima_oracle.h"
+
+typedef struct {
+  uint64_t s[4];
+} rng_t;
+
+static uint64_t splitmix(uint64_t *x) {
+  uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+static uint64_t xoshiro(rng_t *r) {
+  uint64_t *s = r->s;
+  const uint64_t res = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl(s[3], 45);
+  return res;
+}
+static double unif(rng_t *r) { return (double)(xoshiro(r) >> 11) * (1.0 / 9007199254740992.0); }
+static double norm01(rng_t *r) {
+  const double u1 = 1.0 - unif(r), u2 = unif(r);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+typedef struct { /* one genealogy of one chain */
+  int n, root;
+  int *up0, *up1, *down, *pop;
+  double *time;
+  int nmig, *mig_edge, *mig_to;
+  double *mig_time;
+  ora_hky_state st;
+  double pdg;
+  double *rec; /* weight record (layout of the device's records) */
+} gene_t;
+
+typedef struct {
+  /* loci */
+  int L;
+  const int *S, *totsites, *seq_off, *mult_off, *samppop;
+  const int *seq, *mult;
+  const double *pi, *u;
+  double kappa;
+  /* model */
+  int P, npsize, nmig;
+  const int *plist, *npresent, *popdown, *pope, *join;
+  const int *theta_off, *theta_wp, *mig_off, *mig_wp;
+  const double *qmax, *mmax;
+  double tmax, scale_width;
+} prob_t;
+
+static int rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
+
+static ora_tree tree_of(const gene_t *g) {
+  ora_tree t = {g->n, g->up0, g->up1, g->down, g->time, g->root};
+  return t;
+}
+
+static void weights(const prob_t *pb, int l, const gene_t *g, const double *tv, double *rec) {
+  const int P = pb->P, P2 = P * P, P3 = P2 * P;
+  int cc[100], mc[1000];
+  double fc[100], hcc[100], fm[1000], len;
+  ora_tree t = tree_of(g);
+  ora_treeweight(&t, g->pop, g->nmig, g->mig_edge, g->mig_time, g->mig_to, P, P, pb->plist,
+                 pb->npresent, pb->popdown, pb->pope, pb->join, tv, pb->samppop + l * P, 1.0,
+                 cc, fc, hcc, mc, fm, &len);
+  for (int i = 0; i < P2; i++) {
+    rec[i] = cc[i];
+    rec[P2 + i] = fc[i];
+    rec[2 * P2 + i] = hcc[i];
+  }
+  for (int i = 0; i < P3; i++) {
+    rec[3 * P2 + i] = mc[i];
+    rec[3 * P2 + P3 + i] = fm[i];
+  }
+  rec[3 * P2 + 2 * P3] = len;
+}
+
+/* integrals of every parameter at the sums `all`; terms whose (count,
+ * weight) equal the cached ones are reused */
+static double prior_terms(const prob_t *pb, const double *all, const int *qc, const double *qf,
+                          const double *qt, int *nc, double *nf, double *nt) {
+  const int P = pb->P, P2 = P * P, P3 = P2 * P;
+  double q = 0.0, m = 0.0;
+  for (int t = 0; t < pb->npsize + pb->nmig; t++) {
+    int c = 0;
+    double f = 0.0, h = 0.0;
+    if (t < pb->npsize) {
+      for (int w = pb->theta_off[t]; w < pb->theta_off[t + 1]; w++) {
+        const int k = pb->theta_wp[2 * w], i = pb->theta_wp[2 * w + 1];
+        c += (int)all[k * P + i];
+        f += all[P2 + k * P + i];
+        h += all[2 * P2 + k * P + i];
+      }
+    } else {
+      const int mm = t - pb->npsize;
+      for (int w = pb->mig_off[mm]; w < pb->mig_off[mm + 1]; w++) {
+        const int k = pb->mig_wp[3 * w], i = pb->mig_wp[3 * w + 1], j = pb->mig_wp[3 * w + 2];
+        c += (int)all[3 * P2 + (k * P + i) * P + j];
+        f += all[3 * P2 + P3 + (k * P + i) * P + j];
+      }
+    }
+    nc[t] = c;
+    nf[t] = f;
+    if (qc && qc[t] == c && qf[t] == f)
+      nt[t] = qt[t];
+    else if (t < pb->npsize)
+      nt[t] = ora_integrate_coalescent_term(c, f, h, pb->qmax[t], 0.0);
+    else
+      nt[t] = ora_integrate_migration_term(c, f, pb->mmax[t - pb->npsize], 0.0);
+    if (t < pb->npsize)
+      q += nt[t];
+    else
+      m += nt[t];
+  }
+  return q + m;
+}
+
+static double hky(const prob_t *pb, int l, gene_t *g, int e) {
+  ora_tree t = tree_of(g);
+  return ora_likelihoodHKY(&t, pb->S[l], pb->seq + pb->seq_off[l], pb->mult + pb->mult_off[l],
+                           pb->totsites[l], pb->pi + 4 * l, pb->u[l], pb->kappa, e, e, e, e,
+                           &g->st);
+}
+
+static void hky_settle(gene_t *g, int accept) {
+  ora_tree t = tree_of(g);
+  if (accept) {
+    ora_copyfraclike(&t, &g->st);
+    ora_storescalefactors(&t, &g->st);
+  } else {
+    ora_restorescalefactors(&t, &g->st);
+  }
+}
+
+typedef struct {
+  gene_t *g;  /* [L] */
+  double *tv; /* [P-1] */
+  double *all, *hold, *nrec;
+  double *tback, *mback; /* node and migration times before a rescale */
+  int *qc, *nc;
+  double *qf, *qt, *nf, *nt;
+  double probg, beta;
+} chain_t;
+
+static void slide(const prob_t *pb, chain_t *ch, int l, rng_t *r, long *acc) {
+  gene_t *g = &ch->g[l];
+  const int n = g->n, R = rec_len(pb->P);
+  int e = n + (int)(unif(r) * (n - 1));
+  if (e > 2 * n - 2) e = 2 * n - 2;
+  const int a = g->up0[e], b = g->up1[e];
+  const double T = g->time[a];
+  double lo = fmax(a >= n ? g->time[g->up0[a]] : 0.0, b >= n ? g->time[g->up0[b]] : 0.0);
+  double hi = g->down[e] == -1 ? INFINITY : g->time[e];
+  int k = 0;
+  while (k < pb->P - 1 && ch->tv[k] <= T) k++;
+  if (k > 0) lo = fmax(lo, ch->tv[k - 1]);
+  if (k < pb->P - 1) hi = fmin(hi, ch->tv[k]);
+  for (int j = 0; j < g->nmig; j++) {
+    if (g->mig_edge[j] == a || g->mig_edge[j] == b) lo = fmax(lo, g->mig_time[j]);
+    else if (g->mig_edge[j] == e) hi = fmin(hi, g->mig_time[j]);
+  }
+  const int open = isinf(hi);
+  const double sigma = open ? 0.5 * fmax(lo, 0.05) : 0.5 * (hi - lo);
+  double Tn = T + sigma * norm01(r);
+  if (open) {
+    Tn = lo + fabs(Tn - lo);
+  } else {
+    const double w = hi - lo;
+    double y = fmod(Tn - lo, 2.0 * w);
+    if (y < 0) y += 2.0 * w;
+    if (y > w) y = 2.0 * w - y;
+    Tn = lo + y;
+  }
+  if (!(Tn > lo) || !(Tn < hi)) Tn = T;
+  g->time[a] = g->time[b] = Tn;
+  const double np = hky(pb, l, g, e);
+  double *nrec = ch->nrec;
+  weights(pb, l, g, ch->tv, nrec);
+  const int P2 = pb->P * pb->P, P3 = P2 * pb->P;
+  for (int i = 0; i < R; i++) {
+    const double x = ch->all[i], o = g->rec[i], w = nrec[i];
+    ch->hold[i] = x;
+    if ((i >= P2 && i < 2 * P2) || (i >= 3 * P2 + P3 && i < 3 * P2 + 2 * P3))
+      ch->all[i] = fmax(0.0, (x - o) + w);
+    else if (i < P2 || (i >= 3 * P2 && i < 3 * P2 + P3))
+      ch->all[i] = x + (w - o);
+    else
+      ch->all[i] = (x - o) + w;
+  }
+  const double pn = prior_terms(pb, ch->all, ch->qc, ch->qf, ch->qt, ch->nc, ch->nf, ch->nt);
+  const double U = unif(r);
+  const int ok = U < fmin(1.0, exp(ch->beta * ((pn - ch->probg) + (np - g->pdg))));
+  if (ok) {
+    g->pdg = np;
+    ch->probg = pn;
+    memcpy(g->rec, nrec, sizeof(double) * R);
+    const int np_ = pb->npsize + pb->nmig;
+    memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+    memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+    memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+    (*acc)++;
+  } else {
+    g->time[a] = g->time[b] = T;
+    memcpy(ch->all, ch->hold, sizeof(double) * R);
+  }
+  hky_settle(g, ok);
+}
+
+static void rescale(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
+  const int L = pb->L, R = rec_len(pb->P), np_ = pb->npsize + pb->nmig;
+  const double ls = pb->scale_width * (unif(r) - 0.5), s = exp(ls);
+  double a = 0.0;
+  double *newp = (double *)malloc(sizeof(double) * L);
+  double *recs = (double *)malloc(sizeof(double) * L * R);
+  double tv_new[16];
+  int ok = 1;
+  for (int k = 0; k < pb->P - 1; k++) {
+    tv_new[k] = ch->tv[k] * s;
+    if (tv_new[k] > pb->tmax) ok = 0;
+  }
+  size_t tb = 0, mb = 0;
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    for (int i = 0; i < 2 * g->n - 1; i++) {
+      ch->tback[tb++] = g->time[i];
+      if (g->down[i] != -1) g->time[i] *= s;
+    }
+    for (int j = 0; j < g->nmig; j++) {
+      ch->mback[mb++] = g->mig_time[j];
+      g->mig_time[j] *= s;
+    }
+    newp[l] = hky(pb, l, g, -1);
+    weights(pb, l, g, tv_new, recs + (size_t)l * R);
+    a += ch->beta * (newp[l] - g->pdg) + (g->n - 1 + g->nmig) * ls;
+  }
+  double *fresh = (double *)calloc(R, sizeof(double));
+  for (int l = 0; l < L; l++)
+    for (int i = 0; i < R; i++) fresh[i] += recs[(size_t)l * R + i];
+  const double pn = prior_terms(pb, fresh, NULL, NULL, NULL, ch->nc, ch->nf, ch->nt);
+  a += ch->beta * (pn - ch->probg) + (pb->P - 1) * ls;
+  const int dec = ok && unif(r) < fmin(1.0, exp(a));
+  tb = mb = 0;
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    if (dec) {
+      g->pdg = newp[l];
+      memcpy(g->rec, recs + (size_t)l * R, sizeof(double) * R);
+    } else {
+      for (int i = 0; i < 2 * g->n - 1; i++) g->time[i] = ch->tback[tb++];
+      for (int j = 0; j < g->nmig; j++) g->mig_time[j] = ch->mback[mb++];
+    }
+    hky_settle(g, dec);
+  }
+  if (dec) {
+    memcpy(ch->all, fresh, sizeof(double) * R);
+    memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+    memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+    memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+    ch->probg = pn;
+    for (int k = 0; k < pb->P - 1; k++) ch->tv[k] = tv_new[k];
+    (*acc)++;
+  }
+  free(newp);
+  free(recs);
+  free(fresh);
+}
+
+/*
+ * Runs sweeps over nchains chains x L loci for about `seconds` of wall
+ * time, or exactly max_sweeps sweeps when max_sweeps > 0, and returns the
+ * number of sweeps done; stats
+ * (may be NULL, 4 entries) receives genealogy proposals, accepts, rescale
+ * proposals and accepts.  Genealogies are packed chain-major, locus-minor:
+ * tree arrays of 2n_l-1 entries each (up0, up1, down, pop, time), roots,
+ * migration counts and their (edge, time, destination) packed in order.
+ * tvs[c][0..P-2] are chain c's split times; times, migration times and
+ * split times are updated in place.
+ */
+long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
+                   const int *seq_off, const int *mult, const int *mult_off,
+                   const int *totsites, const double *pi, const double *u, double kappa,
+                   const int *samppop, int P, const int *plist, const int *npresent,
+                   const int *popdown, const int *pope, const int *join, int npsize,
+                   const int *theta_off, const int *theta_wp, int nmigp, const int *mig_off,
+                   const int *mig_wp, const double *qmax, const double *mmax, double tmax,
+                   const int *up0, const int *up1, const int *down, const int *pop,
+                   double *times, const int *roots, const int *nmig, const int *mig_edge,
+                   double *mig_time, const int *mig_to, double *tvs,
+                   const double *beta0, double scale_width, uint64_t seed, double seconds,
+                   long max_sweeps, long *stats) {
+  prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi, u, kappa,
+               P, npsize, nmigp, plist, npresent, popdown, pope, join,
+               theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width};
+  const int R = rec_len(P), np_ = npsize + nmigp;
+  if (P > 10 || P - 1 > 16) return -1;
+  chain_t *ch = (chain_t *)calloc(nchains, sizeof(chain_t));
+  size_t off = 0, moff = 0;
+  for (int c = 0; c < nchains; c++) {
+    chain_t *h = &ch[c];
+    h->g = (gene_t *)calloc(L, sizeof(gene_t));
+    h->tv = (double *)malloc(sizeof(double) * (P > 1 ? P - 1 : 1));
+    for (int k = 0; k < P - 1; k++) h->tv[k] = tvs[c * (P - 1) + k];
+    h->all = (double *)calloc(R, sizeof(double));
+    h->hold = (double *)calloc(R, sizeof(double));
+    h->nrec = (double *)calloc(R, sizeof(double));
+    h->qc = (int *)calloc(np_ + 1, sizeof(int));
+    h->nc = (int *)calloc(np_ + 1, sizeof(int));
+    h->qf = (double *)calloc(np_ + 1, sizeof(double));
+    h->qt = (double *)calloc(np_ + 1, sizeof(double));
+    h->nf = (double *)calloc(np_ + 1, sizeof(double));
+    h->nt = (double *)calloc(np_ + 1, sizeof(double));
+    h->beta = beta0[c];
+    for (int l = 0; l < L; l++) {
+      gene_t *g = &h->g[l];
+      const int n = n_l[l], ne = 2 * n - 1;
+      g->n = n;
+      g->root = roots[c * L + l];
+      g->up0 = (int *)(up0 + off);
+      g->up1 = (int *)(up1 + off);
+      g->down = (int *)(down + off);
+      g->pop = (int *)(pop + off);
+      g->time = times + off;
+      g->nmig = nmig[c * L + l];
+      g->mig_edge = (int *)(mig_edge + moff);
+      g->mig_to = (int *)(mig_to + moff);
+      g->mig_time = mig_time + moff;
+      off += ne;
+      moff += g->nmig;
+      g->st.n = n;
+      g->st.S = S[l];
+      g->st.frac = (double *)calloc((size_t)(n - 1) * S[l] * 4, sizeof(double));
+      g->st.newfrac = (double *)calloc((size_t)(n - 1) * S[l] * 4, sizeof(double));
+      g->st.scale = (double *)calloc((size_t)(n - 1) * S[l], sizeof(double));
+      g->st.oldscale = (double *)calloc((size_t)(n - 1) * S[l], sizeof(double));
+      g->pdg = hky(&pb, l, g, -1);
+      hky_settle(g, 1);
+      g->rec = (double *)malloc(sizeof(double) * R);
+      weights(&pb, l, g, h->tv, g->rec);
+      for (int i = 0; i < R; i++) h->all[i] += g->rec[i];
+    }
+    size_t nt = 0, nm = 0;
+    for (int l = 0; l < L; l++) {
+      nt += 2 * (size_t)h->g[l].n - 1;
+      nm += (size_t)h->g[l].nmig;
+    }
+    h->tback = (double *)malloc(sizeof(double) * (nt + 1));
+    h->mback = (double *)malloc(sizeof(double) * (nm + 1));
+    h->probg = prior_terms(&pb, h->all, NULL, NULL, NULL, h->qc, h->qf, h->qt);
+  }
+  rng_t r;
+  uint64_t sm = seed;
+  for (int i = 0; i < 4; i++) r.s[i] = splitmix(&sm);
+  long acc[2] = {0, 0}, props[2] = {0, 0}, sweeps = 0;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  double el = 0.0;
+  do {
+    for (int c = 0; c < nchains; c++) {
+      for (int l = 0; l < L; l++) {
+        slide(&pb, &ch[c], l, &r, &acc[0]);
+        props[0]++;
+      }
+      if (sweeps & 1) {
+        rescale(&pb, &ch[c], &r, &acc[1]);
+        props[1]++;
+      }
+    }
+    /* swap (swapchains.cpp:527-653) on S = sum pdg + probg */
+    const int swaptries = nchains / 10 > 1 ? nchains / 10 : 1;
+    for (int i = 0; i < swaptries && nchains > 1; i++) {
+      const int ci = (int)(unif(&r) * nchains);
+      int cj, cjmin = 0, cjrange = nchains;
+      if (nchains >= 17) {
+        cjmin = ci - 7 > 0 ? ci - 7 : 0;
+        cjrange = (ci + 7 < nchains ? ci + 7 : nchains) - cjmin;
+      }
+      do cj = cjmin + (int)(unif(&r) * cjrange);
+      while (cj == ci);
+      double Si = ch[ci].probg, Sj = ch[cj].probg;
+      for (int l = 0; l < L; l++) {
+        Si += ch[ci].g[l].pdg;
+        Sj += ch[cj].g[l].pdg;
+      }
+      const double w = exp((ch[ci].beta - ch[cj].beta) * (Sj - Si));
+      if (w >= 1.0 || w > unif(&r)) {
+        const double b = ch[ci].beta;
+        ch[ci].beta = ch[cj].beta;
+        ch[cj].beta = b;
+      }
+    }
+    sweeps++;
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    el = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  } while (max_sweeps > 0 ? sweeps < max_sweeps : el < seconds);
+  if (stats) {
+    stats[0] = props[0];
+    stats[1] = acc[0];
+    stats[2] = props[1];
+    stats[3] = acc[1];
+  }
+  for (int c = 0; c < nchains; c++) {
+    for (int k = 0; k < P - 1; k++) tvs[c * (P - 1) + k] = ch[c].tv[k];
+    for (int l = 0; l < L; l++) {
+      gene_t *g = &ch[c].g[l];
+      free(g->st.frac);
+      free(g->st.newfrac);
+      free(g->st.scale);
+      free(g->st.oldscale);
+      free(g->rec);
+    }
+    free(ch[c].g);
+    free(ch[c].tv);
+    free(ch[c].all);
+    free(ch[c].hold);
+    free(ch[c].nrec);
+    free(ch[c].tback);
+    free(ch[c].mback);
+    free(ch[c].qc);
+    free(ch[c].nc);
+    free(ch[c].qf);
+    free(ch[c].qt);
+    free(ch[c].nf);
+    free(ch[c].nt);
+  }
+  free(ch);
+  return sweeps;
+}
