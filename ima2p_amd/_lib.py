@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libima_gpu.so")
 
 IMA_OK = 0
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_SIZE, ERR_NODEV = 1, 2, 3, 4, 5
 MODEL_HKY, MODEL_IS, MODEL_SW = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE, STATUS_PRIOR_BAD = 0, 1, 2, 3, 4
@@ -61,6 +62,21 @@ PROTOS = {
     "ima_mc3_set_prior": (C.c_int, [vp, dp, C.c_double]),
     "ima_mc3_read_prior": (C.c_int, [vp, dp, dp]),
     "ima_slots_read_mig": (C.c_int, [vp, C.c_int, dp]),
+    "ima_server_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]),
+    "ima_server_destroy": (None, [vp]),
+    "ima_server_last_error": (C.c_char_p, [vp]),
+    "ima_server_table": (C.c_int, [vp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_int)]),
+    "ima_server_submit_lmode": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.POINTER(C.c_int64)]),
+    "ima_server_submit_analysis": (C.c_int, [vp, vp, C.POINTER(C.c_int64)]),
+    "ima_server_step": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ima_server_status": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int), dp, dp]),
+    "ima_server_lmode_result": (C.c_int, [vp, C.c_int64, dp, dp]),
+    "ima_server_analysis_result": (C.c_int, [vp, C.c_int64, dp, dp, dp, C.POINTER(C.c_uint64)]),
+    "ima_server_release": (C.c_int, [vp, C.c_int64]),
+    "ima_server_latency": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp, dp]),
+    "ima_server_usage": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ima_score_repeat": (C.c_int, [vp, C.POINTER(Item), C.c_int, C.c_int, C.c_int, dp, i32p]),
     "ima_is_site_nodes": (C.c_int, [vp, C.c_int, i32p]),
     "ima_sw_dlike": (C.c_int, [vp, C.POINTER(Item), dp]),

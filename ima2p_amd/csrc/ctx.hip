@@ -11,115 +11,9 @@
 #include <vector>
 
 #include "ima_internal.h"
+#include "ctx_impl.h"
 
 using namespace ima;
-
-namespace {
-
-struct LTableHost {
-  bool live = false;
-  float *rows = nullptr;
-  int G = 0, K = 0, npsize = 0, nmig = 0, npops = 0;
-};
-
-}  // namespace
-
-struct ima_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;  // genealogy weights beside the likelihood launch
-  std::string err;
-
-  // loci (host mirror + device copy)
-  std::vector<LocusMeta> loci;
-  std::vector<int8_t> seq;
-  std::vector<int32_t> mult;
-  std::vector<uint64_t> mask;
-  bool loci_dirty = true;
-  DevBuf<LocusMeta> d_loci;
-  DevBuf<int8_t> d_seq;
-  DevBuf<int32_t> d_mult;
-  DevBuf<uint64_t> d_mask;
-
-  // genealogy slots
-  std::vector<SlotMeta> slots;
-  size_t nodes = 0, progs = 0;
-  std::vector<int32_t> up0, up1, down, A;
-  std::vector<double> time;
-  std::vector<ProgOp> prog;
-  DevBuf<SlotMeta> d_slots;
-  DevBuf<int32_t> d_up0, d_up1, d_down, d_A;
-  DevBuf<double> d_time;
-  DevBuf<ProgOp> d_prog;
-  bool slots_meta_dirty = true;
-  std::vector<int> scratch_slot;  // per locus, for ima_likelihood_hky
-  // population labels and migration events of the genealogies (prior)
-  std::vector<int32_t> pop, mig_edge, mig_to;
-  std::vector<double> mig_time;
-  size_t migs = 0;
-  DevBuf<int32_t> d_pop, d_mig_edge, d_mig_to;
-  DevBuf<double> d_mig_time;
-
-  // population models of the genealogy prior
-  std::vector<ModelMeta> models;
-  std::vector<int32_t> mint;
-  std::vector<double> mdbl, logfact;
-  bool models_dirty = true;
-  size_t logfact_dev = 0;
-  DevBuf<ModelMeta> d_models;
-  DevBuf<int32_t> d_mint;
-  DevBuf<double> d_mdbl, d_logfact;
-  DevBuf<double> d_rec, d_tv, d_probg, d_qint, d_mintg;
-  DevBuf<int32_t> d_pslots, d_prow, d_goff, d_pstatus;
-
-  // scoring scratch
-  DevBuf<ima_item> d_items;
-  DevBuf<int32_t> d_idx;
-  DevBuf<double> d_out;
-  DevBuf<int32_t> d_status;
-  DevBuf<int32_t> d_aux;
-  DevBuf<double> d_aux64;
-  DevBuf<int4> d_units, d_red;
-  DevBuf<double> d_partial;
-
-  // L mode
-  std::vector<LTableHost> tables;
-  DevBuf<double> d_lx, d_lwork, d_lout, d_less;
-
-  // swap
-  DevBuf<double> d_S, d_beta;
-  DevBuf<int32_t> d_pairs;
-
-  Timing timing;
-  int max_lds = 65536;
-
-  // device-resident MC3 sweep
-  struct {
-    bool live = false;
-    int C = 0, L = 0, first = 0, precision = IMA_PREC_FP32, swaptries = 1;
-    int nh = 0, ni = 0, ns = 0, hmax_n = 2, hmax_slots = 1, imax_n = 2, imax_w = 1;
-    MC3Dev dev{};
-    DevBuf<double> pdg, newpdg, beta, S, rec_t, tbackup, logs;
-    DevBuf<int32_t> status, rec_v, decision, idx, node_item;
-    DevBuf<int2> iblk;
-    DevBuf<unsigned> done;
-    DevBuf<uint32_t> acc, cacc;
-    int64_t scale_sweeps = 0;
-    DevBuf<uint64_t> counter;
-    DevBuf<unsigned long long> stats;
-    DevBuf<ima_item> items;
-    DevBuf<int4> units, red;
-    DevBuf<double> partial;
-    // genealogy prior
-    DevBuf<double> tv, tvbackup, wrec, wnew, wall, qterm, qf, probg, mtbackup;
-    DevBuf<int32_t> qc, tv_row, tw_slots, wstatus;
-    HkyPlan plan;
-    hipGraphExec_t exec[2]{nullptr, nullptr};
-    hipGraph_t graph[2]{nullptr, nullptr};
-    hipEvent_t ev[4]{};
-    hipEvent_t fj[4]{};  // fork / join of the side stream, per scoring pass
-  } mc3;
-};
 
 static void mc3_release(ima_ctx *c);
 
@@ -1163,6 +1057,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   HIPCHK(c, hipMemsetAsync(m.acc.p, 0, N * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(m.cacc.p, 0, nchains * 4, c->stream));
   m.scale_sweeps = 0;
+  m.host_sweeps = 0;
   HIPCHK(c, hipMemcpyAsync(m.items.p, items.data(), N * sizeof(ima_item), hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(m.idx.p, idx.data(), N * 4, hipMemcpyHostToDevice, c->stream));
@@ -1261,7 +1156,12 @@ static int mc3_enqueue(ima_ctx *c, bool with_scale, hipEvent_t *score_ev) {
   return IMA_OK;
 }
 
-int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
+}  // extern "C"
+
+// Enqueues nsweeps sweeps on the context's stream without waiting (timed
+// runs synchronise per sweep).  The sweep counter's host copy decides which
+// sweeps rescale, so nothing is read back from the device.
+int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   if (!c || nsweeps < 0) return IMA_ERR_ARG;
   auto &m = c->mc3;
   if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
@@ -1284,8 +1184,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
       HIPCHK(c, hipGraphInstantiate(&m.exec[k], g, nullptr, nullptr, 0));
     }
   }
-  uint64_t done = 0;
-  HIPCHK(c, hipMemcpy(&done, m.counter.p, 8, hipMemcpyDeviceToHost));
+  const uint64_t done = m.host_sweeps;
   if (nsweeps > 0) HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
   for (int s = 0; s < nsweeps; ++s) {
     const bool with_scale = scale && ((done + s) & 1);
@@ -1317,6 +1216,15 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
     }
   }
   if (nsweeps > 0) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+  m.host_sweeps += (uint64_t)nsweeps;
+  return IMA_OK;
+}
+
+extern "C" {
+
+int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
+  int rc = mc3_run_async(c, nsweeps, flags);
+  if (rc != IMA_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return IMA_OK;
 }
@@ -1818,3 +1726,38 @@ int ima_timing_reset(ima_ctx *c) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ helpers for serve.hip --
+
+LTable ctx_ltable(ima_ctx *c, int table_id) {
+  if (!c || table_id < 0 || table_id >= (int)c->tables.size() || !c->tables[table_id].live)
+    return LTable{};
+  return ltable(c->tables[table_id]);
+}
+
+int ctx_set_error(ima_ctx *c, int rc, const char *msg) { return fail(c, rc, "%s", msg); }
+
+size_t ctx_device_bytes(const ima_ctx *c) {
+  size_t b = 0;
+  auto add = [&](size_t n, size_t sz) { b += n * sz; };
+  add(c->d_loci.n, sizeof(LocusMeta));
+  add(c->d_seq.n, 1);
+  add(c->d_mult.n, 4);
+  add(c->d_mask.n, 8);
+  add(c->d_slots.n, sizeof(SlotMeta));
+  add(c->d_up0.n + c->d_up1.n + c->d_down.n + c->d_A.n + c->d_pop.n, 4);
+  add(c->d_time.n, 8);
+  add(c->d_prog.n, sizeof(ProgOp));
+  add(c->d_mig_edge.n + c->d_mig_to.n, 4);
+  add(c->d_mig_time.n, 8);
+  add(c->d_items.n, sizeof(ima_item));
+  add(c->d_out.n + c->d_partial.n + c->d_rec.n + c->d_lwork.n, 8);
+  const auto &m = c->mc3;
+  add(m.pdg.n + m.newpdg.n + m.tbackup.n + m.partial.n + m.wrec.n + m.wnew.n + m.wall.n +
+          m.mtbackup.n,
+      8);
+  add(m.units.n + m.red.n, sizeof(int4));
+  for (const auto &t : c->tables)
+    if (t.live) add((size_t)t.G * t.K, 4);
+  return b;
+}

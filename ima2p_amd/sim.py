@@ -386,6 +386,25 @@ def im_model_three_pop(rng):
     return pt, theta, ts, mig
 
 
+def im_model_random(rng, P):
+    """P populations on the ladder tree (((0,1),2),...): theta U(1,10) for the
+    2P-1 populations, split times U(0.5,3) sorted, migration U(0,1) between
+    every pair that coexists before the last split.  P = 3 gives the tree
+    of im_model_three_pop."""
+    joins = [(0, 1)] + [(P + k - 1, k + 1) for k in range(1, P - 1)]
+    pt = PopTree(P, joins)
+    theta = rng.uniform(1, 10, 2 * P - 1)
+    ts = np.sort(rng.uniform(0.5, 3, P - 1))
+    mig = np.zeros((2 * P - 1, 2 * P - 1))
+    for period in range(P - 1):
+        act = pt.active(period)
+        for p in act:
+            for q in act:
+                if p != q:
+                    mig[p, q] = rng.uniform(0, 1)
+    return pt, theta, ts, mig
+
+
 # ------------------------------------------------------- L-mode tables --
 
 def lmode_table(rng, G, npsize=3, nmig=2, nsplit=1, theta=5.0, m=0.5, nloci=5):

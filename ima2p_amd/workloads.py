@@ -99,3 +99,74 @@ def two_pop_is(nchains=4, nloci=5, per_pop=10, seed=1, pool=4):
         chains.append(row)
     return Workload("2pop-5xIS-n20-4chains", nchains, "is", loci, np.ones(nloci), 1.0, chains,
                     geometric_betas(nchains), seed, pt, np.asarray(ts, float), [per_pop] * 2)
+
+
+# ----------------------------------------------------------------- serving --
+
+@dataclass
+class LocusPool:
+    """Loci simulated under one IM model, drawn from by serving analyses.
+    entries[i] = (kind, data, genealogy, allele states or None, u, samppop)."""
+    npops: int
+    poptree: sim.PopTree
+    tsplit: np.ndarray
+    model: object
+    entries: list
+
+
+def serving_pool(rng, P, size=48, mix=(0.5, 1 / 3, 1 / 6), copies=(2, 10),
+                 hky_sites=(200, 1500)):
+    """Mixed HKY / IS / SW loci (the 150:100:50 proportions of the uneven
+    batch) with copies[0]..copies[1] gene copies per sampled population."""
+    from .immodel import build
+    pt, theta, ts, mig = sim.im_model_random(rng, P)
+    model = build(pt, qmax=20.0, mmax=5.0)
+    kinds = rng.choice(["hky", "is", "sw"], size, p=list(mix))
+    entries = []
+    for kind in kinds:
+        samp = [int(x) for x in rng.integers(copies[0], copies[1] + 1, P)]
+        while True:
+            g = sim.simulate_genealogy(rng, samp, theta, ts, mig, pt)
+            if kind == "hky":
+                u = float(rng.lognormal(0.0, 0.5))
+                nsites = int(rng.integers(hky_sites[0], hky_sites[1] + 1))
+                data = sim.encode_hky(sim.simulate_hky_sequences(
+                    rng, g, nsites, rng.dirichlet([10] * 4), 4.0, u))
+                A = None
+            elif kind == "is":
+                u = float(rng.uniform(1.0, 3.0))
+                data = sim.simulate_is(rng, g, u)
+                A = None
+                if data.seq.shape[1] == 0:
+                    continue
+            else:
+                u = float(rng.uniform(1.0, 3.0))
+                A = sim.simulate_sw(rng, g, u)
+                data = g.n
+            break
+        entries.append((str(kind), data, g, A, u, samp))
+    return LocusPool(P, pt, np.asarray(ts, float), model, entries)
+
+
+def serving_analysis(rng, pool: LocusPool, nloci, nchains, nsweeps=100, seed=1,
+                     precision=None):
+    """An IM analysis of nloci loci drawn from the pool.  Every chain starts
+    on the genealogies that generated the data (compatible with the IS
+    sites), at the generating split times."""
+    from . import _lib as L
+    from .serve import AnalysisSpec
+    idx = rng.integers(0, len(pool.entries), nloci)
+    ent = [pool.entries[i] for i in idx]
+    loci = [(k, d) for k, d, *_ in ent]
+    base = [g for _, _, g, *_ in ent]
+    As = [A for _, _, _, A, *_ in ent]
+    u = np.array([e[4] for e in ent])
+    kappa = np.array([4.0 if e[0] == "hky" else 1.0 for e in ent])
+    P = pool.npops
+    return AnalysisSpec(
+        loci=loci, model=pool.model, samppop=[e[5] for e in ent],
+        chains=[base] * nchains, u=np.tile(u, (nchains, 1)),
+        kappa=np.tile(kappa, (nchains, 1)), beta=geometric_betas(nchains),
+        tvals=np.tile(pool.tsplit, (nchains, 1)) if P > 1 else np.zeros((nchains, 0)),
+        tmax=5.0, seed=seed, precision=L.PREC_FP32 if precision is None else precision,
+        nsweeps=nsweeps, A=[As] * nchains)

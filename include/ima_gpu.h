@@ -271,6 +271,87 @@ int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
  * moves them); mig_time has that slot's migration count entries. */
 int ima_slots_read_mig(ima_ctx *ctx, int slot, double *mig_time);
 
+/* ---- serving ----------------------------------------------------------- */
+/* A request loop for independent IM analyses and L-mode queries arriving
+ * over time (no counterpart in the reference, which serves one analysis
+ * per process).  Requests wait in arrival order and are admitted, first
+ * fit, while their HBM footprint fits the budget; admitted L-mode queries
+ * on one table are batched into shared launches, admitted analyses run in
+ * quanta of sweeps on streams of their own, all overlapping on the device.
+ * ima_server_step makes progress without blocking (except while building
+ * an admitted analysis' context); latency runs from submission to the step
+ * that sees the request finish. */
+typedef struct ima_server ima_server;
+enum { IMA_REQ_WAITING = 0, IMA_REQ_RUNNING = 1, IMA_REQ_DONE = 2, IMA_REQ_FAILED = 3 };
+enum { IMA_REQ_LMODE = 0, IMA_REQ_ANALYSIS = 1 };
+
+/* One IM analysis, flat (every array is copied at submission).  Loci:
+ * model[l] (IMA_MODEL_*), n[l] gene copies, S[l] patterns (HKY) or sites
+ * (IS), totsites[l], pi[l][4]; seq packs the HKY and IS loci' n x S data in
+ * locus order, mult the HKY pattern counts; samppop[l][npops].  The
+ * population model takes ima_model_add's arrays.  Genealogies are packed
+ * chain-major (slot c*nloci + l, 2n-1 entries each): up0, up1, down, time,
+ * pop, A (may be NULL), root[c*nloci + l], nmigev[c*nloci + l] and the
+ * migrations (edge, time, destination) packed in slot order.  u and kappa
+ * per (chain, locus), beta per chain, tvals[c][npops-1] split times with
+ * uniform prior on [0, tmax]; nsweeps sweeps of ima_mc3_run with
+ * IMA_MC3_GRAPH | IMA_MC3_SCALE. */
+typedef struct {
+  int nloci, nchains, npops;
+  const int32_t *model, *n, *S, *totsites;
+  const int8_t *seq;
+  const int32_t *mult;
+  const double *pi;
+  const int32_t *samppop;
+  const int32_t *plist, *npresent, *popdown, *pope, *join;
+  int npsize;
+  const int32_t *theta_off, *theta_wp;
+  int nmig;
+  const int32_t *mig_off, *mig_wp;
+  const double *qmax, *mmax;
+  const int32_t *up0, *up1, *down, *root, *A, *pop, *nmigev, *mig_edge, *mig_to;
+  const double *time, *mig_time;
+  const double *u, *kappa, *beta, *tvals;
+  double tmax;
+  uint64_t seed;
+  int precision, nsweeps;
+} ima_analysis;
+
+/* budget_bytes <= 0: 80% of the device's free memory; quantum <= 0: 25
+ * sweeps per analysis step. */
+int ima_server_create(int device, double budget_bytes, int quantum, ima_server **out);
+void ima_server_destroy(ima_server *srv);
+const char *ima_server_last_error(const ima_server *srv);
+/* A resident genealogy table (ima_lmode_table's arguments). */
+int ima_server_table(ima_server *srv, const float *table, int G, int K, int npsize,
+                     int nmig, int npops, int *table_id);
+/* jointp at npoints points (ima_lmode_jointp's arguments). */
+int ima_server_submit_lmode(ima_server *srv, int table_id, const double *x, int npoints,
+                            int modeltype, int prange0, int prange1, int mode,
+                            int64_t *req);
+int ima_server_submit_analysis(ima_server *srv, const ima_analysis *a, int64_t *req);
+/* Progress: completions, admissions, launches.  *pending (may be NULL)
+ * receives the requests not yet finished. */
+int ima_server_step(ima_server *srv, int *pending);
+/* state (IMA_REQ_*), latency (-1 until finished) and the HBM footprint the
+ * request is charged: estimated from its sizes while it waits, measured
+ * from its context once admitted. */
+int ima_server_status(ima_server *srv, int64_t req, int *state, double *latency_s,
+                      double *bytes);
+int ima_server_lmode_result(ima_server *srv, int64_t req, double *out, double *ess);
+/* pdg[nchains*nloci], probg[nchains], beta[nchains], stats[8] as
+ * ima_mc3_read's. */
+int ima_server_analysis_result(ima_server *srv, int64_t req, double *pdg, double *probg,
+                               double *beta, uint64_t *stats);
+/* Forget a finished request (frees its host copy and results). */
+int ima_server_release(ima_server *srv, int64_t req);
+/* p50 / p99 / mean latency in seconds of finished requests of a kind
+ * (IMA_REQ_LMODE, IMA_REQ_ANALYSIS, -1 for all). */
+int ima_server_latency(ima_server *srv, int kind, int *count, double *p50, double *p99,
+                       double *mean);
+int ima_server_usage(ima_server *srv, double *used_bytes, double *budget_bytes,
+                     int *waiting, int *running);
+
 /* ---- timing ------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the context's stream (off by default).
  * ima_timing_get returns, for kernel class k (0 HKY FP64, 1 HKY FP32,
