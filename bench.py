@@ -172,6 +172,94 @@ def lmode_leg(G=100000, npoints=4096, seed=4):
     return out
 
 
+def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
+    """Poisson request stream at `rate`/s for `duration_s` s: 70% L-mode
+    query batches of 1..4096 points (log-uniform) on resident tables of
+    1e4, 1e5 and 1e6 genealogies, 30% new IM analyses with 2..4 populations,
+    5..300 mixed HKY/IS/SW loci and 16..128 chains running `nsweeps` sweeps.
+    Requests are generated (and analyses flattened) before the stream
+    starts; latency runs from a request's arrival time to the step that
+    sees it finish, so time a request waits behind the host loop counts."""
+    from ima2p_amd import _lib as L
+    from ima2p_amd import sim
+    from ima2p_amd.serve import REQ_DONE, Server
+    rng = np.random.default_rng(seed)
+    pools = {P: workloads.serving_pool(rng, P, size=48) for P in (2, 3, 4)}
+    tab_sizes = (10000, 100000, 1000000)
+    tabs = [sim.lmode_table(rng, G) for G in tab_sizes]
+    lo, hi = np.array([0.5, 0.5, 0.5, 0.01, 0.01]), np.array([10, 10, 10, 2, 2])
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        if rng.random() < 0.7:
+            k = int(rng.integers(0, len(tabs)))
+            npts = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+            reqs.append((t, "q", (k, rng.uniform(lo, hi, (npts, 5)))))
+        else:
+            P = int(rng.integers(2, 5))
+            spec = workloads.serving_analysis(rng, pools[P], int(rng.integers(5, 301)),
+                                              int(rng.integers(16, 129)), nsweeps=nsweeps,
+                                              seed=int(rng.integers(1, 1 << 30)))
+            reqs.append((t, "a", (spec.nchains, spec.flatten())))
+    srv = Server(0, 0.0, 25)
+    tids = [srv.table(tb, 3, 2, 2) for tb in tabs]
+    # one warm query per table and one small analysis before the stream
+    for tid in tids:
+        srv.submit_lmode(tid, lo[None, :] + 1.0)
+    warm = workloads.serving_analysis(rng, pools[2], 5, 16, nsweeps=2)
+    srv.submit_analysis(warm)
+    srv.run(120)
+    rec = []  # (kind, arrival, submit offset, id, size)
+    steps = []  # host time of each step call
+    t0 = time.perf_counter()
+    i = 0
+    pending = 0
+    while i < len(reqs) or pending:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i][0] <= now:
+            at, kind, data = reqs[i]
+            sub = time.perf_counter() - t0
+            if kind == "q":
+                k, x = data
+                rid = srv.submit_lmode(tids[k], x)
+                rec.append(("q", at, sub, rid, len(x)))
+            else:
+                nch, flat = data
+                rid = srv.submit_analysis(flat)
+                rec.append(("a", at, sub, rid, nch))
+            i += 1
+        ts = time.perf_counter()
+        pending = srv.step()
+        steps.append(time.perf_counter() - ts)
+        if now > duration_s + 180.0:
+            raise TimeoutError("serving stream did not drain within 180 s of its end")
+        if not pending and i < len(reqs):
+            time.sleep(max(0.0, reqs[i][0] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    out = {}
+    for kind, name in (("q", "lmode"), ("a", "analysis")):
+        lat, work = [], 0
+        for k, at, sub, rid, size in rec:
+            if k != kind:
+                continue
+            st, l, _ = srv.status(rid)
+            assert st == REQ_DONE, (kind, rid, st)
+            lat.append((sub - at) + l)
+            work += size * (1 if kind == "q" else nsweeps)
+        lat = np.sort(np.asarray(lat)) * 1e3
+        out[name] = {"count": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                     "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat[-1]),
+                     ("points_per_s" if kind == "q" else "chain_steps_per_s"): work / wall}
+    srv.close()
+    out.update({"requests": len(rec), "arrival_rate_per_s": rate, "stream_s": duration_s,
+                "wall_s": wall, "tables": list(tab_sizes), "sweeps_per_analysis": nsweeps,
+                "step_calls": len(steps), "step_max_ms": max(steps) * 1e3,
+                "step_p99_ms": float(np.percentile(steps, 99)) * 1e3})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +269,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64/plain/L-mode legs")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the serving stream (0 skips it)")
+    ap.add_argument("--serve-only", action="store_true", help="run the serving stream only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run 5 plain device copies of MIB MiB (fp32 copy_), the "
                          "known-size kernel that scales PMC FETCH_SIZE/WRITE_SIZE readings")
@@ -189,6 +280,9 @@ def main():
         print(json.dumps({"error": "this engine runs all heated chains on one GPU; "
                                    "multi-GPU runs are not part of this build (use --gpus 1)"}))
         sys.exit(2)
+    if args.serve_only:
+        print(json.dumps({"serving": serving_leg(duration_s=args.serve_seconds or 20.0)}))
+        return
 
     if args.calibrate_copy > 0:
         import torch
@@ -257,6 +351,8 @@ def main():
         extra["speedup_vs_plain_fp64"] = value / plain
         e64.close()
         extra["lmode"] = lmode_leg()
+    if args.serve_seconds > 0:
+        extra["serving"] = serving_leg(duration_s=args.serve_seconds)
 
     line = {
         "metric": "chain-steps/sec (all loci)",

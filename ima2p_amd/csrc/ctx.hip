@@ -37,8 +37,10 @@ int fail(ima_ctx *c, int rc, const char *fmt, ...) {
                   __FILE__, __LINE__);                                               \
   } while (0)
 
+// Grow b to hold n elements; with keep, the old contents move over on
+// stream st, after the work already queued there.
 template <typename T>
-hipError_t reserve(DevBuf<T> &b, size_t n, bool keep = false) {
+hipError_t reserve(DevBuf<T> &b, size_t n, bool keep = false, hipStream_t st = nullptr) {
   if (n <= b.n && b.p) return hipSuccess;
   size_t cap = std::max<size_t>(n, b.n * 2);
   if (cap == 0) cap = 1;
@@ -46,7 +48,7 @@ hipError_t reserve(DevBuf<T> &b, size_t n, bool keep = false) {
   hipError_t e = hipMalloc(&p, cap * sizeof(T));
   if (e != hipSuccess) return e;
   if (keep && b.p && b.n) {
-    e = hipMemcpy(p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice);
+    e = hipMemcpyAsync(p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
   }
   if (b.p) hipFree(b.p);
@@ -60,6 +62,15 @@ void release(DevBuf<T> &b) {
   if (b.p) hipFree(b.p);
   b.p = nullptr;
   b.n = 0;
+}
+
+// A copy ordered after the context's queued work that returns once done.
+// Every host-synchronous copy of the library goes through the context's
+// own stream: nothing runs on the null stream, so one context's copies
+// never wait for another context's kernels.
+hipError_t copy_now(ima_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind k) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, k, c->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
 }
 
 int upload_loci(ima_ctx *c) {
@@ -148,7 +159,7 @@ int upload_models(ima_ctx *c) {
   }
   if (c->logfact.size() > c->logfact_dev) {
     HIPCHK(c, reserve(c->d_logfact, c->logfact.size()));
-    HIPCHK(c, hipMemcpy(c->d_logfact.p, c->logfact.data(), c->logfact.size() * 8,
+    HIPCHK(c, copy_now(c, c->d_logfact.p, c->logfact.data(), c->logfact.size() * 8,
                         hipMemcpyHostToDevice));
     c->logfact_dev = c->logfact.size();
   }
@@ -431,13 +442,13 @@ int ima_slots_alloc(ima_ctx *c, int count, const int32_t *locus_of, int *first) 
   c->A.resize(c->nodes, 0);
   c->time.resize(c->nodes, 0.0);
   c->prog.resize(c->progs);
-  HIPCHK(c, reserve(c->d_up0, c->nodes, true));
-  HIPCHK(c, reserve(c->d_pop, c->nodes, true));
-  HIPCHK(c, reserve(c->d_up1, c->nodes, true));
-  HIPCHK(c, reserve(c->d_down, c->nodes, true));
-  HIPCHK(c, reserve(c->d_A, c->nodes, true));
-  HIPCHK(c, reserve(c->d_time, c->nodes, true));
-  HIPCHK(c, reserve(c->d_prog, c->progs + 1, true));
+  HIPCHK(c, reserve(c->d_up0, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_up1, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_down, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_A, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_time, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_prog, c->progs + 1, true, c->stream));
   c->slots_meta_dirty = true;
   return IMA_OK;
 }
@@ -507,11 +518,11 @@ int ima_slots_read(ima_ctx *c, int first, int count, int32_t *up0, int32_t *up1,
   const SlotMeta &s0 = c->slots[first];
   const SlotMeta &s1 = c->slots[first + count - 1];
   const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
-  if (up0) HIPCHK(c, hipMemcpy(up0, c->d_up0.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
-  if (up1) HIPCHK(c, hipMemcpy(up1, c->d_up1.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
-  if (down) HIPCHK(c, hipMemcpy(down, c->d_down.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
-  if (time) HIPCHK(c, hipMemcpy(time, c->d_time.p + s0.node_off, nb * 8, hipMemcpyDeviceToHost));
-  if (A) HIPCHK(c, hipMemcpy(A, c->d_A.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (up0) HIPCHK(c, copy_now(c, up0, c->d_up0.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (up1) HIPCHK(c, copy_now(c, up1, c->d_up1.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (down) HIPCHK(c, copy_now(c, down, c->d_down.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  if (time) HIPCHK(c, copy_now(c, time, c->d_time.p + s0.node_off, nb * 8, hipMemcpyDeviceToHost));
+  if (A) HIPCHK(c, copy_now(c, A, c->d_A.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
   if (root)
     for (int i = 0; i < count; ++i) root[i] = c->slots[first + i].root;
   return IMA_OK;
@@ -1235,13 +1246,13 @@ int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_
   if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
   hipSetDevice(c->device);
   if (pdg)
-    HIPCHK(c, hipMemcpy(pdg, m.pdg.p, (size_t)m.C * m.L * 8, hipMemcpyDeviceToHost));
-  if (beta) HIPCHK(c, hipMemcpy(beta, m.beta.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, pdg, m.pdg.p, (size_t)m.C * m.L * 8, hipMemcpyDeviceToHost));
+  if (beta) HIPCHK(c, copy_now(c, beta, m.beta.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
   if (stats) {
-    HIPCHK(c, hipMemcpy(stats, m.stats.p, 8 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, stats, m.stats.p, 8 * 8, hipMemcpyDeviceToHost));
     std::vector<uint32_t> acc((size_t)m.C * m.L), cacc(m.C);
-    HIPCHK(c, hipMemcpy(acc.data(), m.acc.p, acc.size() * 4, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(cacc.data(), m.cacc.p, cacc.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, acc.data(), m.acc.p, acc.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, cacc.data(), m.cacc.p, cacc.size() * 4, hipMemcpyDeviceToHost));
     uint64_t a = 0, ca = 0;
     for (uint32_t x : acc) a += x;
     for (uint32_t x : cacc) ca += x;
@@ -1249,7 +1260,7 @@ int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_
     stats[2] = (uint64_t)m.scale_sweeps * m.C;
     stats[3] = ca;
   }
-  if (sweeps) HIPCHK(c, hipMemcpy(sweeps, m.counter.p, 8, hipMemcpyDeviceToHost));
+  if (sweeps) HIPCHK(c, copy_now(c, sweeps, m.counter.p, 8, hipMemcpyDeviceToHost));
   return IMA_OK;
 }
 
@@ -1373,9 +1384,9 @@ int ima_mc3_read_prior(ima_ctx *c, double *probg, double *tvals) {
   auto &m = c->mc3;
   if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
   hipSetDevice(c->device);
-  if (probg) HIPCHK(c, hipMemcpy(probg, m.probg.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
+  if (probg) HIPCHK(c, copy_now(c, probg, m.probg.p, (size_t)m.C * 8, hipMemcpyDeviceToHost));
   if (tvals && m.dev.P > 1)
-    HIPCHK(c, hipMemcpy(tvals, m.tv.p, (size_t)m.C * (m.dev.P - 1) * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, tvals, m.tv.p, (size_t)m.C * (m.dev.P - 1) * 8, hipMemcpyDeviceToHost));
   return IMA_OK;
 }
 
@@ -1383,7 +1394,7 @@ int ima_slots_read_mig(ima_ctx *c, int slot, double *mig_time) {
   if (!c || !mig_time || slot < 0 || slot >= (int)c->slots.size()) return IMA_ERR_ARG;
   const SlotMeta &s = c->slots[slot];
   if (s.nmig)
-    HIPCHK(c, hipMemcpy(mig_time, c->d_mig_time.p + s.mig_off, (size_t)s.nmig * 8,
+    HIPCHK(c, copy_now(c, mig_time, c->d_mig_time.p + s.mig_off, (size_t)s.nmig * 8,
                         hipMemcpyDeviceToHost));
   return IMA_OK;
 }
@@ -1535,12 +1546,12 @@ int ima_slots_write_pops(ima_ctx *c, int first, int count, const int32_t *pop,
   const SlotMeta &s0 = c->slots[first];
   const SlotMeta &s1 = c->slots[first + count - 1];
   const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
-  HIPCHK(c, reserve(c->d_pop, c->nodes, true));
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s0.node_off, c->pop.data() + s0.node_off, nb * 4,
                            hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true));
-  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true));
-  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true));
+  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
   if (c->migs) {
     HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p, c->mig_edge.data(), c->migs * 4,
                              hipMemcpyHostToDevice, c->stream));

@@ -154,44 +154,131 @@ __device__ inline void stage_rows(const LArgs &a, double *sh, int g0, int cnt) {
 // --------------------------------------------------------------- EXACT --
 // Per (point, chunk): running max M and s = sum exp(p - M), s2 = sum
 // exp(2(p - M)) for the effective sample number.
-__global__ __launch_bounds__(kLTile) void jointp_exact_partial(LArgs a, double *work) {
-  extern __shared__ double shrows[];
-  const int pt = blockIdx.x * kLTile + threadIdx.x;
-  const int chunk = blockIdx.y;
-  const int np = a.npsize + a.nmig;
+// Staging folds each row into a compact FP64
+// record that no longer depends on the point: c0 = (the row's base term)
+// minus the hcc of the parameters in range, then (count, weight) of each
+// parameter in [r0, r1).  A point's term is then
+//   p = c0 + sum_k count_k * A_k + weight_k * F_k
+// with A = log 2 - log x (theta, models 0/1; else 0) or log x (migration),
+// F = -2/x (theta: 2 fc / x) or -x (migration); (2/x) fc equals (2 fc)/x
+// exactly, the remaining differences from joint_term's order are the
+// fused multiply-adds and the summation order (about 1e-16 relative).
+// 256 lanes per workgroup share one staged block; each lane carries PTS
+// points, so every record read from LDS serves PTS independent exp chains.
+constexpr int kXLanes = 256;
+
+__host__ __device__ inline int xrec_len(int nr) { return 1 + 2 * nr; }
+
+__device__ inline void stage_compact(const LArgs &a, double *sh, int g0, int cnt, int nr) {
   const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointCoef T;
-  const bool active = pt < a.npoints;
-  point_coef(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
-  double M = -DBL_MAX, s = 0.0, s2 = 0.0;
+  const int rl = xrec_len(nr);
+  for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+    const float *g = a.rows + (size_t)(g0 + r) * a.K;
+    double base;
+    if (a.npops == 2) {
+      base = -(double)g[L.probgp];
+    } else {
+      double t = 0.0;
+      if (a.modeltype == 1)
+        for (int i = 0; i < a.npsize; ++i) t += (double)g[L.qip + i];
+      else
+        for (int i = 0; i < a.nmig; ++i) t += (double)g[L.mip + i];
+      base = -t;
+    }
+    double *o = sh + (size_t)r * rl;
+    for (int k = 0; k < nr; ++k) {
+      const int i = a.r0 + k;
+      if (i < a.npsize) {
+        base -= (double)g[L.hccp + i];
+        o[1 + 2 * k] = (double)g[L.ccp + i];
+        o[2 + 2 * k] = (double)g[L.fcp + i];
+      } else {
+        o[1 + 2 * k] = (double)g[L.mcp + i - a.npsize];
+        o[2 + 2 * k] = (double)g[L.fmp + i - a.npsize];
+      }
+    }
+    o[0] = base;
+  }
+}
+
+template <int NR, int PTS>
+__global__ __launch_bounds__(kXLanes) void jointp_exact_fast(LArgs a, double *work) {
+  extern __shared__ double shrows[];
+  const int np = a.npsize + a.nmig;
+  double A[PTS][NR], F[PTS][NR], M[PTS], s[PTS], s2[PTS];
+  int pt[PTS];
+#pragma unroll
+  for (int j = 0; j < PTS; ++j) {
+    pt[j] = blockIdx.x * (kXLanes * PTS) + j * kXLanes + threadIdx.x;
+    const double *xp = a.x + (size_t)(pt[j] < a.npoints ? pt[j] : 0) * np;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int i = a.r0 + k;
+      const double x = xp[i];
+      if (i < a.npsize) {
+        A[j][k] = (a.modeltype == 0 || a.modeltype == 1) ? kLn2 - log(x) : 0.0;
+        F[j][k] = -2.0 * (1.0 / x);
+      } else {
+        A[j][k] = log(x);
+        F[j][k] = -x;
+      }
+    }
+    M[j] = -DBL_MAX;
+    s[j] = 0.0;
+    s2[j] = 0.0;
+  }
+  const int chunk = blockIdx.y;
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
+  constexpr int RL = 1 + 2 * NR;
   for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
     const int cnt = min(a.stage, gend - g0);
     __syncthreads();
-    stage_rows(a, shrows, g0, cnt);
+    stage_compact(a, shrows, g0, cnt, NR);
     __syncthreads();
     for (int r = 0; r < cnt; ++r) {
-      const double p = joint_term(shrows + (size_t)r * a.K, L, a.npsize, a.nmig, a.npops,
-                                  a.modeltype, a.r0, a.r1, T);
-      if (p > M) {
-        const double sc = exp(M - p);
-        s = s * sc + 1.0;
-        s2 = s2 * sc * sc + 1.0;
-        M = p;
-      } else {
-        const double w = exp(p - M);
-        s += w;
-        s2 += w * w;
+      const double *o = shrows + (size_t)r * RL;
+      double v[RL];
+#pragma unroll
+      for (int q = 0; q < RL; ++q) v[q] = o[q];
+#pragma unroll
+      for (int j = 0; j < PTS; ++j) {
+        double p = v[0];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          p = fma(v[1 + 2 * k], A[j][k], p);
+          p = fma(v[2 + 2 * k], F[j][k], p);
+        }
+        if (p > M[j]) {
+          const double sc = exp(M[j] - p);
+          s[j] = s[j] * sc + 1.0;
+          s2[j] = s2[j] * sc * sc + 1.0;
+          M[j] = p;
+        } else {
+          const double w = exp(p - M[j]);
+          s[j] += w;
+          s2[j] = fma(w, w, s2[j]);
+        }
       }
     }
   }
-  if (active) {
-    double *o = work + ((size_t)pt * a.nchunks + chunk) * 3;
-    o[0] = M;
-    o[1] = s;
-    o[2] = s2;
-  }
+#pragma unroll
+  for (int j = 0; j < PTS; ++j)
+    if (pt[j] < a.npoints) {
+      double *o = work + ((size_t)pt[j] * a.nchunks + chunk) * 3;
+      o[0] = M[j];
+      o[1] = s[j];
+      o[2] = s2[j];
+    }
+}
+
+template <int NR>
+hipError_t launch_exact_fast(const LArgs &a, dim3 grid_y, size_t lds, hipStream_t st,
+                             double *work) {
+  constexpr int PTS = NR <= 8 ? 2 : 1;
+  const dim3 grid((a.npoints + kXLanes * PTS - 1) / (kXLanes * PTS), grid_y.y);
+  hipLaunchKernelGGL((jointp_exact_fast<NR, PTS>), grid, dim3(kXLanes), lds, st, a, work);
+  return hipSuccess;
 }
 
 __global__ void jointp_exact_final(LArgs a, const double *work, double *out, double *ess) {
@@ -365,6 +452,12 @@ static void chunking(const LTable &t, int npoints, int *nchunks, int *chunk_rows
 
 size_t jointp_lds_bytes(const LTable &t) { return (size_t)stage_of(t) * t.K * 8; }
 
+// Compact records of the fast EXACT kernel: stage_of(t) rows of at most
+// 1 + 2 * 16 doubles.
+static size_t exact_lds_bytes(const LTable &t, int nr) {
+  return (size_t)stage_of(t) * xrec_len(nr) * 8;
+}
+
 size_t jointp_work_bytes(const LTable &t, int npoints) {
   int nc, cr;
   chunking(t, npoints, &nc, &cr);
@@ -383,7 +476,20 @@ hipError_t launch_jointp(const LTable &t, const double *x, int npoints, int mode
   const size_t lds = jointp_lds_bytes(t);
   const int fin_blocks = (npoints + 255) / 256;
   if (mode == IMA_LMODE_EXACT) {
-    hipLaunchKernelGGL(jointp_exact_partial, grid, dim3(kLTile), lds, st, a, work);
+    const int nr = r1 - r0;
+    const size_t xl = exact_lds_bytes(t, nr);
+    switch (nr) {
+#define IMA_XCASE(N) \
+  case N:            \
+    launch_exact_fast<N>(a, grid, xl, st, work); \
+    break;
+      IMA_XCASE(1) IMA_XCASE(2) IMA_XCASE(3) IMA_XCASE(4) IMA_XCASE(5) IMA_XCASE(6)
+      IMA_XCASE(7) IMA_XCASE(8) IMA_XCASE(9) IMA_XCASE(10) IMA_XCASE(11) IMA_XCASE(12)
+      IMA_XCASE(13) IMA_XCASE(14) IMA_XCASE(15) IMA_XCASE(16)
+#undef IMA_XCASE
+      default:
+        return hipErrorInvalidValue;
+    }
     hipLaunchKernelGGL(jointp_exact_final, dim3(fin_blocks), dim3(256), 0, st, a, work, out,
                        ess);
   } else {

@@ -8,17 +8,23 @@
 // points, splitting large queries across launches, so a query that arrives
 // while another is on the GPU joins the next launch.  An admitted analysis
 // gets a context of its own (own stream, own captured sweep graphs) and
-// advances `quantum` sweeps per step, overlapping the other analyses and
-// the L-mode launches on the device.  Completion is polled through events;
-// nothing blocks except building an analysis' context at admission.
-// Latency is measured from submission to the step that sees the request
-// complete.  Results do not depend on what else is running: an L-mode
+// advances a quantum of sweeps per step, overlapping the other analyses and
+// the L-mode launches on the device.  Completion is polled through events.
+// Building an admitted analysis' context (uploads, set-up, graph capture,
+// first quantum) and collecting a finished one (result copies, context
+// teardown, whose frees wait for the device) run on a worker thread, so
+// the polling loop never waits for the device.  Latency is measured from
+// submission to the step that sees the request complete.  Results do not depend on what else is running: an L-mode
 // point's sums use the table's fixed chunking, and an analysis' sweep is
 // the same however its sweeps are split into runs.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
+#include <thread>
 #include <map>
 #include <memory>
 #include <string>
@@ -36,6 +42,9 @@ double now_s() {
 }
 
 enum { RK_LMODE = 0, RK_ANALYSIS = 1 };
+// analysis phases: loop-owned (IDLE), worker jobs pending (BUILDING,
+// COLLECTING), worker done (BUILT, COLLECTED)
+enum { PH_IDLE = 0, PH_BUILDING = 1, PH_BUILT = 2, PH_COLLECTING = 3, PH_COLLECTED = 4 };
 
 struct AnalysisData {  // host copy of an ima_analysis
   int nloci = 0, nchains = 0, npops = 0, npsize = 0, nmig = 0, precision = 0, nsweeps = 0;
@@ -59,12 +68,18 @@ struct Req {
   int table = -1, mode = 0, modeltype = 0, r0 = 0, r1 = 0, np = 0, npoints = 0;
   int dispatched = 0, finished = 0;
   std::vector<double> x, out, ess;
-  // analysis
+  // analysis.  `phase` hands the request between the polling loop and the
+  // worker: the worker owns ctx, ev and the results while a job of its is
+  // pending (phase BUILDING or COLLECTING), the loop owns them otherwise.
   std::unique_ptr<AnalysisData> a;
   ima_ctx *ctx = nullptr;
   int remaining = 0;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr, ev0 = nullptr;  // quantum end / start
+  int q_last = 0;                           // sweeps in the quantum in flight
+  double sweep_ms = -1.0;                   // measured GPU time per sweep
   bool inflight = false;
+  std::atomic<int> phase{0};
+  int wrc = 0;  // the worker's result code
   std::vector<double> pdg, probg, beta;
   uint64_t stats[8] = {};
 };
@@ -90,11 +105,23 @@ struct ima_server {
   std::vector<int64_t> active;  // admitted, not finished
   Batch batch;
   int64_t next_id = 1;
-  int quantum = 25;
+  int quantum = 25;  // most sweeps per quantum
+  // Quanta are sized to about this much GPU time: streams share the
+  // device's few hardware queues, so work queued by one analysis can hold
+  // up an L-mode launch behind it for as long as one quantum runs.
+  double quantum_ms = 4.0;
   int max_points = 8192;
   int max_analyses = 8;
   std::vector<std::pair<int, double>> lat;  // (kind, seconds) of finished requests
   std::string err;
+  // worker thread: builds and collects analyses
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Req *> jobs;
+  bool stop = false;
+  // bytes of finished analyses whose contexts the worker is still freeing
+  std::atomic<long long> freeing{0};
 };
 
 namespace {
@@ -137,7 +164,8 @@ void finish(ima_server *s, Req &r, int state) {
 
 void drop_ctx(Req &r) {
   if (r.ev) hipEventDestroy(r.ev);
-  r.ev = nullptr;
+  if (r.ev0) hipEventDestroy(r.ev0);
+  r.ev = r.ev0 = nullptr;
   if (r.ctx) ima_ctx_destroy(r.ctx);
   r.ctx = nullptr;
 }
@@ -196,14 +224,22 @@ int build_analysis(ima_server *s, Req &r) {
   STRY(ima_mc3_setup(c, a.nchains, a.nloci, first, a.u.data(), a.kappa.data(), a.beta.data(),
                      a.seed, a.precision, std::max(1, a.nchains / 10), 0.2));
   STRY(ima_mc3_set_prior(c, a.tvals.empty() ? nullptr : a.tvals.data(), a.tmax));
-  if (hipEventCreateWithFlags(&r.ev, hipEventDisableTiming) != hipSuccess) return IMA_ERR_HIP;
+  if (hipEventCreate(&r.ev) != hipSuccess || hipEventCreate(&r.ev0) != hipSuccess)
+    return IMA_ERR_HIP;
   return IMA_OK;
 }
 
+// The first quantum is one sweep, to measure; later ones hold about
+// quantum_ms of GPU time at the measured rate.
 int run_quantum(ima_server *s, Req &r) {
-  const int q = std::min(s->quantum, r.remaining);
+  int q = 1;
+  if (r.sweep_ms > 0)
+    q = (int)std::max(1.0, std::min((double)s->quantum, s->quantum_ms / r.sweep_ms));
+  q = std::min(q, r.remaining);
+  if (hipEventRecord(r.ev0, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
   STRY(mc3_run_async(r.ctx, q, IMA_MC3_GRAPH | IMA_MC3_SCALE));
   if (hipEventRecord(r.ev, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
+  r.q_last = q;
   r.remaining -= q;
   r.inflight = true;
   return IMA_OK;
@@ -219,6 +255,65 @@ int collect_analysis(Req &r) {
   std::vector<double> tv((size_t)a.nchains * std::max(a.npops - 1, 1));
   STRY(ima_mc3_read_prior(r.ctx, r.probg.data(), tv.data()));
   return IMA_OK;
+}
+
+// Hand r to the worker for `phase` (PH_BUILDING or PH_COLLECTING).
+void post(ima_server *s, Req &r, int phase) {
+  r.phase.store(phase, std::memory_order_release);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->jobs.push_back(&r);
+  }
+  s->cv.notify_one();
+}
+
+// The worker: builds admitted analyses (and launches their first quantum,
+// capturing the sweep graphs) and collects finished ones.  A collected
+// request is published before its context is torn down, so the frees'
+// wait for the device does not delay the completion; the bytes stay
+// counted in `freeing` until they are back.
+void worker_main(ima_server *s) {
+  hipSetDevice(s->device);
+  for (;;) {
+    Req *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !s->jobs.empty(); });
+      if (s->stop) return;
+      r = s->jobs.front();
+      s->jobs.pop_front();
+    }
+    if (r->phase.load(std::memory_order_acquire) == PH_BUILDING) {
+      int rc = build_analysis(s, *r);
+      if (rc == IMA_OK && r->remaining > 0) rc = run_quantum(s, *r);
+      if (rc != IMA_OK) {
+        r->err = r->ctx ? ima_ctx_last_error(r->ctx) : "analysis set-up failed";
+        drop_ctx(*r);
+      }
+      r->wrc = rc;
+      r->phase.store(PH_BUILT, std::memory_order_release);
+      continue;
+    }
+    // PH_COLLECTING: wrc != IMA_OK marks a failed run, torn down unread
+    ima_ctx *ctx = r->ctx;
+    hipEvent_t ev = r->ev, ev0 = r->ev0;
+    const long long bytes = (long long)r->bytes;
+    int rc = r->wrc;
+    if (rc == IMA_OK) {
+      rc = collect_analysis(*r);
+      if (rc != IMA_OK) r->err = ima_ctx_last_error(ctx);
+    }
+    r->ctx = nullptr;
+    r->ev = r->ev0 = nullptr;
+    r->wrc = rc;
+    s->freeing.fetch_add(bytes);
+    r->phase.store(PH_COLLECTED, std::memory_order_release);
+    // r may be released from here on
+    if (ev) hipEventDestroy(ev);
+    if (ev0) hipEventDestroy(ev0);
+    if (ctx) ima_ctx_destroy(ctx);
+    s->freeing.fetch_sub(bytes);
+  }
 }
 
 int ensure_batch_buffers(ima_server *s, int npoints, int np, size_t work) {
@@ -331,12 +426,19 @@ int ima_server_create(int device, double budget_bytes, int quantum, ima_server *
     s->budget = fr / 10 * 8;
   }
   if (quantum > 0) s->quantum = quantum;
+  s->worker = std::thread(worker_main, s.get());
   *out = s.release();
   return IMA_OK;
 }
 
 void ima_server_destroy(ima_server *s) {
   if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->worker.joinable()) s->worker.join();
   hipSetDevice(s->device);
   for (auto &kv : s->reqs) {
     Req &r = *kv.second;
@@ -365,6 +467,15 @@ int ima_server_table(ima_server *s, const float *rows, int G, int K, int npsize,
   const int rc = ima_lmode_table(s->lctx, rows, G, K, npsize, nmig, npops, table);
   if (rc != IMA_OK) return sfail(s, rc, ima_ctx_last_error(s->lctx));
   s->used += (size_t)G * K * 4;
+  // size the shared launch buffers for a full launch on this table now, so
+  // that no step has to grow them (a free waits for the whole device)
+  // (buffers in use by a launch grow later, when it has finished)
+  const LTable t = ctx_ltable(s->lctx, *table);
+  if (!s->batch.inflight) {
+    const int brc = ensure_batch_buffers(s, s->max_points, t.npsize + t.nmig,
+                                         jointp_work_bytes(t, s->max_points));
+    if (brc != IMA_OK) return sfail(s, brc, "L-mode launch buffers");
+  }
   return IMA_OK;
 }
 
@@ -498,36 +609,54 @@ int ima_server_step(ima_server *s, int *pending) {
   const std::vector<int64_t> act = s->active;
   for (int64_t id : act) {
     Req &r = *s->reqs[id];
-    if (r.kind != RK_ANALYSIS || !r.inflight) continue;
+    if (r.kind != RK_ANALYSIS) continue;
+    const int ph = r.phase.load(std::memory_order_acquire);
+    if (ph == PH_BUILDING || ph == PH_COLLECTING) continue;
+    if (ph == PH_COLLECTED) {
+      finish(s, r, r.wrc == IMA_OK ? IMA_REQ_DONE : IMA_REQ_FAILED);
+      continue;
+    }
+    if (ph == PH_BUILT) {
+      r.phase.store(PH_IDLE, std::memory_order_relaxed);
+      if (r.wrc != IMA_OK) {
+        finish(s, r, IMA_REQ_FAILED);
+        continue;
+      }
+      // charge what the context holds rather than the estimate
+      const size_t real = ctx_device_bytes(r.ctx) + kCtxOverhead;
+      s->used = s->used - std::min(s->used, r.bytes) + real;
+      r.bytes = real;
+      if (!r.inflight) {  // no sweeps asked for
+        post(s, r, PH_COLLECTING);
+        continue;
+      }
+    }
+    if (!r.inflight) continue;
     const hipError_t q = hipEventQuery(r.ev);
     if (q == hipErrorNotReady) continue;
     r.inflight = false;
-    if (q != hipSuccess) {
-      r.err = hipGetErrorString(q);
-      drop_ctx(r);
-      finish(s, r, IMA_REQ_FAILED);
-      continue;
+    int rc = q == hipSuccess ? IMA_OK : IMA_ERR_HIP;
+    if (rc != IMA_OK) r.err = hipGetErrorString(q);
+    float ms = 0.0f;
+    if (rc == IMA_OK && r.q_last > 0 && hipEventElapsedTime(&ms, r.ev0, r.ev) == hipSuccess) {
+      const double per = (double)ms / r.q_last;
+      r.sweep_ms = r.sweep_ms > 0 ? 0.5 * (r.sweep_ms + per) : per;
     }
-    int rc = IMA_OK;
-    if (r.remaining > 0) {
+    if (rc == IMA_OK && r.remaining > 0) {
       rc = run_quantum(s, r);
-    } else {
-      rc = collect_analysis(r);
-      drop_ctx(r);
-      if (rc == IMA_OK) finish(s, r, IMA_REQ_DONE);
+      if (rc == IMA_OK) continue;
+      r.err = ima_ctx_last_error(r.ctx);
     }
-    if (rc != IMA_OK) {
-      r.err = r.ctx ? ima_ctx_last_error(r.ctx) : "analysis failed";
-      drop_ctx(r);
-      finish(s, r, IMA_REQ_FAILED);
-    }
+    r.wrc = rc;
+    post(s, r, PH_COLLECTING);
   }
   // 2. admission, first fit in arrival order
   int running = 0;
   for (int64_t id : s->active) running += s->reqs[id]->kind == RK_ANALYSIS;
+  const size_t freeing = (size_t)std::max(0LL, s->freeing.load());
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     Req &r = *s->reqs[*it];
-    const bool room = s->used + r.bytes <= s->budget &&
+    const bool room = s->used + freeing + r.bytes <= s->budget &&
                       (r.kind != RK_ANALYSIS || running < s->max_analyses);
     if (!room) {
       ++it;
@@ -539,24 +668,7 @@ int ima_server_step(ima_server *s, int *pending) {
     s->active.push_back(r.id);
     if (r.kind == RK_ANALYSIS) {
       ++running;
-      int rc = build_analysis(s, r);
-      if (rc == IMA_OK) {
-        // charge what the context holds rather than the estimate
-        const size_t real = ctx_device_bytes(r.ctx) + kCtxOverhead;
-        s->used = s->used - r.bytes + real;
-        r.bytes = real;
-        rc = r.remaining > 0 ? run_quantum(s, r) : IMA_OK;
-        if (rc == IMA_OK && r.remaining == 0 && !r.inflight) {
-          rc = collect_analysis(r);
-          drop_ctx(r);
-          if (rc == IMA_OK) finish(s, r, IMA_REQ_DONE);
-        }
-      }
-      if (rc != IMA_OK) {
-        r.err = r.ctx ? ima_ctx_last_error(r.ctx) : "analysis set-up failed";
-        drop_ctx(r);
-        finish(s, r, IMA_REQ_FAILED);
-      }
+      post(s, r, PH_BUILDING);
     }
   }
   // 3. the next L-mode launch

@@ -317,8 +317,11 @@ typedef struct {
   int precision, nsweeps;
 } ima_analysis;
 
-/* budget_bytes <= 0: 80% of the device's free memory; quantum <= 0: 25
- * sweeps per analysis step. */
+/* budget_bytes <= 0: 80% of the device's free memory.  An analysis runs in
+ * quanta of sweeps: the first is one sweep, later ones about 4 ms of GPU
+ * time at its measured sweep rate, at most `quantum` sweeps (<= 0: 25).
+ * Short quanta keep L-mode launches from queueing behind long runs of
+ * sweeps on the hardware queues that streams share. */
 int ima_server_create(int device, double budget_bytes, int quantum, ima_server **out);
 void ima_server_destroy(ima_server *srv);
 const char *ima_server_last_error(const ima_server *srv);
