@@ -28,6 +28,15 @@ class Item(C.Structure):
                 ("kappa", C.c_double), ("tr", C.c_double)]
 
 
+class SWMove(C.Structure):
+    """ima_sw_move."""
+    _fields_ = [(f, C.c_int32) for f in ("slot", "edge", "downedge", "sisedge", "newsisedge",
+                                          "copyA0", "copyA1", "down_was_root", "hold_down_A",
+                                          "minA", "maxA", "flip")] + \
+               [("copy_dlikeA", C.c_double * 3), ("newsis_dlikeA", C.c_double),
+                ("u", C.c_double), ("uni", C.c_double)]
+
+
 i32p = C.POINTER(C.c_int32)
 i8p = C.POINTER(C.c_int8)
 dp = C.POINTER(C.c_double)
@@ -62,6 +71,7 @@ PROTOS = {
     "ima_mc3_set_prior": (C.c_int, [vp, dp, C.c_double]),
     "ima_mc3_read_prior": (C.c_int, [vp, dp, dp]),
     "ima_slots_read_mig": (C.c_int, [vp, C.c_int, dp]),
+    "ima_sw_update_A": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, dp, i32p, dp]),
     "ima_server_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]),
     "ima_server_destroy": (None, [vp]),
     "ima_server_last_error": (C.c_char_p, [vp]),
@@ -125,4 +135,4 @@ def declared_symbols(header=None):
     import re
     header = header or os.path.join(os.path.dirname(HERE), "include", "ima_gpu.h")
     txt = open(header).read()
-    return sorted(set(re.findall(r"\b(ima_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(ima_[A-Za-z0-9_]+)\s*\(", txt)))

@@ -764,6 +764,67 @@ int ima_sw_dlike(ima_ctx *c, const ima_item *item, double *dlikeA) {
   return IMA_OK;
 }
 
+int ima_sw_update_A(ima_ctx *c, const ima_sw_move *moves, int nmoves, int calc, double *dlike,
+                    double *Aterm, int32_t *newA, double *edge_dlikeA) {
+  if (!c || nmoves < 0 || (nmoves > 0 && !moves)) return IMA_ERR_ARG;
+  if (nmoves == 0) return IMA_OK;
+  std::vector<int32_t> seen;
+  for (int i = 0; i < nmoves; ++i) {
+    const ima_sw_move &m = moves[i];
+    if (m.slot < 0 || m.slot >= (int)c->slots.size())
+      return fail(c, IMA_ERR_ARG, "move %d: slot %d out of range", i, m.slot);
+    const SlotMeta &s = c->slots[m.slot];
+    if (c->loci[s.locus].model != IMA_MODEL_SW)
+      return fail(c, IMA_ERR_ARG, "move %d: slot %d is not a stepwise locus", i, m.slot);
+    if (s.root < 0) return fail(c, IMA_ERR_STATE, "move %d: slot holds no genealogy", i);
+    const int ne = 2 * s.n - 1;
+    const int32_t ed[4] = {m.edge, m.downedge, m.sisedge, m.newsisedge};
+    for (int k = 0; k < 4; ++k)
+      if (ed[k] < 0 || ed[k] >= ne) return fail(c, IMA_ERR_ARG, "move %d: edge out of range", i);
+    if (m.edge == m.downedge || m.newsisedge == m.downedge || m.edge == m.newsisedge ||
+        m.sisedge == m.downedge || m.sisedge == m.edge)
+      return fail(c, IMA_ERR_ARG, "move %d: edges must be distinct", i);
+    if (!(m.uni > 0.0 && m.uni <= 1.0) || m.minA > m.maxA || !(m.u > 0.0))
+      return fail(c, IMA_ERR_ARG, "move %d: uni in (0, 1], minA <= maxA, u > 0", i);
+    seen.push_back(m.slot);
+  }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(c, IMA_ERR_ARG, "two moves on one slot");
+  hipSetDevice(c->device);
+  int rc;
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  HIPCHK(c, reserve(c->d_swmoves, (size_t)nmoves));
+  HIPCHK(c, reserve(c->d_aux64, (size_t)nmoves * 6));
+  HIPCHK(c, reserve(c->d_aux, (size_t)nmoves * 2));
+  HIPCHK(c, hipMemcpyAsync(c->d_swmoves.p, moves, (size_t)nmoves * sizeof(ima_sw_move),
+                           hipMemcpyHostToDevice, c->stream));
+  // pass 0: structure of every move; nothing is written unless all pass
+  HIPCHK(c, launch_sw_update(view(c), c->d_A.p, c->d_swmoves.p, nmoves, 0, calc, c->d_aux64.p,
+                             c->d_aux.p, c->stream));
+  std::vector<int32_t> iout((size_t)nmoves * 2);
+  HIPCHK(c, copy_now(c, iout.data(), c->d_aux.p, iout.size() * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < nmoves; ++i)
+    if (iout[2 * i + 1])
+      return fail(c, IMA_ERR_ARG, "move %d: edge %d and newsisedge %d are not under downedge %d",
+                  i, moves[i].edge, moves[i].newsisedge, moves[i].downedge);
+  HIPCHK(c, launch_sw_update(view(c), c->d_A.p, c->d_swmoves.p, nmoves, 1, calc, c->d_aux64.p,
+                             c->d_aux.p, c->stream));
+  std::vector<double> out((size_t)nmoves * 6);
+  HIPCHK(c, hipMemcpyAsync(out.data(), c->d_aux64.p, out.size() * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, copy_now(c, iout.data(), c->d_aux.p, iout.size() * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < nmoves; ++i) {
+    if (dlike) dlike[i] = out[6 * i];
+    if (Aterm) Aterm[i] = out[6 * i + 1];
+    if (newA) newA[i] = iout[2 * i];
+    if (edge_dlikeA)
+      for (int k = 0; k < 4; ++k) edge_dlikeA[4 * i + k] = out[6 * i + 2 + k];
+  }
+  return IMA_OK;
+}
+
 int ima_likelihood_hky(ima_ctx *c, int locus, const int32_t *up0, const int32_t *up1,
                        const int32_t *down, const double *time, int root, double u,
                        double kappa, int e1, int e2, int e3, int e4, double *loglik) {

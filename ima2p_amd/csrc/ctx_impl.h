@@ -79,6 +79,7 @@ struct ima_ctx {
   DevBuf<int32_t> d_status;
   DevBuf<int32_t> d_aux;
   DevBuf<double> d_aux64;
+  DevBuf<ima_sw_move> d_swmoves;  // ima_sw_update_A's moves
   DevBuf<int4> d_units, d_red;
   DevBuf<double> d_partial;
 

@@ -167,6 +167,9 @@ hipError_t launch_is(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, int max_n, int max_w,
                      double *out, int32_t *status, int32_t *site_node,
                      int32_t *counts, hipStream_t st);
+hipError_t launch_sw_update(const DeviceView &v, int32_t *A, const ima_sw_move *moves,
+                            int nmoves, int pass, int calc, double *out, int32_t *iout,
+                            hipStream_t st);
 hipError_t launch_sw(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, double *out,
                      int32_t *status, double *dlike, hipStream_t st);

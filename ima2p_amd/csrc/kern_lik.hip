@@ -277,6 +277,91 @@ __global__ __launch_bounds__(kWave) void sw_kernel(
   }
 }
 
+// finishSWupdateA (update_gtree_common.cpp:2219-2362), one lane per move,
+// in the reference's order of operations.  Pass 0 checks every move's
+// structure (status 1: edge and newsisedge are not under downedge) and
+// writes nothing else; pass 1, run only when all moves passed, proposes,
+// stores and scores.
+__device__ inline double sw_upper(const int32_t *up0, const double *tm, int n, int e) {
+  return e >= n ? tm[up0[e]] : 0.0;
+}
+
+__global__ __launch_bounds__(kWave) void sw_update_kernel(
+    DeviceView v, int32_t *__restrict__ Aw, const ima_sw_move *__restrict__ moves, int nmoves,
+    int pass, int calc, double *__restrict__ out, int32_t *__restrict__ iout) {
+  const int i = blockIdx.x * kWave + threadIdx.x;
+  if (i >= nmoves) return;
+  const ima_sw_move m = moves[i];
+  const SlotMeta sm = v.slots[m.slot];
+  const int n = sm.n;
+  const int32_t *up0 = v.up0 + sm.node_off, *dn = v.down + sm.node_off;
+  const double *tm = v.time + sm.node_off;
+  int32_t *A = Aw + sm.node_off;  // the view's allele array, writable
+  if (pass == 0) {
+    iout[2 * i + 1] = (dn[m.edge] == m.downedge && dn[m.newsisedge] == m.downedge) ? 0 : 1;
+    return;
+  }
+  const int oldA = A[m.downedge];
+  const double holdsis = m.newsisedge != m.sisedge ? m.newsis_dlikeA : 0.0;
+  const double oldlikeadj = m.copy_dlikeA[0] + m.copy_dlikeA[1] + m.copy_dlikeA[2] + holdsis;
+  const int e[3] = {m.edge, m.newsisedge, dn[m.edge]};
+  double t[3] = {0.0, 0.0, 0.0};
+  int wsum = 0, j = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (dn[e[k]] == -1) continue;
+    t[k] = tm[e[k]] - sw_upper(up0, tm, n, e[k]);
+    const int d = (k < 2 ? A[e[k]] : A[dn[e[k]]]) - oldA;
+    wsum += abs(d);
+    ++j;
+  }
+  double gnew = j / (double)(wsum + j);
+  if (gnew > 0.95) gnew = 0.95;
+  int dA = (int)ceil(log(m.uni) / log(1.0 - gnew)) - 1;
+  if (m.flip) dA = -dA;
+  const int newA = dA >= 0 ? min(m.maxA, oldA + dA) : max(m.minA, oldA + dA);
+  A[m.downedge] = newA;
+  dA = newA - oldA;
+  double sis_dl = 0.0;
+  if (dn[m.sisedge] != -1 && m.sisedge != m.newsisedge) {
+    const double ts = tm[m.sisedge] - sw_upper(up0, tm, n, m.sisedge);
+    const int d = A[m.sisedge] - A[dn[m.sisedge]];
+    sis_dl = -ts * m.u + log(d_bessi(d, ts * m.u));
+  }
+  double likeadj = sis_dl;
+  double dl[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 3; ++k) {
+    if (dn[e[k]] == -1) continue;
+    const int d = (k < 2 ? A[e[k]] : A[dn[e[k]]]) - newA;
+    dl[k] = -(t[k] * m.u) + log(d_bessi(d, t[k] * m.u));
+    likeadj += dl[k];
+  }
+  int rsum = abs(m.copyA0 - newA) + abs(m.copyA1 - newA), r = 2;
+  if (!m.down_was_root) {
+    rsum += abs(m.hold_down_A - newA);
+    ++r;
+  }
+  double gold = r / (double)(rsum + r);
+  if (gold > 0.95) gold = 0.95;
+  double *o = out + 6 * i;
+  o[0] = calc ? likeadj - oldlikeadj : 0.0;
+  o[1] = calc ? (abs(dA) * log(1 - gold) + log(gold)) - (abs(dA) * log(1 - gnew) + log(gnew))
+              : 0.0;
+  o[2] = m.sisedge == m.newsisedge ? dl[1] : sis_dl;
+  o[3] = dl[0];
+  o[4] = dl[1];
+  o[5] = dl[2];
+  iout[2 * i] = newA;
+}
+
+hipError_t launch_sw_update(const DeviceView &v, int32_t *A, const ima_sw_move *moves,
+                            int nmoves, int pass, int calc, double *out, int32_t *iout,
+                            hipStream_t st) {
+  if (nmoves == 0) return hipSuccess;
+  hipLaunchKernelGGL(sw_update_kernel, dim3((nmoves + kWave - 1) / kWave), dim3(kWave), 0, st,
+                     v, A, moves, nmoves, pass, calc, out, iout);
+  return hipGetLastError();
+}
+
 hipError_t launch_sw(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, double *out,
                      int32_t *status, double *dlike, hipStream_t st) {

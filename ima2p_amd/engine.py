@@ -261,6 +261,25 @@ class Engine:
         self._chk(self.lib.ima_sw_dlike(self.h, it, out.ctypes.data_as(L.dp)), "sw_dlike")
         return out
 
+    def sw_update_A(self, moves, calc=True):
+        """finishSWupdateA for a batch of moves (objects with the fields of
+        ima_sw_move, e.g. sim.SWMove with .slot set).  Returns (dlike,
+        Aterm, newA, edge_dlikeA[n, 4])."""
+        n = len(moves)
+        arr = (L.SWMove * max(n, 1))()
+        for i, m in enumerate(moves):
+            arr[i] = L.SWMove(m.slot, m.edge, m.downedge, m.sisedge, m.newsisedge, m.copyA0,
+                              m.copyA1, int(m.down_was_root), m.hold_down_A, m.minA, m.maxA,
+                              int(m.flip), (C.c_double * 3)(*m.copy_dlikeA), m.newsis_dlikeA,
+                              m.u, m.uni)
+        dl, at, ed = np.zeros(n), np.zeros(n), np.zeros((n, 4))
+        na = np.zeros(n, np.int32)
+        self._chk(self.lib.ima_sw_update_A(self.h, C.cast(arr, C.c_void_p), n, int(bool(calc)),
+                                           dl.ctypes.data_as(L.dp), at.ctypes.data_as(L.dp),
+                                           na.ctypes.data_as(L.i32p), ed.ctypes.data_as(L.dp)),
+                  "sw_update_A")
+        return dl, at, na, ed
+
     def likelihood_hky(self, locus, g, u, kappa, e=(-1, -1, -1, -1)):
         up0, p0 = _i32(g.up0)
         up1, p1 = _i32(g.up1)

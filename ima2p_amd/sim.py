@@ -216,6 +216,105 @@ def relabel_internal(g: Genealogy, rng: np.random.Generator) -> Genealogy:
     return Genealogy(n, up0, up1, down, time, int(perm[g.root]), pop, mig)
 
 
+# ------------------------------------------------------ genealogy moves --
+
+def move_edge(g: Genealogy, edge: int, newsis: int, newtime: float) -> Genealogy:
+    """The reference's genealogy move (update_gtree.cpp:760-800): detach
+    `edge` with its parent node, join the old sister to the parent's parent
+    (joinsisdown), and re-attach `edge` on `newsis` at `newtime` through the
+    freed node (splitsisdown).  `time` is at the rootward end of an edge;
+    newtime must lie above both edges' upper ends and below newsis' lower
+    end.  Population labels and migrations are left as they were (single
+    population use)."""
+    h = g.copy()
+    down = h.down[edge]
+    assert down != -1 and newsis not in (edge, down)
+    sis = h.up1[down] if h.up0[down] == edge else h.up0[down]
+    par = h.down[down]
+    # joinsisdown
+    h.down[sis] = par
+    h.time[sis] = h.time[down]
+    if par == -1:
+        h.root = int(sis)
+    elif h.up0[par] == down:
+        h.up0[par] = sis
+    else:
+        h.up1[par] = sis
+    # splitsisdown
+    upper = lambda e: h.time[h.up0[e]] if h.up0[e] != -1 else 0.0  # noqa: E731
+    assert newtime > upper(edge) and newtime > upper(newsis) and newtime < h.time[newsis]
+    q = h.down[newsis]
+    h.up0[down], h.up1[down] = edge, newsis
+    h.down[down] = q
+    h.time[down] = h.time[newsis]
+    h.time[newsis] = newtime
+    h.time[edge] = newtime
+    h.down[edge] = h.down[newsis] = down
+    if q == -1:
+        h.root = int(down)
+    elif h.up0[q] == newsis:
+        h.up0[q] = down
+    else:
+        h.up1[q] = down
+    return h
+
+
+@dataclass
+class SWMove:
+    """Inputs of finishSWupdateA (update_gtree_common.cpp:2219-2362) for one
+    move: the edges, the stored copies (storeAinfo, :820-846) and the draws
+    of geometric() and bitran()."""
+    edge: int
+    downedge: int
+    sisedge: int
+    newsisedge: int
+    copyA0: int
+    copyA1: int
+    down_was_root: bool
+    hold_down_A: int
+    minA: int
+    maxA: int
+    flip: int
+    copy_dlikeA: tuple
+    newsis_dlikeA: float
+    u: float
+    uni: float
+    slot: int = 0
+
+
+def random_sw_move(rng, g: Genealogy, A, dlikeA, u, minA, maxA):
+    """A random valid move on g with the SWMove that scores it; returns
+    (genealogy after the move, move)."""
+    N = 2 * g.n - 1
+    while True:
+        edge = int(rng.integers(0, N))
+        if g.down[edge] == -1:
+            continue
+        down = int(g.down[edge])
+        sis = int(g.up1[down] if g.up0[down] == edge else g.up0[down])
+        cands = [e for e in range(N) if e not in (edge, down)]
+        newsis = int(rng.choice(cands))
+        # the tree after joinsisdown decides newsis' span
+        t_lo = lambda e: g.time[g.up0[e]] if g.up0[e] != -1 else 0.0  # noqa: E731
+        lo = max(t_lo(edge), t_lo(newsis))
+        hi = g.time[down] if newsis == sis else g.time[newsis]
+        if newsis == g.root or (newsis == sis and g.down[down] == -1):
+            hi = lo + 2.0 * (1.0 + lo)
+        if not hi > lo:
+            continue
+        newtime = float(rng.uniform(lo, hi))
+        if newtime <= lo or newtime >= hi:
+            continue
+        h = move_edge(g, edge, newsis, newtime)
+        par = int(g.down[down])
+        mv = SWMove(edge, down, sis, newsis, int(A[edge]), int(A[sis]), par == -1,
+                    int(A[par]) if par != -1 else -1, minA, maxA, int(rng.integers(0, 2)),
+                    (float(dlikeA[edge]), float(dlikeA[sis]),
+                     float(dlikeA[down]) if par != -1 else 0.0),
+                    float(dlikeA[newsis]), float(u), float(rng.uniform(1e-12, 1.0)))
+        return h, mv
+
+
 # ---------------------------------------------------------------- HKY data --
 
 def hky_pmatrix(pi, kappa, mt):

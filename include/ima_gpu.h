@@ -135,6 +135,35 @@ int ima_is_site_nodes(ima_ctx *ctx, int slot, int32_t *site_node);
 /* SW per-edge terms dlikeA (calc_prob_data.cpp:879-889) for one item. */
 int ima_sw_dlike(ima_ctx *ctx, const ima_item *item, double *dlikeA);
 
+/* The allele proposal of a genealogy move at a stepwise locus,
+ * finishSWupdateA (update_gtree_common.cpp:2219-2362).  The host moved
+ * `edge` onto `newsisedge` through the freed node `downedge` (old sister
+ * `sisedge`) and wrote the moved genealogy to `slot`; the copy fields are
+ * what storeAinfo (:820-846) saved before the move, uni and flip the draws
+ * of geometric() (utilities.cpp:609-617, by inversion) and bitran(). */
+typedef struct {
+  int32_t slot, edge, downedge, sisedge, newsisedge;
+  int32_t copyA0, copyA1;  /* A of edge and sisedge before the move */
+  int32_t down_was_root;   /* downedge was the root edge before the move */
+  int32_t hold_down_A;     /* A of downedge's old parent (holddownA) */
+  int32_t minA, maxA;      /* the locus' allele range (L[li].minA/maxA) */
+  int32_t flip;            /* bitran(): the step goes down */
+  double copy_dlikeA[3];   /* dlikeA of edge, sisedge, downedge before */
+  double newsis_dlikeA;    /* dlikeA of newsisedge before the move */
+  double u;                /* mutation scalar of the locus */
+  double uni;              /* uniform() draw of geometric(), in (0, 1] */
+} ima_sw_move;
+/* All moves in one launch, one lane each (no two on the same slot).  The
+ * proposed allele goes into the slot (A[downedge]); per move: the
+ * log-likelihood change dlike[i] (0 when calc_likelihood is 0, as with
+ * DONTCALCLIKELIHOODMUTATION), the Hastings term Aterm[i] (0 likewise),
+ * the allele newA[i], and edge_dlikeA[4i..4i+3], the new terms of sisedge,
+ * edge, newsisedge and downedge (0 for the root edge).  A move whose slot
+ * does not hold edge and newsisedge under downedge fails the call with
+ * IMA_ERR_ARG and changes nothing. */
+int ima_sw_update_A(ima_ctx *ctx, const ima_sw_move *moves, int nmoves, int calc_likelihood,
+                    double *dlike, double *Aterm, int32_t *newA, double *edge_dlikeA);
+
 /* ---- reference-shaped single call (plain per-locus FP64 path) --------- */
 /* likelihoodHKY(ci, li, mutrate, kappa, e1, e2, e3, e4) for one genealogy
  * given as host arrays: uploads it, launches one FP64 kernel for this

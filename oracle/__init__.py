@@ -55,6 +55,8 @@ def _load():
     lib.ora_bessi.argtypes = [C.c_int, C.c_double]
     lib.ora_likelihoodSW.restype = C.c_double
     lib.ora_likelihoodSW.argtypes = [C.c_void_p, ip, C.c_double, C.c_double, dp]
+    lib.ora_finishSWupdateA.restype = C.c_double
+    lib.ora_finishSWupdateA.argtypes = [C.c_void_p, ip, C.c_void_p, C.c_int, dp, ip, dp]
     lib.ora_swapweight.restype = C.c_double
     lib.ora_swapweight.argtypes = [C.c_double] * 4
     lib.ora_setheat_geometric.restype = C.c_double
@@ -223,6 +225,31 @@ def likelihood_sw(g, A, u, tr=1.0):
     d = np.zeros(2 * g.n - 1)
     v = lib.ora_likelihoodSW(t.ref, _ip(A), float(u), float(tr), _dp(d))
     return v, d
+
+
+class _SWMove(C.Structure):
+    _fields_ = [(f, C.c_int) for f in ("edge", "downedge", "sisedge", "newsisedge", "copyA0",
+                                        "copyA1", "down_was_root", "hold_down_A", "minA",
+                                        "maxA", "flip")] + \
+               [("copy_dlikeA", C.c_double * 3), ("newsis_dlikeA", C.c_double),
+                ("u", C.c_double), ("uni", C.c_double)]
+
+
+def finish_sw_update_A(g, A, mv, calc=True):
+    """finishSWupdateA on genealogy g (after the move) with allele states A;
+    mv is a sim.SWMove.  Returns (dlike, Aterm, newA, edge dlikeA[4], A
+    with the new allele)."""
+    t = Tree(g)
+    A = np.ascontiguousarray(A, np.int32).copy()
+    m = _SWMove(mv.edge, mv.downedge, mv.sisedge, mv.newsisedge, mv.copyA0, mv.copyA1,
+                int(mv.down_was_root), mv.hold_down_A, mv.minA, mv.maxA, int(mv.flip),
+                (C.c_double * 3)(*mv.copy_dlikeA), mv.newsis_dlikeA, mv.u, mv.uni)
+    at = np.zeros(1)
+    na = np.zeros(1, np.int32)
+    ed = np.zeros(4)
+    v = lib.ora_finishSWupdateA(t.ref, _ip(A), C.byref(m), int(bool(calc)), _dp(at), _ip(na),
+                                _dp(ed))
+    return v, float(at[0]), int(na[0]), ed, A
 
 
 def swapweight(bi, bj, si, sj):

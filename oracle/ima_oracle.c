@@ -401,6 +401,86 @@ double ora_likelihoodSW(const ora_tree *t, const int *A, double u, double tr,
 }
 
 /* =====================================================================
+ * finishSWupdateA -- update_gtree_common.cpp:2219-2362
+ * ===================================================================== */
+
+static double ora_upper_time(const ora_tree *t, int e) {
+  return e >= t->n ? t->time[t->up0[e]] : 0.0; /* :2247-2256 */
+}
+
+double ora_finishSWupdateA(const ora_tree *t, int *A, const ora_sw_move *m, int calc,
+                           double *Aterm, int *newA_out, double *edge_dlikeA) {
+  const int oldA = A[m->downedge];
+  /* holdsisdlikeA (:2232-2235) and the old adjustment (:2240-2242) */
+  const double holdsis = m->newsisedge != m->sisedge ? m->newsis_dlikeA : 0.0;
+  const double oldlikeadj = m->copy_dlikeA[0] + m->copy_dlikeA[1] + m->copy_dlikeA[2] + holdsis;
+  int e[3];
+  double tt[3] = {0.0, 0.0, 0.0};
+  e[0] = m->edge;
+  e[1] = m->newsisedge;
+  e[2] = t->down[e[0]];
+  for (int i = 0; i < 3; i++)
+    if (t->down[e[i]] != -1) tt[i] = t->time[e[i]] - ora_upper_time(t, e[i]);
+  /* forward geometric parameter from the neighbours of the node (:2264-2278) */
+  int wsum = 0, j = 0;
+  for (int i = 0; i < 3; i++) {
+    if (t->down[e[i]] == -1) continue;
+    const int d = (i < 2 ? A[e[i]] : A[t->down[e[i]]]) - oldA;
+    wsum += abs(d);
+    j++;
+  }
+  double gnew = j / (double)(wsum + j);
+  if (gnew > 0.95) gnew = 0.95;
+  int dA = (int)ceil(log(m->uni) / log(1.0 - gnew)) - 1;
+  if (m->flip) dA = -dA;
+  int newA;
+  if (dA >= 0)
+    newA = oldA + dA < m->maxA ? oldA + dA : m->maxA;
+  else
+    newA = oldA + dA > m->minA ? oldA + dA : m->minA;
+  A[m->downedge] = newA;
+  dA = newA - oldA;
+  /* the old sister now runs into the old downedge's parent (:2294-2313) */
+  double sis_dl = 0.0;
+  if (t->down[m->sisedge] != -1 && m->sisedge != m->newsisedge) {
+    const double ts = t->time[m->sisedge] - ora_upper_time(t, m->sisedge);
+    const int d = A[m->sisedge] - A[t->down[m->sisedge]];
+    sis_dl = -ts * m->u + log(ora_bessi(d, ts * m->u));
+  }
+  double likeadj = sis_dl;
+  double dl[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < 3; i++) {
+    if (t->down[e[i]] == -1) continue;
+    const int d = (i < 2 ? A[e[i]] : A[t->down[e[i]]]) - newA;
+    dl[i] = -(tt[i] * m->u) + log(ora_bessi(d, tt[i] * m->u));
+    likeadj += dl[i];
+  }
+  /* reverse geometric parameter from the node's neighbours before the move
+   * (:2327-2343) */
+  wsum = abs(m->copyA0 - newA) + abs(m->copyA1 - newA);
+  j = 2;
+  if (!m->down_was_root) {
+    wsum += abs(m->hold_down_A - newA);
+    j++;
+  }
+  double gold = j / (double)(wsum + j);
+  if (gold > 0.95) gold = 0.95;
+  if (edge_dlikeA) {
+    edge_dlikeA[0] = m->sisedge == m->newsisedge ? dl[1] : sis_dl;
+    edge_dlikeA[1] = dl[0];
+    edge_dlikeA[2] = dl[1];
+    edge_dlikeA[3] = dl[2];
+  }
+  if (newA_out) *newA_out = newA;
+  if (!calc) {
+    *Aterm = 0.0;
+    return 0.0;
+  }
+  *Aterm = (abs(dA) * log(1 - gold) + log(gold)) - (abs(dA) * log(1 - gnew) + log(gnew));
+  return likeadj - oldlikeadj;
+}
+
+/* =====================================================================
  * MC3 swap -- swapchains.cpp:13-34 (swapweight), :72-178 (setheat)
  * ===================================================================== */
 

@@ -84,6 +84,32 @@ double ora_bessi(int n, double x);
 double ora_likelihoodSW(const ora_tree *t, const int *A, double u, double tr,
                         double *dlikeA);
 
+/* ---------------- SW allele of a moved node --------------------------
+ * finishSWupdateA (update_gtree_common.cpp:2219-2362): after a genealogy
+ * move that re-attached `edge` on `newsisedge` through the freed node
+ * `downedge` (old sister `sisedge`), propose the freed node's allele and
+ * score the change.  The copy fields are the reference's copyedge /
+ * holddownA / holdsisdlikeA inputs (storeAinfo, :820-846); uni and flip
+ * are the draws of geometric() (utilities.cpp:609-617, by inversion) and
+ * bitran(). */
+typedef struct {
+  int edge, downedge, sisedge, newsisedge;
+  int copyA0, copyA1;   /* A of edge and sisedge before the move */
+  int down_was_root;    /* downedge was the root edge before the move */
+  int hold_down_A;      /* A of downedge's old parent */
+  int minA, maxA;       /* the locus' allele range */
+  int flip;             /* bitran(): the step goes down */
+  double copy_dlikeA[3];/* dlikeA of edge, sisedge, downedge before the move */
+  double newsis_dlikeA; /* dlikeA of newsisedge before the move */
+  double u, uni;
+} ora_sw_move;
+/* t: the genealogy after the move; A[downedge] receives the new allele.
+ * Returns the log-likelihood change (0 when calc is 0, as with
+ * DONTCALCLIKELIHOODMUTATION); edge_dlikeA[4] = the new dlikeA of sisedge,
+ * edge, newsisedge and downedge (0 for a root edge). */
+double ora_finishSWupdateA(const ora_tree *t, int *A, const ora_sw_move *m, int calc,
+                           double *Aterm, int *newA, double *edge_dlikeA);
+
 /* ---------------- MC3 swap (swapchains.cpp) ---------------- */
 double ora_swapweight(double beta_i, double beta_j, double sum_i, double sum_j);
 double ora_setheat_geometric(int i, int nchains, double hval1, double hval2);
