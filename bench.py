@@ -167,7 +167,13 @@ def lmode_leg(G=100000, npoints=4096, seed=4):
                      "ms_per_query": dt * 1e3}
     out["G"] = G
     out["points"] = npoints
-    out["fp64_roofline_pairs_per_s"] = FP64_PEAK_TFLOPS * 1e12 / 48.0
+    # EXACT per (point, genealogy) pair with nr = 5 parameters in range:
+    # 2 nr FMAs (4 nr flops), one FP64 exp (about 20 flops of range
+    # reduction and polynomial), the running-max compare and the s / s2
+    # updates (4): 44 flops
+    out["flops_per_pair"] = 44
+    out["fp64_roofline_pairs_per_s"] = FP64_PEAK_TFLOPS * 1e12 / 44.0
+    out["exact_frac_of_fp64_peak"] = out["exact"]["pairs_per_s"] / out["fp64_roofline_pairs_per_s"]
     e.close()
     return out
 
