@@ -1365,9 +1365,9 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
     return fail(c, IMA_ERR_SIZE, "a genealogy has %d prior events", emax);
   // per-chain kernels: static swap staging + prior state + a chunk of loci
   const size_t fixed = 2 * (size_t)kSwapLds * 8 + 2048;
-  int pchunk = 64;
+  int pchunk = kSpecDepth;  // loci per speculative group: 2^D - 1 scenarios per parameter
   while (pchunk > 1 && mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
-    pchunk /= 2;
+    --pchunk;
   if (mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "prior state of %d populations exceeds LDS", P);
   grow_logfact(c, lfn);
@@ -1437,6 +1437,33 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
     m.exec[k] = nullptr;
     m.graph[k] = nullptr;
   }
+  return IMA_OK;
+}
+
+int ima_mc3_set_prior_group(ima_ctx *c, int loci) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
+  if (loci < 1 || loci > kSpecDepth)
+    return fail(c, IMA_ERR_ARG, "loci per group must be in 1..%d", kSpecDepth);
+  const size_t fixed = 2 * (size_t)kSwapLds * 8 + 2048;
+  if (mc3_prior_lds_bytes(m.dev.R, m.dev.nparam, loci, m.dev.mints) + fixed > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "%d loci per group exceed LDS", loci);
+  m.dev.pchunk = loci;
+  for (int k = 0; k < 2; ++k) {  // the captured sweeps hold the old group size
+    if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
+    if (m.graph[k]) hipGraphDestroy(m.graph[k]);
+    m.exec[k] = nullptr;
+    m.graph[k] = nullptr;
+  }
+  return IMA_OK;
+}
+
+int ima_mc3_prior_group(ima_ctx *c, int *loci) {
+  if (!c || !loci) return IMA_ERR_ARG;
+  if (!c->mc3.live || c->mc3.dev.prior < 0)
+    return fail(c, IMA_ERR_STATE, "the sweep has no prior");
+  *loci = c->mc3.dev.pchunk;
   return IMA_OK;
 }
 

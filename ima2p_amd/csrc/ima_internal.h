@@ -242,7 +242,7 @@ struct MC3Dev {
   int32_t *tv_row;       // [C*L] chain of each item
   int32_t *tw_slots;     // [C*L] slot of each item
   int32_t *wstatus;      // [C*L] weights status of the proposals
-  int pchunk;            // loci prefetched per chunk by the accept kernel
+  int pchunk;            // loci decided per speculative group by the accept kernel
   int mints;             // length of the model's int block (staged in LDS)
 };
 
@@ -251,16 +251,22 @@ struct MC3Dev {
 // [LC] | dold dnew [LC][cap] | qmax/mmax, log Gamma cache [np], then ints
 // qc nc [np] | ok nch dec [LC] | didx [LC][cap] | aff [LC][aw] | the
 // model's int block [mints] | counts of the log Gamma cache [np].
-constexpr int kPriorCap = 24;  // changed weight entries kept per locus
+// Dynamic LDS of the per-chain prior kernels.  The accept kernel decides
+// the loci of a chain in groups of D (1..kSpecDepth): for every parameter
+// it evaluates, in one wavefront, the integral under each of the 2^D - 1
+// (locus, earlier decisions) scenarios of the group, one lane each.
+constexpr int kSpecDepth = 6;
 struct PriorLds {
-  int R, np, LC, cap, aw, mints;
+  int R, np, LC, W, aw, mints;  // LC = D loci per group, W = 2^D scenario slots
   __host__ __device__ PriorLds(int R_, int np_, int LC_, int mints_)
-      : R(R_), np(np_), LC(LC_), cap(kPriorCap), aw((np_ + 31) / 32), mints(mints_) {}
+      : R(R_), np(np_), LC(LC_), W(1 << LC_), aw((np_ + 31) / 32), mints(mints_) {}
+  // per-locus rows come in three buffers: one group is decided while the
+  // previous one is committed and the next one's records are read
   __host__ __device__ size_t doubles() const {
-    return (size_t)2 * R + 6 * np + 3 * LC + (size_t)2 * LC * cap;
+    return (size_t)R + 5 * np + 9 * LC + (size_t)6 * LC * R + (size_t)2 * np * W;
   }
   __host__ __device__ size_t ints() const {
-    return (size_t)3 * np + (size_t)LC * (3 + cap + aw) + mints;
+    return (size_t)2 * np + (size_t)3 * LC * (2 + R + aw) + (size_t)np * W + mints;
   }
   __host__ __device__ size_t bytes() const { return doubles() * 8 + ints() * 4; }
 };

@@ -260,6 +260,14 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane l's value of x, for every lane (l uniform).
+__device__ inline double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Entry r of the all-loci sums after replacing one genealogy's weights
 // (sum_subtract_treeinfo, ginfo.cpp:249-285): counts add the difference,
 // fc / fm subtract then add and stay >= 0, hcc subtracts then adds.
@@ -274,35 +282,44 @@ __device__ inline double wsum_update(int r, int P, double all, double old, doubl
 }
 
 // Pointers into the dynamic LDS of the per-chain prior kernels (PriorLds).
+//   all[R]           the chain's all-loci sums
+//   qt/qf/qc[np]     committed integral, weight, count of every parameter
+//   nt/nf/nc[np]     the same for the locus being decided
+//   npd/pdg/U/ok/dec per locus of the group
+//   dold/dnew/flag   [D][R] the group's committed and proposed records and
+//                    which entries differ
+//   aff[D][aw]       parameters whose entries a locus changed
+//   sterm/sf/sc      [np][W] per (parameter, scenario) integral, weight, count
 struct PriorSh {
-  double *all, *hold, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *lga;
-  int *qc, *nc, *ok, *nch, *dec, *didx, *mi, *lgc;
+  double *all, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *sterm, *sf;
+  int *qc, *nc, *ok, *dec, *flag, *sc, *mi;
   uint32_t *aff;
 };
 __device__ inline PriorSh prior_carve(double *dyn, const PriorLds &Lz) {
   PriorSh p;
+  const int L2 = 3 * Lz.LC;  // three buffers of per-locus rows
+  const size_t DR = (size_t)L2 * Lz.R, NW = (size_t)Lz.np * Lz.W;
   p.all = dyn;
-  p.hold = p.all + Lz.R;
-  p.qt = p.hold + Lz.R;
+  p.qt = p.all + Lz.R;
   p.qf = p.qt + Lz.np;
   p.nt = p.qf + Lz.np;
   p.nf = p.nt + Lz.np;
-  p.npd = p.nf + Lz.np;
-  p.pdg = p.npd + Lz.LC;
-  p.U = p.pdg + Lz.LC;
-  p.dold = p.U + Lz.LC;
-  p.dnew = p.dold + (size_t)Lz.LC * Lz.cap;
-  p.mdl = p.dnew + (size_t)Lz.LC * Lz.cap;
-  p.lga = p.mdl + Lz.np;
-  p.qc = reinterpret_cast<int *>(p.lga + Lz.np);
+  p.mdl = p.nf + Lz.np;
+  p.npd = p.mdl + Lz.np;
+  p.pdg = p.npd + L2;
+  p.U = p.pdg + L2;
+  p.dold = p.U + L2;
+  p.dnew = p.dold + DR;
+  p.sterm = p.dnew + DR;
+  p.sf = p.sterm + NW;
+  p.qc = reinterpret_cast<int *>(p.sf + NW);
   p.nc = p.qc + Lz.np;
   p.ok = p.nc + Lz.np;
-  p.nch = p.ok + Lz.LC;
-  p.dec = p.nch + Lz.LC;
-  p.didx = p.dec + Lz.LC;
-  p.aff = reinterpret_cast<uint32_t *>(p.didx + (size_t)Lz.LC * Lz.cap);
-  p.mi = reinterpret_cast<int *>(p.aff + (size_t)Lz.LC * Lz.aw);
-  p.lgc = p.mi + Lz.mints;
+  p.dec = p.ok + L2;
+  p.flag = p.dec + L2;
+  p.sc = p.flag + DR;
+  p.aff = reinterpret_cast<uint32_t *>(p.sc + NW);
+  p.mi = reinterpret_cast<int *>(p.aff + (size_t)L2 * Lz.aw);
   return p;
 }
 
@@ -352,46 +369,114 @@ __global__ __launch_bounds__(256) void mc3_prior_init(DeviceView v, MC3Dev m) {
   if (threadIdx.x == 0) m.probg[c] = pg;
 }
 
-// True when parameter t adds up one of the record entries in idx[0..n).
-__device__ inline bool param_touches(const ModelView &md, int P, int npsize, int t,
-                                     const int *idx, int n) {
+// True when parameter t adds up a record entry flagged in flag[0..R).
+__device__ inline bool param_hit(const ModelView &md, int P, int npsize, int t,
+                                 const int *flag) {
   const int P2 = P * P, P3 = P2 * P;
   if (t < npsize) {
     for (int w = md.theta_off[t]; w < md.theta_off[t + 1]; ++w) {
       const int e = md.theta_wp[2 * w] * P + md.theta_wp[2 * w + 1];
-      for (int i = 0; i < n; ++i) {
-        const int r = idx[i];
-        if (r == e || r == P2 + e || r == 2 * P2 + e) return true;
-      }
+      if (flag[e] | flag[P2 + e] | flag[2 * P2 + e]) return true;
     }
   } else {
     const int mm = t - npsize;
     for (int w = md.mig_off[mm]; w < md.mig_off[mm + 1]; ++w) {
       const int e = (md.mig_wp[3 * w] * P + md.mig_wp[3 * w + 1]) * P + md.mig_wp[3 * w + 2];
-      for (int i = 0; i < n; ++i) {
-        const int r = idx[i];
-        if (r == 3 * P2 + e || r == 3 * P2 + P3 + e) return true;
-      }
+      if (flag[3 * P2 + e] | flag[3 * P2 + P3 + e]) return true;
     }
   }
   return false;
 }
 
-// Loci of chain c in order, each accepted with beta (dprobg + dpdg).  The
-// work that does not depend on earlier decisions is done in parallel, a
-// chunk of loci at a time: each wavefront reads one locus' likelihood,
-// acceptance uniform and the weight entries its proposal changed (up to
-// kPriorCap, else the locus takes the full record from memory) and marks
-// the parameters those entries feed.  The first wavefront then decides the
-// chunk's loci in order from LDS only, recomputing just the marked
-// parameters' integrals; commits and restores are applied in parallel.
+// Loci of chain c in order, each accepted with beta (dprobg + dpdg), where
+// a locus' prior term depends on the all-loci sums the earlier decisions
+// left.  The loci are taken in groups of D:
+//  1. one wavefront per locus reads its likelihood, uniform and records,
+//     flags the entries its proposal changed and marks the parameters they
+//     feed (for group g+1 while group g is being decided);
+//  2. one wavefront per parameter evaluates, one lane per scenario, the
+//     parameter's sums and integral for every locus j of the group under
+//     every combination S of the decisions on loci 0..j-1 (2^D - 1 lanes),
+//     applying sum_subtract_treeinfo's rules in locus order;
+//  3. the first wavefront walks the group in order, picking each locus'
+//     terms from the scenario its actual earlier decisions select (the
+//     committed term when count and weight are unchanged,
+//     update_gtree_common.cpp:1997-2006), deciding, and applying the
+//     accepted changes to the sums in locus order;
+//  4. the group's records are committed or restored while the next group
+//     runs step 2.
+// The decisions and sums are those of deciding the loci one at a time.
+
+// Step 1 for the loci [l0, l0 + nl) of chain c into buffer rows b0.., by
+// wavefronts w0.. of the workgroup.
+__device__ inline void prior_fetch(const MC3Dev &m, const ModelView &md, int npsize,
+                                   const PriorSh &sh, const PriorLds &Lz, int c, int l0,
+                                   int nl, int b0, int w0, uint64_t sweep) {
+  const int R = m.R, np = m.nparam, P = m.P;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  for (int x = wave - w0; x < nl; x += nwaves - w0) {
+    if (x < 0) break;
+    const int it = c * m.L + l0 + x, row = b0 + x;
+    const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
+    for (int r = lane; r < R; r += kWave) {
+      const double o = wo[r], w = wn[r];
+      sh.dold[row * R + r] = o;
+      sh.dnew[row * R + r] = w;
+      sh.flag[row * R + r] = o != w ? 1 : 0;
+    }
+    if (lane == 0) {
+      sh.ok[row] = m.status[it] == IMA_STATUS_OK && m.wstatus[it] == IMA_STATUS_OK;
+      sh.npd[row] = new_loglik(m, it);
+      sh.pdg[row] = m.pdg[it];
+      PhiloxStream rr = item_rng(m, it, sweep, 1);
+      sh.U[row] = rr.uniform();
+    }
+    wave_sync();
+    for (int t0 = 0; t0 < np; t0 += kWave) {
+      const int t = t0 + lane;
+      const bool hit = t < np && param_hit(md, P, npsize, t, sh.flag + row * R);
+      const unsigned long long bal = __ballot(hit);
+      if (lane == 0) {
+        sh.aff[row * Lz.aw + t0 / 32] = (uint32_t)bal;
+        if (t0 / 32 + 1 < Lz.aw) sh.aff[row * Lz.aw + t0 / 32 + 1] = (uint32_t)(bal >> 32);
+      }
+    }
+  }
+}
+
+// Step 4 for the loci [l0, l0 + nl) held in buffer rows b0.., by the
+// workgroup's wavefronts from the last one down.
+__device__ inline void prior_commit(const DeviceView &v, const MC3Dev &m, const PriorSh &sh,
+                                    int c, int l0, int nl, int b0) {
+  const int R = m.R;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  for (int x = nwaves - 1 - wave; x < nl; x += nwaves) {
+    const int it = c * m.L + l0 + x, row = b0 + x;
+    const bool acc = sh.dec[row] != 0;
+    if (acc) {
+      double *wc = m.wrec + (int64_t)it * R;
+      for (int r = lane; r < R; r += kWave)
+        if (sh.flag[row * R + r]) wc[r] = sh.dnew[row * R + r];
+    }
+    if (lane == 0) {
+      if (acc)
+        m.pdg[it] = sh.npd[row];
+      else
+        restore_item(v, m, it);
+      m.acc[it] += acc ? 1u : 0u;
+    }
+  }
+}
+
 __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
                              double *dyn) {
   const int R = m.R, np = m.nparam, P = m.P;
   const PriorLds Lz(R, np, m.pchunk, m.mints);
+  const int D = Lz.LC, W = Lz.W;
   const PriorSh sh = prior_carve(dyn, Lz);
-  // the model's maps and each parameter's log Gamma live in LDS, so the
-  // in-order loop below reads no global memory
+  // the model's maps live in LDS, so the decision loop reads no global memory
   ModelMeta mm = v.models[m.prior];
   for (int x = threadIdx.x; x < m.mints; x += blockDim.x) sh.mi[x] = v.mint[mm.ioff + x];
   for (int t = threadIdx.x; t < np; t += blockDim.x) sh.mdl[t] = v.mdbl[mm.doff + t];
@@ -403,165 +488,170 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
     sh.qt[t] = m.qterm[c * np + t];
     sh.qf[t] = m.qf[c * np + t];
-    const int cq = m.qc[c * np + t];
-    sh.qc[t] = cq;
-    sh.lgc[t] = cq;
-    sh.lga[t] = prior_param_lga(mm.npsize, t, cq, v.logfact);
+    sh.qc[t] = m.qc[c * np + t];
   }
   __syncthreads();
   const ModelView md = model_view(sh.mi, sh.mdl, mm);
   double probg = m.probg[c];
   const double beta = m.beta[c];
-  for (int l0 = 0; l0 < m.L; l0 += Lz.LC) {
-    const int nl = min(Lz.LC, m.L - l0);
-    // ---- parallel: one wavefront per locus of the chunk
-    for (int x = wave; x < nl; x += nwaves) {
-      const int it = c * m.L + l0 + x;
-      const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
-      int count = 0;
-      for (int r0 = 0; r0 < R; r0 += kWave) {
-        const int r = r0 + lane;
-        double o = 0.0, w = 0.0;
-        if (r < R) {
-          o = wo[r];
-          w = wn[r];
-        }
-        const bool diff = r < R && o != w;
-        const unsigned long long bal = __ballot(diff);
-        const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
-        if (diff && pos < Lz.cap) {
-          sh.didx[x * Lz.cap + pos] = r;
-          sh.dold[x * Lz.cap + pos] = o;
-          sh.dnew[x * Lz.cap + pos] = w;
-        }
-        count += __popcll(bal);
-      }
-      if (lane == 0) {
-        sh.ok[x] = m.status[it] == IMA_STATUS_OK && m.wstatus[it] == IMA_STATUS_OK;
-        sh.npd[x] = new_loglik(m, it);
-        sh.pdg[x] = m.pdg[it];
-        PhiloxStream rr = item_rng(m, it, sweep, 1);
-        sh.U[x] = rr.uniform();
-        sh.nch[x] = count;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const bool dense = count > Lz.cap;
-      for (int t0 = 0; t0 < np; t0 += kWave) {
-        const int t = t0 + lane;
-        const bool hit = t < np && (dense || param_touches(md, P, mm.npsize, t,
-                                                          sh.didx + x * Lz.cap, count));
-        const unsigned long long bal = __ballot(hit);
-        if (lane == 0) {
-          sh.aff[x * Lz.aw + t0 / 32] = (uint32_t)bal;
-          if (t0 / 32 + 1 < Lz.aw) sh.aff[x * Lz.aw + t0 / 32 + 1] = (uint32_t)(bal >> 32);
+  prior_fetch(m, md, mm.npsize, sh, Lz, c, 0, min(D, m.L), 0, 0, sweep);
+  __syncthreads();
+  // wavefronts left over by the parameters fetch the next group during
+  // step 2; with none left, the others fetch it while wave 0 decides
+  const bool fetch_early = np < nwaves;
+  // wave 0 keeps each parameter's committed (term, weight, count) in lane
+  // t's registers (np <= 64) and sums the terms across lanes
+  const bool regs = np <= kWave;
+  double rqt = 0.0, rqf = 0.0;
+  int rqc = 0;
+  if (wave == 0 && regs && lane < np) {
+    rqt = sh.qt[lane];
+    rqf = sh.qf[lane];
+    rqc = sh.qc[lane];
+  }
+  int pl0 = 0, pnl = 0, pb0 = 0;  // the previous group, still to commit
+  for (int l0 = 0, g = 0; l0 < m.L; l0 += D, ++g) {
+    const int nl = min(D, m.L - l0), b0 = (g % 3) * D, nb0 = ((g + 1) % 3) * D;
+    const int nnl = l0 + D < m.L ? min(D, m.L - l0 - D) : 0;
+    // 2. one wavefront per parameter, one lane per (locus j, decisions S)
+    for (int t = wave; t < np; t += nwaves) {
+      const int s = lane;
+      if (s < W - 1) {
+        const int j = 31 - __clz(s + 1);
+        const int S = s + 1 - (1 << j);
+        if (j < nl && sh.ok[b0 + j] &&
+            ((sh.aff[(b0 + j) * Lz.aw + t / 32] >> (t % 32)) & 1u)) {
+          auto get = [&](int e) {
+            double val = sh.all[e];
+            for (int i = 0; i <= j; ++i) {
+              const int row = b0 + i;
+              if ((i == j || ((S >> i) & 1)) && sh.flag[row * R + e])
+                val = wsum_update(e, P, val, sh.dold[row * R + e], sh.dnew[row * R + e]);
+            }
+            return val;
+          };
+          int cc;
+          double f, h;
+          prior_param_sums_of(md, P, mm.npsize, t, get, &cc, &f, &h);
+          sh.sc[t * W + s] = cc;
+          sh.sf[t * W + s] = f;
+          sh.sterm[t * W + s] = prior_param_term(md, mm.npsize, t, cc, f, h, v.logfact);
         }
       }
     }
+    // 4. for the previous group; 1. for the next one
+    prior_commit(v, m, sh, c, pl0, pnl, pb0);
+    if (fetch_early && nnl > 0)
+      prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, np, sweep);
     __syncthreads();
-    // ---- in order: wave 0 updates the sums and decides; the marked
-    // parameters' integrals are spread one per wavefront, so their
-    // different series run side by side rather than diverging in one wave
-    double hv = 0.0;
-    for (int x = 0; x < nl; ++x) {
-      const int it = c * m.L + l0 + x;
-      const bool ok = sh.ok[x] != 0;
-      const int cnt = sh.nch[x];
-      const bool dense = cnt > Lz.cap;
-      if (wave == 0 && ok) {
-        if (!dense) {
-          if (lane < cnt) {
-            const int r = sh.didx[x * Lz.cap + lane];
-            hv = sh.all[r];
-            sh.all[r] = wsum_update(r, P, hv, sh.dold[x * Lz.cap + lane],
-                                    sh.dnew[x * Lz.cap + lane]);
+    if (wave == 0) {
+      // 3. decisions in order, then the accepted changes into the sums
+      int Smask = 0;
+      if (regs) {
+        uint32_t affm = 0;  // bit j: this lane's parameter is marked by locus j
+        double xU = 0.0, xd = 0.0;  // lane j: U and the likelihood change of locus j
+        int okm = 0;
+        for (int j = 0; j < nl; ++j) {
+          const int row = b0 + j;
+          if (lane < np) affm |= ((sh.aff[row * Lz.aw + lane / 32] >> (lane % 32)) & 1u) << j;
+          okm |= (sh.ok[row] ? 1 : 0) << j;
+        }
+        if (lane < nl) {
+          xU = sh.U[b0 + lane];
+          xd = sh.npd[b0 + lane] - sh.pdg[b0 + lane];
+        }
+        for (int j = 0; j < nl; ++j) {
+          if (!((okm >> j) & 1)) continue;
+          const int s = (1 << j) - 1 + Smask;
+          double term = rqt, f = rqf;
+          int cc = rqc;
+          if ((affm >> j) & 1u) {
+            const int c2 = sh.sc[lane * W + s];
+            const double f2 = sh.sf[lane * W + s];
+            if (!(c2 == cc && f2 == f)) term = sh.sterm[lane * W + s];
+            cc = c2;
+            f = f2;
           }
-        } else {
-          const double *wo = m.wrec + (int64_t)it * R, *wn = m.wnew + (int64_t)it * R;
-          for (int r = lane; r < R; r += kWave) {
-            const double a = sh.all[r];
-            sh.hold[r] = a;
-            sh.all[r] = wsum_update(r, P, a, wo[r], wn[r]);
+          double q = 0.0, mg = 0.0;
+          for (int t = 0; t < mm.npsize; ++t) q += readlane_d(term, t);
+          for (int t = mm.npsize; t < np; ++t) mg += readlane_d(term, t);
+          const double pn = q + mg;
+          const bool acc =
+              readlane_d(xU, j) < fmin(1.0, exp(beta * ((pn - probg) + readlane_d(xd, j))));
+          if (acc) {
+            probg = pn;
+            Smask |= 1 << j;
+            rqt = term;
+            rqf = f;
+            rqc = cc;
           }
         }
-      }
-      __syncthreads();
-      if (ok) {
-        for (int t = wave; t < np; t += nwaves) {
-          if ((sh.aff[x * Lz.aw + t / 32] >> (t % 32)) & 1u) {
-            int cc;
-            double f, h;
-            prior_param_sums(md, P, mm.npsize, t, sh.all, &cc, &f, &h);
-            double term = sh.qt[t];
-            if (!(cc == sh.qc[t] && f == sh.qf[t])) {
-              const double lga = cc == sh.lgc[t] ? sh.lga[t]
-                                                  : prior_param_lga(mm.npsize, t, cc, v.logfact);
-              term = prior_param_term_lga(md, mm.npsize, t, cc, f, h, lga);
-            }
-            if (lane == 0) {
-              sh.nc[t] = cc;
-              sh.nf[t] = f;
-              sh.nt[t] = term;
-            }
-          } else if (lane == 0) {
-            sh.nc[t] = sh.qc[t];
-            sh.nf[t] = sh.qf[t];
-            sh.nt[t] = sh.qt[t];
+        if (lane < nl) sh.dec[b0 + lane] = (Smask >> lane) & 1;
+      } else {
+        for (int j = 0; j < nl; ++j) {
+          const int row = b0 + j;
+          if (!sh.ok[row]) {
+            if (lane == 0) sh.dec[row] = 0;
+            continue;
           }
-        }
-      }
-      __syncthreads();
-      if (wave == 0) {
-        if (!ok) {
-          if (lane == 0) sh.dec[x] = 0;
-        } else {
+          const int s = (1 << j) - 1 + Smask;
+          for (int t = lane; t < np; t += kWave) {
+            double term = sh.qt[t], f = sh.qf[t];
+            int cc = sh.qc[t];
+            if ((sh.aff[row * Lz.aw + t / 32] >> (t % 32)) & 1u) {
+              const int c2 = sh.sc[t * W + s];
+              const double f2 = sh.sf[t * W + s];
+              if (!(c2 == cc && f2 == f)) term = sh.sterm[t * W + s];
+              cc = c2;
+              f = f2;
+            }
+            sh.nt[t] = term;
+            sh.nc[t] = cc;
+            sh.nf[t] = f;
+          }
+          wave_sync();
           double q = 0.0, mg = 0.0;
           for (int t = 0; t < mm.npsize; ++t) q += sh.nt[t];
           for (int t = mm.npsize; t < np; ++t) mg += sh.nt[t];
           const double pn = q + mg;
           const bool acc =
-              sh.U[x] < fmin(1.0, exp(beta * ((pn - probg) + (sh.npd[x] - sh.pdg[x]))));
+              sh.U[row] < fmin(1.0, exp(beta * ((pn - probg) + (sh.npd[row] - sh.pdg[row]))));
           if (acc) {
             probg = pn;
+            Smask |= 1 << j;
             for (int t = lane; t < np; t += kWave) {
               sh.qt[t] = sh.nt[t];
               sh.qf[t] = sh.nf[t];
               sh.qc[t] = sh.nc[t];
             }
-          } else if (!dense) {
-            if (lane < cnt) sh.all[sh.didx[x * Lz.cap + lane]] = hv;
-          } else {
-            for (int r = lane; r < R; r += kWave) sh.all[r] = sh.hold[r];
           }
-          if (lane == 0) sh.dec[x] = acc ? 1 : 0;
+          if (lane == 0) sh.dec[row] = acc ? 1 : 0;
+          wave_sync();
         }
       }
+      for (int r = lane; r < R; r += kWave) {
+        double a = sh.all[r];
+        for (int j = 0; j < nl; ++j)
+          if ((Smask >> j) & 1 && sh.flag[(b0 + j) * R + r])
+            a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
+        sh.all[r] = a;
+      }
+    } else if (!fetch_early && nnl > 0) {
+      // 1. the next group
+      prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, 1, sweep);
     }
     __syncthreads();
-    // ---- parallel: commit or restore the chunk's loci
-    for (int x = wave; x < nl; x += nwaves) {
-      const int it = c * m.L + l0 + x;
-      const bool acc = sh.dec[x] != 0;
-      if (acc) {
-        double *wc = m.wrec + (int64_t)it * R;
-        const double *wn = m.wnew + (int64_t)it * R;
-        if (sh.nch[x] > Lz.cap) {
-          for (int r = lane; r < R; r += kWave) wc[r] = wn[r];
-        } else if (lane < sh.nch[x]) {
-          wc[sh.didx[x * Lz.cap + lane]] = sh.dnew[x * Lz.cap + lane];
-        }
-      }
-      if (lane == 0) {
-        if (acc)
-          m.pdg[it] = sh.npd[x];
-        else
-          restore_item(v, m, it);
-        m.acc[it] += acc ? 1u : 0u;
-      }
-    }
-    __syncthreads();
+    pl0 = l0;
+    pnl = nl;
+    pb0 = b0;
   }
+  prior_commit(v, m, sh, c, pl0, pnl, pb0);
+  if (wave == 0 && regs && lane < np) {
+    sh.qt[lane] = rqt;
+    sh.qf[lane] = rqf;
+    sh.qc[lane] = rqc;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) m.probg[c] = probg;
   for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = sh.all[r];
   for (int t = threadIdx.x; t < np; t += blockDim.x) {

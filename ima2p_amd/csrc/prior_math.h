@@ -253,28 +253,35 @@ __device__ inline double pri_migration_term(int cm, double fm, double max, doubl
 // Count, weight and hcc sums of parameter t (population sizes first, then
 // migration rates) over its (period, position) entries of a weight record,
 // in parameter-map order (integrate_tree_prob's loops).
-__device__ inline void prior_param_sums(const ModelView &md, int P, int npsize, int t,
-                                        const double *rec, int *c, double *f, double *h) {
+// get(r) returns record entry r.
+template <typename Get>
+__device__ inline void prior_param_sums_of(const ModelView &md, int P, int npsize, int t,
+                                           const Get &get, int *c, double *f, double *h) {
   int cc = 0;
   double ff = 0.0, hh = 0.0;
   if (t < npsize) {
     for (int w = md.theta_off[t]; w < md.theta_off[t + 1]; ++w) {
       const int kk = md.theta_wp[2 * w], ii = md.theta_wp[2 * w + 1];
-      cc += (int)rec[kk * P + ii];
-      ff += rec[P * P + kk * P + ii];
-      hh += rec[2 * P * P + kk * P + ii];
+      cc += (int)get(kk * P + ii);
+      ff += get(P * P + kk * P + ii);
+      hh += get(2 * P * P + kk * P + ii);
     }
   } else {
     const int m = t - npsize;
     for (int w = md.mig_off[m]; w < md.mig_off[m + 1]; ++w) {
       const int kk = md.mig_wp[3 * w], ii = md.mig_wp[3 * w + 1], jj = md.mig_wp[3 * w + 2];
-      cc += (int)rec[3 * P * P + (kk * P + ii) * P + jj];
-      ff += rec[3 * P * P + P * P * P + (kk * P + ii) * P + jj];
+      cc += (int)get(3 * P * P + (kk * P + ii) * P + jj);
+      ff += get(3 * P * P + P * P * P + (kk * P + ii) * P + jj);
     }
   }
   *c = cc;
   *f = ff;
   *h = hh;
+}
+
+__device__ inline void prior_param_sums(const ModelView &md, int P, int npsize, int t,
+                                        const double *rec, int *c, double *f, double *h) {
+  prior_param_sums_of(md, P, npsize, t, [rec](int r) { return rec[r]; }, c, f, h);
 }
 
 // log Gamma of the incomplete gamma order parameter t's integral uses at

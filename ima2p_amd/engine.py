@@ -372,6 +372,15 @@ class Engine:
         self._chk(self.lib.ima_mc3_set_prior(self.h, tv.ctypes.data_as(L.dp), float(tmax)),
                   "mc3_set_prior")
 
+    def mc3_prior_group(self, loci=None):
+        """Loci per speculative group of the accept step; sets it when
+        `loci` is given.  Returns the value in force."""
+        if loci is not None:
+            self._chk(self.lib.ima_mc3_set_prior_group(self.h, int(loci)), "mc3_set_prior_group")
+        out = C.c_int()
+        self._chk(self.lib.ima_mc3_prior_group(self.h, C.byref(out)), "mc3_prior_group")
+        return out.value
+
     def mc3_read_prior(self):
         C_, _ = self.mc3_shape
         P = self.mc3_P

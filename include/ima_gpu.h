@@ -294,6 +294,13 @@ int ima_mc3_read(ima_ctx *ctx, double *pdg, double *beta, uint64_t *stats,
  * chain's all-loci weights (update_gtree.cpp:827-961).  Call after
  * ima_mc3_setup. */
 int ima_mc3_set_prior(ima_ctx *ctx, const double *tvals, double tmax);
+/* Loci decided per group by the sweep's accept step (1..6; set_prior picks
+ * the largest that fits LDS).  A group of D loci evaluates each
+ * parameter's integral under all 2^D - 1 (locus, earlier decisions)
+ * scenarios at once; decisions are the same for every D.  No counterpart
+ * in the reference, which decides one locus at a time. */
+int ima_mc3_set_prior_group(ima_ctx *ctx, int loci);
+int ima_mc3_prior_group(ima_ctx *ctx, int *loci);
 /* probg[nchains] and split times tvals[nchains][P-1] of the sweep. */
 int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
 /* Migration times of one slot as the device holds them (the rescaling

@@ -200,3 +200,20 @@ def test_three_gene_copies_interval_ratio():
     print("three copies: KS D", kstest(r.ravel(), cdf).statistic)
     assert _check_law(r, cdf, (0.1, 0.25, 0.5, 0.75, 0.9)) < 4.5
     e.close()
+
+
+def test_decisions_equal_for_every_group_size():
+    """Deciding the loci in speculative groups of 1..6 gives the sweep of
+    deciding them one at a time, bit for bit."""
+    out = []
+    for D in (1, 2, 3, 6):
+        e, *_ = _three_pop_chains(5, 7, 52, L.PREC_FP64)
+        assert e.mc3_prior_group(D) == D
+        e.mc3_run(30, L.MC3_GRAPH | L.MC3_SCALE)
+        pdg, beta, stats, _ = e.mc3_read()
+        probg, tv = e.mc3_read_prior()
+        out.append((pdg, beta, probg, tv, stats))
+        e.close()
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            np.testing.assert_array_equal(a, b)
