@@ -65,6 +65,38 @@ class AnalysisSpec:
     def nloci(self):
         return len(self.loci)
 
+    def load(self, engine, scale_width=0.2, swaptries=None, prior=True):
+        """Set the analysis up on an Engine as the server does (same loci
+        order, slots chain-major, IS sumlogk from chain 0's genealogy, MC3
+        with the prior).  Returns (first slot, locus ids)."""
+        m = self.model
+        mid = engine.model_add(m) if prior else None
+        lids = []
+        for l, (kind, data) in enumerate(self.loci):
+            lid = (engine.add_locus_hky(data) if kind == "hky" else
+                   engine.add_locus_is(data) if kind == "is" else engine.add_locus_sw(data))
+            if prior:
+                engine.locus_set_model(lid, mid, self.samppop[l])
+            lids.append(lid)
+        Cc, Lc = self.nchains, self.nloci
+        first = engine.alloc_slots([lids[i % Lc] for i in range(Cc * Lc)])
+        gens = [g for row in self.chains for g in row]
+        A = [a if a is not None else np.zeros(2 * g.n - 1, np.int32)
+             for row, grow in zip(self.A or [[None] * Lc] * Cc, self.chains)
+             for a, g in zip(row, grow)]
+        engine.write_slots(first, gens, A=A)
+        if prior:
+            engine.write_pops(first, gens)
+        for l, (kind, _) in enumerate(self.loci):
+            if kind == "is":
+                engine.sumlogk_from_slot(lids[l], first + l)
+        engine.mc3_setup(Cc, Lc, first, np.ravel(self.u), np.ravel(self.kappa), self.beta,
+                         seed=self.seed, precision=self.precision, swaptries=swaptries,
+                         scale_width=scale_width)
+        if prior:
+            engine.mc3_set_prior(self.tvals, self.tmax, m.npops)
+        return first, lids
+
     def flatten(self):
         """(ctypes struct, arrays kept alive for the duration of a call)."""
         Lc, Cc, m = self.nloci, self.nchains, self.model

@@ -170,3 +170,56 @@ def serving_analysis(rng, pool: LocusPool, nloci, nchains, nsweeps=100, seed=1,
         tvals=np.tile(pool.tsplit, (nchains, 1)) if P > 1 else np.zeros((nchains, 0)),
         tmax=5.0, seed=seed, precision=L.PREC_FP32 if precision is None else precision,
         nsweeps=nsweeps, A=[As] * nchains)
+
+
+def uneven_mixed(nchains=256, seed=3, hky=150, is_=100, sw=50, copies=(4, 60),
+                 hky_sites=(200, 3000), precision=None):
+    """The uneven batch: a two-population IM model (theta 5/5/5, t = 1,
+    m = 0.5 both ways) with 300 loci, 150 HKY of U{200..3000} bp, 100
+    infinite-sites and 50 stepwise (starting allele 20, walk simulated on
+    every branch), U{4..60} gene copies per locus split across the two
+    populations, seed 3; 256 chains starting on the generating genealogies
+    at the true split time."""
+    from . import _lib as L
+    from .immodel import build
+    from .serve import AnalysisSpec
+    rng = np.random.default_rng(seed)
+    pt, theta, ts, mig = sim.im_model_two_pop()
+    model = build(pt, qmax=20.0, mmax=5.0)
+    kinds = ["hky"] * hky + ["is"] * is_ + ["sw"] * sw
+    loci, base, As, u, samp = [], [], [], [], []
+    for kind in kinds:
+        n = int(rng.integers(copies[0], copies[1] + 1))
+        a = int(rng.integers(1, n))
+        sp = [a, n - a]
+        while True:
+            g = sim.simulate_genealogy(rng, sp, theta, ts, mig, pt)
+            A = None
+            if kind == "hky":
+                uu = float(rng.lognormal(0.0, 0.5))
+                nsites = int(rng.integers(hky_sites[0], hky_sites[1] + 1))
+                data = sim.encode_hky(sim.simulate_hky_sequences(
+                    rng, g, nsites, rng.dirichlet([10] * 4), 4.0, uu))
+            elif kind == "is":
+                uu = float(rng.uniform(0.5, 2.0))
+                data = sim.simulate_is(rng, g, uu)
+                if data.seq.shape[1] == 0:
+                    continue
+            else:
+                uu = float(rng.uniform(0.5, 2.0))
+                A = sim.simulate_sw(rng, g, uu, start=20)
+                data = g.n
+            break
+        loci.append((kind, data))
+        base.append(g)
+        As.append(A)
+        u.append(uu)
+        samp.append(sp)
+    kappa = np.array([4.0 if k == "hky" else 1.0 for k in kinds])
+    return AnalysisSpec(loci=loci, model=model, samppop=samp, chains=[base] * nchains,
+                        u=np.tile(u, (nchains, 1)), kappa=np.tile(kappa, (nchains, 1)),
+                        beta=geometric_betas(nchains), tvals=np.tile(np.asarray(ts, float),
+                                                                     (nchains, 1)),
+                        tmax=5.0, seed=seed,
+                        precision=L.PREC_FP32 if precision is None else precision,
+                        nsweeps=0, A=[As] * nchains)

@@ -1,0 +1,185 @@
+"""Parity at the benchmark configurations' full sizes.
+
+* cfg1 (two populations, 5 infinite-sites loci of 10+10 gene copies, 4
+  chains): every item's log-likelihood and 0/1 nodes against the oracle,
+  then 200 device sweeps with the prior, after which every locus'
+  log-likelihood and every chain's probg equal the oracle's on the
+  genealogies and split times the device holds.
+* cfg2 (three populations, 50 HKY loci x 1000 bp, 45 gene copies, 64
+  chains): all 3200 items in one launch, FP64 within 1e-10 and FP32
+  within 2e-6 of the oracle, every item.
+* cfg3 (the uneven batch: 300 loci of 4..60 gene copies, 150 HKY of
+  200..3000 bp, 100 infinite-sites, 50 stepwise; 256 chains): all 76 800
+  items in one launch per model, a random sample of 800 against the
+  oracle; then sweeps with the prior, after which a sample of chains
+  matches the oracle as for cfg1.
+* cfg4 (L mode, 1e5 stored genealogies): 4096 points of the 64^5 grid in
+  one launch per mode; EXACT and REF_COMPAT at sampled points against the
+  oracle (1e-10), and at every point REF_COMPAT - EXACT within what the
+  10-nat window can drop, 0 <= diff <= log(1 + G e^-10).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ima2p_amd import _lib as L
+from ima2p_amd import sim, workloads
+from ima2p_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _current(e, g, slot, times):
+    """Genealogy g with the node and migration times the device now holds."""
+    h = g.copy()
+    h.time = times.copy()
+    nm = sum(len(x) for x in g.mig)
+    mt = e.read_mig_times(slot, nm)
+    k = 0
+    for edge in range(2 * g.n - 1):
+        lst = sorted(g.mig[edge])
+        h.mig[edge] = [(float(mt[k + j]), q) for j, (_, q) in enumerate(lst)]
+        k += len(lst)
+    return h
+
+
+def _oracle_loglik(kind, data, g, A, u, kappa, sumlogk):
+    if kind == "hky":
+        return oracle.likelihood_hky(g, data, u, kappa)[0]
+    if kind == "is":
+        return oracle.likelihood_is(g, data, u, sumlogk)[0]
+    return oracle.likelihood_sw(g, A, u)[0]
+
+
+def _check_chains(e, spec, first, chains, tol):
+    """pdg of every locus and probg of each chain in `chains` against the
+    oracle on the device's current genealogies and split times."""
+    C, Lc = spec.nchains, spec.nloci
+    pdg, _, _, _ = e.mc3_read()
+    probg, tv = e.mc3_read_prior()
+    sl = {l: oracle.calc_sumlogk(spec.chains[0][l], d)[1]
+          for l, (k, d) in enumerate(spec.loci) if k == "is"}
+    for c in chains:
+        _, _, _, tm, _, A = e.read_slots(first + c * Lc, Lc)
+        off, ws = 0, []
+        for l, (kind, data) in enumerate(spec.loci):
+            g = spec.chains[c][l]
+            ne = 2 * g.n - 1
+            h = _current(e, g, first + c * Lc + l, tm[off:off + ne])
+            ref = _oracle_loglik(kind, data, h, A[off:off + ne], spec.u[c][l], spec.kappa[c][l],
+                                 sl.get(l))
+            off += ne
+            assert rel(pdg[c, l], ref) <= tol, (c, l, kind, pdg[c, l], ref)
+            ws.append(oracle.treeweight(h, spec.model, tv[c], spec.samppop[l]))
+        rp, _, _ = oracle.integrate_tree_prob(ws, spec.model)
+        assert rel(probg[c], rp) <= 1e-9, (c, probg[c], rp)
+
+
+def _cfg1_spec():
+    wl = workloads.two_pop_is()
+    from ima2p_amd import immodel
+    from ima2p_amd.serve import AnalysisSpec
+    model = immodel.build(wl.poptree, qmax=20.0, mmax=5.0)
+    C = wl.nchains
+    return AnalysisSpec(loci=[("is", d) for d in wl.loci], model=model,
+                        samppop=[wl.samppop] * wl.nloci, chains=wl.chains,
+                        u=np.tile(wl.u, (C, 1)), kappa=np.ones((C, wl.nloci)), beta=wl.beta,
+                        tvals=np.tile(wl.tsplit, (C, 1)), tmax=wl.tmax, seed=1,
+                        precision=L.PREC_FP64)
+
+
+def test_cfg1_infinite_sites_sweeps():
+    spec = _cfg1_spec()
+    e = Engine(0)
+    first, lids = spec.load(e)
+    C, Lc = spec.nchains, spec.nloci
+    items = e.items(first + np.arange(C * Lc), np.ravel(spec.u))
+    out, st = e.score(items)
+    for i in range(C * Lc):
+        c, l = divmod(i, Lc)
+        g, d = spec.chains[c][l], spec.loci[l][1]
+        _, sl = oracle.calc_sumlogk(spec.chains[0][l], d)
+        ref, rst, nodes = oracle.likelihood_is(g, d, spec.u[c][l], sl)
+        assert st[i] == 0 and rst == 0 and rel(out[i], ref) < 1e-12
+        assert (e.is_site_nodes(first + i, d.seq.shape[1]) == nodes).all()
+    e.mc3_run(200, L.MC3_GRAPH | L.MC3_SCALE)
+    _check_chains(e, spec, first, range(C), 1e-10)
+    e.close()
+
+
+def test_cfg2_all_items_both_precisions():
+    wl = workloads.three_pop_hky()
+    e = Engine(0)
+    lids = [e.add_locus_hky(loc) for loc in wl.loci]
+    C, Lc = wl.nchains, wl.nloci
+    first = e.alloc_slots([lids[l] for c in range(C) for l in range(Lc)])
+    e.write_slots(first, [g for row in wl.chains for g in row])
+    items = e.items(first + np.arange(C * Lc), np.tile(wl.u, C), wl.kappa)
+    o64, _ = e.score(items, L.PREC_FP64)
+    o32, _ = e.score(items, L.PREC_FP32)
+    worst = 0.0
+    for i in range(C * Lc):
+        c, l = divmod(i, Lc)
+        ref, _ = oracle.likelihood_hky(wl.chains[c][l], wl.loci[l], wl.u[l], wl.kappa)
+        assert rel(o64[i], ref) < 1e-10, (i, o64[i], ref)
+        worst = max(worst, rel(o32[i], ref))
+    print("cfg2 FP32 worst relative error", worst)
+    assert worst < 2e-6
+    e.close()
+
+
+def test_cfg3_uneven_batch():
+    spec = workloads.uneven_mixed(precision=L.PREC_FP64)
+    e = Engine(0)
+    first, lids = spec.load(e)
+    C, Lc = spec.nchains, spec.nloci
+    N = C * Lc
+    items = e.items(first + np.arange(N), np.ravel(spec.u), np.ravel(spec.kappa))
+    o64, s64 = e.score(items, L.PREC_FP64)
+    o32, s32 = e.score(items, L.PREC_FP32)
+    rng = np.random.default_rng(33)
+    sl = {l: oracle.calc_sumlogk(spec.chains[0][l], d)[1]
+          for l, (k, d) in enumerate(spec.loci) if k == "is"}
+    worst = 0.0
+    for i in rng.choice(N, 800, replace=False):
+        c, l = divmod(int(i), Lc)
+        kind, data = spec.loci[l]
+        ref = _oracle_loglik(kind, data, spec.chains[c][l], spec.A[c][l], spec.u[c][l],
+                             spec.kappa[c][l], sl.get(l))
+        assert s64[i] == 0 and rel(o64[i], ref) < 1e-10, (i, kind, o64[i], ref)
+        if kind == "hky":
+            worst = max(worst, rel(o32[i], ref))
+        else:  # IS and SW are FP64 in both precisions
+            assert o32[i] == o64[i]
+    print("cfg3 FP32 worst relative error", worst)
+    assert worst < 2e-6
+    e.mc3_run(6, L.MC3_GRAPH | L.MC3_SCALE)
+    pdg, beta, stats, sweeps = e.mc3_read()
+    assert sweeps == 6 and stats[1] > 0
+    _check_chains(e, spec, first, [0, 97, 255], 1e-9)
+    e.close()
+
+
+def test_cfg4_lmode_grid_slice():
+    rng = np.random.default_rng(4)
+    G = 100000
+    tab = sim.lmode_table(rng, G)
+    grid = sim.lmode_grid([0.5, 0.5, 0.5, 0.01, 0.01], [15.0, 15.0, 15.0, 3.0, 3.0], 64)
+    x = grid[rng.choice(len(grid), 4096, replace=False)]
+    e = Engine(0)
+    tid = e.lmode_table(tab, 3, 2, 2)
+    ex, _ = e.jointp(tid, x, mode=L.LMODE_EXACT)
+    rc, _ = e.jointp(tid, x, mode=L.LMODE_REF_COMPAT)
+    for i in rng.choice(4096, 12, replace=False):
+        assert rel(ex[i], oracle.jointp_exact(tab, 3, 2, 2, x[i])) < 1e-10
+        assert rel(rc[i], oracle.jointp(tab, 3, 2, 2, x[i])) < 1e-10
+    # out = log G - log(sum): the window keeps a subset of the terms, so
+    # REF_COMPAT >= EXACT, by at most log(1 + G e^-10) (every dropped term
+    # is below e^-10 of the maximum)
+    diff = rc - ex
+    assert (diff >= -1e-9).all() and (diff <= np.log1p(G * np.exp(-10.0)) + 1e-9).all()
+    e.close()

@@ -27,26 +27,7 @@ pytestmark = pytest.mark.gpu
 def _direct(spec):
     """The analysis run straight through the engine, all sweeps at once."""
     e = Engine(0)
-    mid = e.model_add(spec.model)
-    lids = []
-    for l, (kind, data) in enumerate(spec.loci):
-        lid = (e.add_locus_hky(data) if kind == "hky" else
-               e.add_locus_is(data) if kind == "is" else e.add_locus_sw(data))
-        e.locus_set_model(lid, mid, spec.samppop[l])
-        lids.append(lid)
-    C, Lc = spec.nchains, spec.nloci
-    first = e.alloc_slots([lids[i % Lc] for i in range(C * Lc)])
-    gens = [g for row in spec.chains for g in row]
-    A = [a if a is not None else np.zeros(2 * g.n - 1, np.int32)
-         for row, grow in zip(spec.A, spec.chains) for a, g in zip(row, grow)]
-    e.write_slots(first, gens, A=A)
-    e.write_pops(first, gens)
-    for l, (kind, _) in enumerate(spec.loci):
-        if kind == "is":
-            e.sumlogk_from_slot(lids[l], first + l)
-    e.mc3_setup(C, Lc, first, np.ravel(spec.u), np.ravel(spec.kappa), spec.beta,
-                seed=spec.seed, precision=spec.precision, scale_width=0.2)
-    e.mc3_set_prior(spec.tvals, spec.tmax, spec.model.npops)
+    spec.load(e)
     e.mc3_run(spec.nsweeps, L.MC3_GRAPH | L.MC3_SCALE)
     pdg, beta, stats, _ = e.mc3_read()
     probg, _ = e.mc3_read_prior()
