@@ -318,6 +318,8 @@ def main():
     from ima2p_amd import _lib as L
     e, loci, first, model = load_engine(wl, L.PREC_FP32)
     setup_s = time.perf_counter() - t_setup
+    # autotune the accept step's group size (the chains are left as they were)
+    group, group_ms = e.mc3_tune(10)
     e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE)
     t0 = time.perf_counter()
     e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE)
@@ -392,6 +394,8 @@ def main():
                    "rescale": float(stats[3]) / max(1, stats[2]),
                    "swap": float(stats[5]) / max(1, stats[4])},
         "eager_sweep_ms": sweep_ms / max(1, sweep_n),
+        "tuned": {"accept_group_loci": group,
+                  "ms_per_sweep_by_group": [round(float(x), 4) for x in group_ms]},
         "setup_s": setup_s,
     }
     line.update(extra)

@@ -1459,6 +1459,118 @@ int ima_mc3_set_prior_group(ima_ctx *c, int loci) {
   return IMA_OK;
 }
 
+// The sweep's persistent device state (what one run leaves for the next),
+// copied to and from one device buffer on the context's stream.
+namespace {
+struct Region {
+  void *p;
+  size_t bytes;
+};
+std::vector<Region> mc3_state_regions(ima_ctx *c) {
+  auto &m = c->mc3;
+  const MC3Dev &d = m.dev;
+  const size_t N = (size_t)m.C * m.L;
+  std::vector<Region> r;
+  auto add = [&](void *p, size_t bytes) {
+    if (p && bytes) r.push_back({p, bytes});
+  };
+  add(c->d_time.p + d.node_first, (size_t)d.nnodes * 8);
+  add(c->d_mig_time.p, c->migs * 8);
+  add(m.pdg.p, N * 8);
+  add(m.beta.p, (size_t)m.C * 8);
+  add(m.counter.p, 8);
+  add(m.stats.p, 8 * 8);
+  add(m.acc.p, N * 4);
+  add(m.cacc.p, (size_t)m.C * 4);
+  if (d.prior >= 0) {
+    const size_t np = (size_t)std::max(d.nparam, 1);
+    add(m.tv.p, (size_t)m.C * mc3_tw(d.P) * 8);
+    add(m.wrec.p, N * d.R * 8);
+    add(m.wall.p, (size_t)m.C * d.R * 8);
+    add(m.qterm.p, (size_t)m.C * np * 8);
+    add(m.qf.p, (size_t)m.C * np * 8);
+    add(m.qc.p, (size_t)m.C * np * 4);
+    add(m.probg.p, (size_t)m.C * 8);
+  }
+  return r;
+}
+int mc3_state_copy(ima_ctx *c, char *buf, bool save) {
+  size_t off = 0;
+  for (const Region &g : mc3_state_regions(c)) {
+    HIPCHK(c, hipMemcpyAsync(save ? buf + off : g.p, save ? g.p : buf + off, g.bytes,
+                             hipMemcpyDeviceToDevice, c->stream));
+    off += (g.bytes + 255) & ~(size_t)255;
+  }
+  return IMA_OK;
+}
+size_t mc3_state_bytes(ima_ctx *c) {
+  size_t off = 0;
+  for (const Region &g : mc3_state_regions(c)) off += (g.bytes + 255) & ~(size_t)255;
+  return off;
+}
+}  // namespace
+
+int ima_mc3_tune(ima_ctx *c, int sweeps, int *best, double *ms_per_sweep) {
+  if (!c || sweeps < 1) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  hipSetDevice(c->device);
+  for (int k = 0; k < kSpecDepth; ++k)
+    if (ms_per_sweep) ms_per_sweep[k] = -1.0;
+  if (m.dev.prior < 0) {  // nothing to tune: the accept step has no groups
+    if (best) *best = 0;
+    return IMA_OK;
+  }
+  const size_t bytes = mc3_state_bytes(c);
+  char *buf = nullptr;
+  HIPCHK(c, hipMalloc(&buf, bytes));
+  const uint64_t hs = m.host_sweeps;
+  const int64_t ss = m.scale_sweeps;
+  const int keep = m.dev.pchunk;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = mc3_state_copy(c, buf, true);
+  if (rc == IMA_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(c, IMA_ERR_HIP, "event creation failed");
+  int pick = keep;
+  float pick_ms = 0.0f;
+  for (int D = 1; D <= kSpecDepth && rc == IMA_OK; ++D) {
+    if (ima_mc3_set_prior_group(c, D) != IMA_OK) break;  // larger groups do not fit LDS
+    float ms = 0.0f;
+    for (int pass = 0; pass < 2 && rc == IMA_OK; ++pass) {  // capture + warm, then timed
+      rc = mc3_state_copy(c, buf, false);
+      m.host_sweeps = hs;
+      m.scale_sweeps = ss;
+      const int n = pass == 0 ? 2 : sweeps;
+      if (rc == IMA_OK && pass == 1 && hipEventRecord(e0, c->stream) != hipSuccess)
+        rc = fail(c, IMA_ERR_HIP, "event record failed");
+      if (rc == IMA_OK) rc = mc3_run_async(c, n, IMA_MC3_GRAPH | IMA_MC3_SCALE);
+      if (rc == IMA_OK && pass == 1) {
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+          rc = fail(c, IMA_ERR_HIP, "timing failed");
+      }
+    }
+    if (rc != IMA_OK) break;
+    if (ms_per_sweep) ms_per_sweep[D - 1] = ms / sweeps;
+    if (D == 1 || ms < pick_ms) {
+      pick = D;
+      pick_ms = ms;
+    }
+  }
+  // the chains go back to where they were, with the fastest group size
+  if (rc == IMA_OK) rc = mc3_state_copy(c, buf, false);
+  m.host_sweeps = hs;
+  m.scale_sweeps = ss;
+  if (rc == IMA_OK) rc = ima_mc3_set_prior_group(c, pick);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == IMA_OK)
+    rc = fail(c, IMA_ERR_HIP, "synchronize failed");
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(buf);
+  if (rc == IMA_OK && best) *best = pick;
+  return rc;
+}
+
 int ima_mc3_prior_group(ima_ctx *c, int *loci) {
   if (!c || !loci) return IMA_ERR_ARG;
   if (!c->mc3.live || c->mc3.dev.prior < 0)

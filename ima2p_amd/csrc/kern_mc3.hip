@@ -545,7 +545,7 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
       prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, np, sweep);
     __syncthreads();
     if (wave == 0) {
-      // 3. decisions in order, then the accepted changes into the sums
+      // 3. decisions in order
       int Smask = 0;
       if (regs) {
         uint32_t affm = 0;  // bit j: this lane's parameter is marked by locus j
@@ -629,16 +629,18 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           wave_sync();
         }
       }
-      for (int r = lane; r < R; r += kWave) {
-        double a = sh.all[r];
-        for (int j = 0; j < nl; ++j)
-          if ((Smask >> j) & 1 && sh.flag[(b0 + j) * R + r])
-            a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
-        sh.all[r] = a;
-      }
     } else if (!fetch_early && nnl > 0) {
       // 1. the next group
       prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, 1, sweep);
+    }
+    __syncthreads();
+    // the accepted changes into the sums, in locus order per entry
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+      double a = sh.all[r];
+      for (int j = 0; j < nl; ++j)
+        if (sh.dec[b0 + j] && sh.flag[(b0 + j) * R + r])
+          a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
+      sh.all[r] = a;
     }
     __syncthreads();
     pl0 = l0;

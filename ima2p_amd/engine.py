@@ -381,6 +381,15 @@ class Engine:
         self._chk(self.lib.ima_mc3_prior_group(self.h, C.byref(out)), "mc3_prior_group")
         return out.value
 
+    def mc3_tune(self, sweeps=20):
+        """Autotune the accept step's group size; the chains are left as
+        they were.  Returns (chosen size, ms per sweep for sizes 1..6)."""
+        best = C.c_int()
+        ms = np.zeros(6)
+        self._chk(self.lib.ima_mc3_tune(self.h, int(sweeps), C.byref(best),
+                                        ms.ctypes.data_as(L.dp)), "mc3_tune")
+        return best.value, ms
+
     def mc3_read_prior(self):
         C_, _ = self.mc3_shape
         P = self.mc3_P

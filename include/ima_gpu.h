@@ -301,6 +301,12 @@ int ima_mc3_set_prior(ima_ctx *ctx, const double *tvals, double tmax);
  * in the reference, which decides one locus at a time. */
 int ima_mc3_set_prior_group(ima_ctx *ctx, int loci);
 int ima_mc3_prior_group(ima_ctx *ctx, int *loci);
+/* Autotuner of the sweep: times `sweeps` captured sweeps for every group
+ * size that fits, each from the same saved state, keeps the fastest
+ * (*best; 0 without the prior) and restores the chains, so the sweeps
+ * that follow are those of an untuned run.  ms_per_sweep (may be NULL,
+ * 6 entries) receives each size's time, -1 where it was not tried. */
+int ima_mc3_tune(ima_ctx *ctx, int sweeps, int *best, double *ms_per_sweep);
 /* probg[nchains] and split times tvals[nchains][P-1] of the sweep. */
 int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
 /* Migration times of one slot as the device holds them (the rescaling

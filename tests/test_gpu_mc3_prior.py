@@ -217,3 +217,23 @@ def test_decisions_equal_for_every_group_size():
     for other in out[1:]:
         for a, b in zip(out[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def test_tuner_leaves_the_chains_where_they_were():
+    """ima_mc3_tune times every group size from a saved state and restores
+    it: the sweeps that follow equal those of an untuned context."""
+    out = []
+    for tune in (False, True):
+        e, *_ = _three_pop_chains(6, 9, 61, L.PREC_FP32)
+        e.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE)
+        if tune:
+            best, ms = e.mc3_tune(8)
+            assert 1 <= best <= 6 and ms[best - 1] > 0 and ms[best - 1] == ms[ms > 0].min()
+            assert e.mc3_prior_group() == best
+        e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE)
+        pdg, beta, stats, sweeps = e.mc3_read()
+        probg, tv = e.mc3_read_prior()
+        out.append((pdg, beta, probg, tv, stats, sweeps))
+        e.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
