@@ -7,14 +7,15 @@
 // layout (ginfo.cpp:288-316): cc | fc | hcc (npsize each), mc | fm (nmig
 // each), qintegrate (npsize), mintegrate (nmig), pdg, probg, split times.
 //
-// Mapping: one thread per query point, one workgroup per (64-point tile,
-// chunk of rows).  The workgroup stages its rows in LDS, converted to FP64
-// once, and every lane walks them in order, so each row is read from HBM
-// once per point tile and broadcast from LDS.  A point's per-parameter
-// coefficients sit in registers (fully unrolled parameter loop).  Chunk
-// partials are merged by a second kernel in chunk order.  The chunk size
-// depends on the table only, never on how many points share a launch, so
-// a point's result is the same alone and batched with others.
+// Mapping (jointp): one workgroup of 256 lanes per (tile of points, chunk
+// of rows), each lane carrying one or two points.  The workgroup stages
+// its rows in LDS as compact FP64 records that no longer depend on the
+// point, and every lane walks them in order, so each row is read from HBM
+// once per point tile and broadcast from LDS; a point's coefficients sit
+// in registers.  Chunk partials are merged by a second kernel in chunk
+// order.  The chunk size depends on the table only, never on how many
+// points share a launch, so a point's result is the same alone and batched
+// with others.  margincalc: one thread per value, 64-value tiles.
 #include <float.h>
 
 #include "ima_internal.h"
@@ -23,7 +24,6 @@ namespace ima {
 
 constexpr int kLTile = 64;      // points per workgroup
 constexpr int kLStage = 128;    // rows staged in LDS at a time
-constexpr int kMaxParams = 16;  // npsize + nmig
 
 struct JointLayout {
   int ccp, fcp, hccp, mcp, fmp, qip, mip, pdgp, probgp;
@@ -41,63 +41,6 @@ __host__ __device__ inline JointLayout joint_layout(int npsize, int nmig) {
   L.pdgp = L.mip + nmig;
   L.probgp = L.pdgp + 1;
   return L;
-}
-
-// Per-point coefficients of the parameters in [r0, r1): a1 multiplies the
-// count (cc: log 2 - log x when population sizes are in the model, else 0;
-// mc: log x), a2 the weight (fc: 1/x; fm: x).
-struct PointCoef {
-  double a1[kMaxParams], a2[kMaxParams];
-};
-
-__device__ inline void point_coef(const double *xp, int npsize, int np, int modeltype, int r0,
-                                  int r1, PointCoef &T) {
-#pragma unroll
-  for (int i = 0; i < kMaxParams; ++i) {
-    double a1 = 0.0, a2 = 0.0;
-    if (i < np) {
-      const double x = xp[i];
-      if (i < npsize) {
-        if (modeltype == 0 || modeltype == 1) a1 = kLn2 - log(x);
-        if (i >= r0 && i < r1) a2 = 1.0 / x;
-      } else if (i >= r0 && i < r1) {
-        a1 = log(x);
-        a2 = x;
-      }
-    }
-    T.a1[i] = a1;
-    T.a2[i] = a2;
-  }
-}
-
-// Per-genealogy term of jointp (jointfind.cpp:966-995), same operation order.
-__device__ inline double joint_term(const double *g, const JointLayout &L, int npsize,
-                                    int nmig, int npops, int modeltype, int r0, int r1,
-                                    const PointCoef &T) {
-  double p;
-  if (npops == 2) {
-    p = -g[L.probgp];
-  } else {
-    double s = 0.0;
-    if (modeltype == 1)
-      for (int i = 0; i < npsize; ++i) s += g[L.qip + i];
-    else
-      for (int i = 0; i < nmig; ++i) s += g[L.mip + i];
-    p = -s;
-  }
-#pragma unroll
-  for (int i = 0; i < kMaxParams; ++i) {
-    if (i >= r0 && i < r1) {
-      if (i < npsize) {
-        const double dg = 2.0 * g[L.fcp + i];
-        p += g[L.ccp + i] * T.a1[i] - g[L.hccp + i] - dg * T.a2[i];
-      } else {
-        const int i1 = i - npsize;
-        p += g[L.mcp + i1] * T.a1[i] - g[L.fmp + i1] * T.a2[i];
-      }
-    }
-  }
-  return p;
 }
 
 // eexp (utilities.cpp:1507-1539): e^x as m * 10^z, with a degree-10 Taylor
@@ -125,31 +68,12 @@ __device__ inline void d_eexp(double x, double *m, int *z) {
   }
 }
 
-__device__ inline double pow10i(int k) {
-  // exact powers of ten (10^k is exactly representable for 0 <= k <= 22)
-  double r = 1.0;
-  if (k >= 0) {
-    for (int i = 0; i < k; ++i) r *= 10.0;
-  } else {
-    r = pow(10.0, (double)k);
-  }
-  return r;
-}
-
 struct LArgs {
   const float *rows;
   int G, K, npsize, nmig, npops, modeltype, r0, r1;
   const double *x;  // [npoints][np]
   int npoints, nchunks, chunk_rows, stage;
 };
-
-// Stage rows [g0, g0+cnt) into LDS as FP64 (the float to double conversion
-// is exact, so every lane sees the table's values).
-__device__ inline void stage_rows(const LArgs &a, double *sh, int g0, int cnt) {
-  const int tot = cnt * a.K;
-  const float *src = a.rows + (size_t)g0 * a.K;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) sh[i] = (double)src[i];
-}
 
 // --------------------------------------------------------------- EXACT --
 // Per (point, chunk): running max M and s = sum exp(p - M), s2 = sum
@@ -161,7 +85,7 @@ __device__ inline void stage_rows(const LArgs &a, double *sh, int g0, int cnt) {
 //   p = c0 + sum_k count_k * A_k + weight_k * F_k
 // with A = log 2 - log x (theta, models 0/1; else 0) or log x (migration),
 // F = -2/x (theta: 2 fc / x) or -x (migration); (2/x) fc equals (2 fc)/x
-// exactly, the remaining differences from joint_term's order are the
+// exactly, the remaining differences from the reference's order are the
 // fused multiply-adds and the summation order (about 1e-16 relative).
 // 256 lanes per workgroup share one staged block; each lane carries PTS
 // points, so every record read from LDS serves PTS independent exp chains.
@@ -199,6 +123,37 @@ __device__ inline void stage_compact(const LArgs &a, double *sh, int g0, int cnt
     }
     o[0] = base;
   }
+}
+
+// Per-point coefficients of the compact records: A = log 2 - log x
+// (theta, models 0/1; else 0) or log x (migration), F = -2/x or -x.
+template <int NR>
+__device__ inline void point_coef_compact(const LArgs &a, const double *xp, double (&A)[NR],
+                                          double (&F)[NR]) {
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int i = a.r0 + k;
+    const double x = xp[i];
+    if (i < a.npsize) {
+      A[k] = (a.modeltype == 0 || a.modeltype == 1) ? kLn2 - log(x) : 0.0;
+      F[k] = -2.0 * (1.0 / x);
+    } else {
+      A[k] = log(x);
+      F[k] = -x;
+    }
+  }
+}
+
+template <int NR>
+__device__ inline double compact_term(const double *v, const double (&A)[NR],
+                                      const double (&F)[NR]) {
+  double p = v[0];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    p = fma(v[1 + 2 * k], A[k], p);
+    p = fma(v[2 + 2 * k], F[k], p);
+  }
+  return p;
 }
 
 template <int NR, int PTS>
@@ -311,30 +266,6 @@ __global__ void jointp_exact_final(LArgs a, const double *work, double *out, dou
 // the maximum of the earlier chunks and each lane walks its rows in order;
 // pass 3: merge chunks in order and apply the list cap.
 
-__global__ __launch_bounds__(kLTile) void jointp_ref_pass1(LArgs a, double *cmax) {
-  extern __shared__ double shrows[];
-  const int pt = blockIdx.x * kLTile + threadIdx.x;
-  const int chunk = blockIdx.y;
-  const int np = a.npsize + a.nmig;
-  const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointCoef T;
-  const bool active = pt < a.npoints;
-  point_coef(a.x + (size_t)(active ? pt : 0) * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
-  double M = -DBL_MAX;
-  const int gbeg = chunk * a.chunk_rows;
-  const int gend = min(a.G, gbeg + a.chunk_rows);
-  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
-    const int cnt = min(a.stage, gend - g0);
-    __syncthreads();
-    stage_rows(a, shrows, g0, cnt);
-    __syncthreads();
-    for (int r = 0; r < cnt; ++r)
-      M = fmax(M, joint_term(shrows + (size_t)r * a.K, L, a.npsize, a.nmig, a.npops,
-                             a.modeltype, a.r0, a.r1, T));
-  }
-  if (active) cmax[(size_t)pt * a.nchunks + chunk] = M;
-}
-
 // Per (point, chunk) record of pass 2.
 struct RefPartial {
   double sum, sum2;   // sum of scaled eexp terms (and squares) in the window
@@ -343,62 +274,124 @@ struct RefPartial {
   int32_t outside;    // a kept value lies outside the final window
 };
 
-__global__ __launch_bounds__(kLTile) void jointp_ref_pass2(LArgs a, const double *cmax,
-                                                           RefPartial *part) {
+// REF_COMPAT from the same compact records as EXACT, PTS points per lane.
+// Pass 1: chunk maxima.
+template <int NR, int PTS>
+__global__ __launch_bounds__(kXLanes) void jointp_ref_fast1(LArgs a, double *cmax) {
   extern __shared__ double shrows[];
-  const int pt = blockIdx.x * kLTile + threadIdx.x;
-  const int chunk = blockIdx.y;
   const int np = a.npsize + a.nmig;
-  const JointLayout L = joint_layout(a.npsize, a.nmig);
-  PointCoef T;
-  const bool active = pt < a.npoints;
-  const int ptc = active ? pt : 0;
-  point_coef(a.x + (size_t)ptc * np, a.npsize, np, a.modeltype, a.r0, a.r1, T);
-  const double *cm = cmax + (size_t)ptc * a.nchunks;
-  double Mfin = -DBL_MAX, run = -DBL_MAX;
-  for (int c = 0; c < a.nchunks; ++c) {
-    Mfin = fmax(Mfin, cm[c]);
-    if (c < chunk) run = fmax(run, cm[c]);
+  double A[PTS][NR], F[PTS][NR], M[PTS];
+  int pt[PTS];
+#pragma unroll
+  for (int j = 0; j < PTS; ++j) {
+    pt[j] = blockIdx.x * (kXLanes * PTS) + j * kXLanes + threadIdx.x;
+    point_coef_compact<NR>(a, a.x + (size_t)(pt[j] < a.npoints ? pt[j] : 0) * np, A[j], F[j]);
+    M[j] = -DBL_MAX;
   }
-  double mM;
-  int zM;
-  d_eexp(Mfin, &mM, &zM);
-  const int maxz = zM - 10;
-  RefPartial r{0.0, 0.0, DBL_MAX, 0.0, 0, 0};
+  const int chunk = blockIdx.y;
   const int gbeg = chunk * a.chunk_rows;
   const int gend = min(a.G, gbeg + a.chunk_rows);
+  constexpr int RL = 1 + 2 * NR;
   for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
     const int cnt = min(a.stage, gend - g0);
     __syncthreads();
-    stage_rows(a, shrows, g0, cnt);
+    stage_compact(a, shrows, g0, cnt, NR);
+    __syncthreads();
+    for (int r = 0; r < cnt; ++r) {
+      const double *o = shrows + (size_t)r * RL;
+      double v[RL];
+#pragma unroll
+      for (int q = 0; q < RL; ++q) v[q] = o[q];
+#pragma unroll
+      for (int j = 0; j < PTS; ++j) M[j] = fmax(M[j], compact_term<NR>(v, A[j], F[j]));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PTS; ++j)
+    if (pt[j] < a.npoints) cmax[(size_t)pt[j] * a.nchunks + chunk] = M[j];
+}
+
+// Pass 2: per (point, chunk), the window sums.  A kept value's term is
+// e^p / 10^maxz (the reference's eexp mantissa m 10^(z - maxz)), taken here
+// as exp(p - maxz ln 10): the window, the list cap and the ordering of the
+// running maximum are the reference's, the term itself differs from the
+// degree-10 Taylor series of eexp by at most its truncation, ~4e-10
+// relative, which moves the result by less than 1e-12 relative.
+template <int NR, int PTS>
+__global__ __launch_bounds__(kXLanes) void jointp_ref_fast2(LArgs a, const double *cmax,
+                                                            RefPartial *part) {
+  extern __shared__ double shrows[];
+  const int np = a.npsize + a.nmig;
+  const int chunk = blockIdx.y;
+  double A[PTS][NR], F[PTS][NR], Mfin[PTS], run[PTS], sc[PTS];
+  RefPartial r[PTS];
+  int pt[PTS];
+#pragma unroll
+  for (int j = 0; j < PTS; ++j) {
+    pt[j] = blockIdx.x * (kXLanes * PTS) + j * kXLanes + threadIdx.x;
+    const int ptc = pt[j] < a.npoints ? pt[j] : 0;
+    point_coef_compact<NR>(a, a.x + (size_t)ptc * np, A[j], F[j]);
+    const double *cm = cmax + (size_t)ptc * a.nchunks;
+    double mf = -DBL_MAX, ru = -DBL_MAX;
+    for (int c = 0; c < a.nchunks; ++c) {
+      mf = fmax(mf, cm[c]);
+      if (c < chunk) ru = fmax(ru, cm[c]);
+    }
+    Mfin[j] = mf;
+    run[j] = ru;
+    double mM;
+    int zM;
+    d_eexp(mf, &mM, &zM);
+    sc[j] = (double)(zM - 10) * 2.3025850929940456840;  // maxz ln 10
+    r[j] = RefPartial{0.0, 0.0, DBL_MAX, 0.0, 0, 0};
+  }
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.G, gbeg + a.chunk_rows);
+  constexpr int RL = 1 + 2 * NR;
+  for (int g0 = gbeg; g0 < gend; g0 += a.stage) {
+    const int cnt = min(a.stage, gend - g0);
+    __syncthreads();
+    stage_compact(a, shrows, g0, cnt, NR);
     __syncthreads();
     for (int rr = 0; rr < cnt; ++rr) {
-      const double p = joint_term(shrows + (size_t)rr * a.K, L, a.npsize, a.nmig, a.npops,
-                                  a.modeltype, a.r0, a.r1, T);
+      const double *o = shrows + (size_t)rr * RL;
+      double v[RL];
+#pragma unroll
+      for (int q = 0; q < RL; ++q) v[q] = o[q];
       const int g = g0 + rr;
-      const bool kept = (g == 0) || (run - p < 10);
-      if (kept && p > run) run = p;
-      if (Mfin - p < 10) {
-        double m;
-        int z;
-        d_eexp(p, &m, &z);
-        const int zadj = z - maxz;
-        const double term = (zadj > -308 && zadj < 308)
-                                ? m * pow10i(zadj)
-                                : (zadj <= -308 ? 0.0 : DBL_MAX);
-        r.sum += term;
-        r.sum2 += term * term;
-        r.cnt += 1;
-        if (p < r.minp) {
-          r.minp = p;
-          r.minterm = term;
+#pragma unroll
+      for (int j = 0; j < PTS; ++j) {
+        const double p = compact_term<NR>(v, A[j], F[j]);
+        const bool kept = (g == 0) || (run[j] - p < 10);
+        if (kept && p > run[j]) run[j] = p;
+        if (Mfin[j] - p < 10) {
+          const double term = exp(p - sc[j]);
+          r[j].sum += term;
+          r[j].sum2 += term * term;
+          r[j].cnt += 1;
+          if (p < r[j].minp) {
+            r[j].minp = p;
+            r[j].minterm = term;
+          }
+        } else if (kept) {
+          r[j].outside = 1;
         }
-      } else if (kept) {
-        r.outside = 1;
       }
     }
   }
-  if (active) part[(size_t)pt * a.nchunks + chunk] = r;
+#pragma unroll
+  for (int j = 0; j < PTS; ++j)
+    if (pt[j] < a.npoints) part[(size_t)pt[j] * a.nchunks + chunk] = r[j];
+}
+
+template <int NR>
+hipError_t launch_ref_fast(const LArgs &a, dim3 grid_y, size_t lds, hipStream_t st,
+                           double *cmax, RefPartial *part) {
+  constexpr int PTS = NR <= 8 ? 2 : 1;
+  const dim3 grid((a.npoints + kXLanes * PTS - 1) / (kXLanes * PTS), grid_y.y);
+  hipLaunchKernelGGL((jointp_ref_fast1<NR, PTS>), grid, dim3(kXLanes), lds, st, a, cmax);
+  hipLaunchKernelGGL((jointp_ref_fast2<NR, PTS>), grid, dim3(kXLanes), lds, st, a, cmax, part);
+  return hipSuccess;
 }
 
 __global__ void jointp_ref_final(LArgs a, const double *cmax, const RefPartial *part,
@@ -495,8 +488,20 @@ hipError_t launch_jointp(const LTable &t, const double *x, int npoints, int mode
   } else {
     double *cmax = work;
     RefPartial *part = reinterpret_cast<RefPartial *>(cmax + (size_t)npoints * a.nchunks);
-    hipLaunchKernelGGL(jointp_ref_pass1, grid, dim3(kLTile), lds, st, a, cmax);
-    hipLaunchKernelGGL(jointp_ref_pass2, grid, dim3(kLTile), lds, st, a, cmax, part);
+    const int nr = r1 - r0;
+    const size_t xl = exact_lds_bytes(t, nr);
+    switch (nr) {
+#define IMA_RCASE(N) \
+  case N:            \
+    launch_ref_fast<N>(a, grid, xl, st, cmax, part); \
+    break;
+      IMA_RCASE(1) IMA_RCASE(2) IMA_RCASE(3) IMA_RCASE(4) IMA_RCASE(5) IMA_RCASE(6)
+      IMA_RCASE(7) IMA_RCASE(8) IMA_RCASE(9) IMA_RCASE(10) IMA_RCASE(11) IMA_RCASE(12)
+      IMA_RCASE(13) IMA_RCASE(14) IMA_RCASE(15) IMA_RCASE(16)
+#undef IMA_RCASE
+      default:
+        return hipErrorInvalidValue;
+    }
     hipLaunchKernelGGL(jointp_ref_final, dim3(fin_blocks), dim3(256), 0, st, a, cmax, part,
                        out, ess);
   }
