@@ -75,6 +75,7 @@ PROTOS = {
     "ima_mc3_set_prior_group": (C.c_int, [vp, C.c_int]),
     "ima_mc3_prior_group": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ima_mc3_tune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp]),
+    "ima_device_memory": (C.c_int, [C.c_int, dp, dp]),
     "ima_server_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]),
     "ima_server_destroy": (None, [vp]),
     "ima_server_last_error": (C.c_char_p, [vp]),

@@ -121,6 +121,8 @@ DeviceView view(ima_ctx *c) {
   v.mult = c->d_mult.p;
   v.mask = c->d_mask.p;
   v.pop = c->d_pop.p;
+  v.mig_first = c->d_mig_first.p;
+  v.mig_count = c->d_mig_count.p;
   v.mig_edge = c->d_mig_edge.p;
   v.mig_to = c->d_mig_to.p;
   v.mig_time = c->d_mig_time.p;
@@ -261,6 +263,18 @@ int ima_device_count(int *count) {
   return IMA_OK;
 }
 
+int ima_device_memory(int device, double *free_bytes, double *total_bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IMA_ERR_NODEV;
+  if (device < 0 || device >= n) return IMA_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return IMA_ERR_HIP;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return IMA_ERR_HIP;
+  if (free_bytes) *free_bytes = (double)fr;
+  if (total_bytes) *total_bytes = (double)tot;
+  return IMA_OK;
+}
+
 int ima_ctx_create(int device, ima_ctx **out) {
   if (!out) return IMA_ERR_ARG;
   *out = nullptr;
@@ -317,6 +331,26 @@ void ima_ctx_destroy(ima_ctx *c) {
   release(c->d_S);
   release(c->d_beta);
   release(c->d_pairs);
+  release(c->d_pop);
+  release(c->d_mig_edge);
+  release(c->d_mig_to);
+  release(c->d_mig_first);
+  release(c->d_mig_count);
+  release(c->d_mig_time);
+  release(c->d_models);
+  release(c->d_mint);
+  release(c->d_mdbl);
+  release(c->d_logfact);
+  release(c->d_rec);
+  release(c->d_tv);
+  release(c->d_probg);
+  release(c->d_qint);
+  release(c->d_mintg);
+  release(c->d_pslots);
+  release(c->d_prow);
+  release(c->d_goff);
+  release(c->d_pstatus);
+  release(c->d_swmoves);
   for (auto &t : c->tables)
     if (t.rows) hipFree(t.rows);
   for (int k = 0; k < 8; ++k) {
@@ -437,6 +471,8 @@ int ima_slots_alloc(ima_ctx *c, int count, const int32_t *locus_of, int *first) 
   }
   c->up0.resize(c->nodes, -1);
   c->pop.resize(c->nodes, 0);
+  c->mig_first.resize(c->nodes, 0);
+  c->mig_count.resize(c->nodes, 0);
   c->up1.resize(c->nodes, -1);
   c->down.resize(c->nodes, -1);
   c->A.resize(c->nodes, 0);
@@ -1735,6 +1771,13 @@ int ima_slots_write_pops(ima_ctx *c, int first, int count, const int32_t *pop,
       c->mig_time.resize(c->migs, 0.0);
     }
     s.nmig = nmig[i];
+    std::fill(c->mig_count.begin() + s.node_off, c->mig_count.begin() + s.node_off + ne, 0);
+    std::fill(c->mig_first.begin() + s.node_off, c->mig_first.begin() + s.node_off + ne, 0);
+    for (int j = nmig[i] - 1; j >= 0; --j) {
+      const int e = mig_edge[msrc + j];
+      c->mig_first[s.node_off + e] = j;
+      c->mig_count[s.node_off + e] += 1;
+    }
     if (nmig[i]) {
       std::copy(mig_edge + msrc, mig_edge + msrc + nmig[i], c->mig_edge.begin() + s.mig_off);
       std::copy(mig_to + msrc, mig_to + msrc + nmig[i], c->mig_to.begin() + s.mig_off);
@@ -1749,6 +1792,12 @@ int ima_slots_write_pops(ima_ctx *c, int first, int count, const int32_t *pop,
   HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s0.node_off, c->pop.data() + s0.node_off, nb * 4,
                            hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, reserve(c->d_mig_first, c->nodes, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_count, c->nodes, true, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p + s0.node_off, c->mig_first.data() + s0.node_off,
+                           nb * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p + s0.node_off, c->mig_count.data() + s0.node_off,
+                           nb * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
   HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
   HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
@@ -1959,7 +2008,7 @@ size_t ctx_device_bytes(const ima_ctx *c) {
   add(c->d_up0.n + c->d_up1.n + c->d_down.n + c->d_A.n + c->d_pop.n, 4);
   add(c->d_time.n, 8);
   add(c->d_prog.n, sizeof(ProgOp));
-  add(c->d_mig_edge.n + c->d_mig_to.n, 4);
+  add(c->d_mig_edge.n + c->d_mig_to.n + c->d_mig_first.n + c->d_mig_count.n, 4);
   add(c->d_mig_time.n, 8);
   add(c->d_items.n, sizeof(ima_item));
   add(c->d_out.n + c->d_partial.n + c->d_rec.n + c->d_lwork.n, 8);

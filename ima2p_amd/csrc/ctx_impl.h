@@ -54,10 +54,10 @@ struct ima_ctx {
   bool slots_meta_dirty = true;
   std::vector<int> scratch_slot;  // per locus, for ima_likelihood_hky
   // population labels and migration events of the genealogies (prior)
-  std::vector<int32_t> pop, mig_edge, mig_to;
+  std::vector<int32_t> pop, mig_edge, mig_to, mig_first, mig_count;
   std::vector<double> mig_time;
   size_t migs = 0;
-  DevBuf<int32_t> d_pop, d_mig_edge, d_mig_to;
+  DevBuf<int32_t> d_pop, d_mig_edge, d_mig_to, d_mig_first, d_mig_count;
   DevBuf<double> d_mig_time;
 
   // population models of the genealogy prior

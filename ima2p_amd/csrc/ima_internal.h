@@ -114,6 +114,9 @@ struct DeviceView {
   const int32_t *pop;        // population of each edge at its lower end
   const int32_t *mig_edge, *mig_to;
   const double *mig_time;
+  // per edge (indexed like time[]): its first migration within the slot's
+  // events and how many it has (events are sorted by edge, then time)
+  const int32_t *mig_first, *mig_count;
   const ModelMeta *models;
   const int32_t *mint;
   const double *mdbl;

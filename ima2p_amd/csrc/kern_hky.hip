@@ -117,7 +117,15 @@ __device__ __forceinline__ void to_nxt(const Mid &m, const HkyCoef<real> *coef,
   if (m.type != OP_SS) x.pb = colP[m.b * 4 + m.tb];
 }
 
-// Combine, rescale by a power of two and store one node at stack depth d.
+// Below this a node's partials are rescaled (a power of two: 2^-40 in
+// FP32, 2^-400 in FP64), far above the smallest normal number.
+__device__ __forceinline__ float rescale_floor(float) { return 0x1p-40f; }
+__device__ __forceinline__ double rescale_floor(double) { return 0x1p-400; }
+
+// Combine two operand vectors into a node's partials.  When some lane's
+// largest partial falls below the floor, every lane rescales by an exact
+// power of two (the exponent is carried in E), so the partials' mantissas
+// are those of rescaling at every node.
 template <typename real>
 __device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &vb, int &E) {
   Q4<real> L;
@@ -126,12 +134,14 @@ __device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &
   L.z = va.z * vb.z;
   L.w = va.w * vb.w;
   const real mx = fmax(fmax(L.x, L.y), fmax(L.z, L.w));
-  const int ex = frexp_e(mx);
-  E += ex;
-  L.x = ldexp_r(L.x, -ex);
-  L.y = ldexp_r(L.y, -ex);
-  L.z = ldexp_r(L.z, -ex);
-  L.w = ldexp_r(L.w, -ex);
+  if (__ballot(mx < rescale_floor(mx))) {
+    const int ex = frexp_e(mx);
+    E += ex;
+    L.x = ldexp_r(L.x, -ex);
+    L.y = ldexp_r(L.y, -ex);
+    L.z = ldexp_r(L.z, -ex);
+    L.w = ldexp_r(L.w, -ex);
+  }
   return L;
 }
 

@@ -98,13 +98,14 @@ __device__ void propose_item(const DeviceView &v, const MC3Dev &m, int it, uint6
     while (k < m.P - 1 && tv[k] <= T) k++;
     if (k > 0) lo = fmax(lo, tv[k - 1]);
     if (k < m.P - 1) hi = fmin(hi, tv[k]);
-    const int32_t *me = v.mig_edge + sm.mig_off;
+    // an edge's migrations are in time order: the last ones on the two
+    // daughter edges and the first one on the node's own edge bound it
     const double *mt = v.mig_time + sm.mig_off;
-    for (int j = 0; j < sm.nmig; ++j) {
-      const int x = me[j];
-      if (x == a || x == b) lo = fmax(lo, mt[j]);
-      else if (x == e) hi = fmin(hi, mt[j]);
-    }
+    const int32_t *mf = v.mig_first + sm.node_off, *mc = v.mig_count + sm.node_off;
+    const int ca = mc[a], cb = mc[b], ce = mc[e];
+    if (ca) lo = fmax(lo, mt[mf[a] + ca - 1]);
+    if (cb) lo = fmax(lo, mt[mf[b] + cb - 1]);
+    if (ce) hi = fmin(hi, mt[mf[e]]);
   }
   const bool open = hi == DBL_MAX;
   const double sigma = open ? 0.5 * fmax(lo, 0.05) : 0.5 * (hi - lo);
@@ -332,8 +333,24 @@ __device__ double chain_prior_fresh(const DeviceView &v, const MC3Dev &m, int c,
   const int R = m.R, np = m.nparam;
   const ModelMeta mm = v.models[m.prior];
   const ModelView md = model_view(v, mm);
-  for (int r = threadIdx.x; r < R; r += blockDim.x)
-    sh.all[r] = ordered_rec_sum(w + (int64_t)c * m.L * R, R, m.L, r);
+  // the chain's records go through LDS a block of loci at a time (read
+  // coalesced by the whole workgroup, in the rows of the accept step's
+  // record buffers), each entry summed over the loci in order
+  const int chunk = 6 * m.pchunk;
+  double *stage = sh.dold;  // dold and dnew: 6 * pchunk * R doubles
+  const double *base = w + (int64_t)c * m.L * R;
+  for (int l0 = 0; l0 < m.L; l0 += chunk) {
+    const int nl = min(chunk, m.L - l0);
+    const int cnt = nl * R;
+    __syncthreads();
+    for (int x = threadIdx.x; x < cnt; x += blockDim.x) stage[x] = base[(int64_t)l0 * R + x];
+    __syncthreads();
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+      double acc = l0 == 0 ? 0.0 : sh.all[r];
+      for (int i = 0; i < nl; ++i) acc += stage[i * R + r];
+      sh.all[r] = acc;
+    }
+  }
   __syncthreads();
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int cc;

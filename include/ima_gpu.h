@@ -70,6 +70,8 @@ typedef struct {
 
 /* ---- context ---------------------------------------------------------- */
 int ima_device_count(int *count);
+/* Free and total device memory in bytes (hipMemGetInfo). */
+int ima_device_memory(int device, double *free_bytes, double *total_bytes);
 int ima_ctx_create(int device, ima_ctx **out);
 void ima_ctx_destroy(ima_ctx *ctx);
 const char *ima_ctx_last_error(const ima_ctx *ctx);

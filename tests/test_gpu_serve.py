@@ -154,3 +154,32 @@ def test_admission_by_footprint():
     assert small.status(qid)[0] == REQ_DONE and small.latency(REQ_LMODE)["count"] == 1
     small.close()
     srv.close()
+
+
+def test_contexts_return_their_memory():
+    """Building and destroying analyses' contexts (what the server does per
+    request) leaves the device's free memory where it was."""
+    import ctypes
+
+    lib = L.load()
+
+    def free():
+        f, t = ctypes.c_double(), ctypes.c_double()
+        assert lib.ima_device_memory(0, ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    specs = _analyses()
+
+    def cycle():
+        for s in specs:
+            e = Engine(0)
+            s.load(e)
+            e.mc3_run(3, L.MC3_GRAPH | L.MC3_SCALE)
+            e.close()
+
+    cycle()  # first-use allocations of the runtime
+    free0 = free()
+    for _ in range(10):
+        cycle()
+    free1 = free()
+    assert free0 - free1 < (8 << 20), (free0 - free1)
