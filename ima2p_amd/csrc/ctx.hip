@@ -16,6 +16,7 @@
 using namespace ima;
 
 static void mc3_release(ima_ctx *c);
+static void mc3_drop_graphs(ima_ctx *c);
 
 namespace {
 
@@ -1022,14 +1023,23 @@ int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx,
 
 // ------------------------------------------------------------------- MC3 --
 
-static void mc3_release(ima_ctx *c) {
+// The captured sweeps hold the sweep's configuration; drop them when it
+// changes (they are captured again at the next run).
+static void mc3_drop_graphs(ima_ctx *c) {
   auto &m = c->mc3;
   for (int k = 0; k < 2; ++k) {
     if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
     if (m.graph[k]) hipGraphDestroy(m.graph[k]);
-    m.exec[k] = nullptr;
-    m.graph[k] = nullptr;
+    if (m.bexec[k]) hipGraphExecDestroy(m.bexec[k]);
+    if (m.bgraph[k]) hipGraphDestroy(m.bgraph[k]);
+    m.exec[k] = m.bexec[k] = nullptr;
+    m.graph[k] = m.bgraph[k] = nullptr;
   }
+}
+
+static void mc3_release(ima_ctx *c) {
+  auto &m = c->mc3;
+  mc3_drop_graphs(c);
   for (int k = 0; k < 4; ++k) {
     if (m.ev[k]) {
       hipEventDestroy(m.ev[k]);
@@ -1279,6 +1289,7 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   const bool scale = (flags & IMA_MC3_SCALE) != 0;
   const bool timed = (flags & IMA_MC3_TIMED) != 0;
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  const int bk = scale ? 1 : 0;
   if (use_graph && !timed) {
     for (int k = 0; k < 2; ++k) {
       if (m.exec[k]) continue;
@@ -1291,11 +1302,31 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
       m.graph[k] = g;
       HIPCHK(c, hipGraphInstantiate(&m.exec[k], g, nullptr, nullptr, 0));
     }
+    if (!m.bexec[bk] && nsweeps >= kSweepBlock) {
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = IMA_OK;
+      for (int s = 0; s < kSweepBlock && rc == IMA_OK; ++s)
+        rc = mc3_enqueue(c, scale && (s & 1), nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc != IMA_OK) return rc;
+      HIPCHK(c, e);
+      m.bgraph[bk] = g;
+      HIPCHK(c, hipGraphInstantiate(&m.bexec[bk], g, nullptr, nullptr, 0));
+    }
   }
   const uint64_t done = m.host_sweeps;
   if (nsweeps > 0) HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
   for (int s = 0; s < nsweeps; ++s) {
     const bool with_scale = scale && ((done + s) & 1);
+    if (use_graph && !timed && m.bexec[bk] && ((done + s) & 1) == 0 &&
+        nsweeps - s >= kSweepBlock) {
+      // a block of sweeps in one launch
+      HIPCHK(c, hipGraphLaunch(m.bexec[bk], c->stream));
+      m.scale_sweeps += scale ? kSweepBlock / 2 : 0;
+      s += kSweepBlock - 1;
+      continue;
+    }
     m.scale_sweeps += with_scale ? 1 : 0;
     if (timed) {
       // events bracket the whole sweep and each batched scoring launch
@@ -1467,12 +1498,7 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
                   m.first + i);
     }
   // graphs captured before hold the sweep without the prior
-  for (int k = 0; k < 2; ++k) {
-    if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
-    if (m.graph[k]) hipGraphDestroy(m.graph[k]);
-    m.exec[k] = nullptr;
-    m.graph[k] = nullptr;
-  }
+  mc3_drop_graphs(c);
   return IMA_OK;
 }
 
@@ -1486,12 +1512,7 @@ int ima_mc3_set_prior_group(ima_ctx *c, int loci) {
   if (mc3_prior_lds_bytes(m.dev.R, m.dev.nparam, loci, m.dev.mints) + fixed > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "%d loci per group exceed LDS", loci);
   m.dev.pchunk = loci;
-  for (int k = 0; k < 2; ++k) {  // the captured sweeps hold the old group size
-    if (m.exec[k]) hipGraphExecDestroy(m.exec[k]);
-    if (m.graph[k]) hipGraphDestroy(m.graph[k]);
-    m.exec[k] = nullptr;
-    m.graph[k] = nullptr;
-  }
+  mc3_drop_graphs(c);
   return IMA_OK;
 }
 

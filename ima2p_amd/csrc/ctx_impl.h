@@ -116,8 +116,12 @@ struct ima_ctx {
     DevBuf<double> tv, tvbackup, wrec, wnew, wall, qterm, qf, probg, mtbackup;
     DevBuf<int32_t> qc, tv_row, tw_slots, wstatus;
     HkyPlan plan;
-    hipGraphExec_t exec[2]{nullptr, nullptr};
+    hipGraphExec_t exec[2]{nullptr, nullptr};  // one sweep without / with rescale
     hipGraph_t graph[2]{nullptr, nullptr};
+    // kSweepBlock sweeps in one graph (from an even sweep index), without /
+    // with the alternating rescale
+    hipGraphExec_t bexec[2]{nullptr, nullptr};
+    hipGraph_t bgraph[2]{nullptr, nullptr};
     hipEvent_t ev[4]{};
     hipEvent_t fj[4]{};  // fork / join of the side stream, per scoring pass
   } mc3;

@@ -284,6 +284,7 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
+constexpr int kSweepBlock = 8;  // sweeps captured in one graph
 constexpr int kSwapLds = 1024;  // chains whose S and beta the swap stages in LDS
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 

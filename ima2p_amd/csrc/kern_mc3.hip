@@ -574,7 +574,10 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           okm |= (sh.ok[row] ? 1 : 0) << j;
         }
         if (lane < nl) {
-          xU = sh.U[b0 + lane];
+          // U < min(1, e^x)  <=>  log U < x (and U < 1): the logarithms are
+          // taken here, off the decision chain; +inf stands for U = 1
+          const double U = sh.U[b0 + lane];
+          xU = U < 1.0 ? log(U) : INFINITY;
           xd = sh.npd[b0 + lane] - sh.pdg[b0 + lane];
         }
         for (int j = 0; j < nl; ++j) {
@@ -593,8 +596,9 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           for (int t = 0; t < mm.npsize; ++t) q += readlane_d(term, t);
           for (int t = mm.npsize; t < np; ++t) mg += readlane_d(term, t);
           const double pn = q + mg;
-          const bool acc =
-              readlane_d(xU, j) < fmin(1.0, exp(beta * ((pn - probg) + readlane_d(xd, j))));
+          const double x = beta * ((pn - probg) + readlane_d(xd, j));
+          const double lu = readlane_d(xU, j);
+          const bool acc = x != x ? lu != INFINITY : lu < x;
           if (acc) {
             probg = pn;
             Smask |= 1 << j;
