@@ -447,7 +447,10 @@ __device__ inline void prior_fetch(const MC3Dev &m, const ModelView &md, int nps
       sh.npd[row] = new_loglik(m, it);
       sh.pdg[row] = m.pdg[it];
       PhiloxStream rr = item_rng(m, it, sweep, 1);
-      sh.U[row] = rr.uniform();
+      // the decision U < min(1, e^x) is taken as log U < x (and U < 1);
+      // +inf stands for U = 1
+      const double U = rr.uniform();
+      sh.U[row] = U < 1.0 ? log(U) : INFINITY;
     }
     wave_sync();
     for (int t0 = 0; t0 < np; t0 += kWave) {
@@ -574,10 +577,7 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           okm |= (sh.ok[row] ? 1 : 0) << j;
         }
         if (lane < nl) {
-          // U < min(1, e^x)  <=>  log U < x (and U < 1): the logarithms are
-          // taken here, off the decision chain; +inf stands for U = 1
-          const double U = sh.U[b0 + lane];
-          xU = U < 1.0 ? log(U) : INFINITY;
+          xU = sh.U[b0 + lane];  // log U
           xd = sh.npd[b0 + lane] - sh.pdg[b0 + lane];
         }
         for (int j = 0; j < nl; ++j) {
@@ -635,8 +635,8 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           for (int t = 0; t < mm.npsize; ++t) q += sh.nt[t];
           for (int t = mm.npsize; t < np; ++t) mg += sh.nt[t];
           const double pn = q + mg;
-          const bool acc =
-              sh.U[row] < fmin(1.0, exp(beta * ((pn - probg) + (sh.npd[row] - sh.pdg[row]))));
+          const double x = beta * ((pn - probg) + (sh.npd[row] - sh.pdg[row]));
+          const bool acc = x != x ? sh.U[row] != INFINITY : sh.U[row] < x;
           if (acc) {
             probg = pn;
             Smask |= 1 << j;
