@@ -178,6 +178,54 @@ def lmode_leg(G=100000, npoints=4096, seed=4):
     return out
 
 
+def uneven_leg(sweeps=40):
+    """The uneven batch: 256 chains x 300 loci (150 HKY of 200..3000 bp, 100
+    infinite-sites, 50 stepwise; 4..60 gene copies), all scored in one
+    launch per model each sweep, with the prior; group size autotuned."""
+    from ima2p_amd import _lib as L
+    from ima2p_amd.engine import Engine
+    spec = workloads.uneven_mixed()
+    e = Engine(0)
+    spec.load(e)
+    group, _ = e.mc3_tune(4)
+    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE)
+    t0 = time.perf_counter()
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE)
+    dt = time.perf_counter() - t0
+    e.close()
+    kinds = [k for k, _ in spec.loci]
+    return {"chain_steps_per_s": spec.nchains * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
+            "chains": spec.nchains, "loci": spec.nloci,
+            "loci_by_model": {k: kinds.count(k) for k in ("hky", "is", "sw")},
+            "items_per_sweep": spec.nchains * spec.nloci, "tuned_group_loci": group}
+
+
+def is_leg(sweeps=2000):
+    """The CPU-runnable case (two populations, 5 infinite-sites loci of
+    10+10 gene copies, 4 chains) on the device, FP64, with the prior."""
+    from ima2p_amd import _lib as L
+    from ima2p_amd import immodel
+    from ima2p_amd.engine import Engine
+    from ima2p_amd.serve import AnalysisSpec
+    wl = workloads.two_pop_is()
+    model = immodel.build(wl.poptree, qmax=20.0, mmax=5.0)
+    C = wl.nchains
+    spec = AnalysisSpec(loci=[("is", d) for d in wl.loci], model=model,
+                        samppop=[wl.samppop] * wl.nloci, chains=wl.chains,
+                        u=np.tile(wl.u, (C, 1)), kappa=np.ones((C, wl.nloci)), beta=wl.beta,
+                        tvals=np.tile(wl.tsplit, (C, 1)), tmax=wl.tmax, seed=1,
+                        precision=L.PREC_FP64)
+    e = Engine(0)
+    spec.load(e)
+    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE)
+    t0 = time.perf_counter()
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE)
+    dt = time.perf_counter() - t0
+    e.close()
+    return {"chain_steps_per_s": C * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
+            "workload": wl.name}
+
+
 def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
     """Poisson request stream at `rate`/s for `duration_s` s: 70% L-mode
     query batches of 1..4096 points (log-uniform) on resident tables of
@@ -359,6 +407,8 @@ def main():
         extra["speedup_vs_plain_fp64"] = value / plain
         e64.close()
         extra["lmode"] = lmode_leg()
+        extra["uneven_batch"] = uneven_leg()
+        extra["is_case"] = is_leg()
     if args.serve_seconds > 0:
         extra["serving"] = serving_leg(duration_s=args.serve_seconds)
 
