@@ -410,7 +410,7 @@ __device__ inline bool param_hit(const ModelView &md, int P, int npsize, int t,
 // left.  The loci are taken in groups of D:
 //  1. one wavefront per locus reads its likelihood, uniform and records,
 //     flags the entries its proposal changed and marks the parameters they
-//     feed (for group g+1 while group g is being decided);
+//     feed (for group g+2 while group g is being decided);
 //  2. one wavefront per parameter evaluates, one lane per scenario, the
 //     parameter's sums and integral for every locus j of the group under
 //     every combination S of the decisions on loci 0..j-1 (2^D - 1 lanes),
@@ -514,11 +514,11 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   const ModelView md = model_view(sh.mi, sh.mdl, mm);
   double probg = m.probg[c];
   const double beta = m.beta[c];
+  // the first two groups; later ones are fetched two ahead while wave 0
+  // decides (their rows are those of the group committed just before)
   prior_fetch(m, md, mm.npsize, sh, Lz, c, 0, min(D, m.L), 0, 0, sweep);
+  if (D < m.L) prior_fetch(m, md, mm.npsize, sh, Lz, c, D, min(D, m.L - D), D, 0, sweep);
   __syncthreads();
-  // wavefronts left over by the parameters fetch the next group during
-  // step 2; with none left, the others fetch it while wave 0 decides
-  const bool fetch_early = np < nwaves;
   // wave 0 keeps each parameter's committed (term, weight, count) in lane
   // t's registers (np <= 64) and sums the terms across lanes
   const bool regs = np <= kWave;
@@ -531,8 +531,8 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   }
   int pl0 = 0, pnl = 0, pb0 = 0;  // the previous group, still to commit
   for (int l0 = 0, g = 0; l0 < m.L; l0 += D, ++g) {
-    const int nl = min(D, m.L - l0), b0 = (g % 3) * D, nb0 = ((g + 1) % 3) * D;
-    const int nnl = l0 + D < m.L ? min(D, m.L - l0 - D) : 0;
+    const int nl = min(D, m.L - l0), b0 = (g % 3) * D, nb0 = ((g + 2) % 3) * D;
+    const int nnl = l0 + 2 * D < m.L ? min(D, m.L - l0 - 2 * D) : 0;  // group g + 2
     // 2. one wavefront per parameter, one lane per (locus j, decisions S)
     for (int t = wave; t < np; t += nwaves) {
       const int s = lane;
@@ -559,10 +559,8 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
         }
       }
     }
-    // 4. for the previous group; 1. for the next one
+    // 4. for the previous group (whose rows group g + 2 takes next)
     prior_commit(v, m, sh, c, pl0, pnl, pb0);
-    if (fetch_early && nnl > 0)
-      prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, np, sweep);
     __syncthreads();
     if (wave == 0) {
       // 3. decisions in order
@@ -588,7 +586,8 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           if ((affm >> j) & 1u) {
             const int c2 = sh.sc[lane * W + s];
             const double f2 = sh.sf[lane * W + s];
-            if (!(c2 == cc && f2 == f)) term = sh.sterm[lane * W + s];
+            const double t2 = sh.sterm[lane * W + s];
+            if (!(c2 == cc && f2 == f)) term = t2;
             cc = c2;
             f = f2;
           }
@@ -650,9 +649,9 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           wave_sync();
         }
       }
-    } else if (!fetch_early && nnl > 0) {
-      // 1. the next group
-      prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + D, nnl, nb0, 1, sweep);
+    } else if (nnl > 0) {
+      // 1. group g + 2
+      prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + 2 * D, nnl, nb0, 1, sweep);
     }
     __syncthreads();
     // the accepted changes into the sums, in locus order per entry
