@@ -37,13 +37,17 @@ __host__ __device__ inline Philox4 philox4x32_10(Philox4 ctr, uint32_t k0, uint3
 }
 
 // A stream of uniforms in [0, 1) for key (seed) and a 64-bit counter.
+// The block's four words are separate members picked by selects: an
+// array indexed by the run-time position would live in scratch memory
+// inside device loops.
 struct PhiloxStream {
   uint32_t k0, k1;
   uint64_t ctr;
-  uint32_t buf[4];
+  uint32_t b0, b1, b2, b3;
   int left;
   __host__ __device__ PhiloxStream(uint64_t seed, uint64_t counter)
-      : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), ctr(counter), left(0) {}
+      : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), ctr(counter), b0(0), b1(0), b2(0),
+        b3(0), left(0) {}
   __host__ __device__ uint32_t next32() {
     if (left == 0) {
       Philox4 c;
@@ -52,11 +56,15 @@ struct PhiloxStream {
       c.v[2] = 0x5EED0001u;
       c.v[3] = 0u;
       const Philox4 r = philox4x32_10(c, k0, k1);
-      for (int i = 0; i < 4; ++i) buf[i] = r.v[i];
+      b0 = r.v[0];
+      b1 = r.v[1];
+      b2 = r.v[2];
+      b3 = r.v[3];
       left = 4;
       ++ctr;
     }
-    return buf[4 - (left--)];
+    const int i = 4 - (left--);
+    return i == 0 ? b0 : i == 1 ? b1 : i == 2 ? b2 : b3;
   }
   __host__ __device__ double uniform() {
     const uint64_t hi = next32(), lo = next32();
