@@ -71,6 +71,18 @@ struct ModelView {
 //   cc[P][P] | fc[P][P] | hcc[P][P] | mc[P][P][P] | fm[P][P][P] | length
 __host__ __device__ inline int prior_rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
 
+// Dynamic LDS budget of the combined likelihood-and-weights launch (the
+// static tables add a few KB).
+constexpr size_t kTwLdsBudget = 61440;
+
+// Orders one wavefront's LDS writes before its later LDS reads (lanes read
+// what other lanes of the same wavefront wrote).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One postorder operation of the HKY pruning program: compute the partial
 // at the top of edge `node` from daughters a and b.  The program is a stack
 // machine: OP_TT combines two tips and pushes, OP_ST combines the stack top
@@ -166,6 +178,23 @@ size_t hky_lds_bytes(int precision, int max_n);
 hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                       int precision, int max_n, int max_slots, double *out,
                       int32_t *status, hipStream_t st);
+// Genealogy weights run by the combined scoring launch: entries [0, n) of
+// slots / tv_row / rec_out / status (treeweight_kernel's arguments), emax
+// events at most per genealogy.
+struct TwLaunch {
+  const int32_t *slots;
+  int n, emax;
+  const double *tvals;
+  const int32_t *tv_row;
+  double *rec_out;
+  int32_t *status;
+};
+// HKY units (no merge step) and treeweight in one launch, when its LDS
+// (hky_tw_lds_bytes) is within kTwLdsBudget.
+size_t hky_tw_lds_bytes(int precision, int max_n, int P, int emax);
+hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
+                         int precision, int max_n, int max_slots, double *out, int32_t *status,
+                         const TwLaunch &tw, int P, hipStream_t st);
 hipError_t launch_is(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, int max_n, int max_w,
                      double *out, int32_t *status, int32_t *site_node,

@@ -32,6 +32,7 @@
 // lane carries the sum of the node scale factors (the reference keeps
 // log(max) per node, :211-217, 465).
 #include "ima_internal.h"
+#include "tw_dev.h"
 
 namespace ima {
 namespace {
@@ -199,13 +200,16 @@ void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int 
   *nblocks_total += nb;
 }
 
+// Work unit `unit` (item, pattern block, first partial, blocks) of one
+// 64-lane workgroup, with smem its dynamic LDS of hky_lds_bytes.
 template <typename real, int D>
-__global__ __launch_bounds__(kWave) void hky_kernel(
-    DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
-    double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
-  const int4 unit = units[blockIdx.x];
+__device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__restrict__ items,
+                                         const int4 unit, double *__restrict__ partial,
+                                         double *__restrict__ out,
+                                         int32_t *__restrict__ status, unsigned char *smem) {
+  // the unit is one wavefront's own work: its LDS slice is its own and it
+  // synchronises with wavefront-scope fences only
+  const int lane = threadIdx.x & (kWave - 1);
   const int ii = unit.x, blk = unit.y;
   const ima_item item = items[ii];
   const SlotMeta sm = v.slots[item.slot];
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   {
     const int p = blk * kWave + lane;
     const bool valid = p < S;
-    __syncthreads();
+    wave_lds_fence();
     // tip states of this block: n rows of 16 aligned dwords, all loads issued
     // before the LDS writes
     for (int w0 = 0; w0 < words; w0 += 16 * kWave) {
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
         if (w < words) tips32[w] = r[j];
       }
     }
-    __syncthreads();
+    wave_lds_fence();
 
     Q4<real> stk[D];
 #pragma unroll
@@ -332,6 +336,32 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   }
 }
 
+template <typename real, int D>
+__global__ __launch_bounds__(kWave) void hky_kernel(
+    DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
+    double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  hky_unit<real, D>(v, items, units[blockIdx.x], partial, out, status, smem);
+}
+
+// The sweep's scoring in one launch: workgroups [0, tw.n) weigh the
+// proposals' genealogies (treeweight), the rest run the likelihood units.
+// One launch instead of two side by side on two streams saves the
+// cross-stream fork and join between the accept steps.
+template <typename real, int D>
+__global__ __launch_bounds__(kWave) void hky_tw_kernel(
+    DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
+    double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status,
+    TwLaunch tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if ((int)blockIdx.x < tw.n) {
+    twdev::treeweight_one(v, tw.slots, blockIdx.x, tw.tvals, tw.tv_row, tw.rec_out, tw.status,
+                          reinterpret_cast<double *>(smem));
+    return;
+  }
+  hky_unit<real, D>(v, items, units[blockIdx.x - tw.n], partial, out, status, smem);
+}
+
 __global__ void hky_reduce(const int4 *__restrict__ red, int nred,
                            const double *__restrict__ partial, double *__restrict__ out,
                            int32_t *__restrict__ status) {
@@ -357,6 +387,40 @@ static void launch_depth(int D, dim3 g, size_t lds, hipStream_t st, const Device
   else
     hipLaunchKernelGGL((hky_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
                        p.partial, out, status);
+}
+
+template <typename real>
+static void launch_depth_tw(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
+                            const ima_item *items, const HkyPlan &p, double *out,
+                            int32_t *status, const TwLaunch &tw) {
+  if (D <= 4)
+    hipLaunchKernelGGL((hky_tw_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status, tw);
+  else if (D <= 6)
+    hipLaunchKernelGGL((hky_tw_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status, tw);
+  else
+    hipLaunchKernelGGL((hky_tw_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
+                       p.partial, out, status, tw);
+}
+
+size_t hky_tw_lds_bytes(int precision, int max_n, int P, int emax) {
+  return std::max(hky_lds_bytes(precision, max_n), treeweight_lds_bytes(P, emax));
+}
+
+hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
+                         int precision, int max_n, int max_slots, double *out, int32_t *status,
+                         const TwLaunch &tw, int P, hipStream_t st) {
+  if (plan.nunits == 0 || tw.n == 0) return hipErrorInvalidValue;
+  if (max_slots > 8 || plan.nred > 0) return hipErrorInvalidValue;
+  const size_t lds = hky_tw_lds_bytes(precision, max_n, P, tw.emax);
+  if (lds > kTwLdsBudget) return hipErrorInvalidValue;
+  const dim3 g(plan.nunits + tw.n);
+  if (precision == IMA_PREC_FP32)
+    launch_depth_tw<float>(max_slots, g, lds, st, v, items, plan, out, status, tw);
+  else
+    launch_depth_tw<double>(max_slots, g, lds, st, v, items, plan, out, status, tw);
+  return hipGetLastError();
 }
 
 hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
