@@ -62,7 +62,9 @@ prior_kernel(DeviceView v, int model, const double *recs, const int32_t *group_o
 }  // namespace
 
 size_t treeweight_lds_bytes(int P, int emax) {
-  return (size_t)prior_rec_len(P) * 8 + (size_t)emax * 24;
+  // record, unsorted and sorted events, one chunk's terms
+  return (size_t)prior_rec_len(P) * 8 + (size_t)emax * 24 +
+         (size_t)(2 * (2 * P - 1) + 1) * kWave * 8;
 }
 
 hipError_t launch_treeweight(const DeviceView &v, const int32_t *slots, int nslots,

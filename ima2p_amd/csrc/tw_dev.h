@@ -94,7 +94,10 @@ __device__ __forceinline__ void treeweight_one(const DeviceView &v, const int32_
   double *rec = lds;                       // [R]
   double *et = rec + R;                    // [E] unsorted times
   double *st = et + E;                     // [E] sorted times
-  int *ek = (int *)(st + E);               // [E] unsorted codes
+  double *tfc = st + E;                    // [T][64] fc terms of a chunk
+  double *tfm = tfc + T * kWave;           // [T][64] fm terms
+  double *tln = tfm + T * kWave;           // [64] length terms
+  int *ek = (int *)(tln + kWave);          // [E] unsorted codes
   int *sk = ek + E;                        // [E] sorted codes
   for (int r = lane; r < R; r += kWave) rec[r] = 0.0;
 
@@ -131,78 +134,93 @@ __device__ __forceinline__ void treeweight_one(const DeviceView &v, const int32_
   }
   __syncthreads();
 
-  // sweep: lane p follows population p; the period's values stay in
-  // registers until a split changes the period.  Every lane adds its terms
-  // at every event: a lane's sums are written only while its population is
-  // present and restart at each split, so the terms a lane adds while its
-  // population is absent (or in the last period, where fm is not kept)
-  // are never read.  The events of one period run in a loop of their own,
-  // so the per-event work carries no branch that changes registers.
+  // walk: the sorted events in chunks of up to 64 (lane l = event j0 + l),
+  // never past the split that ends the current period.  A population's
+  // lineage count before each event of the chunk is its count at the
+  // chunk's start plus the migrations into it, minus the coalescences and
+  // migrations out of it, among the chunk's earlier events (lane ballots);
+  // every event's terms n(n-1) dt / (2h), n dt and (all lineages) dt then
+  // come out in parallel, and lane p adds population p's terms in event
+  // order, the sequence of additions of the event-by-event sweep.  Lanes
+  // add the terms of absent populations too (and fm in the last period):
+  // those sums are never written, and every sum restarts at a split.
   const int p = lane;
   const bool mine = p < T;
-  int cnt = (p < P) ? lm.samppop[p] : 0;
+  int cnt = (p < P) ? lm.samppop[p] : 0;  // lane p: population p's count
   int nsum = n, k = 0, cc = 0, mybad = 0;
   double last = 0.0, len = 0.0, fcacc = 0.0, fmacc = 0.0;
   const double h2term = 1 / (2 * lm.hval);
   int ip = mine ? s_pos[p] : -1;
-  // the sorted events are read from registers: lane j holds event
-  // 64 * (j / 64) + j of the current block and v_readlane broadcasts it
-  double t_reg = 0.0;
-  int c_reg = 0;
-  int j = 0;
-  for (;;) {
-    const int jsplit = k < kroot ? s_spos[k] : E;  // the period ends at split k
-    for (;; ++j) {
-      if (j >= E) break;
-      const int jj = j & (kWave - 1);
-      if (jj == 0) {
-        t_reg = j + lane < E ? st[j + lane] : 0.0;
-        c_reg = j + lane < E ? sk[j + lane] : 0;
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
+  int j0 = 0;
+  while (j0 < E) {
+    const int jsplit = k < kroot ? s_spos[k] : E;  // split k ends the period
+    // split times out of order would put split k behind the walk: flagged,
+    // and the walk still advances
+    if (jsplit < j0) mybad = 1;
+    const int nc = min(kWave, min(max(jsplit + 1, j0 + 1), E) - j0);
+    const int j = j0 + lane;
+    const bool in = lane < nc;
+    const double t = in ? st[j] : 0.0;
+    const int code = in ? sk[j] : 0;
+    const double tp = __shfl_up(t, 1, kWave);
+    const double dt = t - (lane == 0 ? last : tp);
+    const int type = code & 15, a = (code >> 4) & 255, b = (code >> 12) - 1;
+    const bool ev = in && j != jsplit;  // a coalescence or migration
+    const bool coal = ev && type == EV_COAL, mig = ev && type == EV_MIG;
+    // period positions of the event's populations; the bad cases of the
+    // sequential sweep (a lineage in an absent population, a migration to
+    // one)
+    const int ipa = (a < T) ? s_pos[k * T + a] : -1;
+    const int ib = (b >= 0 && b < T) ? s_pos[k * T + b] : -1;
+    if ((coal && ipa < 0) || (mig && (ipa < 0 || ib < 0))) mybad = 1;
+    if (mig && ipa >= 0 && ib >= 0) atomicAdd(&rec[3 * P * P + (k * P + ipa) * P + ib], 1.0);
+    const uint64_t mcoal = __ballot(coal);
+    const int nbefore = nsum - (int)__popcll(mcoal & below);
+    tln[lane] = nbefore * dt;
+    for (int q = 0; q < T; ++q) {
+      const uint64_t mfrom = __ballot((coal || mig) && a == q);
+      const uint64_t mto = __ballot(mig && b == q);
+      const int c0 = __builtin_amdgcn_readlane(cnt, q);
+      const int c = c0 + (int)__popcll(mto & below) - (int)__popcll(mfrom & below);
+      tfc[q * kWave + lane] = ((double)c * ((double)c - 1)) * dt * h2term;
+      tfm[q * kWave + lane] = c * dt;
+      if (lane == q) {
+        cnt = c0 + (int)__popcll(mto) - (int)__popcll(mfrom);
+        cc += (int)__popcll(mfrom & mcoal);
       }
-      const double t = bcast_d(t_reg, jj);
-      const int code = __builtin_amdgcn_readlane(c_reg, jj);
-      const double dt = t - last;
-      len += nsum * dt;
-      last = t;
-      fcacc += ((double)cnt * ((double)cnt - 1)) * dt * h2term;
-      fmacc += cnt * dt;
-      if (j == jsplit) break;
-      const int type = code & 15, a = (code >> 4) & 255, b = (code >> 12) - 1;
-      const int from = p == a ? 1 : 0;
-      const int coal = type == EV_COAL ? 1 : 0;
-      int mb = from & (ip < 0 ? 1 : 0);
-      if (!coal && from) {
-        // migration: count it in the period's (from, to) positions
-        const int ib = (b >= 0 && b < T) ? s_pos[k * T + b] : -1;
-        if (ib < 0)
-          mb = 1;
-        else if (ip >= 0)
-          rec[3 * P * P + (k * P + ip) * P + ib] += 1.0;
+    }
+    wave_lds_fence();
+    for (int l = 0; l < nc; ++l) {
+      len += tln[l];
+      if (mine) {
+        fcacc += tfc[p * kWave + l];
+        fmacc += tfm[p * kWave + l];
       }
-      mybad |= mb;
-      cc += coal & from;
-      cnt += (!coal && p == b ? 1 : 0) - from;
-      nsum -= coal;
     }
-    if (j >= E) break;
-    // split k (event j): the new population takes the two merged counts
-    const int ja = s_join[3 * k], jb_ = s_join[3 * k + 1], jn = s_join[3 * k + 2];
-    const int na = __shfl(cnt, ja < 0 ? 0 : ja, kWave);
-    const int nbb = __shfl(cnt, jb_ < 0 ? 0 : jb_, kWave);
-    if (ip >= 0) {
-      rec[k * P + ip] = (double)cc;
-      rec[P * P + k * P + ip] = fcacc;
-      if (k < nper - 1)
-        for (int q = 0; q < s_npresent[k]; ++q)
-          if (q != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + q] = fmacc;
+    nsum -= (int)__popcll(mcoal);
+    last = bcast_d(t, nc - 1);
+    j0 += nc;
+    wave_lds_fence();  // the term buffers are refilled by the next chunk
+    if (j0 == jsplit + 1) {
+      // split k: the new population takes the two merged counts
+      const int ja = s_join[3 * k], jb_ = s_join[3 * k + 1], jn = s_join[3 * k + 2];
+      const int na = __shfl(cnt, ja < 0 ? 0 : ja, kWave);
+      const int nbb = __shfl(cnt, jb_ < 0 ? 0 : jb_, kWave);
+      if (ip >= 0) {
+        rec[k * P + ip] = (double)cc;
+        rec[P * P + k * P + ip] = fcacc;
+        if (k < nper - 1)
+          for (int q = 0; q < s_npresent[k]; ++q)
+            if (q != ip) rec[3 * P * P + P * P * P + (k * P + ip) * P + q] = fmacc;
+      }
+      if (p == jn) cnt = na + nbb;
+      if (p == ja || p == jb_) cnt = 0;
+      cc = 0;
+      fcacc = fmacc = 0.0;
+      k++;
+      ip = mine ? s_pos[k * T + p] : -1;
     }
-    if (p == jn) cnt = na + nbb;
-    if (p == ja || p == jb_) cnt = 0;
-    cc = 0;
-    fcacc = fmacc = 0.0;
-    k++;
-    ip = mine ? s_pos[k * T + p] : -1;
-    ++j;
   }
   if (ip >= 0) {
     rec[k * P + ip] = (double)cc;
