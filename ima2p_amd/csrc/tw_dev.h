@@ -191,11 +191,13 @@ __device__ __forceinline__ void treeweight_one(const DeviceView &v, const int32_
       }
     }
     wave_lds_fence();
-    for (int l = 0; l < nc; ++l) {
-      len += tln[l];
-      if (mine) {
-        fcacc += tfc[p * kWave + l];
-        fmacc += tfm[p * kWave + l];
+    {
+      // lanes past T add population 0's terms: never written
+      const double *fc = tfc + (mine ? p : 0) * kWave, *fm = tfm + (mine ? p : 0) * kWave;
+      for (int l = 0; l < nc; ++l) {
+        len += tln[l];
+        fcacc += fc[l];
+        fmacc += fm[l];
       }
     }
     nsum -= (int)__popcll(mcoal);
