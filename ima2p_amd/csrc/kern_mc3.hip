@@ -860,23 +860,30 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   const bool prior = m.prior >= 0;
   const size_t dl = prior ? PriorLds(m.R, m.nparam, m.pchunk, m.mints).bytes() : 0;
   // score_ev (optional): events recorded around the two scoring passes.
-  // The proposals' genealogy weights run in the HKY launch; without HKY
-  // items (or for genealogies too large for that launch's LDS) they run on
-  // the side stream beside the likelihood launches, and the accept kernel
-  // waits for both
+  // The proposals' genealogy weights run in the HKY launch (IS and SW
+  // launches beside it on the side stream); without HKY items (or for
+  // genealogies too large for that launch's LDS) they run on the side
+  // stream beside the likelihood launches.  The accept kernel waits for
+  // both streams
   const TwLaunch tw{m.tw_slots, m.C * m.L, m.emax, m.tv, m.tv_row, m.wnew, m.wstatus};
   auto score = [&](int pass) -> hipError_t {
     hipError_t e = hipSuccess;
     if (prior && nh && hky_tw_lds_bytes(precision, hmax_n, m.P, m.emax) <= kTwLdsBudget) {
+      // IS and SW items, if any, are scored on the side stream beside it
+      const bool fork = ni || ns;
       if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);
+      if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass], st);
+      if (e == hipSuccess && fork) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
+      if (e == hipSuccess && ni)
+        e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
+                      nullptr, side);
+      if (e == hipSuccess && ns)
+        e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, side);
+      if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass + 1], side);
       if (e == hipSuccess)
         e = launch_hky_tw(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, tw,
                           m.P, st);
-      if (e == hipSuccess && ni)
-        e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
-                      nullptr, st);
-      if (e == hipSuccess && ns)
-        e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
+      if (e == hipSuccess && fork) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
       if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
       return e;
     }
