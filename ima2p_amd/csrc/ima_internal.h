@@ -295,7 +295,7 @@ struct PriorLds {
   // per-locus rows come in three buffers: one group is decided while the
   // previous one is committed and the next one's records are read
   __host__ __device__ size_t doubles() const {
-    return (size_t)R + 5 * np + 9 * LC + (size_t)6 * LC * R + (size_t)2 * np * W;
+    return (size_t)R + 5 * np + 9 * LC + (size_t)6 * LC * R + (size_t)3 * np * W + W;
   }
   __host__ __device__ size_t ints() const {
     return (size_t)2 * np + (size_t)3 * LC * (2 + R + aw) + (size_t)np * W + mints;
@@ -314,7 +314,7 @@ hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
 constexpr int kSweepBlock = 8;  // sweeps captured in one graph
-constexpr int kSwapLds = 1024;  // chains whose S and beta the swap stages in LDS
+constexpr int kSwapLds = 256;  // chains whose S and beta the swap stages in LDS
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 
 // Swap (kern_swap.hip).

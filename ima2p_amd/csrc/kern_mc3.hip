@@ -291,8 +291,10 @@ __device__ inline double wsum_update(int r, int P, double all, double old, doubl
 //                    which entries differ
 //   aff[D][aw]       parameters whose entries a locus changed
 //   sterm/sf/sc      [np][W] per (parameter, scenario) integral, weight, count
+//   su               [np][W] the term a scenario's decision uses
+//   pns              [W] a scenario's prior total (sum of su over parameters)
 struct PriorSh {
-  double *all, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *sterm, *sf;
+  double *all, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *sterm, *sf, *su, *pns;
   int *qc, *nc, *ok, *dec, *flag, *sc, *mi;
   uint32_t *aff;
 };
@@ -313,7 +315,9 @@ __device__ inline PriorSh prior_carve(double *dyn, const PriorLds &Lz) {
   p.dnew = p.dold + DR;
   p.sterm = p.dnew + DR;
   p.sf = p.sterm + NW;
-  p.qc = reinterpret_cast<int *>(p.sf + NW);
+  p.su = p.sf + NW;
+  p.pns = p.su + NW;
+  p.qc = reinterpret_cast<int *>(p.pns + Lz.W);
   p.nc = p.qc + Lz.np;
   p.ok = p.nc + Lz.np;
   p.dec = p.ok + L2;
@@ -519,16 +523,9 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   prior_fetch(m, md, mm.npsize, sh, Lz, c, 0, min(D, m.L), 0, 0, sweep);
   if (D < m.L) prior_fetch(m, md, mm.npsize, sh, Lz, c, D, min(D, m.L - D), D, 0, sweep);
   __syncthreads();
-  // wave 0 keeps each parameter's committed (term, weight, count) in lane
-  // t's registers (np <= 64) and sums the terms across lanes
+  // np <= 64: each scenario's prior total is formed before the decisions
+  // (one lane per scenario), and a decision reads it from LDS
   const bool regs = np <= kWave;
-  double rqt = 0.0, rqf = 0.0;
-  int rqc = 0;
-  if (wave == 0 && regs && lane < np) {
-    rqt = sh.qt[lane];
-    rqf = sh.qf[lane];
-    rqc = sh.qc[lane];
-  }
   int pl0 = 0, pnl = 0, pb0 = 0;  // the previous group, still to commit
   for (int l0 = 0, g = 0; l0 < m.L; l0 += D, ++g) {
     const int nl = min(D, m.L - l0), b0 = (g % 3) * D, nb0 = ((g + 2) % 3) * D;
@@ -558,6 +555,33 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           sh.sterm[t * W + s] = prior_param_term(md, mm.npsize, t, cc, f, h, v.logfact);
         }
       }
+      if (regs) {
+        // the term scenario (j, S) uses for parameter t: from the group's
+        // committed (term, weight, count), each accepted locus i of S that
+        // marks t, in order, then locus j, takes its scenario's values,
+        // keeping the committed term when count and weight are unchanged
+        // (update_gtree_common.cpp:1997-2006)
+        wave_sync();  // the scenario values of this wavefront's lanes
+        if (s < W - 1) {
+          const int j = 31 - __clz(s + 1);
+          const int S = s + 1 - (1 << j);
+          if (j < nl) {
+            double term = sh.qt[t], f = sh.qf[t];
+            int cc = sh.qc[t];
+            for (int i = 0; i <= j; ++i) {
+              if (i < j && !((S >> i) & 1)) continue;
+              if (!((sh.aff[(b0 + i) * Lz.aw + t / 32] >> (t % 32)) & 1u)) continue;
+              const int si = i == j ? s : (1 << i) - 1 + (S & ((1 << i) - 1));
+              const int c2 = sh.sc[t * W + si];
+              const double f2 = sh.sf[t * W + si];
+              if (!(c2 == cc && f2 == f)) term = sh.sterm[t * W + si];
+              cc = c2;
+              f = f2;
+            }
+            sh.su[t * W + s] = term;
+          }
+        }
+      }
     }
     // 4. for the previous group (whose rows group g + 2 takes next)
     prior_commit(v, m, sh, c, pl0, pnl, pb0);
@@ -566,45 +590,52 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
       // 3. decisions in order
       int Smask = 0;
       if (regs) {
-        uint32_t affm = 0;  // bit j: this lane's parameter is marked by locus j
+        // each scenario's prior total, the parameters in order (population
+        // sizes, then migration rates)
+        if (lane < W - 1 && 31 - __clz(lane + 1) < nl) {
+          double q = 0.0, mg = 0.0;
+          for (int t = 0; t < mm.npsize; ++t) q += sh.su[t * W + lane];
+          for (int t = mm.npsize; t < np; ++t) mg += sh.su[t * W + lane];
+          sh.pns[lane] = q + mg;
+        }
         double xU = 0.0, xd = 0.0;  // lane j: U and the likelihood change of locus j
         int okm = 0;
-        for (int j = 0; j < nl; ++j) {
-          const int row = b0 + j;
-          if (lane < np) affm |= ((sh.aff[row * Lz.aw + lane / 32] >> (lane % 32)) & 1u) << j;
-          okm |= (sh.ok[row] ? 1 : 0) << j;
-        }
+        for (int j = 0; j < nl; ++j) okm |= (sh.ok[b0 + j] ? 1 : 0) << j;
         if (lane < nl) {
           xU = sh.U[b0 + lane];  // log U
           xd = sh.npd[b0 + lane] - sh.pdg[b0 + lane];
         }
+        wave_sync();
         for (int j = 0; j < nl; ++j) {
           if (!((okm >> j) & 1)) continue;
-          const int s = (1 << j) - 1 + Smask;
-          double term = rqt, f = rqf;
-          int cc = rqc;
-          if ((affm >> j) & 1u) {
-            const int c2 = sh.sc[lane * W + s];
-            const double f2 = sh.sf[lane * W + s];
-            const double t2 = sh.sterm[lane * W + s];
-            if (!(c2 == cc && f2 == f)) term = t2;
-            cc = c2;
-            f = f2;
-          }
-          double q = 0.0, mg = 0.0;
-          for (int t = 0; t < mm.npsize; ++t) q += readlane_d(term, t);
-          for (int t = mm.npsize; t < np; ++t) mg += readlane_d(term, t);
-          const double pn = q + mg;
+          const double pn = sh.pns[(1 << j) - 1 + Smask];
           const double x = beta * ((pn - probg) + readlane_d(xd, j));
           const double lu = readlane_d(xU, j);
           const bool acc = x != x ? lu != INFINITY : lu < x;
           if (acc) {
             probg = pn;
             Smask |= 1 << j;
-            rqt = term;
-            rqf = f;
-            rqc = cc;
           }
+        }
+        // the committed (term, weight, count) of each parameter after the
+        // group: the same rule over the accepted loci
+        if (lane < np) {
+          const int t = lane;
+          double term = sh.qt[t], f = sh.qf[t];
+          int cc = sh.qc[t];
+          for (int i = 0; i < nl; ++i) {
+            if (!((Smask >> i) & 1)) continue;
+            if (!((sh.aff[(b0 + i) * Lz.aw + t / 32] >> (t % 32)) & 1u)) continue;
+            const int si = (1 << i) - 1 + (Smask & ((1 << i) - 1));
+            const int c2 = sh.sc[t * W + si];
+            const double f2 = sh.sf[t * W + si];
+            if (!(c2 == cc && f2 == f)) term = sh.sterm[t * W + si];
+            cc = c2;
+            f = f2;
+          }
+          sh.qt[t] = term;
+          sh.qf[t] = f;
+          sh.qc[t] = cc;
         }
         if (lane < nl) sh.dec[b0 + lane] = (Smask >> lane) & 1;
       } else {
@@ -668,11 +699,6 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
     pb0 = b0;
   }
   prior_commit(v, m, sh, c, pl0, pnl, pb0);
-  if (wave == 0 && regs && lane < np) {
-    sh.qt[lane] = rqt;
-    sh.qf[lane] = rqf;
-    sh.qc[lane] = rqc;
-  }
   __syncthreads();
   if (threadIdx.x == 0) m.probg[c] = probg;
   for (int r = threadIdx.x; r < R; r += blockDim.x) m.wall[(int64_t)c * R + r] = sh.all[r];
