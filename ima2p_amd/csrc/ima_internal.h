@@ -298,7 +298,7 @@ struct PriorLds {
     return (size_t)R + 5 * np + 9 * LC + (size_t)6 * LC * R + (size_t)3 * np * W + W;
   }
   __host__ __device__ size_t ints() const {
-    return (size_t)2 * np + (size_t)3 * LC * (2 + R + aw) + (size_t)np * W + mints;
+    return (size_t)2 * np + (size_t)3 * LC * (2 + R + aw) + (size_t)np * W + mints + R;
   }
   __host__ __device__ size_t bytes() const { return doubles() * 8 + ints() * 4; }
 };

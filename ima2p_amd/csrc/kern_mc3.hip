@@ -293,9 +293,10 @@ __device__ inline double wsum_update(int r, int P, double all, double old, doubl
 //   sterm/sf/sc      [np][W] per (parameter, scenario) integral, weight, count
 //   su               [np][W] the term a scenario's decision uses
 //   pns              [W] a scenario's prior total (sum of su over parameters)
+//   fmk              [R] bit i: locus i of the group changed the entry
 struct PriorSh {
   double *all, *qt, *qf, *nt, *nf, *npd, *pdg, *U, *dold, *dnew, *mdl, *sterm, *sf, *su, *pns;
-  int *qc, *nc, *ok, *dec, *flag, *sc, *mi;
+  int *qc, *nc, *ok, *dec, *flag, *sc, *mi, *fmk;
   uint32_t *aff;
 };
 __device__ inline PriorSh prior_carve(double *dyn, const PriorLds &Lz) {
@@ -325,6 +326,7 @@ __device__ inline PriorSh prior_carve(double *dyn, const PriorLds &Lz) {
   p.sc = p.flag + DR;
   p.aff = reinterpret_cast<uint32_t *>(p.sc + NW);
   p.mi = reinterpret_cast<int *>(p.aff + (size_t)L2 * Lz.aw);
+  p.fmk = p.mi + Lz.mints;
   return p;
 }
 
@@ -523,6 +525,15 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   prior_fetch(m, md, mm.npsize, sh, Lz, c, 0, min(D, m.L), 0, 0, sweep);
   if (D < m.L) prior_fetch(m, md, mm.npsize, sh, Lz, c, D, min(D, m.L - D), D, 0, sweep);
   __syncthreads();
+  // which loci of the group change each entry (group 0 here, group g + 1
+  // beside the sums' update of group g)
+  auto entry_masks = [&](int r, int b, int n) {
+    int mk = 0;
+    for (int j = 0; j < n; ++j) mk |= (sh.flag[(b + j) * R + r] ? 1 : 0) << j;
+    sh.fmk[r] = mk;
+  };
+  for (int r = threadIdx.x; r < R; r += blockDim.x) entry_masks(r, 0, min(D, m.L));
+  __syncthreads();
   // np <= 64: each scenario's prior total is formed before the decisions
   // (one lane per scenario), and a decision reads it from LDS
   const bool regs = np <= kWave;
@@ -538,12 +549,13 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
         const int S = s + 1 - (1 << j);
         if (j < nl && sh.ok[b0 + j] &&
             ((sh.aff[(b0 + j) * Lz.aw + t / 32] >> (t % 32)) & 1u)) {
+          // the accepted loci of S and locus j, in order, that changed entry e
+          const uint32_t sel = (uint32_t)S | (1u << j);
           auto get = [&](int e) {
             double val = sh.all[e];
-            for (int i = 0; i <= j; ++i) {
-              const int row = b0 + i;
-              if ((i == j || ((S >> i) & 1)) && sh.flag[row * R + e])
-                val = wsum_update(e, P, val, sh.dold[row * R + e], sh.dnew[row * R + e]);
+            for (uint32_t bits = (uint32_t)sh.fmk[e] & sel; bits; bits &= bits - 1) {
+              const int row = b0 + (__ffs(bits) - 1);
+              val = wsum_update(e, P, val, sh.dold[row * R + e], sh.dnew[row * R + e]);
             }
             return val;
           };
@@ -685,13 +697,16 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
       prior_fetch(m, md, mm.npsize, sh, Lz, c, l0 + 2 * D, nnl, nb0, 1, sweep);
     }
     __syncthreads();
-    // the accepted changes into the sums, in locus order per entry
+    // the accepted changes into the sums, in locus order per entry; the
+    // next group's entry masks
+    const int n1 = l0 + D < m.L ? min(D, m.L - l0 - D) : 0, b1 = ((g + 1) % 3) * D;
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
       double a = sh.all[r];
       for (int j = 0; j < nl; ++j)
         if (sh.dec[b0 + j] && sh.flag[(b0 + j) * R + r])
           a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
       sh.all[r] = a;
+      entry_masks(r, b1, n1);
     }
     __syncthreads();
     pl0 = l0;
