@@ -629,26 +629,6 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
             Smask |= 1 << j;
           }
         }
-        // the committed (term, weight, count) of each parameter after the
-        // group: the same rule over the accepted loci
-        if (lane < np) {
-          const int t = lane;
-          double term = sh.qt[t], f = sh.qf[t];
-          int cc = sh.qc[t];
-          for (int i = 0; i < nl; ++i) {
-            if (!((Smask >> i) & 1)) continue;
-            if (!((sh.aff[(b0 + i) * Lz.aw + t / 32] >> (t % 32)) & 1u)) continue;
-            const int si = (1 << i) - 1 + (Smask & ((1 << i) - 1));
-            const int c2 = sh.sc[t * W + si];
-            const double f2 = sh.sf[t * W + si];
-            if (!(c2 == cc && f2 == f)) term = sh.sterm[t * W + si];
-            cc = c2;
-            f = f2;
-          }
-          sh.qt[t] = term;
-          sh.qf[t] = f;
-          sh.qc[t] = cc;
-        }
         if (lane < nl) sh.dec[b0 + lane] = (Smask >> lane) & 1;
       } else {
         for (int j = 0; j < nl; ++j) {
@@ -707,6 +687,28 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
       sh.all[r] = a;
       entry_masks(r, b1, n1);
+    }
+    if (regs && wave == nwaves - 1 && lane < np) {
+      // the committed (term, weight, count) of parameter t after the group:
+      // the scenario rule over the accepted loci
+      const int t = lane;
+      int Smask = 0;
+      for (int i = 0; i < nl; ++i) Smask |= (sh.dec[b0 + i] ? 1 : 0) << i;
+      double term = sh.qt[t], f = sh.qf[t];
+      int cc = sh.qc[t];
+      for (int i = 0; i < nl; ++i) {
+        if (!((Smask >> i) & 1)) continue;
+        if (!((sh.aff[(b0 + i) * Lz.aw + t / 32] >> (t % 32)) & 1u)) continue;
+        const int si = (1 << i) - 1 + (Smask & ((1 << i) - 1));
+        const int c2 = sh.sc[t * W + si];
+        const double f2 = sh.sf[t * W + si];
+        if (!(c2 == cc && f2 == f)) term = sh.sterm[t * W + si];
+        cc = c2;
+        f = f2;
+      }
+      sh.qt[t] = term;
+      sh.qf[t] = f;
+      sh.qc[t] = cc;
     }
     __syncthreads();
     pl0 = l0;
