@@ -680,20 +680,21 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
     // the accepted changes into the sums, in locus order per entry; the
     // next group's entry masks
     const int n1 = l0 + D < m.L ? min(D, m.L - l0 - D) : 0, b1 = ((g + 1) % 3) * D;
+    int accm = 0;  // the group's accepted loci
+    for (int j = 0; j < nl; ++j) accm |= (sh.dec[b0 + j] ? 1 : 0) << j;
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
       double a = sh.all[r];
-      for (int j = 0; j < nl; ++j)
-        if (sh.dec[b0 + j] && sh.flag[(b0 + j) * R + r])
-          a = wsum_update(r, P, a, sh.dold[(b0 + j) * R + r], sh.dnew[(b0 + j) * R + r]);
+      for (uint32_t bits = (uint32_t)(sh.fmk[r] & accm); bits; bits &= bits - 1) {
+        const int row = b0 + (__ffs(bits) - 1);
+        a = wsum_update(r, P, a, sh.dold[row * R + r], sh.dnew[row * R + r]);
+      }
       sh.all[r] = a;
       entry_masks(r, b1, n1);
     }
     if (regs && wave == nwaves - 1 && lane < np) {
       // the committed (term, weight, count) of parameter t after the group:
       // the scenario rule over the accepted loci
-      const int t = lane;
-      int Smask = 0;
-      for (int i = 0; i < nl; ++i) Smask |= (sh.dec[b0 + i] ? 1 : 0) << i;
+      const int t = lane, Smask = accm;
       double term = sh.qt[t], f = sh.qf[t];
       int cc = sh.qc[t];
       for (int i = 0; i < nl; ++i) {
