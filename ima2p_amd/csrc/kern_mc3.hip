@@ -151,8 +151,10 @@ __global__ void mc3_revert(DeviceView v, MC3Dev m) {
 // Likelihood of item `it` from the batched launch: HKY items spanning
 // several pattern blocks are merged here in block order.
 __device__ inline double new_loglik(const MC3Dev &m, int it) {
+  // both loads in flight together: most items are one block
+  const double direct = m.newpdg[it];
   const int2 ib = m.iblk[it];
-  if (ib.y <= 1) return m.newpdg[it];
+  if (ib.y <= 1) return direct;
   double s = 0.0;
   for (int b = 0; b < ib.y; ++b) s += m.partial[ib.x + b];
   return s;
@@ -340,10 +342,12 @@ __device__ double chain_prior_fresh(const DeviceView &v, const MC3Dev &m, int c,
   const ModelMeta mm = v.models[m.prior];
   const ModelView md = model_view(v, mm);
   // the chain's records go through LDS a block of loci at a time (read
-  // coalesced by the whole workgroup, in the rows of the accept step's
-  // record buffers), each entry summed over the loci in order
-  const int chunk = 6 * m.pchunk;
-  double *stage = sh.dold;  // dold and dnew: 6 * pchunk * R doubles
+  // coalesced by the whole workgroup, in the accept step's record rows and
+  // scenario arrays: dold .. pns are contiguous), each entry summed over
+  // the loci in order
+  const PriorLds Lz(R, np, m.pchunk, m.mints);
+  const int chunk = (int)((6 * (size_t)Lz.LC * R + 3 * (size_t)np * Lz.W + Lz.W) / R);
+  double *stage = sh.dold;
   const double *base = w + (int64_t)c * m.L * R;
   for (int l0 = 0; l0 < m.L; l0 += chunk) {
     const int nl = min(chunk, m.L - l0);
