@@ -1239,6 +1239,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.tbackup = m.tbackup.p;
   d.seed = seed;
   d.scale_width = scale_width;
+  d.ry1 = 0;
   d.prior = -1;
   d.P = 1;
   d.R = 0;
@@ -1288,6 +1289,11 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   const bool use_graph = (flags & IMA_MC3_GRAPH) != 0;
   const bool scale = (flags & IMA_MC3_SCALE) != 0;
   const bool timed = (flags & IMA_MC3_TIMED) != 0;
+  const int ry1 = (flags & IMA_MC3_RY1) ? 1 : 0;
+  if (ry1 != m.dev.ry1) {
+    mc3_drop_graphs(c);  // the graphs hold the sweep's parameters
+    m.dev.ry1 = ry1;
+  }
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
   const int bk = scale ? 1 : 0;
   if (use_graph && !timed) {

@@ -256,6 +256,7 @@ struct MC3Dev {
   unsigned *done;        // chains finished in the current sweep (last one swaps)
   uint64_t seed;
   double scale_width;
+  int ry1;               // 1: the time update is the rubber-band move of one split time
   // genealogy prior (prior < 0: the sweep scores the data likelihood only)
   int prior;             // population model id
   int P, R, nparam;      // populations, weight record length, parameters
@@ -314,6 +315,7 @@ hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
 constexpr int kSweepBlock = 8;  // sweeps captured in one graph
+constexpr double kTimeMax = 1000000.0;  // the reference's TIMEMAX (imamp.hpp:129)
 constexpr int kSwapLds = 256;  // chains whose S and beta the swap stages in LDS
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 

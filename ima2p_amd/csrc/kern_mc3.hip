@@ -732,6 +732,101 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
   __syncthreads();
 }
 
+// changet_RY1's move of chain c (update_t_RY.cpp:223-330, after Rannala
+// and Yang 2003): split k drawn uniformly (randposint), its new time newt
+// from getnewt's window (update_gtree_common.cpp:2501-2518): width
+// (t_d' - t_u') / (log(L + 1) (P - k)) around the old time, reflected at
+// the prior bounds t_u' = max(0, t_u), t_d' = min(tmax, t_d), where t_u and
+// t_d are the neighbouring split times (0 and TIMEMAX at the ends).  Every
+// node time (on each edge below the root) and migration time in (t_u,
+// oldt] is stretched by beforesplit, in (oldt, t_d) by aftersplit
+// (update_t_RY.cpp:59-82); the moved events are counted (node times appear
+// on both daughter edges, so their counts are halved) and the Hastings
+// term (ecd + emd) log t_d_hterm + (ecu + emu) log t_u_hterm goes to
+// m.logs[c].  Times are backed up for the restore as by the rescale.
+__device__ void ry1_stretch(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
+                            double *red) {
+  __shared__ double ry[5];  // k, oldt, newt, t_u, t_d
+  const int tw = mc3_tw(m.P), ns = m.P - 1;
+  if (threadIdx.x == 0) {
+    PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 2);
+    int k = (int)(r.uniform() * ns);
+    if (k > ns - 1) k = ns - 1;
+    const double *tv = m.tv + (int64_t)c * tw;
+    const double oldt = tv[k];
+    const double t_u = k == 0 ? 0.0 : tv[k - 1];
+    const double t_d = k == ns - 1 ? kTimeMax : tv[k + 1];
+    const double tdp = fmin(m.tmax, t_d), tup = fmax(0.0, t_u);
+    const double twin = (tdp - tup) / (log((double)m.L + 1.0) * (double)(m.P - k));
+    double newt = (oldt - twin / 2) + r.uniform() * twin;
+    if (newt >= tdp)
+      newt = 2.0 * tdp - newt;
+    else if (newt <= tup)
+      newt = 2.0 * tup - newt;
+    ry[0] = k;
+    ry[1] = oldt;
+    ry[2] = newt;
+    ry[3] = t_u;
+    ry[4] = t_d;
+  }
+  __syncthreads();
+  const int k = (int)ry[0];
+  const double oldt = ry[1], newt = ry[2], t_u = ry[3], t_d = ry[4];
+  const bool first = k == 0, last = k == ns - 1;
+  auto before = [&](double t) {
+    return first ? t * newt / oldt : t_u + (t - t_u) * (newt - t_u) / (oldt - t_u);
+  };
+  auto after = [&](double t) {
+    return last ? t + newt - oldt : t_d - (t_d - newt) * (t_d - t) / (t_d - oldt);
+  };
+  double nu = 0.0, nd = 0.0, mu = 0.0, md = 0.0;  // events moved up / down
+  int64_t j0, j1;
+  chain_nodes(v, m, c, &j0, &j1);
+  double *tm = const_cast<double *>(v.time) + m.node_first;
+  const int32_t *dn = v.down + m.node_first;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    const double t = tm[j];
+    m.tbackup[j] = t;
+    if (dn[j] == -1) continue;
+    if (t <= oldt && t > t_u) {
+      tm[j] = before(t);
+      nu += 1.0;
+    } else if (t > oldt && t < t_d) {
+      tm[j] = after(t);
+      nd += 1.0;
+    }
+  }
+  double *mt = const_cast<double *>(v.mig_time);
+  for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+    const SlotMeta sm = v.slots[m.first_slot + c * m.L + l];
+    for (int x = 0; x < sm.nmig; ++x) {
+      const double t = mt[sm.mig_off + x];
+      m.mtbackup[sm.mig_off + x] = t;
+      if (t <= oldt && t > t_u) {
+        mt[sm.mig_off + x] = before(t);
+        mu += 1.0;
+      } else if (t > oldt && t < t_d) {
+        mt[sm.mig_off + x] = after(t);
+        md += 1.0;
+      }
+    }
+  }
+  for (int x = threadIdx.x; x < tw; x += blockDim.x) {
+    m.tvbackup[c * tw + x] = m.tv[c * tw + x];
+    if (x == k) m.tv[c * tw + x] = newt;
+  }
+  nu = block_sum(nu, red);
+  nd = block_sum(nd, red);
+  mu = block_sum(mu, red);
+  md = block_sum(md, red);
+  if (threadIdx.x == 0) {
+    const double ecu = (double)((long long)nu / 2), ecd = (double)((long long)nd / 2);
+    const double t_u_hterm = (newt - t_u) / (oldt - t_u);
+    const double t_d_hterm = last ? 1.0 : (t_d - newt) / (t_d - oldt);
+    m.logs[c] = (ecd + md) * log(t_d_hterm) + (ecu + mu) * log(t_u_hterm);
+  }
+}
+
 // One workgroup per chain.  Accept or reject every locus' genealogy
 // proposal (update_gtree.cpp:919-927; rejection restores the moved time).
 // Rescaling sweep: back up and rescale every coalescence time of the chain
@@ -777,6 +872,10 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
     return;
   }
   __syncthreads();  // restored times visible to the whole workgroup
+  if (m.ry1 && m.prior >= 0 && m.P > 1) {
+    ry1_stretch(v, m, c, sweep, sh);
+    return;
+  }
   const double ls = chain_log_scale(m, c, sweep);
   const double s = exp(ls);
   if (threadIdx.x == 0) m.logs[c] = ls;
@@ -819,6 +918,8 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   const int c = blockIdx.x;
   const uint64_t sweep = *m.counter;
   const bool prior = m.prior >= 0;
+  // rubber band: m.logs[c] holds the whole Hastings term
+  const bool ry = m.ry1 && prior && m.P > 1;
   if (threadIdx.x == 0) okall = 1;
   __syncthreads();
   double a = 0.0;
@@ -827,7 +928,7 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
     if (m.status[it] != IMA_STATUS_OK) okall = 0;
     if (prior && m.wstatus[it] != IMA_STATUS_OK) okall = 0;
     const SlotMeta sm = v.slots[m.first_slot + it];
-    const int scaled = (sm.n - 1) + (prior ? sm.nmig : 0);
+    const int scaled = ry ? 0 : (sm.n - 1) + (prior ? sm.nmig : 0);
     a += m.beta[c] * (new_loglik(m, it) - m.pdg[it]) + scaled * m.logs[c];
   }
   a = block_sum(a, sh);
@@ -838,7 +939,7 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   double pn = 0.0;
   if (prior) {
     pn = chain_prior_fresh(v, m, c, m.wnew, psh);
-    a += m.beta[c] * (pn - m.probg[c]) + (m.P - 1) * m.logs[c];
+    a += m.beta[c] * (pn - m.probg[c]) + (ry ? 1 : m.P - 1) * m.logs[c];
     if (threadIdx.x == 0)
       for (int k = 0; k < m.P - 1; ++k)
         if (m.tv[c * tw + k] > m.tmax) okall = 0;

@@ -276,6 +276,16 @@ int ima_prior(ima_ctx *ctx, const int32_t *slots, const int32_t *group_off,
 #define IMA_MC3_GRAPH 1 /* replay each sweep as a captured hipGraph */
 #define IMA_MC3_SCALE 2 /* whole-chain rescaling on alternate sweeps */
 #define IMA_MC3_TIMED 4 /* eager launches with HIP events per scoring pass */
+/* With IMA_MC3_SCALE and the genealogy prior (two or more populations), the
+ * alternate-sweep time update is the reference's rubber-band update of one
+ * split time (changet_RY1, update_t_RY.cpp:223-517): split k drawn
+ * uniformly, its new time in getnewt's reflected window
+ * (update_gtree_common.cpp:2501-2518), node and migration times between
+ * the neighbouring split times stretched by beforesplit / aftersplit
+ * (update_t_RY.cpp:59-82), Hastings term (ecd + emd) log t_d_hterm +
+ * (ecu + emu) log t_u_hterm, every locus re-scored.  Without this flag it
+ * is a global rescale of every time by exp(w (U - 1/2)). */
+#define IMA_MC3_RY1 8
 int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
                   const double *u, const double *kappa, const double *beta,
                   uint64_t seed, int precision, int swaptries,

@@ -80,7 +80,7 @@ def _load():
                                   + [ip] * 5 + [C.c_int, ip, ip, C.c_int, ip, ip, dp, dp,
                                                 C.c_double] + [ip] * 4
                                   + [dp, ip, ip, ip, dp, ip, dp, dp, C.c_double, C.c_uint64,
-                                     C.c_double, C.c_long, C.POINTER(C.c_long)])
+                                     C.c_double, C.c_long, C.POINTER(C.c_long), C.c_int])
     return lib
 
 
@@ -410,12 +410,13 @@ def integrate_tree_prob(weights_list, model):
 
 
 def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
-              seconds=1.0, max_sweeps=0, scale_width=0.2):
+              seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False):
     """The CPU baseline's C sweep over chains[c][l] (Genealogy objects with
     population labels and migrations; updated in place) with split times
-    tvs[c] (a float array [nchains, P-1], updated in place).  Returns
-    (sweeps, stats[4] = genealogy proposals, accepts, rescale proposals,
-    accepts)."""
+    tvs[c] (a float array [nchains, P-1], updated in place).  ry1: the
+    alternate-sweep time update is changet_RY1's rubber band instead of the
+    global rescale.  Returns (sweeps, stats[4] = genealogy proposals,
+    accepts, time-update proposals, accepts)."""
     from ima2p_amd.immodel import mig_arrays
     L = len(loci)
     P = model.npops
@@ -459,7 +460,7 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
                            model.nmig, _ip(moff), _ip(mwp), _dp(qm), _dp(mm), float(tmax),
                            _ip(up0), _ip(up1), _ip(dn), _ip(pop), _dp(tm), _ip(roots), _ip(nm),
                            _ip(me), _dp(mt), _ip(mo), _dp(tv), _dp(b), float(scale_width),
-                           int(seed), float(seconds), int(max_sweeps), stats)
+                           int(seed), float(seconds), int(max_sweeps), stats, int(bool(ry1)))
     off = moff_ = 0
     for g, m in zip(gens, migs):
         ne = 2 * g.n - 1

@@ -16,7 +16,9 @@
  * (swapchains.cpp:527-662).  The proposals are the device sweep's
  * (kern_mc3.hip): a reflected normal slide of one coalescence inside its
  * period and between the migrations around it, and a global rescale by
- * exp(d (U - 1/2)).  Randomness: xoshiro256** seeded by splitmix64.
+ * exp(d (U - 1/2)) -- or, with ry1, the reference's rubber-band update of
+ * one split time (changet_RY1, update_t_RY.cpp:223-517).  Randomness:
+ * xoshiro256** seeded by splitmix64.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -77,6 +79,7 @@ typedef struct {
   const int *theta_off, *theta_wp, *mig_off, *mig_wp;
   const double *qmax, *mmax;
   double tmax, scale_width;
+  int ry1;
 } prob_t;
 
 static int rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
@@ -296,6 +299,104 @@ static void rescale(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
   free(fresh);
 }
 
+/* changet_RY1 (update_t_RY.cpp:223-517): split k = randposint(P - 1);
+ * getnewt (update_gtree_common.cpp:2501-2518): a window of width
+ * (t_d' - t_u') / (log(L + 1) (P - k)) centred on the old time, reflected
+ * at t_u' = max(0, t_u) and t_d' = min(tmax, t_d), with t_u, t_d the
+ * neighbouring split times (0, and TIMEMAX = 1e6 after the last split).
+ * Node times of edges below the root and migration times in (t_u, oldt]
+ * follow beforesplit, those in (oldt, t_d) aftersplit (update_t_RY.cpp:
+ * 59-82).  Node times sit on both daughter edges, so edge counts are
+ * halved; the Hastings term is (ecd + emd) log t_d_hterm + (ecu + emu)
+ * log t_u_hterm (:259-266, :397-406).  Every locus is re-scored; accepted
+ * when log U < min(1, beta (dpdg + dprobg) + Hastings). */
+static void ry1(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
+  const int L = pb->L, R = rec_len(pb->P), np_ = pb->npsize + pb->nmig, ns = pb->P - 1;
+  int k = (int)(unif(r) * ns);
+  if (k > ns - 1) k = ns - 1;
+  const double oldt = ch->tv[k];
+  const double t_u = k == 0 ? 0.0 : ch->tv[k - 1];
+  const double t_d = k == ns - 1 ? 1000000.0 : ch->tv[k + 1];
+  const double tdp = fmin(pb->tmax, t_d), tup = fmax(0.0, t_u);
+  const double twin = (tdp - tup) / (log((double)L + 1.0) * (double)(pb->P - k));
+  double newt = (oldt - twin / 2) + unif(r) * twin;
+  if (newt >= tdp)
+    newt = 2.0 * tdp - newt;
+  else if (newt <= tup)
+    newt = 2.0 * tup - newt;
+  double tv_new[16];
+  for (int x = 0; x < ns; x++) tv_new[x] = ch->tv[x];
+  tv_new[k] = newt;
+  long ecu = 0, ecd = 0, emu = 0, emd = 0;
+#define RY1_MOVE(t, cu, cd)                                                    \
+  do {                                                                         \
+    if ((t) <= oldt && (t) > t_u) {                                            \
+      (t) = k == 0 ? (t) * newt / oldt                                         \
+                   : t_u + ((t) - t_u) * (newt - t_u) / (oldt - t_u);          \
+      (cu)++;                                                                  \
+    } else if ((t) > oldt && (t) < t_d) {                                      \
+      (t) = k == ns - 1 ? (t) + newt - oldt                                    \
+                        : t_d - (t_d - newt) * (t_d - (t)) / (t_d - oldt);     \
+      (cd)++;                                                                  \
+    }                                                                          \
+  } while (0)
+  double a = 0.0;
+  double *newp = (double *)malloc(sizeof(double) * L);
+  double *recs = (double *)malloc(sizeof(double) * L * R);
+  size_t tb = 0, mb = 0;
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    for (int i = 0; i < 2 * g->n - 1; i++) {
+      ch->tback[tb++] = g->time[i];
+      if (g->down[i] != -1) RY1_MOVE(g->time[i], ecu, ecd);
+    }
+    for (int j = 0; j < g->nmig; j++) {
+      ch->mback[mb++] = g->mig_time[j];
+      RY1_MOVE(g->mig_time[j], emu, emd);
+    }
+    newp[l] = hky(pb, l, g, -1);
+    weights(pb, l, g, tv_new, recs + (size_t)l * R);
+    a += ch->beta * (newp[l] - g->pdg);
+  }
+#undef RY1_MOVE
+  ecu /= 2;
+  ecd /= 2;
+  double *fresh = (double *)calloc(R, sizeof(double));
+  for (int l = 0; l < L; l++)
+    for (int i = 0; i < R; i++) fresh[i] += recs[(size_t)l * R + i];
+  const double pn = prior_terms(pb, fresh, NULL, NULL, NULL, ch->nc, ch->nf, ch->nt);
+  const double t_u_hterm = (newt - t_u) / (oldt - t_u);
+  const double t_d_hterm = k == ns - 1 ? 1.0 : (t_d - newt) / (t_d - oldt);
+  a += ch->beta * (pn - ch->probg) + (double)(ecd + emd) * log(t_d_hterm) +
+       (double)(ecu + emu) * log(t_u_hterm);
+  const double lu = log(unif(r));
+  const int dec = lu < fmin(1.0, a);
+  tb = mb = 0;
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    if (dec) {
+      g->pdg = newp[l];
+      memcpy(g->rec, recs + (size_t)l * R, sizeof(double) * R);
+    } else {
+      for (int i = 0; i < 2 * g->n - 1; i++) g->time[i] = ch->tback[tb++];
+      for (int j = 0; j < g->nmig; j++) g->mig_time[j] = ch->mback[mb++];
+    }
+    hky_settle(g, dec);
+  }
+  if (dec) {
+    memcpy(ch->all, fresh, sizeof(double) * R);
+    memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+    memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+    memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+    ch->probg = pn;
+    ch->tv[k] = newt;
+    (*acc)++;
+  }
+  free(newp);
+  free(recs);
+  free(fresh);
+}
+
 /*
  * Runs sweeps over nchains chains x L loci for about `seconds` of wall
  * time, or exactly max_sweeps sweeps when max_sweeps > 0, and returns the
@@ -305,7 +406,8 @@ static void rescale(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
  * tree arrays of 2n_l-1 entries each (up0, up1, down, pop, time), roots,
  * migration counts and their (edge, time, destination) packed in order.
  * tvs[c][0..P-2] are chain c's split times; times, migration times and
- * split times are updated in place.
+ * split times are updated in place.  ry1 selects the rubber-band time
+ * update (two or more populations) in place of the global rescale.
  */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,
@@ -318,10 +420,11 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    double *times, const int *roots, const int *nmig, const int *mig_edge,
                    double *mig_time, const int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
-                   long max_sweeps, long *stats) {
+                   long max_sweeps, long *stats, int ry1_update) {
   prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi, u, kappa,
                P, npsize, nmigp, plist, npresent, popdown, pope, join,
-               theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width};
+               theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width,
+               ry1_update && P > 1};
   const int R = rec_len(P), np_ = npsize + nmigp;
   if (P > 10 || P - 1 > 16) return -1;
   chain_t *ch = (chain_t *)calloc(nchains, sizeof(chain_t));
@@ -392,7 +495,10 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
         props[0]++;
       }
       if (sweeps & 1) {
-        rescale(&pb, &ch[c], &r, &acc[1]);
+        if (pb.ry1)
+          ry1(&pb, &ch[c], &r, &acc[1]);
+        else
+          rescale(&pb, &ch[c], &r, &acc[1]);
         props[1]++;
       }
     }

@@ -74,12 +74,14 @@ def _current(e, g, slot, times):
     return h
 
 
-@pytest.mark.parametrize("prec,tol", [(L.PREC_FP64, 1e-9), (L.PREC_FP32, 2e-6)])
-def test_state_matches_oracle_after_sweeps(prec, tol):
+@pytest.mark.parametrize("prec,tol,ry1", [(L.PREC_FP64, 1e-9, False), (L.PREC_FP32, 2e-6, False),
+                                          (L.PREC_FP64, 1e-9, True)])
+def test_state_matches_oracle_after_sweeps(prec, tol, ry1):
+    """ry1: the time update is changet_RY1's rubber band on one split time."""
     C, Lc = 6, 5
     e, model, samp, locs, gens, first, u, ts = _three_pop_chains(C, Lc, 31, prec)
     p0, _ = e.mc3_read_prior()
-    e.mc3_run(40, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(40, L.MC3_GRAPH | L.MC3_SCALE | (L.MC3_RY1 if ry1 else 0))
     pdg, beta, stats, sweeps = e.mc3_read()
     probg, tv = e.mc3_read_prior()
     assert sweeps == 40 and stats[1] > 0 and stats[3] > 0
@@ -104,9 +106,10 @@ def test_state_matches_oracle_after_sweeps(prec, tol):
     e.close()
 
 
-def test_graph_equals_eager_with_prior():
+@pytest.mark.parametrize("ry1", [0, L.MC3_RY1], ids=["rescale", "rubber_band"])
+def test_graph_equals_eager_with_prior(ry1):
     out = []
-    for flags in (L.MC3_GRAPH | L.MC3_SCALE, L.MC3_SCALE):
+    for flags in (L.MC3_GRAPH | L.MC3_SCALE | ry1, L.MC3_SCALE | ry1):
         e, *_ = _three_pop_chains(4, 3, 44, L.PREC_FP64)
         e.mc3_run(25, flags)
         pdg, beta, stats, _ = e.mc3_read()

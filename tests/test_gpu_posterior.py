@@ -10,6 +10,8 @@ time -- the fraction of each sampler's draws below the quantile must agree:
 |difference| / (standard error from the independent chains of both) < 4.5.
 This is the two-sample comparison of marginal posteriors, with the
 autocorrelation inside a chain accounted for by the chain-level spread.
+Both time updates are compared: the global rescale, and changet_RY1's
+rubber band on one split time (IMA_MC3_RY1).
 """
 import math
 
@@ -42,7 +44,7 @@ def _stats(gens, tv):
     return [float(g.time[g.up0[g.root]]) for g in gens] + [float(tv[0])]
 
 
-def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
+def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
     e = Engine(0)
     mid = e.model_add(model)
     lids = []
@@ -58,10 +60,11 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
     e.mc3_setup(C, Lc, first, 2.0, 4.0, np.ones(C), seed=77, precision=L.PREC_FP64,
                 scale_width=0.3)
     e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
-    e.mc3_run(burn)
+    flags = L.MC3_GRAPH | L.MC3_SCALE | (L.MC3_RY1 if ry1 else 0)
+    e.mc3_run(burn, flags)
     out = []
     for _ in range(nsamp):
-        e.mc3_run(thin)
+        e.mc3_run(thin, flags)
         _, _, dn, tm, rt, _ = e.read_slots(first, C * Lc)
         _, tv = e.mc3_read_prior()
         row, off = [], 0
@@ -80,7 +83,7 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
     return np.array(out).transpose(1, 0, 2)  # [chain, sample, stat]
 
 
-def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
+def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
     chains = [[g.copy() for g in base] for _ in range(C)]
     tvs = np.ascontiguousarray(np.tile(ts, (C, 1)))
     k = 0
@@ -88,7 +91,7 @@ def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
     def run(n):
         nonlocal k
         oracle.mc3_bench(loci, [2.0, 2.0], 4.0, samp, model, chains, tvs, np.ones(C), 5.0,
-                         seed=90000 + k, max_sweeps=n, scale_width=0.3)
+                         seed=90000 + k, max_sweeps=n, scale_width=0.3, ry1=ry1)
         k += 1
 
     run(burn)
@@ -99,11 +102,12 @@ def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin):
     return np.array(out).transpose(1, 0, 2)
 
 
-def test_device_and_oracle_sample_the_same_posterior():
+@pytest.mark.parametrize("ry1", [False, True], ids=["rescale", "rubber_band"])
+def test_device_and_oracle_sample_the_same_posterior(ry1):
     model, samp, loci, base, ts = _setup(8)
     C, burn, nsamp, thin = 48, 400, 150, 20
-    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin)
-    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin)
+    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1)
+    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1)
     worst = 0.0
     for s in range(a.shape[2]):
         pooled = np.concatenate([a[:, :, s].ravel(), b[:, :, s].ravel()])
