@@ -81,7 +81,7 @@ def _cpu_worker(args):
     t0 = time.perf_counter()
     sweeps, _ = oracle.mc3_bench(wl.loci, wl.u, wl.kappa, wl.samppop, immodel.build(wl.poptree),
                                  chains, tvs, workloads.geometric_betas(len(chain_ids)),
-                                 wl.tmax, seed=100 + wid, seconds=seconds)
+                                 wl.tmax, seed=100 + wid, seconds=seconds, ry1=True)
     return len(chain_ids) * sweeps, time.perf_counter() - t0
 
 
@@ -136,7 +136,7 @@ def plain_fp64_path(e, loci, model, wl, sweeps, nchains):
         return e.likelihood_hky(loci[l], g, wl.u[l], wl.kappa, e=(dirty[0],) * 4)
 
     h = HostMC3(chains, workloads.geometric_betas(nchains), lik, lambda c, l, a: None, seed=7,
-                prior=EnginePrior(e, model, slots), tv=wl.tsplit, tmax=wl.tmax)
+                prior=EnginePrior(e, model, slots), tv=wl.tsplit, tmax=wl.tmax, ry1=True)
     t0 = time.perf_counter()
     for k in range(sweeps):
         h.sweep(with_scale=bool(k & 1))
@@ -188,9 +188,9 @@ def uneven_leg(sweeps=40):
     e = Engine(0)
     spec.load(e)
     group, _ = e.mc3_tune(4)
-    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     dt = time.perf_counter() - t0
     e.close()
     kinds = [k for k, _ in spec.loci]
@@ -217,9 +217,9 @@ def is_leg(sweeps=2000):
                         precision=L.PREC_FP64)
     e = Engine(0)
     spec.load(e)
-    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     dt = time.perf_counter() - t0
     e.close()
     return {"chain_steps_per_s": C * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
@@ -368,9 +368,9 @@ def main():
     setup_s = time.perf_counter() - t_setup
     # autotune the accept step's group size (the chains are left as they were)
     group, group_ms = e.mc3_tune(10)
-    e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     t0 = time.perf_counter()
-    e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE)
+    e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
     dt = time.perf_counter() - t0
     value = C * args.steps / dt
     pdg, beta, stats, sweeps = e.mc3_read()
@@ -379,7 +379,7 @@ def main():
     e.timing(True)
     e.timing_reset()
     ksweeps = max(10, min(args.steps, 100))
-    e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE)
+    e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE | L.MC3_RY1)
     sweep_n, sweep_ms = e.timing_get(L.K_MC3)
     # HKY kernel: the sweep's scoring launch (every chain x locus of the
     # chains' current genealogies) repeated back to back between one pair
@@ -397,10 +397,10 @@ def main():
     extra = {}
     if not args.no_extra:
         e64, loci64, _, model64 = load_engine(wl, L.PREC_FP64)
-        e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE)
+        e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
         t1 = time.perf_counter()
         k64 = max(10, args.steps // 4)
-        e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE)
+        e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
         extra["batched_fp64_chain_steps_per_s"] = C * k64 / (time.perf_counter() - t1)
         plain = plain_fp64_path(e64, loci64, model64, wl, sweeps=4, nchains=4)
         extra["plain_per_locus_fp64_chain_steps_per_s"] = plain
@@ -429,9 +429,9 @@ def main():
                    "genes_per_locus": n, "patterns_mean": float(S.mean()),
                    "patterns_min": int(S.min()), "patterns_max": int(S.max()),
                    "sweep": "1 genealogy proposal per (chain, locus) scored with data likelihood "
-                            "and integrated genealogy prior, loci in order + whole-chain rescale "
-                            "of node, migration and split times "
-                            "with full re-score on alternate sweeps + swap",
+                            "and integrated genealogy prior, loci in order + rubber-band update "
+                            "of one split time (changet_RY1) with full re-score of every locus "
+                            "on alternate sweeps + swap",
                    "parallelism": "all chains on one GPU"},
         "roofline": {"bound": "mfma", "pipe": "VALU FP32 (pruning is not a dense contraction)",
                      "kernel": "hky_kernel<float>", "achieved": achieved,
@@ -441,7 +441,7 @@ def main():
                      "launches_timed": launches},
         "cpu_baseline": cpu,
         "accept": {"genealogy": float(stats[1]) / max(1, stats[0]),
-                   "rescale": float(stats[3]) / max(1, stats[2]),
+                   "time_update": float(stats[3]) / max(1, stats[2]),
                    "swap": float(stats[5]) / max(1, stats[4])},
         "eager_sweep_ms": sweep_ms / max(1, sweep_n),
         "tuned": {"accept_group_loci": group,

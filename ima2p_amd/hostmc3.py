@@ -3,8 +3,10 @@
 This is the reference's control structure (qupdate, ima_main_mpi.cpp:
 1788-2047): chains are visited one by one, each locus is proposed, scored
 and accepted or rejected before the next (update_gtree.cpp:723-966);
-alternate sweeps rescale every genealogy of a chain and re-score all its
-loci; a swap closes the sweep (swapchains.cpp:527-662).  The proposals are
+alternate sweeps rescale every genealogy of a chain -- globally, or with
+``ry1`` by changet_RY1's rubber band on one split time (update_t_RY.cpp:
+223-517) -- and re-score all its loci; a swap closes the sweep
+(swapchains.cpp:527-662).  The proposals are
 the ones the device sweep uses (kern_mc3.hip), drawn here from numpy.
 
 ``lik(c, l, g, dirty, commit)`` returns the log-likelihood of genealogy g
@@ -125,7 +127,7 @@ class EnginePrior(HostPrior):
 
 class HostMC3:
     def __init__(self, chains, beta, lik, commit, seed=1, scale_width=0.2, swaptries=None,
-                 prior=None, tv=None, tmax=math.inf):
+                 prior=None, tv=None, tmax=math.inf, ry1=False):
         self.chains = chains          # chains[c][l] Genealogy (modified in place)
         self.beta = np.array(beta, float)
         self.lik = lik
@@ -134,6 +136,7 @@ class HostMC3:
         self.C = len(chains)
         self.L = len(chains[0])
         self.scale_width = scale_width
+        self.ry1 = ry1
         self.swaptries = max(1, self.C // 10) if swaptries is None else swaptries
         self.prior = prior
         self.tmax = tmax
@@ -215,12 +218,93 @@ class HostMC3:
         self.stats[2] += 1
         self.stats[3] += int(acc)
 
+    def _rubber_band(self, c):
+        """changet_RY1 (update_t_RY.cpp:223-517): split k uniform, the new
+        time from getnewt's reflected window (update_gtree_common.cpp:
+        2501-2518), times between the neighbouring splits stretched by
+        beforesplit / aftersplit (update_t_RY.cpp:59-82), Hastings term
+        (ecd + emd) log t_d_hterm + (ecu + emu) log t_u_hterm."""
+        rng = self.rng
+        gens = self.chains[c]
+        tv = self.tv[c]
+        ns = len(tv)
+        k = min(int(rng.random() * ns), ns - 1)
+        oldt = float(tv[k])
+        t_u = 0.0 if k == 0 else float(tv[k - 1])
+        t_d = 1e6 if k == ns - 1 else float(tv[k + 1])
+        tdp, tup = min(self.tmax, t_d), max(0.0, t_u)
+        twin = (tdp - tup) / (math.log(self.L + 1.0) * (ns + 1 - k))
+        newt = (oldt - twin / 2) + rng.random() * twin
+        if newt >= tdp:
+            newt = 2.0 * tdp - newt
+        elif newt <= tup:
+            newt = 2.0 * tup - newt
+
+        def move(t):
+            if t_u < t <= oldt:
+                return (t * newt / oldt if k == 0 else
+                        t_u + (t - t_u) * (newt - t_u) / (oldt - t_u)), 1, 0
+            if oldt < t < t_d:
+                return (t + newt - oldt if k == ns - 1 else
+                        t_d - (t_d - newt) * (t_d - t) / (t_d - oldt)), 0, 1
+            return t, 0, 0
+
+        old = [g.copy() for g in gens]
+        ecu = ecd = emu = emd = 0
+        newp = np.zeros(self.L)
+        a = 0.0
+        for l, g in enumerate(gens):
+            for i in range(2 * g.n - 1):
+                if g.down[i] != -1:
+                    g.time[i], u, d = move(float(g.time[i]))
+                    ecu += u
+                    ecd += d
+            mig = []
+            for m in g.mig:
+                row = []
+                for t, q in m:
+                    t2, u, d = move(t)
+                    emu += u
+                    emd += d
+                    row.append((t2, q))
+                mig.append(row)
+            g.mig = mig
+            newp[l] = self.lik(c, l, g, (-1,), False)
+            a += self.beta[c] * (newp[l] - self.pdg[c, l])
+        ecu //= 2
+        ecd //= 2
+        tv_new = tv.copy()
+        tv_new[k] = newt
+        pn = self.prior.try_chain(c, gens, tv_new)
+        t_u_hterm = (newt - t_u) / (oldt - t_u)
+        t_d_hterm = 1.0 if k == ns - 1 else (t_d - newt) / (t_d - oldt)
+        a += (self.beta[c] * (pn - self.probg[c]) + (ecd + emd) * math.log(t_d_hterm)
+              + (ecu + emu) * math.log(t_u_hterm))
+        acc = math.log(max(rng.random(), 1e-300)) < min(1.0, a)
+        for l, g in enumerate(gens):
+            if acc:
+                self.pdg[c, l] = newp[l]
+            else:
+                g.time[:] = old[l].time
+                g.mig = old[l].mig
+            self.commit(c, l, acc)
+        if acc:
+            self.probg[c] = pn
+            self.tv[c] = tv_new
+        self.prior.settle_chain(c, acc, None if acc else gens, tv_new)
+        self.stats[2] += 1
+        self.stats[3] += int(acc)
+
     def sweep(self, with_scale):
         for c in range(self.C):
             for l in range(self.L):
                 self._locus(c, l)
             if with_scale:
-                self._rescale(c)
+                if self.ry1 and self.prior is not None and self.tv[c] is not None \
+                        and len(self.tv[c]) > 0:
+                    self._rubber_band(c)
+                else:
+                    self._rescale(c)
         self.swap()
         self.sweeps += 1
 
