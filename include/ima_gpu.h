@@ -371,8 +371,10 @@ typedef struct {
   int precision, nsweeps;
 } ima_analysis;
 
-/* budget_bytes <= 0: 80% of the device's free memory.  An analysis runs in
- * quanta of sweeps: the first is one sweep, later ones about 4 ms of GPU
+/* budget_bytes <= 0: 80% of the device's free memory.  An analysis' sweeps
+ * are ima_mc3_run's with IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 (the
+ * rubber-band split-time update when it has two or more populations).  It
+ * runs in quanta of sweeps: the first is one sweep, later ones about 4 ms of GPU
  * time at its measured sweep rate, at most `quantum` sweeps (<= 0: 25).
  * Short quanta keep L-mode launches from queueing behind long runs of
  * sweeps on the hardware queues that streams share. */
