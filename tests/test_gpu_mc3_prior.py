@@ -240,3 +240,23 @@ def test_tuner_leaves_the_chains_where_they_were():
         e.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_swaps_permute_the_heats_beyond_the_lds_staged_chains():
+    """With more chains than the swap stages in LDS (256) the last chain
+    swaps on the arrays in HBM: after sweeps with swaps every beta is still
+    one of the initial ones, each used once, the swap counters advance, and
+    graph replay equals eager launches."""
+    out = []
+    for flags in (L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1, L.MC3_SCALE | L.MC3_RY1):
+        e, *_ = _three_pop_chains(300, 2, 63, L.PREC_FP32)
+        pdg0, beta0, _, _ = e.mc3_read()
+        e.mc3_run(12, flags)
+        pdg, beta, stats, sweeps = e.mc3_read()
+        assert sweeps == 12 and stats[4] == 12 * 30 and stats[5] > 0
+        np.testing.assert_array_equal(np.sort(beta), np.sort(beta0))
+        probg, tv = e.mc3_read_prior()
+        out.append((pdg, beta, probg, tv, stats))
+        e.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
