@@ -81,7 +81,8 @@ def _cpu_worker(args):
     t0 = time.perf_counter()
     sweeps, _ = oracle.mc3_bench(wl.loci, wl.u, wl.kappa, wl.samppop, immodel.build(wl.poptree),
                                  chains, tvs, workloads.geometric_betas(len(chain_ids)),
-                                 wl.tmax, seed=100 + wid, seconds=seconds, ry1=True)
+                                 wl.tmax, seed=100 + wid, seconds=seconds, ry1=True,
+                                 urate=True)
     return len(chain_ids) * sweeps, time.perf_counter() - t0
 
 
@@ -125,7 +126,9 @@ def load_engine(wl, prec, prior=True):
 def plain_fp64_path(e, loci, model, wl, sweeps, nchains):
     """One likelihood call = one upload + one FP64 launch + one readback;
     one prior call per proposal likewise (the chain's genealogies in
-    slots of their own)."""
+    slots of their own).  Its step has no mutation-rate rounds (the
+    batched step's changeu adds 2 likelihoods per locus every fifth step),
+    so it slightly flatters this path."""
     from ima2p_amd.hostmc3 import EnginePrior, HostMC3
     chains = [[g.copy() for g in wl.chains[c]] for c in range(nchains)]
     Lc = wl.nloci
@@ -188,9 +191,9 @@ def uneven_leg(sweeps=40):
     e = Engine(0)
     spec.load(e)
     group, _ = e.mc3_tune(4)
-    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     dt = time.perf_counter() - t0
     e.close()
     kinds = [k for k, _ in spec.loci]
@@ -217,9 +220,9 @@ def is_leg(sweeps=2000):
                         precision=L.PREC_FP64)
     e = Engine(0)
     spec.load(e)
-    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     dt = time.perf_counter() - t0
     e.close()
     return {"chain_steps_per_s": C * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
@@ -360,7 +363,7 @@ def main():
         rate, steps, wall = cpu_baseline(wl, args.cpu_seconds, cores)
         cpu = {"value": rate, "unit": "chain-steps/s", "cores": cores, "kind": "port",
                "sample": f"the oracle's C sweep (likelihoodHKY incremental as the reference, "
-                         f"treeweight, prior integrals, same proposals and swap): {cores} "
+                         f"treeweight, prior integrals, same proposals, changeu and swap): {cores} "
                          f"processes x 4 chains x 50 loci, {steps} chain-steps in {wall:.1f} s"}
 
     from ima2p_amd import _lib as L
@@ -368,9 +371,9 @@ def main():
     setup_s = time.perf_counter() - t_setup
     # autotune the accept step's group size (the chains are left as they were)
     group, group_ms = e.mc3_tune(10)
-    e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     t0 = time.perf_counter()
-    e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     dt = time.perf_counter() - t0
     value = C * args.steps / dt
     pdg, beta, stats, sweeps = e.mc3_read()
@@ -379,7 +382,7 @@ def main():
     e.timing(True)
     e.timing_reset()
     ksweeps = max(10, min(args.steps, 100))
-    e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     sweep_n, sweep_ms = e.timing_get(L.K_MC3)
     # HKY kernel: the sweep's scoring launch (every chain x locus of the
     # chains' current genealogies) repeated back to back between one pair
@@ -397,10 +400,10 @@ def main():
     extra = {}
     if not args.no_extra:
         e64, loci64, _, model64 = load_engine(wl, L.PREC_FP64)
-        e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+        e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
         t1 = time.perf_counter()
         k64 = max(10, args.steps // 4)
-        e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+        e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
         extra["batched_fp64_chain_steps_per_s"] = C * k64 / (time.perf_counter() - t1)
         plain = plain_fp64_path(e64, loci64, model64, wl, sweeps=4, nchains=4)
         extra["plain_per_locus_fp64_chain_steps_per_s"] = plain
@@ -431,7 +434,8 @@ def main():
                    "sweep": "1 genealogy proposal per (chain, locus) scored with data likelihood "
                             "and integrated genealogy prior, loci in order + rubber-band update "
                             "of one split time (changet_RY1) with full re-score of every locus "
-                            "on alternate sweeps + swap",
+                            "on alternate sweeps + swap; every fifth sweep also changeu: per "
+                            "chain 50 pair moves of (u, kappa), each re-scoring two loci",
                    "parallelism": "all chains on one GPU"},
         "roofline": {"bound": "mfma", "pipe": "VALU FP32 (pruning is not a dense contraction)",
                      "kernel": "hky_kernel<float>", "achieved": achieved,
@@ -442,7 +446,8 @@ def main():
         "cpu_baseline": cpu,
         "accept": {"genealogy": float(stats[1]) / max(1, stats[0]),
                    "time_update": float(stats[3]) / max(1, stats[2]),
-                   "swap": float(stats[5]) / max(1, stats[4])},
+                   "swap": float(stats[5]) / max(1, stats[4]),
+                   "rates": float(stats[7]) / max(1, stats[6])},
         "eager_sweep_ms": sweep_ms / max(1, sweep_n),
         "tuned": {"accept_group_loci": group,
                   "ms_per_sweep_by_group": [round(float(x), 4) for x in group_ms]},

@@ -21,6 +21,7 @@ REJECT_IS = -1000000000.0
 MAX_GENES = 256
 K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3 = range(7)
 MC3_GRAPH, MC3_SCALE, MC3_TIMED, MC3_RY1 = 1, 2, 4, 8
+MC3_UPDATE_U, MC3_U_SERIAL = 16, 32
 
 
 class Item(C.Structure):
@@ -109,6 +110,7 @@ PROTOS = {
                                 C.c_int, C.c_double]),
     "ima_mc3_run": (C.c_int, [vp, C.c_int, C.c_int]),
     "ima_mc3_read": (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ima_mc3_read_rates": (C.c_int, [vp, dp, dp]),
     "ima_timing_enable": (C.c_int, [vp, C.c_int]),
     "ima_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), dp]),
     "ima_timing_reset": (C.c_int, [vp]),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libima_gpu.so")
 SOURCES = ["ctx.hip", "kern_hky.hip", "kern_lik.hip", "kern_lmode.hip", "kern_swap.hip", "kern_mc3.hip",
-           "kern_prior.hip", "serve.hip"]
+           "kern_prior.hip", "kern_urate.hip", "serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

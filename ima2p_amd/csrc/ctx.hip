@@ -1034,6 +1034,12 @@ static void mc3_drop_graphs(ima_ctx *c) {
     if (m.bgraph[k]) hipGraphDestroy(m.bgraph[k]);
     m.exec[k] = m.bexec[k] = nullptr;
     m.graph[k] = m.bgraph[k] = nullptr;
+    for (int p = 0; p < 2; ++p) {
+      if (m.uexec[k][p]) hipGraphExecDestroy(m.uexec[k][p]);
+      if (m.ugraph[k][p]) hipGraphDestroy(m.ugraph[k][p]);
+      m.uexec[k][p] = nullptr;
+      m.ugraph[k][p] = nullptr;
+    }
   }
 }
 
@@ -1085,6 +1091,16 @@ static void mc3_release(ima_ctx *c) {
   release(m.tv_row);
   release(m.tw_slots);
   release(m.wstatus);
+  release(m.uitems);
+  release(m.ustep);
+  release(m.ulist0);
+  release(m.ulist1);
+  release(m.uuni);
+  release(m.uorder);
+  release(m.uloff);
+  release(m.ulast);
+  release(m.ucnt);
+  m.udev = URateDev{};
   m.dev.prior = -1;
   m.live = false;
 }
@@ -1205,6 +1221,35 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
                              c->stream));
     HIPCHK(c, reserve(m.done, 1));
     HIPCHK(c, hipMemsetAsync(m.done.p, 0, sizeof(unsigned), c->stream));
+    // mutation-rate rounds (update_mc_params.cpp:24-429): log(UMAX)/L
+    // window, reflection at 3 log(UMAX) (initialize.cpp:1448-1450,
+    // update_mc_params.cpp:52-56), kappa window 2 up to KAPPAMAX
+    // (initialize.cpp:1499-1501)
+    const int J = nloci > 2 ? nloci : 1;
+    HIPCHK(c, reserve(m.uitems, N));
+    HIPCHK(c, reserve(m.ustep, (size_t)nchains * J));
+    HIPCHK(c, reserve(m.uuni, (size_t)nchains * J));
+    HIPCHK(c, reserve(m.uorder, (size_t)nchains * J));
+    HIPCHK(c, reserve(m.uloff, (size_t)nchains * (J + 2)));
+    HIPCHK(c, reserve(m.ulast, N));
+    HIPCHK(c, reserve(m.ulist0, units.size() + 1));
+    HIPCHK(c, reserve(m.ulist1, units.size() + 1));
+    HIPCHK(c, reserve(m.ucnt, 2));
+    URateDev &u = m.udev;
+    u.J = J;
+    u.uitems = m.uitems.p;
+    u.step = m.ustep.p;
+    u.uni = m.uuni.p;
+    u.order = m.uorder.p;
+    u.loff = m.uloff.p;
+    u.last = m.ulast.p;
+    u.list[0] = m.ulist0.p;
+    u.list[1] = m.ulist1.p;
+    u.cnt = m.ucnt.p;
+    u.win = log(10000.0) / nloci;
+    u.maxr = 3.0 * log(10000.0);
+    u.kwin = 2.0;
+    u.kmax = 100.0;
   }
   HIPCHK(c, hipMemcpyAsync(m.beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
   const uint64_t zero = 0;
@@ -1277,9 +1322,40 @@ static int mc3_enqueue(ima_ctx *c, bool with_scale, hipEvent_t *score_ev) {
 
 }  // extern "C"
 
+// The mutation-rate round after the sweep whose counter is ctr - 1: the
+// host draws each chain's partners from the same streams as the device to
+// know how many levels to launch (the deepest chain's).
+static int mc3_urate_round(ima_ctx *c, uint64_t ctr, bool serial) {
+  auto &m = c->mc3;
+  const int L = m.L, J = m.udev.J;
+  int nlev = 1;
+  if (serial) {
+    nlev = J;
+  } else {
+    std::vector<int32_t> &last = m.ulevel_scratch;
+    last.assign((size_t)L, 0);
+    for (int ch = 0; ch < m.C; ++ch) {
+      std::fill(last.begin(), last.end(), 0);
+      for (int s = 0; s < J; ++s) {
+        PhiloxStream r = urate_rng(m.dev.seed, ch, L, s, ctr);
+        const int k = urate_partner(r, s, L);
+        const int lev = 1 + std::max(last[s], k >= 0 ? last[k] : 0);
+        last[s] = lev;
+        if (k >= 0) last[k] = lev;
+        nlev = std::max(nlev, lev);
+      }
+    }
+  }
+  HIPCHK(c, enqueue_urate_round(view(c), m.dev, m.udev, m.items.p, m.idx.p, m.plan, m.nh, m.ni,
+                                m.ns, m.precision, m.hmax_n, m.hmax_slots, m.imax_n, m.imax_w,
+                                ctr, nlev, serial ? 1 : 0, c->stream));
+  return IMA_OK;
+}
+
 // Enqueues nsweeps sweeps on the context's stream without waiting (timed
 // runs synchronise per sweep).  The sweep counter's host copy decides which
-// sweeps rescale, so nothing is read back from the device.
+// sweeps rescale and which end with a mutation-rate round, so nothing is
+// read back from the device.
 int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   if (!c || nsweeps < 0) return IMA_ERR_ARG;
   auto &m = c->mc3;
@@ -1289,6 +1365,10 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   const bool use_graph = (flags & IMA_MC3_GRAPH) != 0;
   const bool scale = (flags & IMA_MC3_SCALE) != 0;
   const bool timed = (flags & IMA_MC3_TIMED) != 0;
+  const bool userial = (flags & IMA_MC3_U_SERIAL) != 0;
+  // changeu needs two loci; changekappa a lone HKY locus (every item HKY)
+  const bool urate = (flags & IMA_MC3_UPDATE_U) && (m.L >= 2 || m.nh == m.C * m.L);
+  auto u_due = [&](uint64_t q) { return urate && q % 5 == 4; };
   const int ry1 = (flags & IMA_MC3_RY1) ? 1 : 0;
   if (ry1 != m.dev.ry1) {
     mc3_drop_graphs(c);  // the graphs hold the sweep's parameters
@@ -1296,71 +1376,105 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   }
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
   const int bk = scale ? 1 : 0;
+  // n sweeps from one of the given parity; odd sweeps rescale when sc
+  auto capture = [&](hipGraphExec_t *exec, hipGraph_t *graph, int n, int parity,
+                     bool sc) -> int {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = IMA_OK;
+    for (int s = 0; s < n && r == IMA_OK; ++s)
+      r = mc3_enqueue(c, sc && ((parity + s) & 1), nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != IMA_OK) return r;
+    HIPCHK(c, e);
+    *graph = g;
+    HIPCHK(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return IMA_OK;
+  };
   if (use_graph && !timed) {
-    for (int k = 0; k < 2; ++k) {
-      if (m.exec[k]) continue;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = mc3_enqueue(c, k == 1, nullptr);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (rc != IMA_OK) return rc;
-      HIPCHK(c, e);
-      m.graph[k] = g;
-      HIPCHK(c, hipGraphInstantiate(&m.exec[k], g, nullptr, nullptr, 0));
-    }
-    if (!m.bexec[bk] && nsweeps >= kSweepBlock) {
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = IMA_OK;
-      for (int s = 0; s < kSweepBlock && rc == IMA_OK; ++s)
-        rc = mc3_enqueue(c, scale && (s & 1), nullptr);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (rc != IMA_OK) return rc;
-      HIPCHK(c, e);
-      m.bgraph[bk] = g;
-      HIPCHK(c, hipGraphInstantiate(&m.bexec[bk], g, nullptr, nullptr, 0));
-    }
+    for (int k = 0; k < 2; ++k)
+      if (!m.exec[k] && (rc = capture(&m.exec[k], &m.graph[k], 1, k, true)) != IMA_OK) return rc;
+    if (!urate && !m.bexec[bk] && nsweeps >= kSweepBlock &&
+        (rc = capture(&m.bexec[bk], &m.bgraph[bk], kSweepBlock, 0, scale)) != IMA_OK)
+      return rc;
+    if (urate && nsweeps >= kURateBlock)
+      for (int p = 0; p < 2; ++p)
+        if (!m.uexec[bk][p] &&
+            (rc = capture(&m.uexec[bk][p], &m.ugraph[bk][p], kURateBlock, p, scale)) != IMA_OK)
+          return rc;
   }
   const uint64_t done = m.host_sweeps;
-  if (nsweeps > 0) HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
-  for (int s = 0; s < nsweeps; ++s) {
-    const bool with_scale = scale && ((done + s) & 1);
-    if (use_graph && !timed && m.bexec[bk] && ((done + s) & 1) == 0 &&
-        nsweeps - s >= kSweepBlock) {
-      // a block of sweeps in one launch
-      HIPCHK(c, hipGraphLaunch(m.bexec[bk], c->stream));
-      m.scale_sweeps += scale ? kSweepBlock / 2 : 0;
-      s += kSweepBlock - 1;
-      continue;
+  bool pending = false;  // the next sweep's genealogy proposals are in place
+  for (int s = 0; s < nsweeps;) {
+    const uint64_t q = done + s;
+    if (!pending) {
+      HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
+      pending = true;
     }
-    m.scale_sweeps += with_scale ? 1 : 0;
-    if (timed) {
-      // events bracket the whole sweep and each batched scoring launch
-      Timing &T = c->timing;
-      HIPCHK(c, hipEventRecord(T.ev0[K_MC3], c->stream));
-      if ((rc = mc3_enqueue(c, with_scale, m.ev)) != IMA_OK) return rc;
-      HIPCHK(c, hipEventRecord(T.ev1[K_MC3], c->stream));
-      HIPCHK(c, hipEventSynchronize(T.ev1[K_MC3]));
-      float ms = 0;
-      HIPCHK(c, hipEventElapsedTime(&ms, T.ev0[K_MC3], T.ev1[K_MC3]));
-      T.ms[K_MC3] += ms;
-      T.launches[K_MC3] += 1;
-      const int ks = m.precision == IMA_PREC_FP32 ? K_HKY32 : K_HKY64;
-      HIPCHK(c, hipEventElapsedTime(&ms, m.ev[0], m.ev[1]));
-      T.ms[ks] += ms;
-      T.launches[ks] += 1;
-      if (with_scale) {
-        HIPCHK(c, hipEventElapsedTime(&ms, m.ev[2], m.ev[3]));
+    int ran = 0;
+    if (use_graph && !timed) {
+      if (!urate && m.bexec[bk] && (q & 1) == 0 && nsweeps - s >= kSweepBlock) {
+        // a block of sweeps in one launch
+        HIPCHK(c, hipGraphLaunch(m.bexec[bk], c->stream));
+        m.scale_sweeps += scale ? kSweepBlock / 2 : 0;
+        ran = kSweepBlock;
+      } else if (urate && m.uexec[bk][q & 1] && nsweeps - s >= kURateBlock) {
+        bool clear = true;  // no round after any but the block's last sweep
+        for (int i = 0; i + 1 < kURateBlock; ++i) clear = clear && !u_due(q + i);
+        if (clear) {
+          HIPCHK(c, hipGraphLaunch(m.uexec[bk][q & 1], c->stream));
+          for (int i = 0; i < kURateBlock; ++i) m.scale_sweeps += (scale && ((q + i) & 1)) ? 1 : 0;
+          ran = kURateBlock;
+        }
+      }
+    }
+    if (!ran) {
+      ran = 1;
+      const bool with_scale = scale && (q & 1);
+      m.scale_sweeps += with_scale ? 1 : 0;
+      if (timed) {
+        // events bracket the whole sweep (and its mutation-rate round) and
+        // each batched scoring launch
+        Timing &T = c->timing;
+        HIPCHK(c, hipEventRecord(T.ev0[K_MC3], c->stream));
+        if ((rc = mc3_enqueue(c, with_scale, m.ev)) != IMA_OK) return rc;
+        if (u_due(q)) {
+          HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+          pending = false;
+          if ((rc = mc3_urate_round(c, q + 1, userial)) != IMA_OK) return rc;
+        }
+        HIPCHK(c, hipEventRecord(T.ev1[K_MC3], c->stream));
+        HIPCHK(c, hipEventSynchronize(T.ev1[K_MC3]));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, T.ev0[K_MC3], T.ev1[K_MC3]));
+        T.ms[K_MC3] += ms;
+        T.launches[K_MC3] += 1;
+        const int ks = m.precision == IMA_PREC_FP32 ? K_HKY32 : K_HKY64;
+        HIPCHK(c, hipEventElapsedTime(&ms, m.ev[0], m.ev[1]));
         T.ms[ks] += ms;
         T.launches[ks] += 1;
+        if (with_scale) {
+          HIPCHK(c, hipEventElapsedTime(&ms, m.ev[2], m.ev[3]));
+          T.ms[ks] += ms;
+          T.launches[ks] += 1;
+        }
+      } else if (use_graph) {
+        HIPCHK(c, hipGraphLaunch(m.exec[with_scale ? 1 : 0], c->stream));
+      } else {
+        if ((rc = mc3_enqueue(c, with_scale, nullptr)) != IMA_OK) return rc;
       }
-    } else if (use_graph) {
-      HIPCHK(c, hipGraphLaunch(m.exec[with_scale ? 1 : 0], c->stream));
-    } else {
-      if ((rc = mc3_enqueue(c, with_scale, nullptr)) != IMA_OK) return rc;
+    }
+    s += ran;
+    const uint64_t qlast = done + s - 1;
+    if (pending && u_due(qlast)) {
+      // the round scores the committed genealogies: take the next sweep's
+      // proposals back, and make them again (the same ones) afterwards
+      HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+      pending = false;
+      if ((rc = mc3_urate_round(c, qlast + 1, userial)) != IMA_OK) return rc;
     }
   }
-  if (nsweeps > 0) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+  if (pending) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
   m.host_sweeps += (uint64_t)nsweeps;
   return IMA_OK;
 }
@@ -1371,6 +1485,21 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
   int rc = mc3_run_async(c, nsweeps, flags);
   if (rc != IMA_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IMA_OK;
+}
+
+int ima_mc3_read_rates(ima_ctx *c, double *u, double *kappa) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  hipSetDevice(c->device);
+  const size_t N = (size_t)m.C * m.L;
+  std::vector<ima_item> it(N);
+  HIPCHK(c, copy_now(c, it.data(), m.items.p, N * sizeof(ima_item), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < N; ++i) {
+    if (u) u[i] = it[i].u;
+    if (kappa) kappa[i] = it[i].kappa;
+  }
   return IMA_OK;
 }
 

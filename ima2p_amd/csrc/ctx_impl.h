@@ -21,6 +21,7 @@ using ima::SlotMeta;
 using ima::ProgOp;
 using ima::ModelMeta;
 using ima::MC3Dev;
+using ima::URateDev;
 using ima::HkyPlan;
 using ima::Timing;
 
@@ -124,6 +125,16 @@ struct ima_ctx {
     hipGraph_t bgraph[2]{nullptr, nullptr};
     hipEvent_t ev[4]{};
     hipEvent_t fj[4]{};  // fork / join of the side stream, per scoring pass
+    // mutation-rate rounds: kURateBlock sweeps in one graph, by rescale and
+    // by the parity of the first sweep
+    hipGraphExec_t uexec[2][2]{{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraph_t ugraph[2][2]{{nullptr, nullptr}, {nullptr, nullptr}};
+    URateDev udev{};
+    DevBuf<ima_item> uitems;
+    DevBuf<int4> ustep, ulist0, ulist1;
+    DevBuf<double4> uuni;
+    DevBuf<int32_t> uorder, uloff, ulast, ucnt;
+    std::vector<int32_t> ulevel_scratch;  // host copy of the level bookkeeping
   } mc3;
 };
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/ima_gpu.h"
+#include "philox.h"
 
 namespace ima {
 
@@ -315,6 +316,58 @@ hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
 constexpr int kSweepBlock = 8;  // sweeps captured in one graph
+constexpr int kURateBlock = 4;  // sweeps per graph when mutation-rate rounds interleave
+
+// Mutation-rate round (changeu, update_mc_params.cpp:24-382; changekappa
+// :387-429 for a lone HKY locus).  Each chain runs J steps; step s moves
+// the rates of loci s and k_s (k_s drawn first in the step's own stream).
+// Steps are grouped into levels: a step's level is one more than the
+// highest level of the earlier steps touching either of its loci, so the
+// steps of one level touch disjoint loci and each reads the rates the
+// sequential loop would have left.
+struct URateDev {
+  int J;                // steps per chain
+  ima_item *uitems;     // [C*L] the items with the proposed rates
+  int4 *step;           // [C*J] {k (-1: kappa only), level, 0, 0}
+  double4 *uni;         // [C*J] uniforms {window, kappa of s, kappa of k, accept}
+  int32_t *order;       // [C*J] each chain's steps sorted by level
+  int32_t *loff;        // [C*(J+2)] start of each level in order[] (levels from 1)
+  int32_t *last;        // [C*L] plan scratch: highest level touching each locus
+  int4 *list[2];        // HKY units of a level's proposals, by level parity
+  int32_t *cnt;         // [2] units in list[0], list[1]
+  double win, maxr;     // window and reflection bound of log(u_j / u_k)
+  double kwin, kmax;    // kappa window and upper bound
+};
+
+// Stream of step s of chain c in the round whose sweep counter is ctr (the
+// same on host and device: the host counts the levels from it).
+__host__ __device__ inline PhiloxStream urate_rng(uint64_t seed, int c, int L, int s,
+                                                  uint64_t ctr) {
+  const uint64_t item = (uint64_t)c * (uint64_t)L + (uint64_t)s;
+  return PhiloxStream(seed ^ (0x9E3779B97F4A7C15ull * (item + 1)), ((ctr << 3) | 5u) << 16);
+}
+
+// k of step s (j = s): uniform over the other loci; with two loci only
+// step 0 runs and k = 1; one locus: -1 (kappa only).
+__host__ __device__ inline int urate_partner(PhiloxStream &r, int s, int L) {
+  if (L == 1) return -1;
+  if (L == 2) return 1;
+  int k;
+  do {
+    k = (int)(r.uniform() * L);
+  } while (k == s || k < 0 || k >= L);
+  return k;
+}
+
+hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URateDev &u,
+                               const ima_item *items, const int32_t *idx, const HkyPlan &plan,
+                               int nh, int ni, int ns, int precision, int hmax_n,
+                               int hmax_slots, int imax_n, int imax_w, uint64_t ctr,
+                               int nlev, int serial, hipStream_t st);
+hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int4 *list,
+                           const int32_t *cnt, int32_t *cnt_next, int grid, int precision,
+                           int max_n, int max_slots, double *partial, double *out,
+                           int32_t *status, hipStream_t st);
 constexpr double kTimeMax = 1000000.0;  // the reference's TIMEMAX (imamp.hpp:129)
 constexpr int kSwapLds = 256;  // chains whose S and beta the swap stages in LDS
 hipError_t enqueue_mc3_revert(const DeviceView &v, const MC3Dev &m, hipStream_t st);

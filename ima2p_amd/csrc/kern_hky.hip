@@ -344,6 +344,23 @@ __global__ __launch_bounds__(kWave) void hky_kernel(
   hky_unit<real, D>(v, items, units[blockIdx.x], partial, out, status, smem);
 }
 
+// The mutation-rate round's scoring: the pattern-block units of one
+// level's proposals, listed by the previous launch (*cnt of them).  Block 0
+// clears the other list's counter for the next level.
+template <typename real, int D>
+__global__ __launch_bounds__(kWave) void hky_list_kernel(
+    DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ list,
+    const int32_t *__restrict__ cnt, int32_t *__restrict__ cnt_next,
+    double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *cnt_next = 0;
+  const int n = *cnt;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    wave_lds_fence();
+    hky_unit<real, D>(v, items, list[k], partial, out, status, smem);
+  }
+}
+
 // The sweep's scoring in one launch: workgroups [0, tw.n) weigh the
 // proposals' genealogies (treeweight), the rest run the likelihood units.
 // One launch instead of two side by side on two streams saves the
@@ -402,6 +419,37 @@ static void launch_depth_tw(int D, dim3 g, size_t lds, hipStream_t st, const Dev
   else
     hipLaunchKernelGGL((hky_tw_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
                        p.partial, out, status, tw);
+}
+
+template <typename real>
+static void launch_list_depth(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
+                              const ima_item *items, const int4 *list, const int32_t *cnt,
+                              int32_t *cnt_next, double *partial, double *out,
+                              int32_t *status) {
+  if (D <= 4)
+    hipLaunchKernelGGL((hky_list_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, list, cnt,
+                       cnt_next, partial, out, status);
+  else if (D <= 6)
+    hipLaunchKernelGGL((hky_list_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, list, cnt,
+                       cnt_next, partial, out, status);
+  else
+    hipLaunchKernelGGL((hky_list_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, list, cnt,
+                       cnt_next, partial, out, status);
+}
+
+hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int4 *list,
+                           const int32_t *cnt, int32_t *cnt_next, int grid, int precision,
+                           int max_n, int max_slots, double *partial, double *out,
+                           int32_t *status, hipStream_t st) {
+  if (grid <= 0 || max_slots > 8) return hipErrorInvalidValue;
+  const size_t lds = hky_lds_bytes(precision, max_n);
+  if (precision == IMA_PREC_FP32)
+    launch_list_depth<float>(max_slots, dim3(grid), lds, st, v, items, list, cnt, cnt_next,
+                             partial, out, status);
+  else
+    launch_list_depth<double>(max_slots, dim3(grid), lds, st, v, items, list, cnt, cnt_next,
+                              partial, out, status);
+  return hipGetLastError();
 }
 
 size_t hky_tw_lds_bytes(int precision, int max_n, int P, int emax) {

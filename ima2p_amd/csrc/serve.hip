@@ -237,7 +237,7 @@ int run_quantum(ima_server *s, Req &r) {
     q = (int)std::max(1.0, std::min((double)s->quantum, s->quantum_ms / r.sweep_ms));
   q = std::min(q, r.remaining);
   if (hipEventRecord(r.ev0, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
-  STRY(mc3_run_async(r.ctx, q, IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1));
+  STRY(mc3_run_async(r.ctx, q, IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 | IMA_MC3_UPDATE_U));
   if (hipEventRecord(r.ev, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
   r.q_last = q;
   r.remaining -= q;

@@ -390,6 +390,15 @@ class Engine:
                                         ms.ctypes.data_as(L.dp)), "mc3_tune")
         return best.value, ms
 
+    def mc3_read_rates(self):
+        """Each chain's mutation-rate scalars and kappa, [nchains, nloci] each."""
+        C_, L_ = self.mc3_shape
+        u = np.zeros(C_ * L_)
+        k = np.zeros(C_ * L_)
+        self._chk(self.lib.ima_mc3_read_rates(self.h, u.ctypes.data_as(L.dp),
+                                              k.ctypes.data_as(L.dp)), "mc3_read_rates")
+        return u.reshape(C_, L_), k.reshape(C_, L_)
+
     def mc3_read_prior(self):
         C_, _ = self.mc3_shape
         P = self.mc3_P

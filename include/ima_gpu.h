@@ -286,15 +286,34 @@ int ima_prior(ima_ctx *ctx, const int32_t *slots, const int32_t *group_off,
  * (ecu + emu) log t_u_hterm, every locus re-scored.  Without this flag it
  * is a global rescale of every time by exp(w (U - 1/2)). */
 #define IMA_MC3_RY1 8
+/* Mutation rates and kappa (changeu, update_mc_params.cpp:24-382, as qupdate
+ * calls it every fifth step, ima_main_mpi.cpp:1917-1945): the sweep whose
+ * counter q has q % 5 == 4 is followed by a round in which every chain, for
+ * j = 0 .. nloci-1 (only j = 0 with two loci), picks k != j uniformly,
+ * moves log(u_j/u_k) by a uniform window of log(10000)/nloci reflected at
+ * +-3 log(10000), sets u_j *= d, u_k /= d (d = exp(half the move)), moves
+ * each HKY locus' kappa (window 2, reflected at 0 and 100), re-scores both
+ * loci and accepts with U < min(1, exp(beta sum(new - old))).  A lone HKY
+ * locus gets changekappa (:387-429) instead.  Steps touching disjoint loci
+ * are scored together; the result is the sequential loop's. */
+#define IMA_MC3_UPDATE_U 16
+/* With IMA_MC3_UPDATE_U: one step per scoring launch, in the loop's order
+ * (the same result, for checking the grouped schedule). */
+#define IMA_MC3_U_SERIAL 32
 int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
                   const double *u, const double *kappa, const double *beta,
                   uint64_t seed, int precision, int swaptries,
                   double scale_width);
 int ima_mc3_run(ima_ctx *ctx, int nsweeps, int flags);
 /* pdg[nchains*nloci], beta[nchains]; stats[8] = genealogy proposals,
- * accepts, rescale proposals, accepts, swap tries, swaps; sweeps done. */
+ * accepts, rescale proposals, accepts, swap tries, swaps, mutation-rate
+ * steps, accepts; sweeps done. */
 int ima_mc3_read(ima_ctx *ctx, double *pdg, double *beta, uint64_t *stats,
                  uint64_t *sweeps);
+/* The chains' current mutation-rate scalars and kappa, u[nchains*nloci] and
+ * kappa[nchains*nloci] (either may be NULL); the device copy of
+ * C[ci]->G[li].uvals[0] and kappaval (imamp.hpp:967-968). */
+int ima_mc3_read_rates(ima_ctx *ctx, double *u, double *kappa);
 
 /* Adds the genealogy prior to the sweep: every MC3 slot's locus has a
  * population model (ima_locus_set_model) and its population labels and
@@ -372,8 +391,9 @@ typedef struct {
 } ima_analysis;
 
 /* budget_bytes <= 0: 80% of the device's free memory.  An analysis' sweeps
- * are ima_mc3_run's with IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 (the
- * rubber-band split-time update when it has two or more populations).  It
+ * are ima_mc3_run's with IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 |
+ * IMA_MC3_UPDATE_U (the rubber-band split-time update when it has two or
+ * more populations, the mutation-rate rounds every fifth sweep).  It
  * runs in quanta of sweeps: the first is one sweep, later ones about 4 ms of GPU
  * time at its measured sweep rate, at most `quantum` sweeps (<= 0: 25).
  * Short quanta keep L-mode launches from queueing behind long runs of

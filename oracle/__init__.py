@@ -76,11 +76,12 @@ def _load():
                                    C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_int, C.c_double]
     lib.ora_mc3_bench.restype = C.c_long
-    lib.ora_mc3_bench.argtypes = ([C.c_int, C.c_int] + [ip] * 7 + [dp, dp, C.c_double, ip, C.c_int]
+    lib.ora_mc3_bench.argtypes = ([C.c_int, C.c_int] + [ip] * 7 + [dp, dp, dp, ip, C.c_int]
                                   + [ip] * 5 + [C.c_int, ip, ip, C.c_int, ip, ip, dp, dp,
                                                 C.c_double] + [ip] * 4
                                   + [dp, ip, ip, ip, dp, ip, dp, dp, C.c_double, C.c_uint64,
-                                     C.c_double, C.c_long, C.POINTER(C.c_long), C.c_int])
+                                     C.c_double, C.c_long, C.POINTER(C.c_long), C.c_int,
+                                     C.c_int])
     return lib
 
 
@@ -410,13 +411,18 @@ def integrate_tree_prob(weights_list, model):
 
 
 def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
-              seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False):
+              seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False, urate=False,
+              rates=None):
     """The CPU baseline's C sweep over chains[c][l] (Genealogy objects with
     population labels and migrations; updated in place) with split times
     tvs[c] (a float array [nchains, P-1], updated in place).  ry1: the
     alternate-sweep time update is changet_RY1's rubber band instead of the
-    global rescale.  Returns (sweeps, stats[4] = genealogy proposals,
-    accepts, time-update proposals, accepts)."""
+    global rescale.  urate: changeu after every fifth sweep.  u (per locus
+    or [nchains, nloci]) and kappa (scalar or [nchains, nloci]) start each
+    chain; rates=(u, kappa), two float64 [nchains, nloci] arrays, take their
+    place and receive the final values.  Returns (sweeps, stats[6] =
+    genealogy proposals, accepts, time-update proposals, accepts,
+    mutation-rate steps, accepts)."""
     from ima2p_amd.immodel import mig_arrays
     L = len(loci)
     P = model.npops
@@ -428,7 +434,14 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     mult_off = np.cumsum([0] + [len(lc.mult) for lc in loci])[:-1].astype(np.int32)
     tot = np.array([lc.totsites for lc in loci], np.int32)
     pi = np.concatenate([np.asarray(lc.pi, float) for lc in loci])
-    uu = np.ascontiguousarray(u, float)
+    nch = len(chains)
+    if rates is None:
+        uu = np.ascontiguousarray(np.broadcast_to(np.asarray(u, float), (nch, L)), float)
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(kappa, float), (nch, L)), float)
+    else:
+        uu, kk = rates
+        assert all(a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (nch, L)
+                   for a in (uu, kk))
     sp = np.ascontiguousarray(np.broadcast_to(np.asarray(samppop, np.int32), (L, P)), np.int32)
     toff = np.cumsum([0] + [len(w) for w in model.theta_wp]).astype(np.int32)
     twp = np.array([x for w in model.theta_wp for kv in w for x in kv] or [0], np.int32)
@@ -453,14 +466,15 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     mm = np.ascontiguousarray(model.mmax if model.nmig else [1.0], float)
     arrs = [np.ascontiguousarray(a, np.int32) for a in
             (model.plist, model.npresent, model.popdown, model.pope, model.join)]
-    stats = (C.c_long * 4)()
+    stats = (C.c_long * 6)()
     sw = lib.ora_mc3_bench(len(chains), L, _ip(n_l), _ip(S), _ip(seq), _ip(seq_off), _ip(mult),
-                           _ip(mult_off), _ip(tot), _dp(pi), _dp(uu), float(kappa), _ip(sp), P,
+                           _ip(mult_off), _ip(tot), _dp(pi), _dp(uu), _dp(kk), _ip(sp), P,
                            *[_ip(a) for a in arrs], model.npsize, _ip(toff), _ip(twp),
                            model.nmig, _ip(moff), _ip(mwp), _dp(qm), _dp(mm), float(tmax),
                            _ip(up0), _ip(up1), _ip(dn), _ip(pop), _dp(tm), _ip(roots), _ip(nm),
                            _ip(me), _dp(mt), _ip(mo), _dp(tv), _dp(b), float(scale_width),
-                           int(seed), float(seconds), int(max_sweeps), stats, int(bool(ry1)))
+                           int(seed), float(seconds), int(max_sweeps), stats, int(bool(ry1)),
+                           int(bool(urate)))
     off = moff_ = 0
     for g, m in zip(gens, migs):
         ne = 2 * g.n - 1

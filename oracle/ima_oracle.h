@@ -150,7 +150,7 @@ int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_e
 /* ---------------- host MC3 sweep of the CPU baseline (ora_mc3.c) -------- */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,
-                   const int *totsites, const double *pi, const double *u, double kappa,
+                   const int *totsites, const double *pi, double *u, double *kappa,
                    const int *samppop, int P, const int *plist, const int *npresent,
                    const int *popdown, const int *pope, const int *join, int npsize,
                    const int *theta_off, const int *theta_wp, int nmigp, const int *mig_off,
@@ -159,7 +159,7 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    double *times, const int *roots, const int *nmig, const int *mig_edge,
                    double *mig_time, const int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
-                   long max_sweeps, long *stats, int ry1);
+                   long max_sweeps, long *stats, int ry1, int urate);
 
 #ifdef __cplusplus
 }

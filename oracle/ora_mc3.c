@@ -62,6 +62,7 @@ typedef struct { /* one genealogy of one chain */
   int nmig, *mig_edge, *mig_to;
   double *mig_time;
   ora_hky_state st;
+  double u, kappa; /* mutation-rate scalar and kappa of this chain's locus */
   double pdg;
   double *rec; /* weight record (layout of the device's records) */
 } gene_t;
@@ -71,8 +72,7 @@ typedef struct {
   int L;
   const int *S, *totsites, *seq_off, *mult_off, *samppop;
   const int *seq, *mult;
-  const double *pi, *u;
-  double kappa;
+  const double *pi;
   /* model */
   int P, npsize, nmig;
   const int *plist, *npresent, *popdown, *pope, *join;
@@ -152,7 +152,7 @@ static double prior_terms(const prob_t *pb, const double *all, const int *qc, co
 static double hky(const prob_t *pb, int l, gene_t *g, int e) {
   ora_tree t = tree_of(g);
   return ora_likelihoodHKY(&t, pb->S[l], pb->seq + pb->seq_off[l], pb->mult + pb->mult_off[l],
-                           pb->totsites[l], pb->pi + 4 * l, pb->u[l], pb->kappa, e, e, e, e,
+                           pb->totsites[l], pb->pi + 4 * l, g->u, g->kappa, e, e, e, e,
                            &g->st);
 }
 
@@ -397,12 +397,99 @@ static void ry1(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
   free(fresh);
 }
 
+/* kappa move of an HKY locus (update_mc_params.cpp:258-273, :394-404) */
+static double kappa_move(double k, double U) {
+  const double win = 2.0, kmax = 100.0; /* initialize.cpp:1499-1501, KAPPAMAX */
+  double nk;
+  if (U > 0.5) {
+    nk = k + (2.0 * U - 1.0) * win;
+    if (nk > kmax) nk = 2.0 * kmax - nk;
+  } else {
+    nk = k - win * U * 2.0;
+    if (nk < 0) nk = -nk;
+  }
+  return nk;
+}
+
+/* changeu (update_mc_params.cpp:24-382) over one chain as qupdate runs it
+ * (ima_main_mpi.cpp:1917-1945): for j = 0 .. L-1 (only j = 0 when L = 2)
+ * k is uniform over the other loci, log(u_j/u_k) moves by a uniform window
+ * of log(UMAX)/L (initialize.cpp:1448-1450) reflected at 3 log(UMAX)
+ * (:52), u_j *= d and u_k /= d with d = exp(half the move), each locus'
+ * kappa moves, both loci are re-scored and the pair is accepted with
+ * U < min(1, exp(beta sum(new - old))).  One locus: changekappa (:387-429),
+ * accepted when the ratio is >= 1 or > U.  Every locus here is HKY. */
+static void change_rates(const prob_t *pb, chain_t *ch, rng_t *r, long *tries, long *acc) {
+  const int L = pb->L;
+  if (L == 1) {
+    gene_t *g = &ch->g[0];
+    const double ok = g->kappa;
+    g->kappa = kappa_move(ok, unif(r));
+    const double np = hky(pb, 0, g, -1);
+    const double mh = exp(ch->beta * (np - g->pdg));
+    (*tries)++;
+    if (mh >= 1.0 || mh > unif(r)) {
+      g->pdg = np;
+      hky_settle(g, 1);
+      (*acc)++;
+    } else {
+      g->kappa = ok;
+      hky_settle(g, 0);
+    }
+    return;
+  }
+  const double UMAX = 10000.0;
+  const double win = log(UMAX) / L, maxr = 3.0 * log(UMAX);
+  const int J = L > 2 ? L : 1;
+  for (int j = 0; j < J; j++) {
+    int k = 1;
+    if (L > 2) {
+      do k = (int)(unif(r) * L);
+      while (k == j || k < 0 || k >= L);
+    }
+    gene_t *gl[2] = {&ch->g[j], &ch->g[k]};
+    const int li[2] = {j, k};
+    const double ou[2] = {gl[0]->u, gl[1]->u}, ok[2] = {gl[0]->kappa, gl[1]->kappa};
+    const double rr = log(ou[0] / ou[1]);
+    const double U = unif(r);
+    double nr = U > 0.5 ? rr + (2.0 * U - 1.0) * win : rr - win * U * 2.0;
+    if (nr > maxr)
+      nr = 2.0 * maxr - nr;
+    else if (nr < -maxr)
+      nr = 2.0 * (-maxr) - nr;
+    const double d = exp((nr - rr) / 2);
+    gl[0]->u *= d;
+    gl[1]->u /= d;
+    double np[2], sum = 0.0;
+    for (int i = 0; i < 2; i++) {
+      gl[i]->kappa = kappa_move(gl[i]->kappa, unif(r));
+      np[i] = hky(pb, li[i], gl[i], -1);
+      sum += np[i] - gl[i]->pdg;
+    }
+    const double mh = exp(ch->beta * sum);
+    (*tries)++;
+    const int a = unif(r) < fmin(1.0, mh);
+    for (int i = 0; i < 2; i++) {
+      if (a) {
+        gl[i]->pdg = np[i];
+      } else {
+        gl[i]->u = ou[i];
+        gl[i]->kappa = ok[i];
+      }
+      hky_settle(gl[i], a);
+    }
+    *acc += a;
+  }
+}
+
 /*
  * Runs sweeps over nchains chains x L loci for about `seconds` of wall
  * time, or exactly max_sweeps sweeps when max_sweeps > 0, and returns the
  * number of sweeps done; stats
- * (may be NULL, 4 entries) receives genealogy proposals, accepts, rescale
- * proposals and accepts.  Genealogies are packed chain-major, locus-minor:
+ * (may be NULL, 6 entries) receives genealogy proposals, accepts, rescale
+ * proposals, accepts, mutation-rate steps and accepts.  u and kappa are
+ * per (chain, locus), read and updated in place; urate adds changeu after
+ * every sweep s with s % 5 == 4.  Genealogies are packed chain-major, locus-minor:
  * tree arrays of 2n_l-1 entries each (up0, up1, down, pop, time), roots,
  * migration counts and their (edge, time, destination) packed in order.
  * tvs[c][0..P-2] are chain c's split times; times, migration times and
@@ -411,7 +498,7 @@ static void ry1(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
  */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,
-                   const int *totsites, const double *pi, const double *u, double kappa,
+                   const int *totsites, const double *pi, double *u, double *kappa,
                    const int *samppop, int P, const int *plist, const int *npresent,
                    const int *popdown, const int *pope, const int *join, int npsize,
                    const int *theta_off, const int *theta_wp, int nmigp, const int *mig_off,
@@ -420,8 +507,8 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    double *times, const int *roots, const int *nmig, const int *mig_edge,
                    double *mig_time, const int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
-                   long max_sweeps, long *stats, int ry1_update) {
-  prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi, u, kappa,
+                   long max_sweeps, long *stats, int ry1_update, int urate_update) {
+  prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi,
                P, npsize, nmigp, plist, npresent, popdown, pope, join,
                theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width,
                ry1_update && P > 1};
@@ -460,6 +547,8 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
       g->mig_time = mig_time + moff;
       off += ne;
       moff += g->nmig;
+      g->u = u[c * L + l];
+      g->kappa = kappa[c * L + l];
       g->st.n = n;
       g->st.S = S[l];
       g->st.frac = (double *)calloc((size_t)(n - 1) * S[l] * 4, sizeof(double));
@@ -484,7 +573,7 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
   rng_t r;
   uint64_t sm = seed;
   for (int i = 0; i < 4; i++) r.s[i] = splitmix(&sm);
-  long acc[2] = {0, 0}, props[2] = {0, 0}, sweeps = 0;
+  long acc[3] = {0, 0, 0}, props[3] = {0, 0, 0}, sweeps = 0;
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   double el = 0.0;
@@ -525,6 +614,8 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
         ch[cj].beta = b;
       }
     }
+    if (urate_update && sweeps % 5 == 4)
+      for (int c = 0; c < nchains; c++) change_rates(&pb, &ch[c], &r, &props[2], &acc[2]);
     sweeps++;
     clock_gettime(CLOCK_MONOTONIC, &t1);
     el = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
@@ -534,11 +625,15 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
     stats[1] = acc[0];
     stats[2] = props[1];
     stats[3] = acc[1];
+    stats[4] = props[2];
+    stats[5] = acc[2];
   }
   for (int c = 0; c < nchains; c++) {
     for (int k = 0; k < P - 1; k++) tvs[c * (P - 1) + k] = ch[c].tv[k];
     for (int l = 0; l < L; l++) {
       gene_t *g = &ch[c].g[l];
+      u[c * L + l] = g->u;
+      kappa[c * L + l] = g->kappa;
       free(g->st.frac);
       free(g->st.newfrac);
       free(g->st.scale);

@@ -11,7 +11,9 @@ time -- the fraction of each sampler's draws below the quantile must agree:
 This is the two-sample comparison of marginal posteriors, with the
 autocorrelation inside a chain accounted for by the chain-level spread.
 Both time updates are compared: the global rescale, and changet_RY1's
-rubber band on one split time (IMA_MC3_RY1).
+rubber band on one split time (IMA_MC3_RY1); with the mutation-rate rounds
+(IMA_MC3_UPDATE_U, changeu every fifth sweep) the rate ratio log(u0/u1)
+and both loci's kappa are compared too.
 """
 import math
 
@@ -44,7 +46,7 @@ def _stats(gens, tv):
     return [float(g.time[g.up0[g.root]]) for g in gens] + [float(tv[0])]
 
 
-def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
+def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
     e = Engine(0)
     mid = e.model_add(model)
     lids = []
@@ -60,13 +62,15 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
     e.mc3_setup(C, Lc, first, 2.0, 4.0, np.ones(C), seed=77, precision=L.PREC_FP64,
                 scale_width=0.3)
     e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
-    flags = L.MC3_GRAPH | L.MC3_SCALE | (L.MC3_RY1 if ry1 else 0)
+    flags = (L.MC3_GRAPH | L.MC3_SCALE | (L.MC3_RY1 if ry1 else 0)
+             | (L.MC3_UPDATE_U if urate else 0))
     e.mc3_run(burn, flags)
     out = []
     for _ in range(nsamp):
         e.mc3_run(thin, flags)
         _, _, dn, tm, rt, _ = e.read_slots(first, C * Lc)
         _, tv = e.mc3_read_prior()
+        un, kn = e.mc3_read_rates()
         row, off = [], 0
         for c in range(C):
             vals = []
@@ -77,37 +81,49 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
                 vals.append(float(t[g.up0[g.root]]))
                 off += ne
             vals.append(float(tv[c][0]))
+            if urate:
+                vals += [float(np.log(un[c, 0] / un[c, 1])), float(kn[c, 0]), float(kn[c, 1])]
             row.append(vals)
         out.append(row)
     e.close()
     return np.array(out).transpose(1, 0, 2)  # [chain, sample, stat]
 
 
-def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1):
+def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
     chains = [[g.copy() for g in base] for _ in range(C)]
     tvs = np.ascontiguousarray(np.tile(ts, (C, 1)))
+    rates = (np.full((C, 2), 2.0), np.full((C, 2), 4.0))
     k = 0
 
     def run(n):
         nonlocal k
-        oracle.mc3_bench(loci, [2.0, 2.0], 4.0, samp, model, chains, tvs, np.ones(C), 5.0,
-                         seed=90000 + k, max_sweeps=n, scale_width=0.3, ry1=ry1)
+        oracle.mc3_bench(loci, None, None, samp, model, chains, tvs, np.ones(C), 5.0,
+                         seed=90000 + k, max_sweeps=n, scale_width=0.3, ry1=ry1, urate=urate,
+                         rates=rates)
         k += 1
+
+    def stats(c):
+        v = _stats(chains[c], tvs[c])
+        if urate:
+            u, kap = rates
+            v += [float(np.log(u[c, 0] / u[c, 1])), float(kap[c, 0]), float(kap[c, 1])]
+        return v
 
     run(burn)
     out = []
     for _ in range(nsamp):
         run(thin)
-        out.append([_stats(chains[c], tvs[c]) for c in range(C)])
+        out.append([stats(c) for c in range(C)])
     return np.array(out).transpose(1, 0, 2)
 
 
-@pytest.mark.parametrize("ry1", [False, True], ids=["rescale", "rubber_band"])
-def test_device_and_oracle_sample_the_same_posterior(ry1):
+@pytest.mark.parametrize("ry1,urate", [(False, False), (True, False), (True, True)],
+                         ids=["rescale", "rubber_band", "rubber_band_rates"])
+def test_device_and_oracle_sample_the_same_posterior(ry1, urate):
     model, samp, loci, base, ts = _setup(8)
     C, burn, nsamp, thin = 48, 400, 150, 20
-    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1)
-    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1)
+    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate)
+    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate)
     worst = 0.0
     for s in range(a.shape[2]):
         pooled = np.concatenate([a[:, :, s].ravel(), b[:, :, s].ravel()])

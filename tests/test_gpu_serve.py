@@ -28,7 +28,7 @@ def _direct(spec):
     """The analysis run straight through the engine, all sweeps at once."""
     e = Engine(0)
     spec.load(e)
-    e.mc3_run(spec.nsweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(spec.nsweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     pdg, beta, stats, _ = e.mc3_read()
     probg, _ = e.mc3_read_prior()
     e.close()
@@ -174,7 +174,7 @@ def test_contexts_return_their_memory():
         for s in specs:
             e = Engine(0)
             s.load(e)
-            e.mc3_run(3, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1)
+            e.mc3_run(3, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
             e.close()
 
     cycle()  # first-use allocations of the runtime

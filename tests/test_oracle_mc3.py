@@ -142,3 +142,96 @@ def test_rubber_band_and_rescale_sample_the_same_posterior():
             print(f"stat {s} q={qn:.2f} ry1={fa.mean():.4f} rescale={fb.mean():.4f} z={z:.2f}")
             worst = max(worst, z)
     assert worst < 4.5
+
+
+def _fracs(values, taus):
+    """Worst |mean fraction below tau - q| / chain standard error over the
+    (q, tau) pairs; values is [chains, samples]."""
+    v = np.asarray(values)
+    worst = 0.0
+    for qn, tau in taus:
+        frac = (v <= tau).mean(axis=1)
+        se = frac.std(ddof=1) / math.sqrt(v.shape[0])
+        worst = max(worst, abs(frac.mean() - qn) / max(se, 1e-3))
+    return worst
+
+
+QS = (0.1, 0.25, 0.5, 0.75, 0.9)
+
+
+def test_flat_likelihood_rates_walk_their_reflecting_bounds():
+    """changeu and its kappa moves under a flat likelihood (u ~ 1e-15): every
+    pair move is accepted, so log(u0/u1) is a walk reflected at
+    +-3 log(10000) and each kappa one reflected at 0 and 100; both are
+    uniform on their ranges at stationarity (the chains start there), and
+    u0 u1 never changes."""
+    C, n = 32, 2
+    rng = np.random.default_rng(4)
+    model, loc, chains = _flat_one_pop(n, C, 5)
+    chains = [row + other for row, other in zip(chains, _flat_one_pop(n, C, 6)[2])]
+    tvs = np.zeros((C, 0))
+    R = 3 * math.log(10000.0)
+    r0 = rng.uniform(-R, R, C)
+    u = np.ascontiguousarray(np.stack([1e-15 * np.exp(r0 / 2), 1e-15 * np.exp(-r0 / 2)], 1))
+    kappa = np.ascontiguousarray(rng.uniform(0.0, 100.0, (C, 2)))
+    lr, ks = [], []
+    for k in range(60):
+        _, st = oracle.mc3_bench([loc, loc], None, None, [[n]], model, chains, tvs, np.ones(C),
+                                 10.0, seed=500 + k, max_sweeps=500, scale_width=1.5,
+                                 urate=True, rates=(u, kappa))
+        assert st[4] == 100 * C and st[5] == st[4]  # one step a round, all accepted
+        np.testing.assert_allclose(np.log(u).sum(axis=1), 2 * math.log(1e-15), rtol=0,
+                                   atol=1e-9)
+        lr.append(np.log(u[:, 0] / u[:, 1]))
+        ks.append(kappa[:, 0].copy())
+    lr, ks = np.array(lr).T, np.array(ks).T
+    assert np.abs(lr).max() <= R and ks.min() >= 0 and ks.max() <= 100
+    assert _fracs(lr, [(qn, -R + 2 * R * qn) for qn in QS]) < 4.5
+    assert _fracs(ks, [(qn, 100.0 * qn) for qn in QS]) < 4.5
+
+
+def test_changekappa_samples_the_kappa_posterior():
+    """A lone HKY locus of two gene copies (changekappa, update_mc_params.cpp
+    :387-429): the sampled kappa follows its marginal posterior
+    p(kappa) ~ int L(t, kappa) (2/q) E1(2t/q) dt on [0, 100], the
+    integral taken on a grid with the oracle's likelihood.  A flipped
+    acceptance ratio or a missing beta moves the quantiles."""
+    rng = np.random.default_rng(21)
+    pt = sim.PopTree(1, [])
+    q = 8.0
+    model = immodel.build(pt, qmax=q, mmax=5.0)
+    truth = sim.simulate_genealogy(rng, [2], [2.0], [], np.zeros((1, 1)), pt)
+    truth.time[truth.down == truth.root] = 1.0  # the coalescence: top of both tip edges
+    pi = np.array([0.3, 0.2, 0.2, 0.3])
+    u = 60.0
+    loc = sim.encode_hky(sim.simulate_hky_sequences(rng, truth, 300, pi, 4.0, u))
+    # marginal posterior of kappa on a grid
+    tg = np.geomspace(0.02, 20.0, 400)
+    kg = np.linspace(0.05, 30.0, 240)
+    g = truth.copy()
+    ll = np.empty((len(kg), len(tg)))
+    for i, kap in enumerate(kg):
+        for j, t in enumerate(tg):
+            g.time[g.down == g.root] = t
+            ll[i, j] = oracle.likelihood_hky(g, loc, u, kap)[0]
+    prior = (2 / q) * exp1(2 * tg / q)
+    w = np.exp(ll - ll.max()) * prior
+    mk = np.trapezoid(w, tg, axis=1)
+    assert mk[-1] < 1e-6 * mk.max() and w[:, [0, -1]].max() < 1e-4 * w.max()
+    cdfk = np.concatenate([[0.0], np.cumsum(0.5 * (mk[1:] + mk[:-1]) * np.diff(kg))])
+    cdfk /= cdfk[-1]
+
+    C = 32
+    chains = [[truth.copy()] for _ in range(C)]
+    rates = (np.full((C, 1), u), np.full((C, 1), 4.0))
+    tvs = np.zeros((C, 0))
+    samples = []
+    for k in range(150):
+        _, st = oracle.mc3_bench([loc], None, None, [[2]], model, chains, tvs, np.ones(C), 10.0,
+                                 seed=90 + k, max_sweeps=50, scale_width=1.5, urate=True,
+                                 rates=rates)
+        assert st[4] == 10 * C and 0 < st[5] < st[4]
+        if k >= 20:
+            samples.append(rates[1][:, 0].copy())
+    v = np.array(samples).T
+    assert _fracs(v, [(qn, float(np.interp(qn, cdfk, kg))) for qn in QS]) < 4.5
