@@ -1235,6 +1235,10 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     HIPCHK(c, reserve(m.ulist0, units.size() + 1));
     HIPCHK(c, reserve(m.ulist1, units.size() + 1));
     HIPCHK(c, reserve(m.ucnt, 2));
+    HIPCHK(c, hipMemsetAsync(m.ucnt.p, 0, 2 * sizeof(int32_t), c->stream));
+    if (urate_plan_lds(nloci, J) > 65536)
+      return fail(c, IMA_ERR_ARG, "%d loci per chain: the mutation-rate plan needs more LDS",
+                  nloci);
     URateDev &u = m.udev;
     u.J = J;
     u.uitems = m.uitems.p;

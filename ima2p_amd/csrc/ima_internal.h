@@ -359,6 +359,9 @@ __host__ __device__ inline int urate_partner(PhiloxStream &r, int s, int L) {
   return k;
 }
 
+// LDS of a round's plan: per-locus levels, level offsets, and per step its
+// partner, level and position (L + 4J + 2 ints).
+size_t urate_plan_lds(int L, int J);
 hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URateDev &u,
                                const ima_item *items, const int32_t *idx, const HkyPlan &plan,
                                int nh, int ni, int ns, int precision, int hmax_n,

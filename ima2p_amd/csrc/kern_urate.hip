@@ -132,39 +132,43 @@ __device__ void urate_decide(const MC3Dev &m, const URateDev &u, ima_item *items
   if (acc) atomicAdd(&m.stats[7], 1ull);
 }
 
-__global__ void urate_begin(int32_t *cnt) {
-  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
-}
-
 // One workgroup per chain: copy the chain's items, draw every step's
 // partner and uniforms in the reference's order (partner, window, kappa of
-// j, kappa of k, acceptance; a kappa uniform only for an HKY locus), give
-// each step its level, sort the steps by level and propose level 1.
-// serial != 0 puts step s at level s + 1 (the sequential loop itself).
+// j, kappa of k, acceptance; a kappa uniform only for an HKY locus), one
+// lane per step since each step has its own stream; then one lane gives
+// each step its level (bookkeeping in LDS), the steps are sorted by level
+// and level 1 is proposed.  serial != 0 puts step s at level s + 1 (the
+// sequential loop itself).  Dynamic LDS: urate_plan_lds(L, J).
 __global__ __launch_bounds__(256) void urate_plan(DeviceView v, MC3Dev m, URateDev u,
                                                   const ima_item *items, uint64_t ctr,
                                                   int serial) {
-  __shared__ int lvl1[2];
+  extern __shared__ int32_t pl[];
   const int c = blockIdx.x, L = m.L, J = u.J;
-  for (int i = threadIdx.x; i < L; i += blockDim.x) u.uitems[c * L + i] = items[c * L + i];
+  int32_t *last = pl, *off = pl + L, *kk = off + (J + 2), *lv = kk + J, *ord = lv + J;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    u.uitems[c * L + i] = items[c * L + i];
+    last[i] = 0;
+  }
+  for (int l = threadIdx.x; l <= J + 1; l += blockDim.x) off[l] = 0;
+  for (int s = threadIdx.x; s < J; s += blockDim.x) {
+    PhiloxStream r = urate_rng(m.seed, c, L, s, ctr);
+    const int k = urate_partner(r, s, L);
+    double4 U;
+    U.x = k >= 0 ? r.uniform() : 0.0;
+    U.y = item_is_hky(v, m, c * L + s) ? r.uniform() : 0.0;
+    U.z = (k >= 0 && item_is_hky(v, m, c * L + k)) ? r.uniform() : 0.0;
+    U.w = r.uniform();
+    u.uni[c * J + s] = U;
+    kk[s] = k;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int32_t *last = u.last + (int64_t)c * L;
-    for (int i = 0; i < L; ++i) last[i] = 0;
-    int32_t *off = u.loff + (int64_t)c * (J + 2);
-    for (int l = 0; l <= J + 1; ++l) off[l] = 0;
     for (int s = 0; s < J; ++s) {
-      PhiloxStream r = urate_rng(m.seed, c, L, s, ctr);
-      const int k = urate_partner(r, s, L);
-      double4 U;
-      U.x = k >= 0 ? r.uniform() : 0.0;
-      U.y = item_is_hky(v, m, c * L + s) ? r.uniform() : 0.0;
-      U.z = (k >= 0 && item_is_hky(v, m, c * L + k)) ? r.uniform() : 0.0;
-      U.w = r.uniform();
+      const int k = kk[s];
       const int lev = serial ? s + 1 : 1 + max(last[s], k >= 0 ? last[k] : 0);
       last[s] = lev;
       if (k >= 0) last[k] = lev;
-      u.step[c * J + s] = make_int4(k, lev, 0, 0);
-      u.uni[c * J + s] = U;
+      lv[s] = lev;
       off[lev] += 1;
     }
     // off[l] <- first position of level l; last[l - 1] is level l's cursor
@@ -175,14 +179,17 @@ __global__ __launch_bounds__(256) void urate_plan(DeviceView v, MC3Dev m, URateD
       if (l <= J) last[l - 1] = run;
       run += t;
     }
-    int32_t *ord = u.order + (int64_t)c * J;
-    for (int s = 0; s < J; ++s) ord[last[u.step[c * J + s].y - 1]++] = s;
-    lvl1[0] = off[1];
-    lvl1[1] = off[2];
+    for (int s = 0; s < J; ++s) ord[last[lv[s] - 1]++] = s;
   }
   __syncthreads();
-  for (int i = lvl1[0] + threadIdx.x; i < lvl1[1]; i += blockDim.x)
-    urate_propose(v, m, u, items, c, u.order[(int64_t)c * J + i], 1);
+  for (int s = threadIdx.x; s < J; s += blockDim.x) {
+    u.step[c * J + s] = make_int4(kk[s], lv[s], 0, 0);
+    u.order[(int64_t)c * J + s] = ord[s];
+  }
+  for (int l = threadIdx.x; l <= J + 1; l += blockDim.x) u.loff[(int64_t)c * (J + 2) + l] = off[l];
+  __syncthreads();  // the steps' records visible to the proposals
+  for (int i = off[1] + threadIdx.x; i < off[2]; i += blockDim.x)
+    urate_propose(v, m, u, items, c, ord[i], 1);
 }
 
 // One workgroup per chain: decide level lev, then (unless it is the last
@@ -194,7 +201,11 @@ __global__ __launch_bounds__(256) void urate_step(DeviceView v, MC3Dev m, URateD
   const int32_t *ord = u.order + (int64_t)c * J;
   for (int i = off[lev] + threadIdx.x; i < off[lev + 1]; i += blockDim.x)
     urate_decide(m, u, items, c, ord[i]);
-  if (last) return;
+  if (last) {
+    // the next round's plan queues level 1 on list 1 (nothing queues now)
+    if (c == 0 && threadIdx.x < 2) u.cnt[threadIdx.x] = 0;
+    return;
+  }
   __syncthreads();  // the level's commits visible to the next level's proposals
   for (int i = off[lev + 1] + threadIdx.x; i < off[lev + 2]; i += blockDim.x)
     urate_propose(v, m, u, items, c, ord[i], (lev + 1) & 1);
@@ -202,15 +213,19 @@ __global__ __launch_bounds__(256) void urate_step(DeviceView v, MC3Dev m, URateD
 
 }  // namespace
 
+size_t urate_plan_lds(int L, int J) { return 4 * ((size_t)L + 4 * (size_t)J + 2); }
+
 hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URateDev &u,
                                const ima_item *items, const int32_t *idx, const HkyPlan &plan,
                                int nh, int ni, int ns, int precision, int hmax_n,
                                int hmax_slots, int imax_n, int imax_w, uint64_t ctr,
                                int nlev, int serial, hipStream_t st) {
   if (nlev < 1 || nlev > u.J) return hipErrorInvalidValue;
+  const size_t lds = urate_plan_lds(m.L, u.J);
+  if (lds > 65536) return hipErrorInvalidValue;
   ima_item *it = const_cast<ima_item *>(items);
-  hipLaunchKernelGGL(urate_begin, dim3(1), dim3(kWave), 0, st, u.cnt);
-  hipLaunchKernelGGL(urate_plan, dim3(m.C), dim3(256), 0, st, v, m, u, items, ctr, serial);
+  // u.cnt is zero here: set up so, and left so by every round's last step
+  hipLaunchKernelGGL(urate_plan, dim3(m.C), dim3(256), lds, st, v, m, u, items, ctr, serial);
   for (int lev = 1; lev <= nlev; ++lev) {
     hipError_t e = hipSuccess;
     if (nh)
