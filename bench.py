@@ -190,7 +190,7 @@ def uneven_leg(sweeps=40):
     spec = workloads.uneven_mixed()
     e = Engine(0)
     spec.load(e)
-    group, _ = e.mc3_tune(4)
+    group, _ = e.mc3_tune(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     t0 = time.perf_counter()
     e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
@@ -241,25 +241,10 @@ def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
     from ima2p_amd import sim
     from ima2p_amd.serve import REQ_DONE, Server
     rng = np.random.default_rng(seed)
-    pools = {P: workloads.serving_pool(rng, P, size=48) for P in (2, 3, 4)}
     tab_sizes = (10000, 100000, 1000000)
-    tabs = [sim.lmode_table(rng, G) for G in tab_sizes]
-    lo, hi = np.array([0.5, 0.5, 0.5, 0.01, 0.01]), np.array([10, 10, 10, 2, 2])
-    t, reqs = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration_s:
-            break
-        if rng.random() < 0.7:
-            k = int(rng.integers(0, len(tabs)))
-            npts = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
-            reqs.append((t, "q", (k, rng.uniform(lo, hi, (npts, 5)))))
-        else:
-            P = int(rng.integers(2, 5))
-            spec = workloads.serving_analysis(rng, pools[P], int(rng.integers(5, 301)),
-                                              int(rng.integers(16, 129)), nsweeps=nsweeps,
-                                              seed=int(rng.integers(1, 1 << 30)))
-            reqs.append((t, "a", (spec.nchains, spec.flatten())))
+    tabs, pools, stream = workloads.serving_requests(rng, duration_s, rate, nsweeps, tab_sizes)
+    lo = np.array([0.5, 0.5, 0.5, 0.01, 0.01])
+    reqs = [(t, k, (d.nchains, d.flatten()) if k == "a" else d) for t, k, d in stream]
     srv = Server(0, 0.0, 25)
     tids = [srv.table(tb, 3, 2, 2) for tb in tabs]
     # one warm query per table and one small analysis before the stream
@@ -370,7 +355,7 @@ def main():
     e, loci, first, model = load_engine(wl, L.PREC_FP32)
     setup_s = time.perf_counter() - t_setup
     # autotune the accept step's group size (the chains are left as they were)
-    group, group_ms = e.mc3_tune(10)
+    group, group_ms = e.mc3_tune(10, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     t0 = time.perf_counter()
     e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)

@@ -75,7 +75,7 @@ PROTOS = {
     "ima_sw_update_A": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, dp, i32p, dp]),
     "ima_mc3_set_prior_group": (C.c_int, [vp, C.c_int]),
     "ima_mc3_prior_group": (C.c_int, [vp, C.POINTER(C.c_int)]),
-    "ima_mc3_tune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp]),
+    "ima_mc3_tune": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]),
     "ima_device_memory": (C.c_int, [C.c_int, dp, dp]),
     "ima_server_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]),
     "ima_server_destroy": (None, [vp]),
@@ -111,6 +111,9 @@ PROTOS = {
     "ima_mc3_run": (C.c_int, [vp, C.c_int, C.c_int]),
     "ima_mc3_read": (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ima_mc3_read_rates": (C.c_int, [vp, dp, dp]),
+    "ima_mc3_decide": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, i32p]),
+    "ima_mc3_prior_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "ima_timing_enable": (C.c_int, [vp, C.c_int]),
     "ima_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), dp]),
     "ima_timing_reset": (C.c_int, [vp]),

@@ -291,11 +291,14 @@ int ima_ctx_create(int device, ima_ctx **out) {
     if (hipEventCreate(&c->timing.ev0[k]) != hipSuccess) return IMA_ERR_HIP;
     if (hipEventCreate(&c->timing.ev1[k]) != hipSuccess) return IMA_ERR_HIP;
   }
+  // Every launch keeps its dynamic LDS within 64 KiB even where the device
+  // allows more per workgroup (160 KiB on gfx950): group sizes shrink to fit
+  // and larger requests fail with IMA_ERR_SIZE before any launch.
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
           hipSuccess &&
       lds > 0)
-    c->max_lds = lds;
+    c->max_lds = std::min(lds, kMaxLaunchLds);
   *out = c.release();
   return IMA_OK;
 }
@@ -1100,6 +1103,12 @@ static void mc3_release(ima_ctx *c) {
   release(m.uloff);
   release(m.ulast);
   release(m.ucnt);
+  release(m.hast);
+  release(m.uhost);
+  release(m.idec);
+  release(m.ptw);
+  release(m.pitems);
+  m.gview_set = false;
   m.udev = URateDev{};
   m.dev.prior = -1;
   m.live = false;
@@ -1295,6 +1304,12 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.nparam = 0;
   d.pchunk = 1;
   d.mints = 0;
+  d.host = 0;
+  d.gbeta = 1.0;
+  d.hast = nullptr;
+  d.uhost = nullptr;
+  d.idec = nullptr;
+  d.prop_first = 0;
   for (int k = 0; k < 4; ++k) {
     HIPCHK(c, hipEventCreate(&m.ev[k]));
     HIPCHK(c, hipEventCreateWithFlags(&m.fj[k], hipEventDisableTiming));
@@ -1379,6 +1394,14 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
     m.dev.ry1 = ry1;
   }
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  {
+    // captured graphs hold the device buffers of their capture time
+    const DeviceView cur = view(c);
+    if (m.gview_set && memcmp(&cur, &m.gview, sizeof cur) != 0) mc3_drop_graphs(c);
+    m.gview = cur;
+    m.gview_set = true;
+  }
   const int bk = scale ? 1 : 0;
   // n sweeps from one of the given parity; odd sweeps rescale when sc
   auto capture = [&](hipGraphExec_t *exec, hipGraph_t *graph, int n, int parity,
@@ -1531,6 +1554,34 @@ int ima_mc3_read(ima_ctx *c, double *pdg, double *beta, uint64_t *stats, uint64_
   return IMA_OK;
 }
 
+// Dynamic LDS of the sweep's per-chain prior kernels (accept, rescale
+// commit) at D loci per group, with the swap's static staging.
+static size_t prior_kernel_lds(int R, int np, int D, int mints) {
+  return mc3_prior_lds_bytes(R, np, D, mints) + 2 * (size_t)kSwapLds * 8 + 2048;
+}
+
+// Largest accept group size 1..kSpecDepth whose LDS fits max_lds; 0 if none.
+static int prior_group_plan(int R, int np, int mints, int max_lds) {
+  for (int D = kSpecDepth; D >= 1; --D)
+    if (prior_kernel_lds(R, np, D, mints) <= (size_t)max_lds) return D;
+  return 0;
+}
+
+int ima_mc3_prior_plan(int npops, int npsize, int ntheta_wp, int nmig, int nmig_wp,
+                       int *loci, int64_t *lds_bytes) {
+  if (npops < 1 || npops > kMaxPops || npsize < 0 || nmig < 0 || ntheta_wp < 0 || nmig_wp < 0 ||
+      !loci)
+    return IMA_ERR_ARG;
+  const int P = npops, T = 2 * P - 1, R = prior_rec_len(P), np = npsize + nmig;
+  // the model's int block as ima_model_add lays it out
+  const int mints = P * P + P + 2 * T + 3 * P + P * T + (npsize + 1) + 2 * ntheta_wp +
+                    (nmig + 1) + 3 * nmig_wp;
+  const int D = prior_group_plan(R, np, mints, kMaxLaunchLds);
+  *loci = D;
+  if (lds_bytes) *lds_bytes = (int64_t)prior_kernel_lds(R, np, D > 0 ? D : 1, mints);
+  return D > 0 ? IMA_OK : IMA_ERR_SIZE;
+}
+
 int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   if (!c) return IMA_ERR_ARG;
   auto &m = c->mc3;
@@ -1569,13 +1620,9 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   }
   if (treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "a genealogy has %d prior events", emax);
-  // per-chain kernels: static swap staging + prior state + a chunk of loci
-  const size_t fixed = 2 * (size_t)kSwapLds * 8 + 2048;
-  int pchunk = kSpecDepth;  // loci per speculative group: 2^D - 1 scenarios per parameter
-  while (pchunk > 1 && mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
-    --pchunk;
-  if (mc3_prior_lds_bytes(R, np, pchunk, mm.nints) + fixed > (size_t)c->max_lds)
-    return fail(c, IMA_ERR_SIZE, "prior state of %d populations exceeds LDS", P);
+  // per-chain kernels: loci per speculative group, 2^D - 1 scenarios per parameter
+  const int pchunk = prior_group_plan(R, np, mm.nints, c->max_lds);
+  if (pchunk < 1) return fail(c, IMA_ERR_SIZE, "prior state of %d populations exceeds LDS", P);
   grow_logfact(c, lfn);
   int rc;
   if ((rc = upload_loci(c)) != IMA_OK) return rc;
@@ -1647,8 +1694,7 @@ int ima_mc3_set_prior_group(ima_ctx *c, int loci) {
   if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
   if (loci < 1 || loci > kSpecDepth)
     return fail(c, IMA_ERR_ARG, "loci per group must be in 1..%d", kSpecDepth);
-  const size_t fixed = 2 * (size_t)kSwapLds * 8 + 2048;
-  if (mc3_prior_lds_bytes(m.dev.R, m.dev.nparam, loci, m.dev.mints) + fixed > (size_t)c->max_lds)
+  if (prior_kernel_lds(m.dev.R, m.dev.nparam, loci, m.dev.mints) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "%d loci per group exceed LDS", loci);
   m.dev.pchunk = loci;
   mc3_drop_graphs(c);
@@ -1678,6 +1724,7 @@ std::vector<Region> mc3_state_regions(ima_ctx *c) {
   add(m.stats.p, 8 * 8);
   add(m.acc.p, N * 4);
   add(m.cacc.p, (size_t)m.C * 4);
+  add(m.items.p, N * sizeof(ima_item));  // u and kappa, moved by the rate rounds
   if (d.prior >= 0) {
     const size_t np = (size_t)std::max(d.nparam, 1);
     add(m.tv.p, (size_t)m.C * mc3_tw(d.P) * 8);
@@ -1706,8 +1753,10 @@ size_t mc3_state_bytes(ima_ctx *c) {
 }
 }  // namespace
 
-int ima_mc3_tune(ima_ctx *c, int sweeps, int *best, double *ms_per_sweep) {
+int ima_mc3_tune(ima_ctx *c, int sweeps, int flags, int *best, double *ms_per_sweep) {
   if (!c || sweeps < 1) return IMA_ERR_ARG;
+  // the sweeps the caller will run, replayed from captured graphs
+  const int run_flags = (flags & ~(IMA_MC3_TIMED | IMA_MC3_U_SERIAL)) | IMA_MC3_GRAPH;
   auto &m = c->mc3;
   if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
   hipSetDevice(c->device);
@@ -1739,7 +1788,7 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int *best, double *ms_per_sweep) {
       const int n = pass == 0 ? 2 : sweeps;
       if (rc == IMA_OK && pass == 1 && hipEventRecord(e0, c->stream) != hipSuccess)
         rc = fail(c, IMA_ERR_HIP, "event record failed");
-      if (rc == IMA_OK) rc = mc3_run_async(c, n, IMA_MC3_GRAPH | IMA_MC3_SCALE);
+      if (rc == IMA_OK) rc = mc3_run_async(c, n, run_flags);
       if (rc == IMA_OK && pass == 1) {
         if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
@@ -1765,6 +1814,157 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int *best, double *ms_per_sweep) {
   hipFree(buf);
   if (rc == IMA_OK && best) *best = pick;
   return rc;
+}
+
+// Decisions for a batch of proposals written by the host (IMa2p's own
+// updategenealogy proposals): score, decide each chain's loci in order,
+// copy the accepted ones into the chains' slots.
+int ima_mc3_decide(ima_ctx *c, int prop_first, const double *hastings, const double *uniforms,
+                   double gbeta, int32_t *accepted) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  const int N = m.C * m.L;
+  if (prop_first < 0 || prop_first + N > (int)c->slots.size())
+    return fail(c, IMA_ERR_ARG, "proposal slots [%d,%d) out of range", prop_first, prop_first + N);
+  if (prop_first < m.first + N && m.first < prop_first + N)
+    return fail(c, IMA_ERR_ARG, "proposal slots overlap the chains' slots");
+  if (!(gbeta >= 0.0) || !std::isfinite(gbeta)) return fail(c, IMA_ERR_ARG, "gbeta must be >= 0");
+  hipSetDevice(c->device);
+  const bool prior = m.dev.prior >= 0;
+  const int P = m.dev.P;
+  int hmax_slots = m.hmax_slots, emax = m.dev.emax;
+  size_t lfn = 4;
+  bool grew = false;
+  for (int ch = 0; ch < m.C; ++ch) {
+    size_t tot = 4;
+    for (int l = 0; l < m.L; ++l) {
+      const int i = ch * m.L + l;
+      SlotMeta &d = c->slots[m.first + i];
+      const SlotMeta &s = c->slots[prop_first + i];
+      if (s.locus != d.locus)
+        return fail(c, IMA_ERR_ARG, "proposal slot %d is not of its chain slot's locus", prop_first + i);
+      if (s.root < 0) return fail(c, IMA_ERR_STATE, "proposal slot %d holds no genealogy", prop_first + i);
+      if (c->loci[s.locus].model == IMA_MODEL_HKY) hmax_slots = std::max(hmax_slots, s.nslots);
+      if (prior) {
+        emax = std::max(emax, (s.n - 1) + s.nmig + P - 1);
+        tot += (size_t)s.n - 1 + (size_t)std::max(s.nmig, d.nmig);
+        if (s.nmig > d.mig_cap) {
+          // the chain's slot must hold the proposal's migrations: move its
+          // events to a larger region (the device copy keeps their times)
+          const int cap = s.nmig + s.nmig / 2 + 4;
+          const int64_t old = d.mig_off;
+          d.mig_off = (int64_t)c->migs;
+          d.mig_cap = cap;
+          c->migs += (size_t)cap;
+          c->mig_edge.resize(c->migs, 0);
+          c->mig_to.resize(c->migs, 0);
+          c->mig_time.resize(c->migs, 0.0);
+          std::copy(c->mig_edge.begin() + old, c->mig_edge.begin() + old + d.nmig,
+                    c->mig_edge.begin() + d.mig_off);
+          std::copy(c->mig_to.begin() + old, c->mig_to.begin() + old + d.nmig,
+                    c->mig_to.begin() + d.mig_off);
+          HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
+          HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
+          HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
+          if (d.nmig) {
+            HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + d.mig_off, c->d_mig_edge.p + old,
+                                     (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + d.mig_off, c->d_mig_to.p + old,
+                                     (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + d.mig_off, c->d_mig_time.p + old,
+                                     (size_t)d.nmig * 8, hipMemcpyDeviceToDevice, c->stream));
+          }
+          c->slots_meta_dirty = true;
+          grew = true;
+        }
+      }
+    }
+    lfn = std::max(lfn, tot);
+  }
+  if (prior && treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "a proposal has %d prior events", emax);
+  if (grew) {
+    HIPCHK(c, reserve(m.mtbackup, c->migs + 1));
+    m.dev.mtbackup = m.mtbackup.p;
+  }
+  if (emax != m.dev.emax || hmax_slots != m.hmax_slots || grew) {
+    // the chains' slots may now hold these genealogies: later sweeps use
+    // the larger bounds
+    m.dev.emax = emax;
+    m.hmax_slots = hmax_slots;
+    mc3_drop_graphs(c);
+  }
+  grow_logfact(c, lfn);
+  int rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  HIPCHK(c, reserve(m.idec, N));
+  HIPCHK(c, reserve(m.ptw, N));
+  HIPCHK(c, reserve(m.pitems, N));
+  MC3Dev d = m.dev;
+  d.host = 1;
+  d.gbeta = gbeta;
+  d.idec = m.idec.p;
+  d.prop_first = prop_first;
+  d.hast = nullptr;
+  d.uhost = nullptr;
+  if (hastings) {
+    for (int i = 0; i < N; ++i)
+      if (std::isnan(hastings[i])) return fail(c, IMA_ERR_ARG, "hastings[%d] is NaN", i);
+    HIPCHK(c, reserve(m.hast, N));
+    HIPCHK(c, hipMemcpyAsync(m.hast.p, hastings, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    d.hast = m.hast.p;
+  }
+  if (uniforms) {
+    for (int i = 0; i < N; ++i)
+      if (!(uniforms[i] >= 0.0 && uniforms[i] <= 1.0))
+        return fail(c, IMA_ERR_ARG, "uniforms[%d] outside [0, 1]", i);
+    HIPCHK(c, reserve(m.uhost, N));
+    HIPCHK(c, hipMemcpyAsync(m.uhost.p, uniforms, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    d.uhost = m.uhost.p;
+  }
+  const DeviceView v = view(c);
+  HIPCHK(c, enqueue_mc3_prop_items(d, m.items.p, m.pitems.p, m.ptw.p, c->stream));
+  HkyPlan p = m.plan;
+  p.nred = 0;  // the decision kernel merges pattern blocks itself
+  HIPCHK(c, enqueue_mc3_score(v, d, m.pitems.p, m.ptw.p, m.idx.p, p, m.nh, m.ni, m.ns,
+                              m.precision, m.hmax_n, hmax_slots, m.imax_n, m.imax_w, nullptr, 0,
+                              c->stream, c->side, m.fj));
+  HIPCHK(c, enqueue_mc3_decide(v, d, c->stream));
+  std::vector<int32_t> dec(N);
+  HIPCHK(c, copy_now(c, dec.data(), m.idec.p, (size_t)N * 4, hipMemcpyDeviceToHost));
+  m.host_sweeps += 1;  // the device counter advanced with the decisions
+  // the host mirror of the chains' slots follows the device copy
+  for (int i = 0; i < N; ++i) {
+    if (!dec[i]) continue;
+    SlotMeta &dst = c->slots[m.first + i];
+    const SlotMeta &src = c->slots[prop_first + i];
+    const int64_t so = src.node_off, dn = dst.node_off, ne = 2 * (int64_t)src.n - 1;
+    std::copy(c->up0.begin() + so, c->up0.begin() + so + ne, c->up0.begin() + dn);
+    std::copy(c->up1.begin() + so, c->up1.begin() + so + ne, c->up1.begin() + dn);
+    std::copy(c->down.begin() + so, c->down.begin() + so + ne, c->down.begin() + dn);
+    std::copy(c->time.begin() + so, c->time.begin() + so + ne, c->time.begin() + dn);
+    std::copy(c->A.begin() + so, c->A.begin() + so + ne, c->A.begin() + dn);
+    std::copy(c->pop.begin() + so, c->pop.begin() + so + ne, c->pop.begin() + dn);
+    std::copy(c->mig_first.begin() + so, c->mig_first.begin() + so + ne, c->mig_first.begin() + dn);
+    std::copy(c->mig_count.begin() + so, c->mig_count.begin() + so + ne, c->mig_count.begin() + dn);
+    std::copy(c->prog.begin() + src.prog_off, c->prog.begin() + src.prog_off + src.n - 1,
+              c->prog.begin() + dst.prog_off);
+    if (src.nmig) {
+      std::copy(c->mig_edge.begin() + src.mig_off, c->mig_edge.begin() + src.mig_off + src.nmig,
+                c->mig_edge.begin() + dst.mig_off);
+      std::copy(c->mig_to.begin() + src.mig_off, c->mig_to.begin() + src.mig_off + src.nmig,
+                c->mig_to.begin() + dst.mig_off);
+      std::copy(c->mig_time.begin() + src.mig_off, c->mig_time.begin() + src.mig_off + src.nmig,
+                c->mig_time.begin() + dst.mig_off);
+    }
+    dst.root = src.root;
+    dst.nslots = src.nslots;
+    dst.nmig = src.nmig;
+  }
+  if (accepted) std::copy(dec.begin(), dec.end(), accepted);
+  return IMA_OK;
 }
 
 int ima_mc3_prior_group(ima_ctx *c, int *loci) {

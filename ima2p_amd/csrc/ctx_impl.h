@@ -135,6 +135,14 @@ struct ima_ctx {
     DevBuf<double4> uuni;
     DevBuf<int32_t> uorder, uloff, ulast, ucnt;
     std::vector<int32_t> ulevel_scratch;  // host copy of the level bookkeeping
+    // ima_mc3_decide: Hastings terms, uniforms, decisions, proposal items
+    DevBuf<double> hast, uhost;
+    DevBuf<int32_t> idec, ptw;
+    DevBuf<ima_item> pitems;
+    // device buffers the captured graphs point at: a change (a buffer grown
+    // by later slot allocations or writes) drops the graphs
+    ima::DeviceView gview{};
+    bool gview_set = false;
   } mc3;
 };
 

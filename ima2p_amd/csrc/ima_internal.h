@@ -18,6 +18,7 @@ constexpr int kMaxEdges = 2 * kMaxGenes - 1;
 constexpr int kTipSlot = 0xFFFF;   // program operand that is a tip
 constexpr double kLn2 = 0.69314718055994530941723212146;
 constexpr int kMaxPops = 10;       // sampled populations (the reference's MAXPOPS)
+constexpr int kMaxLaunchLds = 65536;  // dynamic LDS any launch may take, in bytes
 
 // Per-locus metadata resident on the device.
 struct LocusMeta {
@@ -278,6 +279,15 @@ struct MC3Dev {
   int32_t *wstatus;      // [C*L] weights status of the proposals
   int pchunk;            // loci decided per speculative group by the accept kernel
   int mints;             // length of the model's int block (staged in LDS)
+  // decisions for proposals written by the host (ima_mc3_decide): no device
+  // proposal is pending, a rejection leaves the chain's slot as it is, the
+  // decision is log U - hast < beta (dprobg + gbeta dpdg)
+  int host;              // 1 inside ima_mc3_decide's launches
+  double gbeta;          // multiplier of dpdg (1 outside thermodynamic integration)
+  const double *hast;    // [C*L] Hastings terms, or null
+  const double *uhost;   // [C*L] the decisions' uniforms, or null (Philox)
+  int32_t *idec;         // [C*L] decisions of the last ima_mc3_decide
+  int prop_first;        // first proposal slot (slot prop_first + it)
 };
 
 // Dynamic LDS of the per-chain prior kernels (kern_mc3.hip), the same on
@@ -313,6 +323,22 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
                              hipStream_t st, hipStream_t side, hipEvent_t *fj);
 
 hipError_t enqueue_mc3_propose(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+// One scoring pass of the sweep: the items' likelihoods into m.newpdg /
+// m.status and (with the prior) the genealogy weights of the slots
+// tw_slots[0..C*L) into m.wnew / m.wstatus.
+hipError_t enqueue_mc3_score(const DeviceView &v, const MC3Dev &m, const ima_item *items,
+                             const int32_t *tw_slots, const int32_t *idx, const HkyPlan &plan,
+                             int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots,
+                             int imax_n, int imax_w, hipEvent_t *score_ev, int pass,
+                             hipStream_t st, hipStream_t side, hipEvent_t *fj);
+// ima_mc3_decide's launches after the scoring pass (m.host = 1): the
+// decisions of every chain's loci in order, the accepted proposals copied
+// into the chains' slots, the sweep counter advanced.
+hipError_t enqueue_mc3_decide(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+// Items and weight slots of the proposals: items[it] with slot
+// prop_first + it, tw_slots[it] = prop_first + it.
+hipError_t enqueue_mc3_prop_items(const MC3Dev &m, const ima_item *items, ima_item *pitems,
+                                  int32_t *ptw, hipStream_t st);
 hipError_t enqueue_mc3_prior_init(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 size_t mc3_prior_lds_bytes(int R, int nparam, int pchunk, int mints);
 constexpr int kSweepBlock = 8;  // sweeps captured in one graph

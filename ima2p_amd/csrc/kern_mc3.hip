@@ -458,9 +458,10 @@ __device__ inline void prior_fetch(const MC3Dev &m, const ModelView &md, int nps
       sh.pdg[row] = m.pdg[it];
       PhiloxStream rr = item_rng(m, it, sweep, 1);
       // the decision U < min(1, e^x) is taken as log U < x (and U < 1);
-      // +inf stands for U = 1
-      const double U = rr.uniform();
-      sh.U[row] = U < 1.0 ? log(U) : INFINITY;
+      // +inf stands for U = 1.  A host proposal's Hastings term h moves to
+      // this side: log U - h < beta (dprobg + gbeta dpdg)
+      const double U = m.uhost ? m.uhost[it] : rr.uniform();
+      sh.U[row] = (U < 1.0 ? log(U) : INFINITY) - (m.hast ? m.hast[it] : 0.0);
     }
     wave_sync();
     for (int t0 = 0; t0 < np; t0 += kWave) {
@@ -493,8 +494,9 @@ __device__ inline void prior_commit(const DeviceView &v, const MC3Dev &m, const 
     if (lane == 0) {
       if (acc)
         m.pdg[it] = sh.npd[row];
-      else
+      else if (!m.host)
         restore_item(v, m, it);
+      if (m.host) m.idec[it] = acc ? 1 : 0;
       m.acc[it] += acc ? 1u : 0u;
     }
   }
@@ -619,7 +621,7 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
         for (int j = 0; j < nl; ++j) okm |= (sh.ok[b0 + j] ? 1 : 0) << j;
         if (lane < nl) {
           xU = sh.U[b0 + lane];  // log U
-          xd = sh.npd[b0 + lane] - sh.pdg[b0 + lane];
+          xd = m.gbeta * (sh.npd[b0 + lane] - sh.pdg[b0 + lane]);
         }
         wave_sync();
         for (int j = 0; j < nl; ++j) {
@@ -661,7 +663,7 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
           for (int t = 0; t < mm.npsize; ++t) q += sh.nt[t];
           for (int t = mm.npsize; t < np; ++t) mg += sh.nt[t];
           const double pn = q + mg;
-          const double x = beta * ((pn - probg) + (sh.npd[row] - sh.pdg[row]));
+          const double x = beta * ((pn - probg) + m.gbeta * (sh.npd[row] - sh.pdg[row]));
           const bool acc = x != x ? sh.U[row] != INFINITY : sh.U[row] < x;
           if (acc) {
             probg = pn;
@@ -996,6 +998,172 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
 }
 
+// ---- decisions for proposals written by the host (ima_mc3_decide) -------
+
+// One workgroup per chain: every locus' proposal (in slot prop_first + it,
+// scored into newpdg / wnew) accepted or rejected in order as
+// updategenealogy decides (update_gtree.cpp:900-961):
+// U < min(1, exp(beta (dprobg + gbeta dpdg) + h)), h the host's
+// migweight + slideweight + Atermsum; an incompatible infinite-sites
+// genealogy or a zero stepwise term is rejected.  Decisions go to idec[];
+// the chain sum to S[c].
+__global__ __launch_bounds__(1024) void mc3_decide(DeviceView v, MC3Dev m) {
+  __shared__ double sh[1024 / kWave];
+  extern __shared__ double dyn[];
+  const int c = blockIdx.x;
+  const uint64_t sweep = *m.counter;
+  double part = 0.0;
+  if (m.prior < 0) {
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      const int it = c * m.L + l;
+      PhiloxStream r = item_rng(m, it, sweep, 1);
+      const double U = m.uhost ? m.uhost[it] : r.uniform();
+      const double np = new_loglik(m, it);
+      const double h = m.hast ? m.hast[it] : 0.0;
+      bool acc = false;
+      if (m.status[it] == IMA_STATUS_OK)
+        acc = U < fmin(1.0, exp(m.beta[c] * (m.gbeta * (np - m.pdg[it])) + h));
+      if (acc) m.pdg[it] = np;
+      m.idec[it] = acc ? 1 : 0;
+      m.acc[it] += acc ? 1u : 0u;
+      part += m.pdg[it];
+    }
+  } else {
+    prior_accept(v, m, c, sweep, dyn);
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) part += m.pdg[c * m.L + l];
+    if (threadIdx.x == 0) part += m.probg[c];
+  }
+  part = block_sum(part, sh);
+  if (threadIdx.x == 0) m.S[c] = part;
+}
+
+// One workgroup per item: an accepted proposal replaces the chain's
+// genealogy (edges, times, populations, alleles, program, migrations and
+// the slot's root, stack depth and migration count).  The host checked
+// that the chain's slot holds the proposal's migrations.
+__global__ __launch_bounds__(256) void mc3_copy_accepted(DeviceView v, MC3Dev m) {
+  const int it = blockIdx.x;
+  if (!m.idec[it]) return;
+  SlotMeta *slots = const_cast<SlotMeta *>(v.slots);
+  const SlotMeta src = slots[m.prop_first + it];
+  SlotMeta &dst = slots[m.first_slot + it];
+  const int64_t so = src.node_off, d = dst.node_off;
+  const int ne = 2 * src.n - 1;
+  int32_t *up0 = const_cast<int32_t *>(v.up0), *up1 = const_cast<int32_t *>(v.up1);
+  int32_t *dn = const_cast<int32_t *>(v.down), *pop = const_cast<int32_t *>(v.pop);
+  int32_t *A = const_cast<int32_t *>(v.A);
+  int32_t *mf = const_cast<int32_t *>(v.mig_first), *mc = const_cast<int32_t *>(v.mig_count);
+  double *tm = const_cast<double *>(v.time);
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    up0[d + e] = up0[so + e];
+    up1[d + e] = up1[so + e];
+    dn[d + e] = dn[so + e];
+    tm[d + e] = tm[so + e];
+    if (pop) pop[d + e] = pop[so + e];
+    if (A) A[d + e] = A[so + e];
+    if (mf) mf[d + e] = mf[so + e];
+    if (mc) mc[d + e] = mc[so + e];
+  }
+  ProgOp *prog = const_cast<ProgOp *>(v.prog);
+  for (int k = threadIdx.x; k < src.n - 1; k += blockDim.x)
+    prog[dst.prog_off + k] = prog[src.prog_off + k];
+  int32_t *me = const_cast<int32_t *>(v.mig_edge), *mto = const_cast<int32_t *>(v.mig_to);
+  double *mt = const_cast<double *>(v.mig_time);
+  for (int k = threadIdx.x; k < src.nmig; k += blockDim.x) {
+    me[dst.mig_off + k] = me[src.mig_off + k];
+    mto[dst.mig_off + k] = mto[src.mig_off + k];
+    mt[dst.mig_off + k] = mt[src.mig_off + k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    dst.root = src.root;
+    dst.nslots = src.nslots;
+    dst.nmig = src.nmig;
+  }
+}
+
+__global__ void mc3_host_finish(MC3Dev m) {
+  *m.counter += 1;
+  m.stats[0] += (unsigned long long)(m.C * m.L);
+}
+
+__global__ void mc3_prop_items(MC3Dev m, const ima_item *items, ima_item *pitems, int32_t *ptw) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= m.C * m.L) return;
+  ima_item x = items[it];
+  x.slot = m.prop_first + it;
+  pitems[it] = x;
+  ptw[it] = m.prop_first + it;
+}
+
+hipError_t enqueue_mc3_prop_items(const MC3Dev &m, const ima_item *items, ima_item *pitems,
+                                  int32_t *ptw, hipStream_t st) {
+  const int N = m.C * m.L;
+  hipLaunchKernelGGL(mc3_prop_items, dim3((N + 255) / 256), dim3(256), 0, st, m, items, pitems,
+                     ptw);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_mc3_decide(const DeviceView &v, const MC3Dev &m, hipStream_t st) {
+  const size_t dl = m.prior >= 0 ? PriorLds(m.R, m.nparam, m.pchunk, m.mints).bytes() : 0;
+  hipLaunchKernelGGL(mc3_decide, dim3(m.C), dim3(1024), dl, st, v, m);
+  hipLaunchKernelGGL(mc3_copy_accepted, dim3(m.C * m.L), dim3(256), 0, st, v, m);
+  hipLaunchKernelGGL(mc3_host_finish, dim3(1), dim3(1), 0, st, m);
+  return hipGetLastError();
+}
+
+// One scoring pass (see ima_internal.h).  The proposals' genealogy weights
+// run in the HKY launch (IS and SW launches beside it on the side stream);
+// without HKY items (or for genealogies too large for that launch's LDS)
+// they run on the side stream beside the likelihood launches.  The caller's
+// next launch on st waits for both streams.
+hipError_t enqueue_mc3_score(const DeviceView &v, const MC3Dev &m, const ima_item *items,
+                             const int32_t *tw_slots, const int32_t *idx, const HkyPlan &p,
+                             int nh, int ni, int ns, int precision, int hmax_n, int hmax_slots,
+                             int imax_n, int imax_w, hipEvent_t *score_ev, int pass,
+                             hipStream_t st, hipStream_t side, hipEvent_t *fj) {
+  const bool prior = m.prior >= 0;
+  const TwLaunch tw{tw_slots, m.C * m.L, m.emax, m.tv, m.tv_row, m.wnew, m.wstatus};
+  hipError_t e = hipSuccess;
+  if (prior && nh && hky_tw_lds_bytes(precision, hmax_n, m.P, m.emax) <= kTwLdsBudget) {
+    // IS and SW items, if any, are scored on the side stream beside it
+    const bool fork = ni || ns;
+    if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);
+    if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass], st);
+    if (e == hipSuccess && fork) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
+    if (e == hipSuccess && ni)
+      e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
+                    nullptr, side);
+    if (e == hipSuccess && ns)
+      e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, side);
+    if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass + 1], side);
+    if (e == hipSuccess)
+      e = launch_hky_tw(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, tw,
+                        m.P, st);
+    if (e == hipSuccess && fork) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
+    if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
+    return e;
+  }
+  if (prior) {
+    e = hipEventRecord(fj[2 * pass], st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
+    if (e == hipSuccess)
+      e = launch_treeweight(v, tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax, m.wnew,
+                            m.wstatus, side);
+    if (e == hipSuccess) e = hipEventRecord(fj[2 * pass + 1], side);
+  }
+  if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass], st);
+  if (e == hipSuccess && nh)
+    e = launch_hky(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, st);
+  if (e == hipSuccess && ni)
+    e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr, nullptr, st);
+  if (e == hipSuccess && ns)
+    e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
+  if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
+  if (e == hipSuccess && prior) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
+  return e;
+}
+
 // Enqueue the kernels of one sweep (no host synchronisation).  The sweep's
 // proposals are already in place (mc3_propose or the previous sweep).
 hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_scale,
@@ -1014,46 +1182,9 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   // genealogies too large for that launch's LDS) they run on the side
   // stream beside the likelihood launches.  The accept kernel waits for
   // both streams
-  const TwLaunch tw{m.tw_slots, m.C * m.L, m.emax, m.tv, m.tv_row, m.wnew, m.wstatus};
   auto score = [&](int pass) -> hipError_t {
-    hipError_t e = hipSuccess;
-    if (prior && nh && hky_tw_lds_bytes(precision, hmax_n, m.P, m.emax) <= kTwLdsBudget) {
-      // IS and SW items, if any, are scored on the side stream beside it
-      const bool fork = ni || ns;
-      if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);
-      if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass], st);
-      if (e == hipSuccess && fork) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
-      if (e == hipSuccess && ni)
-        e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
-                      nullptr, side);
-      if (e == hipSuccess && ns)
-        e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, side);
-      if (e == hipSuccess && fork) e = hipEventRecord(fj[2 * pass + 1], side);
-      if (e == hipSuccess)
-        e = launch_hky_tw(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, tw,
-                          m.P, st);
-      if (e == hipSuccess && fork) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
-      if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
-      return e;
-    }
-    if (prior) {
-      e = hipEventRecord(fj[2 * pass], st);
-      if (e == hipSuccess) e = hipStreamWaitEvent(side, fj[2 * pass], 0);
-      if (e == hipSuccess)
-        e = launch_treeweight(v, m.tw_slots, m.C * m.L, m.tv, m.tv_row, m.P, m.emax, m.wnew,
-                              m.wstatus, side);
-      if (e == hipSuccess) e = hipEventRecord(fj[2 * pass + 1], side);
-    }
-    if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass], st);
-    if (e == hipSuccess && nh)
-      e = launch_hky(v, items, p, precision, hmax_n, hmax_slots, m.newpdg, m.status, st);
-    if (e == hipSuccess && ni)
-      e = launch_is(v, items, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr, nullptr, st);
-    if (e == hipSuccess && ns)
-      e = launch_sw(v, items, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
-    if (e == hipSuccess && score_ev) e = hipEventRecord(score_ev[2 * pass + 1], st);
-    if (e == hipSuccess && prior) e = hipStreamWaitEvent(st, fj[2 * pass + 1], 0);
-    return e;
+    return enqueue_mc3_score(v, m, items, m.tw_slots, idx, p, nh, ni, ns, precision, hmax_n,
+                             hmax_slots, imax_n, imax_w, score_ev, pass, st, side, fj);
   };
   hipError_t e = score(0);
   if (e != hipSuccess) return e;

@@ -363,6 +363,19 @@ class Engine:
                                         C.byref(sweeps)), "mc3_read")
         return pdg.reshape(C_, L_), beta, stats, sweeps.value
 
+    def mc3_decide(self, prop_first, hastings=None, uniforms=None, gbeta=1.0):
+        """Decisions for host-written proposals in slots prop_first + c*L + l
+        (ima_mc3_decide); returns the accepted flags [nchains, nloci]."""
+        C_, L_ = self.mc3_shape
+        acc = np.zeros(C_ * L_, np.int32)
+        h = None if hastings is None else np.ascontiguousarray(np.ravel(hastings), np.float64)
+        u = None if uniforms is None else np.ascontiguousarray(np.ravel(uniforms), np.float64)
+        self._chk(self.lib.ima_mc3_decide(
+            self.h, int(prop_first), None if h is None else h.ctypes.data_as(L.dp),
+            None if u is None else u.ctypes.data_as(L.dp), float(gbeta),
+            acc.ctypes.data_as(L.i32p)), "mc3_decide")
+        return acc.reshape(C_, L_)
+
     def mc3_set_prior(self, tvals, tmax, P):
         """Genealogy prior in the sweep; tvals [nchains, P-1] split times."""
         C_, _ = self.mc3_shape
@@ -381,12 +394,16 @@ class Engine:
         self._chk(self.lib.ima_mc3_prior_group(self.h, C.byref(out)), "mc3_prior_group")
         return out.value
 
-    def mc3_tune(self, sweeps=20):
-        """Autotune the accept step's group size; the chains are left as
-        they were.  Returns (chosen size, ms per sweep for sizes 1..6)."""
+    def mc3_tune(self, sweeps=20, flags=None):
+        """Autotune the accept step's group size on the sweeps `flags`
+        describe (ima_mc3_run's flags; default GRAPH | SCALE); the chains
+        are left as they were.  Returns (chosen size, ms per sweep for
+        sizes 1..6)."""
+        if flags is None:
+            flags = L.MC3_GRAPH | L.MC3_SCALE
         best = C.c_int()
         ms = np.zeros(6)
-        self._chk(self.lib.ima_mc3_tune(self.h, int(sweeps), C.byref(best),
+        self._chk(self.lib.ima_mc3_tune(self.h, int(sweeps), int(flags), C.byref(best),
                                         ms.ctypes.data_as(L.dp)), "mc3_tune")
         return best.value, ms
 

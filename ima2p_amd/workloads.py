@@ -223,3 +223,34 @@ def uneven_mixed(nchains=256, seed=3, hky=150, is_=100, sw=50, copies=(4, 60),
                         tmax=5.0, seed=seed,
                         precision=L.PREC_FP32 if precision is None else precision,
                         nsweeps=0, A=[As] * nchains)
+
+
+def serving_requests(rng, duration_s=20.0, rate=20.0, nsweeps=100,
+                     tab_sizes=(10000, 100000, 1000000), pool_size=48):
+    """The serving stream's requests, generated before it starts: Poisson
+    arrivals at `rate`/s for `duration_s` s; 70% L-mode query batches of
+    1..4096 points (log-uniform) on resident tables of `tab_sizes`
+    genealogies, 30% new IM analyses with 2..4 populations, 5..300 mixed
+    HKY/IS/SW loci and 16..128 chains running `nsweeps` sweeps.
+
+    Returns (tables, pools, requests); requests[i] = (arrival time, "q",
+    (table index, points)) or (arrival time, "a", AnalysisSpec)."""
+    pools = {P: serving_pool(rng, P, size=pool_size) for P in (2, 3, 4)}
+    tabs = [sim.lmode_table(rng, G) for G in tab_sizes]
+    lo, hi = np.array([0.5, 0.5, 0.5, 0.01, 0.01]), np.array([10, 10, 10, 2, 2])
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        if rng.random() < 0.7:
+            k = int(rng.integers(0, len(tabs)))
+            npts = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+            reqs.append((t, "q", (k, rng.uniform(lo, hi, (npts, 5)))))
+        else:
+            P = int(rng.integers(2, 5))
+            spec = serving_analysis(rng, pools[P], int(rng.integers(5, 301)),
+                                    int(rng.integers(16, 129)), nsweeps=nsweeps,
+                                    seed=int(rng.integers(1, 1 << 30)))
+            reqs.append((t, "a", spec))
+    return tabs, pools, reqs

@@ -331,13 +331,48 @@ int ima_mc3_set_prior(ima_ctx *ctx, const double *tvals, double tmax);
  * scenarios at once; decisions are the same for every D.  No counterpart
  * in the reference, which decides one locus at a time. */
 int ima_mc3_set_prior_group(ima_ctx *ctx, int loci);
+/* Host-only planning, no device needed: the accept group size set_prior
+ * would choose for a population model with these sizes (ima_model_add's
+ * npops, npsize, nmig and the lengths of theta_wp / mig_wp in entries) and
+ * the dynamic LDS the sweep's per-chain kernels then take.  Every launch
+ * stays within 64 KiB of dynamic LDS; IMA_ERR_SIZE (with *loci = 0) when
+ * even one locus per group does not fit. */
+int ima_mc3_prior_plan(int npops, int npsize, int ntheta_wp, int nmig, int nmig_wp,
+                       int *loci, int64_t *lds_bytes);
 int ima_mc3_prior_group(ima_ctx *ctx, int *loci);
 /* Autotuner of the sweep: times `sweeps` captured sweeps for every group
  * size that fits, each from the same saved state, keeps the fastest
- * (*best; 0 without the prior) and restores the chains, so the sweeps
- * that follow are those of an untuned run.  ms_per_sweep (may be NULL,
- * 6 entries) receives each size's time, -1 where it was not tried. */
-int ima_mc3_tune(ima_ctx *ctx, int sweeps, int *best, double *ms_per_sweep);
+ * (*best; 0 without the prior) and restores the chains -- genealogies,
+ * likelihoods, weights, split times, rates and counters -- so the sweeps
+ * that follow are those of an untuned run.  `flags` are the ima_mc3_run
+ * flags the caller will use (IMA_MC3_GRAPH is implied, IMA_MC3_TIMED and
+ * IMA_MC3_U_SERIAL are ignored): the group size is timed on those sweeps.
+ * ms_per_sweep (may be NULL, 6 entries) receives each size's time, -1
+ * where it was not tried. */
+int ima_mc3_tune(ima_ctx *ctx, int sweeps, int flags, int *best, double *ms_per_sweep);
+/* Decisions for genealogy proposals made by IMa2p's own updategenealogy
+ * (update_gtree.cpp:723-966: slider / joinsisdown / splitsisdown /
+ * addmigration on the host).  The host wrote chain c's proposal at locus l
+ * into slot prop_first + c*nloci + l (ima_slots_write, ima_slots_write_pops,
+ * ima_sw_update_A for the allele at a stepwise locus); those slots must not
+ * overlap the chains' own.  One call scores every proposal in one batched
+ * launch per model with its genealogy weights, then decides each chain's
+ * loci in order, as the sequence of updategenealogy calls in qupdate does
+ * (ima_main_mpi.cpp:1821-1841): accept when
+ *   U < min(1, exp(beta (dprobg + gbeta (newpdg - pdg)) + h))
+ * (update_gtree.cpp:919-927) with probg from integrate_tree_prob over the
+ * chain's all-loci weights after the earlier loci' decisions, h =
+ * hastings[i] (migweight + slideweight + Atermsum; NULL: 0) and U =
+ * uniforms[i] (the uniform() of the decision; NULL: the engine's own
+ * counter-based stream).  An incompatible infinite-sites genealogy or a
+ * zero stepwise term is rejected (rejectIS).  gbeta is the reference's
+ * thermodynamic-integration multiplier (1 otherwise).  Accepted proposals
+ * replace the chains' genealogies (slots, pdg, weights, probg); rejected
+ * ones leave them as they were.  accepted (may be NULL) receives 1 per
+ * accepted item.  The chains' pending device proposals are never touched:
+ * call between ima_mc3_run calls. */
+int ima_mc3_decide(ima_ctx *ctx, int prop_first, const double *hastings,
+                   const double *uniforms, double gbeta, int32_t *accepted);
 /* probg[nchains] and split times tvals[nchains][P-1] of the sweep. */
 int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
 /* Migration times of one slot as the device holds them (the rescaling

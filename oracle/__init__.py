@@ -410,6 +410,37 @@ def integrate_tree_prob(weights_list, model):
     return float(sum(q) + sum(m)), q, m
 
 
+def updategenealogy_decisions(pdg, weights, new_pdg, new_weights, ok, hastings, uniforms,
+                              beta, model, gbeta=1.0):
+    """The accept step of updategenealogy (update_gtree.cpp:900-961) for one
+    chain's loci taken in order, as qupdate calls it (ima_main_mpi.cpp:
+    1821-1841): the chain's all-loci weights are the sum of its loci'
+    weights (allgweight); locus l's proposal has log-likelihood new_pdg[l],
+    weights new_weights[l] and Hastings term hastings[l] (migweight +
+    slideweight + Atermsum); ok[l] is False for a rejected infinite-sites or
+    zero stepwise likelihood (rejectIS).  tpw = probg' - probg with probg'
+    from integrate_tree_prob over the weights with locus l's replaced;
+    accept iff U < min(1, exp(beta (tpw + gbeta (new_pdg - pdg)) + h)).
+    Returns (decisions, pdg, weights, probg) after the last locus."""
+    pdg = list(pdg)
+    weights = list(weights)
+    probg = integrate_tree_prob(weights, model)[0]
+    dec = []
+    for l in range(len(pdg)):
+        acc = False
+        if ok[l]:
+            trial = weights[:l] + [new_weights[l]] + weights[l + 1:]
+            pn = integrate_tree_prob(trial, model)[0]
+            x = beta * ((pn - probg) + gbeta * (new_pdg[l] - pdg[l])) + hastings[l]
+            acc = uniforms[l] < min(1.0, float(np.exp(x)))
+            if acc:
+                weights = trial
+                pdg[l] = new_pdg[l]
+                probg = pn
+        dec.append(bool(acc))
+    return dec, pdg, weights, probg
+
+
 def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
               seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False, urate=False,
               rates=None):

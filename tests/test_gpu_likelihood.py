@@ -1,7 +1,7 @@
 """GPU parity of the batched data likelihoods against the oracle.
 
 Bars: FP64 HKY, IS and SW within 1e-10 relative of the oracle; IS 0/1 node
-of every site bit-exact; FP32-scaled HKY within 2e-6 relative (the stated
+of every site bit-exact; FP32-scaled HKY within 1e-6 relative (the stated
 FP32 tolerance, measured error is printed).
 """
 import numpy as np
@@ -15,7 +15,7 @@ from tests import datasets as D
 pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-10
-FP32_RTOL = 2e-6
+FP32_RTOL = 1e-6
 
 
 def rel(a, b):

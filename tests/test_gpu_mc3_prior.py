@@ -74,7 +74,7 @@ def _current(e, g, slot, times):
     return h
 
 
-@pytest.mark.parametrize("prec,tol,ry1", [(L.PREC_FP64, 1e-9, False), (L.PREC_FP32, 2e-6, False),
+@pytest.mark.parametrize("prec,tol,ry1", [(L.PREC_FP64, 1e-9, False), (L.PREC_FP32, 1e-6, False),
                                           (L.PREC_FP64, 1e-9, True)])
 def test_state_matches_oracle_after_sweeps(prec, tol, ry1):
     """ry1: the time update is changet_RY1's rubber band on one split time."""
@@ -222,21 +222,26 @@ def test_decisions_equal_for_every_group_size():
             np.testing.assert_array_equal(a, b)
 
 
-def test_tuner_leaves_the_chains_where_they_were():
-    """ima_mc3_tune times every group size from a saved state and restores
-    it: the sweeps that follow equal those of an untuned context."""
+@pytest.mark.parametrize("flags", [L.MC3_GRAPH | L.MC3_SCALE,
+                                   L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U])
+def test_tuner_leaves_the_chains_where_they_were(flags):
+    """ima_mc3_tune times every group size, on the caller's sweeps (with
+    the rubber band and the mutation-rate rounds in the second case), from
+    a saved state and restores it -- rates included: the sweeps that follow
+    equal those of an untuned context."""
     out = []
     for tune in (False, True):
         e, *_ = _three_pop_chains(6, 9, 61, L.PREC_FP32)
-        e.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE)
+        e.mc3_run(5, flags)
         if tune:
-            best, ms = e.mc3_tune(8)
+            best, ms = e.mc3_tune(8, flags)
             assert 1 <= best <= 6 and ms[best - 1] > 0 and ms[best - 1] == ms[ms > 0].min()
             assert e.mc3_prior_group() == best
-        e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE)
+        e.mc3_run(20, flags)
         pdg, beta, stats, sweeps = e.mc3_read()
         probg, tv = e.mc3_read_prior()
-        out.append((pdg, beta, probg, tv, stats, sweeps))
+        u, kappa = e.mc3_read_rates()
+        out.append((pdg, beta, probg, tv, stats, sweeps, u, kappa))
         e.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)

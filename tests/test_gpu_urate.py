@@ -8,7 +8,7 @@ changekappa (:387-429) for a lone HKY locus.
   Both on HKY, IS and SW loci together, and with the genealogy prior.
 * After rounds every stored log-likelihood equals the oracle's at the
   chain's current (u, kappa) for the genealogy in its slot (1e-10 FP64,
-  2e-6 FP32); each chain's product of rates is unchanged (u_j d, u_k / d);
+  1e-6 FP32); each chain's product of rates is unchanged (u_j d, u_k / d);
   kappa moves only for HKY loci and stays in [0, 100].
 * A lone HKY locus moves kappa only.
 """
@@ -67,7 +67,7 @@ def test_grouped_equals_serial_with_the_prior():
     assert runs[0][2][6] == 6 * 8 * 12
 
 
-@pytest.mark.parametrize("prec,tol", [(L.PREC_FP64, 1e-10), (L.PREC_FP32, 2e-6)])
+@pytest.mark.parametrize("prec,tol", [(L.PREC_FP64, 1e-10), (L.PREC_FP32, 1e-6)])
 def test_state_consistent_after_rate_rounds(prec, tol):
     C = 6
     e = Engine(0)

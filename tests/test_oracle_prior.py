@@ -128,6 +128,30 @@ def test_treeweight_matches_lineage_tracing(seed, three):
     np.testing.assert_array_equal(w["mc"], mc)
     np.testing.assert_allclose(w["fc"], fc, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(w["fm"], fm, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("h", [0.25, 1.5, 4.0])
+def test_treeweight_inheritance_scalar(h):
+    """With inheritance scalar h != 1 (L[li].hval, update_gtree_common.cpp:
+    1823, 1922-1927) the coalescent weight is n(n-1) dt / (2h) and
+    hcc = cc log h; counts and migration weights do not change.  Checked
+    against lineage tracing with the same h."""
+    rng = np.random.default_rng(11)
+    pt, theta, ts, mig = sim.im_model_three_pop(rng)
+    samp = [5, 6, 4]
+    model = immodel.build(pt)
+    g = sim.relabel_internal(sim.simulate_genealogy(rng, samp, theta, ts, mig, pt), rng)
+    w = oracle.treeweight(g, model, ts, samp, hval=h)
+    cc, fc, mc, fm = lineage_weights(g, model, ts, samp, h=h)
+    np.testing.assert_array_equal(w["cc"], cc)
+    np.testing.assert_array_equal(w["mc"], mc)
+    np.testing.assert_allclose(w["fc"], fc, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(w["fm"], fm, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(w["hcc"], cc * math.log(h), rtol=1e-14, atol=0)
+    # h scales fc exactly: the h = 1 weights divided by h
+    w1 = oracle.treeweight(g, model, ts, samp, hval=1.0)
+    np.testing.assert_allclose(w["fc"], np.asarray(w1["fc"]) / h, rtol=1e-13, atol=1e-13)
+    assert not np.any(w1["hcc"])
     assert abs(w["length"] - g.length()) <= 1e-12 * g.length()
     assert w["cc"].sum() == g.n - 1
 

@@ -207,3 +207,76 @@ def test_swapchains_sequence_by_hand():
     assert acc == 2 and used == 6
     assert pairs.tolist() == [[0, 1, 1], [3, 2, 1]]
     np.testing.assert_array_equal(b, [0.9, 1.0, 0.7, 0.8])
+
+
+def test_margincalc_exponential_migration_prior():
+    """margincalc with an exponential prior on migration rates
+    (surface_call_functions.cpp:128-134, 150-153): each genealogy's term
+    gains log(1/mean) - x/mean; population-size parameters are unaffected.
+    Checked against numpy with and without the logarithm."""
+    rng = np.random.default_rng(5)
+    tab = sim.lmode_table(rng, 800).astype(np.float32)
+    L = tab.astype(np.float64)
+    mean, yadj = 0.7, 0.25
+    for x in (0.05, 0.8, 3.5):
+        for p in range(2):
+            mc = np.floor(L[:, 9 + p] + 0.5)
+            t = -L[:, 16 + p] + math.log(1.0 / mean) - x / mean + mc * math.log(x) - L[:, 11 + p] * x
+            ref = math.log(np.exp(t).mean()) - yadj
+            got = oracle.margincalc(tab, 3, 2, x, 3 + p, 1, yadj, True, mean)
+            assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+            got_lin = oracle.margincalc(tab, 3, 2, x, 3 + p, 0, 0.0, True, mean)
+            assert abs(got_lin - np.exp(t).mean()) <= 1e-12 * np.exp(t).mean()
+        # a population-size parameter ignores the migration prior
+        assert oracle.margincalc(tab, 3, 2, x, 1, 1, yadj, True, mean) == \
+            oracle.margincalc(tab, 3, 2, x, 1, 1, yadj, False, mean)
+
+
+def test_setheat_geometric_values_by_hand():
+    """setheat, HGEOMETRIC (swapchains.cpp:152-153):
+    beta_i = 1 - (1 - hb) i ha^(N-1-i) / (N-1).  Worked by hand for N = 4,
+    ha = 0.96, hb = 0.9:  beta_1 = 1 - 0.1 * 0.9216 / 3 = 0.96928,
+    beta_2 = 1 - 0.2 * 0.96 / 3 = 0.936, beta_3 = 1 - 0.3 / 3 = 0.9.
+    Without the ha^(N-1-i) factor beta_1 would be 0.966666..."""
+    got = [oracle.setheat_geometric(i, 4, 0.96, 0.9) for i in range(4)]
+    assert got[0] == 1.0
+    assert got[1] == pytest.approx(0.96928, abs=1e-14)
+    assert got[2] == pytest.approx(0.936, abs=1e-14)
+    assert got[3] == pytest.approx(0.9, abs=1e-14)
+    # N = 3, ha = 0.5, hb = 0.6: beta_1 = 1 - 0.4 * 0.5 / 2 = 0.9
+    assert oracle.setheat_geometric(1, 3, 0.5, 0.6) == pytest.approx(0.9, abs=1e-14)
+
+
+def test_swapchains_partner_window_by_hand():
+    """The partner window of swapchains (swapchains.cpp:554-567): with at
+    least 2*SWAPDIST + 3 = 17 chains, cj = cjmin + int(U cjrange) with
+    cjmin = max(0, ci - 7) and cjrange = min(N, ci + 7) - cjmin, so cj runs
+    over ci-7 .. ci+6; with fewer chains over every chain.  Every index
+    below is worked by hand; the weights are made >= 1 (lower S on the
+    hotter chain is never chosen) or < 1 as stated, so the uniforms each
+    try consumes are known."""
+    N = 20
+    beta = np.linspace(1.0, 0.62, N)   # strictly decreasing
+    S = -100.0 - np.arange(N, dtype=float)  # hotter chains have lower S
+    # (b_i - b_j)(S_j - S_i): for i < j, b_i > b_j and S_j < S_i -> w < 1;
+    # the acceptance uniform is then drawn and 0.999 rejects every try.
+    # try 1: U = 0.52 -> ci = 10; cjmin = 3, cjrange = 17 - 3 = 14;
+    #        U = 0.0 -> cj = 3; w < 1, U = 0.999 -> rejected
+    # try 2: U = 0.0 -> ci = 0; cjmin = 0, cjrange = 7; U = 0.99 -> cj = 6
+    # try 3: U = 0.999 -> ci = 19; cjmin = 12, cjrange = 8; U = 0.0 -> cj = 12
+    # try 4: U = 0.5 -> ci = 10; U = 0.999 -> cj = 3 + 13 = 16 (= ci + 6)
+    uni = [0.52, 0.0, 0.999, 0.0, 0.99, 0.999, 0.999, 0.0, 0.999, 0.5, 0.999, 0.999]
+    b, acc, pairs, used = oracle.swapchains(S, beta, 4, uni)
+    assert used == 12 and acc == 0
+    assert pairs.tolist() == [[10, 3, 0], [0, 6, 0], [19, 12, 0], [10, 16, 0]]
+    np.testing.assert_array_equal(b, beta)
+    # 17 chains is windowed: U = 0.5 -> ci = 8; cjmin = 1, cjrange = 15 - 1
+    # = 14; U = 0.999 -> cj = 14.  16 chains is not: U = 0.5 -> ci = 8,
+    # U = 0.999 -> cj = int(0.999 * 16) = 15.
+    for n, cj in ((17, 14), (16, 15)):
+        _, _, pairs, used = oracle.swapchains(S[:n], beta[:n], 1, [0.5, 0.999, 0.999])
+        assert pairs.tolist() == [[8, cj, 0]] and used == 3
+    # a partner equal to ci is drawn again: N = 20, ci = 10, U = 7/14 + eps
+    # gives cj = 3 + 7 = 10 -> redraw U = 0.2 -> cj = 3 + 2 = 5
+    _, _, pairs, used = oracle.swapchains(S, beta, 1, [0.52, 0.51, 0.2, 0.999])
+    assert pairs.tolist() == [[10, 5, 0]] and used == 4
