@@ -123,28 +123,51 @@ def load_engine(wl, prec, prior=True):
     return e, loci, first, model
 
 
-def plain_fp64_path(e, loci, model, wl, sweeps, nchains):
-    """One likelihood call = one upload + one FP64 launch + one readback;
-    one prior call per proposal likewise (the chain's genealogies in
-    slots of their own).  Its step has no mutation-rate rounds (the
-    batched step's changeu adds 2 likelihoods per locus every fifth step),
-    so it slightly flatters this path."""
-    from ima2p_amd.hostmc3 import EnginePrior, HostMC3
-    chains = [[g.copy() for g in wl.chains[c]] for c in range(nchains)]
-    Lc = wl.nloci
-    first = e.alloc_slots([loci[l] for c in range(nchains) for l in range(Lc)])
-    slots = [[first + c * Lc + l for l in range(Lc)] for c in range(nchains)]
-
-    def lik(c, l, g, dirty, commit):
-        return e.likelihood_hky(loci[l], g, wl.u[l], wl.kappa, e=(dirty[0],) * 4)
-
-    h = HostMC3(chains, workloads.geometric_betas(nchains), lik, lambda c, l, a: None, seed=7,
-                prior=EnginePrior(e, model, slots), tv=wl.tsplit, tmax=wl.tmax, ry1=True)
-    t0 = time.perf_counter()
-    for k in range(sweeps):
-        h.sweep(with_scale=bool(k & 1))
-    dt = time.perf_counter() - t0
-    return nchains * sweeps / dt
+def plain_leg(wl, sweeps=2):
+    """The plain per-locus FP64 GPU path (libima_plain.so, C++ over the
+    ABI): every chain, every locus, one proposal at a time -- one slot
+    upload, one FP64 likelihood launch, one treeweight launch, the chain's
+    all-loci sums updated on the host, one prior-integral launch and the
+    decision, then one swap per sweep.  The batched comparator runs the
+    same step (genealogy update of every locus with the prior, then the
+    swap; no time or rate updates) from captured graphs."""
+    from ima2p_amd import _lib as L
+    from ima2p_amd import immodel, plain
+    from ima2p_amd.engine import Engine
+    C, Lc = wl.nchains, wl.nloci
+    e = Engine(0)
+    model = immodel.build(wl.poptree)
+    mid = e.model_add(model)
+    loci = [e.add_locus_hky(loc) for loc in wl.loci]
+    for lid in loci:
+        e.locus_set_model(lid, mid, wl.samppop)
+    first = e.alloc_slots([loci[l] for c in range(C) for l in range(Lc)])
+    gens = [wl.chains[c][l] for c in range(C) for l in range(Lc)]
+    e.write_slots(first, gens)
+    e.write_pops(first, gens)
+    u = np.array([wl.u[l] for c in range(C) for l in range(Lc)])
+    plain.run(e, first, mid, model.npops, gens, C, Lc, u, wl.kappa, wl.beta,
+              np.tile(wl.tsplit, (C, 1)), nsweeps=0)  # first use of every launch
+    r = plain.run(e, first, mid, model.npops, gens, C, Lc, u, wl.kappa, wl.beta,
+                  np.tile(wl.tsplit, (C, 1)), nsweeps=sweeps)
+    e.close()
+    out = {"chain_steps_per_s": C * sweeps / r["seconds"], "sweeps": sweeps,
+           "us_per_proposal": r["seconds"] / (sweeps * C * Lc) * 1e6,
+           "accept": r["accepts"] / (sweeps * C * Lc)}
+    same = {}
+    for name, prec in (("fp32", L.PREC_FP32), ("fp64", L.PREC_FP64)):
+        eb, _, _, _ = load_engine(wl, prec)
+        eb.mc3_tune(6, L.MC3_GRAPH)
+        eb.mc3_run(16, L.MC3_GRAPH)
+        t0 = time.perf_counter()
+        eb.mc3_run(96, L.MC3_GRAPH)
+        same[name] = C * 96 / (time.perf_counter() - t0)
+        eb.close()
+    out["batched_same_step_fp32_chain_steps_per_s"] = same["fp32"]
+    out["batched_same_step_fp64_chain_steps_per_s"] = same["fp64"]
+    out["speedup_batched_fp32_vs_plain_fp64"] = same["fp32"] / out["chain_steps_per_s"]
+    out["speedup_batched_fp64_vs_plain_fp64"] = same["fp64"] / out["chain_steps_per_s"]
+    return out
 
 
 def lmode_leg(G=100000, npoints=4096, seed=4):
@@ -390,10 +413,16 @@ def main():
         k64 = max(10, args.steps // 4)
         e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
         extra["batched_fp64_chain_steps_per_s"] = C * k64 / (time.perf_counter() - t1)
-        plain = plain_fp64_path(e64, loci64, model64, wl, sweeps=4, nchains=4)
-        extra["plain_per_locus_fp64_chain_steps_per_s"] = plain
-        extra["speedup_vs_plain_fp64"] = value / plain
         e64.close()
+        pl = plain_leg(wl)
+        extra["plain_per_locus_fp64"] = pl
+        extra["plain_per_locus_fp64_chain_steps_per_s"] = pl["chain_steps_per_s"]
+        # the north star's ratio: batched FP32-scaled over plain per-locus
+        # FP64, both on the same step (genealogy updates with the prior + swap)
+        extra["speedup_vs_plain_fp64"] = pl["speedup_batched_fp32_vs_plain_fp64"]
+        extra["batched_fp32_vs_batched_fp64"] = value / extra["batched_fp64_chain_steps_per_s"]
+        if cpu:
+            extra["batched_fp32_vs_cpu_baseline"] = value / cpu["value"]
         extra["lmode"] = lmode_leg()
         extra["uneven_batch"] = uneven_leg()
         extra["is_case"] = is_leg()

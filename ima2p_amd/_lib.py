@@ -19,7 +19,7 @@ STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE, STATUS_PRIOR_BAD 
 LMODE_REF_COMPAT, LMODE_EXACT = 0, 1
 REJECT_IS = -1000000000.0
 MAX_GENES = 256
-K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3 = range(7)
+K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3, K_ACCEPT = range(8)
 MC3_GRAPH, MC3_SCALE, MC3_TIMED, MC3_RY1 = 1, 2, 4, 8
 MC3_UPDATE_U, MC3_U_SERIAL = 16, 32
 
@@ -112,6 +112,13 @@ PROTOS = {
     "ima_mc3_read": (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ima_mc3_read_rates": (C.c_int, [vp, dp, dp]),
     "ima_mc3_decide": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, i32p]),
+    "ima_mc3_record": (C.c_int, [vp, C.c_int]),
+    "ima_mc3_read_weights": (C.c_int, [vp, dp, dp]),
+    "ima_lmode_table_reserve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_int)]),
+    "ima_lmode_table_rows": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ima_lmode_table_read": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, fp]),
+    "ima_integrate_tree_prob": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp, dp]),
     "ima_mc3_prior_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "ima_timing_enable": (C.c_int, [vp, C.c_int]),

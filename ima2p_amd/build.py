@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libima_gpu.so")
+PLAIN = os.path.join(HERE, "libima_plain.so")
+CXX = os.environ.get("CXX", "g++")
 SOURCES = ["ctx.hip", "kern_hky.hip", "kern_lik.hip", "kern_lmode.hip", "kern_swap.hip", "kern_mc3.hip",
            "kern_prior.hip", "kern_urate.hip", "serve.hip"]
 
@@ -53,6 +55,15 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(out)
         raise RuntimeError("hipcc failed linking libima_gpu.so")
     os.replace(tmp, LIB)
+    # the plain per-locus path (a host program over the ABI, for measurement)
+    tmp = PLAIN + ".tmp"
+    rc, out = _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp,
+                    os.path.join(HERE, "plain", "plain_mc3.cpp"), f"-L{HERE}", "-lima_gpu",
+                    "-Wl,-rpath,$ORIGIN"], verbose)
+    if rc != 0:
+        sys.stderr.write(out)
+        raise RuntimeError("building libima_plain.so failed")
+    os.replace(tmp, PLAIN)
     return LIB
 
 

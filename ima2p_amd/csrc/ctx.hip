@@ -108,6 +108,19 @@ int upload_slot_meta(ima_ctx *c) {
   return IMA_OK;
 }
 
+// The metadata of slots [first, first + count) after a write: only those
+// records go to the device when its copy is otherwise current (a per-call
+// writer such as IMa2p's per-locus path changes one slot at a time), else
+// the whole table is marked for upload.  Enqueued on the context's stream.
+hipError_t upload_slot_meta_range(ima_ctx *c, int first, int count) {
+  if (c->slots_meta_dirty || !c->d_slots.p || c->d_slots.n < c->slots.size()) {
+    c->slots_meta_dirty = true;
+    return hipSuccess;
+  }
+  return hipMemcpyAsync(c->d_slots.p + first, c->slots.data() + first,
+                        (size_t)count * sizeof(SlotMeta), hipMemcpyHostToDevice, c->stream);
+}
+
 DeviceView view(ima_ctx *c) {
   DeviceView v;
   v.loci = c->d_loci.p;
@@ -546,8 +559,8 @@ int ima_slots_write(ima_ctx *c, int first, int count, const int32_t *up0, const 
   if (pb)
     HIPCHK(c, hipMemcpyAsync(c->d_prog.p + s0.prog_off, c->prog.data() + s0.prog_off,
                              pb * sizeof(ProgOp), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, upload_slot_meta_range(c, first, count));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->slots_meta_dirty = true;
   return IMA_OK;
 }
 
@@ -933,12 +946,53 @@ int ima_lmode_table(ima_ctx *c, const float *table, int G, int K, int npsize, in
   HIPCHK(c, hipStreamSynchronize(c->stream));
   t.live = true;
   t.G = G;
+  t.cap = G;
   t.K = K;
   t.npsize = npsize;
   t.nmig = nmig;
   t.npops = npops;
   *table_id = (int)c->tables.size();
   c->tables.push_back(t);
+  return IMA_OK;
+}
+
+int ima_lmode_table_reserve(ima_ctx *c, int capacity, int npsize, int nmig, int npops,
+                            int *table_id) {
+  if (!c || !table_id || capacity < 1 || npsize < 0 || nmig < 0 || npops < 1) return IMA_ERR_ARG;
+  if (npsize + nmig > 16 || npsize + nmig < 1) return fail(c, IMA_ERR_SIZE, "1..16 parameters supported");
+  const int K = 4 * npsize + 3 * nmig + 2 + (npops - 1);
+  if ((size_t)8 * K * 8 > (size_t)c->max_lds) return fail(c, IMA_ERR_SIZE, "row too long");
+  hipSetDevice(c->device);
+  LTableHost t;
+  HIPCHK(c, hipMalloc(&t.rows, (size_t)capacity * K * sizeof(float)));
+  t.live = true;
+  t.G = 0;
+  t.cap = capacity;
+  t.K = K;
+  t.npsize = npsize;
+  t.nmig = nmig;
+  t.npops = npops;
+  *table_id = (int)c->tables.size();
+  c->tables.push_back(t);
+  return IMA_OK;
+}
+
+int ima_lmode_table_rows(ima_ctx *c, int id, int *G, int *K) {
+  if (!c || id < 0 || id >= (int)c->tables.size() || !c->tables[id].live) return IMA_ERR_ARG;
+  if (G) *G = c->tables[id].G;
+  if (K) *K = c->tables[id].K;
+  return IMA_OK;
+}
+
+int ima_lmode_table_read(ima_ctx *c, int id, int first, int count, float *rows) {
+  if (!c || !rows || id < 0 || id >= (int)c->tables.size() || !c->tables[id].live) return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  if (first < 0 || count < 0 || first + count > h.G)
+    return fail(c, IMA_ERR_ARG, "rows [%d,%d) outside the table's %d", first, first + count, h.G);
+  if (count == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  HIPCHK(c, copy_now(c, rows, h.rows + (size_t)first * h.K, (size_t)count * h.K * sizeof(float),
+                     hipMemcpyDeviceToHost));
   return IMA_OK;
 }
 
@@ -970,6 +1024,7 @@ int ima_lmode_jointp(ima_ctx *c, int id, const double *x, int npoints, int model
     return IMA_ERR_ARG;
   const LTableHost &h = c->tables[id];
   const int np = h.npsize + h.nmig;
+  if (h.G < 1) return fail(c, IMA_ERR_STATE, "table %d holds no genealogies", id);
   if (r0 < 0 || r1 > np || r0 >= r1) return fail(c, IMA_ERR_ARG, "bad parameter range");
   if (modeltype < 0 || modeltype > 2 || (mode != IMA_LMODE_EXACT && mode != IMA_LMODE_REF_COMPAT))
     return IMA_ERR_ARG;
@@ -1005,6 +1060,7 @@ int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx,
       !c->tables[id].live)
     return IMA_ERR_ARG;
   const LTableHost &h = c->tables[id];
+  if (h.G < 1) return fail(c, IMA_ERR_STATE, "table %d holds no genealogies", id);
   if (param < 0 || param >= h.npsize + h.nmig) return fail(c, IMA_ERR_ARG, "bad parameter index");
   if (expo && !(prior_mean > 0)) return fail(c, IMA_ERR_ARG, "prior mean must be > 0");
   if (nx == 0) return IMA_OK;
@@ -1265,6 +1321,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     u.kmax = 100.0;
   }
   HIPCHK(c, hipMemcpyAsync(m.beta.p, beta, nchains * 8, hipMemcpyHostToDevice, c->stream));
+  m.beta0.assign(beta, beta + nchains);
   const uint64_t zero = 0;
   const unsigned long long zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(m.counter.p, &zero, 8, hipMemcpyHostToDevice, c->stream));
@@ -1484,6 +1541,14 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
           HIPCHK(c, hipEventElapsedTime(&ms, m.ev[2], m.ev[3]));
           T.ms[ks] += ms;
           T.launches[ks] += 1;
+        }
+        // the accept step: from the end of the first scoring pass to the
+        // start of the second (rescaling sweeps) or to the sweep's end
+        // (others, without a mutation-rate round)
+        if (with_scale || !u_due(q)) {
+          HIPCHK(c, hipEventElapsedTime(&ms, m.ev[1], with_scale ? m.ev[2] : T.ev1[K_MC3]));
+          T.ms[K_ACCEPT] += ms;
+          T.launches[K_ACCEPT] += 1;
         }
       } else if (use_graph) {
         HIPCHK(c, hipGraphLaunch(m.exec[with_scale ? 1 : 0], c->stream));
@@ -1849,6 +1914,8 @@ int ima_mc3_decide(ima_ctx *c, int prop_first, const double *hastings, const dou
       if (prior) {
         emax = std::max(emax, (s.n - 1) + s.nmig + P - 1);
         tot += (size_t)s.n - 1 + (size_t)std::max(s.nmig, d.nmig);
+      }
+      {
         if (s.nmig > d.mig_cap) {
           // the chain's slot must hold the proposal's migrations: move its
           // events to a larger region (the device copy keeps their times)
@@ -1884,6 +1951,16 @@ int ima_mc3_decide(ima_ctx *c, int prop_first, const double *hastings, const dou
   }
   if (prior && treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "a proposal has %d prior events", emax);
+  // per-edge migration indexes of every slot on the device (slots allocated
+  // after the last ima_slots_write_pops have none there yet)
+  if (c->d_mig_first.n < c->nodes || c->d_mig_count.n < c->nodes) {
+    HIPCHK(c, reserve(c->d_mig_first, c->nodes, false, c->stream));
+    HIPCHK(c, reserve(c->d_mig_count, c->nodes, false, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p, c->mig_first.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p, c->mig_count.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  }
   if (grew) {
     HIPCHK(c, reserve(m.mtbackup, c->migs + 1));
     m.dev.mtbackup = m.mtbackup.p;
@@ -1964,6 +2041,38 @@ int ima_mc3_decide(ima_ctx *c, int prop_first, const double *hastings, const dou
     dst.nmig = src.nmig;
   }
   if (accepted) std::copy(dec.begin(), dec.end(), accepted);
+  return IMA_OK;
+}
+
+int ima_mc3_record(ima_ctx *c, int table_id) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
+  if (table_id < 0 || table_id >= (int)c->tables.size() || !c->tables[table_id].live)
+    return fail(c, IMA_ERR_ARG, "table %d unknown", table_id);
+  LTableHost &h = c->tables[table_id];
+  const ModelMeta &mm = c->models[m.dev.prior];
+  if (h.npsize != mm.npsize || h.nmig != mm.nmig || h.npops != mm.npops)
+    return fail(c, IMA_ERR_ARG, "table %d does not have the sweep's parameters", table_id);
+  if (h.G >= h.cap)
+    return fail(c, IMA_ERR_SIZE, "table %d is full (%d genealogies)", table_id, h.cap);
+  if (std::find(m.beta0.begin(), m.beta0.end(), 1.0) == m.beta0.end())
+    return fail(c, IMA_ERR_STATE, "no chain has beta = 1 (no cold chain to record)");
+  hipSetDevice(c->device);
+  HIPCHK(c, enqueue_mc3_record(view(c), m.dev, h.rows + (size_t)h.G * h.K, c->stream));
+  h.G += 1;
+  return IMA_OK;
+}
+
+int ima_mc3_read_weights(ima_ctx *c, double *wall, double *qint) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live || m.dev.prior < 0) return fail(c, IMA_ERR_STATE, "the sweep has no prior");
+  hipSetDevice(c->device);
+  if (wall)
+    HIPCHK(c, copy_now(c, wall, m.wall.p, (size_t)m.C * m.dev.R * 8, hipMemcpyDeviceToHost));
+  if (qint && m.dev.nparam)
+    HIPCHK(c, copy_now(c, qint, m.qterm.p, (size_t)m.C * m.dev.nparam * 8, hipMemcpyDeviceToHost));
   return IMA_OK;
 }
 
@@ -2169,8 +2278,8 @@ int ima_slots_write_pops(ima_ctx *c, int first, int count, const int32_t *pop,
     HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p, c->mig_time.data(), c->migs * 8,
                              hipMemcpyHostToDevice, c->stream));
   }
+  HIPCHK(c, upload_slot_meta_range(c, first, count));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->slots_meta_dirty = true;
   return IMA_OK;
 }
 
@@ -2256,6 +2365,59 @@ int ima_treeweight(ima_ctx *c, const int32_t *slots, int nslots, const double *t
     }
     if (length) length[i] = r[R - 1];
   }
+  return IMA_OK;
+}
+
+int ima_integrate_tree_prob(ima_ctx *c, int model, const double *wsum, int ngroups,
+                            double *probg, double *qint, double *mint) {
+  if (!c || !wsum || !probg || ngroups < 0) return IMA_ERR_ARG;
+  if (model < 0 || model >= (int)c->models.size()) return fail(c, IMA_ERR_ARG, "model %d unknown", model);
+  if (ngroups == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  const ModelMeta &mm = c->models[model];
+  const int P = mm.npops, R = prior_rec_len(P), P2 = P * P, P3 = P2 * P;
+  // counts (cc, mc) are whole numbers >= 0, weights finite: the log-factorial
+  // table must reach the largest parameter count
+  size_t lfn = 4;
+  for (int g = 0; g < ngroups; ++g) {
+    const double *w = wsum + (size_t)g * R;
+    double tot = 0.0;
+    for (int r = 0; r < R; ++r) {
+      if (!std::isfinite(w[r])) return fail(c, IMA_ERR_ARG, "group %d: entry %d not finite", g, r);
+      const bool count = r < P2 || (r >= 3 * P2 && r < 3 * P2 + P3);
+      if (count) {
+        if (w[r] < 0 || w[r] != std::floor(w[r]))
+          return fail(c, IMA_ERR_ARG, "group %d: count entry %d is not a whole number", g, r);
+        tot += w[r];
+      }
+    }
+    if (tot > 1e8) return fail(c, IMA_ERR_SIZE, "group %d: %g events", g, tot);
+    lfn = std::max(lfn, (size_t)tot + 4);
+  }
+  grow_logfact(c, lfn);
+  int rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  std::vector<int32_t> goff(ngroups + 1);
+  for (int g = 0; g <= ngroups; ++g) goff[g] = g;
+  HIPCHK(c, reserve(c->d_rec, (size_t)ngroups * R));
+  HIPCHK(c, reserve(c->d_goff, ngroups + 1));
+  HIPCHK(c, reserve(c->d_probg, ngroups));
+  HIPCHK(c, reserve(c->d_qint, (size_t)ngroups * std::max(1, mm.npsize)));
+  HIPCHK(c, reserve(c->d_mintg, (size_t)ngroups * std::max(1, mm.nmig)));
+  HIPCHK(c, hipMemcpyAsync(c->d_rec.p, wsum, (size_t)ngroups * R * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_goff.p, goff.data(), (ngroups + 1) * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, launch_prior(view(c), model, P, mm.npsize + mm.nmig, c->d_rec.p, c->d_goff.p, ngroups,
+                         c->d_probg.p, c->d_qint.p, c->d_mintg.p, c->stream));
+  HIPCHK(c, hipMemcpyAsync(probg, c->d_probg.p, ngroups * 8, hipMemcpyDeviceToHost, c->stream));
+  if (qint && mm.npsize)
+    HIPCHK(c, hipMemcpyAsync(qint, c->d_qint.p, (size_t)ngroups * mm.npsize * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (mint && mm.nmig)
+    HIPCHK(c, hipMemcpyAsync(mint, c->d_mintg.p, (size_t)ngroups * mm.nmig * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return IMA_OK;
 }
 

@@ -10,6 +10,7 @@ struct LTableHost {
   bool live = false;
   float *rows = nullptr;
   int G = 0, K = 0, npsize = 0, nmig = 0, npops = 0;
+  int cap = 0;  // rows the device buffer holds (G <= cap)
 };
 
 }  // namespace ima
@@ -139,6 +140,7 @@ struct ima_ctx {
     DevBuf<double> hast, uhost;
     DevBuf<int32_t> idec, ptw;
     DevBuf<ima_item> pitems;
+    std::vector<double> beta0;  // the ladder set up (swaps permute it)
     // device buffers the captured graphs point at: a change (a buffer grown
     // by later slot allocations or writes) drops the graphs
     ima::DeviceView gview{};

@@ -113,7 +113,7 @@ struct Timing {
 };
 
 enum KClass { K_HKY64 = 0, K_HKY32 = 1, K_IS = 2, K_SW = 3, K_LMODE = 4,
-              K_SWAP = 5, K_MC3 = 6 };
+              K_SWAP = 5, K_MC3 = 6, K_ACCEPT = 7 };
 
 struct DeviceView {
   const LocusMeta *loci;
@@ -335,6 +335,8 @@ hipError_t enqueue_mc3_score(const DeviceView &v, const MC3Dev &m, const ima_ite
 // decisions of every chain's loci in order, the accepted proposals copied
 // into the chains' slots, the sweep counter advanced.
 hipError_t enqueue_mc3_decide(const DeviceView &v, const MC3Dev &m, hipStream_t st);
+// The cold chain's genealogy-table row (savegsampinf) into row[0..K).
+hipError_t enqueue_mc3_record(const DeviceView &v, const MC3Dev &m, float *row, hipStream_t st);
 // Items and weight slots of the proposals: items[it] with slot
 // prop_first + it, tw_slots[it] = prop_first + it.
 hipError_t enqueue_mc3_prop_items(const MC3Dev &m, const ima_item *items, ima_item *pitems,

@@ -998,6 +998,72 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
 }
 
+// ---- the genealogy table's rows (savegsampinf, ginfo.cpp:319-377) ----------
+
+// One row for the cold chain (the last chain whose beta is exactly 1.0,
+// whichiscoldchain ima_main_mpi.cpp:3023-3033) in the layout L mode reads
+// (ginfo.cpp:288-316): per population-size parameter the count (an int sum
+// of cc), then fc and hcc accumulated in float over the parameter's wp
+// positions in order; per migration parameter the count of mc and fm in
+// float; qintegrate and mintegrate; pdg (the chain's loci summed in order);
+// probg; the split times.  Everything is stored as float.
+__global__ void mc3_record_row(DeviceView v, MC3Dev m, float *row) {
+  __shared__ int z;
+  if (threadIdx.x == 0) {
+    int w = -1;
+    for (int c = 0; c < m.C; ++c)
+      if (m.beta[c] == 1.0) w = c;
+    z = w;
+  }
+  __syncthreads();
+  if (z < 0) return;  // the host checked that the ladder holds beta = 1
+  const ModelMeta mm = v.models[m.prior];
+  const ModelView md = model_view(v, mm);
+  const int P = m.P, P2 = P * P, P3 = P2 * P, R = m.R;
+  const int nps = mm.npsize, nmg = mm.nmig, np = m.nparam;
+  const double *all = m.wall + (int64_t)z * R;
+  const int ccp = 0, fcp = nps, hccp = 2 * nps, mcp = 3 * nps, fmp = 3 * nps + nmg;
+  const int qip = 3 * nps + 2 * nmg, mip = 4 * nps + 2 * nmg, pdgp = 4 * nps + 3 * nmg;
+  for (int i = threadIdx.x; i < nps; i += blockDim.x) {
+    int c = 0;
+    float f = 0.0f, hc = 0.0f;
+    for (int w = md.theta_off[i]; w < md.theta_off[i + 1]; ++w) {
+      const int e = md.theta_wp[2 * w] * P + md.theta_wp[2 * w + 1];
+      c += (int)all[e];
+      f += (float)all[P2 + e];
+      hc += (float)all[2 * P2 + e];
+    }
+    row[ccp + i] = (float)c;
+    row[fcp + i] = f;
+    row[hccp + i] = hc;
+    row[qip + i] = (float)m.qterm[(int64_t)z * np + i];
+  }
+  for (int i = threadIdx.x; i < nmg; i += blockDim.x) {
+    int c = 0;
+    float f = 0.0f;
+    for (int w = md.mig_off[i]; w < md.mig_off[i + 1]; ++w) {
+      const int e = (md.mig_wp[3 * w] * P + md.mig_wp[3 * w + 1]) * P + md.mig_wp[3 * w + 2];
+      c += (int)all[3 * P2 + e];
+      f += (float)all[3 * P2 + P3 + e];
+    }
+    row[mcp + i] = (float)c;
+    row[fmp + i] = f;
+    row[mip + i] = (float)m.qterm[(int64_t)z * np + nps + i];
+  }
+  if (threadIdx.x == 0) {
+    double pd = 0.0;
+    for (int l = 0; l < m.L; ++l) pd += m.pdg[(int64_t)z * m.L + l];
+    row[pdgp] = (float)pd;
+    row[pdgp + 1] = (float)m.probg[z];
+    for (int k = 0; k < P - 1; ++k) row[pdgp + 2 + k] = (float)m.tv[(int64_t)z * mc3_tw(P) + k];
+  }
+}
+
+hipError_t enqueue_mc3_record(const DeviceView &v, const MC3Dev &m, float *row, hipStream_t st) {
+  hipLaunchKernelGGL(mc3_record_row, dim3(1), dim3(kWave), 0, st, v, m, row);
+  return hipGetLastError();
+}
+
 // ---- decisions for proposals written by the host (ima_mc3_decide) -------
 
 // One workgroup per chain: every locus' proposal (in slot prop_first + it,

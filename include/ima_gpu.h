@@ -199,6 +199,17 @@ int ima_swap(ima_ctx *ctx, int nchains, const double *S, double *beta,
 int ima_lmode_table(ima_ctx *ctx, const float *table, int G, int K,
                     int npsize, int nmig, int npops, int *table_id);
 int ima_lmode_table_free(ima_ctx *ctx, int table_id);
+/* An empty resident table for `capacity` genealogies of the savegsampinf
+ * layout with npsize / nmig parameters and npops - 1 split times, filled
+ * on the device by ima_mc3_record (the reference's gsampinf array sized
+ * by MAXGENEALOGIESTOSAVE, imamp.hpp:149).  L-mode calls use the rows
+ * recorded so far. */
+int ima_lmode_table_reserve(ima_ctx *ctx, int capacity, int npsize, int nmig, int npops,
+                            int *table_id);
+/* Genealogies a table holds and values per genealogy. */
+int ima_lmode_table_rows(ima_ctx *ctx, int table_id, int *G, int *K);
+/* Rows [first, first + count) of a table, K floats each. */
+int ima_lmode_table_read(ima_ctx *ctx, int table_id, int first, int count, float *rows);
 /* jointp at npoints points x[npoints][npsize+nmig]; params
  * [prange0, prange1) are used; modeltype 0 all, 1 popsize-only, 2
  * migration-only (npops > 2).  mode REF_COMPAT reproduces the reference's
@@ -261,6 +272,15 @@ int ima_treeweight(ima_ctx *ctx, const int32_t *slots, int nslots,
 int ima_prior(ima_ctx *ctx, const int32_t *slots, const int32_t *group_off,
               int ngroups, const double *tvals, double *probg, double *qint,
               double *mint);
+/* integrate_tree_prob (update_gtree_common.cpp:1945-2053) with the
+ * reference's own argument: a chain's all-loci weights allgweight, which
+ * IMa2p keeps up to date one locus at a time (sum_subtract_treeinfo,
+ * ginfo.cpp:249-285).  wsum[g] is group g's record of R = 3P^2 + 2P^3 + 1
+ * doubles: cc[P][P] | fc[P][P] | hcc[P][P] | mc[P][P][P] | fm[P][P][P] |
+ * length (ima_treeweight's outputs, summed), counts as whole numbers.
+ * probg[g], qint[g][npsize], mint[g][nmig] as ima_prior's. */
+int ima_integrate_tree_prob(ima_ctx *ctx, int model, const double *wsum, int ngroups,
+                            double *probg, double *qint, double *mint);
 
 /* ---- device-resident MC3 sweep ------------------------------------------ */
 /* All heated chains on this device: slots first_slot + c*nloci + l hold the
@@ -373,6 +393,21 @@ int ima_mc3_tune(ima_ctx *ctx, int sweeps, int flags, int *best, double *ms_per_
  * call between ima_mc3_run calls. */
 int ima_mc3_decide(ima_ctx *ctx, int prop_first, const double *hastings,
                    const double *uniforms, double gbeta, int32_t *accepted);
+/* savegsampinf (ginfo.cpp:319-377), as savegenealogyinfo calls it every
+ * recording interval: appends the cold chain's genealogy summary (the last
+ * chain whose beta is exactly 1.0, whichiscoldchain ima_main_mpi.cpp:
+ * 3023-3033) to a table made by ima_lmode_table_reserve with the sweep's
+ * parameters, on the device, without waiting: per population-size
+ * parameter the count of cc and the float sums of fc and hcc over its
+ * positions, per migration parameter the count of mc and the float sum of
+ * fm, qintegrate, mintegrate, the summed pdg, probg and the split times,
+ * all as float.  IMA_ERR_SIZE once the table is full (the reference stops
+ * at MAXGENEALOGIESTOSAVE). */
+int ima_mc3_record(ima_ctx *ctx, int table_id);
+/* The chains' all-loci weights wall[nchains][R] (allgweight, R as for
+ * ima_integrate_tree_prob) and their integrals qint[nchains][npsize +
+ * nmig] (allpcalc.qintegrate then mintegrate); either may be NULL. */
+int ima_mc3_read_weights(ima_ctx *ctx, double *wall, double *qint);
 /* probg[nchains] and split times tvals[nchains][P-1] of the sweep. */
 int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
 /* Migration times of one slot as the device holds them (the rescaling
@@ -469,8 +504,9 @@ int ima_server_usage(ima_server *srv, double *used_bytes, double *budget_bytes,
 /* ---- timing ------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the context's stream (off by default).
  * ima_timing_get returns, for kernel class k (0 HKY FP64, 1 HKY FP32,
- * 2 IS, 3 SW, 4 L-mode, 5 swap, 6 MC3 sweep), the number of timed
- * launches and their total milliseconds. */
+ * 2 IS, 3 SW, 4 L-mode, 5 swap, 6 MC3 sweep, 7 the sweep's accept step),
+ * the number of timed launches and their total milliseconds.  Classes 6
+ * and 7 are timed by ima_mc3_run with IMA_MC3_TIMED. */
 int ima_timing_enable(ima_ctx *ctx, int on);
 int ima_timing_get(ima_ctx *ctx, int kclass, int64_t *launches,
                    double *total_ms);
