@@ -113,6 +113,10 @@ PROTOS = {
     "ima_mc3_read_rates": (C.c_int, [vp, dp, dp]),
     "ima_mc3_decide": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, i32p]),
     "ima_mc3_record": (C.c_int, [vp, C.c_int]),
+    "ima_ti_write": (C.c_int, [C.c_char_p, C.c_char_p, fp, C.c_int, C.c_int, C.c_int]),
+    "ima_ti_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
+    "ima_ti_load": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_int, fp,
+                              C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ima_mc3_read_weights": (C.c_int, [vp, dp, dp]),
     "ima_lmode_table_reserve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_int)]),

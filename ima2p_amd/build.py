@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libima_gpu.so")
 PLAIN = os.path.join(HERE, "libima_plain.so")
 CXX = os.environ.get("CXX", "g++")
 SOURCES = ["ctx.hip", "kern_hky.hip", "kern_lik.hip", "kern_lmode.hip", "kern_swap.hip", "kern_mc3.hip",
-           "kern_prior.hip", "kern_urate.hip", "serve.hip"]
+           "kern_prior.hip", "kern_urate.hip", "serve.hip", "ti_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -40,7 +40,7 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs = [os.path.join(OBJ, s.replace(".hip", ".o")) for s in srcs]
+    objs = [os.path.join(OBJ, os.path.splitext(s)[0] + ".o") for s in srcs]
     cmds = [[HIPCC, *FLAGS, "-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(srcs, objs)]
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16))
     with ThreadPoolExecutor(jobs) as ex:

@@ -53,7 +53,9 @@ class Engine:
     def _chk(self, rc, what):
         if rc != 0:
             msg = self.lib.ima_ctx_last_error(self.h).decode(errors="replace")
-            raise EngineError(f"{what}: code {rc}: {msg}")
+            err = EngineError(f"{what}: code {rc}: {msg}")
+            err.code = rc
+            raise err
 
     # -------------------------------------------------------------- loci --
     def add_locus_hky(self, loc) -> int:
@@ -310,6 +312,41 @@ class Engine:
                                            t.shape[1], npsize, nmig, npops, C.byref(out)),
                   "lmode_table")
         return out.value
+
+    def lmode_table_reserve(self, capacity, npsize, nmig, npops) -> int:
+        """An empty resident table the device sweep appends rows to."""
+        out = C.c_int()
+        self._chk(self.lib.ima_lmode_table_reserve(self.h, int(capacity), int(npsize), int(nmig),
+                                                   int(npops), C.byref(out)), "lmode_table_reserve")
+        return out.value
+
+    def lmode_table_rows(self, tid):
+        G, K = C.c_int(), C.c_int()
+        self._chk(self.lib.ima_lmode_table_rows(self.h, tid, C.byref(G), C.byref(K)),
+                  "lmode_table_rows")
+        return G.value, K.value
+
+    def lmode_table_read(self, tid, first=0, count=None):
+        G, K = self.lmode_table_rows(tid)
+        if count is None:
+            count = G - first
+        out = np.zeros((count, K), np.float32)
+        self._chk(self.lib.ima_lmode_table_read(self.h, tid, int(first), int(count),
+                                                out.ctypes.data_as(L.fp)), "lmode_table_read")
+        return out
+
+    def mc3_record(self, tid):
+        """Append the cold chain's genealogy summary row (savegsampinf)."""
+        self._chk(self.lib.ima_mc3_record(self.h, int(tid)), "mc3_record")
+
+    def mc3_read_weights(self, R, nparam):
+        """Each chain's all-loci weight record [C, R] and integrals [C, nparam]."""
+        C_, _ = self.mc3_shape
+        w = np.zeros((C_, R))
+        q = np.zeros((C_, max(nparam, 1)))
+        self._chk(self.lib.ima_mc3_read_weights(self.h, w.ctypes.data_as(L.dp),
+                                                q.ctypes.data_as(L.dp)), "mc3_read_weights")
+        return w, q[:, :nparam]
 
     def lmode_free(self, tid):
         self._chk(self.lib.ima_lmode_table_free(self.h, tid), "lmode_free")

@@ -226,6 +226,26 @@ int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
                          const double *x, int nx, int logi, double yadjust,
                          int expo, double prior_mean, double *out);
 
+/* ---- genealogy-table files (.ti), host only --------------------------- */
+/* Writes G rows of K floats in the reference's .ti layout: with append = 0
+ * a new file with the header block of ima_main_mpi.cpp:2121-2142 around
+ * `header` (the reference puts its run's parameter text there) and the
+ * line VALUESSTART; then each row as K values "%.6f\t" and a newline
+ * (savegenealogyfile, output.cpp:662-685).  append = 1 adds rows only. */
+int ima_ti_write(const char *path, const char *header, const float *rows, int G, int K,
+                 int append);
+/* Genealogies (lines after VALUESSTART) in a .ti file. */
+int ima_ti_count(const char *path, int *G);
+/* loadgenealogyvalues (ima_main_mpi.cpp:3216-3440) over npaths files in
+ * the given order: every line after VALUESSTART is one genealogy of
+ * exactly K values read as "%f" (fewer or more is IMA_ERR_ARG); with more
+ * than max_rows genealogies in all, a genealogy is kept while
+ * loaded / (loaded + skipped) <= max_rows / total, the first always.
+ * rows holds max_rows * K floats; *loaded and *total (may be NULL) receive
+ * the counts. */
+int ima_ti_load(const char *const *paths, int npaths, int K, int max_rows, float *rows,
+                int *loaded, int *total);
+
 /* ---- genealogy prior (update_gtree_common.cpp:1680-2053) ----------------- */
 /* Population model, flattened (the reference's poptree and parameter maps,
  * build_poptree.cpp, setup_iparams initialize.cpp:202-760).  P = npops

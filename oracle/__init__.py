@@ -410,6 +410,50 @@ def integrate_tree_prob(weights_list, model):
     return float(sum(q) + sum(m)), q, m
 
 
+def gsampinf_row(wall, integrals, pdg, probg, tvals, model):
+    """savegsampinf (ginfo.cpp:319-377) for one chain: wall is the chain's
+    all-loci weight record (cc[P][P] | fc[P][P] | hcc[P][P] | mc[P][P][P] |
+    fm[P][P][P] | length, as doubles), integrals its qintegrate then
+    mintegrate, pdg its loci' log-likelihoods (summed in locus order),
+    probg, tvals the split times.  Per population-size parameter: the int
+    sum of cc and the float sums of fc and hcc over its wp positions in
+    order; per migration parameter: the int sum of mc and the float sum of
+    fm; then qintegrate, mintegrate, pdg, probg, t -- all float32, in the
+    layout of ginfo.cpp:288-316."""
+    P = model.npops
+    P2, P3 = P * P, P * P * P
+    w = np.asarray(wall, np.float64)
+    cc, fc, hcc = w[:P2].reshape(P, P), w[P2:2 * P2].reshape(P, P), w[2 * P2:3 * P2].reshape(P, P)
+    mc = w[3 * P2:3 * P2 + P3].reshape(P, P, P)
+    fm = w[3 * P2 + P3:3 * P2 + 2 * P3].reshape(P, P, P)
+    f32 = np.float32
+    ccs, fcs, hcs, mcs, fms = [], [], [], [], []
+    for wp in model.theta_wp:
+        c, f, h = 0, f32(0.0), f32(0.0)
+        for k, r in wp:
+            c += int(cc[k, r])
+            f = f32(f + f32(fc[k, r]))
+            h = f32(h + f32(hcc[k, r]))
+        ccs.append(f32(c))
+        fcs.append(f)
+        hcs.append(h)
+    for wp in model.mig_wp:
+        c, f = 0, f32(0.0)
+        for k, a, b in wp:
+            c += int(mc[k, a, b])
+            f = f32(f + f32(fm[k, a, b]))
+        mcs.append(f32(c))
+        fms.append(f)
+    nps = len(model.theta_wp)
+    integ = [f32(x) for x in integrals]
+    pd = 0.0
+    for x in pdg:
+        pd += float(x)
+    row = ccs + fcs + hcs + mcs + fms + integ[:nps] + integ[nps:] + [f32(pd), f32(probg)] + \
+        [f32(t) for t in list(tvals)[:P - 1]]
+    return np.array(row, np.float32)
+
+
 def updategenealogy_decisions(pdg, weights, new_pdg, new_weights, ok, hastings, uniforms,
                               beta, model, gbeta=1.0):
     """The accept step of updategenealogy (update_gtree.cpp:900-961) for one
