@@ -48,6 +48,23 @@ def hky_flops(n, S):
     return (36 * n - 58) * S
 
 
+def prior_integral_us():
+    """Latency of one parameter's integral (integrate_coalescent_term's
+    incomplete-gamma chain on one lane), the critical path of each group of
+    the accept step: the average duration of the one-wavefront prior_kernel
+    (one chain's integrals side by side) in the committed rocprofv3 summary
+    of the plain per-locus path, which launches it once per proposal."""
+    path = os.path.join(ROOT, "profiles", "mc3_accept_3pop_50x1000bp_64chains_trace.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("prior_kernel_avg_us:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def hky_traffic():
     """HBM bytes per HKY launch of this workload, from the committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by
@@ -193,13 +210,23 @@ def lmode_leg(G=100000, npoints=4096, seed=4):
                      "ms_per_query": dt * 1e3}
     out["G"] = G
     out["points"] = npoints
-    # EXACT per (point, genealogy) pair with nr = 5 parameters in range:
-    # 2 nr FMAs (4 nr flops), one FP64 exp (about 20 flops of range
-    # reduction and polynomial), the running-max compare and the s / s2
-    # updates (4): 44 flops
-    out["flops_per_pair"] = 44
-    out["fp64_roofline_pairs_per_s"] = FP64_PEAK_TFLOPS * 1e12 / 44.0
+    # per (genealogy, point) pair, two populations: F = 2 K_f + 4 + 16 with
+    # K_f = 3 P_theta + 2 P_m + 1 = 14 table terms, one exp counted as 16
+    out["flops_per_pair"] = 48
+    out["fp64_roofline_pairs_per_s"] = FP64_PEAK_TFLOPS * 1e12 / 48.0
     out["exact_frac_of_fp64_peak"] = out["exact"]["pairs_per_s"] / out["fp64_roofline_pairs_per_s"]
+    # the L-mode drivers on the batched launch: the differential-evolution
+    # joint peak of the full model and one marginal peak with its 95% limits
+    lo = [1e-7] * 5
+    hi = [20.0, 20.0, 20.0, 5.0, 5.0]
+    t0 = time.perf_counter()
+    _, logp, _, evals = e.joint_peak(tid, lo, hi, seed=3, mode=L.LMODE_REF_COMPAT)
+    dt = time.perf_counter() - t0
+    out["joint_peak"] = {"seconds": dt, "jointp_points": evals, "points_per_s": evals / dt,
+                         "log_p": logp}
+    t0 = time.perf_counter()
+    e.marginal_peak(tid, 0, 1e-7, 20.0)
+    out["marginal_peak_ms"] = (time.perf_counter() - t0) * 1e3
     e.close()
     return out
 
@@ -325,6 +352,23 @@ def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
     return out
 
 
+def accept_model(avg_us, nloci, group):
+    """The accept step (mc3_accept, one workgroup per chain) is bound by
+    latency, not throughput: each chain's loci are decided in order, in
+    speculative groups of `group` loci, and every group waits for its
+    scenario integrals, whose critical path is one incomplete-gamma chain.
+    The latency bound is groups x that chain's latency; frac = bound /
+    measured."""
+    groups = -(-nloci // max(1, group))
+    t_int = prior_integral_us()
+    out = {"avg_us": avg_us, "groups_per_chain": groups, "us_per_group": avg_us / groups,
+           "bound": "latency", "integral_latency_us": t_int}
+    if t_int:
+        out["latency_bound_us"] = groups * t_int
+        out["frac"] = groups * t_int / avg_us
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,6 +436,7 @@ def main():
     ksweeps = max(10, min(args.steps, 100))
     e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
     sweep_n, sweep_ms = e.timing_get(L.K_MC3)
+    acc_n, acc_ms = e.timing_get(L.K_ACCEPT)
     # HKY kernel: the sweep's scoring launch (every chain x locus of the
     # chains' current genealogies) repeated back to back between one pair
     # of HIP events on the context's stream
@@ -463,6 +508,7 @@ def main():
                    "swap": float(stats[5]) / max(1, stats[4]),
                    "rates": float(stats[7]) / max(1, stats[6])},
         "eager_sweep_ms": sweep_ms / max(1, sweep_n),
+        "accept_step": accept_model(acc_ms / max(1, acc_n) * 1e3, Lc, group),
         "tuned": {"accept_group_loci": group,
                   "ms_per_sweep_by_group": [round(float(x), 4) for x in group_ms]},
         "setup_s": setup_s,

@@ -21,7 +21,7 @@ REJECT_IS = -1000000000.0
 MAX_GENES = 256
 K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3, K_ACCEPT = range(8)
 MC3_GRAPH, MC3_SCALE, MC3_TIMED, MC3_RY1 = 1, 2, 4, 8
-MC3_UPDATE_U, MC3_U_SERIAL = 16, 32
+MC3_UPDATE_U, MC3_U_SERIAL, MC3_TOPOLOGY = 16, 32, 64
 
 
 class Item(C.Structure):
@@ -104,6 +104,10 @@ PROTOS = {
     "ima_lmode_table_free": (C.c_int, [vp, C.c_int]),
     "ima_lmode_jointp": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, dp, dp]),
+    "ima_lmode_joint_peak": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_uint64,
+                                       C.c_int, dp, dp, dp, C.POINTER(C.c_int64)]),
+    "ima_lmode_marginal_peak": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                          C.c_double, dp]),
     "ima_lmode_margincalc": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int,
                                        C.c_double, C.c_int, C.c_double, dp]),
     "ima_mc3_setup": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_uint64, C.c_int,
@@ -112,6 +116,7 @@ PROTOS = {
     "ima_mc3_read": (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ima_mc3_read_rates": (C.c_int, [vp, dp, dp]),
     "ima_mc3_decide": (C.c_int, [vp, C.c_int, dp, dp, C.c_double, i32p]),
+    "ima_slots_read_pops": (C.c_int, [vp, C.c_int, C.c_int, i32p]),
     "ima_mc3_record": (C.c_int, [vp, C.c_int]),
     "ima_ti_write": (C.c_int, [C.c_char_p, C.c_char_p, fp, C.c_int, C.c_int, C.c_int]),
     "ima_ti_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
@@ -120,7 +125,7 @@ PROTOS = {
     "ima_mc3_read_weights": (C.c_int, [vp, dp, dp]),
     "ima_lmode_table_reserve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_int)]),
-    "ima_lmode_table_rows": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ima_lmode_table_info": (C.c_int, [vp, C.c_int] + [C.POINTER(C.c_int)] * 5),
     "ima_lmode_table_read": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, fp]),
     "ima_integrate_tree_prob": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp, dp]),
     "ima_mc3_prior_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

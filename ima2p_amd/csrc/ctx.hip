@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -581,6 +582,18 @@ int ima_slots_read(ima_ctx *c, int first, int count, int32_t *up0, int32_t *up1,
   return IMA_OK;
 }
 
+int ima_slots_read_pops(ima_ctx *c, int first, int count, int32_t *pop) {
+  if (!c || !pop || first < 0 || count < 0 || first + count > (int)c->slots.size())
+    return IMA_ERR_ARG;
+  if (count == 0) return IMA_OK;
+  if (!c->d_pop.p) return fail(c, IMA_ERR_STATE, "no population labels were written");
+  const SlotMeta &s0 = c->slots[first];
+  const SlotMeta &s1 = c->slots[first + count - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  HIPCHK(c, copy_now(c, pop, c->d_pop.p + s0.node_off, nb * 4, hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
 int ima_slots_count(ima_ctx *c, int *count) {
   if (!c || !count) return IMA_ERR_ARG;
   *count = (int)c->slots.size();
@@ -977,10 +990,15 @@ int ima_lmode_table_reserve(ima_ctx *c, int capacity, int npsize, int nmig, int 
   return IMA_OK;
 }
 
-int ima_lmode_table_rows(ima_ctx *c, int id, int *G, int *K) {
+int ima_lmode_table_info(ima_ctx *c, int id, int *G, int *K, int *npsize, int *nmig,
+                         int *npops) {
   if (!c || id < 0 || id >= (int)c->tables.size() || !c->tables[id].live) return IMA_ERR_ARG;
-  if (G) *G = c->tables[id].G;
-  if (K) *K = c->tables[id].K;
+  const LTableHost &h = c->tables[id];
+  if (G) *G = h.G;
+  if (K) *K = h.K;
+  if (npsize) *npsize = h.npsize;
+  if (nmig) *nmig = h.nmig;
+  if (npops) *npops = h.npops;
   return IMA_OK;
 }
 
@@ -1164,6 +1182,11 @@ static void mc3_release(ima_ctx *c) {
   release(m.idec);
   release(m.ptw);
   release(m.pitems);
+  release(m.thast);
+  release(m.titems);
+  release(m.tptw);
+  release(m.arange);
+  m.topo_first = -1;
   m.gview_set = false;
   m.udev = URateDev{};
   m.dev.prior = -1;
@@ -1367,6 +1390,14 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.uhost = nullptr;
   d.idec = nullptr;
   d.prop_first = 0;
+  d.topo = 0;
+  d.thast = nullptr;
+  d.arange = nullptr;
+  d.pitems = nullptr;
+  d.ptw = nullptr;
+  d.maxn = 2;
+  for (int i = 0; i < N; ++i) d.maxn = std::max(d.maxn, c->slots[first_slot + i].n);
+  m.topo_first = -1;
   for (int k = 0; k < 4; ++k) {
     HIPCHK(c, hipEventCreate(&m.ev[k]));
     HIPCHK(c, hipEventCreateWithFlags(&m.fj[k], hipEventDisableTiming));
@@ -1432,6 +1463,94 @@ static int mc3_urate_round(ima_ctx *c, uint64_t ctr, bool serial) {
 // runs synchronise per sweep).  The sweep counter's host copy decides which
 // sweeps rescale and which end with a mutation-rate round, so nothing is
 // read back from the device.
+// Turns the device's own genealogy move on or off (IMA_MC3_TOPOLOGY).  On
+// first use the proposal slots are allocated (one per chain and locus) and
+// the stepwise loci' allele ranges set from chain 0's gene copies
+// (L[li].minA / maxA, build_gtree.cpp:707-722).
+static int mc3_topo_mode(ima_ctx *c, bool on) {
+  auto &m = c->mc3;
+  MC3Dev &d = m.dev;
+  if (!on) {
+    if (d.topo) {
+      d.topo = 0;
+      d.hast = nullptr;
+      mc3_drop_graphs(c);
+    }
+    return IMA_OK;
+  }
+  if (d.topo) return IMA_OK;
+  if (d.prior >= 0 && c->models[d.prior].nmig > 0)
+    return fail(c, IMA_ERR_STATE,
+                "the device's genealogy move needs a model without migration parameters");
+  const int N = m.C * m.L;
+  int rc;
+  if (m.topo_first < 0) {
+    std::vector<int32_t> loci(N);
+    for (int i = 0; i < N; ++i) loci[i] = c->slots[m.first + i].locus;
+    int first = 0;
+    if ((rc = ima_slots_alloc(c, N, loci.data(), &first)) != IMA_OK) return rc;
+    m.topo_first = first;
+    HIPCHK(c, reserve(c->d_mig_first, c->nodes, true, c->stream));
+    HIPCHK(c, reserve(c->d_mig_count, c->nodes, true, c->stream));
+  }
+  HIPCHK(c, reserve(m.thast, N));
+  HIPCHK(c, reserve(m.titems, N));
+  HIPCHK(c, reserve(m.tptw, N));
+  std::vector<int2> ar(m.L, make_int2(1, 1000));
+  for (int l = 0; l < m.L; ++l) {
+    const SlotMeta &s = c->slots[m.first + l];
+    if (c->loci[s.locus].model != IMA_MODEL_SW) continue;
+    int mn = 1000, mx = 0;
+    for (int e = 0; e < s.n; ++e) {
+      mn = std::min(mn, c->A[s.node_off + e]);
+      mx = std::max(mx, c->A[s.node_off + e]);
+    }
+    ar[l] = make_int2(std::max(1, (mx + mn) / 2 - (mx - mn)), std::min(1000, (mx + mn) / 2 + (mx - mn)));
+  }
+  HIPCHK(c, reserve(m.arange, m.L));
+  HIPCHK(c, hipMemcpyAsync(m.arange.p, ar.data(), ar.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // any topology may come up: the pruning stack must hold floor(log2 n)
+  int bound = 1;
+  while ((2 << bound) <= m.hmax_n) ++bound;
+  m.hmax_slots = std::max(m.hmax_slots, bound);
+  d.topo = 1;
+  d.thast = m.thast.p;
+  d.hast = m.thast.p;
+  d.arange = m.arange.p;
+  d.pitems = m.titems.p;
+  d.ptw = m.tptw.p;
+  d.prop_first = m.topo_first;
+  mc3_drop_graphs(c);
+  return IMA_OK;
+}
+
+// After a run with the device's own genealogy move: the host mirror of the
+// chains' slots follows the device (topologies, times, populations,
+// alleles, programs and slot records).
+static int mc3_topo_sync(ima_ctx *c) {
+  auto &m = c->mc3;
+  const int N = m.C * m.L;
+  const SlotMeta &s0 = c->slots[m.first];
+  const SlotMeta &s1 = c->slots[m.first + N - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  const size_t pb = s1.prog_off + (size_t)s1.n - 1 - s0.prog_off;
+  const size_t o = s0.node_off;
+  HIPCHK(c, copy_now(c, c->slots.data() + m.first, c->d_slots.p + m.first, N * sizeof(SlotMeta),
+                     hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->up0.data() + o, c->d_up0.p + o, nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->up1.data() + o, c->d_up1.p + o, nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->down.data() + o, c->d_down.p + o, nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->time.data() + o, c->d_time.p + o, nb * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->pop.data() + o, c->d_pop.p + o, nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->A.data() + o, c->d_A.p + o, nb * 4, hipMemcpyDeviceToHost));
+  if (pb)
+    HIPCHK(c, copy_now(c, c->prog.data() + s0.prog_off, c->d_prog.p + s0.prog_off,
+                       pb * sizeof(ProgOp), hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
 int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   if (!c || nsweeps < 0) return IMA_ERR_ARG;
   auto &m = c->mc3;
@@ -1450,6 +1569,10 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
     mc3_drop_graphs(c);  // the graphs hold the sweep's parameters
     m.dev.ry1 = ry1;
   }
+  // the device's own genealogy move: proposals are made at every sweep's
+  // start, nothing is left pending between sweeps
+  const bool topo = (flags & IMA_MC3_TOPOLOGY) != 0;
+  if ((rc = mc3_topo_mode(c, topo)) != IMA_OK) return rc;
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
   if ((rc = upload_models(c)) != IMA_OK) return rc;
   {
@@ -1491,7 +1614,7 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   bool pending = false;  // the next sweep's genealogy proposals are in place
   for (int s = 0; s < nsweeps;) {
     const uint64_t q = done + s;
-    if (!pending) {
+    if (!pending && !topo) {
       HIPCHK(c, enqueue_mc3_propose(view(c), m.dev, c->stream));
       pending = true;
     }
@@ -1523,7 +1646,7 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
         HIPCHK(c, hipEventRecord(T.ev0[K_MC3], c->stream));
         if ((rc = mc3_enqueue(c, with_scale, m.ev)) != IMA_OK) return rc;
         if (u_due(q)) {
-          HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+          if (pending) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
           pending = false;
           if ((rc = mc3_urate_round(c, q + 1, userial)) != IMA_OK) return rc;
         }
@@ -1558,10 +1681,10 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
     }
     s += ran;
     const uint64_t qlast = done + s - 1;
-    if (pending && u_due(qlast)) {
+    if (u_due(qlast) && (pending || topo)) {
       // the round scores the committed genealogies: take the next sweep's
       // proposals back, and make them again (the same ones) afterwards
-      HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
+      if (pending) HIPCHK(c, enqueue_mc3_revert(view(c), m.dev, c->stream));
       pending = false;
       if ((rc = mc3_urate_round(c, qlast + 1, userial)) != IMA_OK) return rc;
     }
@@ -1577,6 +1700,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
   int rc = mc3_run_async(c, nsweeps, flags);
   if (rc != IMA_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->mc3.dev.topo) return mc3_topo_sync(c);
   return IMA_OK;
 }
 
@@ -1790,6 +1914,18 @@ std::vector<Region> mc3_state_regions(ima_ctx *c) {
   add(m.acc.p, N * 4);
   add(m.cacc.p, (size_t)m.C * 4);
   add(m.items.p, N * sizeof(ima_item));  // u and kappa, moved by the rate rounds
+  if (d.topo) {
+    // the device's own genealogy move changes topologies, labels and alleles
+    const int64_t nf = d.node_first;
+    add(c->d_up0.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_up1.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_down.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_pop.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_A.p + nf, (size_t)d.nnodes * 4);
+    const SlotMeta &s0 = c->slots[m.first], &s1 = c->slots[m.first + N - 1];
+    add(c->d_prog.p + s0.prog_off, (s1.prog_off + s1.n - 1 - s0.prog_off) * sizeof(ProgOp));
+    add(c->d_slots.p + m.first, N * sizeof(SlotMeta));
+  }
   if (d.prior >= 0) {
     const size_t np = (size_t)std::max(d.nparam, 1);
     add(m.tv.p, (size_t)m.C * mc3_tw(d.P) * 8);
@@ -1831,6 +1967,9 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int flags, int *best, double *ms_per_sw
     if (best) *best = 0;
     return IMA_OK;
   }
+  // the state saved and restored is that of the sweeps to be timed
+  int rc0 = mc3_topo_mode(c, (run_flags & IMA_MC3_TOPOLOGY) != 0);
+  if (rc0 != IMA_OK) return rc0;
   const size_t bytes = mc3_state_bytes(c);
   char *buf = nullptr;
   HIPCHK(c, hipMalloc(&buf, bytes));
@@ -1981,6 +2120,7 @@ int ima_mc3_decide(ima_ctx *c, int prop_first, const double *hastings, const dou
   HIPCHK(c, reserve(m.pitems, N));
   MC3Dev d = m.dev;
   d.host = 1;
+  d.topo = 0;
   d.gbeta = gbeta;
   d.idec = m.idec.p;
   d.prop_first = prop_first;

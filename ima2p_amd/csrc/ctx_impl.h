@@ -141,6 +141,13 @@ struct ima_ctx {
     DevBuf<int32_t> idec, ptw;
     DevBuf<ima_item> pitems;
     std::vector<double> beta0;  // the ladder set up (swaps permute it)
+    // the reference's genealogy move (IMA_MC3_TOPOLOGY): proposal slots
+    // made once, their Hastings terms and items, stepwise allele ranges
+    int topo_first = -1;
+    DevBuf<double> thast;
+    DevBuf<ima_item> titems;
+    DevBuf<int32_t> tptw;
+    DevBuf<int2> arange;
     // device buffers the captured graphs point at: a change (a buffer grown
     // by later slot allocations or writes) drops the graphs
     ima::DeviceView gview{};

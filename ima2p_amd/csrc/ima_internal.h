@@ -288,6 +288,15 @@ struct MC3Dev {
   const double *uhost;   // [C*L] the decisions' uniforms, or null (Philox)
   int32_t *idec;         // [C*L] decisions of the last ima_mc3_decide
   int prop_first;        // first proposal slot (slot prop_first + it)
+  // the reference's genealogy move (topo = 1): proposals are made on the
+  // device in the proposal slots, with Hastings terms thast; accepted ones
+  // are copied into the chains' slots by the accept step
+  int topo;
+  double *thast;         // [C*L]
+  const int2 *arange;    // [L] allele range (minA, maxA) of each stepwise locus
+  ima_item *pitems;      // [C*L] the proposals' scoring items
+  int32_t *ptw;          // [C*L] the proposals' slots
+  int maxn;              // largest gene-copy count of the chains' loci
 };
 
 // Dynamic LDS of the per-chain prior kernels (kern_mc3.hip), the same on
@@ -337,6 +346,11 @@ hipError_t enqueue_mc3_score(const DeviceView &v, const MC3Dev &m, const ima_ite
 hipError_t enqueue_mc3_decide(const DeviceView &v, const MC3Dev &m, hipStream_t st);
 // The cold chain's genealogy-table row (savegsampinf) into row[0..K).
 hipError_t enqueue_mc3_record(const DeviceView &v, const MC3Dev &m, float *row, hipStream_t st);
+// The reference's genealogy move for every item into its proposal slot
+// (kern_topo.hip), dynamic LDS topo_lds_bytes(max_n).
+size_t topo_lds_bytes(int max_n);
+hipError_t enqueue_mc3_propose_topo(const DeviceView &v, const MC3Dev &m, int max_n,
+                                    hipStream_t st);
 // Items and weight slots of the proposals: items[it] with slot
 // prop_first + it, tw_slots[it] = prop_first + it.
 hipError_t enqueue_mc3_prop_items(const MC3Dev &m, const ima_item *items, ima_item *pitems,

@@ -135,6 +135,47 @@ __device__ inline void restore_item(const DeviceView &v, const MC3Dev &m, int it
   tm[up1[e]] = m.rec_t[it];
 }
 
+// An accepted proposal (slot prop_first + it) replaces the chain's genealogy
+// (slot first_slot + it): edges, times, populations, alleles, program,
+// migrations, and the slot's root, stack depth and migration count.
+// Called by the lanes [lane0, lane0 + nl) of one wavefront (nl >= 1).
+__device__ void copy_proposal(const DeviceView &v, const MC3Dev &m, int it, int lane, int nl) {
+  SlotMeta *slots = const_cast<SlotMeta *>(v.slots);
+  const SlotMeta src = slots[m.prop_first + it];
+  SlotMeta &dst = slots[m.first_slot + it];
+  const int64_t so = src.node_off, d = dst.node_off;
+  const int ne = 2 * src.n - 1;
+  int32_t *up0 = const_cast<int32_t *>(v.up0), *up1 = const_cast<int32_t *>(v.up1);
+  int32_t *dn = const_cast<int32_t *>(v.down), *pop = const_cast<int32_t *>(v.pop);
+  int32_t *A = const_cast<int32_t *>(v.A);
+  int32_t *mf = const_cast<int32_t *>(v.mig_first), *mc = const_cast<int32_t *>(v.mig_count);
+  double *tm = const_cast<double *>(v.time);
+  for (int e = lane; e < ne; e += nl) {
+    up0[d + e] = up0[so + e];
+    up1[d + e] = up1[so + e];
+    dn[d + e] = dn[so + e];
+    tm[d + e] = tm[so + e];
+    if (pop) pop[d + e] = pop[so + e];
+    if (A) A[d + e] = A[so + e];
+    if (mf) mf[d + e] = mf[so + e];
+    if (mc) mc[d + e] = mc[so + e];
+  }
+  ProgOp *prog = const_cast<ProgOp *>(v.prog);
+  for (int k = lane; k < src.n - 1; k += nl) prog[dst.prog_off + k] = prog[src.prog_off + k];
+  int32_t *me = const_cast<int32_t *>(v.mig_edge), *mto = const_cast<int32_t *>(v.mig_to);
+  double *mt = const_cast<double *>(v.mig_time);
+  for (int k = lane; k < src.nmig; k += nl) {
+    me[dst.mig_off + k] = me[src.mig_off + k];
+    mto[dst.mig_off + k] = mto[src.mig_off + k];
+    mt[dst.mig_off + k] = mt[src.mig_off + k];
+  }
+  if (lane == 0) {
+    dst.root = src.root;
+    dst.nslots = src.nslots;
+    dst.nmig = src.nmig;
+  }
+}
+
 // Proposals of the current sweep for every item (first sweep of a run).
 __global__ void mc3_propose(DeviceView v, MC3Dev m) {
   const int it = blockIdx.x * blockDim.x + threadIdx.x;
@@ -490,11 +531,12 @@ __device__ inline void prior_commit(const DeviceView &v, const MC3Dev &m, const 
       double *wc = m.wrec + (int64_t)it * R;
       for (int r = lane; r < R; r += kWave)
         if (sh.flag[row * R + r]) wc[r] = sh.dnew[row * R + r];
+      if (m.topo) copy_proposal(v, m, it, lane, kWave);
     }
     if (lane == 0) {
       if (acc)
         m.pdg[it] = sh.npd[row];
-      else if (!m.host)
+      else if (!m.host && !m.topo)
         restore_item(v, m, it);
       if (m.host) m.idec[it] = acc ? 1 : 0;
       m.acc[it] += acc ? 1u : 0u;
@@ -849,23 +891,26 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
       const double U = r.uniform();
       const double np = new_loglik(m, it);
       bool acc = false;
-      if (m.status[it] == IMA_STATUS_OK) acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it])));
+      const double h = m.topo ? m.thast[it] : 0.0;
+      if (m.status[it] == IMA_STATUS_OK)
+        acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it]) + h));
       double p = m.pdg[it];
       if (acc) {
         p = np;
         m.pdg[it] = p;
-      } else {
+        if (m.topo) copy_proposal(v, m, it, 0, 1);
+      } else if (!m.topo) {
         restore_item(v, m, it);
       }
       m.acc[it] += acc ? 1u : 0u;
       part += p;
-      if (!with_scale) propose_item(v, m, it, sweep + 1);
+      if (!with_scale && !m.topo) propose_item(v, m, it, sweep + 1);
     }
   } else {
     prior_accept(v, m, c, sweep, dyn);
     for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
       part += m.pdg[c * m.L + l];
-      if (!with_scale) propose_item(v, m, c * m.L + l, sweep + 1);
+      if (!with_scale && !m.topo) propose_item(v, m, c * m.L + l, sweep + 1);
     }
     if (threadIdx.x == 0) part += m.probg[c];
   }
@@ -994,7 +1039,8 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
     }
   }
   __syncthreads();  // committed times visible before the next proposals
-  for (int l = threadIdx.x; l < m.L; l += blockDim.x) propose_item(v, m, c * m.L + l, sweep + 1);
+  if (!m.topo)
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) propose_item(v, m, c * m.L + l, sweep + 1);
   finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
 }
 
@@ -1249,10 +1295,20 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   // stream beside the likelihood launches.  The accept kernel waits for
   // both streams
   auto score = [&](int pass) -> hipError_t {
-    return enqueue_mc3_score(v, m, items, m.tw_slots, idx, p, nh, ni, ns, precision, hmax_n,
-                             hmax_slots, imax_n, imax_w, score_ev, pass, st, side, fj);
+    // topology moves score the proposal slots; the second (time-update)
+    // pass scores the chains' own
+    const bool prop = m.topo && pass == 0;
+    return enqueue_mc3_score(v, m, prop ? m.pitems : items, prop ? m.ptw : m.tw_slots, idx, p,
+                             nh, ni, ns, precision, hmax_n, hmax_slots, imax_n, imax_w, score_ev,
+                             pass, st, side, fj);
   };
-  hipError_t e = score(0);
+  hipError_t e = hipSuccess;
+  if (m.topo) {
+    e = enqueue_mc3_prop_items(m, items, m.pitems, m.ptw, st);
+    if (e == hipSuccess) e = enqueue_mc3_propose_topo(v, m, m.maxn, st);
+    if (e != hipSuccess) return e;
+  }
+  e = score(0);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), dl, st, v, m, with_scale ? 1 : 0,
                      swaptries);

@@ -133,6 +133,13 @@ class Engine:
                                           A.ctypes.data_as(L.i32p)), "read_slots")
         return up0, up1, dn, tm, rt, A
 
+    def read_pops(self, first, count):
+        n = [self.slot_info(first + i)[1] for i in range(count)]
+        pop = np.zeros(sum(2 * k - 1 for k in n), np.int32)
+        self._chk(self.lib.ima_slots_read_pops(self.h, first, count, pop.ctypes.data_as(L.i32p)),
+                  "read_pops")
+        return pop
+
     # ---------------------------------------------------- genealogy prior --
     def model_add(self, model):
         """Register an immodel.IMModel; returns its id."""
@@ -322,8 +329,8 @@ class Engine:
 
     def lmode_table_rows(self, tid):
         G, K = C.c_int(), C.c_int()
-        self._chk(self.lib.ima_lmode_table_rows(self.h, tid, C.byref(G), C.byref(K)),
-                  "lmode_table_rows")
+        self._chk(self.lib.ima_lmode_table_info(self.h, tid, C.byref(G), C.byref(K), None, None,
+                                                None), "lmode_table_info")
         return G.value, K.value
 
     def lmode_table_read(self, tid, first=0, count=None):
@@ -362,6 +369,31 @@ class Engine:
                                             out.ctypes.data_as(L.dp), ess.ctypes.data_as(L.dp)),
                   "jointp")
         return out, ess
+
+    def joint_peak(self, tid, lower, upper, modeltype=0, xmap=None, mult=100, seed=1,
+                   mode=L.LMODE_REF_COMPAT):
+        """Differential-evolution joint peak (ima_lmode_joint_peak).
+        Returns (peak vector, log P at the peak, ESS, jointp evaluations)."""
+        lo, plo = _f64(lower)
+        hi, phi = _f64(upper)
+        peak = np.zeros(len(lo))
+        lp, es = C.c_double(), C.c_double()
+        ev = C.c_int64()
+        xm = None if xmap is None else np.ascontiguousarray(xmap, np.float64)
+        self._chk(self.lib.ima_lmode_joint_peak(
+            self.h, int(tid), int(modeltype), None if xm is None else xm.ctypes.data_as(L.dp),
+            plo, phi, int(mult), int(seed), int(mode), peak.ctypes.data_as(L.dp), C.byref(lp),
+            C.byref(es), C.byref(ev)), "joint_peak")
+        return peak, lp.value, es.value, ev.value
+
+    def marginal_peak(self, tid, param, lower, upper, expo=False, mean=1.0):
+        """(peak, log density at the peak, lower 95%, upper 95%, no-migration
+        2LLR) of one parameter's marginal posterior (ima_lmode_marginal_peak)."""
+        out = np.zeros(5)
+        self._chk(self.lib.ima_lmode_marginal_peak(self.h, int(tid), int(param), float(lower),
+                                                   float(upper), int(expo), float(mean),
+                                                   out.ctypes.data_as(L.dp)), "marginal_peak")
+        return tuple(out)
 
     def margincalc(self, tid, param, x, logi=1, yadjust=0.0, expo=False, mean=1.0):
         x = np.ascontiguousarray(np.atleast_1d(x), np.float64)

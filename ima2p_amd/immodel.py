@@ -46,7 +46,9 @@ class IMModel:
         return len(self.mig_wp)
 
 
-def build(pt: PopTree, qmax=20.0, mmax=5.0) -> IMModel:
+def build(pt: PopTree, qmax=20.0, mmax=5.0, migration=True) -> IMModel:
+    """migration=False: no migration parameters (the reference's model
+    without migration, -j option)."""
     P = pt.npops
     nper = P
     plist = -np.ones((nper, P), np.int32)
@@ -73,7 +75,7 @@ def build(pt: PopTree, qmax=20.0, mmax=5.0) -> IMModel:
         theta_wp.append([(k, int(np.where(plist[k] == p)[0][0]))
                          for k in range(nper) if p in plist[k][: npresent[k]]])
     mig_wp, pairs = [], {}
-    for k in range(nper - 1):
+    for k in range(nper - 1 if migration else 0):
         pres = list(plist[k][: npresent[k]])
         for i, p in enumerate(pres):
             for j, q in enumerate(pres):

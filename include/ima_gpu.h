@@ -112,6 +112,10 @@ int ima_slots_write(ima_ctx *ctx, int first, int count, const int32_t *up0,
 int ima_slots_read(ima_ctx *ctx, int first, int count, int32_t *up0,
                    int32_t *up1, int32_t *down, double *time, int32_t *root,
                    int32_t *A);
+/* Population labels (gtree[i].pop) of slots first..first+count-1 as the
+ * device holds them, packed like ima_slots_read's arrays (the device's own
+ * genealogy move, IMA_MC3_TOPOLOGY, relabels moved nodes). */
+int ima_slots_read_pops(ima_ctx *ctx, int first, int count, int32_t *pop);
 /* Number of slots in the context, and the locus / gene-copy count of one
  * slot (slots may also be created by ima_likelihood_hky). */
 int ima_slots_count(ima_ctx *ctx, int *count);
@@ -206,8 +210,11 @@ int ima_lmode_table_free(ima_ctx *ctx, int table_id);
  * recorded so far. */
 int ima_lmode_table_reserve(ima_ctx *ctx, int capacity, int npsize, int nmig, int npops,
                             int *table_id);
-/* Genealogies a table holds and values per genealogy. */
-int ima_lmode_table_rows(ima_ctx *ctx, int table_id, int *G, int *K);
+/* Genealogies a table holds, values per genealogy, and its population-size
+ * and migration parameter counts and sampled populations (any may be
+ * NULL). */
+int ima_lmode_table_info(ima_ctx *ctx, int table_id, int *G, int *K, int *npsize, int *nmig,
+                         int *npops);
 /* Rows [first, first + count) of a table, K floats each. */
 int ima_lmode_table_read(ima_ctx *ctx, int table_id, int first, int count, float *rows);
 /* jointp at npoints points x[npoints][npsize+nmig]; params
@@ -220,6 +227,39 @@ int ima_lmode_table_read(ima_ctx *ctx, int table_id, int first, int count, float
 int ima_lmode_jointp(ima_ctx *ctx, int table_id, const double *x, int npoints,
                      int modeltype, int prange0, int prange1, int mode,
                      double *out, double *ess);
+/* The joint posterior peak of a table by differential evolution, as
+ * findjointpeaks / modelloop search it (jointfind.cpp:596-812, 1089-1183),
+ * every generation's trials scored in one batched jointp launch.
+ * modeltype 0: all parameters (two populations); 1: population sizes only;
+ * 2: migration rates only (more populations).  xmap (may be NULL for the
+ * full model) is a nested model's map over the npsize + nmig parameters:
+ * xmap[i] < 0 fixes parameter i at -xmap[i], otherwise it takes varying
+ * value xmap[i] (mapvals, :563-573; positions index the full vector, so a
+ * migration-only model's values start at npsize).  lower/upper: every
+ * parameter's bounds (setbounds :576-594 uses MINPARAMVAL and the prior
+ * maxima).  popsize_multiplier: population per varying value (the
+ * reference's 100).  On return peak[npsize + nmig] holds the peak's full
+ * parameter vector, *logp the log posterior density there (the reference
+ * prints it as log(P); 2LLR of a nested model is 2 (logp_full -
+ * logp_nested)), *ess the effective sample number, *evaluations the jointp
+ * points evaluated (any output but peak may be NULL).  The random numbers
+ * are MT19937's from `seed`. */
+int ima_lmode_joint_peak(ima_ctx *ctx, int table_id, int modeltype, const double *xmap,
+                         const double *lower, const double *upper, int popsize_multiplier,
+                         uint64_t seed, int mode, double *peak, double *logp, double *ess,
+                         int64_t *evaluations);
+/* The marginal posterior peak of parameter `param` over [lower, upper]
+ * (marginalopt / findmarginpeaks, surface_call_functions.cpp:175-470): the
+ * mean density of margincalc over the table, maximised (one batched launch
+ * over a 1024-point grid brackets it, golden section refines it to 1e-7
+ * relative, as goldenmod), with the 95% limits where the density falls
+ * e^1.92 below the peak's (margin95).  out[5] = peak location, log density
+ * there, lower and upper 95% limits (-1 where the density does not fall
+ * that low inside the range), and for a migration rate the no-migration
+ * statistic 2 log(p(peak) / p(MINPARAMVAL)) (0 otherwise).  expo /
+ * prior_mean as for ima_lmode_margincalc. */
+int ima_lmode_marginal_peak(ima_ctx *ctx, int table_id, int param, double lower, double upper,
+                            int expo, double prior_mean, double *out);
 /* margincalc for parameter `param` at nx values x[] (logi, yadjust,
  * exponential migration prior with mean prior_mean when expo != 0). */
 int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
@@ -340,6 +380,20 @@ int ima_integrate_tree_prob(ima_ctx *ctx, int model, const double *wsum, int ngr
 /* With IMA_MC3_UPDATE_U: one step per scoring launch, in the loop's order
  * (the same result, for checking the grouped schedule). */
 #define IMA_MC3_U_SERIAL 32
+/* The genealogy update is the reference's own move (updategenealogy,
+ * update_gtree.cpp:723-966): an edge's top slides along the genealogy
+ * (slider_nomigration :79-253) through joinsisdown / splitsisdown
+ * (:375-528), so topologies change; the root move's Hastings term
+ * (slideweight) and, at stepwise loci, the moved node's allele proposal
+ * with its Hastings term (finishSWupdateA, update_gtree_common.cpp:
+ * 2219-2362) enter the decision.  Proposals are made on the device in
+ * proposal slots the engine allocates on first use, scored in one batched
+ * launch and decided in each chain's locus order; accepted ones replace
+ * the chains' genealogies.  For population models without migration
+ * parameters (or without the prior): the migration-path proposals of
+ * addmigration (:589-665) are not built, and a model with migration
+ * parameters is refused with IMA_ERR_STATE. */
+#define IMA_MC3_TOPOLOGY 64
 int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
                   const double *u, const double *kappa, const double *beta,
                   uint64_t seed, int precision, int swaptries,

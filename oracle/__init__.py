@@ -81,7 +81,7 @@ def _load():
                                                 C.c_double] + [ip] * 4
                                   + [dp, ip, ip, ip, dp, ip, dp, dp, C.c_double, C.c_uint64,
                                      C.c_double, C.c_long, C.POINTER(C.c_long), C.c_int,
-                                     C.c_int])
+                                     C.c_int, C.c_int, ip])
     return lib
 
 
@@ -487,7 +487,7 @@ def updategenealogy_decisions(pdg, weights, new_pdg, new_weights, ok, hastings, 
 
 def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
               seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False, urate=False,
-              rates=None):
+              rates=None, topo=False):
     """The CPU baseline's C sweep over chains[c][l] (Genealogy objects with
     population labels and migrations; updated in place) with split times
     tvs[c] (a float array [nchains, P-1], updated in place).  ry1: the
@@ -495,7 +495,10 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     global rescale.  urate: changeu after every fifth sweep.  u (per locus
     or [nchains, nloci]) and kappa (scalar or [nchains, nloci]) start each
     chain; rates=(u, kappa), two float64 [nchains, nloci] arrays, take their
-    place and receive the final values.  Returns (sweeps, stats[6] =
+    place and receive the final values.  topo: the genealogy update is the
+    reference's own slider move (models without migration; topologies,
+    labels and roots of the Genealogy objects are updated).  Returns
+    (sweeps, stats[6] =
     genealogy proposals, accepts, time-update proposals, accepts,
     mutation-rate steps, accepts)."""
     from ima2p_amd.immodel import mig_arrays
@@ -542,6 +545,7 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     arrs = [np.ascontiguousarray(a, np.int32) for a in
             (model.plist, model.npresent, model.popdown, model.pope, model.join)]
     stats = (C.c_long * 6)()
+    roots_out = np.zeros(len(gens), np.int32)
     sw = lib.ora_mc3_bench(len(chains), L, _ip(n_l), _ip(S), _ip(seq), _ip(seq_off), _ip(mult),
                            _ip(mult_off), _ip(tot), _dp(pi), _dp(uu), _dp(kk), _ip(sp), P,
                            *[_ip(a) for a in arrs], model.npsize, _ip(toff), _ip(twp),
@@ -549,11 +553,19 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
                            _ip(up0), _ip(up1), _ip(dn), _ip(pop), _dp(tm), _ip(roots), _ip(nm),
                            _ip(me), _dp(mt), _ip(mo), _dp(tv), _dp(b), float(scale_width),
                            int(seed), float(seconds), int(max_sweeps), stats, int(bool(ry1)),
-                           int(bool(urate)))
+                           int(bool(urate)), int(bool(topo)), _ip(roots_out))
+    if sw < 0:
+        raise ValueError("mc3_bench: a model with migration has no topology move here")
     off = moff_ = 0
-    for g, m in zip(gens, migs):
+    for gi, (g, m) in enumerate(zip(gens, migs)):
         ne = 2 * g.n - 1
         g.time[:] = tm[off:off + ne]
+        if topo:
+            g.up0[:] = up0[off:off + ne]
+            g.up1[:] = up1[off:off + ne]
+            g.down[:] = dn[off:off + ne]
+            g.pop[:] = pop[off:off + ne]
+            g.root = int(roots_out[gi])
         off += ne
         k = len(m[0])
         times = mt[moff_:moff_ + k]

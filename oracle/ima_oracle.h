@@ -159,7 +159,8 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    double *times, const int *roots, const int *nmig, const int *mig_edge,
                    double *mig_time, const int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
-                   long max_sweeps, long *stats, int ry1, int urate);
+                   long max_sweeps, long *stats, int ry1, int urate, int topo,
+                   int *roots_out);
 
 #ifdef __cplusplus
 }

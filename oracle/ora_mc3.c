@@ -17,7 +17,9 @@
  * (kern_mc3.hip): a reflected normal slide of one coalescence inside its
  * period and between the migrations around it, and a global rescale by
  * exp(d (U - 1/2)) -- or, with ry1, the reference's rubber-band update of
- * one split time (changet_RY1, update_t_RY.cpp:223-517).  Randomness:
+ * one split time (changet_RY1, update_t_RY.cpp:223-517).  With topo the
+ * genealogy update is instead the reference's own move for models without
+ * migration (slider_nomigration / joinsisdown / splitsisdown).  Randomness:
  * xoshiro256** seeded by splitmix64.
  */
 #include <math.h>
@@ -80,6 +82,7 @@ typedef struct {
   const double *qmax, *mmax;
   double tmax, scale_width;
   int ry1;
+  int topo; /* the genealogy update is the reference's slider move */
 } prob_t;
 
 static int rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
@@ -238,6 +241,195 @@ static void slide(const prob_t *pb, chain_t *ch, int l, rng_t *r, long *acc) {
     memcpy(ch->all, ch->hold, sizeof(double) * R);
   }
   hky_settle(g, ok);
+}
+
+/* ---- updategenealogy's own move, models without migration ------------
+ * update_gtree.cpp:723-826 with slider_nomigration (:79-253), joinsisdown
+ * (:375-454), splitsisdown (:457-528) and findjointime (:35-75); the
+ * Hastings term of a root move is log N(d; 0, s_new) - log N(d; 0, s_old)
+ * with s = min(SLIDESTDVMAX = 20, root time / 3) (:780-812).  HKY loci,
+ * full re-scoring, the same acceptance as slide() plus the Hastings term. */
+
+static int o_period(const prob_t *pb, const double *tv, double t) {
+  int k = 0; /* findperiod, update_gtree_common.cpp:1626-1633 */
+  while (k < pb->P - 1 && tv[k] <= t) k++;
+  return k;
+}
+
+static int o_dropped(const prob_t *pb, int p, int q) {
+  /* droppops[p]: the populations merged at the split that opens period p */
+  return pb->join[3 * (p - 1)] == q || pb->join[3 * (p - 1) + 1] == q;
+}
+
+static double o_jointime(const prob_t *pb, const double *tv, int slidepop, int sispop,
+                         double edgeuptime, double sisuptime) {
+  int ep = o_period(pb, tv, edgeuptime), sp = o_period(pb, tv, sisuptime);
+  while (ep < sp) {
+    ep++;
+    if (o_dropped(pb, ep, slidepop)) slidepop = pb->popdown[slidepop];
+  }
+  while (sp < ep) {
+    sp++;
+    if (o_dropped(pb, sp, sispop)) sispop = pb->popdown[sispop];
+  }
+  while (slidepop != sispop) {
+    ep++;
+    if (o_dropped(pb, ep, slidepop)) slidepop = pb->popdown[slidepop];
+    if (o_dropped(pb, ep, sispop)) sispop = pb->popdown[sispop];
+  }
+  return ep == 0 ? 0.0 : tv[ep - 1];
+}
+
+static double o_uptime(const gene_t *g, int e) { return e < g->n ? 0.0 : g->time[g->up0[e]]; }
+
+/* slider_nomigration, the recursion as a loop: *sis and *tp move, *d is the
+ * signed remaining distance (negative: up toward the tips) */
+static void o_slider(const prob_t *pb, const gene_t *g, const double *tv, rng_t *r, int edge,
+                     int *sis, double *tp, double d) {
+  for (;;) {
+    if (d < 0) {
+      d = -d;
+      const double eu = o_uptime(g, edge), su = o_uptime(g, *sis);
+      const int spop = g->pop[edge], ppop = g->pop[*sis];
+      const double pj = spop != ppop ? o_jointime(pb, tv, spop, ppop, eu, su) : 0.0;
+      if (pj > eu && pj > su) {
+        if (d < *tp - pj) { *tp -= d; return; }
+        d -= *tp - pj;
+        *tp = pj;
+      } else if (su == 0 || eu >= su) {
+        if (d < *tp - eu) { *tp -= d; return; }
+        d -= *tp - eu;
+        *tp = eu;
+      } else {
+        if (d < *tp - su) { *tp -= d; return; }
+        d -= *tp - su;
+        *tp = su;
+        *sis = (xoshiro(r) >> 63) ? g->up0[*sis] : g->up1[*sis];
+        d = -d;
+      }
+    } else {
+      if (g->down[*sis] == -1 || *tp + d < g->time[*sis]) {
+        *tp += d;
+        if (*tp >= 1000000.0) *tp = 1000000.0; /* TIMEMAX */
+        return;
+      }
+      d -= g->time[*sis] - *tp;
+      *tp = g->time[*sis];
+      if (xoshiro(r) >> 63) {
+        *sis = g->down[*sis];
+      } else {
+        const int dd = g->down[*sis];
+        *sis = g->up0[dd] == *sis ? g->up1[dd] : g->up0[dd];
+        d = -d;
+      }
+    }
+  }
+}
+
+static double o_lognorm(double s, double x) {
+  return -0.91893853320467274178 - log(s) - 0.5 * (x / s) * (x / s);
+}
+
+static void topo_move(const prob_t *pb, chain_t *ch, int l, rng_t *r, long *acc) {
+  gene_t *g = &ch->g[l];
+  const int n = g->n, ne = 2 * n - 1, R = rec_len(pb->P);
+  int *bk = (int *)malloc(sizeof(int) * 4 * ne);
+  double *bt = (double *)malloc(sizeof(double) * ne);
+  memcpy(bk, g->up0, sizeof(int) * ne);
+  memcpy(bk + ne, g->up1, sizeof(int) * ne);
+  memcpy(bk + 2 * ne, g->down, sizeof(int) * ne);
+  memcpy(bk + 3 * ne, g->pop, sizeof(int) * ne);
+  memcpy(bt, g->time, sizeof(double) * ne);
+  const int broot = g->root;
+  int edge;
+  do edge = (int)(unif(r) * ne);
+  while (edge < 0 || edge >= ne || g->down[edge] == -1);
+  const int freed = g->down[edge];
+  const int oldsis = g->up0[freed] == edge ? g->up1[freed] : g->up0[freed];
+  const double sd0 = fmin(20.0, g->time[g->up0[g->root]] / 3.0);
+  const double hold = sd0 * norm01(r);
+  int rootmove = 0;
+  double roottime = g->time[g->up0[g->root]];
+  /* joinsisdown */
+  g->time[oldsis] = g->time[freed];
+  g->down[oldsis] = g->down[freed];
+  const int dd0 = g->down[oldsis];
+  if (dd0 != -1) {
+    if (g->up0[dd0] == freed) g->up0[dd0] = oldsis;
+    else g->up1[dd0] = oldsis;
+  } else {
+    rootmove = 1;
+    g->root = oldsis;
+    g->time[oldsis] = 1000000.0;
+    roottime = o_uptime(g, oldsis);
+  }
+  /* slide */
+  int sis = oldsis;
+  double tp = g->time[edge];
+  o_slider(pb, g, ch->tv, r, edge, &sis, &tp, hold);
+  /* splitsisdown */
+  g->time[edge] = tp;
+  g->time[freed] = g->time[sis];
+  g->time[sis] = tp;
+  const int dd = g->down[sis];
+  if (dd != -1) {
+    if (g->up0[dd] == sis) g->up0[dd] = freed;
+    else g->up1[dd] = freed;
+  } else {
+    g->root = freed;
+    roottime = tp;
+    rootmove = 1;
+  }
+  g->down[freed] = dd;
+  int nowpop = g->pop[sis];
+  const int j = o_period(pb, ch->tv, tp);
+  while (pb->pope[nowpop] != -1 && pb->pope[nowpop] <= j) nowpop = pb->popdown[nowpop];
+  g->pop[freed] = nowpop;
+  g->down[sis] = freed;
+  g->down[edge] = freed;
+  g->up0[freed] = sis;
+  g->up1[freed] = edge;
+  double hast = 0.0;
+  if (rootmove) hast = o_lognorm(fmin(20.0, roottime / 3.0), hold) - o_lognorm(sd0, hold);
+  /* score and decide as slide() does */
+  const double np = hky(pb, l, g, -1);
+  double *nrec = ch->nrec;
+  weights(pb, l, g, ch->tv, nrec);
+  const int P2 = pb->P * pb->P, P3 = P2 * pb->P;
+  for (int i = 0; i < R; i++) {
+    const double x = ch->all[i], o = g->rec[i], w = nrec[i];
+    ch->hold[i] = x;
+    if ((i >= P2 && i < 2 * P2) || (i >= 3 * P2 + P3 && i < 3 * P2 + 2 * P3))
+      ch->all[i] = fmax(0.0, (x - o) + w);
+    else if (i < P2 || (i >= 3 * P2 && i < 3 * P2 + P3))
+      ch->all[i] = x + (w - o);
+    else
+      ch->all[i] = (x - o) + w;
+  }
+  const double pn = prior_terms(pb, ch->all, ch->qc, ch->qf, ch->qt, ch->nc, ch->nf, ch->nt);
+  const double U = unif(r);
+  const int ok = U < fmin(1.0, exp(ch->beta * ((pn - ch->probg) + (np - g->pdg)) + hast));
+  if (ok) {
+    g->pdg = np;
+    ch->probg = pn;
+    memcpy(g->rec, nrec, sizeof(double) * R);
+    const int np_ = pb->npsize + pb->nmig;
+    memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+    memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+    memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+    (*acc)++;
+  } else {
+    memcpy(g->up0, bk, sizeof(int) * ne);
+    memcpy(g->up1, bk + ne, sizeof(int) * ne);
+    memcpy(g->down, bk + 2 * ne, sizeof(int) * ne);
+    memcpy(g->pop, bk + 3 * ne, sizeof(int) * ne);
+    memcpy(g->time, bt, sizeof(double) * ne);
+    g->root = broot;
+    memcpy(ch->all, ch->hold, sizeof(double) * R);
+  }
+  hky_settle(g, ok);
+  free(bk);
+  free(bt);
 }
 
 static void rescale(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
@@ -507,11 +699,13 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    double *times, const int *roots, const int *nmig, const int *mig_edge,
                    double *mig_time, const int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
-                   long max_sweeps, long *stats, int ry1_update, int urate_update) {
+                   long max_sweeps, long *stats, int ry1_update, int urate_update,
+                   int topo_update, int *roots_out) {
   prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi,
                P, npsize, nmigp, plist, npresent, popdown, pope, join,
                theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width,
-               ry1_update && P > 1};
+               ry1_update && P > 1, topo_update};
+  if (topo_update && nmigp > 0) return -1; /* no migration-path proposals */
   const int R = rec_len(P), np_ = npsize + nmigp;
   if (P > 10 || P - 1 > 16) return -1;
   chain_t *ch = (chain_t *)calloc(nchains, sizeof(chain_t));
@@ -580,7 +774,10 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
   do {
     for (int c = 0; c < nchains; c++) {
       for (int l = 0; l < L; l++) {
-        slide(&pb, &ch[c], l, &r, &acc[0]);
+        if (pb.topo)
+          topo_move(&pb, &ch[c], l, &r, &acc[0]);
+        else
+          slide(&pb, &ch[c], l, &r, &acc[0]);
         props[0]++;
       }
       if (sweeps & 1) {
@@ -632,6 +829,7 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
     for (int k = 0; k < P - 1; k++) tvs[c * (P - 1) + k] = ch[c].tv[k];
     for (int l = 0; l < L; l++) {
       gene_t *g = &ch[c].g[l];
+      if (roots_out) roots_out[c * L + l] = g->root;
       u[c * L + l] = g->u;
       kappa[c * L + l] = g->kappa;
       free(g->st.frac);

@@ -136,3 +136,102 @@ def test_device_and_oracle_sample_the_same_posterior(ry1, urate):
             print(f"stat {s} q={qn:.2f} gpu={fa.mean():.4f} cpu={fb.mean():.4f} z={z:.2f}")
             worst = max(worst, z)
     assert worst < 4.5
+
+
+def _topo_setup(seed):
+    rng = np.random.default_rng(seed)
+    pt, theta, ts, mig = sim.im_model_two_pop(t=0.8, m12=0.0, m21=0.0)
+    model = immodel.build(pt, qmax=15.0, migration=False)
+    samp = [3, 3]
+    loci, base = [], []
+    for _ in range(2):
+        g = sim.simulate_genealogy(rng, samp, theta, ts, np.zeros((2, 2)), pt)
+        loci.append(sim.encode_hky(sim.simulate_hky_sequences(
+            rng, g, 150, np.array([0.3, 0.2, 0.2, 0.3]), 4.0, 2.0)))
+        base.append(g)
+    return model, samp, loci, base, np.asarray(ts, float)
+
+
+def _topo_stats(up0, up1, dn, tm, roots, n, Lc):
+    """Per locus: the root time and whether gene copies 0 and 1 form a
+    cherry; every array packed over the chain's loci."""
+    out, off = [], 0
+    ne = 2 * n - 1
+    for l in range(Lc):
+        u0, u1, t, rt = up0[off:off + ne], up1[off:off + ne], tm[off:off + ne], roots[l]
+        off += ne
+        cherry = any((u0[e] == 0 and u1[e] == 1) or (u0[e] == 1 and u1[e] == 0)
+                     for e in range(n, ne))
+        out += [float(t[u0[rt]]), float(cherry)]
+    return out
+
+
+def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
+    """The reference's own genealogy move (IMA_MC3_TOPOLOGY) on the device
+    and in the oracle's sweep (two populations without migration, two HKY
+    loci, the rubber-band split-time update on alternate sweeps): root
+    times, the split time and a topology statistic (gene copies 0 and 1 a
+    cherry) have the same marginal posteriors (|z| < 4.5)."""
+    model, samp, loci, base, ts = _topo_setup(19)
+    C, burn, nsamp, thin = 48, 400, 150, 20
+    n, Lc = base[0].n, len(loci)
+    # device
+    e = Engine(0)
+    mid = e.model_add(model)
+    lids = []
+    for loc in loci:
+        lid = e.add_locus_hky(loc)
+        e.locus_set_model(lid, mid, samp)
+        lids.append(lid)
+    gens = [base[l].copy() for c in range(C) for l in range(Lc)]
+    first = e.alloc_slots([lids[l] for c in range(C) for l in range(Lc)])
+    e.write_slots(first, gens)
+    e.write_pops(first, gens)
+    e.mc3_setup(C, Lc, first, 2.0, 4.0, np.ones(C), seed=78, precision=L.PREC_FP64,
+                scale_width=0.3)
+    e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
+    flags = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_TOPOLOGY
+    e.mc3_run(burn, flags)
+    a = []
+    for _ in range(nsamp):
+        e.mc3_run(thin, flags)
+        up0, up1, dn, tm, rt, _ = e.read_slots(first, C * Lc)
+        _, tv = e.mc3_read_prior()
+        ne = (2 * n - 1) * Lc
+        a.append([_topo_stats(up0[c * ne:(c + 1) * ne], up1[c * ne:(c + 1) * ne],
+                              dn[c * ne:(c + 1) * ne], tm[c * ne:(c + 1) * ne],
+                              rt[c * Lc:(c + 1) * Lc], n, Lc) + [float(tv[c][0])]
+                  for c in range(C)])
+    e.close()
+    a = np.array(a).transpose(1, 0, 2)
+    # oracle
+    chains = [[g.copy() for g in base] for _ in range(C)]
+    tvs = np.ascontiguousarray(np.tile(ts, (C, 1)))
+    b = []
+    for k in range(burn // thin + nsamp):
+        oracle.mc3_bench(loci, [2.0, 2.0], 4.0, samp, model, chains, tvs, np.ones(C), 5.0,
+                         seed=70000 + k, max_sweeps=thin, scale_width=0.3, ry1=True, topo=True)
+        if k >= burn // thin:
+            row = []
+            for c in range(C):
+                gs = chains[c]
+                row.append(_topo_stats(np.concatenate([g.up0 for g in gs]),
+                                       np.concatenate([g.up1 for g in gs]),
+                                       np.concatenate([g.down for g in gs]),
+                                       np.concatenate([g.time for g in gs]),
+                                       [g.root for g in gs], n, Lc) + [float(tvs[c][0])])
+            b.append(row)
+    b = np.array(b).transpose(1, 0, 2)
+    worst = 0.0
+    for s in range(a.shape[2]):
+        binary = set(np.unique(a[:, :, s])) <= {0.0, 1.0}
+        pooled = np.concatenate([a[:, :, s].ravel(), b[:, :, s].ravel()])
+        for qn in ((0.5,) if binary else (0.1, 0.25, 0.5, 0.75, 0.9)):
+            tau = 0.5 if binary else np.quantile(pooled, qn)
+            fa = (a[:, :, s] <= tau).mean(axis=1)
+            fb = (b[:, :, s] <= tau).mean(axis=1)
+            se = math.sqrt(fa.var(ddof=1) / len(fa) + fb.var(ddof=1) / len(fb))
+            z = abs(fa.mean() - fb.mean()) / max(se, 1e-3)
+            print(f"stat {s} q={qn:.2f} gpu={fa.mean():.4f} cpu={fb.mean():.4f} z={z:.2f}")
+            worst = max(worst, z)
+    assert worst < 4.5

@@ -64,6 +64,77 @@ def test_flat_likelihood_samples_the_prior(n):
     assert _law(v, cdf, (0.1, 0.25, 0.5, 0.75, 0.9)) < 4.5
 
 
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_topology_move_flat_likelihood_samples_the_prior(n):
+    """The oracle's restatement of the reference's own move (slider_nomigration,
+    joinsisdown, splitsisdown, root-move Hastings term) with the time
+    update switched off (scale width 0): n = 2, every move slides the root,
+    the coalescence time follows x E1(x) - e^-x + 1; n = 3, each gene copy
+    is the root's lone daughter with probability 1/3 and the interval ratio
+    follows 3r / (1 + 2r); n = 4, the balanced topology has probability 1/3.
+    Dropping the root move's Hastings term fails the n = 2 law."""
+    C = 32
+    model, loc, chains = _flat_one_pop(n, C, 21 + n)
+    tvs = np.zeros((C, 0))
+    stat, ratio = [], []
+    for k in range(170):
+        oracle.mc3_bench([loc], [1e-9], 1.0, [[n]], model, chains, tvs, np.ones(C), 10.0,
+                         seed=5000 * n + k, max_sweeps=30, scale_width=0.0, topo=True)
+        if k >= 20:
+            row, rrow = [], []
+            for c in range(C):
+                g = chains[c][0]
+                t = np.sort(g.time[g.down != -1])
+                a, b = g.up0[g.root], g.up1[g.root]
+                if n == 2:
+                    row.append(t.max())
+                elif n == 3:
+                    row.append(a if a < 3 else b)
+                    rrow.append(t.min() / t.max())
+                else:
+                    row.append(int(a >= 4 and b >= 4))
+            stat.append(row)
+            ratio.append(rrow)
+    v = np.array(stat).T  # [chain, sample]
+    if n == 2:
+        def cdf(t):
+            x = 2.0 * t / 8.0
+            return x * exp1(x) - math.exp(-x) + 1.0
+        assert _law(v, cdf, (0.1, 0.25, 0.5, 0.75, 0.9)) < 4.5
+        return
+    cats = range(3) if n == 3 else [1]
+    for kcat in cats:
+        f = (v == kcat).mean(axis=1)
+        se = f.std(ddof=1) / math.sqrt(C)
+        assert abs(f.mean() - 1 / 3) < 4.5 * max(se, 1e-3), (kcat, f.mean(), se)
+    if n == 3:
+        assert _law(np.array(ratio).T, lambda r: 3 * r / (1 + 2 * r),
+                    (0.1, 0.25, 0.5, 0.75, 0.9)) < 4.5
+
+
+def test_topology_move_keeps_two_population_labels_valid():
+    """Two populations without migration: after topology moves every
+    genealogy still fits the model (treeweight accepts its labels) and the
+    stored likelihoods equal fresh evaluations."""
+    rng = np.random.default_rng(31)
+    pt, theta, ts, mig = sim.im_model_two_pop(t=0.6, m12=0.0, m21=0.0)
+    model = immodel.build(pt, qmax=15.0, migration=False)
+    samp = [4, 5]
+    g0 = sim.simulate_genealogy(rng, samp, theta, ts, np.zeros((2, 2)), pt)
+    loc = sim.encode_hky(sim.simulate_hky_sequences(rng, g0, 120, [0.3, 0.2, 0.2, 0.3], 4.0, 1.0))
+    C = 6
+    chains = [[g0.copy()] for _ in range(C)]
+    tvs = np.tile(np.asarray(ts, float), (C, 1))
+    oracle.mc3_bench([loc], [1.0], 4.0, [samp], model, chains, tvs, np.ones(C), 5.0, seed=9,
+                     max_sweeps=200, scale_width=0.3, ry1=True, topo=True)
+    changed = 0
+    for c in range(C):
+        g = chains[c][0]
+        changed += int(not np.array_equal(g.up0, g0.up0))
+        oracle.treeweight(g, model, tvs[c], samp)  # raises on labels that do not fit
+    assert changed > 0
+
+
 def test_three_population_sweep_keeps_genealogies_valid():
     wl = workloads.three_pop_hky(nchains=3, nloci=4, nsites=300, per_pop=5, pool=3)
     model = immodel.build(wl.poptree)
