@@ -104,6 +104,8 @@ PROTOS = {
     "ima_lmode_table_free": (C.c_int, [vp, C.c_int]),
     "ima_lmode_jointp": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, dp, dp]),
+    "ima_lmode_jointp_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, vp, vp]),
     "ima_lmode_joint_peak": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_uint64,
                                        C.c_int, dp, dp, dp, C.POINTER(C.c_int64)]),
     "ima_lmode_marginal_peak": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

@@ -290,6 +290,35 @@ int ima_device_memory(int device, double *free_bytes, double *total_bytes) {
   return IMA_OK;
 }
 
+// Work area of ima_lmode_jointp_noalloc: the per (point, chunk) partial
+// results of jointp.  One copy per device, held until the process ends and
+// shared by every context on that device.  Host code never passes this
+// name to a kernel: each device's address comes from hipGetSymbolAddress,
+// taken once per device when its first context is created.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on `device` (which is current), looked up
+// on the first call for that device.
+static hipError_t scratch_address(int device, void **ptr) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 int ima_ctx_create(int device, ima_ctx **out) {
   if (!out) return IMA_ERR_ARG;
   *out = nullptr;
@@ -299,6 +328,7 @@ int ima_ctx_create(int device, ima_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return IMA_ERR_HIP;
   std::unique_ptr<ima_ctx> c(new ima_ctx);
   c->device = device;
+  if (scratch_address(device, &c->scratch) != hipSuccess) return IMA_ERR_HIP;
   if (hipStreamCreate(&c->stream) != hipSuccess) return IMA_ERR_HIP;
   if (hipStreamCreate(&c->side) != hipSuccess) return IMA_ERR_HIP;
   for (int k = 0; k < 8; ++k) {
@@ -1070,6 +1100,32 @@ int ima_lmode_jointp(ima_ctx *c, int id, const double *x, int npoints, int model
                              c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return timing_collect(c, K_LMODE);
+}
+
+int ima_lmode_jointp_noalloc(ima_ctx *c, int id, const double *x, int npoints, int modeltype,
+                             int r0, int r1, int mode, double *out, double *ess) {
+  if (!c || !x || !out || npoints < 0 || id < 0 || id >= (int)c->tables.size() ||
+      !c->tables[id].live)
+    return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  const int np = h.npsize + h.nmig;
+  if (h.G < 1) return fail(c, IMA_ERR_STATE, "table %d holds no genealogies", id);
+  if (r0 < 0 || r1 > np || r0 >= r1) return fail(c, IMA_ERR_ARG, "bad parameter range");
+  if (modeltype < 0 || modeltype > 2 || (mode != IMA_LMODE_EXACT && mode != IMA_LMODE_REF_COMPAT))
+    return IMA_ERR_ARG;
+  if (!c->scratch) return fail(c, IMA_ERR_STATE, "no scratch area on device %d", c->device);
+  if (npoints == 0) return IMA_OK;
+  const LTable t = ltable(h);
+  const size_t wb = jointp_work_bytes(t, npoints);
+  if (wb > kScratchBytes)
+    return fail(c, IMA_ERR_SIZE,
+                "%d points on %d genealogies need %zu B of work area; the limit is %zu B "
+                "(fewer points per call)", npoints, h.G, wb, kScratchBytes);
+  hipSetDevice(c->device);
+  // every (point, chunk) partial is written before it is read: nothing to clear
+  HIPCHK(c, launch_jointp(t, x, npoints, modeltype, r0, r1, mode,
+                          static_cast<double *>(c->scratch), kScratchBytes, out, ess, c->stream));
+  return IMA_OK;
 }
 
 int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx, int logi,

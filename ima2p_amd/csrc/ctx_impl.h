@@ -28,6 +28,7 @@ using ima::Timing;
 
 struct ima_ctx {
   int device = 0;
+  void *scratch = nullptr;  // this device's address of the static work area
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // genealogy weights beside the likelihood launch
   std::string err;

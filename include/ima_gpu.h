@@ -227,6 +227,20 @@ int ima_lmode_table_read(ima_ctx *ctx, int table_id, int first, int count, float
 int ima_lmode_jointp(ima_ctx *ctx, int table_id, const double *x, int npoints,
                      int modeltype, int prange0, int prange1, int mode,
                      double *out, double *ess);
+/* ima_lmode_jointp over DEVICE pointers that allocates nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  x (npoints x (npsize + nmig), values > 0), out and ess
+ * (may be NULL) are device buffers the caller owns; the per (point, chunk)
+ * work area is a static 16 MiB array of the library, one per device, held
+ * until the process ends.  Calls that need more work area than that fail
+ * with IMA_ERR_SIZE (split the points).  The launches are enqueued on the
+ * context's stream and the call returns without waiting.  Every context
+ * on a device shares that device's work area: calls that overlap on one
+ * device race. */
+int ima_lmode_jointp_noalloc(ima_ctx *ctx, int table_id, const double *x, int npoints,
+                             int modeltype, int prange0, int prange1, int mode, double *out,
+                             double *ess);
 /* The joint posterior peak of a table by differential evolution, as
  * findjointpeaks / modelloop search it (jointfind.cpp:596-812, 1089-1183),
  * every generation's trials scored in one batched jointp launch.
