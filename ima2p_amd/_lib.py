@@ -17,11 +17,24 @@ MODEL_HKY, MODEL_IS, MODEL_SW = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 STATUS_OK, STATUS_IS_REJECT, STATUS_SW_ZERO, STATUS_IS_NOSITE, STATUS_PRIOR_BAD = 0, 1, 2, 3, 4
 LMODE_REF_COMPAT, LMODE_EXACT = 0, 1
+POPMIG_CALC, POPMIG_MARGIN = 0, 1
 REJECT_IS = -1000000000.0
 MAX_GENES = 256
 K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3, K_ACCEPT = range(8)
 MC3_GRAPH, MC3_SCALE, MC3_TIMED, MC3_RY1 = 1, 2, 4, 8
 MC3_UPDATE_U, MC3_U_SERIAL, MC3_TOPOLOGY = 16, 32, 64
+
+
+class McfLocus(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n", C.c_int32), ("nsites", C.c_int32)]
+
+
+class McfState(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in
+                ("tvals", "u", "kappa", "pi", "time", "dlikeA", "mig_time")] + \
+               [(k, C.POINTER(C.c_int32)) for k in
+                ("up0", "up1", "down", "mut", "pop", "A", "mig_count", "mig_to")] + \
+               [("mig_capacity", C.c_int64)]
 
 
 class Item(C.Structure):
@@ -110,6 +123,16 @@ PROTOS = {
                                        C.c_int, dp, dp, dp, C.POINTER(C.c_int64)]),
     "ima_lmode_marginal_peak": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                           C.c_double, dp]),
+    "ima_lmode_popmig": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp]),
+    "ima_lmode_popmig_peak": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, C.c_int, dp, dp,
+                                        C.POINTER(C.c_int64)]),
+    "ima_lmode_greater_than": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
+    "ima_slots_read_mig_events": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), i32p, dp, i32p]),
+    "ima_mcf_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, vp]),
+    "ima_mcf_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, dp, vp,
+                               C.POINTER(C.c_int64)]),
     "ima_lmode_margincalc": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int,
                                        C.c_double, C.c_int, C.c_double, dp]),
     "ima_mc3_setup": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_uint64, C.c_int,

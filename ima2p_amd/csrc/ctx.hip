@@ -673,7 +673,7 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
       idx_s.push_back(i);
     }
   }
-  const size_t hlds = hky_lds_bytes(precision, hmax_n);
+  const size_t hlds = hky_lds_bytes(precision, hmax_n, hmax_slots);
   if (!hky.empty() && hmax_slots > 8)
     return fail(c, IMA_ERR_SIZE, "a genealogy needs %d partial slots (limit 8)", hmax_slots);
   if (!hky.empty() && hlds > (size_t)c->max_lds)
@@ -1128,6 +1128,98 @@ int ima_lmode_jointp_noalloc(ima_ctx *c, int id, const double *x, int npoints, i
   return IMA_OK;
 }
 
+int ima_lmode_popmig(ima_ctx *c, int id, int theta_i, int mig_i, double qmax, double mprior,
+                     int expo, int kind, int prob_or_like, int first, int last, const double *x,
+                     int nx, double *out) {
+  if (!c || !x || !out || nx < 0 || id < 0 || id >= (int)c->tables.size() ||
+      !c->tables[id].live)
+    return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  if (h.G < 1) return fail(c, IMA_ERR_STATE, "table %d holds no genealogies", id);
+  if (theta_i < 0 || theta_i >= h.npsize || mig_i < 0 || mig_i >= h.nmig)
+    return fail(c, IMA_ERR_ARG, "no population size %d / migration rate %d in the table", theta_i,
+                mig_i);
+  if (!(qmax > 0) || !(mprior > 0)) return fail(c, IMA_ERR_ARG, "prior bounds must be > 0");
+  if (kind == IMA_POPMIG_CALC) {
+    for (int i = 0; i < nx; ++i)
+      if (!(x[i] > 0)) return fail(c, IMA_ERR_ARG, "2NM values must be > 0");
+  } else if (kind == IMA_POPMIG_MARGIN) {
+    if (first < 0 || last > h.G || first >= last)
+      return fail(c, IMA_ERR_ARG, "genealogy range [%d, %d) outside the table's %d", first, last,
+                  h.G);
+  } else {
+    return IMA_ERR_ARG;
+  }
+  if (nx == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  grow_logfact(c, 16384);
+  int rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  const LTable t = ltable(h);
+  HIPCHK(c, reserve(c->d_lx, (size_t)nx));
+  HIPCHK(c, reserve(c->d_lwork, popmig_work_bytes(t, nx) / 8 + 1));
+  HIPCHK(c, reserve(c->d_lout, (size_t)nx));
+  HIPCHK(c, hipMemcpyAsync(c->d_lx.p, x, (size_t)nx * 8, hipMemcpyHostToDevice, c->stream));
+  if ((rc = timing_begin(c, K_LMODE)) != IMA_OK) return rc;
+  HIPCHK(c, launch_popmig(t, theta_i, mig_i, qmax, mprior, expo, kind, prob_or_like, first, last,
+                          c->d_lx.p, nx, c->d_logfact.p, (int)c->logfact.size(), c->d_lwork.p,
+                          c->d_lout.p, c->stream));
+  if ((rc = timing_end(c, K_LMODE)) != IMA_OK) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->d_lout.p, (size_t)nx * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return timing_collect(c, K_LMODE);
+}
+
+int ima_lmode_greater_than(ima_ctx *c, int id, int kind, int n, const double *prior_max,
+                           double *out) {
+  if (!c || !prior_max || !out || id < 0 || id >= (int)c->tables.size() || !c->tables[id].live ||
+      (kind != 0 && kind != 1))
+    return IMA_ERR_ARG;
+  const LTableHost &h = c->tables[id];
+  if (h.G < 1) return fail(c, IMA_ERR_STATE, "table %d holds no genealogies", id);
+  if (n != (kind ? h.nmig : h.npsize))
+    return fail(c, IMA_ERR_ARG, "the table has %d %s, not %d", kind ? h.nmig : h.npsize,
+                kind ? "migration rates" : "population sizes", n);
+  std::vector<int2> pairs;
+  std::vector<double> pm;
+  for (int i = 0; i < n * n; ++i) out[i] = -1.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      if (i == j || prior_max[i] != prior_max[j]) continue;
+      if (!(prior_max[i] > 0)) return fail(c, IMA_ERR_ARG, "prior bounds must be > 0");
+      if (kind && !(prior_max[i] > 1e-7)) continue;  // MINPARAMVAL
+      pairs.push_back(make_int2(i, j));
+      pm.push_back(prior_max[i]);
+    }
+  if (pairs.empty()) return IMA_OK;
+  hipSetDevice(c->device);
+  grow_logfact(c, 16384);
+  int rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  const LTable t = ltable(h);
+  const int np = (int)pairs.size();
+  const size_t ntrees = std::min(h.G, 20000);
+  HIPCHK(c, reserve(c->d_lx, (size_t)np * 2));
+  HIPCHK(c, reserve(c->d_lwork, (size_t)np * ntrees + (size_t)np));
+  HIPCHK(c, reserve(c->d_lout, (size_t)np));
+  std::vector<double> stage(2 * (size_t)np);
+  memcpy(stage.data(), pairs.data(), (size_t)np * sizeof(int2));
+  memcpy(stage.data() + np, pm.data(), (size_t)np * 8);
+  HIPCHK(c, hipMemcpyAsync(c->d_lx.p, stage.data(), stage.size() * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  if ((rc = timing_begin(c, K_LMODE)) != IMA_OK) return rc;
+  HIPCHK(c, launch_greater_than(t, kind, reinterpret_cast<const int2 *>(c->d_lx.p),
+                                c->d_lx.p + np, np, c->d_logfact.p, (int)c->logfact.size(),
+                                c->d_lwork.p, c->d_lout.p, c->stream));
+  if ((rc = timing_end(c, K_LMODE)) != IMA_OK) return rc;
+  std::vector<double> res(np);
+  HIPCHK(c, hipMemcpyAsync(res.data(), c->d_lout.p, (size_t)np * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < np; ++k) out[pairs[k].x * n + pairs[k].y] = res[k];
+  return timing_collect(c, K_LMODE);
+}
+
 int ima_lmode_margincalc(ima_ctx *c, int id, int param, const double *x, int nx, int logi,
                          double yadjust, int expo, double prior_mean, double *out) {
   if (!c || !x || !out || nx < 0 || id < 0 || id >= (int)c->tables.size() ||
@@ -1292,8 +1384,12 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
       sw.push_back(i);
     }
   }
-  // the sweep's proposals never change a topology, so the stack depth the
-  // programs need is fixed for the run
+  // time-slide proposals keep every topology, so the stack depth the
+  // programs need is fixed for the run unless topology moves are enabled
+  // (mc3_topo_mode raises it to the bound of any topology)
+  if (!hky.empty() && hky_lds_bytes(precision, m.hmax_n, m.hmax_slots) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "HKY loci of %d gene copies need %zu B of LDS (limit %d)",
+                m.hmax_n, hky_lds_bytes(precision, m.hmax_n, m.hmax_slots), c->max_lds);
   m.nh = (int)hky.size();
   m.ni = (int)is.size();
   m.ns = (int)sw.size();
@@ -1538,6 +1634,15 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
   if (d.prior >= 0 && c->models[d.prior].nmig > 0)
     return fail(c, IMA_ERR_STATE,
                 "the device's genealogy move needs a model without migration parameters");
+  {
+    // any topology may come up: the pruning stack must hold floor(log2 n)
+    int bound = 1;
+    while ((2 << bound) <= m.hmax_n) ++bound;
+    if (m.nh && hky_lds_bytes(m.precision, m.hmax_n, std::max(m.hmax_slots, bound)) >
+                    (size_t)c->max_lds)
+      return fail(c, IMA_ERR_SIZE, "topology moves on %d gene copies need more LDS than %d B",
+                  m.hmax_n, c->max_lds);
+  }
   const int N = m.C * m.L;
   int rc;
   if (m.topo_first < 0) {
@@ -2296,6 +2401,24 @@ int ima_slots_read_mig(ima_ctx *c, int slot, double *mig_time) {
   const SlotMeta &s = c->slots[slot];
   if (s.nmig)
     HIPCHK(c, copy_now(c, mig_time, c->d_mig_time.p + s.mig_off, (size_t)s.nmig * 8,
+                        hipMemcpyDeviceToHost));
+  return IMA_OK;
+}
+
+int ima_slots_read_mig_events(ima_ctx *c, int slot, int32_t *count, int32_t *mig_edge,
+                              double *mig_time, int32_t *mig_to) {
+  if (!c || !count || slot < 0 || slot >= (int)c->slots.size()) return IMA_ERR_ARG;
+  const SlotMeta &s = c->slots[slot];
+  *count = s.nmig;
+  if (!s.nmig || (!mig_edge && !mig_time && !mig_to)) return IMA_OK;
+  if (mig_edge)
+    HIPCHK(c, copy_now(c, mig_edge, c->d_mig_edge.p + s.mig_off, (size_t)s.nmig * 4,
+                        hipMemcpyDeviceToHost));
+  if (mig_time)
+    HIPCHK(c, copy_now(c, mig_time, c->d_mig_time.p + s.mig_off, (size_t)s.nmig * 8,
+                        hipMemcpyDeviceToHost));
+  if (mig_to)
+    HIPCHK(c, copy_now(c, mig_to, c->d_mig_to.p + s.mig_off, (size_t)s.nmig * 4,
                         hipMemcpyDeviceToHost));
   return IMA_OK;
 }

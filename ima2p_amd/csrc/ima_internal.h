@@ -176,7 +176,7 @@ void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int 
                   int *nblocks_total);
 
 // Kernel launchers (kern_hky.hip, kern_lik.hip).
-size_t hky_lds_bytes(int precision, int max_n);
+size_t hky_lds_bytes(int precision, int max_n, int max_slots);
 hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                       int precision, int max_n, int max_slots, double *out,
                       int32_t *status, hipStream_t st);
@@ -193,7 +193,7 @@ struct TwLaunch {
 };
 // HKY units (no merge step) and treeweight in one launch, when its LDS
 // (hky_tw_lds_bytes) is within kTwLdsBudget.
-size_t hky_tw_lds_bytes(int precision, int max_n, int P, int emax);
+size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int P, int emax);
 hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                          int precision, int max_n, int max_slots, double *out, int32_t *status,
                          const TwLaunch &tw, int P, hipStream_t st);
@@ -218,6 +218,14 @@ hipError_t launch_jointp(const LTable &t, const double *x, int npoints,
                          double *work, size_t work_bytes, double *out,
                          double *ess, hipStream_t st);
 size_t jointp_work_bytes(const LTable &t, int npoints);
+hipError_t launch_greater_than(const LTable &t, int kind, const int2 *pairs, const double *pmax,
+                               int npairs, const double *lf, int nlf, double *per, double *out,
+                               hipStream_t st);
+size_t popmig_work_bytes(const LTable &t, int nx);
+hipError_t launch_popmig(const LTable &t, int theta_i, int mig_i, double qmax, double mprior,
+                         int expo, int kind, int prob_or_like, int first, int last,
+                         const double *x, int nx, const double *lf, int nlf, double *work,
+                         double *out, hipStream_t st);
 hipError_t launch_margincalc(const LTable &t, int param, const double *x,
                              int nx, int logi, double yadjust, int expo,
                              double prior_mean, double *work, double *out,

@@ -87,10 +87,14 @@ __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlan
 // Three-stage operand pipeline.  Mid: a decoded program entry whose tip
 // states are being read from LDS.  Nxt: an entry whose branch coefficients
 // and tip columns of P are being read.  The compute step of an entry reads
-// nothing from LDS.
+// nothing from LDS.  Every read is made whatever the entry's type, so the
+// staging has no branches: an operand that is not a tip reads tip row
+// min(edge, n - 1) (any row: its value is not used), and an edge index is
+// always below 2n - 1.
 struct Mid {
-  int type, a, b;
-  int ta, tb;  // tip states (when the operand is a tip)
+  int type, a, b;  // the same in every lane
+  int ra, rb;      // tip rows read for a and b
+  int ta, tb;      // this lane's states at those rows
 };
 
 template <typename real>
@@ -100,22 +104,28 @@ struct Nxt {
   Q4<real> pa, pb;  // columns of P for tip operands
 };
 
-__device__ __forceinline__ void to_mid(const ProgOp &raw, const int8_t *tips, int lane, Mid &m) {
-  m.a = uni(raw.a);
-  m.b = uni(raw.b);
+__device__ __forceinline__ void to_mid(const ProgOp &raw, const uint8_t *tips, int lane, int nlast,
+                                       Mid &m) {
+  // a and b stay in vector registers (every lane holds the same value) and
+  // feed the LDS addresses directly; only the type is made scalar, for the
+  // branch
+  m.a = raw.a;
+  m.b = raw.b;
   m.type = uni(raw.type);
-  m.ta = (m.type == OP_TT) ? tips[m.a * kWave + lane] : 0;
-  m.tb = (m.type != OP_SS) ? tips[m.b * kWave + lane] : 0;
+  m.ra = min(m.a, nlast);
+  m.rb = min(m.b, nlast);
+  m.ta = tips[m.ra * kWave + lane];
+  m.tb = tips[m.rb * kWave + lane];
 }
 
 template <typename real>
 __device__ __forceinline__ void to_nxt(const Mid &m, const HkyCoef<real> *coef,
                                        const Q4<real> *colP, Nxt<real> &x) {
   x.type = m.type;
-  if (m.type != OP_TT) x.ca = coef[m.a];
-  if (m.type == OP_SS) x.cb = coef[m.b];
-  if (m.type == OP_TT) x.pa = colP[m.a * 4 + m.ta];
-  if (m.type != OP_SS) x.pb = colP[m.b * 4 + m.tb];
+  x.ca = coef[m.a];
+  x.cb = coef[m.b];
+  x.pa = colP[m.ra * 4 + m.ta];
+  x.pb = colP[m.rb * 4 + m.tb];
 }
 
 // Below this a node's partials are rescaled (a power of two: 2^-40 in
@@ -146,49 +156,51 @@ __device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &
   return L;
 }
 
-// One program operation with the stack holding `d` partials (compile-time),
-// returns the new depth.  TT pushes at d, ST rewrites d-1, SS pops into d-2.
-template <typename real, int D, int d>
-__device__ __forceinline__ int do_op(const Nxt<real> &c, Q4<real> (&stk)[D], const real pi[4],
-                                     int &E) {
-  if (c.type == OP_TT) {
-    if constexpr (d < D) stk[d] = combine<real>(c.pa, c.pb, E);
-    return d + 1;
-  } else if (c.type == OP_ST) {
-    if constexpr (d >= 1) stk[d - 1] = combine<real>(int_vec<real>(c.ca, stk[d - 1], pi), c.pb, E);
-    return d;
-  } else {
-    if constexpr (d >= 2)
-      stk[d - 2] = combine<real>(int_vec<real>(c.ca, stk[d - 2], pi),
-                                 int_vec<real>(c.cb, stk[d - 1], pi), E);
-    return d - 1;
+// One program operation.  The two top partials of the stack live in
+// registers (tos = stk[d-1], nos = stk[d-2]); the levels below them live in
+// this lane's column of the wavefront's LDS stack (level j at low[j * 64]).
+// TT pushes: nos spills to level d-2, tos becomes nos.  ST rewrites tos.
+// SS pops: the combination of nos (daughter a) and tos (daughter b) becomes
+// tos, and level d-3 is read back into nos.  The type and d are
+// wave-uniform, so the three cases are scalar branches.
+// While the stack holds fewer than three partials, level 0 is free: a push
+// spills its meaningless nos there and a pop reads it back unused, so the
+// LDS accesses need no branches.
+template <typename real>
+__device__ __forceinline__ void apply_op(const Nxt<real> &c, Q4<real> &tos, Q4<real> &nos, int &d,
+                                         Q4<real> *low, const real pi[4], int &E) {
+  // three independent wave-uniform tests (not an else-chain), so each case
+  // is one scalar compare and branch
+  const int type = c.type;
+  if (type == OP_TT) {
+    low[max(d - 2, 0) * kWave] = nos;
+    nos = tos;
+    tos = combine<real>(c.pa, c.pb, E);
+    ++d;
   }
-}
-
-template <typename real, int D>
-__device__ __forceinline__ int apply(const Nxt<real> &c, Q4<real> (&stk)[D], int d,
-                                     const real pi[4], int &E) {
-  switch (d) {
-    case 0: return do_op<real, D, 0>(c, stk, pi, E);
-    case 1: return do_op<real, D, 1>(c, stk, pi, E);
-    case 2: return do_op<real, D, 2>(c, stk, pi, E);
-    case 3: return do_op<real, D, 3>(c, stk, pi, E);
-    case 4: return do_op<real, D, 4>(c, stk, pi, E);
-    case 5: if constexpr (D >= 5) return do_op<real, D, 5>(c, stk, pi, E); else return d;
-    case 6: if constexpr (D >= 6) return do_op<real, D, 6>(c, stk, pi, E); else return d;
-    case 7: if constexpr (D >= 7) return do_op<real, D, 7>(c, stk, pi, E); else return d;
-    default: if constexpr (D >= 8) return do_op<real, D, 8>(c, stk, pi, E); else return d;
+  if (type == OP_ST) tos = combine<real>(int_vec<real>(c.ca, tos, pi), c.pb, E);
+  if (type == OP_SS) {
+    tos = combine<real>(int_vec<real>(c.ca, nos, pi), int_vec<real>(c.cb, tos, pi), E);
+    // read into nos itself, last: the wait for it falls at nos's next use,
+    // after the next entry's staging, not here
+    nos = low[max(d - 3, 0) * kWave];
+    --d;
   }
 }
 
 }  // namespace
 
-size_t hky_lds_bytes(int precision, int max_n) {
+// Stack levels kept in LDS for programs of up to `max_slots` partials (the
+// launch's depth class 4, 6 or 8, less the two held in registers).
+static int hky_low_levels(int max_slots) { return max_slots <= 4 ? 2 : max_slots <= 6 ? 4 : 6; }
+
+size_t hky_lds_bytes(int precision, int max_n, int max_slots) {
   const size_t ne = 2 * (size_t)max_n - 1;
   const size_t r = precision == IMA_PREC_FP32 ? 4 : 8;
-  // coefficients per edge, P columns per tip edge, program, tip states
-  return ne * (8 * r) + (size_t)max_n * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) +
-         (size_t)max_n * kWave;
+  // lower stack levels, coefficients per edge, P columns per tip edge,
+  // program, tip states
+  return (size_t)hky_low_levels(max_slots) * kWave * (4 * r) + ne * (8 * r) +
+         (size_t)max_n * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) + (size_t)max_n * kWave;
 }
 
 void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int S,
@@ -216,10 +228,13 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
   const LocusMeta lm = v.loci[sm.locus];
   const int n = sm.n, ne = 2 * n - 1, S = lm.S, nops = n - 1;
 
-  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(smem);           // [ne]
+  constexpr int kLow = D - 2;  // the launch's lower stack levels (D is 4, 6 or 8)
+  Q4<real> *low = reinterpret_cast<Q4<real> *>(smem) + lane;               // [kLow][64]
+  HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(
+      reinterpret_cast<Q4<real> *>(smem) + kLow * kWave);                  // [ne]
   Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);                 // [n][4] tips
   ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)n * 4);           // [nops]
-  int8_t *tips = reinterpret_cast<int8_t *>(ops + nops);                    // [n][64]
+  uint8_t *tips = reinterpret_cast<uint8_t *>(ops + nops);                  // [n][64]
 
   const double *pid = lm.pi;
   const double kappa = item.kappa;
@@ -295,9 +310,8 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
     }
     wave_lds_fence();
 
-    Q4<real> stk[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) stk[i].x = stk[i].y = stk[i].z = stk[i].w = real(0);
+    Q4<real> tos, nos;
+    tos.x = tos.y = tos.z = tos.w = nos.x = nos.y = nos.z = nos.w = real(0);
     int E = 0, d = 0;
     // op k computes while the coefficients and P columns of op k+1, the tip
     // states of op k+2 and the program entry of op k+3 are read from LDS;
@@ -305,24 +319,24 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
     const int last = nops - 1;
     Mid M0, M1;
     Nxt<real> N0, N1;
-    to_mid(ops[0], tips, lane, M0);
-    to_mid(ops[min(1, last)], tips, lane, M1);
+    to_mid(ops[0], tips, lane, n - 1, M0);
+    to_mid(ops[min(1, last)], tips, lane, n - 1, M1);
     to_nxt<real>(M0, coef, colP, N0);
     ProgOp raw = ops[min(2, last)];
     for (int k = 0; k < nops; k += 2) {
       // even: compute N0 (op k), N1 <- M1 (op k+1), M0 <- raw (op k+2)
       to_nxt<real>(M1, coef, colP, N1);
-      to_mid(raw, tips, lane, M0);
+      to_mid(raw, tips, lane, n - 1, M0);
       raw = ops[min(k + 3, last)];
-      d = apply<real, D>(N0, stk, d, pi, E);
+      apply_op<real>(N0, tos, nos, d, low, pi, E);
       if (k + 1 >= nops) break;
       // odd: compute N1 (op k+1), N0 <- M0 (op k+2), M1 <- raw (op k+3)
       to_nxt<real>(M0, coef, colP, N0);
-      to_mid(raw, tips, lane, M1);
+      to_mid(raw, tips, lane, n - 1, M1);
       raw = ops[min(k + 4, last)];
-      d = apply<real, D>(N1, stk, d, pi, E);
+      apply_op<real>(N1, tos, nos, d, low, pi, E);
     }
-    const real fr = pi[0] * stk[0].x + pi[1] * stk[0].y + pi[2] * stk[0].z + pi[3] * stk[0].w;
+    const real fr = pi[0] * tos.x + pi[1] * tos.y + pi[2] * tos.z + pi[3] * tos.w;
     if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
   }
   acc = wave_sum_d(acc);
@@ -442,7 +456,7 @@ hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int
                            int max_n, int max_slots, double *partial, double *out,
                            int32_t *status, hipStream_t st) {
   if (grid <= 0 || max_slots > 8) return hipErrorInvalidValue;
-  const size_t lds = hky_lds_bytes(precision, max_n);
+  const size_t lds = hky_lds_bytes(precision, max_n, max_slots);
   if (precision == IMA_PREC_FP32)
     launch_list_depth<float>(max_slots, dim3(grid), lds, st, v, items, list, cnt, cnt_next,
                              partial, out, status);
@@ -452,8 +466,8 @@ hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int
   return hipGetLastError();
 }
 
-size_t hky_tw_lds_bytes(int precision, int max_n, int P, int emax) {
-  return std::max(hky_lds_bytes(precision, max_n), treeweight_lds_bytes(P, emax));
+size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int P, int emax) {
+  return std::max(hky_lds_bytes(precision, max_n, max_slots), treeweight_lds_bytes(P, emax));
 }
 
 hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
@@ -461,7 +475,7 @@ hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPl
                          const TwLaunch &tw, int P, hipStream_t st) {
   if (plan.nunits == 0 || tw.n == 0) return hipErrorInvalidValue;
   if (max_slots > 8 || plan.nred > 0) return hipErrorInvalidValue;
-  const size_t lds = hky_tw_lds_bytes(precision, max_n, P, tw.emax);
+  const size_t lds = hky_tw_lds_bytes(precision, max_n, max_slots, P, tw.emax);
   if (lds > kTwLdsBudget) return hipErrorInvalidValue;
   const dim3 g(plan.nunits + tw.n);
   if (precision == IMA_PREC_FP32)
@@ -476,7 +490,7 @@ hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan 
                       hipStream_t st) {
   if (plan.nunits == 0) return hipSuccess;
   if (max_slots > 8) return hipErrorInvalidValue;
-  const size_t lds = hky_lds_bytes(precision, max_n);
+  const size_t lds = hky_lds_bytes(precision, max_n, max_slots);
   if (precision == IMA_PREC_FP32)
     launch_depth<float>(max_slots, dim3(plan.nunits), lds, st, v, items, plan, out, status);
   else

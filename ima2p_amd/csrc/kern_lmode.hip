@@ -19,6 +19,7 @@
 #include <float.h>
 
 #include "ima_internal.h"
+#include "prior_math.h"
 
 namespace ima {
 
@@ -589,6 +590,454 @@ hipError_t launch_margincalc(const LTable &t, int param, const double *x, int nx
   hipLaunchKernelGGL(margin_partial, grid, dim3(kLTile), lds, st, a, work);
   hipLaunchKernelGGL(margin_final, dim3((nx + 255) / 256), dim3(256), 0, st, a, logi,
                      yadjust, work, out);
+  return hipGetLastError();
+}
+
+// -------------------------------------------------------------- popmig --
+//
+// The density of 2NM = theta_i m_i / 2 at x, per genealogy the closed form
+// of integrating the coalescent and migration likelihoods over the priors
+// of theta_i and m_i along x = theta m / 2 (popmig.cpp:9-351):
+//   uniform priors: 2 x^mc (fc + fm x)^-(cc+mc) [G(cc+mc, 2(fc+fm x)/qmax)
+//     - G(cc+mc, mmax(fm + fc/x))] e^(-hc - qint - mint), with the
+//     reference's special cases when the counts or sums vanish;
+//   exponential migration prior (mean mmean): its own closed form with one
+//     upper incomplete gamma.
+// One thread per point, genealogy chunks across workgroups, rows staged in
+// LDS as in margincalc.  Uniform terms are summed (terms whose log lies
+// outside (-700, 700) are skipped, as the reference does); exponential-prior
+// terms are summed relative to their running maximum (log-sum-exp), the
+// reference's eexp sum to rounding.
+
+struct PArgs {
+  const float *rows;
+  int K, g0;               // rows [g0, g0 + n)
+  int n, cc, fc, hc, mc, fm, qi, mi;  // column offsets
+  double qmax, mmax, mmean;
+  int expo, kind, prob_or_like;
+  double xmax;             // MARGIN: 1.0 outside [0, xmax]
+  double denom;            // rows averaged over (MARGIN: the reference's divisor)
+  const double *x;
+  int nx, nchunks, chunk_rows, stage;
+  const double *lf;        // log(k!) for k < nlf
+  int nlf;
+};
+
+__device__ inline double pm_lga(const PArgs &a, int k) {  // log Gamma(k) = log((k-1)!)
+  return (k >= 1 && k - 1 < a.nlf) ? a.lf[k - 1] : lgamma((double)k);
+}
+__device__ inline double pm_upper(const PArgs &a, int k, double x) {
+  return pri_uppergamma(k, x, pm_lga(a, k));
+}
+__device__ inline double pm_lower(const PArgs &a, int k, double x) {
+  return pri_lowergamma(k, x, pm_lga(a, k));
+}
+
+// log of the term of one genealogy, or flags a skipped term (uniform: the
+// reference adds exp(t) only for -700 < t < 700).
+__device__ inline double popmig_term(const PArgs &a, const float *g, double x, bool *use) {
+  const int cc = (int)g[a.cc], mc = (int)g[a.mc];
+  const double fc = g[a.fc], hc = g[a.hc], fm = g[a.fm], qintg = g[a.qi], mintg = g[a.mi];
+  *use = true;
+  if (!a.expo) {
+    double t1, t2;
+    if (fc == 0 && cc == 0 && fm > 0) {
+      t1 = kLog2 - (mc * log(fm)) - hc - qintg - mintg;
+      t2 = log(exp(pm_upper(a, mc, 2 * fm * x / a.qmax)) - exp(pm_upper(a, mc, a.mmax * fm)));
+    } else if (fm == 0 && mc == 0 && fc > 0) {
+      t1 = kLog2 - (cc * log(fc)) - hc - qintg - mintg;
+      t2 = log(exp(pm_upper(a, cc, 2 * fc / a.qmax)) - exp(pm_upper(a, cc, fc * a.mmax / x)));
+    } else if (fc == 0 && cc == 0 && mc == 0 && fm == 0) {
+      t1 = log(2 * log(a.mmax * a.qmax / (2 * x))) - hc - qintg - mintg;
+      t2 = 0;
+    } else {
+      t1 = kLog2 + (mc * log(x)) - ((cc + mc) * log(fc + fm * x)) - hc - qintg - mintg;
+      double ua = pm_upper(a, cc + mc, 2 * (fc + fm * x) / a.qmax);
+      double ub = pm_upper(a, cc + mc, a.mmax * (fm + fc / x));
+      if (ua == ub) {  // both gammas saturate: the lower ones differ instead
+        ub = pm_lower(a, cc + mc, 2 * (fc + fm * x) / a.qmax);
+        ua = pm_lower(a, cc + mc, a.mmax * (fm + fc / x));
+      }
+      if (ua > ub) {
+        t2 = pri_logdiff(ua, ub);
+      } else {
+        t1 = t2 = 0.0;  // the reference then adds exp(0)
+      }
+    }
+    const double t = t1 + t2;
+    *use = (t < 700) && (t > -700);
+    return t;
+  }
+  const double mmean = a.mmean;
+  const double u1 = x + fc * mmean + fm * mmean * x;
+  double t2 = kLog2 - hc - log(mmean) - qintg - mintg;
+  const double t3 = 2 * u1 / (mmean * a.qmax);
+  if (mc == 0 && cc == 0) {
+    t2 += pm_upper(a, 0, t3);
+  } else if (cc == 0) {
+    t2 += pm_upper(a, mc, t3) - mc * log(fm + 1 / mmean);
+  } else if (mc == 0) {
+    t2 += pm_upper(a, cc, t3) + -cc * log(fc + x * (fm + 1 / mmean));
+  } else {
+    t2 += pm_upper(a, mc + cc, t3) - cc * log(x) + (cc + mc) * log(mmean * x / u1);
+  }
+  return t2;
+}
+
+// work[pt][chunk] = (sum) for uniform priors, (max, sum relative to max)
+// for the exponential prior
+__global__ __launch_bounds__(kLTile) void popmig_partial(PArgs a, double *work) {
+  extern __shared__ double shrows_raw[];
+  float *shrows = reinterpret_cast<float *>(shrows_raw);
+  const int pt = blockIdx.x * kLTile + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const bool active = pt < a.nx;
+  const double x = a.x[active ? pt : 0];
+  const bool inside = !(a.kind == IMA_POPMIG_MARGIN && (x < 0 || x > a.xmax));
+  double s = 0.0, mx = -INFINITY;
+  const int gbeg = chunk * a.chunk_rows;
+  const int gend = min(a.n, gbeg + a.chunk_rows);
+  for (int r0 = gbeg; r0 < gend; r0 += a.stage) {
+    const int cnt = min(a.stage, gend - r0);
+    __syncthreads();
+    const int tot = cnt * a.K;
+    const float *src = a.rows + (size_t)(a.g0 + r0) * a.K;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) shrows[i] = src[i];
+    __syncthreads();
+    if (!active || !inside) continue;
+    for (int r = 0; r < cnt; ++r) {
+      bool use;
+      const double t = popmig_term(a, shrows + (size_t)r * a.K, x, &use);
+      if (!a.expo) {
+        if (use) s += exp(t);
+      } else if (t > mx) {
+        s = s * exp(mx - t) + 1.0;
+        mx = t;
+      } else {
+        s += exp(t - mx);
+      }
+    }
+  }
+  if (!active) return;
+  if (!a.expo) {
+    work[(size_t)pt * a.nchunks + chunk] = s;
+  } else {
+    work[2 * ((size_t)pt * a.nchunks + chunk)] = mx;
+    work[2 * ((size_t)pt * a.nchunks + chunk) + 1] = s;
+  }
+}
+
+__global__ void popmig_final(PArgs a, const double *work, double *out) {
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= a.nx) return;
+  const double x = a.x[pt];
+  if (a.kind == IMA_POPMIG_MARGIN && (x < 0 || x > a.xmax)) {
+    out[pt] = 1.0;  // OFFSCALEVAL
+    return;
+  }
+  double v;
+  if (!a.expo) {
+    double s = 0.0;
+    for (int c = 0; c < a.nchunks; ++c) s += work[(size_t)pt * a.nchunks + c];
+    v = s / a.denom;
+  } else {
+    double mx = -INFINITY;
+    for (int c = 0; c < a.nchunks; ++c) mx = fmax(mx, work[2 * ((size_t)pt * a.nchunks + c)]);
+    double s = 0.0;
+    for (int c = 0; c < a.nchunks; ++c) {
+      const double m = work[2 * ((size_t)pt * a.nchunks + c)];
+      if (m > -INFINITY) s += work[2 * ((size_t)pt * a.nchunks + c) + 1] * exp(m - mx);
+    }
+    v = exp(mx + log(s) - log(a.denom));
+  }
+  if (a.kind == IMA_POPMIG_MARGIN) {
+    out[pt] = -v;
+    return;
+  }
+  if (a.prob_or_like) {
+    const double prior = a.expo ? 2 * exp(pri_uppergamma(0, 2 * x / (a.mmean * a.qmax), 0.0)) /
+                                      (a.qmax * a.mmean)
+                                : 2 * (log(a.qmax) + log(a.mmax) - log(2 * x)) / (a.qmax * a.mmax);
+    v /= prior;
+  }
+  out[pt] = v;
+}
+
+size_t popmig_work_bytes(const LTable &t, int nx) {
+  int nc, cr;
+  chunking(t, nx, &nc, &cr);
+  return (size_t)nx * nc * 2 * sizeof(double) + 64;
+}
+
+hipError_t launch_popmig(const LTable &t, int theta_i, int mig_i, double qmax, double mprior,
+                         int expo, int kind, int prob_or_like, int first, int last,
+                         const double *x, int nx, const double *lf, int nlf, double *work,
+                         double *out, hipStream_t st) {
+  if (nx == 0) return hipSuccess;
+  const JointLayout L = joint_layout(t.npsize, t.nmig);
+  PArgs a{};
+  a.rows = t.rows;
+  a.K = t.K;
+  a.g0 = kind == IMA_POPMIG_MARGIN ? first : 0;
+  a.n = kind == IMA_POPMIG_MARGIN ? last - first : t.G;
+  a.cc = L.ccp + theta_i;
+  a.fc = L.fcp + theta_i;
+  a.hc = L.hccp + theta_i;
+  a.mc = L.mcp + mig_i;
+  a.fm = L.fmp + mig_i;
+  a.qi = L.qip + theta_i;
+  a.mi = L.mip + mig_i;
+  a.qmax = qmax;
+  a.mmax = expo ? 0.0 : mprior;
+  a.mmean = expo ? mprior : 0.0;
+  a.expo = expo;
+  a.kind = kind;
+  a.prob_or_like = prob_or_like;
+  a.xmax = expo ? 20.0 * mprior : qmax * mprior / 2.0;  // EXPOMIGPLOTSCALE x mean
+  a.denom = kind == IMA_POPMIG_MARGIN ? (double)(last - first + (first == 0)) : (double)t.G;
+  a.x = x;
+  a.nx = nx;
+  a.stage = stage_of(t);
+  a.lf = lf;
+  a.nlf = nlf;
+  LTable sub = t;
+  sub.G = a.n;
+  chunking(sub, nx, &a.nchunks, &a.chunk_rows);
+  const dim3 grid((nx + kLTile - 1) / kLTile, a.nchunks);
+  const size_t lds = (size_t)a.stage * t.K * sizeof(float);
+  hipLaunchKernelGGL(popmig_partial, grid, dim3(kLTile), lds, st, a, work);
+  hipLaunchKernelGGL(popmig_final, dim3((nx + 255) / 256), dim3(256), 0, st, a, work, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------- greater-than --
+//
+// P(parameter i > parameter j) per genealogy (gtint.cpp:1-331): closed
+// forms where the reference has them, else the reference's trapezoid
+// refinement (qtrap: relative change below 1e-4, tested from the sixth
+// refinement, at most 20) of the integrand over (MINPARAMVAL, prior max).
+// One lane per genealogy: each runs the reference's sequence of points and
+// sums them in the same order.  A second launch averages the terms of each
+// ordered pair.
+
+struct GArgs {
+  const float *rows;
+  int K, treeinc, ntrees, kind;
+  int ccp, fcp, hccp, mcp, fmp, qip, mip;
+  const int2 *pairs;  // [npairs] (i, j)
+  const double *pmax; // [npairs] prior maximum of the pair
+  const double *lf;
+  int nlf;
+};
+
+struct GtOne {
+  int cci, ccj, wi, wj;
+  double fci, fcj, hval, denom, fmi, fmj;
+};
+
+__device__ inline double gt_lf(const GArgs &a, int k) {  // log(k!)
+  return k < a.nlf ? a.lf[k] : lgamma((double)k + 1.0);
+}
+__device__ inline double gt_upper(const GArgs &a, int k, double x) {
+  return pri_uppergamma(k, x, k >= 1 ? gt_lf(a, k - 1) : 0.0);
+}
+__device__ inline double gt_lower(const GArgs &a, int k, double x) {
+  return pri_lowergamma(k, x, gt_lf(a, k - 1));
+}
+
+constexpr double kGtSwitch = 1e-15;  // SWITCH_TO_LOWERGAMMA_CRIT
+constexpr double kGtMinParam = 1e-7; // MINPARAMVAL
+
+__device__ inline double gt_mgt(const GArgs &a, const GtOne &s, double mi) {  // mgt_wj_gt_0
+  if (mi < kGtMinParam) return 0.0;
+  const double la = gt_lf(a, s.wj), lb = gt_upper(a, s.wj + 1, s.fmj * mi);
+  if (la <= lb) return 0.0;
+  const double t1 = (la - lb) < kGtSwitch ? gt_lower(a, s.wj + 1, s.fmj * mi) : pri_logdiff(la, lb);
+  double t2 = s.wi * log(mi) - s.fmi * mi - (s.wj + 1) * log(s.fmj) + t1;
+  t2 -= s.denom;
+  return exp(t2);
+}
+
+__device__ inline double gt_pgt(const GArgs &a, const GtOne &s, double qi) {  // pgt_fcj_gt_0
+  if (qi < kGtMinParam) return 0.0;
+  const double fcj2 = 2 * s.fcj, fci2 = 2 * s.fci;
+  if (s.ccj == 0) {
+    const double la = log(qi) - fcj2 / qi;
+    const double lb = log(fcj2) + gt_upper(a, 0, fcj2 / qi);
+    if (la > lb) {
+      const double t1 = pri_logdiff(la, lb);
+      const double t2 = -fci2 / qi + s.cci * log(2 / qi);
+      return exp(t1 + t2 - s.hval - s.denom);
+    }
+    return 0.0;
+  }
+  const double t1 = gt_upper(a, s.ccj - 1, fcj2 / qi);
+  const double t2 = kLog2 + s.cci * log(2 / qi) + (1 - s.ccj) * log(s.fcj) - fci2 / qi;
+  return exp(t2 + t1 - s.hval - s.denom);
+}
+
+__device__ inline double gt_f(const GArgs &a, const GtOne &s, bool mig, double x) {
+  return mig ? gt_mgt(a, s, x) : gt_pgt(a, s, x);
+}
+
+__device__ double gt_qtrap(const GArgs &a, const GtOne &s, bool mig, double lo, double hi) {
+  double st = 0.0, olds = -1.0e100;
+  for (int j = 1; j <= 20; ++j) {
+    if (j == 1) {
+      st = 0.5 * (hi - lo) * (gt_f(a, s, mig, lo) + gt_f(a, s, mig, hi));
+    } else {
+      const int it = 1 << (j - 2);
+      const double tnm = it, del = (hi - lo) / tnm;
+      double x = lo + 0.5 * del, sum = 0.0;
+      for (int k = 1; k <= it; ++k, x += del) sum += gt_f(a, s, mig, x);
+      st = 0.5 * (st + (hi - lo) * sum / tnm);
+    }
+    if (j > 5 && (fabs(st - olds) < 1.0e-4 * fabs(olds) || (st == 0.0 && olds == 0.0)))
+      return st;
+    olds = st;
+  }
+  return st;
+}
+
+__device__ double gtmig_one(const GArgs &a, const float *g, int mi, int mj, double mmax) {
+  GtOne s{};
+  s.fmi = g[a.fmp + mi];
+  s.fmj = g[a.fmp + mj];
+  s.wi = (int)g[a.mcp + mi];
+  s.wj = (int)g[a.mcp + mj];
+  s.denom = (double)g[a.mip + mi] + (double)g[a.mip + mj];
+  double temp;
+  if (s.wj == 0) {
+    if (s.fmj > 0.0) {
+      if (s.wi > 0) {
+        const double la = gt_lf(a, s.wi);
+        const double lb = gt_upper(a, s.wi + 1, (s.fmi + s.fmj) * mmax);
+        const double lc = gt_upper(a, s.wi + 1, s.fmi * mmax);
+        if (la <= lb || la <= lc) {
+          temp = 0.0;
+        } else {
+          double t1 = (la - lb) < kGtSwitch ? gt_lower(a, s.wi + 1, (s.fmi + s.fmj) * mmax)
+                                             : pri_logdiff(la, lb);
+          t1 += -(s.wi + 1) * log(s.fmi + s.fmj);
+          double t2 = (la - lc) < kGtSwitch ? gt_lower(a, s.wi + 1, s.fmi * mmax)
+                                             : pri_logdiff(la, lc);
+          t2 += -(s.wi + 1) * log(s.fmi);
+          temp = t2 <= t1 ? 0.0 : exp(pri_logdiff(t2, t1) - log(s.fmj) - s.denom);
+        }
+      } else if (s.fmi > 0.0) {
+        const double t1 = s.fmi * (exp(-(s.fmi + s.fmj) * mmax) - exp(-s.fmi * mmax));
+        const double t2 = s.fmj * (1 - exp(-s.fmi * mmax));
+        const double t3 = (t1 + t2) / (s.fmi * s.fmj * (s.fmi + s.fmj));
+        temp = exp(log(t3) - s.denom);
+      } else {
+        const double t1 = (s.fmj * mmax - 1.0 + exp(-s.fmj * mmax)) / (s.fmj * s.fmj);
+        temp = exp(log(t1) - s.denom);
+      }
+    } else {
+      if (s.wi > 0) {
+        const double la = gt_lf(a, s.wi + 1), lb = gt_upper(a, s.wi + 2, s.fmi * mmax);
+        if (la <= lb) {
+          temp = 0.0;
+        } else {
+          const double t1 = (la - lb) < kGtSwitch ? gt_lower(a, s.wi + 2, s.fmi * mmax)
+                                                   : pri_logdiff(la, lb);
+          temp = exp(-(s.wi + 2) * log(s.fmi) + t1 - s.denom);
+        }
+      } else if (s.fmi > 0.0) {
+        const double t1 = (1.0 - exp(-s.fmi * mmax) * (s.fmi * mmax + 1.0)) / (s.fmi * s.fmi);
+        temp = exp(log(t1) - s.denom);
+      } else {
+        temp = exp(log(mmax * mmax / 2.0) - s.denom);
+      }
+    }
+  } else {
+    temp = gt_qtrap(a, s, true, kGtMinParam, mmax);
+  }
+  return fmin(1.0, temp);
+}
+
+__device__ double gtpops_one(const GArgs &a, const float *g, int pi, int pj, double qmax) {
+  GtOne s{};
+  s.cci = (int)g[a.ccp + pi];
+  s.ccj = (int)g[a.ccp + pj];
+  s.fci = g[a.fcp + pi];
+  s.fcj = g[a.fcp + pj];
+  s.hval = (double)g[a.hccp + pi] + (double)g[a.hccp + pj];
+  s.denom = (double)g[a.qip + pi] + (double)g[a.qip + pj];
+  double temp;
+  if (s.ccj == 0 && s.fcj == 0) {
+    if (s.fci == 0) {
+      temp = exp(2.0 * log(qmax) - kLog2 - s.hval - s.denom);
+    } else if (s.cci >= 2) {
+      const double t1 = 2 * kLog2 + (2 - s.cci) * log(s.fci);
+      const double t2 = gt_upper(a, s.cci - 2, 2 * s.fci / qmax);
+      temp = exp(t1 + t2 - s.hval - s.denom);
+    } else if (s.cci == 1) {
+      const double t1 = 4 * s.fci * exp(gt_upper(a, 0, 2 * s.fci / qmax));
+      const double t2 = 2 * qmax * exp(-2 * s.fci / qmax) - t1;
+      temp = exp(log(t2) - s.hval - s.denom);
+    } else {
+      const double t1 = exp(gt_upper(a, 0, 2 * s.fci / qmax));
+      const double t2 = (qmax / 2) * (qmax - 2 * s.fci) * exp(-2 * s.fci / qmax) +
+                        2 * s.fci * s.fci * t1;
+      temp = exp(log(t2) - s.hval - s.denom);
+    }
+  } else {
+    temp = gt_qtrap(a, s, false, kGtMinParam, qmax);
+  }
+  return fmin(1.0, temp);
+}
+
+// per[pair][tree]
+__global__ __launch_bounds__(64) void gt_terms(GArgs a, double *per) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (t >= a.ntrees) return;
+  const float *g = a.rows + (size_t)t * a.treeinc * a.K;
+  const int2 ij = a.pairs[p];
+  per[(size_t)p * a.ntrees + t] =
+      a.kind ? gtmig_one(a, g, ij.x, ij.y, a.pmax[p]) : gtpops_one(a, g, ij.x, ij.y, a.pmax[p]);
+}
+
+// one wavefront per pair: lane-strided partial sums in a fixed order
+__global__ __launch_bounds__(64) void gt_mean(GArgs a, const double *per, double *out) {
+  const int p = blockIdx.x, lane = threadIdx.x;
+  double s = 0.0;
+  for (int t = lane; t < a.ntrees; t += 64) s += per[(size_t)p * a.ntrees + t];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[p] = s / a.ntrees;
+}
+
+hipError_t launch_greater_than(const LTable &t, int kind, const int2 *pairs, const double *pmax,
+                               int npairs, const double *lf, int nlf, double *per, double *out,
+                               hipStream_t st) {
+  if (npairs == 0) return hipSuccess;
+  const JointLayout L = joint_layout(t.npsize, t.nmig);
+  GArgs a{};
+  a.rows = t.rows;
+  a.K = t.K;
+  // print_greater_than_tests (gtint.cpp:337-349): at most USETREESMAX = 20000
+  if (t.G > 20000) {
+    a.treeinc = t.G / 20000;
+    a.ntrees = 20000;
+  } else {
+    a.treeinc = 1;
+    a.ntrees = t.G;
+  }
+  a.kind = kind;
+  a.ccp = L.ccp;
+  a.fcp = L.fcp;
+  a.hccp = L.hccp;
+  a.mcp = L.mcp;
+  a.fmp = L.fmp;
+  a.qip = L.qip;
+  a.mip = L.mip;
+  a.pairs = pairs;
+  a.pmax = pmax;
+  a.lf = lf;
+  a.nlf = nlf;
+  hipLaunchKernelGGL(gt_terms, dim3((a.ntrees + 63) / 64, npairs), dim3(64), 0, st, a, per);
+  hipLaunchKernelGGL(gt_mean, dim3(npairs), dim3(64), 0, st, a, per, out);
   return hipGetLastError();
 }
 

@@ -350,3 +350,85 @@ extern "C" int ima_lmode_marginal_peak(ima_ctx *ctx, int table_id, int param, do
   }
   return IMA_OK;
 }
+
+// ---- peak of the 2NM density (popmig.cpp:355-429) ----------------------
+//
+// marginalopt_popmig for one (population size, migration rate) pair: the
+// curve often has several peaks, so the reference scans 100 bins for the
+// lowest -density (the first of equal values), brackets around it, and
+// refines by golden section (goldenmod, surface_search_functions.cpp:
+// 134-184, r = 0.61803399, tolerance 1e-7).  The 100 bins are one launch;
+// each golden-section step evaluates one point.
+
+extern "C" int ima_lmode_popmig_peak(ima_ctx *ctx, int table_id, int theta_i, int mig_i,
+                                     double qmax, double mprior, int expo, int first, int last,
+                                     double *peak, double *mlval, int64_t *evaluations) {
+  if (!ctx || !peak) return IMA_ERR_ARG;
+  constexpr int kBins = 100;          // POPMIG_MARGINBINS
+  constexpr double kMinParam = 1e-7;  // MINPARAMVAL
+  constexpr double r = 0.61803399, cg = 1.0 - r;
+  const double upper = expo ? 20.0 * mprior : qmax * mprior / 2.0;  // EXPOMIGPLOTSCALE
+  int64_t evals = 0;
+  auto f = [&](const double *x, int n, double *out) {
+    evals += n;
+    return ima_lmode_popmig(ctx, table_id, theta_i, mig_i, qmax, mprior, expo, IMA_POPMIG_MARGIN,
+                            0, first, last, x, n, out);
+  };
+  std::vector<double> bx(kBins), fb(kBins);
+  for (int j = 0; j < kBins; ++j) bx[j] = kMinParam + j * (upper / (double)kBins);
+  int rc = f(bx.data(), kBins, fb.data());
+  if (rc != IMA_OK) return rc;
+  double maxf = 1e100;
+  int maxj = -1;
+  for (int j = 0; j < kBins; ++j)
+    if (fb[j] < maxf) {
+      maxf = fb[j];
+      maxj = j;
+    }
+  double ax, bxm, cx;
+  const double w = upper / (double)kBins;
+  if (maxj == 0) {
+    ax = kMinParam;
+    cx = kMinParam + w;
+    bxm = (ax + cx) / 2.0;
+  } else if (maxj == kBins - 1) {
+    ax = kMinParam + ((double)kBins - 1) * w;
+    bxm = kMinParam + ((double)kBins - 2) * w;
+    cx = upper;
+  } else {
+    bxm = kMinParam + ((double)maxj - 1) * w;
+    cx = kMinParam + ((double)maxj + 1) * w;
+    ax = kMinParam + ((double)maxj) * w;
+  }
+  // goldenmod
+  double x0 = ax, x3 = cx, x1, x2, f1, f2;
+  if (std::fabs(cx - bxm) > std::fabs(bxm - ax)) {
+    x1 = bxm;
+    x2 = bxm + cg * (cx - bxm);
+  } else {
+    x2 = bxm;
+    x1 = bxm - cg * (bxm - ax);
+  }
+  if ((rc = f(&x1, 1, &f1)) != IMA_OK) return rc;
+  if ((rc = f(&x2, 1, &f2)) != IMA_OK) return rc;
+  while (std::fabs(x3 - x0) > 1e-7 * (std::fabs(x1) + std::fabs(x2)) && evals < kBins + 2000) {
+    if (f2 < f1) {
+      x0 = x1;
+      x1 = x2;
+      x2 = r * x1 + cg * x3;
+      f1 = f2;
+      if ((rc = f(&x2, 1, &f2)) != IMA_OK) return rc;
+      continue;
+    }
+    x3 = x2;
+    x2 = x1;
+    x1 = r * x2 + cg * x0;
+    f2 = f1;
+    if ((rc = f(&x1, 1, &f1)) != IMA_OK) return rc;
+  }
+  const double res = f1 < f2 ? f1 : f2;
+  *peak = f1 < f2 ? x1 : x2;
+  if (mlval) *mlval = -res;
+  if (evaluations) *evaluations = evals;
+  return IMA_OK;
+}

@@ -320,6 +320,9 @@ class Engine:
                   "lmode_table")
         return out.value
 
+    def lmode_table_free(self, tid):
+        self._chk(self.lib.ima_lmode_table_free(self.h, int(tid)), "lmode_table_free")
+
     def lmode_table_reserve(self, capacity, npsize, nmig, npops) -> int:
         """An empty resident table the device sweep appends rows to."""
         out = C.c_int()
@@ -402,6 +405,43 @@ class Engine:
                                                 len(x), int(logi), float(yadjust), int(expo),
                                                 float(mean), out.ctypes.data_as(L.dp)),
                   "margincalc")
+        return out
+
+    def popmig(self, tid, theta_i, mig_i, x, qmax, mprior, expo=False, prob_or_like=0,
+               kind=L.POPMIG_CALC, first=0, last=0):
+        """2NM density at x (ima_lmode_popmig): calc_popmig / calc_pop_expomig,
+        or with kind=POPMIG_MARGIN marginpopmig over rows [first, last)."""
+        x = np.ascontiguousarray(np.atleast_1d(x), np.float64)
+        out = np.zeros(len(x))
+        self._chk(self.lib.ima_lmode_popmig(self.h, int(tid), int(theta_i), int(mig_i),
+                                            float(qmax), float(mprior), int(expo), int(kind),
+                                            int(prob_or_like), int(first), int(last),
+                                            x.ctypes.data_as(L.dp), len(x),
+                                            out.ctypes.data_as(L.dp)), "popmig")
+        return out
+
+    def popmig_peak(self, tid, theta_i, mig_i, qmax, mprior, expo=False, first=0, last=None):
+        """(peak, density there, points evaluated) of the 2NM density
+        (ima_lmode_popmig_peak, marginalopt_popmig for one pair)."""
+        if last is None:
+            last = self.lmode_table_rows(tid)[0]
+        pk, ml, ev = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.lib.ima_lmode_popmig_peak(self.h, int(tid), int(theta_i), int(mig_i),
+                                                 float(qmax), float(mprior), int(expo),
+                                                 int(first), int(last), C.byref(pk),
+                                                 C.byref(ml), C.byref(ev)), "popmig_peak")
+        return pk.value, ml.value, ev.value
+
+    def greater_than(self, tid, kind, prior_max):
+        """P(row parameter > column parameter) for the population sizes
+        (kind 0) or migration rates (kind 1) with the given prior maxima
+        (ima_lmode_greater_than); -1 where the reference prints "na"."""
+        pm = np.ascontiguousarray(prior_max, np.float64)
+        n = len(pm)
+        out = np.zeros((n, n))
+        self._chk(self.lib.ima_lmode_greater_than(self.h, int(tid), int(kind), n,
+                                                  pm.ctypes.data_as(L.dp),
+                                                  out.ctypes.data_as(L.dp)), "greater_than")
         return out
 
     # --------------------------------------------------------------- MC3 --
@@ -493,6 +533,92 @@ class Engine:
         self._chk(self.lib.ima_mc3_read_prior(self.h, probg.ctypes.data_as(L.dp),
                                               tv.ctypes.data_as(L.dp)), "mc3_read_prior")
         return probg, tv[:, :P - 1]
+
+    def read_mig_events(self, slot):
+        """(edge, time, to) arrays of one slot's migration events."""
+        cnt = C.c_int32()
+        self._chk(self.lib.ima_slots_read_mig_events(self.h, slot, C.byref(cnt), None, None,
+                                                     None), "read_mig_events")
+        m = cnt.value
+        e, t, to = np.zeros(max(m, 1), np.int32), np.zeros(max(m, 1)), np.zeros(max(m, 1),
+                                                                             np.int32)
+        if m:
+            self._chk(self.lib.ima_slots_read_mig_events(
+                self.h, slot, C.byref(cnt), e.ctypes.data_as(L.i32p), t.ctypes.data_as(L.dp),
+                to.ctypes.data_as(L.i32p)), "read_mig_events")
+        return e[:m], t[:m], to[:m]
+
+    def mcf_save(self, path, first, nchains, nloci, layout, pi, u=None, kappa=None,
+                 tvals=None):
+        """Write the chains' genealogies (slots first.., chain after chain)
+        and rates / split times to a .mcf file (ima_mcf_write, writemcf).
+        layout: [(model, n, S)] per locus; pi [nloci, 4] base frequencies.
+        u / kappa / tvals default to the MC3 state's."""
+        from . import mcffile
+        N = nchains * nloci
+        up0, up1, dn, tm, rt, A = self.read_slots(first, N)
+        pop = self.read_pops(first, N)
+        counts, mt, mo = [], [], []
+        for i in range(N):
+            n = layout[i % nloci][1]
+            e, t, to = self.read_mig_events(first + i)
+            c = np.bincount(e, minlength=2 * n - 1) if len(e) else np.zeros(2 * n - 1, int)
+            counts.append(c)
+            order = np.lexsort((t, e))
+            mt.append(t[order])
+            mo.append(to[order])
+        if u is None or kappa is None:
+            ru, rk = self.mc3_read_rates()
+            u = ru if u is None else u
+            kappa = rk if kappa is None else kappa
+        if tvals is None:
+            tvals = self.mc3_read_prior()[1]
+        tvals = np.asarray(tvals, float).reshape(nchains, -1)
+        state = {"tvals": tvals, "u": np.asarray(u, float).reshape(nchains, nloci),
+                 "kappa": np.asarray(kappa, float).reshape(nchains, nloci),
+                 "pi": np.tile(np.asarray(pi, float).reshape(1, nloci, 4), (nchains, 1, 1)),
+                 "up0": up0, "up1": up1, "down": dn, "time": tm, "pop": pop, "A": A,
+                 "mig_count": np.concatenate(counts).astype(np.int32),
+                 "mig_time": np.concatenate(mt) if mt else np.zeros(0),
+                 "mig_to": np.concatenate(mo).astype(np.int32) if mo else np.zeros(0, np.int32)}
+        mcffile.write(path, nchains, tvals.shape[1], layout, state)
+
+    def mcf_load(self, path, first, nchains, nloci, layout, tmax=None):
+        """Read a .mcf file (ima_mcf_read, readmcf) into slots first..
+        (genealogies, populations, migrations); returns the state dict
+        (tvals, u, kappa, pi, ...) for mc3_setup / mc3_set_prior, which
+        re-score everything from the genealogies as readmcf's init_p does."""
+        from . import mcffile
+        from .sim import Genealogy
+        nsplit = 0 if tmax is None else len(tmax)
+        st = mcffile.read(path, nchains, nsplit, layout, tmax=tmax) if tmax is not None else \
+            None
+        if st is None:
+            raise ValueError("mcf_load needs tmax (one bound per split time)")
+        N = nchains * nloci
+        o = 0
+        gens, A, moff = [], [], 0
+        for i in range(N):
+            n = layout[i % nloci][1]
+            E = 2 * n - 1
+            sl = slice(o, o + E)
+            dn = st["down"][sl]
+            g = Genealogy(n=n, up0=st["up0"][sl].copy(), up1=st["up1"][sl].copy(), down=dn.copy(),
+                          time=st["time"][sl].copy(), root=int(np.flatnonzero(dn == -1)[0]),
+                          pop=st["pop"][sl].copy())
+            cnt = st["mig_count"][sl]
+            mig = [[] for _ in range(E)]
+            for e in range(E):
+                for _ in range(int(cnt[e])):
+                    mig[e].append((float(st["mig_time"][moff]), int(st["mig_to"][moff])))
+                    moff += 1
+            g.mig = mig
+            gens.append(g)
+            A.append(st["A"][sl].copy())
+            o += E
+        self.write_slots(first, gens, A=A)
+        self.write_pops(first, gens)
+        return st
 
     def read_mig_times(self, slot, nmig):
         out = np.zeros(max(nmig, 1))

@@ -51,6 +51,8 @@ enum ima_status {
   IMA_STATUS_PRIOR_BAD = 4  /* population labels / migrations do not fit the model */
 };
 enum ima_lmode { IMA_LMODE_REF_COMPAT = 0, IMA_LMODE_EXACT = 1 };
+/* what ima_lmode_popmig evaluates */
+enum ima_popmig_kind { IMA_POPMIG_CALC = 0, IMA_POPMIG_MARGIN = 1 };
 
 #define IMA_REJECT_IS (-1000000000.0) /* REJECTINFINITESITESCONSTANT imamp.hpp:157 */
 #define IMA_MAX_GENES 256             /* gene copies per locus in this engine */
@@ -274,6 +276,48 @@ int ima_lmode_joint_peak(ima_ctx *ctx, int table_id, int modeltype, const double
  * prior_mean as for ima_lmode_margincalc. */
 int ima_lmode_marginal_peak(ima_ctx *ctx, int table_id, int param, double lower, double upper,
                             int expo, double prior_mean, double *out);
+/* The posterior density of the population migration rate 2NM = theta_i *
+ * m_i / 2 at nx points x (popmig.cpp:9-351), replacing calc_popmig /
+ * calc_pop_expomig and marginpopmig / marginpop_expomig: per genealogy the
+ * closed-form integral of the coalescent and migration likelihoods along
+ * theta m / 2 = x over theta_i's uniform prior (0, qmax) and m_i's prior
+ * (uniform on (0, mprior), or exponential with mean mprior when expo != 0),
+ * averaged over the table's genealogies.
+ *   IMA_POPMIG_CALC: calc_popmig / calc_pop_expomig over the whole table
+ *     (first, last ignored), divided by the prior density of 2NM when
+ *     prob_or_like != 0 (x > 0);
+ *   IMA_POPMIG_MARGIN: marginpopmig / marginpop_expomig: minus the sum over
+ *     genealogies [first, last) divided by last - first + (first == 0), as
+ *     the reference divides, and 1.0 for x outside [0, qmax mprior / 2]
+ *     (uniform) or [0, 20 mprior] (exponential).
+ * The genealogy columns read are theta_i's cc/fc/hcc/qintegrate and m_i's
+ * mc/fm/mintegrate.  Integer counts are truncated from the float table as
+ * the reference casts them. */
+int ima_lmode_popmig(ima_ctx *ctx, int table_id, int theta_i, int mig_i, double qmax,
+                     double mprior, int expo, int kind, int prob_or_like, int first, int last,
+                     const double *x, int nx, double *out);
+/* The peak of the 2NM density for one (theta_i, m_i) pair over genealogies
+ * [first, last) (marginalopt_popmig, popmig.cpp:361-429): the
+ * IMA_POPMIG_MARGIN value at 100 bins MINPARAMVAL + j upper / 100 (one
+ * launch) brackets the minimum, golden section refines it (goldenmod,
+ * surface_search_functions.cpp:134-184, tolerance 1e-7).  *peak is the
+ * location, *mlval the density there (minus the minimum); evaluations
+ * (may be NULL) counts the points evaluated. */
+int ima_lmode_popmig_peak(ima_ctx *ctx, int table_id, int theta_i, int mig_i, double qmax,
+                          double mprior, int expo, int first, int last, double *peak,
+                          double *mlval, int64_t *evaluations);
+/* The parameter comparisons of print_greater_than_tests (gtint.cpp:
+ * 333-445): out[i * n + j] = P(parameter i > parameter j) for the n
+ * population sizes (kind 0, gtpops) or migration rates (kind 1, gtmig) of
+ * the table, averaged over its genealogies (every (G / 20000)-th of the
+ * first (G / 20000) * 20000 when G > 20000, USETREESMAX).  prior_max[k] is
+ * parameter k's prior upper bound; pairs whose bounds differ, the diagonal
+ * and (kind 1) rates with a bound <= MINPARAMVAL get -1, which the
+ * reference prints as "na".  Per genealogy the reference's closed forms or
+ * its trapezoid refinement (qtrap, relative change below 1e-4) are used,
+ * capped at 1. */
+int ima_lmode_greater_than(ima_ctx *ctx, int table_id, int kind, int n, const double *prior_max,
+                           double *out);
 /* margincalc for parameter `param` at nx values x[] (logi, yadjust,
  * exponential migration prior with mean prior_mean when expo != 0). */
 int ima_lmode_margincalc(ima_ctx *ctx, int table_id, int param,
@@ -299,6 +343,43 @@ int ima_ti_count(const char *path, int *G);
  * the counts. */
 int ima_ti_load(const char *const *paths, int npaths, int K, int max_rows, float *rows,
                 int *loaded, int *total);
+
+/* ---- MCMC state files (.mcf), host only -------------------------------- */
+/* One locus of a .mcf file: its model, gene copies (2n - 1 edges) and, for
+ * HKY, its site patterns (the conditional-likelihood lines of every
+ * internal edge). */
+typedef struct {
+  int32_t model;
+  int32_t n;
+  int32_t nsites;
+} ima_mcf_locus;
+/* The chains' state, packed chain after chain, and within a chain locus
+ * after locus: tvals [chain][nsplit]; u, kappa [chain][locus] (kappa used
+ * for HKY loci, may be NULL); pi [chain][locus][4]; per edge
+ * [chain][locus][2n - 1]: up0, up1, down, pop, time, A and dlikeA
+ * (stepwise loci; may be NULL), mut (may be NULL); mig_count per edge and,
+ * in the same order, each edge's migration events in time order in
+ * mig_time / mig_to (mig_capacity entries when reading). */
+typedef struct {
+  double *tvals, *u, *kappa, *pi, *time, *dlikeA, *mig_time;
+  int32_t *up0, *up1, *down, *mut, *pop, *A, *mig_count, *mig_to;
+  int64_t mig_capacity;
+} ima_mcf_state;
+/* writemcf (mcmcfile.cpp:204-292): the reference's text layout, item by
+ * item as awrite prints it ("<name> <type> <count> " and the values, "%d "
+ * or "%.10lg ", one item a line).  The conditional likelihoods of HKY
+ * internal edges are written as zeros (readmcf's init_p recomputes them). */
+int ima_mcf_write(const char *path, int nchains, int nsplit, int nloci,
+                  const ima_mcf_locus *loci, const ima_mcf_state *st);
+/* readmcf (mcmcfile.cpp:310-442): every item's name and type checked
+ * (IMA_ERR_ARG on a mismatch); a file that ends before nchains chains is
+ * read again from its start; with tmax (nsplit bounds, may be NULL) a
+ * chain with a split time above its bound is read over by the next one.
+ * More migration events than mig_capacity is IMA_ERR_SIZE.  *nmig (may
+ * be NULL) receives the events stored. */
+int ima_mcf_read(const char *path, int nchains, int nsplit, int nloci,
+                 const ima_mcf_locus *loci, const double *tmax, ima_mcf_state *st,
+                 int64_t *nmig);
 
 /* ---- genealogy prior (update_gtree_common.cpp:1680-2053) ----------------- */
 /* Population model, flattened (the reference's poptree and parameter maps,
@@ -501,6 +582,11 @@ int ima_mc3_read_prior(ima_ctx *ctx, double *probg, double *tvals);
 /* Migration times of one slot as the device holds them (the rescaling
  * moves them); mig_time has that slot's migration count entries. */
 int ima_slots_read_mig(ima_ctx *ctx, int slot, double *mig_time);
+/* The migration events of one slot as the device holds them: *count, and
+ * (any may be NULL) each event's edge, time and entered population, sorted
+ * by edge then time -- what a .mcf file records per edge. */
+int ima_slots_read_mig_events(ima_ctx *ctx, int slot, int32_t *count, int32_t *mig_edge,
+                              double *mig_time, int32_t *mig_to);
 
 /* ---- serving ----------------------------------------------------------- */
 /* A request loop for independent IM analyses and L-mode queries arriving

@@ -75,6 +75,23 @@ def _load():
     lib.ora_margincalc.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_int, C.c_double]
+    lib.ora_popmig.restype = C.c_double
+    lib.ora_popmig.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_double, C.c_int, C.c_int, C.c_double]
+    lib.ora_marginpopmig.restype = C.c_double
+    lib.ora_marginpopmig.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                     C.c_double]
+    lib.ora_popmig_peak.restype = C.c_double
+    lib.ora_popmig_peak.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp]
+    lib.ora_gtpops_one.restype = C.c_double
+    lib.ora_gtpops_one.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+    lib.ora_gtmig_one.restype = C.c_double
+    lib.ora_gtmig_one.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+    lib.ora_greater_than.restype = C.c_double
+    lib.ora_greater_than.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_double]
     lib.ora_mc3_bench.restype = C.c_long
     lib.ora_mc3_bench.argtypes = ([C.c_int, C.c_int] + [ip] * 7 + [dp, dp, dp, ip, C.c_int]
                                   + [ip] * 5 + [C.c_int, ip, ip, C.c_int, ip, ip, dp, dp,
@@ -295,6 +312,54 @@ def margincalc(table, npsize, nmig, x, pi, logi=1, yadjust=0.0, expo=False,
     return lib.ora_margincalc(table.ctypes.data_as(C.POINTER(C.c_float)), G, K,
                               npsize, nmig, float(x), float(yadjust), int(pi),
                               int(logi), int(expo), float(mean))
+
+
+def popmig(table, npsize, nmig, thetai, mi, x, qmax, mprior, expo=False, prob_or_like=0):
+    """calc_popmig / calc_pop_expomig (popmig.cpp:9-172) at one x."""
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    return lib.ora_popmig(table.ctypes.data_as(C.POINTER(C.c_float)), G, K, npsize, nmig,
+                          thetai, mi, float(qmax), float(mprior), int(expo), int(prob_or_like),
+                          float(x))
+
+
+def marginpopmig(table, npsize, nmig, thetai, mi, x, qmax, mprior, expo=False, first=0,
+                 last=None):
+    """marginpopmig / marginpop_expomig (popmig.cpp:176-351) over rows [first, last)."""
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    last = G if last is None else last
+    return lib.ora_marginpopmig(table.ctypes.data_as(C.POINTER(C.c_float)), K, npsize, nmig,
+                                thetai, mi, float(qmax), float(mprior), int(expo), int(first),
+                                int(last), float(x))
+
+
+def popmig_peak(table, npsize, nmig, thetai, mi, qmax, mprior, expo=False, first=0, last=None):
+    """marginalopt_popmig (popmig.cpp:361-429) for one pair: (peak, density there)."""
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    last = G if last is None else last
+    pk = np.zeros(1)
+    v = lib.ora_popmig_peak(table.ctypes.data_as(C.POINTER(C.c_float)), K, npsize, nmig,
+                            thetai, mi, float(qmax), float(mprior), int(expo), int(first),
+                            int(last), _dp(pk))
+    return float(pk[0]), v
+
+
+def greater_than_one(row, npsize, nmig, kind, i, j, prior_max):
+    """One genealogy's term of gtpops (kind 0) or gtmig (kind 1), gtint.cpp:127-331."""
+    row = np.ascontiguousarray(row, np.float32)
+    f = lib.ora_gtmig_one if kind else lib.ora_gtpops_one
+    return f(row.ctypes.data_as(C.POINTER(C.c_float)), npsize, nmig, i, j, float(prior_max))
+
+
+def greater_than(table, npsize, nmig, kind, i, j, prior_max):
+    """P(param i > param j) over the table with print_greater_than_tests's
+    subsampling (gtint.cpp:335-349)."""
+    table = np.ascontiguousarray(table, np.float32)
+    G, K = table.shape
+    return lib.ora_greater_than(table.ctypes.data_as(C.POINTER(C.c_float)), G, K, npsize, nmig,
+                                int(kind), i, j, float(prior_max))
 
 
 lib.ora_swapchains.restype = C.c_int

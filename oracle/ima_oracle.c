@@ -1121,3 +1121,400 @@ int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_e
   free(ev);
   return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * 2NM density -- popmig.cpp:9-351 (calc_popmig, calc_pop_expomig,
+ * marginpopmig, marginpop_expomig) and marginalopt_popmig (:361-429) with
+ * goldenmod (surface_search_functions.cpp:134-184).
+ */
+typedef struct {
+  int ccp, fcp, hcp, mcp, fmp, qip, mip;
+} pm_cols;
+
+static pm_cols pm_columns(int npsize, int nmig, int thetai, int mi) {
+  int ccp = 0, fcp = ccp + npsize, hccp = fcp + npsize, mcp = hccp + npsize;
+  int fmp = mcp + nmig, qip = fmp + nmig, mip = qip + npsize;
+  pm_cols c = {ccp + thetai, fcp + thetai, hccp + thetai, mcp + mi, fmp + mi, qip + thetai,
+               mip + mi};
+  return c;
+}
+
+/* sum over rows [first, last) of the uniform-prior terms (popmig.cpp:27-85) */
+static double pm_sum_uniform(const float *table, int K, pm_cols c, int first, int last,
+                             double x, double qmax, double mmax) {
+  double sum = 0;
+  for (int ei = first; ei < last; ei++) {
+    const float *g = table + (size_t)ei * K;
+    int cc = (int)g[c.ccp];
+    double fc = g[c.fcp], hc = g[c.hcp];
+    int mc = (int)g[c.mcp];
+    double fm = g[c.fmp], qintg = g[c.qip], mintg = g[c.mip], temp1, temp2;
+    if (fc == 0 && cc == 0 && fm > 0) {
+      temp1 = ORA_LOG2 - (mc * log(fm)) - hc - qintg - mintg;
+      temp2 = log(exp(ora_uppergamma(mc, 2 * fm * x / qmax)) -
+                  exp(ora_uppergamma(mc, mmax * fm)));
+    } else if (fm == 0 && mc == 0 && fc > 0) {
+      temp1 = ORA_LOG2 - (cc * log(fc)) - hc - qintg - mintg;
+      temp2 = log(exp(ora_uppergamma(cc, 2 * fc / qmax)) -
+                  exp(ora_uppergamma(cc, fc * mmax / x)));
+    } else if (fc == 0 && cc == 0 && mc == 0 && fm == 0) {
+      temp1 = log(2 * log(mmax * qmax / (2 * x))) - hc - qintg - mintg;
+      temp2 = 0;
+    } else {
+      temp1 = ORA_LOG2 + (mc * log(x)) - ((cc + mc) * log(fc + fm * x)) - hc - qintg - mintg;
+      double a = ora_uppergamma(cc + mc, 2 * (fc + fm * x) / qmax);
+      double b = ora_uppergamma(cc + mc, mmax * (fm + fc / x));
+      if (a == b) {
+        b = ora_lowergamma(cc + mc, 2 * (fc + fm * x) / qmax);
+        a = ora_lowergamma(cc + mc, mmax * (fm + fc / x));
+      }
+      if (a > b)
+        temp2 = log_diff(a, b);
+      else
+        temp1 = temp2 = 0.0;
+    }
+    if ((temp1 + temp2 < 700) && (temp1 + temp2 > -700)) sum += exp(temp1 + temp2);
+  }
+  return sum;
+}
+
+/* log-sum of the exponential-prior terms over rows [first, last) as the
+ * reference sums them: eexp per term, scaled to the largest decade less
+ * OCUTOFF = 10 (popmig.cpp:105-164) */
+static double pm_logsum_expo(const float *table, int K, pm_cols c, int first, int last,
+                             double x, double qmax, double mmean) {
+  int n = last - first, maxz = -1000000000;
+  double *m = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  int *z = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int ei = first; ei < last; ei++) {
+    const float *g = table + (size_t)ei * K;
+    int cc = (int)g[c.ccp];
+    double fc = g[c.fcp], hc = g[c.hcp];
+    int mc = (int)g[c.mcp];
+    double fm = g[c.fmp], qintg = g[c.qip], mintg = g[c.mip];
+    double temp1 = x + fc * mmean + fm * mmean * x;
+    double temp2 = ORA_LOG2 - hc - log(mmean) - qintg - mintg;
+    double temp3 = 2 * temp1 / (mmean * qmax);
+    if (mc == 0 && cc == 0) {
+      temp2 += ora_uppergamma(0, temp3);
+    } else if (cc == 0) {
+      temp2 += ora_uppergamma(mc, temp3) - mc * log(fm + 1 / mmean);
+    } else if (mc == 0) {
+      temp2 += ora_uppergamma(cc, temp3) + -cc * log(fc + x * (fm + 1 / mmean));
+    } else {
+      temp2 += ora_uppergamma(mc + cc, temp3) - cc * log(x) +
+               (cc + mc) * log(mmean * x / temp1);
+    }
+    ora_eexp(temp2, &m[ei - first], &z[ei - first]);
+    if (z[ei - first] > maxz) maxz = z[ei - first];
+  }
+  double acumm = 0;
+  for (int i = 0; i < n; i++) {
+    int zadj = z[i] - (maxz - 10);
+    acumm += m[i] * pow(10.0, (double)zadj);
+  }
+  free(m);
+  free(z);
+  return log(acumm) + (maxz - 10) * ORA_LOG10;
+}
+
+double ora_popmig(const float *table, int G, int K, int npsize, int nmig, int thetai, int mi,
+                  double qmax, double mprior, int expo, int prob_or_like, double x) {
+  pm_cols c = pm_columns(npsize, nmig, thetai, mi);
+  double sum, temp;
+  if (!expo) {
+    sum = pm_sum_uniform(table, K, c, 0, G, x, qmax, mprior) / G;
+    if (prob_or_like) {
+      temp = 2 * (log(qmax) + log(mprior) - log(2 * x)) / (qmax * mprior);
+      sum /= temp;
+    }
+  } else {
+    sum = exp(pm_logsum_expo(table, K, c, 0, G, x, qmax, mprior) - log((double)G));
+    if (prob_or_like) {
+      temp = 2 * exp(ora_uppergamma(0, 2 * x / (mprior * qmax))) / (qmax * mprior);
+      sum /= temp;
+    }
+  }
+  return sum;
+}
+
+double ora_marginpopmig(const float *table, int K, int npsize, int nmig, int thetai, int mi,
+                        double qmax, double mprior, int expo, int first, int last, double x) {
+  pm_cols c = pm_columns(npsize, nmig, thetai, mi);
+  double max = expo ? 20.0 * mprior : qmax * mprior / 2.0; /* EXPOMIGPLOTSCALE */
+  if (x < 0 || x > max) return 1.0;                        /* OFFSCALEVAL */
+  double div = (double)(last - first + (first == 0));
+  if (!expo) return -pm_sum_uniform(table, K, c, first, last, x, qmax, mprior) / div;
+  return -exp(pm_logsum_expo(table, K, c, first, last, x, qmax, mprior) - log(div));
+}
+
+double ora_popmig_peak(const float *table, int K, int npsize, int nmig, int thetai, int mi,
+                       double qmax, double mprior, int expo, int first, int last, double *peak) {
+  const int bins = 100;       /* POPMIG_MARGINBINS */
+  const double minp = 1e-7;   /* MINPARAMVAL */
+  const double r = 0.61803399, cg = 1.0 - r, ftol = 1e-7;
+  double upper = expo ? 20.0 * mprior : qmax * mprior / 2.0;
+  double maxf = 1e100, ax, bx, cx;
+  int maxj = -1;
+#define PMF(xx) ora_marginpopmig(table, K, npsize, nmig, thetai, mi, qmax, mprior, expo, first, \
+                                 last, (xx))
+  for (int j = 0; j < bins; j++) {
+    double a = minp + j * (upper / (double)bins), fa = PMF(a);
+    if (fa < maxf) {
+      maxf = fa;
+      maxj = j;
+    }
+  }
+  if (maxj == 0) {
+    ax = minp;
+    cx = minp + upper / (double)bins;
+    bx = (ax + cx) / 2.0;
+  } else if (maxj == bins - 1) {
+    ax = minp + ((double)bins - 1) * (upper / (double)bins);
+    bx = minp + ((double)bins - 2) * (upper / (double)bins);
+    cx = upper;
+  } else {
+    bx = minp + ((double)maxj - 1) * (upper / (double)bins);
+    cx = minp + ((double)maxj + 1) * (upper / (double)bins);
+    ax = minp + ((double)maxj) * (upper / (double)bins);
+  }
+  double x0 = ax, x3 = cx, x1, x2;
+  if (fabs(cx - bx) > fabs(bx - ax)) {
+    x1 = bx;
+    x2 = bx + cg * (cx - bx);
+  } else {
+    x2 = bx;
+    x1 = bx - cg * (bx - ax);
+  }
+  double f1 = PMF(x1), f2 = PMF(x2);
+  while (fabs(x3 - x0) > ftol * (fabs(x1) + fabs(x2))) {
+    if (f2 < f1) {
+      x0 = x1;
+      x1 = x2;
+      x2 = r * x1 + cg * x3;
+      f1 = f2;
+      f2 = PMF(x2);
+      continue;
+    }
+    x3 = x2;
+    x2 = x1;
+    x1 = r * x2 + cg * x0;
+    f2 = f1;
+    f1 = PMF(x1);
+  }
+#undef PMF
+  *peak = f1 < f2 ? x1 : x2;
+  return -(f1 < f2 ? f1 : f2);
+}
+
+/* ---------------------------------------------------------------------------
+ * Probability that one parameter exceeds another -- gtint.cpp:1-335
+ * (mgt_wj_gt_0, pgt_fcj_gt_0, trapzd, qtrap, gtmig, gtpops) and the
+ * genealogy subsampling of print_greater_than_tests (:335-349).
+ */
+#define GT_SWITCH_CRIT 1e-15 /* SWITCH_TO_LOWERGAMMA_CRIT */
+#define GT_MINPARAM 1e-7     /* MINPARAMVAL */
+
+typedef struct {
+  int cci, ccj, wi, wj;
+  double fci, fcj, hval, denom, qmax, fmi, fmj, mmax;
+} gt_state;
+
+static double gt_mgt(const gt_state *s, double mi) { /* mgt_wj_gt_0 */
+  double temp1, temp2, a, b;
+  if (mi < GT_MINPARAM) return 0.0;
+  a = ora_logfact(s->wj);
+  b = ora_uppergamma(s->wj + 1, s->fmj * mi);
+  if (a <= b) return 0.0;
+  if ((a - b) < GT_SWITCH_CRIT)
+    temp1 = ora_lowergamma(s->wj + 1, s->fmj * mi);
+  else
+    temp1 = log_diff(a, b);
+  temp2 = s->wi * log(mi) - s->fmi * mi - (s->wj + 1) * log(s->fmj) + temp1;
+  temp2 -= s->denom;
+  return exp(temp2);
+}
+
+static double gt_pgt(const gt_state *s, double qi) { /* pgt_fcj_gt_0 */
+  double fcj2, fci2, temp1, temp2, temp3, a, b;
+  if (qi < GT_MINPARAM) return 0.0;
+  fcj2 = 2 * s->fcj;
+  fci2 = 2 * s->fci;
+  if (s->ccj == 0) {
+    a = log(qi) - fcj2 / qi;
+    b = log(fcj2) + ora_uppergamma(0, fcj2 / qi);
+    if (a > b) {
+      temp1 = log_diff(a, b);
+      temp2 = -fci2 / qi + s->cci * log(2 / qi);
+      temp3 = temp1 + temp2 - s->hval - s->denom;
+      return exp(temp3);
+    }
+    return 0.0;
+  }
+  temp1 = ora_uppergamma(s->ccj - 1, fcj2 / qi);
+  temp2 = ORA_LOG2 + s->cci * log(2 / qi) + (1 - s->ccj) * log(s->fcj) - fci2 / qi;
+  temp3 = temp2 + temp1 - s->hval - s->denom;
+  return exp(temp3);
+}
+
+/* trapzd / qtrap (Numerical Recipes, as gtint.cpp:75-122): EPS 1e-4,
+ * JMAX 20, convergence tested only after the fifth refinement */
+static double gt_qtrap(const gt_state *s, int mig, double a, double b) {
+  double st = 0, olds = -1.0e100;
+  for (int j = 1; j <= 20; j++) {
+    if (j == 1) {
+      double fa = mig ? gt_mgt(s, a) : gt_pgt(s, a);
+      double fb = mig ? gt_mgt(s, b) : gt_pgt(s, b);
+      st = 0.5 * (b - a) * (fa + fb);
+    } else {
+      int it = 1;
+      for (int k = 1; k < j - 1; k++) it <<= 1;
+      double tnm = it, del = (b - a) / tnm, x = a + 0.5 * del, sum = 0.0;
+      for (int k = 1; k <= it; k++, x += del) sum += mig ? gt_mgt(s, x) : gt_pgt(s, x);
+      st = 0.5 * (st + (b - a) * sum / tnm);
+    }
+    if (j > 5) {
+      if (fabs(st - olds) < 1.0e-4 * fabs(olds) || (st == 0.0 && olds == 0.0)) return st;
+    }
+    olds = st;
+  }
+  return st;
+}
+
+/* the term of one genealogy of gtmig(mi, mj) (gtint.cpp:127-248) */
+double ora_gtmig_one(const float *g, int npsize, int nmig, int mi, int mj, double mmax) {
+  int fmp = 3 * npsize + nmig, mcp = 3 * npsize, mip = 4 * npsize + 2 * nmig;
+  gt_state s;
+  double temp, temp1, temp2, temp3, temp4, a, b, c;
+  memset(&s, 0, sizeof s);
+  s.mmax = mmax;
+  s.fmi = g[fmp + mi];
+  s.fmj = g[fmp + mj];
+  s.wi = (int)g[mcp + mi];
+  s.wj = (int)g[mcp + mj];
+  s.denom = g[mip + mi] + g[mip + mj];
+  if (s.wj == 0) {
+    if (s.fmj > 0.0) {
+      if (s.wi > 0) {
+        a = ora_logfact(s.wi);
+        b = ora_uppergamma(s.wi + 1, (s.fmi + s.fmj) * mmax);
+        c = ora_uppergamma(s.wi + 1, s.fmi * mmax);
+        if (a <= b || a <= c) {
+          temp = 0.0;
+        } else {
+          if ((a - b) < GT_SWITCH_CRIT)
+            temp1 = ora_lowergamma(s.wi + 1, (s.fmi + s.fmj) * mmax);
+          else
+            temp1 = log_diff(a, b);
+          temp1 += -(s.wi + 1) * log(s.fmi + s.fmj);
+          if ((a - c) < GT_SWITCH_CRIT)
+            temp2 = ora_lowergamma(s.wi + 1, s.fmi * mmax);
+          else
+            temp2 = log_diff(a, c);
+          temp2 += -(s.wi + 1) * log(s.fmi);
+          if (temp2 <= temp1) {
+            temp = 0.0;
+          } else {
+            temp3 = log_diff(temp2, temp1);
+            temp4 = temp3 - log(s.fmj) - s.denom;
+            temp = exp(temp4);
+          }
+        }
+      } else if (s.fmi > 0.0) {
+        temp1 = s.fmi * (exp(-(s.fmi + s.fmj) * mmax) - exp(-s.fmi * mmax));
+        temp2 = s.fmj * (1 - exp(-s.fmi * mmax));
+        temp3 = (temp1 + temp2) / (s.fmi * s.fmj * (s.fmi + s.fmj));
+        temp4 = log(temp3) - s.denom;
+        temp = exp(temp4);
+      } else {
+        temp1 = (s.fmj * mmax - 1.0 + exp(-s.fmj * mmax)) / (s.fmj * s.fmj);
+        temp4 = log(temp1) - s.denom;
+        temp = exp(temp4);
+      }
+    } else {
+      if (s.wi > 0) {
+        a = ora_logfact(s.wi + 1);
+        b = ora_uppergamma(s.wi + 2, s.fmi * mmax);
+        if (a <= b) {
+          temp = 0.0;
+        } else {
+          if ((a - b) < GT_SWITCH_CRIT)
+            temp1 = ora_lowergamma(s.wi + 2, s.fmi * mmax);
+          else
+            temp1 = log_diff(a, b);
+          temp2 = -(s.wi + 2) * log(s.fmi);
+          temp3 = temp2 + temp1 - s.denom;
+          temp = exp(temp3);
+        }
+      } else if (s.fmi > 0.0) {
+        temp1 = (1.0 - exp(-s.fmi * mmax) * (s.fmi * mmax + 1.0)) / (s.fmi * s.fmi);
+        temp2 = log(temp1) - s.denom;
+        temp = exp(temp2);
+      } else {
+        temp1 = log(mmax * mmax / 2.0) - s.denom;
+        temp = exp(temp1);
+      }
+    }
+  } else {
+    temp = gt_qtrap(&s, 1, GT_MINPARAM, mmax);
+  }
+  return temp < 1.0 ? temp : 1.0;
+}
+
+/* the term of one genealogy of gtpops(pi, pj) (gtint.cpp:251-331) */
+double ora_gtpops_one(const float *g, int npsize, int nmig, int pi, int pj, double qmax) {
+  int ccp = 0, fcp = npsize, hccp = 2 * npsize, qip = 3 * npsize + 2 * nmig;
+  gt_state s;
+  double temp, temp1, temp2, temp3;
+  memset(&s, 0, sizeof s);
+  s.qmax = qmax;
+  s.cci = (int)g[ccp + pi];
+  s.ccj = (int)g[ccp + pj];
+  s.fci = g[fcp + pi];
+  s.fcj = g[fcp + pj];
+  s.hval = g[hccp + pi] + g[hccp + pj];
+  s.denom = g[qip + pi] + g[qip + pj];
+  if (s.ccj == 0 && s.fcj == 0) {
+    if (s.fci == 0) {
+      temp = exp(2.0 * log(qmax) - ORA_LOG2 - s.hval - s.denom);
+    } else if (s.cci >= 2) {
+      temp1 = 2 * ORA_LOG2 + (2 - s.cci) * log(s.fci);
+      temp2 = ora_uppergamma(s.cci - 2, 2 * s.fci / qmax);
+      temp3 = temp1 + temp2 - s.hval - s.denom;
+      temp = exp(temp3);
+    } else if (s.cci == 1) {
+      temp1 = 4 * s.fci * exp(ora_uppergamma(0, 2 * s.fci / qmax));
+      temp2 = 2 * qmax * exp(-2 * s.fci / qmax) - temp1;
+      temp3 = log(temp2) - s.hval - s.denom;
+      temp = exp(temp3);
+    } else {
+      temp1 = exp(ora_uppergamma(0, 2 * s.fci / qmax));
+      temp2 = (qmax / 2) * (qmax - 2 * s.fci) * exp(-2 * s.fci / qmax) + 2 * s.fci * s.fci * temp1;
+      temp3 = log(temp2) - s.hval - s.denom;
+      temp = exp(temp3);
+    }
+  } else {
+    temp = gt_qtrap(&s, 0, GT_MINPARAM, qmax);
+  }
+  return temp < 1.0 ? temp : 1.0;
+}
+
+/* gtpops / gtmig over the table with print_greater_than_tests's
+ * subsampling: at most 20000 genealogies (USETREESMAX), every
+ * (G / 20000)-th one when there are more */
+double ora_greater_than(const float *table, int G, int K, int npsize, int nmig, int kind, int i,
+                        int j, double prior_max) {
+  int treeinc = 1, numtrees = G, hitree = G;
+  if (G > 20000) {
+    treeinc = G / 20000;
+    numtrees = 20000;
+    hitree = treeinc * 20000;
+  }
+  double sum = 0;
+  for (int ei = 0; ei < hitree; ei += treeinc) {
+    const float *g = table + (size_t)ei * K;
+    sum += kind ? ora_gtmig_one(g, npsize, nmig, i, j, prior_max)
+                : ora_gtpops_one(g, npsize, nmig, i, j, prior_max);
+  }
+  return sum / numtrees;
+}

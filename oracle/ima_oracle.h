@@ -133,6 +133,23 @@ double ora_jointp_exact(const float *table, int G, int K, int npsize, int nmig,
 double ora_margincalc(const float *table, int G, int K, int npsize, int nmig,
                       double x, double yadjust, int pi, int logi,
                       int expo_prior, double prior_mean);
+/* 2NM density (popmig.cpp:9-351): calc_popmig / calc_pop_expomig at x,
+ * marginpopmig / marginpop_expomig over rows [first, last), and
+ * marginalopt_popmig's peak (returns the density there, *peak its place). */
+double ora_popmig(const float *table, int G, int K, int npsize, int nmig, int thetai, int mi,
+                  double qmax, double mprior, int expo, int prob_or_like, double x);
+double ora_marginpopmig(const float *table, int K, int npsize, int nmig, int thetai, int mi,
+                        double qmax, double mprior, int expo, int first, int last, double x);
+double ora_popmig_peak(const float *table, int K, int npsize, int nmig, int thetai, int mi,
+                       double qmax, double mprior, int expo, int first, int last, double *peak);
+/* P(parameter i > parameter j) (gtint.cpp:1-349): one genealogy's term of
+ * gtpops (population sizes, prior max qmax) or gtmig (migration rates, prior
+ * max mmax), and the mean over the table with the reference's subsampling
+ * (kind 0 population sizes, 1 migration rates). */
+double ora_gtpops_one(const float *g, int npsize, int nmig, int pi, int pj, double qmax);
+double ora_gtmig_one(const float *g, int npsize, int nmig, int mi, int mj, double mmax);
+double ora_greater_than(const float *table, int G, int K, int npsize, int nmig, int kind, int i,
+                        int j, double prior_max);
 
 /* ---------------- genealogy prior (update_gtree_common.cpp, utilities.cpp) */
 double ora_logfact(int k);
