@@ -48,23 +48,6 @@ def hky_flops(n, S):
     return (36 * n - 58) * S
 
 
-def prior_integral_us():
-    """Latency of one parameter's integral (integrate_coalescent_term's
-    incomplete-gamma chain on one lane), the critical path of each group of
-    the accept step: the average duration of the one-wavefront prior_kernel
-    (one chain's integrals side by side) in the committed rocprofv3 summary
-    of the plain per-locus path, which launches it once per proposal."""
-    path = os.path.join(ROOT, "profiles", "mc3_accept_3pop_50x1000bp_64chains_trace.md")
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("prior_kernel_avg_us:"):
-                    return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
-
-
 def hky_traffic():
     """HBM bytes per HKY launch of this workload, from the committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by
@@ -352,20 +335,27 @@ def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
     return out
 
 
-def accept_model(avg_us, nloci, group):
+def accept_model(avg_us, nloci, group, ms_by_group):
     """The accept step (mc3_accept, one workgroup per chain) is bound by
     latency, not throughput: each chain's loci are decided in order, in
     speculative groups of `group` loci, and every group waits for its
-    scenario integrals, whose critical path is one incomplete-gamma chain.
-    The latency bound is groups x that chain's latency; frac = bound /
-    measured."""
+    scenario integrals (incomplete-gamma chains side by side) before its
+    in-order decisions.  The tuner's sweep over group sizes D = 1..6 gives
+    the cost of one group: the sweep time is linear in the number of groups
+    ceil(L / D) (least squares; the slope is the latency of one group, the
+    intercept the rest of the sweep).  The latency bound of the accept step
+    is groups x that slope; frac = bound / measured accept-step time."""
     groups = -(-nloci // max(1, group))
-    t_int = prior_integral_us()
     out = {"avg_us": avg_us, "groups_per_chain": groups, "us_per_group": avg_us / groups,
-           "bound": "latency", "integral_latency_us": t_int}
-    if t_int:
-        out["latency_bound_us"] = groups * t_int
-        out["frac"] = groups * t_int / avg_us
+           "bound": "latency"}
+    if ms_by_group is not None and len(ms_by_group) >= 2:
+        g = np.array([-(-nloci // d) for d in range(1, len(ms_by_group) + 1)], float)
+        t = np.asarray(ms_by_group, float) * 1e3
+        slope, icpt = np.polyfit(g, t, 1)
+        out["group_latency_us"] = float(slope)
+        out["sweep_rest_us"] = float(icpt)
+        out["latency_bound_us"] = float(groups * slope)
+        out["frac"] = float(groups * slope / avg_us)
     return out
 
 
@@ -508,7 +498,7 @@ def main():
                    "swap": float(stats[5]) / max(1, stats[4]),
                    "rates": float(stats[7]) / max(1, stats[6])},
         "eager_sweep_ms": sweep_ms / max(1, sweep_n),
-        "accept_step": accept_model(acc_ms / max(1, acc_n) * 1e3, Lc, group),
+        "accept_step": accept_model(acc_ms / max(1, acc_n) * 1e3, Lc, group, group_ms),
         "tuned": {"accept_group_loci": group,
                   "ms_per_sweep_by_group": [round(float(x), 4) for x in group_ms]},
         "setup_s": setup_s,

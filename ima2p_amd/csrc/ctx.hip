@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -331,7 +332,7 @@ int ima_ctx_create(int device, ima_ctx **out) {
   if (scratch_address(device, &c->scratch) != hipSuccess) return IMA_ERR_HIP;
   if (hipStreamCreate(&c->stream) != hipSuccess) return IMA_ERR_HIP;
   if (hipStreamCreate(&c->side) != hipSuccess) return IMA_ERR_HIP;
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < kTimingClasses; ++k) {
     if (hipEventCreate(&c->timing.ev0[k]) != hipSuccess) return IMA_ERR_HIP;
     if (hipEventCreate(&c->timing.ev1[k]) != hipSuccess) return IMA_ERR_HIP;
   }
@@ -401,7 +402,7 @@ void ima_ctx_destroy(ima_ctx *c) {
   release(c->d_swmoves);
   for (auto &t : c->tables)
     if (t.rows) hipFree(t.rows);
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < kTimingClasses; ++k) {
     hipEventDestroy(c->timing.ev0[k]);
     hipEventDestroy(c->timing.ev1[k]);
   }
@@ -639,6 +640,43 @@ int ima_slot_info(ima_ctx *c, int slot, int *locus, int *n) {
 
 // --------------------------------------------------------------- scoring --
 
+// Orders HKY work units for the L2 of each XCD.  The dispatcher deals a
+// launch's workgroups round-robin to the 8 XCDs (workgroup i to XCD i mod
+// 8), so placing every unit of a locus at positions of one residue class
+// mod 8 keeps the locus' tip states, and the genealogy of an item that
+// spans several pattern blocks, in one XCD's L2: each is read from HBM once
+// instead of once per XCD.  Loci are dealt to the 8 classes greedily,
+// largest unit count first, to the class with the fewest units; the
+// classes then fill positions in rounds, and once one runs out the rest
+// continue in round-robin order.  Only the order changes: a unit's partial
+// index is part of the unit.
+static std::vector<int4> xcd_order(const std::vector<int4> &units,
+                                   const std::vector<int> &locus_of_unit) {
+  constexpr int kXcds = 8;
+  std::map<int, std::vector<int>> by_locus;
+  for (size_t i = 0; i < units.size(); ++i) by_locus[locus_of_unit[i]].push_back((int)i);
+  std::vector<std::pair<int, int>> order;  // (count, locus)
+  for (const auto &kv : by_locus) order.push_back({(int)kv.second.size(), kv.first});
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                     return x.first > y.first;
+                   });
+  std::vector<std::vector<int>> bins(kXcds);
+  for (const auto &cl : order) {
+    int b = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (bins[x].size() < bins[b].size()) b = x;
+    const std::vector<int> &u = by_locus[cl.second];
+    bins[b].insert(bins[b].end(), u.begin(), u.end());
+  }
+  std::vector<int4> out;
+  out.reserve(units.size());
+  for (size_t k = 0; out.size() < units.size(); ++k)
+    for (int x = 0; x < kXcds; ++x)
+      if (k < bins[x].size()) out.push_back(units[bins[x][k]]);
+  return out;
+}
+
 static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precision,
                       double *loglik, int32_t *status, int reps = 1) {
   if (!c || (!items && nitems > 0) || !loglik || nitems < 0 || reps < 1) return IMA_ERR_ARG;
@@ -697,6 +735,11 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
     std::vector<int4> units, red;
     int nblk = 0;
     for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[items[i].slot].locus].S, &nblk);
+    {
+      std::vector<int> lu(units.size());
+      for (size_t k = 0; k < units.size(); ++k) lu[k] = c->slots[items[units[k].x].slot].locus;
+      units = xcd_order(units, lu);
+    }
     HIPCHK(c, reserve(c->d_units, units.size()));
     HIPCHK(c, reserve(c->d_red, red.size() + 1));
     HIPCHK(c, reserve(c->d_partial, (size_t)nblk + 1));
@@ -1439,6 +1482,11 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     std::vector<int4> units, red;
     int nblk = 0;
     for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[first_slot + i].locus].S, &nblk);
+    {
+      std::vector<int> lu(units.size());
+      for (size_t k = 0; k < units.size(); ++k) lu[k] = c->slots[first_slot + units[k].x].locus;
+      units = xcd_order(units, lu);
+    }
     HIPCHK(c, reserve(m.units, units.size() + 1));
     HIPCHK(c, reserve(m.red, red.size() + 1));
     HIPCHK(c, reserve(m.partial, (size_t)nblk + 1));
@@ -2783,8 +2831,11 @@ int ima_prior(ima_ctx *c, const int32_t *slots, const int32_t *group_off, int ng
   const DeviceView v = view(c);
   HIPCHK(c, launch_treeweight(v, c->d_pslots.p, nslots, c->d_tv.p, c->d_prow.p, P, emax,
                               c->d_rec.p, c->d_pstatus.p, c->stream));
+  int trc;
+  if ((trc = timing_begin(c, K_PRIOR)) != IMA_OK) return trc;
   HIPCHK(c, launch_prior(v, model, P, mm.npsize + mm.nmig, c->d_rec.p, c->d_goff.p, ngroups,
                          c->d_probg.p, c->d_qint.p, c->d_mintg.p, c->stream));
+  if ((trc = timing_end(c, K_PRIOR)) != IMA_OK) return trc;
   std::vector<int32_t> st(nslots);
   HIPCHK(c, hipMemcpyAsync(st.data(), c->d_pstatus.p, nslots * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(probg, c->d_probg.p, ngroups * 8, hipMemcpyDeviceToHost, c->stream));
@@ -2795,6 +2846,7 @@ int ima_prior(ima_ctx *c, const int32_t *slots, const int32_t *group_off, int ng
     HIPCHK(c, hipMemcpyAsync(mint, c->d_mintg.p, (size_t)ngroups * mm.nmig * 8,
                              hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((trc = timing_collect(c, K_PRIOR)) != IMA_OK) return trc;
   for (int i = 0; i < nslots; ++i)
     if (st[i] != IMA_STATUS_OK)
       return fail(c, IMA_ERR_ARG, "slot %d: population labels or migrations do not fit the model",
@@ -2811,7 +2863,7 @@ int ima_timing_enable(ima_ctx *c, int on) {
 }
 
 int ima_timing_get(ima_ctx *c, int k, int64_t *launches, double *total_ms) {
-  if (!c || k < 0 || k >= 8) return IMA_ERR_ARG;
+  if (!c || k < 0 || k >= kTimingClasses) return IMA_ERR_ARG;
   if (launches) *launches = c->timing.launches[k];
   if (total_ms) *total_ms = c->timing.ms[k];
   return IMA_OK;
@@ -2819,7 +2871,7 @@ int ima_timing_get(ima_ctx *c, int k, int64_t *launches, double *total_ms) {
 
 int ima_timing_reset(ima_ctx *c) {
   if (!c) return IMA_ERR_ARG;
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < kTimingClasses; ++k) {
     c->timing.launches[k] = 0;
     c->timing.ms[k] = 0;
   }

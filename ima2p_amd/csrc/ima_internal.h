@@ -105,15 +105,15 @@ struct DevBuf {
 };
 
 // Launch plumbing shared between translation units.
+enum KClass { K_HKY64 = 0, K_HKY32 = 1, K_IS = 2, K_SW = 3, K_LMODE = 4,
+              K_SWAP = 5, K_MC3 = 6, K_ACCEPT = 7, K_PRIOR = 8, kTimingClasses = 9 };
+
 struct Timing {
   bool on = false;
-  hipEvent_t ev0[8]{}, ev1[8]{};
-  int64_t launches[8]{};
-  double ms[8]{};
+  hipEvent_t ev0[kTimingClasses]{}, ev1[kTimingClasses]{};
+  int64_t launches[kTimingClasses]{};
+  double ms[kTimingClasses]{};
 };
-
-enum KClass { K_HKY64 = 0, K_HKY32 = 1, K_IS = 2, K_SW = 3, K_LMODE = 4,
-              K_SWAP = 5, K_MC3 = 6, K_ACCEPT = 7 };
 
 struct DeviceView {
   const LocusMeta *loci;

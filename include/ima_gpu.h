@@ -678,9 +678,10 @@ int ima_server_usage(ima_server *srv, double *used_bytes, double *budget_bytes,
 /* ---- timing ------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the context's stream (off by default).
  * ima_timing_get returns, for kernel class k (0 HKY FP64, 1 HKY FP32,
- * 2 IS, 3 SW, 4 L-mode, 5 swap, 6 MC3 sweep, 7 the sweep's accept step),
- * the number of timed launches and their total milliseconds.  Classes 6
- * and 7 are timed by ima_mc3_run with IMA_MC3_TIMED. */
+ * 2 IS, 3 SW, 4 L-mode, 5 swap, 6 MC3 sweep, 7 the sweep's accept step,
+ * 8 ima_prior's integrals), the number of timed launches and their total
+ * milliseconds.  Classes 6 and 7 are timed by ima_mc3_run with
+ * IMA_MC3_TIMED. */
 int ima_timing_enable(ima_ctx *ctx, int on);
 int ima_timing_get(ima_ctx *ctx, int kclass, int64_t *launches,
                    double *total_ms);
