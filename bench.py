@@ -223,17 +223,34 @@ def uneven_leg(sweeps=40):
     spec = workloads.uneven_mixed()
     e = Engine(0)
     spec.load(e)
-    group, _ = e.mc3_tune(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
-    e.mc3_run(4, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    flags = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U
+    group, group_ms = e.mc3_tune(4, flags)
+    block = e.mc3_site_block()
+    e.mc3_run(4, flags)
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(sweeps, flags)
     dt = time.perf_counter() - t0
+    # the other site-block size, for the record
+    other = 64 if block == 128 else 128
+    e.close()
+    e = Engine(0)
+    e.set_site_block(other)
+    spec.load(e)
+    if group:
+        e.mc3_prior_group(group)
+    e.mc3_run(4, flags)
+    t1 = time.perf_counter()
+    e.mc3_run(sweeps, flags)
+    dt_other = time.perf_counter() - t1
     e.close()
     kinds = [k for k, _ in spec.loci]
     return {"chain_steps_per_s": spec.nchains * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
             "chains": spec.nchains, "loci": spec.nloci,
             "loci_by_model": {k: kinds.count(k) for k in ("hky", "is", "sw")},
-            "items_per_sweep": spec.nchains * spec.nloci, "tuned_group_loci": group}
+            "items_per_sweep": spec.nchains * spec.nloci, "tuned_group_loci": group,
+            "tuned_ms_per_sweep_by_group": [round(float(x), 4) for x in group_ms],
+            "tuned_site_block_patterns": block,
+            "ms_per_sweep_other_site_block": {str(other): dt_other / sweeps * 1e3}}
 
 
 def is_leg(sweeps=2000):

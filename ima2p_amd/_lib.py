@@ -130,6 +130,8 @@ PROTOS = {
                                         C.POINTER(C.c_int64)]),
     "ima_lmode_greater_than": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
     "ima_slots_read_mig_events": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), i32p, dp, i32p]),
+    "ima_set_site_block": (C.c_int, [vp, C.c_int]),
+    "ima_mc3_site_block": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ima_mcf_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, vp]),
     "ima_mcf_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, dp, vp,
                                C.POINTER(C.c_int64)]),

@@ -431,7 +431,7 @@ int ima_locus_add_hky(ima_ctx *c, int n, int S, const int8_t *seq, const int32_t
   m.totsites = totsites;
   // rows of 64-pattern blocks, 64-byte aligned, zero padded: one block of
   // one tip is 16 aligned dwords
-  m.Spad = (S + kWave - 1) / kWave * kWave;
+  m.Spad = (S + 2 * kWave - 1) / (2 * kWave) * (2 * kWave);  // blocks of up to 128 patterns
   c->seq.resize((c->seq.size() + 63) / 64 * 64, 0);
   m.seq_off = (int64_t)c->seq.size();
   m.mult_off = (int64_t)c->mult.size();
@@ -677,6 +677,17 @@ static std::vector<int4> xcd_order(const std::vector<int4> &units,
   return out;
 }
 
+// Patterns per lane of HKY site blocks (1: 64, 2: 128) for HKY items in
+// slots[0..n): the context's setting, else 64.  Two patterns per lane share
+// the program, coefficient and tip reads, but each wavefront's program
+// takes longer and holds twice the LDS; on the uneven batch (150 HKY loci
+// of 200-3000 bp) 128-pattern blocks made the sweep 19% slower, so 64 is
+// the default and ima_mc3_tune times both on the chains' sweeps.
+static int hky_auto_ppl(const ima_ctx *c, const std::vector<int> &slots) {
+  (void)slots;
+  return c->site_block == 128 ? 2 : 1;
+}
+
 static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precision,
                       double *loglik, int32_t *status, int reps = 1) {
   if (!c || (!items && nitems > 0) || !loglik || nitems < 0 || reps < 1) return IMA_ERR_ARG;
@@ -711,7 +722,13 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
       idx_s.push_back(i);
     }
   }
-  const size_t hlds = hky_lds_bytes(precision, hmax_n, hmax_slots);
+  std::vector<int> hslots;
+  for (int i : hky) hslots.push_back(items[i].slot);
+  int ppl = hky_auto_ppl(c, hslots);
+  if (ppl == 2 && c->site_block != 128 &&
+      hky_lds_bytes(precision, hmax_n, hmax_slots, 2) > (size_t)c->max_lds)
+    ppl = 1;
+  const size_t hlds = hky_lds_bytes(precision, hmax_n, hmax_slots, ppl);
   if (!hky.empty() && hmax_slots > 8)
     return fail(c, IMA_ERR_SIZE, "a genealogy needs %d partial slots (limit 8)", hmax_slots);
   if (!hky.empty() && hlds > (size_t)c->max_lds)
@@ -734,7 +751,8 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
   if (!hky.empty()) {
     std::vector<int4> units, red;
     int nblk = 0;
-    for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[items[i].slot].locus].S, &nblk);
+    for (int i : hky)
+      hky_plan_add(units, red, i, c->loci[c->slots[items[i].slot].locus].S, &nblk, ppl);
     {
       std::vector<int> lu(units.size());
       for (size_t k = 0; k < units.size(); ++k) lu[k] = c->slots[items[units[k].x].slot].locus;
@@ -750,6 +768,7 @@ static int score_impl(ima_ctx *c, const ima_item *items, int nitems, int precisi
                                hipMemcpyHostToDevice, c->stream));
     plan.units = c->d_units.p;
     plan.nunits = (int)units.size();
+    plan.ppl = ppl;
     plan.red = c->d_red.p;
     plan.nred = (int)red.size();
     plan.partial = c->d_partial.p;
@@ -1384,6 +1403,51 @@ static void mc3_release(ima_ctx *c) {
   m.live = false;
 }
 
+// The MC3 chains' HKY scoring plan with site blocks of 64 * ppl patterns:
+// units in XCD order, block-partial merges, each item's first partial and
+// block count (iblk).  Captured sweeps embed the plan, so they are dropped.
+static int mc3_build_plan(ima_ctx *c, int ppl) {
+  auto &m = c->mc3;
+  const int N = m.C * m.L;
+  std::vector<int4> units, red;
+  int nblk = 0;
+  for (int i = 0; i < N; ++i) {
+    const LocusMeta &lm = c->loci[c->slots[m.first + i].locus];
+    if (lm.model == IMA_MODEL_HKY) hky_plan_add(units, red, i, lm.S, &nblk, ppl);
+  }
+  {
+    std::vector<int> lu(units.size());
+    for (size_t k = 0; k < units.size(); ++k) lu[k] = c->slots[m.first + units[k].x].locus;
+    units = xcd_order(units, lu);
+  }
+  HIPCHK(c, reserve(m.units, units.size() + 1));
+  HIPCHK(c, reserve(m.red, red.size() + 1));
+  HIPCHK(c, reserve(m.partial, (size_t)nblk + 1));
+  if (!units.empty())
+    HIPCHK(c, hipMemcpyAsync(m.units.p, units.data(), units.size() * sizeof(int4),
+                             hipMemcpyHostToDevice, c->stream));
+  if (!red.empty())
+    HIPCHK(c, hipMemcpyAsync(m.red.p, red.data(), red.size() * sizeof(int4),
+                             hipMemcpyHostToDevice, c->stream));
+  m.plan.units = m.units.p;
+  m.plan.nunits = (int)units.size();
+  m.plan.red = m.red.p;
+  m.plan.nred = (int)red.size();
+  m.plan.partial = m.partial.p;
+  m.plan.ppl = ppl;
+  std::vector<int2> iblk(N, make_int2(0, 0));
+  for (const int4 &u : units)
+    if (u.y == 0) iblk[u.x] = make_int2(u.z, u.w);
+  HIPCHK(c, reserve(m.iblk, N));
+  HIPCHK(c, hipMemcpyAsync(m.iblk.p, iblk.data(), N * sizeof(int2), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  m.dev.partial = m.partial.p;
+  m.dev.iblk = m.iblk.p;
+  mc3_drop_graphs(c);
+  return IMA_OK;
+}
+
 int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const double *u,
                   const double *kappa, const double *beta, uint64_t seed, int precision,
                   int swaptries, double scale_width) {
@@ -1430,9 +1494,9 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   // time-slide proposals keep every topology, so the stack depth the
   // programs need is fixed for the run unless topology moves are enabled
   // (mc3_topo_mode raises it to the bound of any topology)
-  if (!hky.empty() && hky_lds_bytes(precision, m.hmax_n, m.hmax_slots) > (size_t)c->max_lds)
+  if (!hky.empty() && hky_lds_bytes(precision, m.hmax_n, m.hmax_slots, 1) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "HKY loci of %d gene copies need %zu B of LDS (limit %d)",
-                m.hmax_n, hky_lds_bytes(precision, m.hmax_n, m.hmax_slots), c->max_lds);
+                m.hmax_n, hky_lds_bytes(precision, m.hmax_n, m.hmax_slots, 1), c->max_lds);
   m.nh = (int)hky.size();
   m.ni = (int)is.size();
   m.ns = (int)sw.size();
@@ -1479,34 +1543,16 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(m.idx.p, idx.data(), N * 4, hipMemcpyHostToDevice, c->stream));
   {
-    std::vector<int4> units, red;
-    int nblk = 0;
-    for (int i : hky) hky_plan_add(units, red, i, c->loci[c->slots[first_slot + i].locus].S, &nblk);
-    {
-      std::vector<int> lu(units.size());
-      for (size_t k = 0; k < units.size(); ++k) lu[k] = c->slots[first_slot + units[k].x].locus;
-      units = xcd_order(units, lu);
-    }
-    HIPCHK(c, reserve(m.units, units.size() + 1));
-    HIPCHK(c, reserve(m.red, red.size() + 1));
-    HIPCHK(c, reserve(m.partial, (size_t)nblk + 1));
-    if (!units.empty())
-      HIPCHK(c, hipMemcpyAsync(m.units.p, units.data(), units.size() * sizeof(int4),
-                               hipMemcpyHostToDevice, c->stream));
-    if (!red.empty())
-      HIPCHK(c, hipMemcpyAsync(m.red.p, red.data(), red.size() * sizeof(int4),
-                               hipMemcpyHostToDevice, c->stream));
-    m.plan.units = m.units.p;
-    m.plan.nunits = (int)units.size();
-    m.plan.red = m.red.p;
-    m.plan.nred = (int)red.size();
-    m.plan.partial = m.partial.p;
-    std::vector<int2> iblk(N, make_int2(0, 0));
-    for (const int4 &u : units)
-      if (u.y == 0) iblk[u.x] = make_int2(u.z, u.w);
-    HIPCHK(c, reserve(m.iblk, N));
-    HIPCHK(c, hipMemcpyAsync(m.iblk.p, iblk.data(), N * sizeof(int2), hipMemcpyHostToDevice,
-                             c->stream));
+    std::vector<int> hslots;
+    for (int i : hky) hslots.push_back(first_slot + i);
+    int ppl = hky_auto_ppl(c, hslots);
+    if (ppl == 2 && hky_lds_bytes(precision, m.hmax_n, m.hmax_slots, 2) > (size_t)c->max_lds)
+      ppl = 1;
+    int rc;
+    if ((rc = mc3_build_plan(c, ppl)) != IMA_OK) return rc;
+    // the rate rounds' unit lists hold up to every unit at 64 patterns
+    size_t units64 = 0;
+    for (int i : hky) units64 += (c->loci[c->slots[first_slot + i].locus].S + kWave - 1) / kWave;
     HIPCHK(c, reserve(m.done, 1));
     HIPCHK(c, hipMemsetAsync(m.done.p, 0, sizeof(unsigned), c->stream));
     // mutation-rate rounds (update_mc_params.cpp:24-429): log(UMAX)/L
@@ -1520,8 +1566,8 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     HIPCHK(c, reserve(m.uorder, (size_t)nchains * J));
     HIPCHK(c, reserve(m.uloff, (size_t)nchains * (J + 2)));
     HIPCHK(c, reserve(m.ulast, N));
-    HIPCHK(c, reserve(m.ulist0, units.size() + 1));
-    HIPCHK(c, reserve(m.ulist1, units.size() + 1));
+    HIPCHK(c, reserve(m.ulist0, units64 + 1));
+    HIPCHK(c, reserve(m.ulist1, units64 + 1));
     HIPCHK(c, reserve(m.ucnt, 2));
     HIPCHK(c, hipMemsetAsync(m.ucnt.p, 0, 2 * sizeof(int32_t), c->stream));
     if (urate_plan_lds(nloci, J) > 65536)
@@ -1686,8 +1732,8 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
     // any topology may come up: the pruning stack must hold floor(log2 n)
     int bound = 1;
     while ((2 << bound) <= m.hmax_n) ++bound;
-    if (m.nh && hky_lds_bytes(m.precision, m.hmax_n, std::max(m.hmax_slots, bound)) >
-                    (size_t)c->max_lds)
+    if (m.nh && hky_lds_bytes(m.precision, m.hmax_n, std::max(m.hmax_slots, bound),
+                              m.plan.ppl) > (size_t)c->max_lds)
       return fail(c, IMA_ERR_SIZE, "topology moves on %d gene copies need more LDS than %d B",
                   m.hmax_n, c->max_lds);
   }
@@ -2215,11 +2261,44 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int flags, int *best, double *ms_per_sw
       pick_ms = ms;
     }
   }
-  // the chains go back to where they were, with the fastest group size
+  if (rc == IMA_OK) rc = ima_mc3_set_prior_group(c, pick);
+  // HKY site blocks of 64 or 128 patterns (one or two per lane), timed at
+  // the chosen group size unless the context fixes the block size
+  if (rc == IMA_OK && m.nh && c->site_block == 0 &&
+      hky_lds_bytes(m.precision, m.hmax_n, m.hmax_slots, 2) <= (size_t)c->max_lds) {
+    const int ppl0 = m.plan.ppl;
+    int best_ppl = ppl0;
+    float best_ms = pick_ms;
+    for (int ppl = 1; ppl <= 2 && rc == IMA_OK; ++ppl) {
+      if (ppl == ppl0) continue;
+      rc = mc3_build_plan(c, ppl);
+      float ms = 0.0f;
+      for (int pass = 0; pass < 2 && rc == IMA_OK; ++pass) {
+        rc = mc3_state_copy(c, buf, false);
+        m.host_sweeps = hs;
+        m.scale_sweeps = ss;
+        const int n = pass == 0 ? 2 : sweeps;
+        if (rc == IMA_OK && pass == 1 && hipEventRecord(e0, c->stream) != hipSuccess)
+          rc = fail(c, IMA_ERR_HIP, "event record failed");
+        if (rc == IMA_OK) rc = mc3_run_async(c, n, run_flags);
+        if (rc == IMA_OK && pass == 1) {
+          if (hipEventRecord(e1, c->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(c, IMA_ERR_HIP, "timing failed");
+        }
+      }
+      if (rc == IMA_OK && ms < best_ms) {
+        best_ms = ms;
+        best_ppl = ppl;
+      }
+    }
+    if (rc == IMA_OK && m.plan.ppl != best_ppl) rc = mc3_build_plan(c, best_ppl);
+  }
+  // the chains go back to where they were, with the fastest shapes
   if (rc == IMA_OK) rc = mc3_state_copy(c, buf, false);
   m.host_sweeps = hs;
   m.scale_sweeps = ss;
-  if (rc == IMA_OK) rc = ima_mc3_set_prior_group(c, pick);
   if (hipStreamSynchronize(c->stream) != hipSuccess && rc == IMA_OK)
     rc = fail(c, IMA_ERR_HIP, "synchronize failed");
   if (e0) hipEventDestroy(e0);
@@ -2227,6 +2306,19 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int flags, int *best, double *ms_per_sw
   hipFree(buf);
   if (rc == IMA_OK && best) *best = pick;
   return rc;
+}
+
+int ima_set_site_block(ima_ctx *c, int patterns) {
+  if (!c || (patterns != 0 && patterns != 64 && patterns != 128)) return IMA_ERR_ARG;
+  c->site_block = patterns;
+  return IMA_OK;
+}
+
+int ima_mc3_site_block(ima_ctx *c, int *patterns) {
+  if (!c || !patterns) return IMA_ERR_ARG;
+  if (!c->mc3.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  *patterns = kWave * c->mc3.plan.ppl;
+  return IMA_OK;
 }
 
 // Decisions for a batch of proposals written by the host (IMa2p's own

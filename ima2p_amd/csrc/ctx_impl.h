@@ -96,6 +96,7 @@ struct ima_ctx {
 
   Timing timing;
   int max_lds = 65536;
+  int site_block = 0;  // HKY site-block size: 0 automatic, 64 or 128 patterns
 
   // device-resident MC3 sweep
   struct {

@@ -30,7 +30,7 @@ struct LocusMeta {
   int64_t mult_off;  // HKY: int32 [S]
   int64_t mask_off;  // IS: uint64 [S][W] tips carrying state 1
   int32_t W;         // IS: 64-bit words per tip mask
-  int32_t Spad;      // HKY: row stride of seq (S rounded up to 64, zero padded)
+  int32_t Spad;      // HKY: row stride of seq (S rounded up to 128, zero padded)
   double pi[4];
   double sumlogk;
   int32_t pmodel;               // population model of the prior, -1 if none
@@ -167,16 +167,17 @@ __device__ inline ModelView model_view(const DeviceView &v, const ModelMeta &m) 
 struct HkyPlan {
   const int4 *units = nullptr;
   int nunits = 0;
+  int ppl = 1;  // patterns per lane: site blocks of 64 * ppl patterns
   const int4 *red = nullptr;
   int nred = 0;
   double *partial = nullptr;
 };
 // Host side: append the units of HKY item ii with S patterns.
 void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int S,
-                  int *nblocks_total);
+                  int *nblocks_total, int ppl);
 
 // Kernel launchers (kern_hky.hip, kern_lik.hip).
-size_t hky_lds_bytes(int precision, int max_n, int max_slots);
+size_t hky_lds_bytes(int precision, int max_n, int max_slots, int ppl);
 hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                       int precision, int max_n, int max_slots, double *out,
                       int32_t *status, hipStream_t st);
@@ -193,7 +194,7 @@ struct TwLaunch {
 };
 // HKY units (no merge step) and treeweight in one launch, when its LDS
 // (hky_tw_lds_bytes) is within kTwLdsBudget.
-size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int P, int emax);
+size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int ppl, int P, int emax);
 hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                          int precision, int max_n, int max_slots, double *out, int32_t *status,
                          const TwLaunch &tw, int P, hipStream_t st);
@@ -419,7 +420,7 @@ hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URate
                                int nlev, int serial, hipStream_t st);
 hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int4 *list,
                            const int32_t *cnt, int32_t *cnt_next, int grid, int precision,
-                           int max_n, int max_slots, double *partial, double *out,
+                           int max_n, int max_slots, int ppl, double *partial, double *out,
                            int32_t *status, hipStream_t st);
 constexpr double kTimeMax = 1000000.0;  // the reference's TIMEMAX (imamp.hpp:129)
 constexpr int kSwapLds = 256;  // chains whose S and beta the swap stages in LDS

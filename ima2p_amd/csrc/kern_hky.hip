@@ -1,23 +1,24 @@
 // kern_hky.hip -- batched HKY Felsenstein pruning (calc_prob_data.cpp:118-607).
 //
-// One 64-lane wavefront per work unit = (scoring item, block of 64 site
-// patterns), one lane per pattern.  A scoring item is a genealogy of one
-// chain at one locus; items of all chains and loci, with uneven gene counts
-// and pattern counts, share one launch, and every wavefront does the same
-// amount of work.  Items spanning several blocks are summed in block order
-// by a second kernel, so results do not depend on scheduling.
+// One 64-lane wavefront per work unit = (scoring item, block of 64 * PPL
+// site patterns), PPL patterns per lane (PPL = 1 or 2, the site-block size
+// the tuner picks: 64 or 128 patterns).  A scoring item is a genealogy of
+// one chain at one locus; items of all chains and loci, with uneven gene
+// counts and pattern counts, share one launch, and every wavefront does the
+// same amount of work.  Items spanning several blocks are summed in block
+// order by a second kernel, so results do not depend on scheduling.  With
+// PPL = 2 a lane carries two patterns through the same program: the
+// program, coefficient and tip-row reads, the addresses and the branches
+// are shared by both, only the arithmetic is doubled.
 //
 // The host turns each genealogy into a postorder stack program (ProgOp):
 // the daughter with the larger need is evaluated first, so the stack never
 // holds more than floor(log2 n) partials (Strahler number), at most 8 for
-// n <= 256.  The stack is a register array of D quads.  Each operation is
-// dispatched on the current stack depth (wave-uniform) into code specialised
-// for that depth, so every stack access is a fixed register: a node's
-// partial never leaves the register file, nothing is shifted and no private
-// (scratch) memory is used.  Nothing per node
-// goes to HBM: every proposal is re-scored over the whole genealogy,
-// ~(36n-58)S flops on chip, instead of reading and writing ~36 B per
-// node-pattern of stored partials.
+// n <= 256.  The two top partials live in registers, the levels below them
+// in the wavefront's LDS, one 16-byte column per lane and pattern.  Nothing
+// per node goes to HBM: every proposal is re-scored over the whole
+// genealogy, ~(36n-58)S flops on chip, instead of reading and writing ~36 B
+// per node-pattern of stored partials.
 //
 // Branch transition (HKY85 closed form, calc_prob_data.cpp:197-208):
 //   P_jk(t) = pi_k (1-e) + [cls(j)=cls(k)] pi_k (e - eA_c)/PI_c + [j=k] eA_c
@@ -27,10 +28,10 @@
 // b_c = (e-eA_c)/PI_c, W = sum pi_k L_k, W_c = sum_{k in c} pi_k L_k.  For a
 // tip in state s the vector is column s of P.  Per branch the five
 // coefficients, and per tip branch the four columns of P, are computed once
-// per item in FP64 and kept in LDS; a tip operand is one 16-byte LDS read.  Partials are
-// rescaled at every node by an exact power of two; one integer exponent per
-// lane carries the sum of the node scale factors (the reference keeps
-// log(max) per node, :211-217, 465).
+// per unit in FP64 and kept in LDS; a tip operand is one 16-byte LDS read.
+// Partials are rescaled by an exact power of two when they fall below a
+// floor; one integer exponent per lane and pattern carries the sum of the
+// scale factors (the reference keeps log(max) per node, :211-217, 465).
 #include "ima_internal.h"
 #include "tw_dev.h"
 
@@ -91,21 +92,25 @@ __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlan
 // staging has no branches: an operand that is not a tip reads tip row
 // min(edge, n - 1) (any row: its value is not used), and an edge index is
 // always below 2n - 1.
+template <int PPL>
 struct Mid {
-  int type, a, b;  // the same in every lane
-  int ra, rb;      // tip rows read for a and b
-  int ta, tb;      // this lane's states at those rows
+  int type, a, b;        // the same in every lane
+  int ra, rb;            // tip rows read for a and b
+  int ta[PPL], tb[PPL];  // this lane's states at those rows
 };
 
-template <typename real>
+template <typename real, int PPL>
 struct Nxt {
   int type;
   HkyCoef<real> ca, cb;
-  Q4<real> pa, pb;  // columns of P for tip operands
+  Q4<real> pa[PPL], pb[PPL];  // columns of P for tip operands
 };
 
+// tips: [n][64 * PPL] states; pattern j * 64 + lane of the block at byte
+// row * 64 * PPL + j * 64 + lane
+template <int PPL>
 __device__ __forceinline__ void to_mid(const ProgOp &raw, const uint8_t *tips, int lane, int nlast,
-                                       Mid &m) {
+                                       Mid<PPL> &m) {
   // a and b stay in vector registers (every lane holds the same value) and
   // feed the LDS addresses directly; only the type is made scalar, for the
   // branch
@@ -114,18 +119,24 @@ __device__ __forceinline__ void to_mid(const ProgOp &raw, const uint8_t *tips, i
   m.type = uni(raw.type);
   m.ra = min(m.a, nlast);
   m.rb = min(m.b, nlast);
-  m.ta = tips[m.ra * kWave + lane];
-  m.tb = tips[m.rb * kWave + lane];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    m.ta[j] = tips[m.ra * (kWave * PPL) + j * kWave + lane];
+    m.tb[j] = tips[m.rb * (kWave * PPL) + j * kWave + lane];
+  }
 }
 
-template <typename real>
-__device__ __forceinline__ void to_nxt(const Mid &m, const HkyCoef<real> *coef,
-                                       const Q4<real> *colP, Nxt<real> &x) {
+template <typename real, int PPL>
+__device__ __forceinline__ void to_nxt(const Mid<PPL> &m, const HkyCoef<real> *coef,
+                                       const Q4<real> *colP, Nxt<real, PPL> &x) {
   x.type = m.type;
   x.ca = coef[m.a];
   x.cb = coef[m.b];
-  x.pa = colP[m.ra * 4 + m.ta];
-  x.pb = colP[m.rb * 4 + m.tb];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    x.pa[j] = colP[m.ra * 4 + m.ta[j]];
+    x.pb[j] = colP[m.rb * 4 + m.tb[j]];
+  }
 }
 
 // Below this a node's partials are rescaled (a power of two: 2^-40 in
@@ -133,57 +144,94 @@ __device__ __forceinline__ void to_nxt(const Mid &m, const HkyCoef<real> *coef,
 __device__ __forceinline__ float rescale_floor(float) { return 0x1p-40f; }
 __device__ __forceinline__ double rescale_floor(double) { return 0x1p-400; }
 
-// Combine two operand vectors into a node's partials.  When some lane's
-// largest partial falls below the floor, every lane rescales by an exact
-// power of two (the exponent is carried in E), so the partials' mantissas
-// are those of rescaling at every node.
 template <typename real>
-__device__ __forceinline__ Q4<real> combine(const Q4<real> &va, const Q4<real> &vb, int &E) {
+__device__ __forceinline__ Q4<real> mul4(const Q4<real> &a, const Q4<real> &b) {
   Q4<real> L;
-  L.x = va.x * vb.x;
-  L.y = va.y * vb.y;
-  L.z = va.z * vb.z;
-  L.w = va.w * vb.w;
-  const real mx = fmax(fmax(L.x, L.y), fmax(L.z, L.w));
-  if (__ballot(mx < rescale_floor(mx))) {
-    const int ex = frexp_e(mx);
-    E += ex;
-    L.x = ldexp_r(L.x, -ex);
-    L.y = ldexp_r(L.y, -ex);
-    L.z = ldexp_r(L.z, -ex);
-    L.w = ldexp_r(L.w, -ex);
-  }
+  L.x = a.x * b.x;
+  L.y = a.y * b.y;
+  L.z = a.z * b.z;
+  L.w = a.w * b.w;
   return L;
+}
+template <typename real>
+__device__ __forceinline__ real max4(const Q4<real> &L) {
+  return fmax(fmax(L.x, L.y), fmax(L.z, L.w));
+}
+
+// A node's partials from the two operand vectors of each pattern.  When some
+// lane's largest partial of some pattern falls below the floor, every
+// pattern of every lane is rescaled by an exact power of two (its exponent
+// carried in E), so the mantissas are those of rescaling at every node.
+template <typename real, int PPL>
+__device__ __forceinline__ void combine(const Q4<real> (&va)[PPL], const Q4<real> (&vb)[PPL],
+                                        Q4<real> (&L)[PPL], int (&E)[PPL]) {
+  real mx[PPL];
+  bool small = false;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    L[j] = mul4(va[j], vb[j]);
+    mx[j] = max4(L[j]);
+    small = small || mx[j] < rescale_floor(mx[j]);
+  }
+  if (__ballot(small)) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const int ex = frexp_e(mx[j]);
+      E[j] += ex;
+      L[j].x = ldexp_r(L[j].x, -ex);
+      L[j].y = ldexp_r(L[j].y, -ex);
+      L[j].z = ldexp_r(L[j].z, -ex);
+      L[j].w = ldexp_r(L[j].w, -ex);
+    }
+  }
 }
 
 // One program operation.  The two top partials of the stack live in
 // registers (tos = stk[d-1], nos = stk[d-2]); the levels below them live in
-// this lane's column of the wavefront's LDS stack (level j at low[j * 64]).
-// TT pushes: nos spills to level d-2, tos becomes nos.  ST rewrites tos.
-// SS pops: the combination of nos (daughter a) and tos (daughter b) becomes
-// tos, and level d-3 is read back into nos.  The type and d are
-// wave-uniform, so the three cases are scalar branches.
+// this lane's column of the wavefront's LDS stack (level l of pattern j at
+// low[(l * PPL + j) * 64]).  TT pushes: nos spills to level d-2, tos becomes
+// nos.  ST rewrites tos.  SS pops: the combination of nos (daughter a) and
+// tos (daughter b) becomes tos, and level d-3 is read back into nos.  The
+// type and d are wave-uniform, so the three cases are scalar branches.
 // While the stack holds fewer than three partials, level 0 is free: a push
 // spills its meaningless nos there and a pop reads it back unused, so the
 // LDS accesses need no branches.
-template <typename real>
-__device__ __forceinline__ void apply_op(const Nxt<real> &c, Q4<real> &tos, Q4<real> &nos, int &d,
-                                         Q4<real> *low, const real pi[4], int &E) {
+template <typename real, int PPL>
+__device__ __forceinline__ void apply_op(const Nxt<real, PPL> &c, Q4<real> (&tos)[PPL],
+                                         Q4<real> (&nos)[PPL], int &d, Q4<real> *low,
+                                         const real pi[4], int (&E)[PPL]) {
   // three independent wave-uniform tests (not an else-chain), so each case
   // is one scalar compare and branch
   const int type = c.type;
   if (type == OP_TT) {
-    low[max(d - 2, 0) * kWave] = nos;
-    nos = tos;
-    tos = combine<real>(c.pa, c.pb, E);
+    const int lv = max(d - 2, 0) * PPL;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      low[(lv + j) * kWave] = nos[j];
+      nos[j] = tos[j];
+    }
+    combine<real, PPL>(c.pa, c.pb, tos, E);
     ++d;
   }
-  if (type == OP_ST) tos = combine<real>(int_vec<real>(c.ca, tos, pi), c.pb, E);
+  if (type == OP_ST) {
+    Q4<real> va[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) va[j] = int_vec<real>(c.ca, tos[j], pi);
+    combine<real, PPL>(va, c.pb, tos, E);
+  }
   if (type == OP_SS) {
-    tos = combine<real>(int_vec<real>(c.ca, nos, pi), int_vec<real>(c.cb, tos, pi), E);
+    Q4<real> va[PPL], vb[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      va[j] = int_vec<real>(c.ca, nos[j], pi);
+      vb[j] = int_vec<real>(c.cb, tos[j], pi);
+    }
+    combine<real, PPL>(va, vb, tos, E);
     // read into nos itself, last: the wait for it falls at nos's next use,
     // after the next entry's staging, not here
-    nos = low[max(d - 3, 0) * kWave];
+    const int lv = max(d - 3, 0) * PPL;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) nos[j] = low[(lv + j) * kWave];
     --d;
   }
 }
@@ -194,18 +242,20 @@ __device__ __forceinline__ void apply_op(const Nxt<real> &c, Q4<real> &tos, Q4<r
 // launch's depth class 4, 6 or 8, less the two held in registers).
 static int hky_low_levels(int max_slots) { return max_slots <= 4 ? 2 : max_slots <= 6 ? 4 : 6; }
 
-size_t hky_lds_bytes(int precision, int max_n, int max_slots) {
+size_t hky_lds_bytes(int precision, int max_n, int max_slots, int ppl) {
   const size_t ne = 2 * (size_t)max_n - 1;
   const size_t r = precision == IMA_PREC_FP32 ? 4 : 8;
   // lower stack levels, coefficients per edge, P columns per tip edge,
   // program, tip states
-  return (size_t)hky_low_levels(max_slots) * kWave * (4 * r) + ne * (8 * r) +
-         (size_t)max_n * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) + (size_t)max_n * kWave;
+  return (size_t)hky_low_levels(max_slots) * ppl * kWave * (4 * r) + ne * (8 * r) +
+         (size_t)max_n * (16 * r) + ((size_t)max_n - 1) * sizeof(ProgOp) +
+         (size_t)max_n * kWave * ppl;
 }
 
 void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int S,
-                  int *nblocks_total) {
-  const int nb = (S + kWave - 1) / kWave;
+                  int *nblocks_total, int ppl) {
+  const int bs = kWave * ppl;
+  const int nb = (S + bs - 1) / bs;
   const int pbase = *nblocks_total;
   for (int b = 0; b < nb; ++b) units.push_back(make_int4(ii, b, pbase, nb));
   if (nb > 1) red.push_back(make_int4(ii, pbase, nb, 0));
@@ -214,7 +264,7 @@ void hky_plan_add(std::vector<int4> &units, std::vector<int4> &red, int ii, int 
 
 // Work unit `unit` (item, pattern block, first partial, blocks) of one
 // 64-lane workgroup, with smem its dynamic LDS of hky_lds_bytes.
-template <typename real, int D>
+template <typename real, int D, int PPL>
 __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__restrict__ items,
                                          const int4 unit, double *__restrict__ partial,
                                          double *__restrict__ out,
@@ -229,12 +279,12 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
   const int n = sm.n, ne = 2 * n - 1, S = lm.S, nops = n - 1;
 
   constexpr int kLow = D - 2;  // the launch's lower stack levels (D is 4, 6 or 8)
-  Q4<real> *low = reinterpret_cast<Q4<real> *>(smem) + lane;               // [kLow][64]
+  Q4<real> *low = reinterpret_cast<Q4<real> *>(smem) + lane;     // [kLow][PPL][64]
   HkyCoef<real> *coef = reinterpret_cast<HkyCoef<real> *>(
-      reinterpret_cast<Q4<real> *>(smem) + kLow * kWave);                  // [ne]
-  Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);                 // [n][4] tips
-  ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)n * 4);           // [nops]
-  uint8_t *tips = reinterpret_cast<uint8_t *>(ops + nops);                  // [n][64]
+      reinterpret_cast<Q4<real> *>(smem) + kLow * PPL * kWave);  // [ne]
+  Q4<real> *colP = reinterpret_cast<Q4<real> *>(coef + ne);       // [n][4] tips
+  ProgOp *ops = reinterpret_cast<ProgOp *>(colP + (size_t)n * 4); // [nops]
+  uint8_t *tips = reinterpret_cast<uint8_t *>(ops + nops);        // [n][64 PPL]
 
   const double *pid = lm.pi;
   const double kappa = item.kappa;
@@ -283,24 +333,23 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
   for (int k = lane; k < nops; k += kWave) ops[k] = prog[k];
   const real pi[4] = {real(pid[0]), real(pid[1]), real(pid[2]), real(pid[3])};
   const uint32_t *seq32 = reinterpret_cast<const uint32_t *>(v.seq + lm.seq_off);
-  const int row32 = lm.Spad / 4;
+  const int row32 = lm.Spad / 4;  // Spad is a multiple of 128: no block reads past its row
   const int32_t *mult = v.mult + lm.mult_off;
   uint32_t *tips32 = reinterpret_cast<uint32_t *>(tips);
-  const int words = n * (kWave / 4);  // dwords of one block: n rows x 16
+  constexpr int kRowWords = kWave / 4 * PPL;  // dwords of one row of a block
+  const int words = n * kRowWords;
 
   double acc = 0.0;
   {
-    const int p = blk * kWave + lane;
-    const bool valid = p < S;
     wave_lds_fence();
-    // tip states of this block: n rows of 16 aligned dwords, all loads issued
-    // before the LDS writes
+    // tip states of this block: n rows of kRowWords aligned dwords, all loads
+    // issued before the LDS writes
     for (int w0 = 0; w0 < words; w0 += 16 * kWave) {
       uint32_t r[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int w = min(w0 + j * kWave + lane, words - 1);
-        r[j] = seq32[(size_t)(w >> 4) * row32 + blk * 16 + (w & 15)];
+        r[j] = seq32[(size_t)(w / kRowWords) * row32 + blk * kRowWords + (w % kRowWords)];
       }
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
@@ -310,34 +359,45 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
     }
     wave_lds_fence();
 
-    Q4<real> tos, nos;
-    tos.x = tos.y = tos.z = tos.w = nos.x = nos.y = nos.z = nos.w = real(0);
-    int E = 0, d = 0;
+    Q4<real> tos[PPL], nos[PPL];
+    int E[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      tos[j].x = tos[j].y = tos[j].z = tos[j].w = real(0);
+      nos[j] = tos[j];
+      E[j] = 0;
+    }
+    int d = 0;
     // op k computes while the coefficients and P columns of op k+1, the tip
     // states of op k+2 and the program entry of op k+3 are read from LDS;
     // unrolled by two so no staged operand is ever copied
     const int last = nops - 1;
-    Mid M0, M1;
-    Nxt<real> N0, N1;
-    to_mid(ops[0], tips, lane, n - 1, M0);
-    to_mid(ops[min(1, last)], tips, lane, n - 1, M1);
-    to_nxt<real>(M0, coef, colP, N0);
+    Mid<PPL> M0, M1;
+    Nxt<real, PPL> N0, N1;
+    to_mid<PPL>(ops[0], tips, lane, n - 1, M0);
+    to_mid<PPL>(ops[min(1, last)], tips, lane, n - 1, M1);
+    to_nxt<real, PPL>(M0, coef, colP, N0);
     ProgOp raw = ops[min(2, last)];
     for (int k = 0; k < nops; k += 2) {
       // even: compute N0 (op k), N1 <- M1 (op k+1), M0 <- raw (op k+2)
-      to_nxt<real>(M1, coef, colP, N1);
-      to_mid(raw, tips, lane, n - 1, M0);
+      to_nxt<real, PPL>(M1, coef, colP, N1);
+      to_mid<PPL>(raw, tips, lane, n - 1, M0);
       raw = ops[min(k + 3, last)];
-      apply_op<real>(N0, tos, nos, d, low, pi, E);
+      apply_op<real, PPL>(N0, tos, nos, d, low, pi, E);
       if (k + 1 >= nops) break;
       // odd: compute N1 (op k+1), N0 <- M0 (op k+2), M1 <- raw (op k+3)
-      to_nxt<real>(M0, coef, colP, N0);
-      to_mid(raw, tips, lane, n - 1, M1);
+      to_nxt<real, PPL>(M0, coef, colP, N0);
+      to_mid<PPL>(raw, tips, lane, n - 1, M1);
       raw = ops[min(k + 4, last)];
-      apply_op<real>(N1, tos, nos, d, low, pi, E);
+      apply_op<real, PPL>(N1, tos, nos, d, low, pi, E);
     }
-    const real fr = pi[0] * tos.x + pi[1] * tos.y + pi[2] * tos.z + pi[3] * tos.w;
-    if (valid) acc += (double)mult[p] * (log((double)fr) + (double)E * kLn2);
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const int p = blk * kWave * PPL + j * kWave + lane;
+      const real fr =
+          pi[0] * tos[j].x + pi[1] * tos[j].y + pi[2] * tos[j].z + pi[3] * tos[j].w;
+      if (p < S) acc += (double)mult[p] * (log((double)fr) + (double)E[j] * kLn2);
+    }
   }
   acc = wave_sum_d(acc);
   if (lane == 0) {
@@ -350,18 +410,18 @@ __device__ __forceinline__ void hky_unit(const DeviceView &v, const ima_item *__
   }
 }
 
-template <typename real, int D>
+template <typename real, int D, int PPL>
 __global__ __launch_bounds__(kWave) void hky_kernel(
     DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
     double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  hky_unit<real, D>(v, items, units[blockIdx.x], partial, out, status, smem);
+  hky_unit<real, D, PPL>(v, items, units[blockIdx.x], partial, out, status, smem);
 }
 
 // The mutation-rate round's scoring: the pattern-block units of one
 // level's proposals, listed by the previous launch (*cnt of them).  Block 0
 // clears the other list's counter for the next level.
-template <typename real, int D>
+template <typename real, int D, int PPL>
 __global__ __launch_bounds__(kWave) void hky_list_kernel(
     DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ list,
     const int32_t *__restrict__ cnt, int32_t *__restrict__ cnt_next,
@@ -371,7 +431,7 @@ __global__ __launch_bounds__(kWave) void hky_list_kernel(
   const int n = *cnt;
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     wave_lds_fence();
-    hky_unit<real, D>(v, items, list[k], partial, out, status, smem);
+    hky_unit<real, D, PPL>(v, items, list[k], partial, out, status, smem);
   }
 }
 
@@ -379,7 +439,7 @@ __global__ __launch_bounds__(kWave) void hky_list_kernel(
 // proposals' genealogies (treeweight), the rest run the likelihood units.
 // One launch instead of two side by side on two streams saves the
 // cross-stream fork and join between the accept steps.
-template <typename real, int D>
+template <typename real, int D, int PPL>
 __global__ __launch_bounds__(kWave) void hky_tw_kernel(
     DeviceView v, const ima_item *__restrict__ items, const int4 *__restrict__ units,
     double *__restrict__ partial, double *__restrict__ out, int32_t *__restrict__ status,
@@ -390,7 +450,7 @@ __global__ __launch_bounds__(kWave) void hky_tw_kernel(
                           reinterpret_cast<double *>(smem));
     return;
   }
-  hky_unit<real, D>(v, items, units[blockIdx.x - tw.n], partial, out, status, smem);
+  hky_unit<real, D, PPL>(v, items, units[blockIdx.x - tw.n], partial, out, status, smem);
 }
 
 __global__ void hky_reduce(const int4 *__restrict__ red, int nred,
@@ -405,83 +465,116 @@ __global__ void hky_reduce(const int4 *__restrict__ red, int nred,
   if (status) status[r.x] = IMA_STATUS_OK;
 }
 
-template <typename real>
-static void launch_depth(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
-                         const ima_item *items, const HkyPlan &p, double *out,
-                         int32_t *status) {
-  if (D <= 4)
-    hipLaunchKernelGGL((hky_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status);
-  else if (D <= 6)
-    hipLaunchKernelGGL((hky_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status);
-  else
-    hipLaunchKernelGGL((hky_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status);
+// Calls f.template operator()<D, PPL>() for the launch's depth class
+// (max_slots up to 4, 6 or 8) and patterns per lane (1 or 2).
+template <typename F>
+static void by_shape(int max_slots, int ppl, const F &f) {
+  if (ppl == 2) {
+    if (max_slots <= 4) f.template operator()<4, 2>();
+    else if (max_slots <= 6) f.template operator()<6, 2>();
+    else f.template operator()<8, 2>();
+  } else {
+    if (max_slots <= 4) f.template operator()<4, 1>();
+    else if (max_slots <= 6) f.template operator()<6, 1>();
+    else f.template operator()<8, 1>();
+  }
 }
 
-template <typename real>
-static void launch_depth_tw(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
-                            const ima_item *items, const HkyPlan &p, double *out,
-                            int32_t *status, const TwLaunch &tw) {
-  if (D <= 4)
-    hipLaunchKernelGGL((hky_tw_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status, tw);
-  else if (D <= 6)
-    hipLaunchKernelGGL((hky_tw_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status, tw);
-  else
-    hipLaunchKernelGGL((hky_tw_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, p.units,
-                       p.partial, out, status, tw);
-}
+// The three launchers' kernels for one (depth class, patterns per lane).
+struct ListLaunch {
+  dim3 g;
+  size_t lds;
+  hipStream_t st;
+  int precision;
+  const DeviceView *v;
+  const ima_item *items;
+  const int4 *list;
+  const int32_t *cnt;
+  int32_t *cnt_next;
+  double *partial, *out;
+  int32_t *status;
+  template <int D, int PPL>
+  void operator()() const {
+    if (precision == IMA_PREC_FP32)
+      hipLaunchKernelGGL((hky_list_kernel<float, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         list, cnt, cnt_next, partial, out, status);
+    else
+      hipLaunchKernelGGL((hky_list_kernel<double, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         list, cnt, cnt_next, partial, out, status);
+  }
+};
 
-template <typename real>
-static void launch_list_depth(int D, dim3 g, size_t lds, hipStream_t st, const DeviceView &v,
-                              const ima_item *items, const int4 *list, const int32_t *cnt,
-                              int32_t *cnt_next, double *partial, double *out,
-                              int32_t *status) {
-  if (D <= 4)
-    hipLaunchKernelGGL((hky_list_kernel<real, 4>), g, dim3(kWave), lds, st, v, items, list, cnt,
-                       cnt_next, partial, out, status);
-  else if (D <= 6)
-    hipLaunchKernelGGL((hky_list_kernel<real, 6>), g, dim3(kWave), lds, st, v, items, list, cnt,
-                       cnt_next, partial, out, status);
-  else
-    hipLaunchKernelGGL((hky_list_kernel<real, 8>), g, dim3(kWave), lds, st, v, items, list, cnt,
-                       cnt_next, partial, out, status);
-}
+struct TwLaunchFn {
+  dim3 g;
+  size_t lds;
+  hipStream_t st;
+  int precision;
+  const DeviceView *v;
+  const ima_item *items;
+  const HkyPlan *plan;
+  double *out;
+  int32_t *status;
+  const TwLaunch *tw;
+  template <int D, int PPL>
+  void operator()() const {
+    if (precision == IMA_PREC_FP32)
+      hipLaunchKernelGGL((hky_tw_kernel<float, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         plan->units, plan->partial, out, status, *tw);
+    else
+      hipLaunchKernelGGL((hky_tw_kernel<double, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         plan->units, plan->partial, out, status, *tw);
+  }
+};
+
+struct PlainLaunch {
+  dim3 g;
+  size_t lds;
+  hipStream_t st;
+  int precision;
+  const DeviceView *v;
+  const ima_item *items;
+  const HkyPlan *plan;
+  double *out;
+  int32_t *status;
+  template <int D, int PPL>
+  void operator()() const {
+    if (precision == IMA_PREC_FP32)
+      hipLaunchKernelGGL((hky_kernel<float, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         plan->units, plan->partial, out, status);
+    else
+      hipLaunchKernelGGL((hky_kernel<double, D, PPL>), g, dim3(kWave), lds, st, *v, items,
+                         plan->units, plan->partial, out, status);
+  }
+};
 
 hipError_t launch_hky_list(const DeviceView &v, const ima_item *items, const int4 *list,
                            const int32_t *cnt, int32_t *cnt_next, int grid, int precision,
-                           int max_n, int max_slots, double *partial, double *out,
+                           int max_n, int max_slots, int ppl, double *partial, double *out,
                            int32_t *status, hipStream_t st) {
-  if (grid <= 0 || max_slots > 8) return hipErrorInvalidValue;
-  const size_t lds = hky_lds_bytes(precision, max_n, max_slots);
-  if (precision == IMA_PREC_FP32)
-    launch_list_depth<float>(max_slots, dim3(grid), lds, st, v, items, list, cnt, cnt_next,
-                             partial, out, status);
-  else
-    launch_list_depth<double>(max_slots, dim3(grid), lds, st, v, items, list, cnt, cnt_next,
-                              partial, out, status);
+  if (grid <= 0 || max_slots > 8 || (ppl != 1 && ppl != 2)) return hipErrorInvalidValue;
+  const size_t lds = hky_lds_bytes(precision, max_n, max_slots, ppl);
+  by_shape(max_slots, ppl,
+           ListLaunch{dim3(grid), lds, st, precision, &v, items, list, cnt, cnt_next, partial,
+                      out, status});
   return hipGetLastError();
 }
 
-size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int P, int emax) {
-  return std::max(hky_lds_bytes(precision, max_n, max_slots), treeweight_lds_bytes(P, emax));
+size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int ppl, int P, int emax) {
+  return std::max(hky_lds_bytes(precision, max_n, max_slots, ppl),
+                  treeweight_lds_bytes(P, emax));
 }
 
 hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                          int precision, int max_n, int max_slots, double *out, int32_t *status,
                          const TwLaunch &tw, int P, hipStream_t st) {
   if (plan.nunits == 0 || tw.n == 0) return hipErrorInvalidValue;
-  if (max_slots > 8 || plan.nred > 0) return hipErrorInvalidValue;
-  const size_t lds = hky_tw_lds_bytes(precision, max_n, max_slots, P, tw.emax);
+  if (max_slots > 8 || plan.nred > 0 || (plan.ppl != 1 && plan.ppl != 2))
+    return hipErrorInvalidValue;
+  const size_t lds = hky_tw_lds_bytes(precision, max_n, max_slots, plan.ppl, P, tw.emax);
   if (lds > kTwLdsBudget) return hipErrorInvalidValue;
-  const dim3 g(plan.nunits + tw.n);
-  if (precision == IMA_PREC_FP32)
-    launch_depth_tw<float>(max_slots, g, lds, st, v, items, plan, out, status, tw);
-  else
-    launch_depth_tw<double>(max_slots, g, lds, st, v, items, plan, out, status, tw);
+  by_shape(max_slots, plan.ppl,
+           TwLaunchFn{dim3(plan.nunits + tw.n), lds, st, precision, &v, items, &plan, out,
+                      status, &tw});
   return hipGetLastError();
 }
 
@@ -489,12 +582,10 @@ hipError_t launch_hky(const DeviceView &v, const ima_item *items, const HkyPlan 
                       int precision, int max_n, int max_slots, double *out, int32_t *status,
                       hipStream_t st) {
   if (plan.nunits == 0) return hipSuccess;
-  if (max_slots > 8) return hipErrorInvalidValue;
-  const size_t lds = hky_lds_bytes(precision, max_n, max_slots);
-  if (precision == IMA_PREC_FP32)
-    launch_depth<float>(max_slots, dim3(plan.nunits), lds, st, v, items, plan, out, status);
-  else
-    launch_depth<double>(max_slots, dim3(plan.nunits), lds, st, v, items, plan, out, status);
+  if (max_slots > 8 || (plan.ppl != 1 && plan.ppl != 2)) return hipErrorInvalidValue;
+  const size_t lds = hky_lds_bytes(precision, max_n, max_slots, plan.ppl);
+  by_shape(max_slots, plan.ppl,
+           PlainLaunch{dim3(plan.nunits), lds, st, precision, &v, items, &plan, out, status});
   if (plan.nred > 0)
     hipLaunchKernelGGL(hky_reduce, dim3((plan.nred + 255) / 256), dim3(256), 0, st, plan.red,
                        plan.nred, plan.partial, out, status);

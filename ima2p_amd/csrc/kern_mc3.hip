@@ -1237,7 +1237,8 @@ hipError_t enqueue_mc3_score(const DeviceView &v, const MC3Dev &m, const ima_ite
   const bool prior = m.prior >= 0;
   const TwLaunch tw{tw_slots, m.C * m.L, m.emax, m.tv, m.tv_row, m.wnew, m.wstatus};
   hipError_t e = hipSuccess;
-  if (prior && nh && hky_tw_lds_bytes(precision, hmax_n, hmax_slots, m.P, m.emax) <= kTwLdsBudget) {
+  if (prior && nh &&
+      hky_tw_lds_bytes(precision, hmax_n, hmax_slots, p.ppl, m.P, m.emax) <= kTwLdsBudget) {
     // IS and SW items, if any, are scored on the side stream beside it
     const bool fork = ni || ns;
     if (score_ev) e = hipEventRecord(score_ev[2 * pass], st);

@@ -231,7 +231,7 @@ hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URate
     if (nh)
       e = launch_hky_list(v, u.uitems, u.list[lev & 1], u.cnt + (lev & 1),
                           u.cnt + ((lev + 1) & 1), plan.nunits, precision, hmax_n, hmax_slots,
-                          m.partial, m.newpdg, m.status, st);
+                          plan.ppl, m.partial, m.newpdg, m.status, st);
     if (e == hipSuccess && ni)
       e = launch_is(v, u.uitems, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
                     nullptr, st);

@@ -516,6 +516,15 @@ class Engine:
                                         ms.ctypes.data_as(L.dp)), "mc3_tune")
         return best.value, ms
 
+    def set_site_block(self, patterns):
+        """HKY site-block size: 0 automatic, 64 or 128 patterns per unit."""
+        self._chk(self.lib.ima_set_site_block(self.h, int(patterns)), "set_site_block")
+
+    def mc3_site_block(self):
+        out = C.c_int()
+        self._chk(self.lib.ima_mc3_site_block(self.h, C.byref(out)), "mc3_site_block")
+        return out.value
+
     def mc3_read_rates(self):
         """Each chain's mutation-rate scalars and kappa, [nchains, nloci] each."""
         C_, L_ = self.mc3_shape

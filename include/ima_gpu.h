@@ -537,8 +537,18 @@ int ima_mc3_prior_group(ima_ctx *ctx, int *loci);
  * flags the caller will use (IMA_MC3_GRAPH is implied, IMA_MC3_TIMED and
  * IMA_MC3_U_SERIAL are ignored): the group size is timed on those sweeps.
  * ms_per_sweep (may be NULL, 6 entries) receives each size's time, -1
- * where it was not tried. */
+ * where it was not tried.  At the chosen size it then times the HKY site
+ * blocks of 64 and 128 patterns the same way and keeps the faster, unless
+ * ima_set_site_block fixed the size (ima_mc3_site_block reads it). */
 int ima_mc3_tune(ima_ctx *ctx, int sweeps, int flags, int *best, double *ms_per_sweep);
+/* HKY site-block size: every HKY work unit (one wavefront) covers 64
+ * patterns, one per lane, or 128, two per lane, which share the program,
+ * coefficient and tip reads.  0 (the default) uses 64 for ima_score and
+ * ima_mc3_setup, and lets ima_mc3_tune time both on the chains' sweeps and
+ * keep the faster; 64 or 128 fixes the size for the calls that follow.
+ * ima_mc3_site_block reads the size the chains' sweeps use. */
+int ima_set_site_block(ima_ctx *ctx, int patterns);
+int ima_mc3_site_block(ima_ctx *ctx, int *patterns);
 /* Decisions for genealogy proposals made by IMa2p's own updategenealogy
  * (update_gtree.cpp:723-966: slider / joinsisdown / splitsisdown /
  * addmigration on the host).  The host wrote chain c's proposal at locus l

@@ -31,12 +31,13 @@ from ima2p_amd.engine import Engine
 pytestmark = pytest.mark.gpu
 
 
-def _three_pop_chains(C, Lc, seed, prec):
+def _three_pop_chains(C, Lc, seed, prec, site_block=0):
     rng = np.random.default_rng(seed)
     pt, theta, ts, mig = sim.im_model_three_pop(rng)
     model = immodel.build(pt, qmax=20.0, mmax=5.0)
     samp = [5, 4, 6]
     e = Engine(0)
+    e.set_site_block(site_block)
     mid = e.model_add(model)
     locs, lids = [], []
     for _ in range(Lc):
