@@ -1387,6 +1387,10 @@ static void mc3_release(ima_ctx *c) {
   release(m.uloff);
   release(m.ulast);
   release(m.ucnt);
+  release(m.uilist0);
+  release(m.uilist1);
+  release(m.uslist0);
+  release(m.uslist1);
   release(m.hast);
   release(m.uhost);
   release(m.idec);
@@ -1568,8 +1572,12 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     HIPCHK(c, reserve(m.ulast, N));
     HIPCHK(c, reserve(m.ulist0, units64 + 1));
     HIPCHK(c, reserve(m.ulist1, units64 + 1));
-    HIPCHK(c, reserve(m.ucnt, 2));
-    HIPCHK(c, hipMemsetAsync(m.ucnt.p, 0, 2 * sizeof(int32_t), c->stream));
+    HIPCHK(c, reserve(m.uilist0, is.size() + 1));
+    HIPCHK(c, reserve(m.uilist1, is.size() + 1));
+    HIPCHK(c, reserve(m.uslist0, sw.size() + 1));
+    HIPCHK(c, reserve(m.uslist1, sw.size() + 1));
+    HIPCHK(c, reserve(m.ucnt, 6));
+    HIPCHK(c, hipMemsetAsync(m.ucnt.p, 0, 6 * sizeof(int32_t), c->stream));
     if (urate_plan_lds(nloci, J) > 65536)
       return fail(c, IMA_ERR_ARG, "%d loci per chain: the mutation-rate plan needs more LDS",
                   nloci);
@@ -1583,6 +1591,10 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
     u.last = m.ulast.p;
     u.list[0] = m.ulist0.p;
     u.list[1] = m.ulist1.p;
+    u.ilist[0] = m.uilist0.p;
+    u.ilist[1] = m.uilist1.p;
+    u.slist[0] = m.uslist0.p;
+    u.slist[1] = m.uslist1.p;
     u.cnt = m.ucnt.p;
     u.win = log(10000.0) / nloci;
     u.maxr = 3.0 * log(10000.0);

@@ -136,7 +136,7 @@ struct ima_ctx {
     DevBuf<ima_item> uitems;
     DevBuf<int4> ustep, ulist0, ulist1;
     DevBuf<double4> uuni;
-    DevBuf<int32_t> uorder, uloff, ulast, ucnt;
+    DevBuf<int32_t> uorder, uloff, ulast, ucnt, uilist0, uilist1, uslist0, uslist1;
     std::vector<int32_t> ulevel_scratch;  // host copy of the level bookkeeping
     // ima_mc3_decide: Hastings terms, uniforms, decisions, proposal items
     DevBuf<double> hast, uhost;

@@ -198,16 +198,21 @@ size_t hky_tw_lds_bytes(int precision, int max_n, int max_slots, int ppl, int P,
 hipError_t launch_hky_tw(const DeviceView &v, const ima_item *items, const HkyPlan &plan,
                          int precision, int max_n, int max_slots, double *out, int32_t *status,
                          const TwLaunch &tw, int P, hipStream_t st);
+// With cnt, item_idx is a list of which only the first *cnt entries are
+// scored (nitems is the list's capacity, the grid), and block 0 clears
+// *cnt_next, the list filled for the next launch.
 hipError_t launch_is(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, int max_n, int max_w,
                      double *out, int32_t *status, int32_t *site_node,
-                     int32_t *counts, hipStream_t st);
+                     int32_t *counts, hipStream_t st, const int32_t *cnt = nullptr,
+                     int32_t *cnt_next = nullptr);
 hipError_t launch_sw_update(const DeviceView &v, int32_t *A, const ima_sw_move *moves,
                             int nmoves, int pass, int calc, double *out, int32_t *iout,
                             hipStream_t st);
 hipError_t launch_sw(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, double *out,
-                     int32_t *status, double *dlike, hipStream_t st);
+                     int32_t *status, double *dlike, hipStream_t st,
+                     const int32_t *cnt = nullptr, int32_t *cnt_next = nullptr);
 
 // L mode (kern_lmode.hip).
 struct LTable {
@@ -385,7 +390,9 @@ struct URateDev {
   int32_t *loff;        // [C*(J+2)] start of each level in order[] (levels from 1)
   int32_t *last;        // [C*L] plan scratch: highest level touching each locus
   int4 *list[2];        // HKY units of a level's proposals, by level parity
-  int32_t *cnt;         // [2] units in list[0], list[1]
+  int32_t *ilist[2];    // IS items of a level's proposals, by level parity
+  int32_t *slist[2];    // SW items of a level's proposals, by level parity
+  int32_t *cnt;         // [6] entries in list[0..1], ilist[0..1], slist[0..1]
   double win, maxr;     // window and reflection bound of log(u_j / u_k)
   double kwin, kmax;    // kappa window and upper bound
 };

@@ -43,9 +43,13 @@ __global__ __launch_bounds__(kWave) void is_kernel(
     DeviceView v, const ima_item *__restrict__ items,
     const int32_t *__restrict__ idx, double *__restrict__ out,
     int32_t *__restrict__ status, int32_t *__restrict__ site_node,
-    int32_t *__restrict__ counts) {
+    int32_t *__restrict__ counts, const int32_t *__restrict__ list_cnt, int32_t *cnt_next) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
+  if (list_cnt) {
+    if (blockIdx.x == 0 && lane == 0) *cnt_next = 0;
+    if ((int)blockIdx.x >= *list_cnt) return;
+  }
   const int ii = idx[blockIdx.x];
   const ima_item item = items[ii];
   const SlotMeta sm = v.slots[item.slot];
@@ -154,22 +158,23 @@ __global__ __launch_bounds__(kWave) void is_kernel(
 hipError_t launch_is(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, int max_n, int max_w,
                      double *out, int32_t *status, int32_t *site_node,
-                     int32_t *counts, hipStream_t st) {
+                     int32_t *counts, hipStream_t st, const int32_t *cnt,
+                     int32_t *cnt_next) {
   if (nitems == 0) return hipSuccess;
   const int ne = 2 * max_n - 1;
   const size_t lds = (size_t)ne * max_w * 8 + (size_t)ne * 8 + (size_t)ne * 4 * 4;
   switch (max_w) {
     case 1:
       hipLaunchKernelGGL(is_kernel<1>, dim3(nitems), dim3(kWave), lds, st, v, items,
-                         item_idx, out, status, site_node, counts);
+                         item_idx, out, status, site_node, counts, cnt, cnt_next);
       break;
     case 2:
       hipLaunchKernelGGL(is_kernel<2>, dim3(nitems), dim3(kWave), lds, st, v, items,
-                         item_idx, out, status, site_node, counts);
+                         item_idx, out, status, site_node, counts, cnt, cnt_next);
       break;
     default:
       hipLaunchKernelGGL(is_kernel<4>, dim3(nitems), dim3(kWave), lds, st, v, items,
-                         item_idx, out, status, site_node, counts);
+                         item_idx, out, status, site_node, counts, cnt, cnt_next);
       break;
   }
   return hipGetLastError();
@@ -241,8 +246,13 @@ __device__ double d_bessi(int n, double x) {
 __global__ __launch_bounds__(kWave) void sw_kernel(
     DeviceView v, const ima_item *__restrict__ items,
     const int32_t *__restrict__ idx, double *__restrict__ out,
-    int32_t *__restrict__ status, double *__restrict__ dlike) {
+    int32_t *__restrict__ status, double *__restrict__ dlike, const int32_t *__restrict__ cnt,
+    int32_t *cnt_next) {
   const int lane = threadIdx.x;
+  if (cnt) {
+    if (blockIdx.x == 0 && lane == 0) *cnt_next = 0;
+    if ((int)blockIdx.x >= *cnt) return;
+  }
   const int ii = idx[blockIdx.x];
   const ima_item item = items[ii];
   const SlotMeta sm = v.slots[item.slot];
@@ -364,10 +374,11 @@ hipError_t launch_sw_update(const DeviceView &v, int32_t *A, const ima_sw_move *
 
 hipError_t launch_sw(const DeviceView &v, const ima_item *items,
                      const int32_t *item_idx, int nitems, double *out,
-                     int32_t *status, double *dlike, hipStream_t st) {
+                     int32_t *status, double *dlike, hipStream_t st, const int32_t *cnt,
+                     int32_t *cnt_next) {
   if (nitems == 0) return hipSuccess;
   hipLaunchKernelGGL(sw_kernel, dim3(nitems), dim3(kWave), 0, st, v, items, item_idx,
-                     out, status, dlike);
+                     out, status, dlike, cnt, cnt_next);
   return hipGetLastError();
 }
 

@@ -15,9 +15,9 @@
 // either of its loci.  The steps of one level touch disjoint loci and
 // read the rates the sequential loop would have left, so running a
 // level's steps together gives the sequential loop's result.  Per level:
-// one scoring launch over the listed HKY units (IS and SW items are
-// scored whole) and one urate_step launch that decides the level and
-// proposes the next.
+// scoring launches over the listed HKY units and the listed IS and SW
+// items, and one urate_step launch that decides the level and proposes the
+// next.
 
 #include <float.h>
 
@@ -28,6 +28,21 @@ namespace {
 
 __device__ inline bool item_is_hky(const DeviceView &v, const MC3Dev &m, int it) {
   return v.loci[v.slots[m.first_slot + it].locus].model == IMA_MODEL_HKY;
+}
+
+// Queue item it for the level of list parity lp: its HKY pattern-block
+// units, or the item itself on the IS or SW list.
+__device__ inline void queue_units(const MC3Dev &m, const URateDev &u, int it, int lp);
+__device__ inline void queue_item(const DeviceView &v, const MC3Dev &m, const URateDev &u,
+                                  int it, int lp) {
+  const int model = v.loci[v.slots[m.first_slot + it].locus].model;
+  if (model == IMA_MODEL_HKY) {
+    queue_units(m, u, it, lp);
+  } else if (model == IMA_MODEL_IS) {
+    u.ilist[lp][atomicAdd(&u.cnt[2 + lp], 1)] = it;
+  } else {
+    u.slist[lp][atomicAdd(&u.cnt[4 + lp], 1)] = it;
+  }
 }
 
 // Log-likelihood of item it from this level's scoring: the direct value, or
@@ -83,17 +98,13 @@ __device__ void urate_propose(const DeviceView &v, const MC3Dev &m, const URateD
     const double d = exp((nr - r) / 2);
     a.u *= d;
     b.u /= d;
-    if (item_is_hky(v, m, ik)) {
-      b.kappa = kappa_move(b.kappa, U.z, u);
-      queue_units(m, u, ik, lp);
-    }
+    if (item_is_hky(v, m, ik)) b.kappa = kappa_move(b.kappa, U.z, u);
     u.uitems[ik] = b;
+    queue_item(v, m, u, ik, lp);
   }
-  if (item_is_hky(v, m, ij)) {
-    a.kappa = kappa_move(a.kappa, U.y, u);
-    queue_units(m, u, ij, lp);
-  }
+  if (item_is_hky(v, m, ij)) a.kappa = kappa_move(a.kappa, U.y, u);
   u.uitems[ij] = a;
+  queue_item(v, m, u, ij, lp);
 }
 
 // Decision of step s of chain c (update_mc_params.cpp:250-287 and 406-409):
@@ -203,7 +214,7 @@ __global__ __launch_bounds__(256) void urate_step(DeviceView v, MC3Dev m, URateD
     urate_decide(m, u, items, c, ord[i]);
   if (last) {
     // the next round's plan queues level 1 on list 1 (nothing queues now)
-    if (c == 0 && threadIdx.x < 2) u.cnt[threadIdx.x] = 0;
+    if (c == 0 && threadIdx.x < 6) u.cnt[threadIdx.x] = 0;
     return;
   }
   __syncthreads();  // the level's commits visible to the next level's proposals
@@ -233,10 +244,11 @@ hipError_t enqueue_urate_round(const DeviceView &v, const MC3Dev &m, const URate
                           u.cnt + ((lev + 1) & 1), plan.nunits, precision, hmax_n, hmax_slots,
                           plan.ppl, m.partial, m.newpdg, m.status, st);
     if (e == hipSuccess && ni)
-      e = launch_is(v, u.uitems, idx + nh, ni, imax_n, imax_w, m.newpdg, m.status, nullptr,
-                    nullptr, st);
+      e = launch_is(v, u.uitems, u.ilist[lev & 1], ni, imax_n, imax_w, m.newpdg, m.status,
+                    nullptr, nullptr, st, u.cnt + 2 + (lev & 1), u.cnt + 2 + ((lev + 1) & 1));
     if (e == hipSuccess && ns)
-      e = launch_sw(v, u.uitems, idx + nh + ni, ns, m.newpdg, m.status, nullptr, st);
+      e = launch_sw(v, u.uitems, u.slist[lev & 1], ns, m.newpdg, m.status, nullptr, st,
+                    u.cnt + 4 + (lev & 1), u.cnt + 4 + ((lev + 1) & 1));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(urate_step, dim3(m.C), dim3(256), 0, st, v, m, u, it, lev,
                        lev == nlev ? 1 : 0);
