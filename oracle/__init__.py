@@ -98,7 +98,15 @@ def _load():
                                                 C.c_double] + [ip] * 4
                                   + [dp, ip, ip, ip, dp, ip, dp, dp, C.c_double, C.c_uint64,
                                      C.c_double, C.c_long, C.POINTER(C.c_long), C.c_int,
-                                     C.c_int, C.c_int, ip])
+                                     C.c_int, C.c_int, ip, C.c_int, ip])
+    lib.ora_getmprob_nw.restype = C.c_double
+    lib.ora_getmprob_nw.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double] + [C.c_int] * 5
+    lib.ora_poisson_cond.restype = C.c_int
+    lib.ora_poisson_cond.argtypes = [C.c_double, C.c_int, C.c_uint64, C.c_uint64]
+    lib.ora_update_mig_tnw.restype = C.c_int
+    lib.ora_update_mig_tnw.argtypes = ([C.c_int, ip, ip, ip, dp, C.c_int, ip, ip, C.c_int, ip, dp,
+                                        ip, C.c_int] + [ip] * 5
+                                       + [dp, C.c_int, C.c_double, C.c_uint64, C.c_uint64, dp, ip])
     return lib
 
 
@@ -550,9 +558,56 @@ def updategenealogy_decisions(pdg, weights, new_pdg, new_weights, ok, hastings, 
     return dec, pdg, weights, probg
 
 
+def getmprob_nw(period, lastp, mrate, mtime, mcount, uppop, dpop, cm2pop, numpops):
+    return lib.ora_getmprob_nw(int(period), int(lastp), float(mrate), float(mtime), int(mcount),
+                               int(uppop), int(dpop), int(cm2pop), int(numpops))
+
+
+def poisson_cond(param, cond, key, ctr=0):
+    return lib.ora_poisson_cond(float(param), int(cond), int(key) & (2**64 - 1), int(ctr))
+
+
+def update_mig_tnw(g, model, tv, period, newt, key, ctr=0, cap=None):
+    """changet_NW's migration update of Genealogy g (update_mig_tNW,
+    update_t_NW.cpp:392-792) for split `period` moving from tv[period] to
+    newt, edges drawing from Philox streams keyed by (key, edge) at counter
+    ctr.  Returns (rc, updated, migweight, new Genealogy); the input is left
+    as it is.  cap bounds the new migration count (default: ample)."""
+    from ima2p_amd.immodel import mig_arrays
+    me, mt, mo = mig_arrays(g)
+    nm = len(me)
+    if cap is None:
+        cap = 4 * nm + 4096
+    ne = 2 * g.n - 1
+    me_ = np.zeros(cap + 1, np.int32)
+    mt_ = np.zeros(cap + 1, np.float64)
+    mo_ = np.zeros(cap + 1, np.int32)
+    me_[:nm], mt_[:nm], mo_[:nm] = me, mt, mo
+    pop = np.ascontiguousarray(g.pop, np.int32).copy()
+    nmig = np.array([nm], np.int32)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in
+            (model.plist, model.npresent, model.popdown, model.pope, model.join)]
+    w = np.zeros(1)
+    upd = np.zeros(1, np.int32)
+    up0, up1, dn = (np.ascontiguousarray(a, np.int32) for a in (g.up0, g.up1, g.down))
+    tm = np.ascontiguousarray(g.time, np.float64)
+    tvv = np.ascontiguousarray(tv, np.float64)
+    rc = lib.ora_update_mig_tnw(g.n, _ip(up0), _ip(up1), _ip(dn), _dp(tm), int(g.root), _ip(pop),
+                                _ip(nmig), int(cap), _ip(me_), _dp(mt_), _ip(mo_), model.npops,
+                                *[_ip(a) for a in arrs], _dp(tvv), int(period), float(newt),
+                                int(key) & (2**64 - 1), int(ctr), _dp(w), _ip(upd))
+    out = g.copy()
+    if rc == 0 and upd[0]:
+        out.pop = pop.copy()
+        out.mig = [[] for _ in range(ne)]
+        for j in range(int(nmig[0])):
+            out.mig[int(me_[j])].append((float(mt_[j]), int(mo_[j])))
+    return int(rc), int(upd[0]), float(w[0]), out
+
+
 def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
               seconds=1.0, max_sweeps=0, scale_width=0.2, ry1=False, urate=False,
-              rates=None, topo=False):
+              rates=None, topo=False, nw=0, mig_room=256):
     """The CPU baseline's C sweep over chains[c][l] (Genealogy objects with
     population labels and migrations; updated in place) with split times
     tvs[c] (a float array [nchains, P-1], updated in place).  ry1: the
@@ -562,7 +617,12 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     chain; rates=(u, kappa), two float64 [nchains, nloci] arrays, take their
     place and receive the final values.  topo: the genealogy update is the
     reference's own slider move (models without migration; topologies,
-    labels and roots of the Genealogy objects are updated).  Returns
+    labels and roots of the Genealogy objects are updated).  nw (models
+    with migration): 1, each time update is changet_NW with probability 1/2
+    (the reference's choice between RY1 and NW); 2, every sweep is one NW
+    update per chain and nothing else; populations and migrations of the
+    Genealogy objects are updated, each genealogy holding at most its
+    migration count plus max(mig_room, that count).  Returns
     (sweeps, stats[6] =
     genealogy proposals, accepts, time-update proposals, accepts,
     mutation-rate steps, accepts)."""
@@ -599,9 +659,20 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     roots = np.array([g.root for g in gens], np.int32)
     migs = [mig_arrays(g) for g in gens]
     nm = np.array([len(m[0]) for m in migs], np.int32)
-    me = np.concatenate([m[0] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
-    mt = np.concatenate([m[1] for m in migs] + [np.zeros(1)]).astype(np.float64)
-    mo = np.concatenate([m[2] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
+    if nw:
+        caps = np.array([k + max(int(mig_room), k) for k in nm], np.int32)
+        room = [np.zeros(int(c - k), np.int32) for c, k in zip(caps, nm)]
+        me = np.concatenate([x for m, r in zip(migs, room) for x in (m[0], r)]
+                            + [np.zeros(1, np.int32)]).astype(np.int32)
+        mt = np.concatenate([x for m, r in zip(migs, room) for x in (m[1], r.astype(float))]
+                            + [np.zeros(1)]).astype(np.float64)
+        mo = np.concatenate([x for m, r in zip(migs, room) for x in (m[2], r)]
+                            + [np.zeros(1, np.int32)]).astype(np.int32)
+    else:
+        caps = None
+        me = np.concatenate([m[0] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
+        mt = np.concatenate([m[1] for m in migs] + [np.zeros(1)]).astype(np.float64)
+        mo = np.concatenate([m[2] for m in migs] + [np.zeros(1, np.int32)]).astype(np.int32)
     tv = tvs if P > 1 else np.zeros(1)
     assert P == 1 or (tv.dtype == np.float64 and tv.flags.c_contiguous and tv.size == len(chains) * (P - 1))
     b = np.ascontiguousarray(beta, float)
@@ -618,7 +689,8 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
                            _ip(up0), _ip(up1), _ip(dn), _ip(pop), _dp(tm), _ip(roots), _ip(nm),
                            _ip(me), _dp(mt), _ip(mo), _dp(tv), _dp(b), float(scale_width),
                            int(seed), float(seconds), int(max_sweeps), stats, int(bool(ry1)),
-                           int(bool(urate)), int(bool(topo)), _ip(roots_out))
+                           int(bool(urate)), int(bool(topo)), _ip(roots_out), int(nw),
+                           _ip(caps) if caps is not None else None)
     if sw < 0:
         raise ValueError("mc3_bench: a model with migration has no topology move here")
     off = moff_ = 0
@@ -631,6 +703,15 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
             g.down[:] = dn[off:off + ne]
             g.pop[:] = pop[off:off + ne]
             g.root = int(roots_out[gi])
+        if nw:
+            g.pop[:] = pop[off:off + ne]
+            g.mig = [[] for _ in range(ne)]
+            for j in range(int(nm[gi])):
+                q = moff_ + j
+                g.mig[int(me[q])].append((float(mt[q]), int(mo[q])))
+            off += ne
+            moff_ += int(caps[gi])
+            continue
         off += ne
         k = len(m[0])
         times = mt[moff_:moff_ + k]

@@ -164,6 +164,32 @@ int ora_treeweight(const ora_tree *t, const int *pop, int nmig, const int *mig_e
                    const int *samppop, double hval, int *cc, double *fc, double *hcc,
                    int *mc, double *fm, double *length);
 
+/* ---------------- changet_NW's migration update (ora_nw.c) ------------- */
+/* One genealogy (tree arrays of 2n-1 edges; pop and the packed migrations,
+ * sorted by edge then time, updated in place, at most cap events) for the
+ * move of split `period` from tv[period] to newt.  Edges draw from Philox
+ * streams keyed by key and the edge, counters from ctr.  Returns 0 (updated
+ * is 0 when the genealogy lies below the interval and stays as it is), 1
+ * when the new migrations would exceed cap, 2 for a node exactly at the
+ * interval's lower end, -1 for bad arguments; nothing is written unless
+ * updated is 1.  migweight receives the Hastings term of the migrations. */
+int ora_update_mig_tnw(int n, const int *up0, const int *up1, const int *down,
+                       const double *time, int root, int *pop, int *nmig, int cap,
+                       int *mig_edge, double *mig_time, int *mig_to, int P, const int *plist,
+                       const int *npresent, const int *popdown, const int *pope,
+                       const int *join, const double *tv, int period, double newt,
+                       uint64_t key, uint64_t ctr, double *migweight, int *updated);
+
+/* getmprob_NW: log probability of the simulated path of one edge part
+ * (period, rate, length, count, populations at its ends, population before
+ * the second-to-last migration, populations of the period, last period
+ * lastp) */
+double ora_getmprob_nw(int period, int lastp, double mrate, double mtime, int mcount, int uppop,
+                       int dpop, int cm2pop, int numpops);
+/* poisson() with its conditions (0 even, 1 odd, 2 not 0, 3 not 1) on the
+ * Philox stream (key, ctr) */
+int ora_poisson_cond(double param, int cond, uint64_t key, uint64_t ctr);
+
 /* ---------------- host MC3 sweep of the CPU baseline (ora_mc3.c) -------- */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,
@@ -172,12 +198,12 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    const int *popdown, const int *pope, const int *join, int npsize,
                    const int *theta_off, const int *theta_wp, int nmigp, const int *mig_off,
                    const int *mig_wp, const double *qmax, const double *mmax, double tmax,
-                   const int *up0, const int *up1, const int *down, const int *pop,
-                   double *times, const int *roots, const int *nmig, const int *mig_edge,
-                   double *mig_time, const int *mig_to, double *tvs,
+                   const int *up0, const int *up1, const int *down, int *pop,
+                   double *times, const int *roots, int *nmig, int *mig_edge,
+                   double *mig_time, int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
                    long max_sweeps, long *stats, int ry1, int urate, int topo,
-                   int *roots_out);
+                   int *roots_out, int nw, const int *mig_cap);
 
 #ifdef __cplusplus
 }

@@ -61,7 +61,7 @@ typedef struct { /* one genealogy of one chain */
   int n, root;
   int *up0, *up1, *down, *pop;
   double *time;
-  int nmig, *mig_edge, *mig_to;
+  int nmig, mig_cap, *mig_edge, *mig_to;
   double *mig_time;
   ora_hky_state st;
   double u, kappa; /* mutation-rate scalar and kappa of this chain's locus */
@@ -83,6 +83,7 @@ typedef struct {
   double tmax, scale_width;
   int ry1;
   int topo; /* the genealogy update is the reference's slider move */
+  int nw;   /* 1: time updates pick RY1 or NW; 2: NW alone, no genealogy update */
 } prob_t;
 
 static int rec_len(int P) { return 3 * P * P + 2 * P * P * P + 1; }
@@ -92,14 +93,14 @@ static ora_tree tree_of(const gene_t *g) {
   return t;
 }
 
-static void weights(const prob_t *pb, int l, const gene_t *g, const double *tv, double *rec) {
+static int weights(const prob_t *pb, int l, const gene_t *g, const double *tv, double *rec) {
   const int P = pb->P, P2 = P * P, P3 = P2 * P;
   int cc[100], mc[1000];
   double fc[100], hcc[100], fm[1000], len;
   ora_tree t = tree_of(g);
-  ora_treeweight(&t, g->pop, g->nmig, g->mig_edge, g->mig_time, g->mig_to, P, P, pb->plist,
-                 pb->npresent, pb->popdown, pb->pope, pb->join, tv, pb->samppop + l * P, 1.0,
-                 cc, fc, hcc, mc, fm, &len);
+  const int st = ora_treeweight(&t, g->pop, g->nmig, g->mig_edge, g->mig_time, g->mig_to, P, P,
+                                pb->plist, pb->npresent, pb->popdown, pb->pope, pb->join, tv,
+                                pb->samppop + l * P, 1.0, cc, fc, hcc, mc, fm, &len);
   for (int i = 0; i < P2; i++) {
     rec[i] = cc[i];
     rec[P2 + i] = fc[i];
@@ -110,6 +111,7 @@ static void weights(const prob_t *pb, int l, const gene_t *g, const double *tv, 
     rec[3 * P2 + P3 + i] = fm[i];
   }
   rec[3 * P2 + 2 * P3] = len;
+  return st;
 }
 
 /* integrals of every parameter at the sums `all`; terms whose (count,
@@ -589,6 +591,100 @@ static void ry1(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
   free(fresh);
 }
 
+/* changet_NW (update_t_NW.cpp:920-1032): split k and newt drawn as for
+ * RY1 (getnewt); every genealogy whose root lies above min(oldt, newt)
+ * gets its migrations in the interval between the two times re-simulated
+ * (ora_update_mig_tnw, one fresh 64-bit stream key per locus); node times
+ * and so the data likelihood do not change.  Accepted when U < min(1,
+ * exp(beta dprobg + migweight)); a genealogy whose new migrations exceed
+ * its capacity, or that treeweight rejects, rejects the move. */
+static void nw(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
+  const int L = pb->L, R = rec_len(pb->P), np_ = pb->npsize + pb->nmig, ns = pb->P - 1;
+  int k = (int)(unif(r) * ns);
+  if (k > ns - 1) k = ns - 1;
+  const double oldt = ch->tv[k];
+  const double t_u = k == 0 ? 0.0 : ch->tv[k - 1];
+  const double t_d = k == ns - 1 ? 1000000.0 : ch->tv[k + 1];
+  const double tdp = fmin(pb->tmax, t_d), tup = fmax(0.0, t_u);
+  const double twin = (tdp - tup) / (log((double)L + 1.0) * (double)(pb->P - k));
+  double newt = (oldt - twin / 2) + unif(r) * twin;
+  if (newt >= tdp)
+    newt = 2.0 * tdp - newt;
+  else if (newt <= tup)
+    newt = 2.0 * tup - newt;
+  double tv_new[16];
+  for (int x = 0; x < ns; x++) tv_new[x] = ch->tv[x];
+  tv_new[k] = newt;
+  double *recs = (double *)malloc(sizeof(double) * L * R);
+  int **bpop = (int **)calloc(L, sizeof(int *)), **bme = (int **)calloc(L, sizeof(int *));
+  int **bmo = (int **)calloc(L, sizeof(int *)), *bnm = (int *)calloc(L, sizeof(int));
+  double **bmt = (double **)calloc(L, sizeof(double *));
+  double mw = 0.0;
+  int ok = 1;
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    const int ne = 2 * g->n - 1, nm = g->nmig;
+    bpop[l] = (int *)malloc(sizeof(int) * ne);
+    memcpy(bpop[l], g->pop, sizeof(int) * ne);
+    bnm[l] = nm;
+    bme[l] = (int *)malloc(sizeof(int) * (nm + 1));
+    bmo[l] = (int *)malloc(sizeof(int) * (nm + 1));
+    bmt[l] = (double *)malloc(sizeof(double) * (nm + 1));
+    memcpy(bme[l], g->mig_edge, sizeof(int) * nm);
+    memcpy(bmo[l], g->mig_to, sizeof(int) * nm);
+    memcpy(bmt[l], g->mig_time, sizeof(double) * nm);
+    const uint64_t key = xoshiro(r);
+    double w = 0.0;
+    int upd = 0;
+    if (ora_update_mig_tnw(g->n, g->up0, g->up1, g->down, g->time, g->root, g->pop, &g->nmig,
+                           g->mig_cap, g->mig_edge, g->mig_time, g->mig_to, pb->P, pb->plist,
+                           pb->npresent, pb->popdown, pb->pope, pb->join, ch->tv, k, newt, key,
+                           0, &w, &upd) != 0)
+      ok = 0;
+    mw += w;
+    if (weights(pb, l, g, tv_new, recs + (size_t)l * R) != 0) ok = 0;
+  }
+  double *fresh = (double *)calloc(R, sizeof(double));
+  for (int l = 0; l < L; l++)
+    for (int i = 0; i < R; i++) fresh[i] += recs[(size_t)l * R + i];
+  const double pn = prior_terms(pb, fresh, NULL, NULL, NULL, ch->nc, ch->nf, ch->nt);
+  const double a = ch->beta * (pn - ch->probg) + mw;
+  const double U = unif(r);
+  const int dec = ok && U < fmin(1.0, exp(a));
+  for (int l = 0; l < L; l++) {
+    gene_t *g = &ch->g[l];
+    if (dec) {
+      memcpy(g->rec, recs + (size_t)l * R, sizeof(double) * R);
+    } else {
+      memcpy(g->pop, bpop[l], sizeof(int) * (2 * g->n - 1));
+      g->nmig = bnm[l];
+      memcpy(g->mig_edge, bme[l], sizeof(int) * bnm[l]);
+      memcpy(g->mig_to, bmo[l], sizeof(int) * bnm[l]);
+      memcpy(g->mig_time, bmt[l], sizeof(double) * bnm[l]);
+    }
+    free(bpop[l]);
+    free(bme[l]);
+    free(bmo[l]);
+    free(bmt[l]);
+  }
+  if (dec) {
+    memcpy(ch->all, fresh, sizeof(double) * R);
+    memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+    memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+    memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+    ch->probg = pn;
+    ch->tv[k] = newt;
+    (*acc)++;
+  }
+  free(bpop);
+  free(bme);
+  free(bmo);
+  free(bmt);
+  free(bnm);
+  free(recs);
+  free(fresh);
+}
+
 /* kappa move of an HKY locus (update_mc_params.cpp:258-273, :394-404) */
 static double kappa_move(double k, double U) {
   const double win = 2.0, kmax = 100.0; /* initialize.cpp:1499-1501, KAPPAMAX */
@@ -686,7 +782,11 @@ static void change_rates(const prob_t *pb, chain_t *ch, rng_t *r, long *tries, l
  * migration counts and their (edge, time, destination) packed in order.
  * tvs[c][0..P-2] are chain c's split times; times, migration times and
  * split times are updated in place.  ry1 selects the rubber-band time
- * update (two or more populations) in place of the global rescale.
+ * update (two or more populations) in place of the global rescale.  nw
+ * (models with migration): 1, each time update is NW with probability 1/2;
+ * 2, every sweep is one NW update per chain and nothing else.  With nw
+ * each genealogy's migrations sit in a region of mig_cap[c * L + l]
+ * entries (regions in the same order), and nmig and pop are written back.
  */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,
@@ -695,17 +795,18 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                    const int *popdown, const int *pope, const int *join, int npsize,
                    const int *theta_off, const int *theta_wp, int nmigp, const int *mig_off,
                    const int *mig_wp, const double *qmax, const double *mmax, double tmax,
-                   const int *up0, const int *up1, const int *down, const int *pop,
-                   double *times, const int *roots, const int *nmig, const int *mig_edge,
-                   double *mig_time, const int *mig_to, double *tvs,
+                   const int *up0, const int *up1, const int *down, int *pop,
+                   double *times, const int *roots, int *nmig, int *mig_edge,
+                   double *mig_time, int *mig_to, double *tvs,
                    const double *beta0, double scale_width, uint64_t seed, double seconds,
                    long max_sweeps, long *stats, int ry1_update, int urate_update,
-                   int topo_update, int *roots_out) {
+                   int topo_update, int *roots_out, int nw_update, const int *mig_cap) {
   prob_t pb = {L, S, totsites, seq_off, mult_off, samppop, seq, mult, pi,
                P, npsize, nmigp, plist, npresent, popdown, pope, join,
                theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width,
-               ry1_update && P > 1, topo_update};
+               ry1_update && P > 1, topo_update, (P > 1 && nmigp > 0) ? nw_update : 0};
   if (topo_update && nmigp > 0) return -1; /* no migration-path proposals */
+  if (pb.nw && !mig_cap) return -1;       /* new migrations need room */
   const int R = rec_len(P), np_ = npsize + nmigp;
   if (P > 10 || P - 1 > 16) return -1;
   chain_t *ch = (chain_t *)calloc(nchains, sizeof(chain_t));
@@ -736,11 +837,12 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
       g->pop = (int *)(pop + off);
       g->time = times + off;
       g->nmig = nmig[c * L + l];
-      g->mig_edge = (int *)(mig_edge + moff);
-      g->mig_to = (int *)(mig_to + moff);
+      g->mig_cap = mig_cap ? mig_cap[c * L + l] : g->nmig;
+      g->mig_edge = mig_edge + moff;
+      g->mig_to = mig_to + moff;
       g->mig_time = mig_time + moff;
       off += ne;
-      moff += g->nmig;
+      moff += g->mig_cap;
       g->u = u[c * L + l];
       g->kappa = kappa[c * L + l];
       g->st.n = n;
@@ -773,6 +875,11 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
   double el = 0.0;
   do {
     for (int c = 0; c < nchains; c++) {
+      if (pb.nw == 2) { /* the NW update alone */
+        nw(&pb, &ch[c], &r, &acc[1]);
+        props[1]++;
+        continue;
+      }
       for (int l = 0; l < L; l++) {
         if (pb.topo)
           topo_move(&pb, &ch[c], l, &r, &acc[0]);
@@ -781,7 +888,11 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
         props[0]++;
       }
       if (sweeps & 1) {
-        if (pb.ry1)
+        /* qupdate's time update: RY1 or NW with equal probability when
+         * the model has migration (ima_main_mpi.cpp:1867-1911) */
+        if (pb.nw == 1 && unif(&r) < 0.5)
+          nw(&pb, &ch[c], &r, &acc[1]);
+        else if (pb.ry1)
           ry1(&pb, &ch[c], &r, &acc[1]);
         else
           rescale(&pb, &ch[c], &r, &acc[1]);
@@ -830,6 +941,7 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
     for (int l = 0; l < L; l++) {
       gene_t *g = &ch[c].g[l];
       if (roots_out) roots_out[c * L + l] = g->root;
+      nmig[c * L + l] = g->nmig;
       u[c * L + l] = g->u;
       kappa[c * L + l] = g->kappa;
       free(g->st.frac);

@@ -1,0 +1,170 @@
+"""The oracle's restatement of changet_NW (oracle/ora_nw.c; update_t_NW.cpp:
+920-1032), pinned by what the mathematics fixes:
+
+* getmprob_NW is the log probability of the path simmpath_tNW draws, so
+  summed over every count and every destination sequence (times integrated
+  over the ordered simplex) it is 1, in the two-population period (counts
+  of one parity) and in periods of three and four populations; a dropped
+  (2 - count) log(np - 1) or a wrong second-to-last factor fails it;
+* the conditioned Poisson draws follow the conditioned Poisson law (the
+  reference's odd draw for small rates stops at 5);
+* an updated genealogy keeps its nodes, keeps every migration outside the
+  interval between the two split times, and fits the new split time
+  (treeweight accepts it);
+* a chain of NW updates started from exact draws of the joint prior of
+  genealogy and split times (parameters from their uniform priors, then
+  the structured coalescent) stays on that prior: split times and
+  migration counts after 40 updates against fresh draws (KS).  Dropping
+  the proposal's population-choice terms, flipping the Hastings term or
+  taking sinh for cosh in the two-population period moves those laws by
+  far more than the test allows.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle
+from ima2p_amd import immodel, sim
+
+
+def _prior_draw(rng, pt, model, samppop, Q, M, tmax):
+    ntree = 2 * pt.npops - 1
+    theta = rng.uniform(0, Q, ntree)
+    mig = np.zeros((ntree, ntree))
+    for wp in model.mig_wp:
+        k, i, j = wp[0]
+        mig[int(model.plist[k, i]), int(model.plist[k, j])] = rng.uniform(0, M)
+    ts = np.sort(rng.uniform(0, tmax, pt.npops - 1))
+    return sim.simulate_genealogy(rng, samppop, theta, list(ts), mig, pt), ts
+
+
+@pytest.mark.parametrize("mrate,mtime", [(0.3, 0.5), (1.7, 2.0), (4.0, 0.8)])
+def test_getmprob_two_population_period_normalises(mrate, mtime):
+    lastp = 2
+    for odd in (0, 1):
+        tot = 0.0
+        for m in range(odd, 80, 2):
+            lp = oracle.getmprob_nw(lastp - 1, lastp, mrate, mtime, m, 0, 1 if odd else 0, -1, 2)
+            tot += math.exp(lp + m * math.log(mtime) - math.lgamma(m + 1))
+        assert abs(tot - 1.0) < 1e-12, (odd, tot)
+    assert oracle.getmprob_nw(lastp, lastp, mrate, mtime, 3, 0, 0, -1, 1) == 0.0
+
+
+@pytest.mark.parametrize("numpops", [3, 4])
+@pytest.mark.parametrize("same", [True, False])
+@pytest.mark.parametrize("mrate,mtime", [(0.6, 0.9), (2.5, 1.5)])
+def test_getmprob_general_period_normalises(numpops, same, mrate, mtime):
+    """Sum over counts m and destination sequences x_1..x_m (x_i != x_i-1,
+    x_m-1 != d, x_m = d) of exp(getmprob) mtime^m / m!, the sequences
+    counted by the population before the second-to-last migration."""
+    u, d = 0, (0 if same else 1)
+    lastp = 3
+    tot = 0.0
+    if same:
+        tot += math.exp(oracle.getmprob_nw(0, lastp, mrate, mtime, 0, u, d, -1, numpops))
+    else:
+        tot += math.exp(oracle.getmprob_nw(0, lastp, mrate, mtime, 1, u, d, -1, numpops)
+                        + math.log(mtime))
+    A = np.ones((numpops, numpops)) - np.eye(numpops)  # one step to another population
+    N = np.zeros(numpops)
+    N[u] = 1.0  # sequences of the first m-2 choices, by the population reached
+    for m in range(2, 90):
+        for y in range(numpops):
+            if N[y] == 0.0:
+                continue
+            choices = numpops - 1 if y == d else numpops - 2
+            lp = oracle.getmprob_nw(0, lastp, mrate, mtime, m, u, d, y, numpops)
+            tot += N[y] * choices * math.exp(lp + m * math.log(mtime) - math.lgamma(m + 1))
+        N = N @ A
+    assert abs(tot - 1.0) < 1e-10, tot
+
+
+@pytest.mark.parametrize("param", [0.1, 0.6, 3.0, 9.0])
+@pytest.mark.parametrize("cond", [0, 1, 2, 3])
+def test_conditioned_poisson_draws(param, cond):
+    N = 20000
+    draws = np.array([oracle.poisson_cond(param, cond, 0x1234567 + 977 * i, 5) for i in range(N)])
+    ks = np.arange(0, 60)
+    pmf = stats.poisson.pmf(ks, param)
+    ok = {0: ks % 2 == 0, 1: ks % 2 == 1, 2: ks != 0, 3: ks != 1}[cond]
+    pmf = np.where(ok, pmf, 0.0)
+    pmf /= pmf.sum()
+    assert ok[draws].all()
+    if param < 0.25 and cond == 1:  # the reference's odd draw stops at 5
+        pmf[5] = pmf[5:].sum()
+        pmf[6:] = 0.0
+    freq = np.bincount(draws, minlength=len(ks))[: len(ks)] / N
+    se = np.sqrt(pmf * (1 - pmf) / N)
+    assert np.all(np.abs(freq - pmf) < 4.5 * np.maximum(se, 1.0 / N)), (freq[:8], pmf[:8])
+
+
+def _mig_set(g):
+    return {(int(e), float(t), int(q)) for e, lst in enumerate(g.mig) for t, q in lst}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_update_keeps_nodes_and_outside_migrations(seed):
+    rng = np.random.default_rng(seed)
+    three = seed % 2 == 1
+    pt = sim.PopTree.three_pop_01_2() if three else sim.PopTree.two_pop()
+    model = immodel.build(pt, qmax=4.0, mmax=2.0)
+    samp = [3, 2, 2] if three else [4, 3]
+    updated = 0
+    for trial in range(40):
+        g, ts = _prior_draw(rng, pt, model, samp, 4.0, 2.0, 3.0)
+        k = int(rng.integers(0, pt.npops - 1))
+        lo = ts[k - 1] if k > 0 else 0.0
+        hi = ts[k + 1] if k + 1 < len(ts) else 3.0
+        newt = rng.uniform(lo, hi)
+        rc, upd, w, h = oracle.update_mig_tnw(g, model, ts, k, newt, key=1000 * seed + trial)
+        assert rc == 0 and math.isfinite(w)
+        if not upd:
+            assert _mig_set(h) == _mig_set(g) and w == 0.0
+            continue
+        updated += 1
+        tu, td = min(ts[k], newt), max(ts[k], newt)
+        assert np.array_equal(h.time, g.time) and np.array_equal(h.up0, g.up0)
+        outside = lambda s: {x for x in s if not (tu < x[1] < td)}
+        assert outside(_mig_set(h)) <= outside(_mig_set(g)) | {x for x in _mig_set(h) if x[1] >= td}
+        assert outside(_mig_set(g)) - {x for x in _mig_set(g) if x[1] >= td} == \
+            outside(_mig_set(h)) - {x for x in _mig_set(h) if x[1] >= td}
+        for lst in h.mig:
+            tt = [t for t, _ in lst]
+            assert tt == sorted(tt)
+        tv_new = np.array(ts, float)
+        tv_new[k] = newt
+        oracle.treeweight(h, model, tv_new, samp)  # raises on labels that do not fit
+    assert updated >= 10
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_nw_chain_keeps_the_joint_prior(P):
+    rng = np.random.default_rng(40 + P)
+    Q, M, tmax = 4.0, (1.5 if P == 2 else 3.0), 3.0
+    pt = sim.PopTree.two_pop() if P == 2 else sim.PopTree.three_pop_01_2()
+    model = immodel.build(pt, qmax=Q, mmax=M)
+    samp = [2, 2] if P == 2 else [3, 3, 2]
+    n = sum(samp)
+    C = 8000 if P == 2 else 6000
+    loc = sim.HKYLocus(n, np.zeros((n, 1), np.int8), np.ones(1, np.int32), 1,
+                       np.array([0.25] * 4))
+    chains, tvs = [], []
+    for _ in range(C):
+        g, ts = _prior_draw(rng, pt, model, samp, Q, M, tmax)
+        chains.append([g])
+        tvs.append(ts)
+    tv = np.ascontiguousarray(np.array(tvs))
+    sw, st = oracle.mc3_bench([loc], [1e-9], 1.0, [samp], model, chains, tv, np.ones(C), tmax,
+                              seed=99 + P, max_sweeps=40, nw=2)
+    assert sw == 40 and 0.3 * C * 40 < st[3] < 0.9 * C * 40
+    fresh = [_prior_draw(rng, pt, model, samp, Q, M, tmax) for _ in range(C)]
+    t_fresh = np.array([f[1] for f in fresh])
+    nm_fresh = np.array([sum(len(x) for x in f[0].mig) for f in fresh])
+    nm = np.array([sum(len(x) for x in ch[0].mig) for ch in chains])
+    for k in range(P - 1):
+        assert stats.ks_2samp(tv[:, k], t_fresh[:, k]).pvalue > 1e-3
+    assert stats.ks_2samp(nm, nm_fresh).pvalue > 1e-3
+    for ch, ts in zip(chains[:200], tv[:200]):
+        oracle.treeweight(ch[0], model, ts, samp)
