@@ -81,6 +81,9 @@ PROTOS = {
     "ima_locus_set_model": (C.c_int, [vp, C.c_int, C.c_int, i32p, C.c_double]),
     "ima_slots_write_pops": (C.c_int, [vp, C.c_int, C.c_int, i32p, i32p, i32p, dp, i32p]),
     "ima_treeweight": (C.c_int, [vp, i32p, C.c_int, dp, i32p, dp, dp, i32p, dp, dp]),
+    "ima_slots_reserve_migs": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "ima_t_nw_update": (C.c_int, [vp, i32p, C.c_int, dp, C.c_int, C.c_double, C.c_uint64,
+                                  C.c_uint64, dp, i32p]),
     "ima_prior": (C.c_int, [vp, i32p, i32p, C.c_int, dp, dp, dp, dp]),
     "ima_mc3_set_prior": (C.c_int, [vp, dp, C.c_double]),
     "ima_mc3_read_prior": (C.c_int, [vp, dp, dp]),

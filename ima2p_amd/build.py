@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libima_gpu.so")
 PLAIN = os.path.join(HERE, "libima_plain.so")
 CXX = os.environ.get("CXX", "g++")
 SOURCES = ["ctx.hip", "kern_hky.hip", "kern_lik.hip", "kern_lmode.hip", "kern_swap.hip", "kern_mc3.hip",
-           "kern_prior.hip", "kern_urate.hip", "kern_topo.hip", "serve.hip", "ti_io.cpp", "mcf_io.cpp",
+           "kern_prior.hip", "kern_urate.hip", "kern_topo.hip", "kern_nw.hip", "serve.hip", "ti_io.cpp", "mcf_io.cpp",
            "lmode_drivers.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -386,6 +386,17 @@ void ima_ctx_destroy(ima_ctx *c) {
   release(c->d_mig_first);
   release(c->d_mig_count);
   release(c->d_mig_time);
+  release(c->d_nw_pop);
+  release(c->d_nw_mf);
+  release(c->d_nw_mc);
+  release(c->d_nw_me);
+  release(c->d_nw_mto);
+  release(c->d_nw_mt);
+  release(c->d_nw_nbk);
+  release(c->d_nw_st);
+  release(c->d_nw_period);
+  release(c->d_nw_w);
+  release(c->d_nw_newt);
   release(c->d_models);
   release(c->d_mint);
   release(c->d_mdbl);
@@ -2786,6 +2797,205 @@ static int prior_prepare(ima_ctx *c, const int32_t *slots, int nslots, int *mode
     *lf_need = std::max(*lf_need, tot);
   }
   return IMA_OK;
+}
+
+// Gives slot sl room for at least cap migration events: its events move
+// to a new region at the end of the event arrays (host mirror and device).
+static int grow_mig_cap(ima_ctx *c, int sl, int cap) {
+  SlotMeta &d = c->slots[sl];
+  if (d.mig_cap >= cap) return IMA_OK;
+  const int64_t old = d.mig_off;
+  d.mig_off = (int64_t)c->migs;
+  d.mig_cap = cap;
+  c->migs += (size_t)cap;
+  c->mig_edge.resize(c->migs, 0);
+  c->mig_to.resize(c->migs, 0);
+  c->mig_time.resize(c->migs, 0.0);
+  std::copy(c->mig_edge.begin() + old, c->mig_edge.begin() + old + d.nmig,
+            c->mig_edge.begin() + d.mig_off);
+  std::copy(c->mig_to.begin() + old, c->mig_to.begin() + old + d.nmig,
+            c->mig_to.begin() + d.mig_off);
+  std::copy(c->mig_time.begin() + old, c->mig_time.begin() + old + d.nmig,
+            c->mig_time.begin() + d.mig_off);
+  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
+  if (d.nmig) {
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + d.mig_off, c->d_mig_edge.p + old,
+                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + d.mig_off, c->d_mig_to.p + old,
+                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + d.mig_off, c->d_mig_time.p + old,
+                             (size_t)d.nmig * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->slots_meta_dirty = true;
+  return IMA_OK;
+}
+
+int ima_slots_reserve_migs(ima_ctx *c, int first, int count, int cap) {
+  if (!c || first < 0 || count < 0 || first + count > (int)c->slots.size() || cap < 0)
+    return IMA_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc;
+  for (int i = 0; i < count; ++i)
+    if ((rc = grow_mig_cap(c, first + i, cap)) != IMA_OK) return rc;
+  return IMA_OK;
+}
+
+// The host mirror of the slots' populations, per-edge indexes and events,
+// uploaded (the NW update reads them on the device).
+static int nw_upload_slots(ima_ctx *c, const int32_t *slots, int nslots) {
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
+  for (int i = 0; i < nslots; ++i) {
+    const SlotMeta &s = c->slots[slots[i]];
+    const size_t ne = 2 * (size_t)s.n - 1;
+    HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s.node_off, c->pop.data() + s.node_off, ne * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p + s.node_off, c->mig_first.data() + s.node_off,
+                             ne * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p + s.node_off, c->mig_count.data() + s.node_off,
+                             ne * 4, hipMemcpyHostToDevice, c->stream));
+    if (s.nmig) {
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + s.mig_off, c->mig_edge.data() + s.mig_off,
+                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + s.mig_off, c->mig_to.data() + s.mig_off,
+                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + s.mig_off, c->mig_time.data() + s.mig_off,
+                               (size_t)s.nmig * 8, hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  return IMA_OK;
+}
+
+// Largest event count of a genealogy whose weights fit treeweight's LDS.
+static int nw_event_limit(ima_ctx *c, int P) {
+  const size_t fixed = treeweight_lds_bytes(P, 0);
+  if ((size_t)c->max_lds <= fixed) return 1;
+  return (int)(((size_t)c->max_lds - fixed) / 24);
+}
+
+// The NW update's backups, indexed like the node and migration arrays.
+static int nw_reserve_backups(ima_ctx *c) {
+  HIPCHK(c, reserve(c->d_nw_pop, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_mf, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_mc, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_me, c->migs + 1));
+  HIPCHK(c, reserve(c->d_nw_mto, c->migs + 1));
+  HIPCHK(c, reserve(c->d_nw_mt, c->migs + 1));
+  // per-edge migration indexes on the device for every slot
+  if (c->d_mig_first.n < c->nodes || c->d_mig_count.n < c->nodes) {
+    HIPCHK(c, reserve(c->d_mig_first, c->nodes, false, c->stream));
+    HIPCHK(c, reserve(c->d_mig_count, c->nodes, false, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p, c->mig_first.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p, c->mig_count.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  return IMA_OK;
+}
+
+// The host mirror of slots whose populations and migrations the device
+// rewrote: slot records, populations, per-edge indexes and events.
+static int nw_sync_slots(ima_ctx *c, const int32_t *slots, int nslots) {
+  for (int i = 0; i < nslots; ++i) {
+    const int sl = slots[i];
+    HIPCHK(c, copy_now(c, &c->slots[sl], c->d_slots.p + sl, sizeof(SlotMeta),
+                       hipMemcpyDeviceToHost));
+    const SlotMeta &s = c->slots[sl];
+    const size_t ne = 2 * (size_t)s.n - 1;
+    HIPCHK(c, copy_now(c, c->pop.data() + s.node_off, c->d_pop.p + s.node_off, ne * 4,
+                       hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_first.data() + s.node_off, c->d_mig_first.p + s.node_off,
+                       ne * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_count.data() + s.node_off, c->d_mig_count.p + s.node_off,
+                       ne * 4, hipMemcpyDeviceToHost));
+    if (s.nmig) {
+      HIPCHK(c, copy_now(c, c->mig_edge.data() + s.mig_off, c->d_mig_edge.p + s.mig_off,
+                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
+      HIPCHK(c, copy_now(c, c->mig_to.data() + s.mig_off, c->d_mig_to.p + s.mig_off,
+                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
+      HIPCHK(c, copy_now(c, c->mig_time.data() + s.mig_off, c->d_mig_time.p + s.mig_off,
+                         (size_t)s.nmig * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  return IMA_OK;
+}
+
+int ima_t_nw_update(ima_ctx *c, const int32_t *slots, int nslots, const double *tvals,
+                    int period, double newt, uint64_t seed, uint64_t counter, double *migweight,
+                    int32_t *updated) {
+  if (!c || (!slots && nslots > 0) || nslots < 0 || !tvals) return IMA_ERR_ARG;
+  if (nslots == 0) return IMA_OK;
+  hipSetDevice(c->device);
+  int model, P, emax, rc;
+  size_t lfn;
+  const int32_t goff[2] = {0, nslots};
+  if ((rc = prior_prepare(c, slots, nslots, &model, &P, &emax, &lfn, goff, 1)) != IMA_OK) return rc;
+  if (P < 2) return fail(c, IMA_ERR_ARG, "the NW update needs two or more populations");
+  if (period < 0 || period > P - 2) return fail(c, IMA_ERR_ARG, "split %d out of range", period);
+  {
+    const double t_u = period == 0 ? 0.0 : tvals[period - 1];
+    const double t_d = period == P - 2 ? kTimeMax : tvals[period + 1];
+    if (!(newt > t_u && newt < t_d) || !(tvals[period] > t_u && tvals[period] < t_d))
+      return fail(c, IMA_ERR_ARG, "split times %g -> %g outside (%g, %g)", tvals[period], newt,
+                  t_u, t_d);
+  }
+  int max_n = 2;
+  for (int i = 0; i < nslots; ++i) max_n = std::max(max_n, c->slots[slots[i]].n);
+  if (nw_lds_bytes(max_n) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "the NW update of %d gene copies needs more LDS than %d B", max_n,
+                c->max_lds);
+  if ((rc = upload_loci(c)) != IMA_OK) return rc;
+  if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
+  if ((rc = upload_models(c)) != IMA_OK) return rc;
+  if ((rc = nw_reserve_backups(c)) != IMA_OK) return rc;
+  if ((rc = nw_upload_slots(c, slots, nslots)) != IMA_OK) return rc;
+  const int tw = P - 1;
+  HIPCHK(c, reserve(c->d_pslots, nslots));
+  HIPCHK(c, reserve(c->d_tv, (size_t)tw));
+  HIPCHK(c, reserve(c->d_nw_period, 1));
+  HIPCHK(c, reserve(c->d_nw_newt, 1));
+  HIPCHK(c, reserve(c->d_nw_nbk, nslots));
+  HIPCHK(c, reserve(c->d_nw_st, nslots));
+  HIPCHK(c, reserve(c->d_nw_w, nslots));
+  HIPCHK(c, hipMemcpyAsync(c->d_pslots.p, slots, nslots * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tv.p, tvals, (size_t)tw * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_nw_period.p, &period, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_nw_newt.p, &newt, 8, hipMemcpyHostToDevice, c->stream));
+  NwArgs a;
+  a.count = nslots;
+  a.slot = c->d_pslots.p;
+  a.tv = c->d_tv.p;
+  a.tw = tw;
+  a.period = c->d_nw_period.p;
+  a.newt = c->d_nw_newt.p;
+  a.seed = seed;
+  a.ctr = counter;
+  a.model = model;
+  a.emax = nw_event_limit(c, P);
+  a.max_n = max_n;
+  a.pop_bk = c->d_nw_pop.p;
+  a.mf_bk = c->d_nw_mf.p;
+  a.mc_bk = c->d_nw_mc.p;
+  a.me_bk = c->d_nw_me.p;
+  a.mto_bk = c->d_nw_mto.p;
+  a.mt_bk = c->d_nw_mt.p;
+  a.nmig_bk = c->d_nw_nbk.p;
+  a.migweight = c->d_nw_w.p;
+  a.status = c->d_nw_st.p;
+  HIPCHK(c, launch_nw_update(view(c), a, c->stream));
+  std::vector<double> w(nslots);
+  std::vector<int32_t> st(nslots);
+  HIPCHK(c, hipMemcpyAsync(w.data(), c->d_nw_w.p, nslots * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_nw_st.p, nslots * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int32_t> changed;
+  for (int i = 0; i < nslots; ++i) {
+    if (migweight) migweight[i] = w[i];
+    if (updated) updated[i] = st[i];
+    if (st[i] == 1) changed.push_back(slots[i]);
+  }
+  return nw_sync_slots(c, changed.data(), (int)changed.size());
 }
 
 int ima_treeweight(ima_ctx *c, const int32_t *slots, int nslots, const double *tvals,

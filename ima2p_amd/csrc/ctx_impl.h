@@ -62,6 +62,12 @@ struct ima_ctx {
   size_t migs = 0;
   DevBuf<int32_t> d_pop, d_mig_edge, d_mig_to, d_mig_first, d_mig_count;
   DevBuf<double> d_mig_time;
+  // the NW update's backups (indexed like the node and migration arrays)
+  // and per-call arrays
+  DevBuf<int32_t> d_nw_pop, d_nw_mf, d_nw_mc, d_nw_me, d_nw_mto;
+  DevBuf<double> d_nw_mt;
+  DevBuf<int32_t> d_nw_nbk, d_nw_st, d_nw_period;
+  DevBuf<double> d_nw_w, d_nw_newt;
 
   // population models of the genealogy prior
   std::vector<ModelMeta> models;

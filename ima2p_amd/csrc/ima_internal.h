@@ -249,6 +249,40 @@ hipError_t launch_prior(const DeviceView &v, int model, int P, int nparams, cons
                         double *mint, hipStream_t st);
 
 // Device-resident MC3 state (kern_mc3.hip).
+// Arguments of the NW migration update (kern_nw.hip): genealogy i is slot
+// slot[i] (or first_slot + i) of chain row row[i] (or 0); the row's split
+// period[row] (-1: no update) moves from tv[row * tw + period] to
+// newt[row].  Edges draw from Philox streams keyed by seed, the
+// genealogy's key id (keyid[i] or i) and the edge, at counter ctr (or the
+// sweep counter's stream 5).  Backups are indexed like the node and
+// migration arrays.
+struct NwArgs {
+  int count = 0;
+  const int32_t *slot = nullptr;
+  int first_slot = 0;
+  const int32_t *row = nullptr;
+  const double *tv = nullptr;
+  int tw = 1;
+  const int32_t *period = nullptr;
+  const double *newt = nullptr;
+  const int32_t *keyid = nullptr;
+  uint64_t seed = 0, ctr = 0;
+  const uint64_t *counter = nullptr;
+  int model = 0;
+  int emax = 0;   // event limit of the genealogy-prior kernels
+  int max_n = 2;  // largest gene-copy count (sizes the LDS)
+  int32_t *pop_bk = nullptr, *mf_bk = nullptr, *mc_bk = nullptr;
+  int32_t *me_bk = nullptr, *mto_bk = nullptr;
+  double *mt_bk = nullptr;
+  int32_t *nmig_bk = nullptr;  // [count]
+  double *migweight = nullptr;  // [count]
+  int32_t *status = nullptr;    // [count]: 0 untouched, 1 updated, 2 left (no room)
+};
+size_t nw_lds_bytes(int max_n);
+hipError_t launch_nw_update(const DeviceView &v, const NwArgs &a, hipStream_t st);
+hipError_t launch_nw_restore(const DeviceView &v, const NwArgs &a, const int32_t *decision,
+                             hipStream_t st);
+
 struct MC3Dev {
   int C, L;
   int first_slot;

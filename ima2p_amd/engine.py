@@ -183,6 +183,24 @@ class Engine:
         self._chk(self.lib.ima_slots_write_pops(self.h, first, len(gens), pp, pn, pme, pmt, pmo),
                   "write_pops")
 
+    def slots_reserve_migs(self, first, count, cap):
+        self._chk(self.lib.ima_slots_reserve_migs(self.h, first, count, int(cap)),
+                  "slots_reserve_migs")
+
+    def t_nw_update(self, slots, tvals, period, newt, seed, counter=0):
+        """changet_NW's migration update of a chain's genealogies (slots) for
+        split `period` moving from tvals[period] to newt; returns
+        (migweight[n], updated[n])."""
+        sl, psl = _i32(slots)
+        tv, ptv = _f64(tvals)
+        w = np.zeros(len(sl))
+        upd = np.zeros(len(sl), np.int32)
+        self._chk(self.lib.ima_t_nw_update(self.h, psl, len(sl), ptv, int(period), float(newt),
+                                           int(seed) & (2**64 - 1), int(counter),
+                                           w.ctypes.data_as(L.dp), upd.ctypes.data_as(L.i32p)),
+                  "t_nw_update")
+        return w, upd
+
     def treeweight(self, slots, tvals, P):
         """Per-slot weights: dict of cc/fc/hcc [n, P, P], mc/fm [n, P, P, P], length."""
         slots = np.ascontiguousarray(slots, np.int32)

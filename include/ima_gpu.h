@@ -427,6 +427,31 @@ int ima_treeweight(ima_ctx *ctx, const int32_t *slots, int nslots,
 int ima_prior(ima_ctx *ctx, const int32_t *slots, const int32_t *group_off,
               int ngroups, const double *tvals, double *probg, double *qint,
               double *mint);
+/* Room for at least cap migration events in each of slots first ..
+ * first + count - 1 (the reference grows each edge's mig array on demand,
+ * checkmig, update_gtree_common.cpp; a slot here holds a fixed region).
+ * Events already written keep their values. */
+int ima_slots_reserve_migs(ima_ctx *ctx, int first, int count, int cap);
+/* changet_NW's migration update (update_mig_tNW, update_t_NW.cpp:392-792,
+ * with addmigration_NW :238-333, simmpath_tNW :144-236 and getmprob_NW
+ * :335-372) of the genealogies in slots[0..nslots) -- a chain's loci --
+ * for the move of split `period` from tvals[period] to newt (strictly
+ * between its neighbours, 0 and TIMEMAX at the ends): every edge that
+ * overlaps the interval between the two times gets its migrations there
+ * drawn again so that its populations fit newt.  Node times do not move.
+ * The slots' populations and migrations are rewritten in place; updated[i]
+ * is 1 for a changed genealogy, 0 for one that lies below the interval
+ * (the reference leaves it as it is), 2 for one whose new migrations would
+ * exceed its slot's room or the genealogy prior's event limit (left as it
+ * is; the caller rejects the move).  migweight[i] receives genealogy i's
+ * Hastings term (the reference's migweight sums them).  Genealogy i draws
+ * from Philox streams keyed by seed, i and the edge, at counter `counter`.
+ * The caller then scores the new genealogies with ima_treeweight at the
+ * new split times and decides with exp(beta dprobg + migweight)
+ * (changet_NW :983-1000), restoring its own copy on rejection. */
+int ima_t_nw_update(ima_ctx *ctx, const int32_t *slots, int nslots, const double *tvals,
+                    int period, double newt, uint64_t seed, uint64_t counter,
+                    double *migweight, int32_t *updated);
 /* integrate_tree_prob (update_gtree_common.cpp:1945-2053) with the
  * reference's own argument: a chain's all-loci weights allgweight, which
  * IMa2p keeps up to date one locus at a time (sum_subtract_treeinfo,

@@ -92,6 +92,7 @@ static double ph_unif(ph_t *r) {
 #define ADDMIGMAX 1000
 #define NW_TIMEMAX 1000000.0
 #define NW_EDGE_MIX 0xC2B2AE3D27D4EB4Full
+#define NW_MAX_TRIES (1 << 20) /* bound of every rejection loop (the engine's) */
 
 static int randposint(ph_t *r, int n) { return (int)floor(ph_unif(r) * n); }
 
@@ -137,7 +138,7 @@ static int poisson_c(ph_t *r, double param, int cond) {
     }
     return i;
   }
-  int i, stop;
+  int i, stop, tries = 0;
   do {
     if (param >= 100.0) {
       const long v = (long)(normdev(r, param, param) + 0.5);
@@ -145,9 +146,13 @@ static int poisson_c(ph_t *r, double param, int cond) {
     } else {
       const double raised = exp(-param);
       double U = ph_unif(r);
-      for (i = 0; U > raised; i++) U *= ph_unif(r);
+      for (i = 0; U > raised && i < NW_MAX_TRIES; i++) U *= ph_unif(r);
     }
     stop = cond == 0 ? !(i & 1) : cond == 1 ? (i & 1) : cond == 2 ? i != 0 : i != 1;
+    if (!stop && ++tries >= NW_MAX_TRIES) { /* a bound no valid rate reaches */
+      i = cond == 1 || cond == 2 ? 1 : 0;
+      stop = 1;
+    }
   } while (!stop);
   if (i > ADDMIGMAX) i = cond == 1 ? ADDMIGMAX - 1 : ADDMIGMAX;
   return i;
@@ -338,7 +343,7 @@ int ora_update_mig_tnw(int n, const int *up0, const int *up1, const int *down,
         upa[e] = upb[e] == addp ? nowpop(&g, e, tu) : upb[e];
       }
     } else {
-      const int o = own[up0[e]];
+      const int o = e < n ? -1 : own[up0[e]];
       if (o < 0) { /* a node exactly at tu: the reference reads no update entry */
         rc = 2;
         goto done;
@@ -369,6 +374,21 @@ int ora_update_mig_tnw(int n, const int *up0, const int *up1, const int *down,
     const int o = own[e];
     const double ut = uptime_of(&g, e);
     mtime[e] = fmin(td, time[e]) - fmax(tu, ut);
+    if (!(mtime[e] > 0)) { /* the reference asserts a positive length */
+      rc = 2;
+      goto done;
+    }
+    if (npopsa >= 2) { /* both path ends must be populations of the period */
+      int ua = 0, ud = 0;
+      for (int q = 0; q < npresent[period_a]; q++) {
+        ua = ua || plist[period_a * P + q] == upa[e];
+        ud = ud || plist[period_a * P + q] == da[o];
+      }
+      if (!ua || !ud) {
+        rc = 2;
+        goto done;
+      }
+    }
     int k = 0, mi = 0, mstart = -1;
     while (k < cnt[e] && omt[first[e] + k] < td) {
       if (omt[first[e] + k] > tu) {
@@ -414,7 +434,7 @@ int ora_update_mig_tnw(int n, const int *up0, const int *up1, const int *down,
        * populations, the last two fixed so the path ends in da */
       const double upt = fmax(tu, ut), tin = mtime[e];
       double *t = newmt + tot;
-      int dup;
+      int dup, tries = 0;
       do {
         for (int j = 0; j < add; j++) t[j] = upt + ph_unif(&r) * tin;
         for (int a = 1; a < add; a++) { /* insertion sort */
@@ -429,21 +449,21 @@ int ora_update_mig_tnw(int n, const int *up0, const int *up1, const int *down,
         dup = 0;
         for (int j = 0; j + 1 < add; j++)
           if (t[j] == t[j + 1]) dup = 1;
-      } while (dup);
+      } while (dup && ++tries < NW_MAX_TRIES);
       const int *pl = plist + period_a * P, npl = npresent[period_a];
       int last = upa[e];
       if (add >= 2) {
         for (int j = 0; j < add - 2; j++) {
-          int q;
+          int q, tries = 0;
           do q = randposint(&r, npl);
-          while (pl[q] == last);
+          while (pl[q] == last && ++tries < NW_MAX_TRIES);
           newmp[tot + j] = pl[q];
           last = pl[q];
         }
         cm2a[e] = last;
-        int q;
+        int q, tries = 0;
         do q = randposint(&r, npl);
-        while (pl[q] == last || pl[q] == da[o]);
+        while ((pl[q] == last || pl[q] == da[o]) && ++tries < NW_MAX_TRIES);
         newmp[tot + add - 2] = pl[q];
       }
       newmp[tot + add - 1] = da[o];
