@@ -22,7 +22,7 @@ REJECT_IS = -1000000000.0
 MAX_GENES = 256
 K_HKY64, K_HKY32, K_IS, K_SW, K_LMODE, K_SWAP, K_MC3, K_ACCEPT, K_PRIOR = range(9)
 MC3_GRAPH, MC3_SCALE, MC3_TIMED, MC3_RY1 = 1, 2, 4, 8
-MC3_UPDATE_U, MC3_U_SERIAL, MC3_TOPOLOGY = 16, 32, 64
+MC3_UPDATE_U, MC3_U_SERIAL, MC3_TOPOLOGY, MC3_NW = 16, 32, 64, 128
 
 
 class McfLocus(C.Structure):

@@ -1411,6 +1411,12 @@ static void mc3_release(ima_ctx *c) {
   release(m.titems);
   release(m.tptw);
   release(m.arange);
+  release(m.nwk);
+  release(m.nwst);
+  release(m.nwnbk);
+  release(m.nwt);
+  release(m.nwmig);
+  m.dev.nw = 0;
   m.topo_first = -1;
   m.gview_set = false;
   m.udev = URateDev{};
@@ -1647,6 +1653,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.seed = seed;
   d.scale_width = scale_width;
   d.ry1 = 0;
+  d.nw = 0;
   d.prior = -1;
   d.P = 1;
   d.R = 0;
@@ -1804,6 +1811,248 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
   return IMA_OK;
 }
 
+// Gives slot sl room for at least cap migration events: its events move
+// to a new region at the end of the event arrays (host mirror and device).
+static int grow_mig_cap(ima_ctx *c, int sl, int cap) {
+  SlotMeta &d = c->slots[sl];
+  if (d.mig_cap >= cap) return IMA_OK;
+  const int64_t old = d.mig_off;
+  d.mig_off = (int64_t)c->migs;
+  d.mig_cap = cap;
+  c->migs += (size_t)cap;
+  c->mig_edge.resize(c->migs, 0);
+  c->mig_to.resize(c->migs, 0);
+  c->mig_time.resize(c->migs, 0.0);
+  std::copy(c->mig_edge.begin() + old, c->mig_edge.begin() + old + d.nmig,
+            c->mig_edge.begin() + d.mig_off);
+  std::copy(c->mig_to.begin() + old, c->mig_to.begin() + old + d.nmig,
+            c->mig_to.begin() + d.mig_off);
+  std::copy(c->mig_time.begin() + old, c->mig_time.begin() + old + d.nmig,
+            c->mig_time.begin() + d.mig_off);
+  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
+  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
+  if (d.nmig) {
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + d.mig_off, c->d_mig_edge.p + old,
+                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + d.mig_off, c->d_mig_to.p + old,
+                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + d.mig_off, c->d_mig_time.p + old,
+                             (size_t)d.nmig * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->slots_meta_dirty = true;
+  return IMA_OK;
+}
+
+// The host mirror of the slots' populations, per-edge indexes and events,
+// uploaded (the NW update reads them on the device).
+static int nw_upload_slots(ima_ctx *c, const int32_t *slots, int nslots) {
+  HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
+  for (int i = 0; i < nslots; ++i) {
+    const SlotMeta &s = c->slots[slots[i]];
+    const size_t ne = 2 * (size_t)s.n - 1;
+    HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s.node_off, c->pop.data() + s.node_off, ne * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p + s.node_off, c->mig_first.data() + s.node_off,
+                             ne * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p + s.node_off, c->mig_count.data() + s.node_off,
+                             ne * 4, hipMemcpyHostToDevice, c->stream));
+    if (s.nmig) {
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + s.mig_off, c->mig_edge.data() + s.mig_off,
+                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + s.mig_off, c->mig_to.data() + s.mig_off,
+                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + s.mig_off, c->mig_time.data() + s.mig_off,
+                               (size_t)s.nmig * 8, hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  return IMA_OK;
+}
+
+// Largest event count of a genealogy whose weights fit treeweight's LDS.
+static int nw_event_limit(ima_ctx *c, int P) {
+  const size_t fixed = treeweight_lds_bytes(P, 0);
+  if ((size_t)c->max_lds <= fixed) return 1;
+  return (int)(((size_t)c->max_lds - fixed) / 24);
+}
+
+// The NW update's backups, indexed like the node and migration arrays.
+static int nw_reserve_backups(ima_ctx *c) {
+  HIPCHK(c, reserve(c->d_nw_pop, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_mf, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_mc, c->nodes + 1));
+  HIPCHK(c, reserve(c->d_nw_me, c->migs + 1));
+  HIPCHK(c, reserve(c->d_nw_mto, c->migs + 1));
+  HIPCHK(c, reserve(c->d_nw_mt, c->migs + 1));
+  // per-edge migration indexes on the device for every slot
+  if (c->d_mig_first.n < c->nodes || c->d_mig_count.n < c->nodes) {
+    HIPCHK(c, reserve(c->d_mig_first, c->nodes, false, c->stream));
+    HIPCHK(c, reserve(c->d_mig_count, c->nodes, false, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p, c->mig_first.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p, c->mig_count.data(), c->nodes * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  return IMA_OK;
+}
+
+// The host mirror of slots whose populations and migrations the device
+// rewrote: slot records, populations, per-edge indexes and events.
+static int nw_sync_slots(ima_ctx *c, const int32_t *slots, int nslots) {
+  for (int i = 0; i < nslots; ++i) {
+    const int sl = slots[i];
+    HIPCHK(c, copy_now(c, &c->slots[sl], c->d_slots.p + sl, sizeof(SlotMeta),
+                       hipMemcpyDeviceToHost));
+    const SlotMeta &s = c->slots[sl];
+    const size_t ne = 2 * (size_t)s.n - 1;
+    HIPCHK(c, copy_now(c, c->pop.data() + s.node_off, c->d_pop.p + s.node_off, ne * 4,
+                       hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_first.data() + s.node_off, c->d_mig_first.p + s.node_off,
+                       ne * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_count.data() + s.node_off, c->d_mig_count.p + s.node_off,
+                       ne * 4, hipMemcpyDeviceToHost));
+    if (s.nmig) {
+      HIPCHK(c, copy_now(c, c->mig_edge.data() + s.mig_off, c->d_mig_edge.p + s.mig_off,
+                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
+      HIPCHK(c, copy_now(c, c->mig_to.data() + s.mig_off, c->d_mig_to.p + s.mig_off,
+                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
+      HIPCHK(c, copy_now(c, c->mig_time.data() + s.mig_off, c->d_mig_time.p + s.mig_off,
+                         (size_t)s.nmig * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  return IMA_OK;
+}
+
+// Before every run with the NW update: the migration-time backup and the
+// NW backups cover the current node and event arrays (slots allocated or
+// moved since may have grown them); a buffer that moved drops the graphs.
+static int mc3_nw_buffers(ima_ctx *c) {
+  auto &m = c->mc3;
+  MC3Dev &d = m.dev;
+  HIPCHK(c, reserve(m.mtbackup, c->migs + 1));
+  int rc;
+  if ((rc = nw_reserve_backups(c)) != IMA_OK) return rc;
+  NwArgs &a = d.nwa;
+  const bool moved = d.mtbackup != m.mtbackup.p || a.pop_bk != c->d_nw_pop.p ||
+                     a.mf_bk != c->d_nw_mf.p || a.mc_bk != c->d_nw_mc.p ||
+                     a.me_bk != c->d_nw_me.p || a.mto_bk != c->d_nw_mto.p ||
+                     a.mt_bk != c->d_nw_mt.p || a.tv != m.tvbackup.p || a.row != m.tv_row.p ||
+                     a.counter != m.counter.p;
+  d.mtbackup = m.mtbackup.p;
+  a.pop_bk = c->d_nw_pop.p;
+  a.mf_bk = c->d_nw_mf.p;
+  a.mc_bk = c->d_nw_mc.p;
+  a.me_bk = c->d_nw_me.p;
+  a.mto_bk = c->d_nw_mto.p;
+  a.mt_bk = c->d_nw_mt.p;
+  a.tv = m.tvbackup.p;
+  a.row = m.tv_row.p;
+  a.counter = m.counter.p;
+  if (moved) mc3_drop_graphs(c);
+  return IMA_OK;
+}
+
+// changet_NW as the other time update (IMA_MC3_NW).  On first use every
+// chain slot gets room for new migrations (kNwRoom beyond twice its
+// count), and the event bound of the genealogy-prior kernels, the
+// log-factorial table and the backups grow to match.
+constexpr int kNwRoom = 32;
+static int mc3_nw_mode(ima_ctx *c, bool on) {
+  auto &m = c->mc3;
+  MC3Dev &d = m.dev;
+  if (!on) {
+    if (d.nw) {
+      d.nw = 0;
+      mc3_drop_graphs(c);
+    }
+    return IMA_OK;
+  }
+  if (d.nw) return mc3_nw_buffers(c);
+  if (d.prior < 0 || d.P < 2 || c->models[d.prior].nmig == 0)
+    return fail(c, IMA_ERR_STATE,
+                "the NW update needs the genealogy prior of a model with migration");
+  const int N = m.C * m.L, P = d.P;
+  int emax = d.emax, maxn = 2, rc;
+  size_t lfn = 4;
+  for (int ch = 0; ch < m.C; ++ch) {
+    size_t tot = 4;
+    for (int l = 0; l < m.L; ++l) {
+      const int sl = m.first + ch * m.L + l;
+      const int nm = c->slots[sl].nmig;
+      const int cap = 2 * nm + kNwRoom;
+      if ((rc = grow_mig_cap(c, sl, cap)) != IMA_OK) return rc;
+      const SlotMeta &s = c->slots[sl];
+      emax = std::max(emax, (s.n - 1) + s.mig_cap + P - 1);
+      maxn = std::max(maxn, s.n);
+      tot += (size_t)s.n - 1 + (size_t)s.mig_cap;
+    }
+    lfn = std::max(lfn, tot);
+  }
+  if (treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "room for NW's migrations: %d prior events exceed LDS", emax);
+  if (nw_lds_bytes(maxn) > (size_t)c->max_lds)
+    return fail(c, IMA_ERR_SIZE, "the NW update of %d gene copies needs more LDS than %d B", maxn,
+                c->max_lds);
+  grow_logfact(c, lfn);
+  HIPCHK(c, reserve(m.nwk, m.C));
+  HIPCHK(c, reserve(m.nwt, m.C));
+  HIPCHK(c, reserve(m.nwst, N));
+  HIPCHK(c, reserve(m.nwnbk, N));
+  HIPCHK(c, reserve(m.nwmig, N));
+  // no chain has an NW move pending
+  HIPCHK(c, hipMemsetAsync(m.nwk.p, 0xff, (size_t)m.C * 4, c->stream));
+  d.emax = emax;
+  d.nwk = m.nwk.p;
+  d.nwt = m.nwt.p;
+  NwArgs &a = d.nwa;
+  a = NwArgs{};
+  a.count = N;
+  a.first_slot = m.first;
+  a.row = m.tv_row.p;
+  a.tv = m.tvbackup.p;
+  a.tw = P - 1;
+  a.period = m.nwk.p;
+  a.newt = m.nwt.p;
+  a.seed = d.seed ^ 0x4E57A1C3B2D5E6F7ull;
+  a.counter = m.counter.p;
+  a.model = d.prior;
+  a.emax = emax;
+  a.max_n = maxn;
+  a.nmig_bk = m.nwnbk.p;
+  a.migweight = m.nwmig.p;
+  a.status = m.nwst.p;
+  d.nw = 1;
+  mc3_drop_graphs(c);
+  return mc3_nw_buffers(c);
+}
+
+// After a run with the NW update: the host mirror of the chains' slot
+// records, populations, per-edge migration indexes and events follows the
+// device.
+static int mc3_nw_sync(ima_ctx *c) {
+  auto &m = c->mc3;
+  const int N = m.C * m.L;
+  const SlotMeta &s0 = c->slots[m.first];
+  const SlotMeta &s1 = c->slots[m.first + N - 1];
+  const size_t nb = s1.node_off + 2 * (size_t)s1.n - 1 - s0.node_off;
+  const size_t o = s0.node_off;
+  HIPCHK(c, copy_now(c, c->slots.data() + m.first, c->d_slots.p + m.first, N * sizeof(SlotMeta),
+                     hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->pop.data() + o, c->d_pop.p + o, nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->mig_first.data() + o, c->d_mig_first.p + o, nb * 4,
+                     hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_now(c, c->mig_count.data() + o, c->d_mig_count.p + o, nb * 4,
+                     hipMemcpyDeviceToHost));
+  if (c->migs) {
+    HIPCHK(c, copy_now(c, c->mig_edge.data(), c->d_mig_edge.p, c->migs * 4,
+                       hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_to.data(), c->d_mig_to.p, c->migs * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_now(c, c->mig_time.data(), c->d_mig_time.p, c->migs * 8,
+                       hipMemcpyDeviceToHost));
+  }
+  return IMA_OK;
+}
+
 // After a run with the device's own genealogy move: the host mirror of the
 // chains' slots follows the device (topologies, times, populations,
 // alleles, programs and slot records).
@@ -1851,6 +2100,9 @@ int mc3_run_async(ima_ctx *c, int nsweeps, int flags) {
   // start, nothing is left pending between sweeps
   const bool topo = (flags & IMA_MC3_TOPOLOGY) != 0;
   if ((rc = mc3_topo_mode(c, topo)) != IMA_OK) return rc;
+  // NW needs the split-time update, so the rescaling sweeps
+  if ((rc = mc3_nw_mode(c, (flags & IMA_MC3_NW) && (flags & IMA_MC3_SCALE))) != IMA_OK)
+    return rc;
   if ((rc = upload_slot_meta(c)) != IMA_OK) return rc;
   if ((rc = upload_models(c)) != IMA_OK) return rc;
   {
@@ -1979,6 +2231,7 @@ int ima_mc3_run(ima_ctx *c, int nsweeps, int flags) {
   if (rc != IMA_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->mc3.dev.topo) return mc3_topo_sync(c);
+  if (c->mc3.dev.nw) return mc3_nw_sync(c);
   return IMA_OK;
 }
 
@@ -2124,6 +2377,7 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   d.R = R;
   d.nparam = np;
   d.emax = emax;
+  d.nw = 0;  // NW's room for new migrations is made again on its next use
   d.tmax = tmax;
   d.tv = m.tv.p;
   d.tvbackup = m.tvbackup.p;
@@ -2799,39 +3053,6 @@ static int prior_prepare(ima_ctx *c, const int32_t *slots, int nslots, int *mode
   return IMA_OK;
 }
 
-// Gives slot sl room for at least cap migration events: its events move
-// to a new region at the end of the event arrays (host mirror and device).
-static int grow_mig_cap(ima_ctx *c, int sl, int cap) {
-  SlotMeta &d = c->slots[sl];
-  if (d.mig_cap >= cap) return IMA_OK;
-  const int64_t old = d.mig_off;
-  d.mig_off = (int64_t)c->migs;
-  d.mig_cap = cap;
-  c->migs += (size_t)cap;
-  c->mig_edge.resize(c->migs, 0);
-  c->mig_to.resize(c->migs, 0);
-  c->mig_time.resize(c->migs, 0.0);
-  std::copy(c->mig_edge.begin() + old, c->mig_edge.begin() + old + d.nmig,
-            c->mig_edge.begin() + d.mig_off);
-  std::copy(c->mig_to.begin() + old, c->mig_to.begin() + old + d.nmig,
-            c->mig_to.begin() + d.mig_off);
-  std::copy(c->mig_time.begin() + old, c->mig_time.begin() + old + d.nmig,
-            c->mig_time.begin() + d.mig_off);
-  HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
-  HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
-  HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
-  if (d.nmig) {
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + d.mig_off, c->d_mig_edge.p + old,
-                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + d.mig_off, c->d_mig_to.p + old,
-                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + d.mig_off, c->d_mig_time.p + old,
-                             (size_t)d.nmig * 8, hipMemcpyDeviceToDevice, c->stream));
-  }
-  c->slots_meta_dirty = true;
-  return IMA_OK;
-}
-
 int ima_slots_reserve_migs(ima_ctx *c, int first, int count, int cap) {
   if (!c || first < 0 || count < 0 || first + count > (int)c->slots.size() || cap < 0)
     return IMA_ERR_ARG;
@@ -2839,85 +3060,6 @@ int ima_slots_reserve_migs(ima_ctx *c, int first, int count, int cap) {
   int rc;
   for (int i = 0; i < count; ++i)
     if ((rc = grow_mig_cap(c, first + i, cap)) != IMA_OK) return rc;
-  return IMA_OK;
-}
-
-// The host mirror of the slots' populations, per-edge indexes and events,
-// uploaded (the NW update reads them on the device).
-static int nw_upload_slots(ima_ctx *c, const int32_t *slots, int nslots) {
-  HIPCHK(c, reserve(c->d_pop, c->nodes, true, c->stream));
-  for (int i = 0; i < nslots; ++i) {
-    const SlotMeta &s = c->slots[slots[i]];
-    const size_t ne = 2 * (size_t)s.n - 1;
-    HIPCHK(c, hipMemcpyAsync(c->d_pop.p + s.node_off, c->pop.data() + s.node_off, ne * 4,
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p + s.node_off, c->mig_first.data() + s.node_off,
-                             ne * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p + s.node_off, c->mig_count.data() + s.node_off,
-                             ne * 4, hipMemcpyHostToDevice, c->stream));
-    if (s.nmig) {
-      HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + s.mig_off, c->mig_edge.data() + s.mig_off,
-                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + s.mig_off, c->mig_to.data() + s.mig_off,
-                               (size_t)s.nmig * 4, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + s.mig_off, c->mig_time.data() + s.mig_off,
-                               (size_t)s.nmig * 8, hipMemcpyHostToDevice, c->stream));
-    }
-  }
-  return IMA_OK;
-}
-
-// Largest event count of a genealogy whose weights fit treeweight's LDS.
-static int nw_event_limit(ima_ctx *c, int P) {
-  const size_t fixed = treeweight_lds_bytes(P, 0);
-  if ((size_t)c->max_lds <= fixed) return 1;
-  return (int)(((size_t)c->max_lds - fixed) / 24);
-}
-
-// The NW update's backups, indexed like the node and migration arrays.
-static int nw_reserve_backups(ima_ctx *c) {
-  HIPCHK(c, reserve(c->d_nw_pop, c->nodes + 1));
-  HIPCHK(c, reserve(c->d_nw_mf, c->nodes + 1));
-  HIPCHK(c, reserve(c->d_nw_mc, c->nodes + 1));
-  HIPCHK(c, reserve(c->d_nw_me, c->migs + 1));
-  HIPCHK(c, reserve(c->d_nw_mto, c->migs + 1));
-  HIPCHK(c, reserve(c->d_nw_mt, c->migs + 1));
-  // per-edge migration indexes on the device for every slot
-  if (c->d_mig_first.n < c->nodes || c->d_mig_count.n < c->nodes) {
-    HIPCHK(c, reserve(c->d_mig_first, c->nodes, false, c->stream));
-    HIPCHK(c, reserve(c->d_mig_count, c->nodes, false, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_first.p, c->mig_first.data(), c->nodes * 4,
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_count.p, c->mig_count.data(), c->nodes * 4,
-                             hipMemcpyHostToDevice, c->stream));
-  }
-  return IMA_OK;
-}
-
-// The host mirror of slots whose populations and migrations the device
-// rewrote: slot records, populations, per-edge indexes and events.
-static int nw_sync_slots(ima_ctx *c, const int32_t *slots, int nslots) {
-  for (int i = 0; i < nslots; ++i) {
-    const int sl = slots[i];
-    HIPCHK(c, copy_now(c, &c->slots[sl], c->d_slots.p + sl, sizeof(SlotMeta),
-                       hipMemcpyDeviceToHost));
-    const SlotMeta &s = c->slots[sl];
-    const size_t ne = 2 * (size_t)s.n - 1;
-    HIPCHK(c, copy_now(c, c->pop.data() + s.node_off, c->d_pop.p + s.node_off, ne * 4,
-                       hipMemcpyDeviceToHost));
-    HIPCHK(c, copy_now(c, c->mig_first.data() + s.node_off, c->d_mig_first.p + s.node_off,
-                       ne * 4, hipMemcpyDeviceToHost));
-    HIPCHK(c, copy_now(c, c->mig_count.data() + s.node_off, c->d_mig_count.p + s.node_off,
-                       ne * 4, hipMemcpyDeviceToHost));
-    if (s.nmig) {
-      HIPCHK(c, copy_now(c, c->mig_edge.data() + s.mig_off, c->d_mig_edge.p + s.mig_off,
-                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
-      HIPCHK(c, copy_now(c, c->mig_to.data() + s.mig_off, c->d_mig_to.p + s.mig_off,
-                         (size_t)s.nmig * 4, hipMemcpyDeviceToHost));
-      HIPCHK(c, copy_now(c, c->mig_time.data() + s.mig_off, c->d_mig_time.p + s.mig_off,
-                         (size_t)s.nmig * 8, hipMemcpyDeviceToHost));
-    }
-  }
   return IMA_OK;
 }
 

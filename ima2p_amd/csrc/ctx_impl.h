@@ -156,6 +156,9 @@ struct ima_ctx {
     DevBuf<ima_item> titems;
     DevBuf<int32_t> tptw;
     DevBuf<int2> arange;
+    // changet_NW as the other time update (IMA_MC3_NW)
+    DevBuf<int32_t> nwk, nwst, nwnbk;
+    DevBuf<double> nwt, nwmig;
     // device buffers the captured graphs point at: a change (a buffer grown
     // by later slot allocations or writes) drops the graphs
     ima::DeviceView gview{};

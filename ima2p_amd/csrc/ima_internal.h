@@ -345,6 +345,13 @@ struct MC3Dev {
   ima_item *pitems;      // [C*L] the proposals' scoring items
   int32_t *ptw;          // [C*L] the proposals' slots
   int maxn;              // largest gene-copy count of the chains' loci
+  // changet_NW as the other time update (nw = 1): a chain that draws it
+  // this sweep has its split nwk[c] (else -1) moving to nwt[c]; the
+  // per-genealogy update (kern_nw.hip) reads nwa
+  int nw;
+  int32_t *nwk;          // [C]
+  double *nwt;           // [C]
+  NwArgs nwa;
 };
 
 // Dynamic LDS of the per-chain prior kernels (kern_mc3.hip), the same on

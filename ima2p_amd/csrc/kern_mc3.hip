@@ -788,31 +788,41 @@ __device__ void prior_accept(const DeviceView &v, const MC3Dev &m, int c, uint64
 // on both daughter edges, so their counts are halved) and the Hastings
 // term (ecd + emd) log t_d_hterm + (ecu + emu) log t_u_hterm goes to
 // m.logs[c].  Times are backed up for the restore as by the rescale.
+// The split time move shared by changet_RY1 and changet_NW: split k drawn
+// uniformly (randposint), its new time from getnewt's window
+// (update_gtree_common.cpp:2501-2518): width (t_d' - t_u') / (log(L + 1)
+// (P - k)) around the old time, reflected at the prior bounds t_u' =
+// max(0, t_u), t_d' = min(tmax, t_d), where t_u and t_d are the
+// neighbouring split times (0 and TIMEMAX at the ends).  Thread 0 only;
+// ry = {k, oldt, newt, t_u, t_d}.
+__device__ void draw_split_move(const MC3Dev &m, int c, uint64_t sweep, double *ry) {
+  const int tw = mc3_tw(m.P), ns = m.P - 1;
+  PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 2);
+  int k = (int)(r.uniform() * ns);
+  if (k > ns - 1) k = ns - 1;
+  const double *tv = m.tv + (int64_t)c * tw;
+  const double oldt = tv[k];
+  const double t_u = k == 0 ? 0.0 : tv[k - 1];
+  const double t_d = k == ns - 1 ? kTimeMax : tv[k + 1];
+  const double tdp = fmin(m.tmax, t_d), tup = fmax(0.0, t_u);
+  const double twin = (tdp - tup) / (log((double)m.L + 1.0) * (double)(m.P - k));
+  double newt = (oldt - twin / 2) + r.uniform() * twin;
+  if (newt >= tdp)
+    newt = 2.0 * tdp - newt;
+  else if (newt <= tup)
+    newt = 2.0 * tup - newt;
+  ry[0] = k;
+  ry[1] = oldt;
+  ry[2] = newt;
+  ry[3] = t_u;
+  ry[4] = t_d;
+}
+
 __device__ void ry1_stretch(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
                             double *red) {
   __shared__ double ry[5];  // k, oldt, newt, t_u, t_d
   const int tw = mc3_tw(m.P), ns = m.P - 1;
-  if (threadIdx.x == 0) {
-    PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 2);
-    int k = (int)(r.uniform() * ns);
-    if (k > ns - 1) k = ns - 1;
-    const double *tv = m.tv + (int64_t)c * tw;
-    const double oldt = tv[k];
-    const double t_u = k == 0 ? 0.0 : tv[k - 1];
-    const double t_d = k == ns - 1 ? kTimeMax : tv[k + 1];
-    const double tdp = fmin(m.tmax, t_d), tup = fmax(0.0, t_u);
-    const double twin = (tdp - tup) / (log((double)m.L + 1.0) * (double)(m.P - k));
-    double newt = (oldt - twin / 2) + r.uniform() * twin;
-    if (newt >= tdp)
-      newt = 2.0 * tdp - newt;
-    else if (newt <= tup)
-      newt = 2.0 * tup - newt;
-    ry[0] = k;
-    ry[1] = oldt;
-    ry[2] = newt;
-    ry[3] = t_u;
-    ry[4] = t_d;
-  }
+  if (threadIdx.x == 0) draw_split_move(m, c, sweep, ry);
   __syncthreads();
   const int k = (int)ry[0];
   const double oldt = ry[1], newt = ry[2], t_u = ry[3], t_d = ry[4];
@@ -871,6 +881,27 @@ __device__ void ry1_stretch(const DeviceView &v, const MC3Dev &m, int c, uint64_
   }
 }
 
+// changet_NW's move of chain c (update_t_NW.cpp:920-1032): the split and
+// its new time as for RY1; the split times are backed up and the new one
+// set (the genealogies' weights are scored at it), and the chain's split
+// and new time are left for the per-genealogy migration update
+// (kern_nw.hip), which reads the old split times from the backup.
+__device__ void nw_start(const MC3Dev &m, int c, uint64_t sweep) {
+  __shared__ double ry[5];
+  const int tw = mc3_tw(m.P);
+  if (threadIdx.x == 0) {
+    draw_split_move(m, c, sweep, ry);
+    m.nwk[c] = (int)ry[0];
+    m.nwt[c] = ry[2];
+  }
+  __syncthreads();
+  const int k = (int)ry[0];
+  for (int x = threadIdx.x; x < tw; x += blockDim.x) {
+    m.tvbackup[c * tw + x] = m.tv[c * tw + x];
+    if (x == k) m.tv[c * tw + x] = ry[2];
+  }
+}
+
 // One workgroup per chain.  Accept or reject every locus' genealogy
 // proposal (update_gtree.cpp:919-927; rejection restores the moved time).
 // Rescaling sweep: back up and rescale every coalescence time of the chain
@@ -919,6 +950,21 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
     return;
   }
   __syncthreads();  // restored times visible to the whole workgroup
+  if (m.nw) {
+    // qupdate's time update: RY1 or NW with equal probability when the
+    // model has migration (ima_main_mpi.cpp:1867-1911)
+    __shared__ int pick;
+    if (threadIdx.x == 0) {
+      PhiloxStream r = item_rng(m, m.C * m.L + c, sweep, 7);
+      pick = r.uniform() < 0.5;
+      if (!pick) m.nwk[c] = -1;
+    }
+    __syncthreads();
+    if (pick) {
+      nw_start(m, c, sweep);
+      return;
+    }
+  }
   if (m.ry1 && m.prior >= 0 && m.P > 1) {
     ry1_stretch(v, m, c, sweep, sh);
     return;
@@ -967,13 +1013,22 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   const bool prior = m.prior >= 0;
   // rubber band: m.logs[c] holds the whole Hastings term
   const bool ry = m.ry1 && prior && m.P > 1;
+  // changet_NW this sweep: node times (so the data likelihood) did not
+  // move; the Hastings term is the genealogies' migweight sum, and
+  // exp(beta dprobg + migweight) decides (update_t_NW.cpp:983-1000)
+  const bool nwc = m.nw && m.nwk[c] >= 0;
   if (threadIdx.x == 0) okall = 1;
   __syncthreads();
   double a = 0.0;
   for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
     const int it = c * m.L + l;
-    if (m.status[it] != IMA_STATUS_OK) okall = 0;
     if (prior && m.wstatus[it] != IMA_STATUS_OK) okall = 0;
+    if (nwc) {
+      if (m.nwa.status[it] == 2) okall = 0;
+      a += m.nwa.migweight[it];
+      continue;
+    }
+    if (m.status[it] != IMA_STATUS_OK) okall = 0;
     const SlotMeta sm = v.slots[m.first_slot + it];
     const int scaled = ry ? 0 : (sm.n - 1) + (prior ? sm.nmig : 0);
     a += m.beta[c] * (new_loglik(m, it) - m.pdg[it]) + scaled * m.logs[c];
@@ -986,7 +1041,7 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   double pn = 0.0;
   if (prior) {
     pn = chain_prior_fresh(v, m, c, m.wnew, psh);
-    a += m.beta[c] * (pn - m.probg[c]) + (ry ? 1 : m.P - 1) * m.logs[c];
+    a += m.beta[c] * (pn - m.probg[c]) + (nwc ? 0.0 : (ry ? 1 : m.P - 1) * m.logs[c]);
     if (threadIdx.x == 0)
       for (int k = 0; k < m.P - 1; ++k)
         if (m.tv[c * tw + k] > m.tmax) okall = 0;
@@ -1004,7 +1059,7 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
   if (decision) {
     for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
       const int it = c * m.L + l;
-      const double np_ = new_loglik(m, it);
+      const double np_ = nwc ? m.pdg[it] : new_loglik(m, it);
       m.pdg[it] = np_;
       part += np_;
     }
@@ -1022,6 +1077,37 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
         part += pn;
       }
     }
+  } else if (nwc) {
+    // the genealogies the update changed get their populations and
+    // migrations back
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) part += m.pdg[c * m.L + l];
+    const NwArgs &na = m.nwa;
+    int32_t *pop = const_cast<int32_t *>(v.pop), *mf = const_cast<int32_t *>(v.mig_first);
+    int32_t *mc = const_cast<int32_t *>(v.mig_count), *me = const_cast<int32_t *>(v.mig_edge);
+    int32_t *mto = const_cast<int32_t *>(v.mig_to);
+    double *mt = const_cast<double *>(v.mig_time);
+    for (int l = 0; l < m.L; ++l) {
+      const int it = c * m.L + l;
+      if (na.status[it] != 1) continue;
+      const int s = m.first_slot + it;
+      const SlotMeta sm = v.slots[s];
+      const int64_t no = sm.node_off, mo = sm.mig_off;
+      const int ne = 2 * sm.n - 1, nm = na.nmig_bk[it];
+      for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        pop[no + e] = na.pop_bk[no + e];
+        mf[no + e] = na.mf_bk[no + e];
+        mc[no + e] = na.mc_bk[no + e];
+      }
+      for (int x = threadIdx.x; x < nm; x += blockDim.x) {
+        me[mo + x] = na.me_bk[mo + x];
+        mto[mo + x] = na.mto_bk[mo + x];
+        mt[mo + x] = na.mt_bk[mo + x];
+      }
+      __syncthreads();  // every lane has read the slot record
+      if (threadIdx.x == 0) const_cast<SlotMeta *>(v.slots)[s].nmig = nm;
+    }
+    for (int k = threadIdx.x; k < tw; k += blockDim.x) m.tv[c * tw + k] = m.tvbackup[c * tw + k];
+    if (threadIdx.x == 0) part += m.probg[c];
   } else {
     for (int l = threadIdx.x; l < m.L; l += blockDim.x) part += m.pdg[c * m.L + l];
     int64_t j0, j1;
@@ -1313,6 +1399,10 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), dl, st, v, m, with_scale ? 1 : 0,
                      swaptries);
+  if (with_scale && m.nw) {
+    e = launch_nw_update(v, m.nwa, st);
+    if (e != hipSuccess) return e;
+  }
   if (with_scale) {
     e = score(1);
     if (e != hipSuccess) return e;

@@ -514,6 +514,18 @@ int ima_integrate_tree_prob(ima_ctx *ctx, int model, const double *wsum, int ngr
  * addmigration (:589-665) are not built, and a model with migration
  * parameters is refused with IMA_ERR_STATE. */
 #define IMA_MC3_TOPOLOGY 64
+/* With IMA_MC3_SCALE and the genealogy prior of a model with migration:
+ * each chain's time update is changet_NW (update_t_NW.cpp:920-1032) with
+ * probability 1/2, as qupdate picks between RY1 and NW (ima_main_mpi.cpp:
+ * 1867-1911), else the RY1 or rescale update above.  NW moves one split
+ * time as RY1 does (getnewt), leaves every node where it is, draws the
+ * migrations of each genealogy's edges that overlap the interval between
+ * the old and new times again (ima_t_nw_update, on the device), scores
+ * the genealogies' weights at the new time and accepts with U < min(1,
+ * exp(beta dprobg + migweight)).  On first use every chain slot gets room
+ * for twice its migrations plus 32; a genealogy whose new migrations
+ * would not fit rejects its chain's move. */
+#define IMA_MC3_NW 128
 int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
                   const double *u, const double *kappa, const double *beta,
                   uint64_t seed, int precision, int swaptries,

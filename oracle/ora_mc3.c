@@ -860,7 +860,7 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
     size_t nt = 0, nm = 0;
     for (int l = 0; l < L; l++) {
       nt += 2 * (size_t)h->g[l].n - 1;
-      nm += (size_t)h->g[l].nmig;
+      nm += (size_t)h->g[l].mig_cap; /* NW may add migrations up to the room */
     }
     h->tback = (double *)malloc(sizeof(double) * (nt + 1));
     h->mback = (double *)malloc(sizeof(double) * (nm + 1));

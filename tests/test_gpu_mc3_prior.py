@@ -7,6 +7,8 @@ integrate_tree_prob over the chain's all-loci weights).
   equals the oracle's integrate_tree_prob over its current genealogies and
   split times (1e-9: the sweep updates the all-loci sums incrementally, the
   oracle sums afresh).  Genealogies stay valid for the model.
+* The same with changet_NW drawn as the other time update: migrations
+  read back as NW rewrote them.
 * Graph replay equals eager launches bit for bit.
 * Stationary law, flat likelihood (u -> 0), one population: the sampled
   time to the most recent common ancestor of two gene copies follows the
@@ -107,7 +109,57 @@ def test_state_matches_oracle_after_sweeps(prec, tol, ry1):
     e.close()
 
 
-@pytest.mark.parametrize("ry1", [0, L.MC3_RY1], ids=["rescale", "rubber_band"])
+def _current_full(e, g, slot, times):
+    """Genealogy g with the node times, populations and migrations the
+    device now holds (the NW update changes the migrations)."""
+    h = g.copy()
+    h.time = times.copy()
+    h.pop = e.read_pops(slot, 1)
+    me, mt, mo = e.read_mig_events(slot)
+    h.mig = [[] for _ in range(2 * g.n - 1)]
+    for x in range(len(me)):
+        h.mig[int(me[x])].append((float(mt[x]), int(mo[x])))
+    return h
+
+
+@pytest.mark.parametrize("prec,tol", [(L.PREC_FP64, 1e-9), (L.PREC_FP32, 1e-6)])
+def test_state_matches_oracle_after_nw_sweeps(prec, tol):
+    """Time updates drawn between changet_RY1 and changet_NW (IMA_MC3_NW):
+    after 60 sweeps every stored likelihood equals the oracle's on the
+    genealogy in its slot, every chain's probg equals integrate_tree_prob
+    over its genealogies -- populations and migrations read back, as NW
+    rewrote them -- at its split times, and the migration counts moved."""
+    C, Lc = 6, 5
+    e, model, samp, locs, gens, first, u, ts = _three_pop_chains(C, Lc, 37, prec)
+    nm0 = [len(e.read_mig_events(first + i)[0]) for i in range(C * Lc)]
+    e.mc3_run(60, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_NW)
+    pdg, beta, stats, sweeps = e.mc3_read()
+    probg, tv = e.mc3_read_prior()
+    assert sweeps == 60 and stats[1] > 0 and stats[3] > 0
+    up0, up1, dn, tm, rt, _ = e.read_slots(first, C * Lc)
+    off, nm1 = 0, []
+    for c in range(C):
+        ws = []
+        for l in range(Lc):
+            i = c * Lc + l
+            g = gens[i]
+            ne = 2 * g.n - 1
+            h = _current_full(e, g, first + i, tm[off:off + ne])
+            off += ne
+            nm1.append(sum(len(x) for x in h.mig))
+            for lst in h.mig:
+                assert [t for t, _ in lst] == sorted(t for t, _ in lst)
+            ref, _ = oracle.likelihood_hky(h, locs[l], u[l], 4.0)
+            assert abs(pdg[c, l] - ref) <= tol * abs(ref), (c, l, pdg[c, l], ref)
+            ws.append(oracle.treeweight(h, model, tv[c], samp))  # raises if labels broke
+        rp, _, _ = oracle.integrate_tree_prob(ws, model)
+        assert abs(probg[c] - rp) <= 1e-9 * abs(rp), (c, probg[c], rp)
+    assert nm1 != nm0
+    e.close()
+
+
+@pytest.mark.parametrize("ry1", [0, L.MC3_RY1, L.MC3_RY1 | L.MC3_NW],
+                         ids=["rescale", "rubber_band", "rubber_band_nw"])
 def test_graph_equals_eager_with_prior(ry1):
     out = []
     for flags in (L.MC3_GRAPH | L.MC3_SCALE | ry1, L.MC3_SCALE | ry1):

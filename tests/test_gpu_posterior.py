@@ -11,7 +11,9 @@ time -- the fraction of each sampler's draws below the quantile must agree:
 This is the two-sample comparison of marginal posteriors, with the
 autocorrelation inside a chain accounted for by the chain-level spread.
 Both time updates are compared: the global rescale, and changet_RY1's
-rubber band on one split time (IMA_MC3_RY1); with the mutation-rate rounds
+rubber band on one split time (IMA_MC3_RY1), alone and drawn against
+changet_NW (IMA_MC3_NW) as qupdate does, with the migration counts
+compared as well; with the mutation-rate rounds
 (IMA_MC3_UPDATE_U, changeu every fifth sweep) the rate ratio log(u0/u1)
 and both loci's kappa are compared too.
 """
@@ -46,7 +48,7 @@ def _stats(gens, tv):
     return [float(g.time[g.up0[g.root]]) for g in gens] + [float(tv[0])]
 
 
-def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
+def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate, nw=False):
     e = Engine(0)
     mid = e.model_add(model)
     lids = []
@@ -63,7 +65,7 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
                 scale_width=0.3)
     e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
     flags = (L.MC3_GRAPH | L.MC3_SCALE | (L.MC3_RY1 if ry1 else 0)
-             | (L.MC3_UPDATE_U if urate else 0))
+             | (L.MC3_UPDATE_U if urate else 0) | (L.MC3_NW if nw else 0))
     e.mc3_run(burn, flags)
     out = []
     for _ in range(nsamp):
@@ -83,13 +85,16 @@ def _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
             vals.append(float(tv[c][0]))
             if urate:
                 vals += [float(np.log(un[c, 0] / un[c, 1])), float(kn[c, 0]), float(kn[c, 1])]
+            if nw:
+                vals.append(float(sum(len(e.read_mig_events(first + c * Lc + l)[0])
+                                      for l in range(Lc))))
             row.append(vals)
         out.append(row)
     e.close()
     return np.array(out).transpose(1, 0, 2)  # [chain, sample, stat]
 
 
-def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
+def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate, nw=False):
     chains = [[g.copy() for g in base] for _ in range(C)]
     tvs = np.ascontiguousarray(np.tile(ts, (C, 1)))
     rates = (np.full((C, 2), 2.0), np.full((C, 2), 4.0))
@@ -99,7 +104,7 @@ def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
         nonlocal k
         oracle.mc3_bench(loci, None, None, samp, model, chains, tvs, np.ones(C), 5.0,
                          seed=90000 + k, max_sweeps=n, scale_width=0.3, ry1=ry1, urate=urate,
-                         rates=rates)
+                         rates=rates, nw=1 if nw else 0)
         k += 1
 
     def stats(c):
@@ -107,6 +112,8 @@ def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
         if urate:
             u, kap = rates
             v += [float(np.log(u[c, 0] / u[c, 1])), float(kap[c, 0]), float(kap[c, 1])]
+        if nw:
+            v.append(float(sum(len(m) for g in chains[c] for m in g.mig)))
         return v
 
     run(burn)
@@ -117,13 +124,17 @@ def _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate):
     return np.array(out).transpose(1, 0, 2)
 
 
-@pytest.mark.parametrize("ry1,urate", [(False, False), (True, False), (True, True)],
-                         ids=["rescale", "rubber_band", "rubber_band_rates"])
-def test_device_and_oracle_sample_the_same_posterior(ry1, urate):
+@pytest.mark.parametrize("ry1,urate,nw", [(False, False, False), (True, False, False),
+                                           (True, True, False), (True, False, True)],
+                         ids=["rescale", "rubber_band", "rubber_band_rates", "rubber_band_nw"])
+def test_device_and_oracle_sample_the_same_posterior(ry1, urate, nw):
+    """With nw, each time update is changet_NW with probability 1/2 (device:
+    IMA_MC3_NW; oracle: nw=1) and the chains' migration counts are compared
+    too."""
     model, samp, loci, base, ts = _setup(8)
     C, burn, nsamp, thin = 48, 400, 150, 20
-    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate)
-    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate)
+    a = _gpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate, nw)
+    b = _cpu_samples(model, samp, loci, base, ts, C, burn, nsamp, thin, ry1, urate, nw)
     worst = 0.0
     for s in range(a.shape[2]):
         pooled = np.concatenate([a[:, :, s].ravel(), b[:, :, s].ravel()])

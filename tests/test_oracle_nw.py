@@ -168,3 +168,36 @@ def test_nw_chain_keeps_the_joint_prior(P):
     assert stats.ks_2samp(nm, nm_fresh).pvalue > 1e-3
     for ch, ts in zip(chains[:200], tv[:200]):
         oracle.treeweight(ch[0], model, ts, samp)
+
+
+def test_mixed_sweep_keeps_the_joint_prior():
+    """The oracle's sweep with time updates drawn between RY1 and NW (nw=1)
+    and the genealogy slides, flat likelihood, started from exact draws of
+    the joint prior: after 200 sweeps split times, migration counts and
+    root times still follow that prior (KS against fresh draws)."""
+    Q, M, TM = 15.0, 3.0, 5.0
+    pt = sim.PopTree.two_pop()
+    model = immodel.build(pt, qmax=Q, mmax=M)
+    samp = [3, 3]
+    loc = sim.HKYLocus(6, np.zeros((6, 1), np.int8), np.ones(1, np.int32), 1,
+                       np.array([0.25] * 4))
+    rng = np.random.default_rng(5)
+    C = 3000
+    chains, tvs = [], []
+    for _ in range(C):
+        g, t = _prior_draw(rng, pt, model, samp, Q, M, TM)
+        chains.append([g])
+        tvs.append(t)
+    tv = np.ascontiguousarray(np.array(tvs))
+    sw, st = oracle.mc3_bench([loc], [1e-9], 1.0, [samp], model, chains, tv, np.ones(C), TM,
+                              seed=82, max_sweeps=200, scale_width=0.3, ry1=True, nw=1)
+    assert st[3] > 0.3 * st[2]
+    fresh = [_prior_draw(rng, pt, model, samp, Q, M, TM) for _ in range(C)]
+    nm = [sum(len(m) for m in ch[0].mig) for ch in chains]
+    nmf = [sum(len(m) for m in f[0].mig) for f in fresh]
+    rt = [ch[0].time[ch[0].up0[ch[0].root]] for ch in chains]
+    rtf = [f[0].time[f[0].up0[f[0].root]] for f in fresh]
+    assert stats.ks_2samp(tv[:, 0], [f[1][0] for f in fresh]).pvalue > 1e-3
+    assert stats.ks_2samp(nm, nmf).pvalue > 1e-3
+    assert stats.ks_2samp(rt, rtf).pvalue > 1e-3
+
