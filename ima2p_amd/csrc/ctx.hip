@@ -1990,9 +1990,11 @@ static int mc3_nw_mode(ima_ctx *c, bool on) {
   }
   if (treeweight_lds_bytes(P, emax) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "room for NW's migrations: %d prior events exceed LDS", emax);
-  if (nw_lds_bytes(maxn) > (size_t)c->max_lds)
+  if (nw_lds_bytes(maxn, 0, 0) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "the NW update of %d gene copies needs more LDS than %d B", maxn,
                 c->max_lds);
+  int maxcap = 0;
+  for (int i = 0; i < N; ++i) maxcap = std::max(maxcap, c->slots[m.first + i].mig_cap);
   grow_logfact(c, lfn);
   HIPCHK(c, reserve(m.nwk, m.C));
   HIPCHK(c, reserve(m.nwt, m.C));
@@ -2018,6 +2020,8 @@ static int mc3_nw_mode(ima_ctx *c, bool on) {
   a.model = d.prior;
   a.emax = emax;
   a.max_n = maxn;
+  a.max_cap = maxcap;
+  a.stage = nw_lds_bytes(maxn, maxcap, 1) <= (size_t)c->max_lds ? 1 : 0;
   a.nmig_bk = m.nwnbk.p;
   a.migweight = m.nwmig.p;
   a.status = m.nwst.p;
@@ -2458,6 +2462,19 @@ std::vector<Region> mc3_state_regions(ima_ctx *c) {
     add(c->d_prog.p + s0.prog_off, (s1.prog_off + s1.n - 1 - s0.prog_off) * sizeof(ProgOp));
     add(c->d_slots.p + m.first, N * sizeof(SlotMeta));
   }
+  if (d.nw) {
+    // the NW update rewrites populations, per-edge indexes, events and the
+    // slots' event counts
+    const int64_t nf = d.node_first;
+    if (!d.topo) {
+      add(c->d_pop.p + nf, (size_t)d.nnodes * 4);
+      add(c->d_slots.p + m.first, N * sizeof(SlotMeta));
+    }
+    add(c->d_mig_first.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_mig_count.p + nf, (size_t)d.nnodes * 4);
+    add(c->d_mig_edge.p, c->migs * 4);
+    add(c->d_mig_to.p, c->migs * 4);
+  }
   if (d.prior >= 0) {
     const size_t np = (size_t)std::max(d.nparam, 1);
     add(m.tv.p, (size_t)m.C * mc3_tw(d.P) * 8);
@@ -2502,6 +2519,10 @@ int ima_mc3_tune(ima_ctx *c, int sweeps, int flags, int *best, double *ms_per_sw
   // the state saved and restored is that of the sweeps to be timed
   int rc0 = mc3_topo_mode(c, (run_flags & IMA_MC3_TOPOLOGY) != 0);
   if (rc0 != IMA_OK) return rc0;
+  rc0 = mc3_nw_mode(c, (run_flags & IMA_MC3_NW) && (run_flags & IMA_MC3_SCALE));
+  if (rc0 != IMA_OK) return rc0;
+  // slot records (room for migrations, moved events) on the device first
+  if ((rc0 = upload_slot_meta(c)) != IMA_OK) return rc0;
   const size_t bytes = mc3_state_bytes(c);
   char *buf = nullptr;
   HIPCHK(c, hipMalloc(&buf, bytes));
@@ -3074,6 +3095,10 @@ int ima_t_nw_update(ima_ctx *c, const int32_t *slots, int nslots, const double *
   const int32_t goff[2] = {0, nslots};
   if ((rc = prior_prepare(c, slots, nslots, &model, &P, &emax, &lfn, goff, 1)) != IMA_OK) return rc;
   if (P < 2) return fail(c, IMA_ERR_ARG, "the NW update needs two or more populations");
+  if (c->mc3.live)
+    for (int i = 0; i < nslots; ++i)
+      if (slots[i] >= c->mc3.first && slots[i] < c->mc3.first + c->mc3.C * c->mc3.L)
+        return fail(c, IMA_ERR_STATE, "slot %d belongs to the device sweep's chains", slots[i]);
   if (period < 0 || period > P - 2) return fail(c, IMA_ERR_ARG, "split %d out of range", period);
   {
     const double t_u = period == 0 ? 0.0 : tvals[period - 1];
@@ -3082,9 +3107,12 @@ int ima_t_nw_update(ima_ctx *c, const int32_t *slots, int nslots, const double *
       return fail(c, IMA_ERR_ARG, "split times %g -> %g outside (%g, %g)", tvals[period], newt,
                   t_u, t_d);
   }
-  int max_n = 2;
-  for (int i = 0; i < nslots; ++i) max_n = std::max(max_n, c->slots[slots[i]].n);
-  if (nw_lds_bytes(max_n) > (size_t)c->max_lds)
+  int max_n = 2, max_cap = 0;
+  for (int i = 0; i < nslots; ++i) {
+    max_n = std::max(max_n, c->slots[slots[i]].n);
+    max_cap = std::max(max_cap, c->slots[slots[i]].mig_cap);
+  }
+  if (nw_lds_bytes(max_n, 0, 0) > (size_t)c->max_lds)
     return fail(c, IMA_ERR_SIZE, "the NW update of %d gene copies needs more LDS than %d B", max_n,
                 c->max_lds);
   if ((rc = upload_loci(c)) != IMA_OK) return rc;
@@ -3116,6 +3144,8 @@ int ima_t_nw_update(ima_ctx *c, const int32_t *slots, int nslots, const double *
   a.model = model;
   a.emax = nw_event_limit(c, P);
   a.max_n = max_n;
+  a.max_cap = max_cap;
+  a.stage = nw_lds_bytes(max_n, max_cap, 1) <= (size_t)c->max_lds ? 1 : 0;
   a.pop_bk = c->d_nw_pop.p;
   a.mf_bk = c->d_nw_mf.p;
   a.mc_bk = c->d_nw_mc.p;

@@ -271,6 +271,8 @@ struct NwArgs {
   int model = 0;
   int emax = 0;   // event limit of the genealogy-prior kernels
   int max_n = 2;  // largest gene-copy count (sizes the LDS)
+  int max_cap = 0;  // largest event count of a genealogy before the update
+  int stage = 0;    // 1: the genealogies are staged in LDS (it has room)
   int32_t *pop_bk = nullptr, *mf_bk = nullptr, *mc_bk = nullptr;
   int32_t *me_bk = nullptr, *mto_bk = nullptr;
   double *mt_bk = nullptr;
@@ -278,7 +280,7 @@ struct NwArgs {
   double *migweight = nullptr;  // [count]
   int32_t *status = nullptr;    // [count]: 0 untouched, 1 updated, 2 left (no room)
 };
-size_t nw_lds_bytes(int max_n);
+size_t nw_lds_bytes(int max_n, int max_cap, int stage);
 hipError_t launch_nw_update(const DeviceView &v, const NwArgs &a, hipStream_t st);
 hipError_t launch_nw_restore(const DeviceView &v, const NwArgs &a, const int32_t *decision,
                              hipStream_t st);

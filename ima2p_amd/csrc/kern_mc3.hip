@@ -1086,25 +1086,27 @@ __global__ __launch_bounds__(1024) void mc3_scale_finish(DeviceView v, MC3Dev m,
     int32_t *mc = const_cast<int32_t *>(v.mig_count), *me = const_cast<int32_t *>(v.mig_edge);
     int32_t *mto = const_cast<int32_t *>(v.mig_to);
     double *mt = const_cast<double *>(v.mig_time);
-    for (int l = 0; l < m.L; ++l) {
+    // one wavefront per locus, its lanes over edges and events
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave, nwv = blockDim.x / kWave;
+    for (int l = wv; l < m.L; l += nwv) {
       const int it = c * m.L + l;
       if (na.status[it] != 1) continue;
       const int s = m.first_slot + it;
       const SlotMeta sm = v.slots[s];
       const int64_t no = sm.node_off, mo = sm.mig_off;
       const int ne = 2 * sm.n - 1, nm = na.nmig_bk[it];
-      for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      for (int e = lane; e < ne; e += kWave) {
         pop[no + e] = na.pop_bk[no + e];
         mf[no + e] = na.mf_bk[no + e];
         mc[no + e] = na.mc_bk[no + e];
       }
-      for (int x = threadIdx.x; x < nm; x += blockDim.x) {
+      for (int x = lane; x < nm; x += kWave) {
         me[mo + x] = na.me_bk[mo + x];
         mto[mo + x] = na.mto_bk[mo + x];
         mt[mo + x] = na.mt_bk[mo + x];
       }
-      __syncthreads();  // every lane has read the slot record
-      if (threadIdx.x == 0) const_cast<SlotMeta *>(v.slots)[s].nmig = nm;
+      // the wavefront's lanes read the slot record above
+      if (lane == 0) const_cast<SlotMeta *>(v.slots)[s].nmig = nm;
     }
     for (int k = threadIdx.x; k < tw; k += blockDim.x) m.tv[c * tw + k] = m.tvbackup[c * tw + k];
     if (threadIdx.x == 0) part += m.probg[c];

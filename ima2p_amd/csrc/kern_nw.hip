@@ -23,11 +23,11 @@
 // sum over the edges' new counts places every edge's new migration list,
 // (D) each lane writes its edges' lists (migrations before the interval,
 // the drawn path, those after it), populations and per-edge indexes, and
-// the path probabilities; (E) one lane sums the Hastings term in the
-// reference's order.  Per-edge state lives in LDS.  Every edge draws from
-// its own Philox stream (first half of its counter range for the
-// population choice, second half for the count, times and destinations),
-// so the result does not depend on how edges map to lanes.
+// the path probabilities; (E) the lanes sum the Hastings term.  Per-edge
+// state lives in LDS, and so does the genealogy when there is room.  Every
+// edge draws from its own Philox stream (first half of its counter range
+// for the population choice, second half for the count, times and
+// destinations), so the result does not depend on how edges map to lanes.
 //
 // The genealogy's old populations and migrations are first copied to the
 // backup arrays (the update reads its old lists from there while it
@@ -160,6 +160,8 @@ struct NwLds {
   int32_t *own, *db, *da, *upa, *upb, *mcount, *mnew, *cm2a, *cm2b, *ncnt, *nfirst;
   double *lpf, *lpfr, *mtime, *mrate, *mrate_r, *gmf, *gmr;
 };
+__host__ __device__ inline size_t nw_state_bytes(int E) { return (size_t)E * (7 * 8 + 11 * 4); }
+
 __device__ inline NwLds nw_carve(char *base, int E) {
   NwLds s;
   double *d = reinterpret_cast<double *>(base);
@@ -234,27 +236,73 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
     }
     return;
   }
+  const int E = 2 * a.max_n - 1;
+  const NwLds L = nw_carve(nw_lds, E);
+  // with room in LDS the genealogy is staged there: the phases below walk
+  // edges and migration lists through chains of dependent reads
+  const bool stage = a.stage && sm.nmig <= a.max_cap;
+  double *s_time = reinterpret_cast<double *>(nw_lds + ((nw_state_bytes(E) + 7) & ~(size_t)7));
+  double *s_mt = s_time + E;
+  int32_t *s_up0 = reinterpret_cast<int32_t *>(s_mt + a.max_cap);
+  int32_t *s_up1 = s_up0 + E, *s_down = s_up1 + E, *s_pop = s_down + E, *s_mf = s_pop + E;
+  int32_t *s_mc = s_mf + E, *s_mto = s_mc + E;
+  __shared__ int32_t s_pope[2 * kMaxPops - 1], s_popdown[2 * kMaxPops - 1];
+  __shared__ double s_tv[kMaxPops];
   // backups: populations, per-edge indexes and migrations as they are
   {
     int32_t *pop = const_cast<int32_t *>(v.pop) + no;
     const int32_t *mf = v.mig_first + no, *mc = v.mig_count + no;
     for (int e = lane; e < ne; e += kWave) {
-      a.pop_bk[no + e] = pop[e];
-      a.mf_bk[no + e] = mf[e];
-      a.mc_bk[no + e] = mc[e];
+      const int32_t p = pop[e], f = mf[e], c = mc[e];
+      a.pop_bk[no + e] = p;
+      a.mf_bk[no + e] = f;
+      a.mc_bk[no + e] = c;
+      if (stage) {
+        s_pop[e] = p;
+        s_mf[e] = f;
+        s_mc[e] = c;
+        s_up0[e] = g.up0[e];
+        s_up1[e] = g.up1[e];
+        s_down[e] = g.down[e];
+        s_time[e] = g.time[e];
+      }
     }
     for (int x = lane; x < sm.nmig; x += kWave) {
-      a.me_bk[mo + x] = v.mig_edge[mo + x];
-      a.mto_bk[mo + x] = v.mig_to[mo + x];
-      a.mt_bk[mo + x] = v.mig_time[mo + x];
+      const int32_t ed = v.mig_edge[mo + x], to = v.mig_to[mo + x];
+      const double t = v.mig_time[mo + x];
+      a.me_bk[mo + x] = ed;
+      a.mto_bk[mo + x] = to;
+      a.mt_bk[mo + x] = t;
+      if (stage) {
+        s_mt[x] = t;
+        s_mto[x] = to;
+      }
     }
+    for (int x = lane; x < 2 * P - 1; x += kWave) {
+      s_pope[x] = md.pope[x];
+      s_popdown[x] = md.popdown[x];
+    }
+    for (int x = lane; x < P - 1; x += kWave) s_tv[x] = tvr[x];
     if (lane == 0) {
       a.nmig_bk[it] = sm.nmig;
       err = 0;
     }
   }
   __syncthreads();
-  const NwLds L = nw_carve(nw_lds, 2 * a.max_n - 1);
+  g.popdown = s_popdown;
+  g.pope = s_pope;
+  g.tv = s_tv;
+  if (stage) {
+    g.up0 = s_up0;
+    g.up1 = s_up1;
+    g.down = s_down;
+    g.pop = s_pop;
+    g.mf = s_mf;
+    g.mc = s_mc;
+    g.mto = s_mto;
+    g.time = s_time;
+    g.mt = s_mt;
+  }
   const int period_a = newt < oldt ? k + 1 : k, period_b = newt < oldt ? k : k + 1;
   const int dropa = md.join[3 * k], dropb = md.join[3 * k + 1], addp = md.join[3 * k + 2];
   const uint64_t key = a.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)((a.keyid ? a.keyid[it] : it) + 1));
@@ -547,32 +595,29 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
           mt[w] = g.mt[f + j];
           mto[w] = g.mto[f + j];
         }
+      // this edge's terms of the forward and reverse path probabilities
       if (L.mrate[e] > 0)
-        L.gmf[e] = getmprob_nw(period_a, lastp, L.mrate[e], L.mtime[e], L.mnew[e], L.upa[e],
-                               L.da[o], L.cm2a[e], npopsa);
+        L.gmf[e] = L.lpf[o] + getmprob_nw(period_a, lastp, L.mrate[e], L.mtime[e], L.mnew[e],
+                                          L.upa[e], L.da[o], L.cm2a[e], npopsa);
       if (L.mrate_r[e] > 0)
-        L.gmr[e] = getmprob_nw(period_b, lastp, L.mrate_r[e], L.mtime[e], L.mcount[e],
-                               L.upb[e], L.db[o], L.cm2b[e], npopsb);
+        L.gmr[e] = L.lpfr[o] + getmprob_nw(period_b, lastp, L.mrate_r[e], L.mtime[e],
+                                           L.mcount[e], L.upb[e], L.db[o], L.cm2b[e], npopsb);
     }
   }
-  __syncthreads();
-  // (E) the Hastings term in the order of the reference's update list
+  // (E) the Hastings term: each lane sums its edges' terms, then the lanes
+  // in a fixed tree order (the reference adds them edge by edge; the sums
+  // agree to rounding)
+  double denom = 0.0, num = 0.0;
+  for (int e = lane; e < ne; e += kWave) {
+    denom += L.gmf[e];
+    num += L.gmr[e];
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    denom += __shfl_xor(denom, off, kWave);
+    num += __shfl_xor(num, off, kWave);
+  }
   if (lane == 0) {
-    double denom = 0.0, num = 0.0;
-    for (int e = 0; e < ne; ++e) {
-      if (L.own[e] != e) continue;
-      int x = e;
-      for (int i = 0; i < 2; ++i) {
-        if (i == 1) {
-          if (!(g.time[e] <= td)) break;
-          const int d = g.down[e];
-          x = g.up0[d] == e ? g.up1[d] : g.up0[d];
-        }
-        if (x == root) continue;
-        if (L.mrate[x] > 0) denom += L.lpf[e] + L.gmf[x];
-        if (L.mrate_r[x] > 0) num += L.lpfr[e] + L.gmr[x];
-      }
-    }
     a.migweight[it] = num - denom;
     a.status[it] = 1;
     const_cast<SlotMeta *>(v.slots)[s].nmig = total;
@@ -612,12 +657,18 @@ __global__ __launch_bounds__(kWave) void nw_restore_kernel(DeviceView v, NwArgs 
 
 }  // namespace
 
-size_t nw_lds_bytes(int max_n) { return (size_t)(2 * max_n - 1) * (7 * 8 + 11 * 4); }
+size_t nw_lds_bytes(int max_n, int max_cap, int stage) {
+  const int E = 2 * max_n - 1;
+  const size_t base = nw_state_bytes(E);
+  if (!stage) return base;
+  return ((base + 7) & ~(size_t)7) + (size_t)E * 8 + (size_t)max_cap * 8 + (size_t)E * 6 * 4 +
+         (size_t)max_cap * 4;
+}
 
 hipError_t launch_nw_update(const DeviceView &v, const NwArgs &a, hipStream_t st) {
   if (a.count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(nw_update_kernel, dim3(a.count), dim3(kWave), nw_lds_bytes(a.max_n), st, v,
-                     a);
+  hipLaunchKernelGGL(nw_update_kernel, dim3(a.count), dim3(kWave),
+                     nw_lds_bytes(a.max_n, a.max_cap, a.stage), st, v, a);
   return hipGetLastError();
 }
 

@@ -276,15 +276,20 @@ def test_decisions_equal_for_every_group_size():
 
 
 @pytest.mark.parametrize("flags", [L.MC3_GRAPH | L.MC3_SCALE,
-                                   L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U])
+                                   L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U,
+                                   L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U
+                                   | L.MC3_NW], ids=["rescale", "rubber_band_rates", "with_nw"])
 def test_tuner_leaves_the_chains_where_they_were(flags):
     """ima_mc3_tune times every group size, on the caller's sweeps (with
-    the rubber band and the mutation-rate rounds in the second case), from
-    a saved state and restores it -- rates included: the sweeps that follow
-    equal those of an untuned context."""
+    the rubber band and the mutation-rate rounds in the second case, and
+    NW drawn against RY1 in the third), from a saved state and restores it
+    -- rates and migrations included: the sweeps that follow equal those of
+    an untuned context.  The site block is fixed at 64 patterns: a tuned
+    block of 128 would sum the FP32 pattern terms in another order."""
     out = []
     for tune in (False, True):
-        e, *_ = _three_pop_chains(6, 9, 61, L.PREC_FP32)
+        e, model, samp, locs, gens, first, u, ts = _three_pop_chains(6, 9, 61, L.PREC_FP32,
+                                                                     site_block=64)
         e.mc3_run(5, flags)
         if tune:
             best, ms = e.mc3_tune(8, flags)
@@ -294,7 +299,9 @@ def test_tuner_leaves_the_chains_where_they_were(flags):
         pdg, beta, stats, sweeps = e.mc3_read()
         probg, tv = e.mc3_read_prior()
         u, kappa = e.mc3_read_rates()
-        out.append((pdg, beta, probg, tv, stats, sweeps, u, kappa))
+        migs = np.concatenate([np.append(e.read_mig_events(first + i)[1], -1.0)
+                               for i in range(6 * 9)])
+        out.append((pdg, beta, probg, tv, stats, sweeps, u, kappa, migs))
         e.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
