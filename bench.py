@@ -8,10 +8,13 @@ chains: per (chain, locus) one genealogy proposal, scored in a single
 batched FP32-scaled likelihood launch plus one launch of genealogy weights,
 then accepted or rejected locus by locus with beta (dprobg + dpdg), probg
 the chain's genealogy prior integrated over the population-size and
-migration priors (update_gtree.cpp:827-961); on alternate sweeps a
-whole-chain rescaling of node, migration and split times that re-scores
-all 50 loci and the prior; then the chain swap.  value = chain-steps/s =
-chains x sweeps / wall time, with all state resident in HBM.
+migration priors (update_gtree.cpp:827-961); on alternate sweeps each
+chain's time update, drawn as qupdate draws it (ima_main_mpi.cpp:
+1867-1911): changet_RY1's rubber band on one split time, re-scoring all
+50 loci and the prior, or changet_NW, which redraws the migrations around
+the moved split time; every fifth sweep the mutation-rate round (changeu);
+then the chain swap.  value = chain-steps/s = chains x sweeps / wall
+time, with all state resident in HBM.
 
 Also reported: the HKY kernel's roofline (HIP events on the context's
 stream around the sweep's scoring launch repeated back to back, over the
@@ -82,8 +85,15 @@ def _cpu_worker(args):
     sweeps, _ = oracle.mc3_bench(wl.loci, wl.u, wl.kappa, wl.samppop, immodel.build(wl.poptree),
                                  chains, tvs, workloads.geometric_betas(len(chain_ids)),
                                  wl.tmax, seed=100 + wid, seconds=seconds, ry1=True,
-                                 urate=True)
+                                 urate=True, nw=1)
     return len(chain_ids) * sweeps, time.perf_counter() - t0
+
+
+def sweep_flags(L, nw=True):
+    """The device sweep of the reference's step: graph replay, the time
+    update on alternate sweeps (RY1, or NW with probability 1/2 for models
+    with migration), the mutation-rate round every fifth sweep."""
+    return L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U | (L.MC3_NW if nw else 0)
 
 
 def cpu_baseline(wl, seconds, cores):
@@ -223,7 +233,7 @@ def uneven_leg(sweeps=40):
     spec = workloads.uneven_mixed()
     e = Engine(0)
     spec.load(e)
-    flags = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U
+    flags = sweep_flags(L)
     group, group_ms = e.mc3_tune(4, flags)
     block = e.mc3_site_block()
     e.mc3_run(4, flags)
@@ -270,9 +280,9 @@ def is_leg(sweeps=2000):
                         precision=L.PREC_FP64)
     e = Engine(0)
     spec.load(e)
-    e.mc3_run(20, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(20, sweep_flags(L))
     t0 = time.perf_counter()
-    e.mc3_run(sweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(sweeps, sweep_flags(L))
     dt = time.perf_counter() - t0
     e.close()
     return {"chain_steps_per_s": C * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
@@ -429,10 +439,10 @@ def main():
     e, loci, first, model = load_engine(wl, L.PREC_FP32)
     setup_s = time.perf_counter() - t_setup
     # autotune the accept step's group size (the chains are left as they were)
-    group, group_ms = e.mc3_tune(10, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
-    e.mc3_run(args.warmup, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    group, group_ms = e.mc3_tune(10, sweep_flags(L))
+    e.mc3_run(args.warmup, sweep_flags(L))
     t0 = time.perf_counter()
-    e.mc3_run(args.steps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(args.steps, sweep_flags(L))
     dt = time.perf_counter() - t0
     value = C * args.steps / dt
     pdg, beta, stats, sweeps = e.mc3_read()
@@ -441,7 +451,7 @@ def main():
     e.timing(True)
     e.timing_reset()
     ksweeps = max(10, min(args.steps, 100))
-    e.mc3_run(ksweeps, L.MC3_TIMED | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(ksweeps, (sweep_flags(L) & ~L.MC3_GRAPH) | L.MC3_TIMED)
     sweep_n, sweep_ms = e.timing_get(L.K_MC3)
     acc_n, acc_ms = e.timing_get(L.K_ACCEPT)
     # HKY kernel: the sweep's scoring launch (every chain x locus of the
@@ -460,12 +470,20 @@ def main():
     extra = {}
     if not args.no_extra:
         e64, loci64, _, model64 = load_engine(wl, L.PREC_FP64)
-        e64.mc3_run(5, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+        e64.mc3_run(5, sweep_flags(L))
         t1 = time.perf_counter()
         k64 = max(10, args.steps // 4)
-        e64.mc3_run(k64, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+        e64.mc3_run(k64, sweep_flags(L))
         extra["batched_fp64_chain_steps_per_s"] = C * k64 / (time.perf_counter() - t1)
         e64.close()
+        # the same FP32 sweep with every time update RY1 (no NW)
+        e_r, _, _, _ = load_engine(wl, L.PREC_FP32)
+        e_r.mc3_prior_group(group)
+        e_r.mc3_run(args.warmup, sweep_flags(L, nw=False))
+        t1 = time.perf_counter()
+        e_r.mc3_run(args.steps, sweep_flags(L, nw=False))
+        extra["ry1_only_chain_steps_per_s"] = C * args.steps / (time.perf_counter() - t1)
+        e_r.close()
         pl = plain_leg(wl)
         extra["plain_per_locus_fp64"] = pl
         extra["plain_per_locus_fp64_chain_steps_per_s"] = pl["chain_steps_per_s"]
@@ -498,10 +516,12 @@ def main():
                    "genes_per_locus": n, "patterns_mean": float(S.mean()),
                    "patterns_min": int(S.min()), "patterns_max": int(S.max()),
                    "sweep": "1 genealogy proposal per (chain, locus) scored with data likelihood "
-                            "and integrated genealogy prior, loci in order + rubber-band update "
-                            "of one split time (changet_RY1) with full re-score of every locus "
-                            "on alternate sweeps + swap; every fifth sweep also changeu: per "
-                            "chain 50 pair moves of (u, kappa), each re-scoring two loci",
+                            "and integrated genealogy prior, loci in order + on alternate sweeps "
+                            "one split-time update per chain, changet_RY1 (rubber band, every "
+                            "locus re-scored) or changet_NW (migrations redrawn around the "
+                            "moved split time) with probability 1/2 each + swap; every fifth "
+                            "sweep also changeu: per chain 50 pair moves of (u, kappa), each "
+                            "re-scoring two loci",
                    "parallelism": "all chains on one GPU"},
         "roofline": {"bound": "mfma", "pipe": "VALU FP32 (pruning is not a dense contraction)",
                      "kernel": "hky_kernel<float>", "achieved": achieved,

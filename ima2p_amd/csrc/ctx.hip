@@ -397,6 +397,8 @@ void ima_ctx_destroy(ima_ctx *c) {
   release(c->d_nw_period);
   release(c->d_nw_w);
   release(c->d_nw_newt);
+  release(c->d_mv_off);
+  release(c->d_mv_cnt);
   release(c->d_models);
   release(c->d_mint);
   release(c->d_mdbl);
@@ -1811,35 +1813,48 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
   return IMA_OK;
 }
 
-// Gives slot sl room for at least cap migration events: its events move
-// to a new region at the end of the event arrays (host mirror and device).
-static int grow_mig_cap(ima_ctx *c, int sl, int cap) {
-  SlotMeta &d = c->slots[sl];
-  if (d.mig_cap >= cap) return IMA_OK;
-  const int64_t old = d.mig_off;
-  d.mig_off = (int64_t)c->migs;
-  d.mig_cap = cap;
-  c->migs += (size_t)cap;
+// Gives slots[i] room for at least caps[i] migration events: the events of
+// each slot that needs more move to a new region at the end of the event
+// arrays (host mirror and device).
+static int grow_mig_caps(ima_ctx *c, const int32_t *slots, const int32_t *caps, int n) {
+  std::vector<int64_t> from, to;
+  std::vector<int32_t> cnt;
+  for (int i = 0; i < n; ++i) {
+    SlotMeta &d = c->slots[slots[i]];
+    if (d.mig_cap >= caps[i]) continue;
+    from.push_back(d.mig_off);
+    to.push_back((int64_t)c->migs);
+    cnt.push_back(d.nmig);
+    d.mig_off = (int64_t)c->migs;
+    d.mig_cap = caps[i];
+    c->migs += (size_t)caps[i];
+  }
+  if (from.empty()) return IMA_OK;
   c->mig_edge.resize(c->migs, 0);
   c->mig_to.resize(c->migs, 0);
   c->mig_time.resize(c->migs, 0.0);
-  std::copy(c->mig_edge.begin() + old, c->mig_edge.begin() + old + d.nmig,
-            c->mig_edge.begin() + d.mig_off);
-  std::copy(c->mig_to.begin() + old, c->mig_to.begin() + old + d.nmig,
-            c->mig_to.begin() + d.mig_off);
-  std::copy(c->mig_time.begin() + old, c->mig_time.begin() + old + d.nmig,
-            c->mig_time.begin() + d.mig_off);
+  for (size_t i = 0; i < from.size(); ++i) {
+    std::copy(c->mig_edge.begin() + from[i], c->mig_edge.begin() + from[i] + cnt[i],
+              c->mig_edge.begin() + to[i]);
+    std::copy(c->mig_to.begin() + from[i], c->mig_to.begin() + from[i] + cnt[i],
+              c->mig_to.begin() + to[i]);
+    std::copy(c->mig_time.begin() + from[i], c->mig_time.begin() + from[i] + cnt[i],
+              c->mig_time.begin() + to[i]);
+  }
   HIPCHK(c, reserve(c->d_mig_edge, c->migs + 1, true, c->stream));
   HIPCHK(c, reserve(c->d_mig_to, c->migs + 1, true, c->stream));
   HIPCHK(c, reserve(c->d_mig_time, c->migs + 1, true, c->stream));
-  if (d.nmig) {
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_edge.p + d.mig_off, c->d_mig_edge.p + old,
-                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_to.p + d.mig_off, c->d_mig_to.p + old,
-                             (size_t)d.nmig * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_mig_time.p + d.mig_off, c->d_mig_time.p + old,
-                             (size_t)d.nmig * 8, hipMemcpyDeviceToDevice, c->stream));
-  }
+  // the device's events (their times may have moved since the host wrote
+  // them) follow to the new regions, one launch for every slot
+  const size_t m = from.size();
+  HIPCHK(c, reserve(c->d_mv_off, 2 * m));
+  HIPCHK(c, reserve(c->d_mv_cnt, m));
+  HIPCHK(c, hipMemcpyAsync(c->d_mv_off.p, from.data(), m * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_mv_off.p + m, to.data(), m * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_mv_cnt.p, cnt.data(), m * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_move_events((int)m, c->d_mv_off.p, c->d_mv_off.p + m, c->d_mv_cnt.p,
+                               c->d_mig_edge.p, c->d_mig_to.p, c->d_mig_time.p, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // the host arrays above are freed on return
   c->slots_meta_dirty = true;
   return IMA_OK;
 }
@@ -1968,19 +1983,24 @@ static int mc3_nw_mode(ima_ctx *c, bool on) {
     return IMA_OK;
   }
   if (d.nw) return mc3_nw_buffers(c);
-  if (d.prior < 0 || d.P < 2 || c->models[d.prior].nmig == 0)
-    return fail(c, IMA_ERR_STATE,
-                "the NW update needs the genealogy prior of a model with migration");
+  // as in the reference, NW is not drawn without migration (ima_main_mpi.cpp:
+  // 1808-1814), nor without split times to move
+  if (d.prior < 0 || d.P < 2 || c->models[d.prior].nmig == 0) return IMA_OK;
   const int N = m.C * m.L, P = d.P;
   int emax = d.emax, maxn = 2, rc;
   size_t lfn = 4;
+  {
+    std::vector<int32_t> sl(N), cp(N);
+    for (int i = 0; i < N; ++i) {
+      sl[i] = m.first + i;
+      cp[i] = 2 * c->slots[m.first + i].nmig + kNwRoom;
+    }
+    if ((rc = grow_mig_caps(c, sl.data(), cp.data(), N)) != IMA_OK) return rc;
+  }
   for (int ch = 0; ch < m.C; ++ch) {
     size_t tot = 4;
     for (int l = 0; l < m.L; ++l) {
       const int sl = m.first + ch * m.L + l;
-      const int nm = c->slots[sl].nmig;
-      const int cap = 2 * nm + kNwRoom;
-      if ((rc = grow_mig_cap(c, sl, cap)) != IMA_OK) return rc;
       const SlotMeta &s = c->slots[sl];
       emax = std::max(emax, (s.n - 1) + s.mig_cap + P - 1);
       maxn = std::max(maxn, s.n);
@@ -3078,10 +3098,9 @@ int ima_slots_reserve_migs(ima_ctx *c, int first, int count, int cap) {
   if (!c || first < 0 || count < 0 || first + count > (int)c->slots.size() || cap < 0)
     return IMA_ERR_ARG;
   hipSetDevice(c->device);
-  int rc;
-  for (int i = 0; i < count; ++i)
-    if ((rc = grow_mig_cap(c, first + i, cap)) != IMA_OK) return rc;
-  return IMA_OK;
+  std::vector<int32_t> sl(count), cp(count, cap);
+  for (int i = 0; i < count; ++i) sl[i] = first + i;
+  return grow_mig_caps(c, sl.data(), cp.data(), count);
 }
 
 int ima_t_nw_update(ima_ctx *c, const int32_t *slots, int nslots, const double *tvals,

@@ -68,6 +68,8 @@ struct ima_ctx {
   DevBuf<double> d_nw_mt;
   DevBuf<int32_t> d_nw_nbk, d_nw_st, d_nw_period;
   DevBuf<double> d_nw_w, d_nw_newt;
+  DevBuf<int64_t> d_mv_off;  // event regions moved to larger ones: from | to
+  DevBuf<int32_t> d_mv_cnt;
 
   // population models of the genealogy prior
   std::vector<ModelMeta> models;

@@ -284,6 +284,10 @@ size_t nw_lds_bytes(int max_n, int max_cap, int stage);
 hipError_t launch_nw_update(const DeviceView &v, const NwArgs &a, hipStream_t st);
 hipError_t launch_nw_restore(const DeviceView &v, const NwArgs &a, const int32_t *decision,
                              hipStream_t st);
+// Copies cnt[i] events of every slot i from offset from[i] to to[i]
+// (regions that do not overlap).
+hipError_t launch_move_events(int n, const int64_t *from, const int64_t *to, const int32_t *cnt,
+                              int32_t *me, int32_t *mto, double *mt, hipStream_t st);
 
 struct MC3Dev {
   int C, L;

@@ -242,10 +242,11 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
   // edges and migration lists through chains of dependent reads
   const bool stage = a.stage && sm.nmig <= a.max_cap;
   double *s_time = reinterpret_cast<double *>(nw_lds + ((nw_state_bytes(E) + 7) & ~(size_t)7));
-  double *s_mt = s_time + E;
-  int32_t *s_up0 = reinterpret_cast<int32_t *>(s_mt + a.max_cap);
+  double *s_mt = s_time + E;  // old event times, then the new list's
+  int32_t *s_up0 = reinterpret_cast<int32_t *>(s_mt + 2 * a.max_cap);
   int32_t *s_up1 = s_up0 + E, *s_down = s_up1 + E, *s_pop = s_down + E, *s_mf = s_pop + E;
-  int32_t *s_mc = s_mf + E, *s_mto = s_mc + E;
+  int32_t *s_mc = s_mf + E, *s_mto = s_mc + E;  // old destinations, then the new list's edges
+                                                 // and destinations
   __shared__ int32_t s_pope[2 * kMaxPops - 1], s_popdown[2 * kMaxPops - 1];
   __shared__ double s_tv[kMaxPops];
   // backups: populations, per-edge indexes and migrations as they are
@@ -511,9 +512,13 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
     int32_t *pop = const_cast<int32_t *>(v.pop) + no;
     int32_t *mf = const_cast<int32_t *>(v.mig_first) + no;
     int32_t *mcnt = const_cast<int32_t *>(v.mig_count) + no;
-    int32_t *me = const_cast<int32_t *>(v.mig_edge) + mo;
-    int32_t *mto = const_cast<int32_t *>(v.mig_to) + mo;
-    double *mt = const_cast<double *>(v.mig_time) + mo;
+    // with staging the new lists are built in LDS (the path's times are
+    // sorted there) and copied out in one coalesced pass below
+    double *n_mt = s_mt + a.max_cap;
+    int32_t *n_me = s_mto + a.max_cap, *n_mto = n_me + a.max_cap;
+    int32_t *me = stage ? n_me : const_cast<int32_t *>(v.mig_edge) + mo;
+    int32_t *mto = stage ? n_mto : const_cast<int32_t *>(v.mig_to) + mo;
+    double *mt = stage ? n_mt : const_cast<double *>(v.mig_time) + mo;
     const int32_t *pl = md.plist + period_a * P;
     const int npl = md.npresent[period_a];
     for (int e = lane; e < ne; e += kWave) {
@@ -604,6 +609,19 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
                                            L.mcount[e], L.upb[e], L.db[o], L.cm2b[e], npopsb);
     }
   }
+  if (stage) {
+    __syncthreads();  // the new lists are complete in LDS
+    int32_t *me = const_cast<int32_t *>(v.mig_edge) + mo;
+    int32_t *mto = const_cast<int32_t *>(v.mig_to) + mo;
+    double *mt = const_cast<double *>(v.mig_time) + mo;
+    const double *n_mt = s_mt + a.max_cap;
+    const int32_t *n_me = s_mto + a.max_cap, *n_mto = n_me + a.max_cap;
+    for (int x = lane; x < total; x += kWave) {
+      me[x] = n_me[x];
+      mt[x] = n_mt[x];
+      mto[x] = n_mto[x];
+    }
+  }
   // (E) the Hastings term: each lane sums its edges' terms, then the lanes
   // in a fixed tree order (the reference adds them edge by edge; the sums
   // agree to rounding)
@@ -655,14 +673,36 @@ __global__ __launch_bounds__(kWave) void nw_restore_kernel(DeviceView v, NwArgs 
   if (lane == 0) const_cast<SlotMeta *>(v.slots)[s].nmig = nm;
 }
 
+__global__ __launch_bounds__(kWave) void move_events_kernel(int n, const int64_t *from,
+                                                            const int64_t *to, const int32_t *cnt,
+                                                            int32_t *me, int32_t *mto, double *mt) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int64_t f = from[i], t = to[i];
+  for (int x = threadIdx.x; x < cnt[i]; x += kWave) {
+    me[t + x] = me[f + x];
+    mto[t + x] = mto[f + x];
+    mt[t + x] = mt[f + x];
+  }
+}
+
 }  // namespace
+
+hipError_t launch_move_events(int n, const int64_t *from, const int64_t *to, const int32_t *cnt,
+                              int32_t *me, int32_t *mto, double *mt, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(move_events_kernel, dim3(n), dim3(kWave), 0, st, n, from, to, cnt, me, mto,
+                     mt);
+  return hipGetLastError();
+}
 
 size_t nw_lds_bytes(int max_n, int max_cap, int stage) {
   const int E = 2 * max_n - 1;
   const size_t base = nw_state_bytes(E);
   if (!stage) return base;
-  return ((base + 7) & ~(size_t)7) + (size_t)E * 8 + (size_t)max_cap * 8 + (size_t)E * 6 * 4 +
-         (size_t)max_cap * 4;
+  // genealogy (edges), old and new event lists
+  return ((base + 7) & ~(size_t)7) + (size_t)E * 8 + (size_t)max_cap * 16 + (size_t)E * 6 * 4 +
+         (size_t)max_cap * 12;
 }
 
 hipError_t launch_nw_update(const DeviceView &v, const NwArgs &a, hipStream_t st) {

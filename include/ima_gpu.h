@@ -524,7 +524,9 @@ int ima_integrate_tree_prob(ima_ctx *ctx, int model, const double *wsum, int ngr
  * the genealogies' weights at the new time and accepts with U < min(1,
  * exp(beta dprobg + migweight)).  On first use every chain slot gets room
  * for twice its migrations plus 32; a genealogy whose new migrations
- * would not fit rejects its chain's move. */
+ * would not fit rejects its chain's move.  Without migration parameters
+ * (or without the prior) the flag changes nothing: the reference draws NW
+ * only for models with migration (ima_main_mpi.cpp:1808-1814). */
 #define IMA_MC3_NW 128
 int ima_mc3_setup(ima_ctx *ctx, int nchains, int nloci, int first_slot,
                   const double *u, const double *kappa, const double *beta,
