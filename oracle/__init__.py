@@ -616,8 +616,9 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     or [nchains, nloci]) and kappa (scalar or [nchains, nloci]) start each
     chain; rates=(u, kappa), two float64 [nchains, nloci] arrays, take their
     place and receive the final values.  topo: the genealogy update is the
-    reference's own slider move (models without migration; topologies,
-    labels and roots of the Genealogy objects are updated).  nw (models
+    reference's own move (slider_nomigration without migration parameters,
+    slider and addmigration with them; topologies, labels, roots and
+    migrations of the Genealogy objects are updated).  nw (models
     with migration): 1, each time update is changet_NW with probability 1/2
     (the reference's choice between RY1 and NW); 2, every sweep is one NW
     update per chain and nothing else; populations and migrations of the
@@ -659,7 +660,8 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
     roots = np.array([g.root for g in gens], np.int32)
     migs = [mig_arrays(g) for g in gens]
     nm = np.array([len(m[0]) for m in migs], np.int32)
-    if nw:
+    roomy = bool(nw) or (bool(topo) and model.nmig > 0)
+    if roomy:
         caps = np.array([k + max(int(mig_room), k) for k in nm], np.int32)
         room = [np.zeros(int(c - k), np.int32) for c, k in zip(caps, nm)]
         me = np.concatenate([x for m, r in zip(migs, room) for x in (m[0], r)]
@@ -692,7 +694,7 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
                            int(bool(urate)), int(bool(topo)), _ip(roots_out), int(nw),
                            _ip(caps) if caps is not None else None)
     if sw < 0:
-        raise ValueError("mc3_bench: a model with migration has no topology move here")
+        raise ValueError("mc3_bench: bad arguments")
     off = moff_ = 0
     for gi, (g, m) in enumerate(zip(gens, migs)):
         ne = 2 * g.n - 1
@@ -703,7 +705,7 @@ def mc3_bench(loci, u, kappa, samppop, model, chains, tvs, beta, tmax, seed=1,
             g.down[:] = dn[off:off + ne]
             g.pop[:] = pop[off:off + ne]
             g.root = int(roots_out[gi])
-        if nw:
+        if roomy:
             g.pop[:] = pop[off:off + ne]
             g.mig = [[] for _ in range(ne)]
             for j in range(int(nm[gi])):

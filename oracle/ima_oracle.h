@@ -190,6 +190,20 @@ double ora_getmprob_nw(int period, int lastp, double mrate, double mtime, int mc
  * Philox stream (key, ctr) */
 int ora_poisson_cond(double param, int cond, uint64_t key, uint64_t ctr);
 
+/* ---------------- updategenealogy's move with migration (ora_mig.c) ----- */
+/* One proposal on one genealogy (tree arrays of 2n-1 edges, root, pop and
+ * the packed migrations, sorted by edge then time, at most cap of them),
+ * all changed in place; split times tv[0..P-2]; uniforms from unif(ustate).
+ * Returns 0, 1 when the new migrations exceed cap (tree arrays changed, the
+ * migrations not written: the caller restores its copy), -1 for bad
+ * arguments.  hastings receives migweight + slideweight; rootmove whether
+ * the root moved. */
+int ora_gene_move_mig(int n, int *up0, int *up1, int *down, double *time, int *root, int *pop,
+                      int *nmig, int cap, int *mig_edge, double *mig_time, int *mig_to, int P,
+                      const int *plist, const int *npresent, const int *popdown,
+                      const int *pope, const double *tv, double (*unif)(void *), void *ustate,
+                      double *hastings, int *rootmove);
+
 /* ---------------- host MC3 sweep of the CPU baseline (ora_mc3.c) -------- */
 long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *seq,
                    const int *seq_off, const int *mult, const int *mult_off,

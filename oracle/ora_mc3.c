@@ -434,6 +434,87 @@ static void topo_move(const prob_t *pb, chain_t *ch, int l, rng_t *r, long *acc)
   free(bt);
 }
 
+/* the reference's move for a model with migration (ora_gene_move_mig):
+ * the whole genealogy, its populations and migrations are held for a
+ * rejection; decided as topo_move with the Hastings term migweight +
+ * slideweight.  A move whose migrations exceed the genealogy's room is
+ * rejected. */
+static double u_of(void *s) { return unif((rng_t *)s); }
+
+static void topo_move_mig(const prob_t *pb, chain_t *ch, int l, rng_t *r, long *acc) {
+  gene_t *g = &ch->g[l];
+  const int n = g->n, ne = 2 * n - 1, R = rec_len(pb->P);
+  int *bk = (int *)malloc(sizeof(int) * 4 * ne);
+  double *bt = (double *)malloc(sizeof(double) * ne);
+  memcpy(bk, g->up0, sizeof(int) * ne);
+  memcpy(bk + ne, g->up1, sizeof(int) * ne);
+  memcpy(bk + 2 * ne, g->down, sizeof(int) * ne);
+  memcpy(bk + 3 * ne, g->pop, sizeof(int) * ne);
+  memcpy(bt, g->time, sizeof(double) * ne);
+  const int broot = g->root, bnm = g->nmig;
+  int *bme = (int *)malloc(sizeof(int) * (bnm + 1)), *bmo = (int *)malloc(sizeof(int) * (bnm + 1));
+  double *bmt = (double *)malloc(sizeof(double) * (bnm + 1));
+  memcpy(bme, g->mig_edge, sizeof(int) * bnm);
+  memcpy(bmo, g->mig_to, sizeof(int) * bnm);
+  memcpy(bmt, g->mig_time, sizeof(double) * bnm);
+  double hast = 0.0;
+  int rm = 0;
+  const int rc = ora_gene_move_mig(n, g->up0, g->up1, g->down, g->time, &g->root, g->pop, &g->nmig,
+                                   g->mig_cap, g->mig_edge, g->mig_time, g->mig_to, pb->P,
+                                   pb->plist, pb->npresent, pb->popdown, pb->pope, ch->tv, u_of, r,
+                                   &hast, &rm);
+  int ok = 0, scored = 0;
+  double np = g->pdg, *nrec = ch->nrec;
+  if (rc == 0 && weights(pb, l, g, ch->tv, nrec) == 0) {
+    np = hky(pb, l, g, -1);
+    scored = 1;
+    const int P2 = pb->P * pb->P, P3 = P2 * pb->P;
+    for (int i = 0; i < R; i++) {
+      const double x = ch->all[i], o = g->rec[i], w = nrec[i];
+      ch->hold[i] = x;
+      if ((i >= P2 && i < 2 * P2) || (i >= 3 * P2 + P3 && i < 3 * P2 + 2 * P3))
+        ch->all[i] = fmax(0.0, (x - o) + w);
+      else if (i < P2 || (i >= 3 * P2 && i < 3 * P2 + P3))
+        ch->all[i] = x + (w - o);
+      else
+        ch->all[i] = (x - o) + w;
+    }
+    const double pn = prior_terms(pb, ch->all, ch->qc, ch->qf, ch->qt, ch->nc, ch->nf, ch->nt);
+    const double U = unif(r);
+    ok = U < fmin(1.0, exp(ch->beta * ((pn - ch->probg) + (np - g->pdg)) + hast));
+    if (ok) {
+      g->pdg = np;
+      ch->probg = pn;
+      memcpy(g->rec, nrec, sizeof(double) * R);
+      const int np_ = pb->npsize + pb->nmig;
+      memcpy(ch->qc, ch->nc, sizeof(int) * np_);
+      memcpy(ch->qf, ch->nf, sizeof(double) * np_);
+      memcpy(ch->qt, ch->nt, sizeof(double) * np_);
+      (*acc)++;
+    } else {
+      memcpy(ch->all, ch->hold, sizeof(double) * R);
+    }
+  }
+  if (!ok) {
+    memcpy(g->up0, bk, sizeof(int) * ne);
+    memcpy(g->up1, bk + ne, sizeof(int) * ne);
+    memcpy(g->down, bk + 2 * ne, sizeof(int) * ne);
+    memcpy(g->pop, bk + 3 * ne, sizeof(int) * ne);
+    memcpy(g->time, bt, sizeof(double) * ne);
+    g->root = broot;
+    g->nmig = bnm;
+    memcpy(g->mig_edge, bme, sizeof(int) * bnm);
+    memcpy(g->mig_to, bmo, sizeof(int) * bnm);
+    memcpy(g->mig_time, bmt, sizeof(double) * bnm);
+  }
+  if (scored) hky_settle(g, ok);
+  free(bk);
+  free(bt);
+  free(bme);
+  free(bmo);
+  free(bmt);
+}
+
 static void rescale(const prob_t *pb, chain_t *ch, rng_t *r, long *acc) {
   const int L = pb->L, R = rec_len(pb->P), np_ = pb->npsize + pb->nmig;
   const double ls = pb->scale_width * (unif(r) - 0.5), s = exp(ls);
@@ -805,8 +886,8 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
                P, npsize, nmigp, plist, npresent, popdown, pope, join,
                theta_off, theta_wp, mig_off, mig_wp, qmax, mmax, tmax, scale_width,
                ry1_update && P > 1, topo_update, (P > 1 && nmigp > 0) ? nw_update : 0};
-  if (topo_update && nmigp > 0) return -1; /* no migration-path proposals */
-  if (pb.nw && !mig_cap) return -1;       /* new migrations need room */
+  /* new migrations (NW, the move with migration paths) need room */
+  if ((pb.nw || (topo_update && nmigp > 0)) && !mig_cap) return -1;
   const int R = rec_len(P), np_ = npsize + nmigp;
   if (P > 10 || P - 1 > 16) return -1;
   chain_t *ch = (chain_t *)calloc(nchains, sizeof(chain_t));
@@ -881,7 +962,9 @@ long ora_mc3_bench(int nchains, int L, const int *n_l, const int *S, const int *
         continue;
       }
       for (int l = 0; l < L; l++) {
-        if (pb.topo)
+        if (pb.topo && nmigp > 0)
+          topo_move_mig(&pb, &ch[c], l, &r, &acc[0]);
+        else if (pb.topo)
           topo_move(&pb, &ch[c], l, &r, &acc[0]);
         else
           slide(&pb, &ch[c], l, &r, &acc[0]);

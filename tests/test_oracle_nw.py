@@ -1,5 +1,7 @@
-"""The oracle's restatement of changet_NW (oracle/ora_nw.c; update_t_NW.cpp:
-920-1032), pinned by what the mathematics fixes:
+"""The oracle's restatements of the moves that draw migration paths --
+changet_NW (oracle/ora_nw.c; update_t_NW.cpp:920-1032) and updategenealogy's
+move for models with migration (oracle/ora_mig.c; update_gtree.cpp:723-826)
+-- pinned by what the mathematics fixes:
 
 * getmprob_NW is the log probability of the path simmpath_tNW draws, so
   summed over every count and every destination sequence (times integrated
@@ -17,7 +19,10 @@
   migration counts after 40 updates against fresh draws (KS).  Dropping
   the proposal's population-choice terms, flipping the Hastings term or
   taking sinh for cosh in the two-population period moves those laws by
-  far more than the test allows.
+  far more than the test allows;
+* the genealogy move with migration, under a flat likelihood from exact
+  draws of the joint prior, keeps migration counts, root times and the
+  topology on that prior.
 """
 import math
 
@@ -201,3 +206,49 @@ def test_mixed_sweep_keeps_the_joint_prior():
     assert stats.ks_2samp(nm, nmf).pvalue > 1e-3
     assert stats.ks_2samp(rt, rtf).pvalue > 1e-3
 
+
+
+def _cherry(g):
+    return int(any({int(g.up0[e]), int(g.up1[e])} == {0, 1} for e in range(g.n, 2 * g.n - 1)))
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_migration_move_keeps_the_joint_prior(P):
+    """updategenealogy's move for a model with migration (oracle/ora_mig.c:
+    slider, joinsisdown, splitsisdown, addmigration with mwork_single_edge,
+    mwork_two_edges, simmpath and getmprob) under a flat likelihood, from
+    exact draws of the joint prior: after 150 sweeps of moves the migration
+    counts, root times and a topology statistic (gene copies 0 and 1 a
+    cherry) still follow that prior (KS / binomial against fresh draws).
+    Dropping the root move's slideweight, flipping migweight, dropping
+    (2 - count) log(np - 1) or the joint final population's probability
+    fails it."""
+    Q, TM = 4.0, 3.0
+    M = 1.5 if P == 2 else 3.0
+    pt = sim.PopTree.two_pop() if P == 2 else sim.PopTree.three_pop_01_2()
+    model = immodel.build(pt, qmax=Q, mmax=M)
+    samp = [2, 2] if P == 2 else [3, 3, 2]
+    n = sum(samp)
+    loc = sim.HKYLocus(n, np.zeros((n, 1), np.int8), np.ones(1, np.int32), 1,
+                       np.array([0.25] * 4))
+    rng = np.random.default_rng(60 + P)
+    C = 4000
+    chains, tvs = [], []
+    for _ in range(C):
+        g, t = _prior_draw(rng, pt, model, samp, Q, M, TM)
+        chains.append([g])
+        tvs.append(t)
+    tv = np.ascontiguousarray(np.array(tvs))
+    sw, st = oracle.mc3_bench([loc], [1e-9], 1.0, [samp], model, chains, tv, np.ones(C), TM,
+                              seed=70 + P, max_sweeps=150, scale_width=0.0, topo=True)
+    assert 0.2 * st[0] < st[1] < 0.9 * st[0]
+    fresh = [_prior_draw(rng, pt, model, samp, Q, M, TM)[0] for _ in range(C)]
+    gs = [ch[0] for ch in chains]
+    nm = lambda xs: [sum(len(m) for m in g.mig) for g in xs]
+    rt = lambda xs: [g.time[g.up0[g.root]] for g in xs]
+    assert stats.ks_2samp(nm(gs), nm(fresh)).pvalue > 1e-3
+    assert stats.ks_2samp(rt(gs), rt(fresh)).pvalue > 1e-3
+    a, b = np.mean([_cherry(g) for g in gs]), np.mean([_cherry(g) for g in fresh])
+    assert abs(a - b) < 4.5 * math.sqrt(a * (1 - a) / C + b * (1 - b) / C)
+    for i in range(200):
+        oracle.treeweight(gs[i], model, tv[i], samp)  # labels fit the split times
