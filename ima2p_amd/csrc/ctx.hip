@@ -1673,6 +1673,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   d.arange = nullptr;
   d.pitems = nullptr;
   d.ptw = nullptr;
+  d.mig_pool = 0;
   d.maxn = 2;
   for (int i = 0; i < N; ++i) d.maxn = std::max(d.maxn, c->slots[first_slot + i].n);
   m.topo_first = -1;
@@ -1745,6 +1746,23 @@ static int mc3_urate_round(ima_ctx *c, uint64_t ctr, bool serial) {
 // first use the proposal slots are allocated (one per chain and locus) and
 // the stepwise loci' allele ranges set from chain 0's gene copies
 // (L[li].minA / maxA, build_gtree.cpp:707-722).
+static int grow_mig_caps(ima_ctx *c, const int32_t *slots, const int32_t *caps, int n);
+// Room for new migrations a slot gets beyond twice its count, when a move
+// that draws migration paths is first used.
+constexpr int kNwRoom = 32;
+
+// The move with migration rewrites migration times that the rescaling
+// backup covers: the backup follows the event arrays' size.
+static int mc3_topo_mig_buffers(ima_ctx *c) {
+  auto &m = c->mc3;
+  HIPCHK(c, reserve(m.mtbackup, c->migs + 1));
+  if (m.dev.mtbackup != m.mtbackup.p) {
+    m.dev.mtbackup = m.mtbackup.p;
+    mc3_drop_graphs(c);
+  }
+  return IMA_OK;
+}
+
 static int mc3_topo_mode(ima_ctx *c, bool on) {
   auto &m = c->mc3;
   MC3Dev &d = m.dev;
@@ -1756,10 +1774,7 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
     }
     return IMA_OK;
   }
-  if (d.topo) return IMA_OK;
-  if (d.prior >= 0 && c->models[d.prior].nmig > 0)
-    return fail(c, IMA_ERR_STATE,
-                "the device's genealogy move needs a model without migration parameters");
+  if (d.topo) return d.mig_pool > 0 ? mc3_topo_mig_buffers(c) : IMA_OK;
   {
     // any topology may come up: the pruning stack must hold floor(log2 n)
     int bound = 1;
@@ -1802,6 +1817,46 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
   int bound = 1;
   while ((2 << bound) <= m.hmax_n) ++bound;
   m.hmax_slots = std::max(m.hmax_slots, bound);
+  d.mig_pool = 0;
+  if (d.prior >= 0 && c->models[d.prior].nmig > 0 && d.P > 1) {
+    // the move draws migration paths: every chain slot and its proposal
+    // slot get the same room (twice the chain's migrations plus kNwRoom),
+    // the prior's event bound and log-factorial table grow to match, and
+    // each wavefront holds an LDS pool of four times the largest room
+    std::vector<int32_t> sl(2 * (size_t)N), cp(2 * (size_t)N);
+    for (int i = 0; i < N; ++i) {
+      const SlotMeta &cs = c->slots[m.first + i];
+      const int cap = std::max(cs.mig_cap, 2 * cs.nmig + kNwRoom);
+      sl[i] = m.first + i;
+      sl[N + i] = m.topo_first + i;
+      cp[i] = cp[N + i] = cap;
+    }
+    if ((rc = grow_mig_caps(c, sl.data(), cp.data(), 2 * N)) != IMA_OK) return rc;
+    int emax = d.emax, maxcap = 0;
+    size_t lfn = 4;
+    for (int ch = 0; ch < m.C; ++ch) {
+      size_t tot = 4;
+      for (int l = 0; l < m.L; ++l) {
+        const SlotMeta &cs = c->slots[m.first + ch * m.L + l];
+        emax = std::max(emax, (cs.n - 1) + cs.mig_cap + d.P - 1);
+        maxcap = std::max(maxcap, cs.mig_cap);
+        tot += (size_t)cs.n - 1 + (size_t)cs.mig_cap;
+      }
+      lfn = std::max(lfn, tot);
+    }
+    if (treeweight_lds_bytes(d.P, emax) > (size_t)c->max_lds)
+      return fail(c, IMA_ERR_SIZE, "room for the move's migrations: %d prior events exceed LDS",
+                  emax);
+    const int pool = 4 * std::max(maxcap, 16);
+    if (topo_mig_lds_bytes(d.maxn, pool) > (size_t)c->max_lds)
+      return fail(c, IMA_ERR_SIZE, "the move with migration on %d gene copies and %d events "
+                  "needs more LDS than %d B", d.maxn, maxcap, c->max_lds);
+    grow_logfact(c, lfn);
+    d.emax = emax;
+    if (d.nw) d.nwa.emax = emax;
+    d.mig_pool = pool;
+    if ((rc = mc3_topo_mig_buffers(c)) != IMA_OK) return rc;
+  }
   d.topo = 1;
   d.thast = m.thast.p;
   d.hast = m.thast.p;
@@ -1971,7 +2026,6 @@ static int mc3_nw_buffers(ima_ctx *c) {
 // chain slot gets room for new migrations (kNwRoom beyond twice its
 // count), and the event bound of the genealogy-prior kernels, the
 // log-factorial table and the backups grow to match.
-constexpr int kNwRoom = 32;
 static int mc3_nw_mode(ima_ctx *c, bool on) {
   auto &m = c->mc3;
   MC3Dev &d = m.dev;
@@ -2099,6 +2153,7 @@ static int mc3_topo_sync(ima_ctx *c) {
   if (pb)
     HIPCHK(c, copy_now(c, c->prog.data() + s0.prog_off, c->d_prog.p + s0.prog_off,
                        pb * sizeof(ProgOp), hipMemcpyDeviceToHost));
+  if (m.dev.mig_pool > 0) return mc3_nw_sync(c);  // the migrations the moves drew
   return IMA_OK;
 }
 
@@ -2402,6 +2457,12 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
   d.nparam = np;
   d.emax = emax;
   d.nw = 0;  // NW's room for new migrations is made again on its next use
+  if (d.topo) {
+    // the genealogy move is set up again for this model on its next use
+    d.topo = 0;
+    d.mig_pool = 0;
+    d.hast = nullptr;
+  }
   d.tmax = tmax;
   d.tv = m.tv.p;
   d.tvbackup = m.tvbackup.p;
@@ -2482,9 +2543,9 @@ std::vector<Region> mc3_state_regions(ima_ctx *c) {
     add(c->d_prog.p + s0.prog_off, (s1.prog_off + s1.n - 1 - s0.prog_off) * sizeof(ProgOp));
     add(c->d_slots.p + m.first, N * sizeof(SlotMeta));
   }
-  if (d.nw) {
-    // the NW update rewrites populations, per-edge indexes, events and the
-    // slots' event counts
+  if (d.nw || d.mig_pool > 0) {
+    // NW and the move with migration rewrite populations, per-edge indexes,
+    // events and the slots' event counts
     const int64_t nf = d.node_first;
     if (!d.topo) {
       add(c->d_pop.p + nf, (size_t)d.nnodes * 4);

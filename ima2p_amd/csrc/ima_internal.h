@@ -351,6 +351,8 @@ struct MC3Dev {
   ima_item *pitems;      // [C*L] the proposals' scoring items
   int32_t *ptw;          // [C*L] the proposals' slots
   int maxn;              // largest gene-copy count of the chains' loci
+  int mig_pool;          // > 0: the genealogy move draws migration paths, with an LDS
+                         // event pool of this many entries per genealogy
   // changet_NW as the other time update (nw = 1): a chain that draws it
   // this sweep has its split nwk[c] (else -1) moving to nwt[c]; the
   // per-genealogy update (kern_nw.hip) reads nwa
@@ -410,6 +412,7 @@ hipError_t enqueue_mc3_record(const DeviceView &v, const MC3Dev &m, float *row, 
 // The reference's genealogy move for every item into its proposal slot
 // (kern_topo.hip), dynamic LDS topo_lds_bytes(max_n).
 size_t topo_lds_bytes(int max_n);
+size_t topo_mig_lds_bytes(int max_n, int pool);
 hipError_t enqueue_mc3_propose_topo(const DeviceView &v, const MC3Dev &m, int max_n,
                                     hipStream_t st);
 // Items and weight slots of the proposals: items[it] with slot

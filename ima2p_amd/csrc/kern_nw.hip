@@ -37,84 +37,16 @@
 #include <float.h>
 
 #include "ima_internal.h"
+#include "mig_dev.h"
 #include "philox.h"
 
 namespace ima {
 namespace {
 
 constexpr double kMigSimFrac = 0.999;  // MIGSIMFRAC
-constexpr double kLog2 = 0.69314718055994530941723212146;
 constexpr double kLog2Half = 0.34657359027997265470861606073;
-constexpr int kAddMigMax = 1000;  // ADDMIGMAX
 constexpr uint64_t kEdgeMix = 0xC2B2AE3D27D4EB4Full;
 constexpr uint64_t kSecondHalf = 1ull << 15;
-constexpr int kMaxTries = 1 << 20;     // bound of every rejection loop
-constexpr int kPoissonMax = 1 << 20;   // bound of one Poisson product loop
-
-__device__ inline int nw_randposint(PhiloxStream &r, int n) {
-  return (int)floor(r.uniform() * n);
-}
-
-__device__ inline double calcmrate(int mc, double mt) {
-  if (mt <= 0.0) return 1.0;
-  if (mc == 0) return mt < 1 ? 0.1 : 0.1 / mt;
-  return mt < 1 ? (double)mc : (double)mc / mt;
-}
-
-__device__ inline double nw_normdev(PhiloxStream &r, double mean, double stdev) {
-#pragma clang fp contract(off)
-  double v1, v2, rsq;
-  do {
-    v1 = 2.0 * r.uniform() - 1.0;
-    v2 = 2.0 * r.uniform() - 1.0;
-    rsq = v1 * v1 + v2 * v2;
-  } while (rsq >= 1.0 || rsq == 0.0);
-  return v2 * sqrt(-2.0 * log(rsq) / rsq) * stdev + mean;
-}
-
-// condition 0 even, 1 odd, 2 not 0, 3 not 1
-__device__ int poisson_cond(PhiloxStream &r, double param, int cond) {
-#pragma clang fp contract(off)
-  if (param < 0.25 && cond == 1) {
-    const double U = r.uniform();
-    if (U < param / sinh(param)) return 1;
-    if (U < param * (6 + param * param) / (6 * sinh(param))) return 3;
-    return 5;
-  }
-  if (param < 0.25 && cond == 2) {
-    double raised = exp(-param);
-    double rcheck = raised = param * raised / (1 - raised);
-    const double U = r.uniform();
-    int i = 1;
-    while (U > rcheck) {
-      raised *= param / (i + 1);
-      rcheck += raised;
-      i++;
-    }
-    return i;
-  }
-  int i, stop, tries = 0;
-  do {
-    if (param >= 100.0) {
-      const long v = (long)(nw_normdev(r, param, param) + 0.5);
-      i = v > 0 ? (int)(v > 2000000000L ? 2000000000L : v) : 0;
-    } else {
-      const double raised = exp(-param);
-      double U = r.uniform();
-      for (i = 0; U > raised && i < kPoissonMax; i++) U *= r.uniform();
-    }
-    stop = cond == 0 ? !(i & 1) : cond == 1 ? (i & 1) : cond == 2 ? i != 0 : i != 1;
-    if (!stop && ++tries >= kMaxTries) {  // a bound no valid rate reaches
-      i = cond == 1 || cond == 2 ? 1 : 0;
-      stop = 1;
-    }
-  } while (!stop);
-  if (i > kAddMigMax) i = cond == 1 ? kAddMigMax - 1 : kAddMigMax;
-  return i;
-}
-
-__device__ inline double logcosh_(double x) { return x < 100 ? log(cosh(x)) : x - kLog2; }
-__device__ inline double logsinh_(double x) { return x < 100 ? log(sinh(x)) : x - kLog2; }
 
 __device__ double getmprob_nw(int period, int lastp, double mrate, double mtime, int mcount,
                               int uppop, int dpop, int cm2pop, int numpops) {
@@ -579,14 +511,14 @@ __global__ __launch_bounds__(kWave) void nw_update_kernel(DeviceView v, NwArgs a
         if (add >= 2) {
           for (int j = 0; j < add - 2; ++j) {
             int q, tries = 0;
-            do q = nw_randposint(r, npl);
+            do q = mig_randposint(r, npl);
             while (pl[q] == last && ++tries < kMaxTries);
             mto[w + j] = pl[q];
             last = pl[q];
           }
           L.cm2a[e] = last;
           int q, tries = 0;
-          do q = nw_randposint(r, npl);
+          do q = mig_randposint(r, npl);
           while ((pl[q] == last || pl[q] == L.da[o]) && ++tries < kMaxTries);
           mto[w + add - 2] = pl[q];
         }

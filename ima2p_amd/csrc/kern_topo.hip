@@ -28,6 +28,7 @@
 #include <float.h>
 
 #include "ima_internal.h"
+#include "mig_dev.h"
 #include "philox.h"
 
 namespace ima {
@@ -94,6 +95,92 @@ __device__ double join_time(const ModelView &md, const double *tv, int P, int a,
 
 __device__ inline double upper_time(const int32_t *u0, const double *tm, int n, int e) {
   return e < n ? 0.0 : tm[u0[e]];
+}
+
+// finishSWupdateA's proposal for the freed node's allele (update_gtree_common
+// .cpp:2219-2362) and its Hastings term; the likelihood is scored in full
+// afterwards.
+__device__ double sw_allele(PhiloxStream &r, const MC3Dev &m, int it, int32_t *A,
+                            const int32_t *dn, int edge, int sis, int freed, int copyA0,
+                            int copyA1, int old_down_of_freed, int hold_down_A) {
+  const int oldA = A[freed];
+  const int e3[3] = {edge, sis, dn[edge]};
+  int j = 0, wsum = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (dn[e3[k]] == -1) continue;
+    wsum += abs((k < 2 ? A[e3[k]] : A[dn[e3[k]]]) - oldA);
+    ++j;
+  }
+  double gnew = j / (double)(wsum + j);
+  if (gnew > 0.95) gnew = 0.95;
+  const double uni = 1.0 - r.uniform();  // geometric() by inversion, in (0, 1]
+  int dA = (int)ceil(log(uni) / log(1.0 - gnew)) - 1;
+  if (r.next32() & 1u) dA = -dA;
+  const int2 rng_ = m.arange[it % m.L];
+  const int newA = dA >= 0 ? min(rng_.y, oldA + dA) : max(rng_.x, oldA + dA);
+  A[freed] = newA;
+  dA = newA - oldA;
+  int rsum = abs(copyA0 - newA) + abs(copyA1 - newA), rr = 2;
+  if (old_down_of_freed >= 0) {
+    rsum += abs(hold_down_A - newA);
+    ++rr;
+  }
+  double gold = rr / (double)(rsum + rr);
+  if (gold > 0.95) gold = 0.95;
+  return (abs(dA) * log(1.0 - gold) + log(gold)) - (abs(dA) * log(1.0 - gnew) + log(gnew));
+}
+
+// The postorder program of a topology: Strahler need bottom-up, then the
+// daughter with the larger need first (the host's emit).  Returns the
+// LDS partial slots the program needs.
+__device__ int emit_program(int n, int ne, int root, const int32_t *u0, const int32_t *u1,
+                            int32_t *need, int32_t *stk, ProgOp *prog) {
+  for (int e = 0; e < ne; ++e) need[e] = e < n ? 0 : -1;
+  int sp = 0;
+  stk[sp++] = root;
+  while (sp > 0) {
+    const int e = stk[sp - 1];
+    if (need[e] >= 0) {
+      --sp;
+      continue;
+    }
+    const int a = u0[e], b = u1[e];
+    if (need[a] >= 0 && need[b] >= 0) {
+      need[e] = need[a] == need[b] ? need[a] + 1 : max(need[a], need[b]);
+      --sp;
+    } else {
+      if (need[a] < 0) stk[sp++] = a;
+      if (need[b] < 0) stk[sp++] = b;
+    }
+  }
+  int nop = 0;
+  sp = 0;
+  stk[sp++] = root;
+  // entries: e (descend) or ~e (emit e's operation)
+  while (sp > 0) {
+    const int x = stk[--sp];
+    const int e = x >= 0 ? x : ~x;
+    int a = u0[e], b = u1[e];
+    if (need[a] < need[b]) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const bool ai = a >= n, bi = b >= n;
+    if (x >= 0) {
+      stk[sp++] = ~e;
+      if (bi) stk[sp++] = b;
+      if (ai) stk[sp++] = a;
+    } else {
+      ProgOp op;
+      op.a = a;
+      op.b = b;
+      op.type = (ai && bi) ? OP_SS : (ai ? OP_ST : OP_TT);
+      op.node = e;
+      prog[nop++] = op;
+    }
+  }
+  return max(1, need[root]);
 }
 
 }  // namespace
@@ -247,86 +334,14 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo(DeviceView v, MC3Dev m
       const double sd1 = fmin(kSlideStdvMax, roottime / 3.0);
       hast += log_normprob(sd1, hold) - log_normprob(sd0, hold);
     }
-    if (sw) {
-      // the freed node's allele (finishSWupdateA's proposal and its
-      // Hastings term; the likelihood is scored in full afterwards)
-      const int oldA = A[freed];
-      const int e3[3] = {edge, sis, dn[edge]};
-      int j = 0, wsum = 0;
-      for (int k = 0; k < 3; ++k) {
-        if (dn[e3[k]] == -1) continue;
-        wsum += abs((k < 2 ? A[e3[k]] : A[dn[e3[k]]]) - oldA);
-        ++j;
-      }
-      double gnew = j / (double)(wsum + j);
-      if (gnew > 0.95) gnew = 0.95;
-      const double uni = 1.0 - r.uniform();  // geometric() by inversion, in (0, 1]
-      int dA = (int)ceil(log(uni) / log(1.0 - gnew)) - 1;
-      if (r.next32() & 1u) dA = -dA;
-      const int2 rng_ = m.arange[it % m.L];
-      const int newA = dA >= 0 ? min(rng_.y, oldA + dA) : max(rng_.x, oldA + dA);
-      A[freed] = newA;
-      dA = newA - oldA;
-      int rsum = abs(copyA0 - newA) + abs(copyA1 - newA), rr = 2;
-      if (old_down_of_freed >= 0) {
-        rsum += abs(hold_down_A - newA);
-        ++rr;
-      }
-      double gold = rr / (double)(rsum + rr);
-      if (gold > 0.95) gold = 0.95;
-      hast += (abs(dA) * log(1.0 - gold) + log(gold)) - (abs(dA) * log(1.0 - gnew) + log(gnew));
-    }
-    // the postorder program: Strahler need bottom-up, then the daughter
-    // with the larger need first
-    for (int e = 0; e < ne; ++e) need[e] = e < n ? 0 : -1;
-    int sp = 0;
-    stk[sp++] = root;
-    while (sp > 0) {
-      const int e = stk[sp - 1];
-      if (need[e] >= 0) {
-        --sp;
-        continue;
-      }
-      const int a = u0[e], b = u1[e];
-      if (need[a] >= 0 && need[b] >= 0) {
-        need[e] = need[a] == need[b] ? need[a] + 1 : max(need[a], need[b]);
-        --sp;
-      } else {
-        if (need[a] < 0) stk[sp++] = a;
-        if (need[b] < 0) stk[sp++] = b;
-      }
-    }
+    if (sw)
+      hast += sw_allele(r, m, it, A, dn, edge, sis, freed, copyA0, copyA1, old_down_of_freed,
+                        hold_down_A);
     ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
-    int nop = 0;
-    sp = 0;
-    stk[sp++] = root;
-    // entries: e (descend) or ~e (emit e's operation)
-    while (sp > 0) {
-      const int x = stk[--sp];
-      const int e = x >= 0 ? x : ~x;
-      int a = u0[e], b = u1[e];
-      if (need[a] < need[b]) {
-        const int t = a;
-        a = b;
-        b = t;
-      }
-      const bool ai = a >= n, bi = b >= n;
-      if (x >= 0) {
-        stk[sp++] = ~e;
-        if (bi) stk[sp++] = b;
-        if (ai) stk[sp++] = a;
-      } else {
-        ProgOp op;
-        op.a = a;
-        op.b = b;
-        op.type = (ai && bi) ? OP_SS : (ai ? OP_ST : OP_TT);
-        op.node = e;
-        prog[nop++] = op;
-      }
-    }
+    const int nslots = emit_program(n, ne, root, u0, u1, need, stk, prog);
     SlotMeta &pm = slots[m.prop_first + it];
     pm.root = root;
-    pm.nslots = max(1, need[root]);
+    pm.nslots = nslots;
     pm.nmig = 0;
     m.thast[it] = ok ? hast : -INFINITY;  // a runaway slide is rejected
   }
@@ -347,6 +362,622 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo(DeviceView v, MC3Dev m
   }
 }
 
+// ---- the move for models with migration --------------------------------
+//
+// updategenealogy with slider (update_gtree.cpp:256-372) and addmigration
+// (:589-665): the edge's lower end slides freely (up to the edge's own
+// daughters' node), joinsisdown and splitsisdown carry the sister's and
+// the parent's migrations, and the edge's migrations -- and the new
+// sister's when the edge now ends at the root -- are drawn again period by
+// period (mwork_single_edge, mwork_two_edges, simmpath, getmprob,
+// update_gtree_common.cpp:330-1583).  Lane 0 makes the move in LDS; each
+// edge's migration list is a (first, count) range of an append-only LDS
+// pool, so the old lists stay readable for the reverse path probability.
+
+namespace {
+
+constexpr double kMigCloseFrac = 0.9;  // MIGCLOSEFRAC
+
+// edgemiginfo: one edge part's periods, times and counts, and its path
+struct DEmi {
+  int edgeid, pop, temppop, fpop, b, e, mpall, first, count, capn;
+  double upt, dnt, mtall;
+  double mtav[kMaxPops];
+  int mp[kMaxPops];
+};
+
+struct MigCtx {
+  ModelView md;
+  int P, lastp, rootpop;
+  const double *tv;
+  double *pt;  // pool times
+  int32_t *pp;  // pool destinations
+  int pool_cap, pool_end;
+  bool full;
+  __device__ double tvx(int k) const { return k < P - 1 ? tv[k] : kTimeMax; }
+  __device__ void settle(int *p, int period) const {
+    while (md.pope[*p] <= period && md.pope[*p] != -1) *p = md.popdown[*p];
+  }
+  __device__ int findperiod(double t) const {
+    int k = 0;
+    while (k < lastp && tv[k] <= t) k++;
+    return k;
+  }
+  __device__ int picktopop(PhiloxStream &r, int period, int nowpop, int notother) const {
+    const int32_t *pl = md.plist + period * P;
+    const int np = md.npresent[period];
+    int q, tries = 0;
+    do q = mig_randposint(r, np);
+    while ((pl[q] == nowpop || pl[q] == notother) && ++tries < kMaxTries);
+    return pl[q];
+  }
+};
+
+__device__ void emi_reset(DEmi &m) {
+  m.edgeid = -1;
+  m.b = m.e = -1;
+  m.pop = m.temppop = m.fpop = -1;
+  m.upt = m.dnt = -1.0;
+  for (int k = 0; k < kMaxPops; ++k) {
+    m.mtav[k] = 0.0;
+    m.mp[k] = 0;
+  }
+  m.mtall = 0.0;
+  m.mpall = 0;
+  m.first = m.count = m.capn = 0;
+}
+
+// fillmiginfoperiods (:1101-1162)
+__device__ void emi_periods(const MigCtx &c, DEmi &m) {
+  m.b = 0;
+  while (m.upt > c.tvx(m.b)) m.b++;
+  m.e = m.b;
+  while (m.dnt > c.tvx(m.e)) m.e++;
+  if (m.e == m.b) {
+    m.mtav[m.b] = m.b == c.lastp ? 0.0 : m.dnt - m.upt;
+  } else {
+    m.mtav[m.b] = c.tvx(m.b) - m.upt;
+    m.mtav[m.e] = m.e == c.lastp ? 0.0 : m.dnt - c.tvx(m.e - 1);
+    for (int i = m.b + 1; i < m.e; ++i) m.mtav[i] = c.tvx(i) - c.tvx(i - 1);
+  }
+  m.mtall = m.b < c.lastp ? fmin(c.tvx(c.lastp - 1), m.dnt) - m.upt : 0.0;
+}
+
+// fillmiginfo (:1170-1249): an edge as it is, its list a range of the pool
+__device__ void emi_old(const MigCtx &c, DEmi &m, int edge, const int32_t *u0, const int32_t *dn,
+                        const double *tm, const int32_t *pop, const int32_t *lf,
+                        const int32_t *lc, int n) {
+  emi_reset(m);
+  m.edgeid = edge;
+  m.pop = pop[edge];
+  m.dnt = tm[edge];
+  m.upt = edge < n ? 0.0 : tm[u0[edge]];
+  m.fpop = pop[dn[edge]];
+  emi_periods(c, m);
+  m.first = lf[edge];
+  m.count = lc[edge];
+  int j = m.b;
+  for (int i = 0; i < m.count; ++i) {
+    while (c.pt[m.first + i] > c.tvx(j)) j++;
+    m.mp[j]++;
+    m.mpall++;
+  }
+}
+
+// simmpath (:330-441): numm events appended to m's pool range; returns the
+// population reached
+__device__ int emi_simmpath(PhiloxStream &r, MigCtx &c, DEmi &m, int period, int numm,
+                            double timein, double upt, int pop, int constrainpop) {
+#pragma clang fp contract(off)
+  if (m.count + numm > m.capn) {
+    c.full = true;
+    numm = m.capn - m.count;
+    if (numm <= 0) return pop;
+  }
+  double *t = c.pt + m.first + m.count;
+  int32_t *q = c.pp + m.first + m.count;
+  int tries = 0;
+  bool dup;
+  do {
+    for (int i = 0; i < numm; ++i) t[i] = upt + r.uniform() * timein;
+    for (int a = 1; a < numm; ++a) {
+      const double x = t[a];
+      int b = a - 1;
+      while (b >= 0 && t[b] > x) {
+        t[b + 1] = t[b];
+        b--;
+      }
+      t[b + 1] = x;
+    }
+    dup = false;
+    for (int i = 0; i + 1 < numm; ++i)
+      if (t[i] == t[i + 1]) dup = true;
+  } while (dup && ++tries < kMaxTries);
+  int last = pop;
+  for (int i = 0; i < numm; ++i) {
+    int d;
+    if (constrainpop >= 0 && i >= numm - 2)
+      d = i == numm - 2 ? c.picktopop(r, period, last, constrainpop) : constrainpop;
+    else
+      d = c.picktopop(r, period, last, -1);
+    q[i] = d;
+    last = d;
+  }
+  m.count += numm;
+  return last;
+}
+
+// mwork_single_edge (:1354-1429)
+__device__ int emi_single(PhiloxStream &r, MigCtx &c, DEmi &m, const DEmi &old, int lastmig) {
+#pragma clang fp contract(off)
+  if (lastmig < m.b) return 0;
+  int mpall = 0, k;
+  double tstart = m.upt;
+  for (k = m.b; k <= lastmig; ++k) {
+    c.settle(&m.temppop, k);
+    const double rate = calcmrate(old.mp[k], old.mtav[k]) * m.mtav[k];
+    int constrain = -1;
+    if (k < m.e) {
+      m.mp[k] = poisson_cond(r, rate, -1);
+    } else if (c.P - k == 2) {
+      m.mp[k] = poisson_cond(r, rate, m.temppop == m.fpop ? 0 : 1);
+    } else {
+      m.mp[k] = poisson_cond(r, rate, m.temppop == m.fpop ? 3 : 2);
+      constrain = m.fpop;
+    }
+    if (m.mp[k] > 0) {
+      m.temppop = emi_simmpath(r, c, m, k, m.mp[k], m.mtav[k], tstart, m.temppop, constrain);
+      mpall += m.mp[k];
+    }
+    tstart = c.tvx(k);
+  }
+  if (m.mtav[k] > 0 && c.md.pope[m.temppop] == k) m.temppop = c.md.popdown[m.temppop];
+  return mpall;
+}
+
+// mwork_two_edges (:1431-1583)
+__device__ void emi_two(PhiloxStream &r, MigCtx &c, DEmi &em, DEmi &sm, const DEmi &oe,
+                        const DEmi &os, int lastmig) {
+#pragma clang fp contract(off)
+  DEmi *mm[2] = {&em, &sm};
+  const DEmi *om[2] = {&oe, &os};
+  double tstart[2] = {em.upt, sm.upt};
+  const int b = min(em.b, sm.b);
+  const int lastk = em.e == c.lastp ? lastmig : lastmig - 1;
+  int k;
+  for (k = b; k <= lastk; ++k)
+    for (int ii = 0; ii < 2; ++ii) {
+      DEmi &m = *mm[ii];
+      if (m.b > k) continue;
+      c.settle(&m.temppop, k);
+      const double rate = calcmrate(om[ii]->mp[k], om[ii]->mtav[k]) * m.mtav[k];
+      m.mp[k] = poisson_cond(r, rate, -1);
+      if (m.mp[k] > 0) {
+        m.temppop = emi_simmpath(r, c, m, k, m.mp[k], m.mtav[k], tstart[ii], m.temppop, -1);
+        m.mpall += m.mp[k];
+      }
+      tstart[ii] = c.tvx(k);
+    }
+  if (k == c.lastp) {
+    em.fpop = sm.fpop = c.rootpop;
+    return;
+  }
+  for (int ii = 0; ii < 2; ++ii)
+    if (c.md.pope[mm[ii]->temppop] == k) mm[ii]->temppop = c.md.popdown[mm[ii]->temppop];
+  int f;
+  if (em.temppop == sm.temppop)
+    f = r.uniform() < kMigCloseFrac ? em.temppop : c.picktopop(r, k, em.temppop, -1);
+  else if (c.P - k == 2)
+    f = r.uniform() < 0.5 ? em.temppop : sm.temppop;
+  else if (r.uniform() < kMigCloseFrac)
+    f = r.uniform() < 0.5 ? em.temppop : sm.temppop;
+  else
+    f = c.picktopop(r, k, em.temppop, sm.temppop);
+  em.fpop = sm.fpop = f;
+  for (int ii = 0; ii < 2; ++ii) {
+    DEmi &m = *mm[ii];
+    const double rate = calcmrate(om[ii]->mp[k], om[ii]->mtav[k]) * m.mtav[k];
+    int constrain = -1;
+    if (c.P - k == 2) {
+      m.mp[k] = poisson_cond(r, rate, m.temppop == m.fpop ? 0 : 1);
+    } else {
+      m.mp[k] = poisson_cond(r, rate, m.temppop == m.fpop ? 3 : 2);
+      constrain = m.fpop;
+    }
+    if (m.mp[k] > 0) {
+      m.temppop = emi_simmpath(r, c, m, k, m.mp[k], m.mtav[k], tstart[ii], m.temppop, constrain);
+      m.mpall += m.mp[k];
+    }
+    if (m.mtav[k] > 0 && c.md.pope[m.temppop] == k) m.temppop = c.md.popdown[m.temppop];
+  }
+}
+
+// one edge part of getmprob: its free periods, then its last period
+__device__ double emi_prob(const MigCtx &c, const DEmi &m, const DEmi &old, int lastmig) {
+#pragma clang fp contract(off)
+  double tempp = 0.0;
+  int cm = 0;
+  for (int k = m.b; k <= m.e; ++k)
+    if (k < lastmig || (k == lastmig && m.e == c.lastp)) {
+      const double rate = calcmrate(old.mp[k], old.mtav[k]) * m.mtav[k];
+      tempp += m.mp[k] * log(rate / (m.mtav[k] * (c.P - (k + 1)))) - rate;
+      cm += m.mp[k];
+    }
+  if (m.e < c.lastp) {
+    const int e = m.e;
+    const double rate = calcmrate(old.mp[e], old.mtav[e]) * m.mtav[e];
+    if (e == c.lastp - 1) {
+      tempp += (m.mp[e] & 1) ? m.mp[e] * log(rate / m.mtav[e]) - logsinh_(rate)
+                             : m.mp[e] * log(rate / m.mtav[e]) - logcosh_(rate);
+    } else {
+      int p0 = cm == 0 ? m.pop : c.pp[m.first + cm - 1];
+      while (c.md.pope[p0] <= e && c.md.pope[p0] != -1) p0 = c.md.popdown[p0];
+      const int popc = c.P - e - 1;
+      const double d = p0 == m.fpop ? log(1 - rate * exp(-rate)) : log(1 - exp(-rate));
+      double nn;
+      if (m.mp[e] == 0) {
+        nn = -rate;
+      } else if (m.mp[e] == 1) {
+        nn = log(rate / m.mtav[e]) - rate;
+      } else {
+        const int l2 = m.mp[e] == 2 ? p0 : c.pp[m.first + m.mpall - 3];
+        const double pathc = l2 == m.fpop ? -log((double)popc) : -log((double)popc - 1);
+        nn = m.mp[e] * log(rate / m.mtav[e]) - rate + (2 - m.mp[e]) * log((double)popc) + pathc;
+      }
+      tempp += nn - d;
+    }
+  }
+  return tempp;
+}
+
+// getmprob (:851-1099)
+__device__ double emi_getmprob(const MigCtx &c, const DEmi &em, const DEmi &sm, const DEmi &oe,
+                               const DEmi &os) {
+#pragma clang fp contract(off)
+  const int lastmig = min(em.e, c.lastp - 1);
+  if (sm.mtall <= 0) return emi_prob(c, em, oe, lastmig);
+  double tempp = 0.0;
+  if (em.mtall > 0 && em.e < c.lastp) {
+    int pp2[2];
+    const DEmi *mm[2] = {&em, &sm};
+    for (int ii = 0; ii < 2; ++ii) {
+      const DEmi &m = *mm[ii];
+      pp2[ii] = m.pop;
+      if (m.mpall > 0 && m.e > 0) {
+        const double t = c.tvx(m.e - 1);
+        int j = -1;
+        while (j + 1 < m.count && c.pt[m.first + j + 1] < t) j++;
+        if (j >= 0) pp2[ii] = c.pp[m.first + j];
+      }
+      if (m.e > 0)
+        while (c.md.pope[pp2[ii]] <= m.e && c.md.pope[pp2[ii]] != -1)
+          pp2[ii] = c.md.popdown[pp2[ii]];
+    }
+    if (pp2[0] == pp2[1])
+      tempp = pp2[0] == em.fpop ? log(kMigCloseFrac)
+                                : log((1.0 - kMigCloseFrac) / (double)(c.P - em.e - 1));
+    else if (c.P - em.e == 2)
+      tempp = log(0.5);
+    else if (em.fpop == pp2[0] || em.fpop == pp2[1])
+      tempp = log(0.5 * kMigCloseFrac);
+    else
+      tempp = log((1.0 - kMigCloseFrac) / (double)(c.P - em.e - 2));
+  }
+  if (em.mtall > 0) tempp += emi_prob(c, em, oe, lastmig);
+  if (sm.mtall > 0) tempp += emi_prob(c, sm, os, lastmig);
+  return tempp;
+}
+
+}  // namespace
+
+// One wavefront per item; dynamic LDS topo_mig_lds_bytes(max_n, pool).
+__global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3Dev m) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int it = blockIdx.x, lane = threadIdx.x;
+  const uint64_t sweep = *m.counter;
+  SlotMeta *slots = const_cast<SlotMeta *>(v.slots);
+  const SlotMeta cs = slots[m.first_slot + it];
+  const SlotMeta ps = slots[m.prop_first + it];
+  const int n = cs.n, ne = 2 * n - 1, E = 2 * m.maxn - 1, POOL = m.mig_pool;
+  const LocusMeta lm = v.loci[cs.locus];
+  const bool sw = lm.model == IMA_MODEL_SW;
+  double *tm = reinterpret_cast<double *>(smem);
+  double *pt = tm + E;
+  int32_t *u0 = reinterpret_cast<int32_t *>(pt + POOL);
+  int32_t *u1 = u0 + E, *dn = u1 + E, *pop = dn + E, *A = pop + E, *need = A + E, *stk = need + E,
+          *lf = stk + E, *lc = lf + E, *nf = lc + E, *pp = nf + E;
+  DEmi *em = reinterpret_cast<DEmi *>(smem + ((((size_t)E * 8 + (size_t)POOL * 8 +
+                                                (size_t)E * 10 * 4 + (size_t)POOL * 4) + 15) &
+                                               ~(size_t)15));
+  __shared__ int s_total, s_bad;
+  const int nm0 = min(cs.nmig, POOL);
+  for (int e = lane; e < ne; e += kWave) {
+    tm[e] = v.time[cs.node_off + e];
+    u0[e] = v.up0[cs.node_off + e];
+    u1[e] = v.up1[cs.node_off + e];
+    dn[e] = v.down[cs.node_off + e];
+    pop[e] = v.pop[cs.node_off + e];
+    A[e] = v.A ? v.A[cs.node_off + e] : 0;
+    lf[e] = v.mig_first[cs.node_off + e];
+    lc[e] = v.mig_count[cs.node_off + e];
+  }
+  for (int x = lane; x < nm0; x += kWave) {
+    pt[x] = v.mig_time[cs.mig_off + x];
+    pp[x] = v.mig_to[cs.mig_off + x];
+  }
+  wave_lds_fence();
+  if (lane == 0 && cs.nmig > POOL) {
+    // more events than the pool holds (the room made at set-up bounds
+    // them): the genealogy is proposed unchanged and rejected
+    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
+    SlotMeta &pm = slots[m.prop_first + it];
+    pm.root = cs.root;
+    pm.nslots = emit_program(n, ne, cs.root, u0, u1, need, stk, prog);
+    s_bad = 1;
+    m.thast[it] = 0.0;
+  } else if (lane == 0) {
+    PhiloxStream r = topo_rng(m, it, sweep);
+    MigCtx c;
+    c.md = model_view(v, v.models[m.prior]);
+    c.P = m.P;
+    c.lastp = m.P - 1;
+    c.rootpop = 2 * m.P - 2;
+    c.tv = m.tv + (int64_t)(it / m.L) * mc3_tw(m.P);
+    c.pt = pt;
+    c.pp = pp;
+    c.pool_cap = POOL;
+    c.pool_end = nm0;
+    c.full = false;
+    DEmi &oe = em[0], &os = em[1], &nw = em[2], &ns = em[3];
+    int root = cs.root;
+    double hast = 0.0;
+    bool ok = true;
+    int edge;
+    do edge = randposint(r, ne);
+    while (dn[edge] == -1);
+    const int freed = dn[edge];
+    const int oldsis = u0[freed] == edge ? u1[freed] : u0[freed];
+    const int old_down_of_freed = dn[freed];
+    const int copyA0 = A[edge], copyA1 = A[oldsis];
+    const int hold_down_A = old_down_of_freed >= 0 ? A[old_down_of_freed] : 0;
+    emi_old(c, oe, edge, u0, dn, tm, pop, lf, lc, n);
+    emi_reset(os);
+    if (freed == root) emi_old(c, os, oldsis, u0, dn, tm, pop, lf, lc, n);
+    lc[edge] = 0;
+    double roottime = tm[u0[root]];
+    const double sd0 = fmin(kSlideStdvMax, roottime / 3.0);
+    const double hold = mig_normdev(r, 0.0, sd0);
+    double dist = hold;
+    bool rootmove = false;
+    // joinsisdown: the sister's list then the freed edge's, as one new range
+    {
+      const int cs_ = lc[oldsis], cf = lc[freed];
+      if (c.pool_end + cs_ + cf > POOL) {
+        c.full = true;
+      } else {
+        const int f0 = c.pool_end;
+        for (int i = 0; i < cs_; ++i) {
+          pt[f0 + i] = pt[lf[oldsis] + i];
+          pp[f0 + i] = pp[lf[oldsis] + i];
+        }
+        for (int i = 0; i < cf; ++i) {
+          pt[f0 + cs_ + i] = pt[lf[freed] + i];
+          pp[f0 + cs_ + i] = pp[lf[freed] + i];
+        }
+        lf[oldsis] = f0;
+        lc[oldsis] = cs_ + cf;
+        c.pool_end += cs_ + cf;
+      }
+      tm[oldsis] = tm[freed];
+      dn[oldsis] = dn[freed];
+      const int dd = dn[oldsis];
+      if (dd != -1) {
+        if (u0[dd] == freed) u0[dd] = oldsis;
+        else u1[dd] = oldsis;
+      } else {
+        rootmove = true;
+        root = oldsis;
+        tm[oldsis] = kTimeMax;
+        lc[oldsis] = 0;
+        roottime = upper_time(u0, tm, n, oldsis);
+      }
+      lc[freed] = 0;
+    }
+    // slider: up to the edge's own daughters' node, down toward the root
+    int sis = oldsis;
+    double tp = tm[edge];
+    const double uplimit = upper_time(u0, tm, n, edge);
+    int steps = 0;
+    for (; steps < kMaxSlideSteps && dist != 0.0; ++steps) {
+      if (dist < 0) {
+        const double d = -dist;
+        const int su = u0[sis];
+        if (su == -1 || uplimit >= tm[su]) {
+          if (d < tp - uplimit) {
+            tp -= d;
+            dist = 0.0;
+          } else {
+            dist = d - (tp - uplimit);
+            tp = uplimit;
+          }
+        } else if (d < tp - tm[su]) {
+          tp -= d;
+          dist = 0.0;
+        } else {
+          const double left = d - (tp - tm[su]);
+          tp = tm[su];
+          sis = r.uniform() < 0.5 ? u0[sis] : u1[sis];
+          dist = -left;
+        }
+      } else {
+        if (dn[sis] == -1 || tp + dist < tm[sis]) {
+          tp += dist;
+          if (tp >= kTimeMax) tp = kTimeMax;
+          dist = 0.0;
+        } else {
+          const double left = dist - (tm[sis] - tp);
+          tp = tm[sis];
+          if (r.uniform() < 0.5) {
+            sis = dn[sis];
+            dist = left;
+          } else {
+            const int d = dn[sis];
+            sis = u0[d] == sis ? u1[d] : u0[d];
+            dist = -left;
+          }
+        }
+      }
+    }
+    if (steps == kMaxSlideSteps) ok = false;
+    // splitsisdown: the new sister's list split at tp
+    {
+      tm[edge] = tp;
+      tm[freed] = tm[sis];
+      tm[sis] = tp;
+      const int dd = dn[sis];
+      if (dd != -1) {
+        if (u0[dd] == sis) u0[dd] = freed;
+        else u1[dd] = freed;
+      } else {
+        root = freed;
+        roottime = tp;
+        rootmove = true;
+      }
+      dn[freed] = dd;
+      int i = 0;
+      while (i < lc[sis] && pt[lf[sis] + i] < tp) i++;
+      int nowpop = i > 0 ? pp[lf[sis] + i - 1] : pop[sis];
+      c.settle(&nowpop, c.findperiod(tp));
+      pop[freed] = nowpop;
+      if (dd != -1) {
+        lf[freed] = lf[sis] + i;
+        lc[freed] = lc[sis] - i;
+      } else {
+        lc[freed] = 0;
+      }
+      lc[sis] = i;
+      dn[sis] = freed;
+      dn[edge] = freed;
+      u0[freed] = sis;
+      u1[freed] = edge;
+    }
+    if (rootmove)
+      hast += log_normprob(fmin(kSlideStdvMax, roottime / 3.0), hold) - log_normprob(sd0, hold);
+    // addmigration: new paths in two pool ranges of equal room
+    emi_reset(nw);
+    emi_reset(ns);
+    const int room = (POOL - c.pool_end) / 2;
+    if (room <= 0) c.full = true;
+    nw.edgeid = edge;
+    nw.upt = upper_time(u0, tm, n, edge);
+    nw.pop = nw.temppop = pop[edge];
+    nw.dnt = tm[edge];
+    emi_periods(c, nw);
+    nw.first = c.pool_end;
+    nw.capn = max(room, 0);
+    const int lastmig = min(nw.e, c.lastp - 1);
+    int newsis = -1;
+    if (dn[edge] == root) {
+      nw.fpop = -1;
+      newsis = u0[root] == edge ? u1[root] : u0[root];
+      ns.edgeid = newsis;
+      ns.upt = upper_time(u0, tm, n, newsis);
+      ns.fpop = -1;
+      ns.pop = ns.temppop = pop[newsis];
+      ns.dnt = tm[newsis];
+      emi_periods(c, ns);
+      ns.first = c.pool_end + max(room, 0);
+      ns.capn = max(room, 0);
+    } else {
+      nw.fpop = pop[dn[edge]];
+    }
+    if (ns.edgeid == -1)
+      nw.mpall = emi_single(r, c, nw, oe, lastmig);
+    else if (nw.mtall <= 0)
+      ns.mpall = emi_single(r, c, ns, os, lastmig);
+    else
+      emi_two(r, c, nw, ns, oe, os, lastmig);
+    if (nw.fpop == -1) nw.fpop = ns.fpop = c.rootpop;
+    if (dn[edge] == root) pop[root] = nw.fpop;
+    if (!c.full) {  // a path cut short by the pool is rejected below
+      const double denom = emi_getmprob(c, nw, ns, oe, os);
+      const double num = emi_getmprob(c, oe, os, nw, ns);
+      hast += num - denom;
+    }
+    // copynewmig_to_gtree
+    lf[edge] = nw.first;
+    lc[edge] = nw.count;
+    if (ns.edgeid >= 0) {
+      lf[newsis] = ns.first;
+      lc[newsis] = ns.count;
+      int p = ns.count > 0 ? pp[ns.first + ns.count - 1] : pop[newsis];
+      c.settle(&p, c.findperiod(tm[newsis]));
+      pop[dn[newsis]] = p;
+    }
+    if (sw)
+      hast += sw_allele(r, m, it, A, dn, edge, sis, freed, copyA0, copyA1, old_down_of_freed,
+                        hold_down_A);
+    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
+    const int nslots = emit_program(n, ne, root, u0, u1, need, stk, prog);
+    SlotMeta &pm = slots[m.prop_first + it];
+    pm.root = root;
+    pm.nslots = nslots;
+    s_bad = (!ok || c.full) ? 1 : 0;
+    m.thast[it] = hast;
+  }
+  wave_lds_fence();
+  // the new lists, packed in order of edge, then time
+  int carry = 0;
+  for (int base = 0; base < ne; base += kWave) {
+    const int e = base + lane;
+    const int x = e < ne ? lc[e] : 0;
+    int incl = x;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += y;
+    }
+    if (e < ne) nf[e] = carry + incl - x;
+    carry += __shfl(incl, kWave - 1, kWave);
+  }
+  // an accepted proposal's events are copied into the chain's slot
+  const int limit = min(min(ps.mig_cap, cs.mig_cap), m.emax - (n - 1) - (m.P - 1));
+  const bool bad = s_bad || carry > limit;
+  const int64_t po = ps.node_off, pmo = ps.mig_off;
+  int32_t *gu0 = const_cast<int32_t *>(v.up0), *gu1 = const_cast<int32_t *>(v.up1);
+  int32_t *gdn = const_cast<int32_t *>(v.down);
+  double *gtm = const_cast<double *>(v.time);
+  int32_t *gme = const_cast<int32_t *>(v.mig_edge), *gmto = const_cast<int32_t *>(v.mig_to);
+  double *gmt = const_cast<double *>(v.mig_time);
+  for (int e = lane; e < ne; e += kWave) {
+    gu0[po + e] = u0[e];
+    gu1[po + e] = u1[e];
+    gdn[po + e] = dn[e];
+    gtm[po + e] = tm[e];
+    const_cast<int32_t *>(v.pop)[po + e] = pop[e];
+    if (v.A) const_cast<int32_t *>(v.A)[po + e] = A[e];
+    const_cast<int32_t *>(v.mig_first)[po + e] = bad ? 0 : nf[e];
+    const_cast<int32_t *>(v.mig_count)[po + e] = bad ? 0 : lc[e];
+    if (!bad)
+      for (int j = 0; j < lc[e]; ++j) {
+        gme[pmo + nf[e] + j] = e;
+        gmt[pmo + nf[e] + j] = pt[lf[e] + j];
+        gmto[pmo + nf[e] + j] = pp[lf[e] + j];
+      }
+  }
+  if (lane == 0) {
+    slots[m.prop_first + it].nmig = bad ? 0 : carry;
+    if (bad) m.thast[it] = -INFINITY;  // no room for the new migrations: rejected
+  }
+}
+
+size_t topo_mig_lds_bytes(int max_n, int pool) {
+  const size_t E = 2 * (size_t)max_n - 1;
+  return ((E * 8 + (size_t)pool * 8 + E * 10 * 4 + (size_t)pool * 4 + 15) & ~(size_t)15) +
+         4 * sizeof(DEmi);
+}
+
 size_t topo_lds_bytes(int max_n) {
   const size_t ne = 2 * (size_t)max_n - 1;
   return ne * 8 + ne * 7 * 4;
@@ -354,8 +985,12 @@ size_t topo_lds_bytes(int max_n) {
 
 hipError_t enqueue_mc3_propose_topo(const DeviceView &v, const MC3Dev &m, int max_n,
                                     hipStream_t st) {
-  hipLaunchKernelGGL(mc3_propose_topo, dim3(m.C * m.L), dim3(kWave), topo_lds_bytes(max_n), st, v,
-                     m);
+  if (m.mig_pool > 0)
+    hipLaunchKernelGGL(mc3_propose_topo_mig, dim3(m.C * m.L), dim3(kWave),
+                       topo_mig_lds_bytes(max_n, m.mig_pool), st, v, m);
+  else
+    hipLaunchKernelGGL(mc3_propose_topo, dim3(m.C * m.L), dim3(kWave), topo_lds_bytes(max_n), st,
+                       v, m);
   return hipGetLastError();
 }
 

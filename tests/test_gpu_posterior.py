@@ -149,14 +149,18 @@ def test_device_and_oracle_sample_the_same_posterior(ry1, urate, nw):
     assert worst < 4.5
 
 
-def _topo_setup(seed):
+def _topo_setup(seed, mig):
     rng = np.random.default_rng(seed)
-    pt, theta, ts, mig = sim.im_model_two_pop(t=0.8, m12=0.0, m21=0.0)
-    model = immodel.build(pt, qmax=15.0, migration=False)
+    if mig:
+        pt, theta, ts, m = sim.im_model_two_pop(t=0.8, m12=0.5, m21=0.5)
+        model = immodel.build(pt, qmax=15.0, mmax=3.0)
+    else:
+        pt, theta, ts, m = sim.im_model_two_pop(t=0.8, m12=0.0, m21=0.0)
+        model = immodel.build(pt, qmax=15.0, migration=False)
     samp = [3, 3]
     loci, base = [], []
     for _ in range(2):
-        g = sim.simulate_genealogy(rng, samp, theta, ts, np.zeros((2, 2)), pt)
+        g = sim.simulate_genealogy(rng, samp, theta, ts, m, pt)
         loci.append(sim.encode_hky(sim.simulate_hky_sequences(
             rng, g, 150, np.array([0.3, 0.2, 0.2, 0.3]), 4.0, 2.0)))
         base.append(g)
@@ -177,13 +181,16 @@ def _topo_stats(up0, up1, dn, tm, roots, n, Lc):
     return out
 
 
-def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
+@pytest.mark.parametrize("mig", [False, True], ids=["no_migration", "migration_nw"])
+def test_device_and_oracle_sample_the_same_posterior_with_topology_moves(mig):
     """The reference's own genealogy move (IMA_MC3_TOPOLOGY) on the device
-    and in the oracle's sweep (two populations without migration, two HKY
-    loci, the rubber-band split-time update on alternate sweeps): root
-    times, the split time and a topology statistic (gene copies 0 and 1 a
-    cherry) have the same marginal posteriors (|z| < 4.5)."""
-    model, samp, loci, base, ts = _topo_setup(19)
+    and in the oracle's sweep (two populations, two HKY loci, the
+    rubber-band split-time update on alternate sweeps): root times, the
+    split time and a topology statistic (gene copies 0 and 1 a cherry) have
+    the same marginal posteriors (|z| < 4.5).  With migration the move draws
+    migration paths, each time update is changet_NW with probability 1/2,
+    and the chains' migration counts are compared too."""
+    model, samp, loci, base, ts = _topo_setup(19, mig)
     C, burn, nsamp, thin = 48, 400, 150, 20
     n, Lc = base[0].n, len(loci)
     # device
@@ -201,7 +208,7 @@ def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
     e.mc3_setup(C, Lc, first, 2.0, 4.0, np.ones(C), seed=78, precision=L.PREC_FP64,
                 scale_width=0.3)
     e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
-    flags = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_TOPOLOGY
+    flags = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_TOPOLOGY | (L.MC3_NW if mig else 0)
     e.mc3_run(burn, flags)
     a = []
     for _ in range(nsamp):
@@ -212,6 +219,8 @@ def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
         a.append([_topo_stats(up0[c * ne:(c + 1) * ne], up1[c * ne:(c + 1) * ne],
                               dn[c * ne:(c + 1) * ne], tm[c * ne:(c + 1) * ne],
                               rt[c * Lc:(c + 1) * Lc], n, Lc) + [float(tv[c][0])]
+                  + ([float(sum(len(e.read_mig_events(first + c * Lc + l)[0])
+                                for l in range(Lc)))] if mig else [])
                   for c in range(C)])
     e.close()
     a = np.array(a).transpose(1, 0, 2)
@@ -221,7 +230,8 @@ def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
     b = []
     for k in range(burn // thin + nsamp):
         oracle.mc3_bench(loci, [2.0, 2.0], 4.0, samp, model, chains, tvs, np.ones(C), 5.0,
-                         seed=70000 + k, max_sweeps=thin, scale_width=0.3, ry1=True, topo=True)
+                         seed=70000 + k, max_sweeps=thin, scale_width=0.3, ry1=True, topo=True,
+                         nw=1 if mig else 0)
         if k >= burn // thin:
             row = []
             for c in range(C):
@@ -230,7 +240,8 @@ def test_device_and_oracle_sample_the_same_posterior_with_topology_moves():
                                        np.concatenate([g.up1 for g in gs]),
                                        np.concatenate([g.down for g in gs]),
                                        np.concatenate([g.time for g in gs]),
-                                       [g.root for g in gs], n, Lc) + [float(tvs[c][0])])
+                                       [g.root for g in gs], n, Lc) + [float(tvs[c][0])]
+                           + ([float(sum(len(m) for g in gs for m in g.mig))] if mig else []))
             b.append(row)
     b = np.array(b).transpose(1, 0, 2)
     worst = 0.0
