@@ -1857,6 +1857,8 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
     d.mig_pool = pool;
     if ((rc = mc3_topo_mig_buffers(c)) != IMA_OK) return rc;
   }
+  HIPCHK(c, reserve(m.idec, N));
+  d.idec = m.idec.p;  // the sweep's decisions, read when it copies accepted proposals
   d.topo = 1;
   d.thast = m.thast.p;
   d.hast = m.thast.p;

@@ -531,14 +531,13 @@ __device__ inline void prior_commit(const DeviceView &v, const MC3Dev &m, const 
       double *wc = m.wrec + (int64_t)it * R;
       for (int r = lane; r < R; r += kWave)
         if (sh.flag[row * R + r]) wc[r] = sh.dnew[row * R + r];
-      if (m.topo) copy_proposal(v, m, it, lane, kWave);
     }
     if (lane == 0) {
       if (acc)
         m.pdg[it] = sh.npd[row];
       else if (!m.host && !m.topo)
         restore_item(v, m, it);
-      if (m.host) m.idec[it] = acc ? 1 : 0;
+      if (m.host || m.topo) m.idec[it] = acc ? 1 : 0;
       m.acc[it] += acc ? 1u : 0u;
     }
   }
@@ -929,10 +928,10 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
       if (acc) {
         p = np;
         m.pdg[it] = p;
-        if (m.topo) copy_proposal(v, m, it, 0, 1);
       } else if (!m.topo) {
         restore_item(v, m, it);
       }
+      if (m.topo) m.idec[it] = acc ? 1 : 0;
       m.acc[it] += acc ? 1u : 0u;
       part += p;
       if (!with_scale && !m.topo) propose_item(v, m, it, sweep + 1);
@@ -944,6 +943,14 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
       if (!with_scale && !m.topo) propose_item(v, m, c * m.L + l, sweep + 1);
     }
     if (threadIdx.x == 0) part += m.probg[c];
+  }
+  if (m.topo) {
+    // the accepted proposals replace the chain's genealogies once every
+    // locus is decided, one wavefront per locus
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (int l = wave; l < m.L; l += blockDim.x / kWave)
+      if (m.idec[c * m.L + l]) copy_proposal(v, m, c * m.L + l, lane, kWave);
   }
   if (!with_scale) {
     finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);

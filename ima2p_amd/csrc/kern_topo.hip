@@ -100,7 +100,7 @@ __device__ inline double upper_time(const int32_t *u0, const double *tm, int n, 
 // finishSWupdateA's proposal for the freed node's allele (update_gtree_common
 // .cpp:2219-2362) and its Hastings term; the likelihood is scored in full
 // afterwards.
-__device__ double sw_allele(PhiloxStream &r, const MC3Dev &m, int it, int32_t *A,
+__device__ __forceinline__ double sw_allele(PhiloxStream &r, const MC3Dev &m, int it, int32_t *A,
                             const int32_t *dn, int edge, int sis, int freed, int copyA0,
                             int copyA1, int old_down_of_freed, int hold_down_A) {
   const int oldA = A[freed];
@@ -130,57 +130,73 @@ __device__ double sw_allele(PhiloxStream &r, const MC3Dev &m, int it, int32_t *A
   return (abs(dA) * log(1.0 - gold) + log(gold)) - (abs(dA) * log(1.0 - gnew) + log(gnew));
 }
 
-// The postorder program of a topology: Strahler need bottom-up, then the
-// daughter with the larger need first (the host's emit).  Returns the
-// LDS partial slots the program needs.
-__device__ int emit_program(int n, int ne, int root, const int32_t *u0, const int32_t *u1,
-                            int32_t *need, int32_t *stk, ProgOp *prog) {
-  for (int e = 0; e < ne; ++e) need[e] = e < n ? 0 : -1;
-  int sp = 0;
-  stk[sp++] = root;
-  while (sp > 0) {
-    const int e = stk[sp - 1];
-    if (need[e] >= 0) {
-      --sp;
-      continue;
+// The postorder program of a topology, as the host's emit builds it: each
+// internal node's operation after those of its daughters' subtrees, the
+// daughter with the larger Strahler need (the LDS partial slots its
+// subtree uses; ties: up0) first.  The wavefront's lanes build it together
+// in rounds that run up the tree and back down (as many as its height):
+// need[e] becomes (internal nodes at or below e) << 8 | (slots e needs),
+// stk[e] the position of the first operation of e's subtree, so e's own
+// operation sits at stk[e] + (internal nodes at or below e) - 1.  Every
+// lane calls it after the genealogy's last LDS write and gets the slot
+// count; down[] must be the parents of up0[] and up1[].
+__device__ __forceinline__ int wave_emit_program(int n, int ne, int root, const int32_t *u0, const int32_t *u1,
+                                 const int32_t *dn, int32_t *need, int32_t *stk, ProgOp *prog,
+                                 int lane) {
+  for (int e = lane; e < ne; e += kWave) need[e] = e < n ? 0 : -1;
+  wave_lds_fence();
+  for (int round = 0; round < ne; ++round) {
+    bool pend = false;
+    for (int e = n + lane; e < ne; e += kWave) {
+      if (need[e] >= 0) continue;
+      const int va = need[u0[e]], vb = need[u1[e]];
+      if (va < 0 || vb < 0) {
+        pend = true;
+        continue;
+      }
+      const int na = va & 255, nb = vb & 255;
+      need[e] = (((va >> 8) + (vb >> 8) + 1) << 8) | (na == nb ? na + 1 : max(na, nb));
     }
-    const int a = u0[e], b = u1[e];
-    if (need[a] >= 0 && need[b] >= 0) {
-      need[e] = need[a] == need[b] ? need[a] + 1 : max(need[a], need[b]);
-      --sp;
-    } else {
-      if (need[a] < 0) stk[sp++] = a;
-      if (need[b] < 0) stk[sp++] = b;
-    }
+    wave_lds_fence();
+    if (!__ballot(pend)) break;
   }
-  int nop = 0;
-  sp = 0;
-  stk[sp++] = root;
-  // entries: e (descend) or ~e (emit e's operation)
-  while (sp > 0) {
-    const int x = stk[--sp];
-    const int e = x >= 0 ? x : ~x;
+  for (int e = lane; e < ne; e += kWave) stk[e] = e == root ? 0 : -1;
+  wave_lds_fence();
+  for (int round = 0; round < ne; ++round) {
+    bool pend = false;
+    for (int e = n + lane; e < ne; e += kWave) {
+      const int p = dn[e];
+      if (stk[e] >= 0 || p < 0) continue;
+      const int sp = stk[p];
+      if (sp < 0) {
+        pend = true;
+        continue;
+      }
+      int a = u0[p];
+      const int b = u1[p];
+      if ((need[a] & 255) < (need[b] & 255)) a = b;
+      stk[e] = e == a ? sp : sp + (a >= n ? need[a] >> 8 : 0);
+    }
+    wave_lds_fence();
+    if (!__ballot(pend)) break;
+  }
+  for (int e = n + lane; e < ne; e += kWave) {
     int a = u0[e], b = u1[e];
-    if (need[a] < need[b]) {
+    if ((need[a] & 255) < (need[b] & 255)) {
       const int t = a;
       a = b;
       b = t;
     }
     const bool ai = a >= n, bi = b >= n;
-    if (x >= 0) {
-      stk[sp++] = ~e;
-      if (bi) stk[sp++] = b;
-      if (ai) stk[sp++] = a;
-    } else {
-      ProgOp op;
-      op.a = a;
-      op.b = b;
-      op.type = (ai && bi) ? OP_SS : (ai ? OP_ST : OP_TT);
-      op.node = e;
-      prog[nop++] = op;
-    }
+    ProgOp op;
+    op.a = a;
+    op.b = b;
+    op.type = (ai && bi) ? OP_SS : (ai ? OP_ST : OP_TT);
+    op.node = e;
+    const int k = stk[e] + (need[e] >> 8) - 1;
+    if (k >= 0 && k < n - 1) prog[k] = op;
   }
-  return max(1, need[root]);
+  return max(1, need[root] & 255);
 }
 
 }  // namespace
@@ -200,6 +216,7 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo(DeviceView v, MC3Dev m
   int32_t *u0 = reinterpret_cast<int32_t *>(tm + ne);
   int32_t *u1 = u0 + ne, *dn = u1 + ne, *pop = dn + ne, *A = pop + ne, *need = A + ne,
           *stk = need + ne;
+  __shared__ int s_root;
   for (int e = lane; e < ne; e += kWave) {
     tm[e] = v.time[cs.node_off + e];
     u0[e] = v.up0[cs.node_off + e];
@@ -337,15 +354,18 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo(DeviceView v, MC3Dev m
     if (sw)
       hast += sw_allele(r, m, it, A, dn, edge, sis, freed, copyA0, copyA1, old_down_of_freed,
                         hold_down_A);
-    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
-    const int nslots = emit_program(n, ne, root, u0, u1, need, stk, prog);
     SlotMeta &pm = slots[m.prop_first + it];
     pm.root = root;
-    pm.nslots = nslots;
     pm.nmig = 0;
+    s_root = root;
     m.thast[it] = ok ? hast : -INFINITY;  // a runaway slide is rejected
   }
   wave_lds_fence();
+  {
+    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
+    const int nslots = wave_emit_program(n, ne, s_root, u0, u1, dn, need, stk, prog, lane);
+    if (lane == 0) slots[m.prop_first + it].nslots = nslots;
+  }
   const int64_t po = ps.node_off;
   int32_t *gu0 = const_cast<int32_t *>(v.up0), *gu1 = const_cast<int32_t *>(v.up1);
   int32_t *gdn = const_cast<int32_t *>(v.down);
@@ -394,16 +414,16 @@ struct MigCtx {
   int32_t *pp;  // pool destinations
   int pool_cap, pool_end;
   bool full;
-  __device__ double tvx(int k) const { return k < P - 1 ? tv[k] : kTimeMax; }
-  __device__ void settle(int *p, int period) const {
+  __device__ __forceinline__ double tvx(int k) const { return k < P - 1 ? tv[k] : kTimeMax; }
+  __device__ __forceinline__ void settle(int *p, int period) const {
     while (md.pope[*p] <= period && md.pope[*p] != -1) *p = md.popdown[*p];
   }
-  __device__ int findperiod(double t) const {
+  __device__ __forceinline__ int findperiod(double t) const {
     int k = 0;
     while (k < lastp && tv[k] <= t) k++;
     return k;
   }
-  __device__ int picktopop(PhiloxStream &r, int period, int nowpop, int notother) const {
+  __device__ __forceinline__ int picktopop(PhiloxStream &r, int period, int nowpop, int notother) const {
     const int32_t *pl = md.plist + period * P;
     const int np = md.npresent[period];
     int q, tries = 0;
@@ -413,7 +433,7 @@ struct MigCtx {
   }
 };
 
-__device__ void emi_reset(DEmi &m) {
+__device__ __forceinline__ void emi_reset(DEmi &m) {
   m.edgeid = -1;
   m.b = m.e = -1;
   m.pop = m.temppop = m.fpop = -1;
@@ -428,7 +448,7 @@ __device__ void emi_reset(DEmi &m) {
 }
 
 // fillmiginfoperiods (:1101-1162)
-__device__ void emi_periods(const MigCtx &c, DEmi &m) {
+__device__ __forceinline__ void emi_periods(const MigCtx &c, DEmi &m) {
   m.b = 0;
   while (m.upt > c.tvx(m.b)) m.b++;
   m.e = m.b;
@@ -444,7 +464,7 @@ __device__ void emi_periods(const MigCtx &c, DEmi &m) {
 }
 
 // fillmiginfo (:1170-1249): an edge as it is, its list a range of the pool
-__device__ void emi_old(const MigCtx &c, DEmi &m, int edge, const int32_t *u0, const int32_t *dn,
+__device__ __forceinline__ void emi_old(const MigCtx &c, DEmi &m, int edge, const int32_t *u0, const int32_t *dn,
                         const double *tm, const int32_t *pop, const int32_t *lf,
                         const int32_t *lc, int n) {
   emi_reset(m);
@@ -466,7 +486,7 @@ __device__ void emi_old(const MigCtx &c, DEmi &m, int edge, const int32_t *u0, c
 
 // simmpath (:330-441): numm events appended to m's pool range; returns the
 // population reached
-__device__ int emi_simmpath(PhiloxStream &r, MigCtx &c, DEmi &m, int period, int numm,
+__device__ __forceinline__ int emi_simmpath(PhiloxStream &r, MigCtx &c, DEmi &m, int period, int numm,
                             double timein, double upt, int pop, int constrainpop) {
 #pragma clang fp contract(off)
   if (m.count + numm > m.capn) {
@@ -508,7 +528,7 @@ __device__ int emi_simmpath(PhiloxStream &r, MigCtx &c, DEmi &m, int period, int
 }
 
 // mwork_single_edge (:1354-1429)
-__device__ int emi_single(PhiloxStream &r, MigCtx &c, DEmi &m, const DEmi &old, int lastmig) {
+__device__ __forceinline__ int emi_single(PhiloxStream &r, MigCtx &c, DEmi &m, const DEmi &old, int lastmig) {
 #pragma clang fp contract(off)
   if (lastmig < m.b) return 0;
   int mpall = 0, k;
@@ -536,7 +556,7 @@ __device__ int emi_single(PhiloxStream &r, MigCtx &c, DEmi &m, const DEmi &old, 
 }
 
 // mwork_two_edges (:1431-1583)
-__device__ void emi_two(PhiloxStream &r, MigCtx &c, DEmi &em, DEmi &sm, const DEmi &oe,
+__device__ __forceinline__ void emi_two(PhiloxStream &r, MigCtx &c, DEmi &em, DEmi &sm, const DEmi &oe,
                         const DEmi &os, int lastmig) {
 #pragma clang fp contract(off)
   DEmi *mm[2] = {&em, &sm};
@@ -593,7 +613,7 @@ __device__ void emi_two(PhiloxStream &r, MigCtx &c, DEmi &em, DEmi &sm, const DE
 }
 
 // one edge part of getmprob: its free periods, then its last period
-__device__ double emi_prob(const MigCtx &c, const DEmi &m, const DEmi &old, int lastmig) {
+__device__ __forceinline__ double emi_prob(const MigCtx &c, const DEmi &m, const DEmi &old, int lastmig) {
 #pragma clang fp contract(off)
   double tempp = 0.0;
   int cm = 0;
@@ -631,7 +651,7 @@ __device__ double emi_prob(const MigCtx &c, const DEmi &m, const DEmi &old, int 
 }
 
 // getmprob (:851-1099)
-__device__ double emi_getmprob(const MigCtx &c, const DEmi &em, const DEmi &sm, const DEmi &oe,
+__device__ __forceinline__ double emi_getmprob(const MigCtx &c, const DEmi &em, const DEmi &sm, const DEmi &oe,
                                const DEmi &os) {
 #pragma clang fp contract(off)
   const int lastmig = min(em.e, c.lastp - 1);
@@ -690,7 +710,7 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
   DEmi *em = reinterpret_cast<DEmi *>(smem + ((((size_t)E * 8 + (size_t)POOL * 8 +
                                                 (size_t)E * 10 * 4 + (size_t)POOL * 4) + 15) &
                                                ~(size_t)15));
-  __shared__ int s_total, s_bad;
+  __shared__ int s_bad, s_root;
   const int nm0 = min(cs.nmig, POOL);
   for (int e = lane; e < ne; e += kWave) {
     tm[e] = v.time[cs.node_off + e];
@@ -710,10 +730,8 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
   if (lane == 0 && cs.nmig > POOL) {
     // more events than the pool holds (the room made at set-up bounds
     // them): the genealogy is proposed unchanged and rejected
-    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
-    SlotMeta &pm = slots[m.prop_first + it];
-    pm.root = cs.root;
-    pm.nslots = emit_program(n, ne, cs.root, u0, u1, need, stk, prog);
+    slots[m.prop_first + it].root = cs.root;
+    s_root = cs.root;
     s_bad = 1;
     m.thast[it] = 0.0;
   } else if (lane == 0) {
@@ -918,11 +936,8 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
     if (sw)
       hast += sw_allele(r, m, it, A, dn, edge, sis, freed, copyA0, copyA1, old_down_of_freed,
                         hold_down_A);
-    ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
-    const int nslots = emit_program(n, ne, root, u0, u1, need, stk, prog);
-    SlotMeta &pm = slots[m.prop_first + it];
-    pm.root = root;
-    pm.nslots = nslots;
+    slots[m.prop_first + it].root = root;
+    s_root = root;
     s_bad = (!ok || c.full) ? 1 : 0;
     m.thast[it] = hast;
   }
@@ -966,7 +981,10 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
         gmto[pmo + nf[e] + j] = pp[lf[e] + j];
       }
   }
+  ProgOp *prog = const_cast<ProgOp *>(v.prog) + ps.prog_off;
+  const int nslots = wave_emit_program(n, ne, s_root, u0, u1, dn, need, stk, prog, lane);
   if (lane == 0) {
+    slots[m.prop_first + it].nslots = nslots;
     slots[m.prop_first + it].nmig = bad ? 0 : carry;
     if (bad) m.thast[it] = -INFINITY;  // no room for the new migrations: rejected
   }

@@ -14,17 +14,17 @@ constexpr int kAddMigMax = 1000;     // ADDMIGMAX
 constexpr int kMaxTries = 1 << 20;   // bound of every rejection loop
 constexpr int kPoissonMax = 1 << 20;  // bound of one Poisson product loop
 
-__device__ inline int mig_randposint(PhiloxStream &r, int n) {
+__device__ __forceinline__ int mig_randposint(PhiloxStream &r, int n) {
   return (int)floor(r.uniform() * n);
 }
 
-__device__ inline double calcmrate(int mc, double mt) {
+__device__ __forceinline__ double calcmrate(int mc, double mt) {
   if (mt <= 0.0) return 1.0;
   if (mc == 0) return mt < 1 ? 0.1 : 0.1 / mt;
   return mt < 1 ? (double)mc : (double)mc / mt;
 }
 
-__device__ inline double mig_normdev(PhiloxStream &r, double mean, double stdev) {
+__device__ __forceinline__ double mig_normdev(PhiloxStream &r, double mean, double stdev) {
 #pragma clang fp contract(off)
   double v1, v2, rsq;
   do {
@@ -37,7 +37,7 @@ __device__ inline double mig_normdev(PhiloxStream &r, double mean, double stdev)
 
 // poisson() (utilities.cpp:517-596): condition -1 any, 0 even, 1 odd,
 // 2 not 0, 3 not 1
-__device__ inline int poisson_cond(PhiloxStream &r, double param, int cond) {
+__device__ __forceinline__ int poisson_cond(PhiloxStream &r, double param, int cond) {
 #pragma clang fp contract(off)
   if (param < 0.25 && cond == 1) {
     const double U = r.uniform();

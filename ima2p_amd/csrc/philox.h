@@ -66,7 +66,7 @@ struct PhiloxStream {
     const int i = 4 - (left--);
     return i == 0 ? b0 : i == 1 ? b1 : i == 2 ? b2 : b3;
   }
-  __host__ __device__ double uniform() {
+  __host__ __device__ __forceinline__ double uniform() {
     const uint64_t hi = next32(), lo = next32();
     const uint64_t x = (hi << 32) | lo;
     return (double)(x >> 11) * (1.0 / 9007199254740992.0);
