@@ -4,9 +4,12 @@
 Workload (default): three-population IM model, 50 HKY loci x 1000 bp,
 45 gene copies per locus, 64 heated chains (geometric heating), synthetic
 data from the package's simulator.  One step = one MC3 sweep over all
-chains: per (chain, locus) one genealogy proposal, scored in a single
-batched FP32-scaled likelihood launch plus one launch of genealogy weights,
-then accepted or rejected locus by locus with beta (dprobg + dpdg), probg
+chains: per (chain, locus) one genealogy proposal, updategenealogy's own
+move (an edge cut and slid to a new sister, its migration path -- and the
+new sister's when it ends at the root -- redrawn period by period, with
+the move's Hastings term), scored in a single batched FP32-scaled
+likelihood launch plus one launch of genealogy weights, then accepted or
+rejected locus by locus with beta (dprobg + dpdg) + Hastings term, probg
 the chain's genealogy prior integrated over the population-size and
 migration priors (update_gtree.cpp:827-961); on alternate sweeps each
 chain's time update, drawn as qupdate draws it (ima_main_mpi.cpp:
@@ -85,15 +88,17 @@ def _cpu_worker(args):
     sweeps, _ = oracle.mc3_bench(wl.loci, wl.u, wl.kappa, wl.samppop, immodel.build(wl.poptree),
                                  chains, tvs, workloads.geometric_betas(len(chain_ids)),
                                  wl.tmax, seed=100 + wid, seconds=seconds, ry1=True,
-                                 urate=True, nw=1)
+                                 urate=True, nw=1, topo=True)
     return len(chain_ids) * sweeps, time.perf_counter() - t0
 
 
 def sweep_flags(L, nw=True):
-    """The device sweep of the reference's step: graph replay, the time
-    update on alternate sweeps (RY1, or NW with probability 1/2 for models
-    with migration), the mutation-rate round every fifth sweep."""
-    return L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U | (L.MC3_NW if nw else 0)
+    """The device sweep of the reference's step: graph replay, the
+    reference's genealogy move, the time update on alternate sweeps (RY1,
+    or NW with probability 1/2 for models with migration), the
+    mutation-rate round every fifth sweep."""
+    return (L.MC3_GRAPH | L.MC3_TOPOLOGY | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U
+            | (L.MC3_NW if nw else 0))
 
 
 def cpu_baseline(wl, seconds, cores):
@@ -515,7 +520,8 @@ def main():
         "config": {"workload": wl.name, "chains": C, "loci": Lc, "sites_per_locus": 1000,
                    "genes_per_locus": n, "patterns_mean": float(S.mean()),
                    "patterns_min": int(S.min()), "patterns_max": int(S.max()),
-                   "sweep": "1 genealogy proposal per (chain, locus) scored with data likelihood "
+                   "sweep": "1 genealogy proposal per (chain, locus), updategenealogy's move "
+                            "(slider + addmigration), scored with data likelihood "
                             "and integrated genealogy prior, loci in order + on alternate sweeps "
                             "one split-time update per chain, changet_RY1 (rubber band, every "
                             "locus re-scored) or changet_NW (migrations redrawn around the "

@@ -509,10 +509,16 @@ int ima_integrate_tree_prob(ima_ctx *ctx, int model, const double *wsum, int ngr
  * 2219-2362) enter the decision.  Proposals are made on the device in
  * proposal slots the engine allocates on first use, scored in one batched
  * launch and decided in each chain's locus order; accepted ones replace
- * the chains' genealogies.  For population models without migration
- * parameters (or without the prior): the migration-path proposals of
- * addmigration (:589-665) are not built, and a model with migration
- * parameters is refused with IMA_ERR_STATE. */
+ * the chains' genealogies.  For a model with migration parameters the
+ * move is slider (:256-372) with addmigration (:589-665): the edge's and,
+ * when it ends at the root, the new sister's migration paths are drawn
+ * again period by period (mwork_single_edge, mwork_two_edges, simmpath,
+ * update_gtree_common.cpp:330-1583) and getmprob's forward and reverse
+ * path probabilities enter the Hastings term; the engine gives each
+ * chain's slot and its proposal slot room for twice the chain's
+ * migrations plus 32 on first use, and a proposal beyond that room or
+ * the prior's event bound is rejected (IMA_ERR_SIZE when the room does
+ * not fit LDS). */
 #define IMA_MC3_TOPOLOGY 64
 /* With IMA_MC3_SCALE and the genealogy prior of a model with migration:
  * each chain's time update is changet_NW (update_t_NW.cpp:920-1032) with

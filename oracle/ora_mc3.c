@@ -18,8 +18,11 @@
  * period and between the migrations around it, and a global rescale by
  * exp(d (U - 1/2)) -- or, with ry1, the reference's rubber-band update of
  * one split time (changet_RY1, update_t_RY.cpp:223-517).  With topo the
- * genealogy update is instead the reference's own move for models without
- * migration (slider_nomigration / joinsisdown / splitsisdown).  Randomness:
+ * genealogy update is instead the reference's own move: slider_nomigration
+ * / joinsisdown / splitsisdown for models without migration, slider with
+ * addmigration's new migration paths (ora_mig.c) for models with it.  With
+ * nw, time updates are changet_NW (ora_nw.c) with probability 1/2 (nw 1)
+ * or always (nw 2).  Randomness:
  * xoshiro256** seeded by splitmix64.
  */
 #include <math.h>
