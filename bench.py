@@ -268,30 +268,44 @@ def uneven_leg(sweeps=40):
             "ms_per_sweep_other_site_block": {str(other): dt_other / sweeps * 1e3}}
 
 
-def is_leg(sweeps=2000):
+def _is_engine(seed=1, analyses=1):
     """The CPU-runnable case (two populations, 5 infinite-sites loci of
-    10+10 gene copies, 4 chains) on the device, FP64, with the prior."""
+    10+10 gene copies, 4 chains) on a context of its own, FP64, with the
+    prior; with analyses > 1, that many uncoupled copies of it (each its
+    own 4 chains and heating ladder, swaps inside each) in one sweep."""
     from ima2p_amd import _lib as L
     from ima2p_amd import immodel
     from ima2p_amd.engine import Engine
     from ima2p_amd.serve import AnalysisSpec
     wl = workloads.two_pop_is()
     model = immodel.build(wl.poptree, qmax=20.0, mmax=5.0)
-    C = wl.nchains
+    C = wl.nchains * analyses
     spec = AnalysisSpec(loci=[("is", d) for d in wl.loci], model=model,
-                        samppop=[wl.samppop] * wl.nloci, chains=wl.chains,
-                        u=np.tile(wl.u, (C, 1)), kappa=np.ones((C, wl.nloci)), beta=wl.beta,
-                        tvals=np.tile(wl.tsplit, (C, 1)), tmax=wl.tmax, seed=1,
+                        samppop=[wl.samppop] * wl.nloci, chains=list(wl.chains) * analyses,
+                        u=np.tile(wl.u, (C, 1)), kappa=np.ones((C, wl.nloci)),
+                        beta=np.tile(wl.beta, analyses),
+                        tvals=np.tile(wl.tsplit, (C, 1)), tmax=wl.tmax, seed=seed,
                         precision=L.PREC_FP64)
     e = Engine(0)
-    spec.load(e)
+    spec.load(e, swaptries=max(1, wl.nchains // 10))  # per analysis, ima_main_mpi.cpp:1376-1378
+    if analyses > 1:
+        e.mc3_set_analyses(wl.nchains)
+    return e, wl
+
+
+def is_leg(sweeps=2000, analyses=1):
+    """The CPU-runnable case on the device: one analysis of 4 chains, or
+    `analyses` uncoupled ones in the same sweep (the shape of the
+    reference's aggregate over 8 concurrent processes)."""
+    from ima2p_amd import _lib as L
+    e, wl = _is_engine(analyses=analyses)
     e.mc3_run(20, sweep_flags(L))
     t0 = time.perf_counter()
     e.mc3_run(sweeps, sweep_flags(L))
     dt = time.perf_counter() - t0
     e.close()
-    return {"chain_steps_per_s": C * sweeps / dt, "ms_per_sweep": dt / sweeps * 1e3,
-            "workload": wl.name}
+    return {"chain_steps_per_s": analyses * wl.nchains * sweeps / dt,
+            "ms_per_sweep": dt / sweeps * 1e3, "analyses": analyses, "workload": wl.name}
 
 
 def serving_leg(duration_s=20.0, rate=20.0, seed=5, nsweeps=100):
@@ -501,6 +515,7 @@ def main():
         extra["lmode"] = lmode_leg()
         extra["uneven_batch"] = uneven_leg()
         extra["is_case"] = is_leg()
+        extra["is_case_8_uncoupled"] = is_leg(analyses=8)
     if args.serve_seconds > 0:
         extra["serving"] = serving_leg(duration_s=args.serve_seconds)
 

@@ -90,6 +90,7 @@ PROTOS = {
     "ima_slots_read_mig": (C.c_int, [vp, C.c_int, dp]),
     "ima_sw_update_A": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, dp, i32p, dp]),
     "ima_mc3_set_prior_group": (C.c_int, [vp, C.c_int]),
+    "ima_mc3_set_analyses": (C.c_int, [vp, C.c_int]),
     "ima_mc3_prior_group": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ima_mc3_tune": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]),
     "ima_device_memory": (C.c_int, [C.c_int, dp, dp]),

@@ -1628,6 +1628,7 @@ int ima_mc3_setup(ima_ctx *c, int nchains, int nloci, int first_slot, const doub
   HIPCHK(c, hipMemcpyAsync(m.stats.p, zeros, sizeof zeros, hipMemcpyHostToDevice, c->stream));
   MC3Dev &d = m.dev;
   d.C = nchains;
+  d.group = nchains;
   d.L = nloci;
   d.first_slot = first_slot;
   d.pdg = m.pdg.p;
@@ -2492,6 +2493,18 @@ int ima_mc3_set_prior(ima_ctx *c, const double *tvals, double tmax) {
                   m.first + i);
     }
   // graphs captured before hold the sweep without the prior
+  mc3_drop_graphs(c);
+  return IMA_OK;
+}
+
+int ima_mc3_set_analyses(ima_ctx *c, int chains_per_analysis) {
+  if (!c) return IMA_ERR_ARG;
+  auto &m = c->mc3;
+  if (!m.live) return fail(c, IMA_ERR_STATE, "ima_mc3_setup has not been called");
+  if (chains_per_analysis < 1 || m.C % chains_per_analysis != 0)
+    return fail(c, IMA_ERR_ARG, "%d chains do not split into analyses of %d", m.C,
+                chains_per_analysis);
+  m.dev.group = chains_per_analysis;
   mc3_drop_graphs(c);
   return IMA_OK;
 }

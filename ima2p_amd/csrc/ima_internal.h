@@ -291,6 +291,7 @@ hipError_t launch_move_events(int n, const int64_t *from, const int64_t *to, con
 
 struct MC3Dev {
   int C, L;
+  int group;  // chains per uncoupled analysis (swaps stay inside one); C for one analysis
   int first_slot;
   double *pdg, *newpdg;  // [C*L]
   int32_t *status;       // [C*L]

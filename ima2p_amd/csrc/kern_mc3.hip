@@ -241,11 +241,15 @@ __device__ void swap_chains(const MC3Dev &m, int swaptries, uint64_t sweep, doub
     }
   __syncthreads();
   if (threadIdx.x == 0) {
-    PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull, stream_counter(sweep, 4));
-    const int SWAPDIST = 7, nchains = m.C;
-    const double *S = staged ? Ssh : m.S;
-    double *beta = staged ? Bsh : m.beta;
+    // each uncoupled analysis (m.group chains) swaps among its own chains,
+    // from a stream of its own; one analysis keeps the first stream
+    const int SWAPDIST = 7, nchains = m.group, ngroups = m.C / m.group;
     unsigned long long acc = 0;
+    for (int g = 0; g < ngroups; ++g) {
+    PhiloxStream rng(m.seed ^ 0x5A5A5A5A5A5A5A5Aull ^ (0x9E3779B97F4A7C15ull * (uint64_t)g),
+                     stream_counter(sweep, 4));
+    const double *S = (staged ? Ssh : m.S) + g * nchains;
+    double *beta = (staged ? Bsh : m.beta) + g * nchains;
     for (int i = 0; i < swaptries && nchains > 1; ++i) {
       int ci, cj, cjmin, cjrange;
       do {
@@ -269,7 +273,8 @@ __device__ void swap_chains(const MC3Dev &m, int swaptries, uint64_t sweep, doub
         ++acc;
       }
     }
-    m.stats[4] += (unsigned long long)(nchains > 1 ? swaptries : 0);
+    }
+    m.stats[4] += (unsigned long long)(nchains > 1 ? swaptries * ngroups : 0);
     m.stats[5] += acc;
     m.stats[0] += (unsigned long long)(m.C * m.L);
     *m.counter = sweep + 1;

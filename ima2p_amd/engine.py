@@ -512,6 +512,12 @@ class Engine:
         self._chk(self.lib.ima_mc3_set_prior(self.h, tv.ctypes.data_as(L.dp), float(tmax)),
                   "mc3_set_prior")
 
+    def mc3_set_analyses(self, chains_per_analysis):
+        """Split the chains into uncoupled analyses of `chains_per_analysis`
+        consecutive chains each: swaps stay inside an analysis."""
+        self._chk(self.lib.ima_mc3_set_analyses(self.h, int(chains_per_analysis)),
+                  "mc3_set_analyses")
+
     def mc3_prior_group(self, loci=None):
         """Loci per speculative group of the accept step; sets it when
         `loci` is given.  Returns the value in force."""

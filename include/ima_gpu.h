@@ -565,6 +565,17 @@ int ima_mc3_set_prior(ima_ctx *ctx, const double *tvals, double tmax);
  * scenarios at once; decisions are the same for every D.  No counterpart
  * in the reference, which decides one locus at a time. */
 int ima_mc3_set_prior_group(ima_ctx *ctx, int loci);
+/* Several uncoupled analyses in one batched sweep: chains [k*n, (k+1)*n)
+ * form analysis k (n = chains_per_analysis, which must divide the chain
+ * count; ima_mc3_setup starts with one analysis of all chains).  Each
+ * analysis swaps only among its own chains, drawing from a swap stream of
+ * its own (analysis 0 keeps the stream of a single analysis, so n = C
+ * changes nothing); everything else of a sweep is per chain already.  The
+ * caller gives each analysis its own heating ladder through beta.  No
+ * counterpart in the reference, where one run is one analysis: it is for
+ * callers that run many small analyses (replicates, served requests) and
+ * want them in the same launches. */
+int ima_mc3_set_analyses(ima_ctx *ctx, int chains_per_analysis);
 /* Host-only planning, no device needed: the accept group size set_prior
  * would choose for a population model with these sizes (ima_model_add's
  * npops, npsize, nmig and the lengths of theta_wp / mig_wp in entries) and
