@@ -138,7 +138,8 @@ __device__ inline void restore_item(const DeviceView &v, const MC3Dev &m, int it
 // An accepted proposal (slot prop_first + it) replaces the chain's genealogy
 // (slot first_slot + it): edges, times, populations, alleles, program,
 // migrations, and the slot's root, stack depth and migration count.
-// Called by the lanes [lane0, lane0 + nl) of one wavefront (nl >= 1).
+// Called by threads lane = 0 .. nl - 1 of one workgroup (nl >= 1); the
+// fields of the chain slot's record it writes are not the ones it reads.
 __device__ void copy_proposal(const DeviceView &v, const MC3Dev &m, int it, int lane, int nl) {
   SlotMeta *slots = const_cast<SlotMeta *>(v.slots);
   const SlotMeta src = slots[m.prop_first + it];
@@ -911,56 +912,11 @@ __device__ void nw_start(const MC3Dev &m, int c, uint64_t sweep) {
 // Rescaling sweep: back up and rescale every coalescence time of the chain
 // (and with the prior its migration and split times).  Otherwise: chain
 // sum, swap by the last chain, next sweep's proposals.
-__global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int with_scale,
-                                                   int swaptries) {
-  __shared__ double sh[1024 / kWave];
-  __shared__ double Ssh[kSwapLds], Bsh[kSwapLds];
-  extern __shared__ double dyn[];
-  const int c = blockIdx.x;
-  const uint64_t sweep = *m.counter;
-  double part = 0.0;
-  if (m.prior < 0) {
-    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
-      const int it = c * m.L + l;
-      PhiloxStream r = item_rng(m, it, sweep, 1);
-      const double U = r.uniform();
-      const double np = new_loglik(m, it);
-      bool acc = false;
-      const double h = m.topo ? m.thast[it] : 0.0;
-      if (m.status[it] == IMA_STATUS_OK)
-        acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it]) + h));
-      double p = m.pdg[it];
-      if (acc) {
-        p = np;
-        m.pdg[it] = p;
-      } else if (!m.topo) {
-        restore_item(v, m, it);
-      }
-      if (m.topo) m.idec[it] = acc ? 1 : 0;
-      m.acc[it] += acc ? 1u : 0u;
-      part += p;
-      if (!with_scale && !m.topo) propose_item(v, m, it, sweep + 1);
-    }
-  } else {
-    prior_accept(v, m, c, sweep, dyn);
-    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
-      part += m.pdg[c * m.L + l];
-      if (!with_scale && !m.topo) propose_item(v, m, c * m.L + l, sweep + 1);
-    }
-    if (threadIdx.x == 0) part += m.probg[c];
-  }
-  if (m.topo) {
-    // the accepted proposals replace the chain's genealogies once every
-    // locus is decided, one wavefront per locus
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    for (int l = wave; l < m.L; l += blockDim.x / kWave)
-      if (m.idec[c * m.L + l]) copy_proposal(v, m, c * m.L + l, lane, kWave);
-  }
-  if (!with_scale) {
-    finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
-    return;
-  }
+// The start of a chain's time update, after its loci are decided: NW or
+// RY1 as qupdate draws them, or the global rescale (one workgroup per
+// chain; sh holds a double per wavefront).
+__device__ void time_update_start(const DeviceView &v, const MC3Dev &m, int c, uint64_t sweep,
+                                  double *sh) {
   __syncthreads();  // restored times visible to the whole workgroup
   if (m.nw) {
     // qupdate's time update: RY1 or NW with equal probability when the
@@ -1010,6 +966,61 @@ __global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int w
       if (k < m.P - 1) m.tv[c * tw + k] = t * s;
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void mc3_accept(DeviceView v, MC3Dev m, int with_scale,
+                                                   int swaptries) {
+  __shared__ double sh[1024 / kWave];
+  __shared__ double Ssh[kSwapLds], Bsh[kSwapLds];
+  extern __shared__ double dyn[];
+  const int c = blockIdx.x;
+  const uint64_t sweep = *m.counter;
+  double part = 0.0;
+  if (m.prior < 0) {
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      const int it = c * m.L + l;
+      PhiloxStream r = item_rng(m, it, sweep, 1);
+      const double U = r.uniform();
+      const double np = new_loglik(m, it);
+      bool acc = false;
+      const double h = m.topo ? m.thast[it] : 0.0;
+      if (m.status[it] == IMA_STATUS_OK)
+        acc = U < fmin(1.0, exp(m.beta[c] * (np - m.pdg[it]) + h));
+      double p = m.pdg[it];
+      if (acc) {
+        p = np;
+        m.pdg[it] = p;
+      } else if (!m.topo) {
+        restore_item(v, m, it);
+      }
+      if (m.topo) m.idec[it] = acc ? 1 : 0;
+      m.acc[it] += acc ? 1u : 0u;
+      part += p;
+      if (!with_scale && !m.topo) propose_item(v, m, it, sweep + 1);
+    }
+  } else {
+    prior_accept(v, m, c, sweep, dyn);
+    for (int l = threadIdx.x; l < m.L; l += blockDim.x) {
+      part += m.pdg[c * m.L + l];
+      if (!with_scale && !m.topo) propose_item(v, m, c * m.L + l, sweep + 1);
+    }
+    if (threadIdx.x == 0) part += m.probg[c];
+  }
+  if (!with_scale) {
+    finish_chain(m, c, block_sum(part, sh), swaptries, sweep, Ssh, Bsh);
+    return;
+  }
+  // with the device's genealogy move the accepted proposals are copied
+  // into the chains' slots by mc3_copy_accepted (every item at once), and
+  // mc3_time_start begins the time update after it
+  if (!m.topo) time_update_start(v, m, c, sweep, sh);
+}
+
+// Time-update sweeps with the device's genealogy move: the update's start,
+// once the accepted proposals are in the chains' slots.
+__global__ __launch_bounds__(1024) void mc3_time_start(DeviceView v, MC3Dev m) {
+  __shared__ double sh[1024 / kWave];
+  time_update_start(v, m, blockIdx.x, *m.counter, sh);
 }
 
 // One workgroup per chain: log acceptance of the rescaling
@@ -1255,43 +1266,7 @@ __global__ __launch_bounds__(1024) void mc3_decide(DeviceView v, MC3Dev m) {
 // that the chain's slot holds the proposal's migrations.
 __global__ __launch_bounds__(256) void mc3_copy_accepted(DeviceView v, MC3Dev m) {
   const int it = blockIdx.x;
-  if (!m.idec[it]) return;
-  SlotMeta *slots = const_cast<SlotMeta *>(v.slots);
-  const SlotMeta src = slots[m.prop_first + it];
-  SlotMeta &dst = slots[m.first_slot + it];
-  const int64_t so = src.node_off, d = dst.node_off;
-  const int ne = 2 * src.n - 1;
-  int32_t *up0 = const_cast<int32_t *>(v.up0), *up1 = const_cast<int32_t *>(v.up1);
-  int32_t *dn = const_cast<int32_t *>(v.down), *pop = const_cast<int32_t *>(v.pop);
-  int32_t *A = const_cast<int32_t *>(v.A);
-  int32_t *mf = const_cast<int32_t *>(v.mig_first), *mc = const_cast<int32_t *>(v.mig_count);
-  double *tm = const_cast<double *>(v.time);
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-    up0[d + e] = up0[so + e];
-    up1[d + e] = up1[so + e];
-    dn[d + e] = dn[so + e];
-    tm[d + e] = tm[so + e];
-    if (pop) pop[d + e] = pop[so + e];
-    if (A) A[d + e] = A[so + e];
-    if (mf) mf[d + e] = mf[so + e];
-    if (mc) mc[d + e] = mc[so + e];
-  }
-  ProgOp *prog = const_cast<ProgOp *>(v.prog);
-  for (int k = threadIdx.x; k < src.n - 1; k += blockDim.x)
-    prog[dst.prog_off + k] = prog[src.prog_off + k];
-  int32_t *me = const_cast<int32_t *>(v.mig_edge), *mto = const_cast<int32_t *>(v.mig_to);
-  double *mt = const_cast<double *>(v.mig_time);
-  for (int k = threadIdx.x; k < src.nmig; k += blockDim.x) {
-    me[dst.mig_off + k] = me[src.mig_off + k];
-    mto[dst.mig_off + k] = mto[src.mig_off + k];
-    mt[dst.mig_off + k] = mt[src.mig_off + k];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    dst.root = src.root;
-    dst.nslots = src.nslots;
-    dst.nmig = src.nmig;
-  }
+  if (m.idec[it]) copy_proposal(v, m, it, threadIdx.x, blockDim.x);
 }
 
 __global__ void mc3_host_finish(MC3Dev m) {
@@ -1413,6 +1388,10 @@ hipError_t enqueue_mc3_sweep(const DeviceView &v, const MC3Dev &m, bool with_sca
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(mc3_accept, dim3(m.C), dim3(1024), dl, st, v, m, with_scale ? 1 : 0,
                      swaptries);
+  if (m.topo) {
+    hipLaunchKernelGGL(mc3_copy_accepted, dim3(m.C * m.L), dim3(kWave), 0, st, v, m);
+    if (with_scale) hipLaunchKernelGGL(mc3_time_start, dim3(m.C), dim3(1024), 0, st, v, m);
+  }
   if (with_scale && m.nw) {
     e = launch_nw_update(v, m.nwa, st);
     if (e != hipSuccess) return e;
