@@ -1849,7 +1849,8 @@ static int mc3_topo_mode(ima_ctx *c, bool on) {
       return fail(c, IMA_ERR_SIZE, "room for the move's migrations: %d prior events exceed LDS",
                   emax);
     const int pool = 4 * std::max(maxcap, 16);
-    if (topo_mig_lds_bytes(d.maxn, pool) > (size_t)c->max_lds)
+    // + 1 KiB: the kernel's static LDS (model maps, split times, flags)
+    if (topo_mig_lds_bytes(d.maxn, pool) + 1024 > (size_t)c->max_lds)
       return fail(c, IMA_ERR_SIZE, "the move with migration on %d gene copies and %d events "
                   "needs more LDS than %d B", d.maxn, maxcap, c->max_lds);
     grow_logfact(c, lfn);

@@ -711,6 +711,15 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
                                                 (size_t)E * 10 * 4 + (size_t)POOL * 4) + 15) &
                                                ~(size_t)15));
   __shared__ int s_bad, s_root;
+  // the population model's plist | npresent | popdown | pope and the
+  // chain's split times, read by every step of the move
+  __shared__ int32_t s_mi[kMaxPops * kMaxPops + 5 * kMaxPops];
+  __shared__ double s_tv[kMaxPops];
+  const ModelMeta mm = v.models[m.prior];
+  const int PM = mm.npops;
+  for (int x = lane; x < PM * PM + 5 * PM - 2; x += kWave) s_mi[x] = v.mint[mm.ioff + x];
+  for (int x = lane; x < m.P - 1; x += kWave)
+    s_tv[x] = m.tv[(int64_t)(it / m.L) * mc3_tw(m.P) + x];
   const int nm0 = min(cs.nmig, POOL);
   for (int e = lane; e < ne; e += kWave) {
     tm[e] = v.time[cs.node_off + e];
@@ -737,11 +746,15 @@ __global__ __launch_bounds__(kWave) void mc3_propose_topo_mig(DeviceView v, MC3D
   } else if (lane == 0) {
     PhiloxStream r = topo_rng(m, it, sweep);
     MigCtx c;
-    c.md = model_view(v, v.models[m.prior]);
+    c.md = ModelView{};
+    c.md.plist = s_mi;
+    c.md.npresent = s_mi + PM * PM;
+    c.md.popdown = c.md.npresent + PM;
+    c.md.pope = c.md.popdown + 2 * PM - 1;
     c.P = m.P;
     c.lastp = m.P - 1;
     c.rootpop = 2 * m.P - 2;
-    c.tv = m.tv + (int64_t)(it / m.L) * mc3_tw(m.P);
+    c.tv = s_tv;
     c.pt = pt;
     c.pp = pp;
     c.pool_cap = POOL;
