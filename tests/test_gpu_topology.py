@@ -115,10 +115,14 @@ def test_four_copies_balanced_topology_one_third():
     e.close()
 
 
-def _two_pop_data(seed, C=6):
+def _two_pop_data(seed, C=6, with_mig=False):
     rng = np.random.default_rng(seed)
-    pt, theta, ts, mig = sim.im_model_two_pop(t=0.8, m12=0.0, m21=0.0)
-    model = immodel.build(pt, qmax=15.0, migration=False)
+    if with_mig:
+        pt, theta, ts, mrates = sim.im_model_two_pop(t=0.8, m12=0.5, m21=0.5)
+        model = immodel.build(pt, qmax=15.0, mmax=3.0)
+    else:
+        pt, theta, ts, mrates = sim.im_model_two_pop(t=0.8, m12=0.0, m21=0.0)
+        model = immodel.build(pt, qmax=15.0, migration=False)
     samp = [4, 4]
     e = Engine(0)
     mid = e.model_add(model)
@@ -126,7 +130,7 @@ def _two_pop_data(seed, C=6):
     loci, lids, base, As, u = [], [], [], [], []
     for kind in kinds:
         while True:
-            g0 = sim.simulate_genealogy(rng, samp, theta, ts, np.zeros((2, 2)), pt)
+            g0 = sim.simulate_genealogy(rng, samp, theta, ts, mrates, pt)
             A = None
             if kind == "hky":
                 data = sim.encode_hky(sim.simulate_hky_sequences(rng, g0, 200,
@@ -160,7 +164,7 @@ def _two_pop_data(seed, C=6):
                 precision=L.PREC_FP64, scale_width=0.3)
     e.mc3_set_prior(np.tile(ts, (C, 1)), 5.0, 2)
     return dict(e=e, first=first, model=model, samp=samp, loci=loci, u=np.array(u), kappa=kappa,
-                sumlogk=sl, base=base, C=C, Lc=Lc)
+                sumlogk=sl, base=base, C=C, Lc=Lc, with_mig=with_mig)
 
 
 def _state_matches_oracle(S, u, kappa):
@@ -181,6 +185,10 @@ def _state_matches_oracle(S, u, kappa):
             sl = slice(off, off + ne)
             g = sim.Genealogy(g0.n, up0[sl].copy(), up1[sl].copy(), dn[sl].copy(), tm[sl].copy(),
                               int(rt[c * Lc + l]), pops[sl].copy(), [[] for _ in range(ne)])
+            if S.get("with_mig"):
+                me, mt, mo = e.read_mig_events(first + c * Lc + l)
+                for x in range(len(me)):
+                    g.mig[int(me[x])].append((float(mt[x]), int(mo[x])))
             a = A[sl].copy()
             off += ne
             changed += int(not np.array_equal(g.up0, g0.up0) or g.root != g0.root)
@@ -201,8 +209,13 @@ def _state_matches_oracle(S, u, kappa):
     return changed
 
 
-def test_topology_sweeps_keep_the_state_consistent():
-    S = _two_pop_data(41)
+@pytest.mark.parametrize("with_mig", [False, True], ids=["no_migration", "migration_nw"])
+def test_topology_sweeps_keep_the_state_consistent(with_mig):
+    """HKY, infinite-sites and stepwise loci; with migration the moves draw
+    migration paths (and move stepwise alleles) and NW is drawn against
+    RY1."""
+    S = _two_pop_data(41, with_mig=with_mig)
+    nw = L.MC3_NW if with_mig else 0
     e, C = S["e"], S["C"]
     _, _, _, _, _, A0 = e.read_slots(S["first"], S["Lc"])
     off, S["tipA"] = 0, {}
@@ -210,9 +223,9 @@ def test_topology_sweeps_keep_the_state_consistent():
         n = S["base"][l].n
         S["tipA"][l] = A0[off:off + n].copy()
         off += 2 * n - 1
-    e.mc3_run(20, TOPO | L.MC3_SCALE | L.MC3_RY1)
+    e.mc3_run(20, TOPO | L.MC3_SCALE | L.MC3_RY1 | nw)
     changed = _state_matches_oracle(S, np.tile(S["u"], (C, 1)), np.tile(S["kappa"], (C, 1)))
-    e.mc3_run(20, L.MC3_TOPOLOGY | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(20, L.MC3_TOPOLOGY | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U | nw)
     u, kappa = e.mc3_read_rates()
     changed += _state_matches_oracle(S, u, kappa)
     _, _, stats, _ = e.mc3_read()
