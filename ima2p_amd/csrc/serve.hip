@@ -143,14 +143,16 @@ constexpr size_t kCtxOverhead = 16u << 20;
 size_t analysis_bytes(const AnalysisData &a) {
   const int P = a.npops, R = prior_rec_len(P);
   size_t b = kCtxOverhead;
+  // each genealogy twice: the chain's slot and the proposal slot of the
+  // genealogy move, each with room for twice its migrations plus 32
   for (int c = 0; c < a.nchains; ++c)
     for (int l = 0; l < a.nloci; ++l) {
       const size_t n = (size_t)a.n[l];
-      b += (2 * n - 1) * (5 * 4 + 2 * 8) + (n - 1) * sizeof(ProgOp);
+      b += 2 * ((2 * n - 1) * (5 * 4 + 2 * 8) + (n - 1) * sizeof(ProgOp) + 32 * 16);
     }
   for (int l = 0; l < a.nloci; ++l) b += (size_t)a.n[l] * (size_t)((a.S[l] + 63) / 64 * 64 + 64);
   b += (size_t)a.nchains * a.nloci * ((size_t)R * 16 + 128);
-  b += a.mig_time.size() * 32;
+  b += a.mig_time.size() * 4 * 16;
   return b;
 }
 
@@ -229,6 +231,12 @@ int build_analysis(ima_server *s, Req &r) {
   return IMA_OK;
 }
 
+// A served analysis' sweep: the reference's step, as the benchmark runs it
+// (updategenealogy's move, RY1 or NW on alternate sweeps, the mutation-rate
+// rounds every fifth sweep, the swap).
+constexpr int kServedSweep = IMA_MC3_GRAPH | IMA_MC3_TOPOLOGY | IMA_MC3_SCALE | IMA_MC3_RY1 |
+                             IMA_MC3_NW | IMA_MC3_UPDATE_U;
+
 // The first quantum is one sweep, to measure; later ones hold about
 // quantum_ms of GPU time at the measured rate.
 int run_quantum(ima_server *s, Req &r) {
@@ -237,7 +245,14 @@ int run_quantum(ima_server *s, Req &r) {
     q = (int)std::max(1.0, std::min((double)s->quantum, s->quantum_ms / r.sweep_ms));
   q = std::min(q, r.remaining);
   if (hipEventRecord(r.ev0, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
-  STRY(mc3_run_async(r.ctx, q, IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 | IMA_MC3_UPDATE_U));
+  STRY(mc3_run_async(r.ctx, q, kServedSweep));
+  if (r.sweep_ms <= 0) {
+    // the first quantum made the move's proposal slots: charge what the
+    // context holds now
+    const size_t real = ctx_device_bytes(r.ctx) + kCtxOverhead;
+    s->used = s->used - std::min(s->used, r.bytes) + real;
+    r.bytes = real;
+  }
   if (hipEventRecord(r.ev, r.ctx->stream) != hipSuccess) return IMA_ERR_HIP;
   r.q_last = q;
   r.remaining -= q;

@@ -701,9 +701,10 @@ typedef struct {
 } ima_analysis;
 
 /* budget_bytes <= 0: 80% of the device's free memory.  An analysis' sweeps
- * are ima_mc3_run's with IMA_MC3_GRAPH | IMA_MC3_SCALE | IMA_MC3_RY1 |
- * IMA_MC3_UPDATE_U (the rubber-band split-time update when it has two or
- * more populations, the mutation-rate rounds every fifth sweep).  It
+ * are ima_mc3_run's with IMA_MC3_GRAPH | IMA_MC3_TOPOLOGY | IMA_MC3_SCALE |
+ * IMA_MC3_RY1 | IMA_MC3_NW | IMA_MC3_UPDATE_U (updategenealogy's move, the
+ * rubber-band or NW split-time update when it has two or more
+ * populations, the mutation-rate rounds every fifth sweep).  It
  * runs in quanta of sweeps: the first is one sweep, later ones about 4 ms of GPU
  * time at its measured sweep rate, at most `quantum` sweeps (<= 0: 25).
  * Short quanta keep L-mode launches from queueing behind long runs of

@@ -23,12 +23,16 @@ from ima2p_amd.serve import REQ_ANALYSIS, REQ_DONE, REQ_LMODE, REQ_RUNNING, REQ_
 
 pytestmark = pytest.mark.gpu
 
+# the sweep a served analysis runs (the reference's step)
+SERVED = (L.MC3_GRAPH | L.MC3_TOPOLOGY | L.MC3_SCALE | L.MC3_RY1 | L.MC3_NW
+          | L.MC3_UPDATE_U)
+
 
 def _direct(spec):
     """The analysis run straight through the engine, all sweeps at once."""
     e = Engine(0)
     spec.load(e)
-    e.mc3_run(spec.nsweeps, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+    e.mc3_run(spec.nsweeps, SERVED)
     pdg, beta, stats, _ = e.mc3_read()
     probg, _ = e.mc3_read_prior()
     e.close()
@@ -174,7 +178,7 @@ def test_contexts_return_their_memory():
         for s in specs:
             e = Engine(0)
             s.load(e)
-            e.mc3_run(3, L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U)
+            e.mc3_run(3, SERVED)
             e.close()
 
     cycle()  # first-use allocations of the runtime
