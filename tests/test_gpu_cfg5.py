@@ -27,7 +27,8 @@ from tests.test_gpu_configs import _check_chains, rel
 
 pytestmark = pytest.mark.gpu
 
-FLAGS = L.MC3_GRAPH | L.MC3_SCALE | L.MC3_RY1 | L.MC3_UPDATE_U
+FLAGS = (L.MC3_GRAPH | L.MC3_TOPOLOGY | L.MC3_SCALE | L.MC3_RY1 | L.MC3_NW
+         | L.MC3_UPDATE_U)  # the step a served analysis runs
 
 
 def _stream(srv, tids, reqs, limit_s=240.0):
@@ -104,7 +105,7 @@ def test_cfg5_stream_results_equal_direct_runs():
             np.testing.assert_array_equal(np.ravel(x), np.ravel(y))
         if i == largest:
             tol = 1e-9 if spec.precision == L.PREC_FP64 else 1e-6
-            _check_chains(e, spec, first, [0, spec.nchains - 1], tol)
+            _check_chains(e, spec, first, [0, spec.nchains - 1], tol, moved=True)
         print(f"analysis {i}: {spec.model.npops} pops, {spec.nloci} loci x {spec.nchains} chains "
               "equal direct", flush=True)
         e.close()

@@ -47,6 +47,20 @@ def _current(e, g, slot, times):
     return h
 
 
+def _moved(e, g, slot):
+    """Genealogy g as the device now holds it after genealogy moves:
+    topology, times, root, populations and migration events."""
+    up0, up1, dn, tm, rt, _ = e.read_slots(slot, 1)
+    h = g.copy()
+    h.up0, h.up1, h.down, h.time, h.root = up0.copy(), up1.copy(), dn.copy(), tm.copy(), int(rt[0])
+    h.pop = e.read_pops(slot, 1)
+    me, mt, mo = e.read_mig_events(slot)
+    h.mig = [[] for _ in range(2 * g.n - 1)]
+    for x in range(len(me)):
+        h.mig[int(me[x])].append((float(mt[x]), int(mo[x])))
+    return h
+
+
 def _oracle_loglik(kind, data, g, A, u, kappa, sumlogk):
     if kind == "hky":
         return oracle.likelihood_hky(g, data, u, kappa)[0]
@@ -55,9 +69,11 @@ def _oracle_loglik(kind, data, g, A, u, kappa, sumlogk):
     return oracle.likelihood_sw(g, A, u)[0]
 
 
-def _check_chains(e, spec, first, chains, tol):
+def _check_chains(e, spec, first, chains, tol, moved=False):
     """pdg of every locus and probg of each chain in `chains` against the
-    oracle on the device's current genealogies and split times."""
+    oracle on the device's current genealogies and split times (moved:
+    after sweeps with the genealogy move, so topologies and migrations are
+    read back too)."""
     C, Lc = spec.nchains, spec.nloci
     pdg, _, _, _ = e.mc3_read()
     probg, tv = e.mc3_read_prior()
@@ -70,7 +86,8 @@ def _check_chains(e, spec, first, chains, tol):
         for l, (kind, data) in enumerate(spec.loci):
             g = spec.chains[c][l]
             ne = 2 * g.n - 1
-            h = _current(e, g, first + c * Lc + l, tm[off:off + ne])
+            h = (_moved(e, g, first + c * Lc + l) if moved
+                 else _current(e, g, first + c * Lc + l, tm[off:off + ne]))
             ref = _oracle_loglik(kind, data, h, A[off:off + ne], u[c][l], kappa[c][l], sl.get(l))
             off += ne
             assert rel(pdg[c, l], ref) <= tol, (c, l, kind, pdg[c, l], ref)
